@@ -1,0 +1,259 @@
+"""CPU oracle for the PIV correlation path -- TEST INFRASTRUCTURE ONLY.
+
+Only tests/, the smoke check in __graft_entry__.py and the cpu_baseline leg of
+bench.py may import this module.  It wraps:
+
+* ``_build/libqlpiv_oracle.so`` -- the C restatement of the reference's Java
+  code (qlpiv_oracle.c; built by ``make -C oracle``);
+* FP64 numpy oracles that follow the mathematics rather than the Java
+  operation order (direct-sum and FFT correlation);
+* readers for the reference's fixture formats (MATFloat).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import struct
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "libqlpiv_oracle.so")
+_lib = None
+
+_f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+_u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+
+
+def build():
+    """Compile the C restatement (gcc; seconds)."""
+    import subprocess
+
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        c_int, c_float = ctypes.c_int, ctypes.c_float
+        L.or_sine_table.argtypes = [c_int, _f32p]
+        L.or_perfect_shuffle.argtypes = [_f32p, c_int]
+        L.or_xcorr_javacpu.argtypes = [_f32p, _f32p, c_float, c_int, _f32p]
+        L.or_xcorr_floats.argtypes = [_f32p, _f32p, c_int, _f32p]
+        L.or_xcorr_javagpu.argtypes = [_f32p, _f32p, c_float, c_float, c_int, _f32p]
+        L.or_xcorr_batch_u8.argtypes = [c_int, c_int, _u8p, _u8p, _f32p, _f32p, c_int, _f32p]
+        L.or_find_max_from_center.argtypes = [_f32p, c_int, c_int, _f32p]
+        L.or_list_script.argtypes = [c_int, c_int, c_int, _f32p, _i32p, _f32p]
+        L.or_list_script.restype = c_int
+        L.or_find_max_multipeaks.argtypes = [_f32p, c_int, c_int, c_int, c_int, _f32p, _f32p]
+        L.or_find_max_multipeaks.restype = c_int
+        L.or_gauss1d.argtypes = [_f32p, c_int, c_int, c_int, c_float, _f32p, _f32p]
+        L.or_guo.argtypes = [_f32p, c_int, c_int, c_int, c_int, c_float, _f32p, _f32p]
+        L.or_poly.argtypes = [_f32p, c_int, c_int, _f32p, _f32p]
+        L.or_clip_tile.argtypes = [_u8p, c_int, c_int, c_int, c_int, c_int, _u8p]
+        L.or_clip_tile.restype = c_float
+        L.or_warp_micro.argtypes = [_u8p, c_int, c_int, c_int, c_int, c_int, c_float, c_float, _u8p]
+        _lib = L
+    return _lib
+
+
+# --------------------------------------------------------------------------
+# Restatement wrappers
+# --------------------------------------------------------------------------
+
+def sine_table(n):
+    out = np.zeros(n, np.float32)
+    lib().or_sine_table(n, out)
+    return out
+
+
+def perfect_shuffle(a):
+    a = np.ascontiguousarray(a, np.float32).copy()
+    lib().or_perfect_shuffle(a, a.size)
+    return a
+
+
+def xcorr_javacpu(A, B, maxA):
+    """FastRealFFTXCorr.computeXCorr(Matrix, Matrix) on float tiles."""
+    A = np.ascontiguousarray(A, np.float32)
+    B = np.ascontiguousarray(B, np.float32)
+    N = A.shape[0]
+    out = np.zeros((2 * N - 1, 2 * N - 1), np.float32)
+    lib().or_xcorr_javacpu(A, B, float(maxA), N, out)
+    return out
+
+
+def xcorr_floats(A, B):
+    """FastRealFFTXCorr.computeXCorr(float[][], float[][]) (no normalisation)."""
+    A = np.ascontiguousarray(A, np.float32)
+    B = np.ascontiguousarray(B, np.float32)
+    N = A.shape[0]
+    out = np.zeros((2 * N - 1, 2 * N - 1), np.float32)
+    lib().or_xcorr_floats(A, B, N, out)
+    return out
+
+
+def xcorr_javagpu(A, B, maxA, maxB):
+    """CrossCorrelationRealFFTParStdJob + kernel arithmetic (Java semantics)."""
+    A = np.ascontiguousarray(A, np.float32)
+    B = np.ascontiguousarray(B, np.float32)
+    N = A.shape[0]
+    out = np.zeros((2 * N - 1, 2 * N - 1), np.float32)
+    lib().or_xcorr_javagpu(A, B, float(maxA), float(maxB), N, out)
+    return out
+
+
+def xcorr_batch_u8(tilesA, tilesB, maxA, maxB, mode):
+    """mode 0: Java CPU path (A max for both); mode 1: GPU job (own maxima)."""
+    tilesA = np.ascontiguousarray(tilesA, np.uint8)
+    tilesB = np.ascontiguousarray(tilesB, np.uint8)
+    n, N = tilesA.shape[0], tilesA.shape[1]
+    out = np.zeros((n, 2 * N - 1, 2 * N - 1), np.float32)
+    lib().or_xcorr_batch_u8(n, N, tilesA, tilesB, np.ascontiguousarray(maxA, np.float32),
+                            np.ascontiguousarray(maxB, np.float32), int(mode), out)
+    return out
+
+
+def find_max_from_center(plane):
+    plane = np.ascontiguousarray(plane, np.float32)
+    out = np.zeros(4, np.float32)
+    lib().or_find_max_from_center(plane, plane.shape[0], plane.shape[1], out)
+    return out  # i, j, value, minFloor
+
+
+def list_script(max_size, kernel_width, ops):
+    """ops: list of (opcode, value, i, j); 0=addInAscendOrder, 1=remove, 2=clear."""
+    arr = np.ascontiguousarray(np.array(ops, np.float32).reshape(-1, 4))
+    acc = np.zeros(len(ops), np.int32)
+    out = np.zeros(3 * 70, np.float32)
+    n = lib().or_list_script(max_size, kernel_width, len(ops), arr, acc, out)
+    return [tuple(out[3 * k:3 * k + 3]) for k in range(n)], acc
+
+
+def find_max_multipeaks(plane, n_peaks, kernel_width):
+    plane = np.ascontiguousarray(plane, np.float32)
+    out = np.zeros(3 * n_peaks, np.float32)
+    mf = np.zeros(1, np.float32)
+    found = lib().or_find_max_multipeaks(plane, plane.shape[0], plane.shape[1], n_peaks,
+                                         kernel_width, out, mf)
+    return out.reshape(n_peaks, 3), int(found), float(mf[0])
+
+
+def gauss1d(plane, pixels, min_floor, peak):
+    plane = np.ascontiguousarray(plane, np.float32)
+    res = np.zeros(4, np.float32)
+    lib().or_gauss1d(plane, plane.shape[0], plane.shape[1], pixels, float(min_floor),
+                     np.ascontiguousarray(peak, np.float32), res)
+    return res
+
+
+def guo(plane, pixels, iterations, min_floor, peak):
+    plane = np.ascontiguousarray(plane, np.float32)
+    res = np.zeros(4, np.float32)
+    lib().or_guo(plane, plane.shape[0], plane.shape[1], pixels, iterations, float(min_floor),
+                 np.ascontiguousarray(peak, np.float32), res)
+    return res
+
+
+def poly(plane, peak):
+    plane = np.ascontiguousarray(plane, np.float32)
+    res = np.zeros(4, np.float32)
+    lib().or_poly(plane, plane.shape[0], plane.shape[1], np.ascontiguousarray(peak, np.float32),
+                  res)
+    return res
+
+
+def clip_tile(img, top, left, N):
+    img = np.ascontiguousarray(img, np.uint8)
+    tile = np.zeros((N, N), np.uint8)
+    mx = lib().or_clip_tile(img, img.shape[0], img.shape[1], int(top), int(left), N, tile)
+    return tile, mx
+
+
+def warp_micro(img, top, left, N, du, dv):
+    img = np.ascontiguousarray(img, np.uint8)
+    tile = np.zeros((N, N), np.uint8)
+    lib().or_warp_micro(img, img.shape[0], img.shape[1], int(top), int(left), N, float(du),
+                        float(dv), tile)
+    return tile
+
+
+# --------------------------------------------------------------------------
+# FP64 mathematics oracles
+# --------------------------------------------------------------------------
+
+def normalise_pair(A, B, maxF, maxG):
+    """x / max * 16 + 1 in float32 as Java evaluates it; F mirrored."""
+    A = np.asarray(A, np.float32)
+    B = np.asarray(B, np.float32)
+    F = (A[::-1, ::-1] / np.float32(maxF)) * np.float32(16.0) + np.float32(1.0)
+    G = (B / np.float32(maxG)) * np.float32(16.0) + np.float32(1.0)
+    return F.astype(np.float32), G.astype(np.float32)
+
+
+def xcorr_direct64(Fm, G):
+    """R[i][j] = sum_nm Fm[n][m] * G[n+i-(N-1)][m+j-(N-1)] where Fm is the
+    mirrored, normalised tile -- the convolution the reference computes
+    (CrossCorrelationTestHelper.localCrossCorrelation :32-73 for the
+    unmirrored form).  FP64 brute force; for small N."""
+    Fm = np.asarray(Fm, np.float64)
+    G = np.asarray(G, np.float64)
+    N = Fm.shape[0]
+    F = Fm[::-1, ::-1]
+    D = 2 * N - 1
+    R = np.zeros((D, D))
+    for i in range(-(N - 1), N):
+        for j in range(-(N - 1), N):
+            n0, n1 = max(0, -i), min(N, N - i)
+            m0, m1 = max(0, -j), min(N, N - j)
+            R[i + N - 1, j + N - 1] = np.sum(F[n0:n1, m0:m1] * G[n0 + i:n1 + i, m0 + j:m1 + j])
+    return R
+
+
+def xcorr_fft64(Fm, G):
+    """Same plane through an FP64 FFT of the zero-padded 2N x 2N product."""
+    Fm = np.asarray(Fm, np.float64)
+    G = np.asarray(G, np.float64)
+    N = Fm.shape[-1]
+    M = 2 * N
+    shp = Fm.shape[:-2] + (M, M)
+    a = np.zeros(shp)
+    b = np.zeros(shp)
+    a[..., :N, :N] = Fm
+    b[..., :N, :N] = G
+    r = np.fft.ifft2(np.fft.fft2(a) * np.fft.fft2(b)).real
+    return r[..., :M - 1, :M - 1]
+
+
+# --------------------------------------------------------------------------
+# Fixture formats
+# --------------------------------------------------------------------------
+
+def read_matfloat(path):
+    """SimpleFloatMatrixImporterExporter format (exporter/SimpleFloatMatrixImporterExporter
+    .java:263-314): 'MATFloat\\0', int16 count, names, then per matrix
+    int16 sizeX, int16 sizeY, int16 0, big-endian float32 data."""
+    with open(path, "rb") as f:
+        d = f.read()
+    assert d[:9] == b"MATFloat\x00"
+    off = 9
+    (n,) = struct.unpack(">h", d[off:off + 2])
+    off += 2
+    names = []
+    for _ in range(n):
+        (ln,) = struct.unpack(">h", d[off:off + 2])
+        off += 2
+        names.append(d[off:off + ln].decode())
+        off += ln
+    mats = []
+    for _ in range(n):
+        sx, sy, _z = struct.unpack(">hhh", d[off:off + 6])
+        off += 6
+        m = np.frombuffer(d, ">f4", sx * sy, off).reshape(sy, sx).astype(np.float32)
+        off += 4 * sx * sy
+        mats.append(m)
+    return names, mats

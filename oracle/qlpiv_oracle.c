@@ -1,0 +1,1140 @@
+/*
+ * qlpiv_oracle.c -- CPU restatement of QuickLabPIV-ng's PIV correlation hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker: it is built into
+ * oracle/_build/libqlpiv_oracle.so and loaded by tests/, by the smoke check of
+ * __graft_entry__.py and by the cpu_baseline leg of bench.py.  Nothing in the
+ * product library links or calls it.
+ *
+ * Every function restates one piece of the reference's Java code with the same
+ * operation order and the same float/double types, so that compiled with
+ * -O2 -ffp-contract=off -fno-fast-math (no FMA contraction, SSE float
+ * arithmetic, FLT_EVAL_METHOD 0) it produces the values the JVM produces.
+ * Paths below are relative to the reference's src/main/java/pt/quickLabPIV/.
+ *
+ * Pinning (what checks each function; see tests/test_oracle_*.py):
+ *   or_sine_table            - closed form sin, float rounding-boundary margin test
+ *   or_perfect_shuffle       - equals the bit-reversal permutation (all sizes)
+ *   or_xcorr_javacpu/_javagpu- direct-sum correlation (KAT 4x4 of FastRealFFTXCorrTests,
+ *                              brute force on random tiles), FP64 FFT oracle, and the
+ *                              reference's golden GPU planes (1024 x 31x31, tol 1e-2)
+ *   or_find_max_from_center  - brute-force argmax with the documented tie-break
+ *   or_multipeaks / or_list_*- FindMaximumMultiPeakTests KAT and the 14
+ *                              SimpleFixedLengthFloatLinkedListTests sequences
+ *   or_gauss1d               - 3 px: closed form; 5/7 px: least-squares optimum;
+ *                              GaussianTestHelper cases (567, tol 8e-3)
+ *   or_guo                   - exact on noise-free Gaussians; HongWeiGuo cases.
+ *                              The singular-matrix SVD branch is parity unpinned.
+ *   or_poly                  - closed form
+ *   or_warp_micro            - identity at zero displacement, pure shift at
+ *                              integer displacement, separable bilinear otherwise
+ *   or_clip_tile             - direct slicing with zero fill; max clamp >= 16
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+
+
+/* ------------------------------------------------------------------------ */
+/* FFT building blocks: jobs/xcorr/SimpleFFT.java                            */
+/* ------------------------------------------------------------------------ */
+
+/* SimpleFFT.initEulerTable :323-330 and CrossCorrelationFFTTemplate.initEulerTable
+ * :268-277: ws[n] = (float)sin(2.0f*PI*(float)n/(float)N), evaluated in double. */
+void or_sine_table(int n, float *ws)
+{
+    for (int k = 0; k < n; k++)
+        ws[k] = (float)sin(2.0 * M_PI * (double)(float)k / (double)(float)n);
+}
+
+/* SimpleFFT.perfectShuffleFFTInput :32-69, restated loop for loop. */
+void or_perfect_shuffle(float *a, int n)
+{
+    for (int osf = 1, ods = n; ods > 2; ods >>= 1, osf <<= 1) {
+        for (int osi = 0; osi < osf; osi++) {
+            int oo = osi * ods;
+            for (int sf = 1, ds = ods; ds > 2; ds >>= 1, sf <<= 1) {
+                for (int si = 0; si < sf; si++) {
+                    int io = si * ds;
+                    for (int i = 0; i < ds / 2; i += 2) {
+                        int src = oo + io + i + 1;
+                        int dst = oo + io + ds / 2 + i;
+                        float t = a[src];
+                        a[src] = a[dst];
+                        a[dst] = t;
+                    }
+                }
+            }
+        }
+    }
+}
+
+/* SimpleFFT.computeFFTStep :122-221 (the "Option A" butterfly with both
+ * results held in temporaries). */
+static void fft_step(float *xr, float *xi, int step, int level, int level2,
+                     int tidx, int N, const float *ws)
+{
+    int k = step * (tidx % level);
+    int i = (tidx / level) * level2 + (tidx % level);
+    int j = i + level;
+    float wr = ws[k + N / 4];
+    float wi = -ws[k];
+    float tlr = xr[i] + (wr * xr[j]) - (wi * xi[j]);
+    float tli = xi[i] + (wr * xi[j]) + (wi * xr[j]);
+    float tur = xr[i] - (wr * xr[j]) + (wi * xi[j]);
+    float tui = xi[i] - (wr * xi[j]) - (wi * xr[j]);
+    xr[i] = tlr;
+    xi[i] = tli;
+    xr[j] = tur;
+    xi[j] = tui;
+}
+
+/* SimpleFFT.computeFFTSerial :97-110 */
+void or_fft_serial(float *xr, float *xi, int N, const float *ws)
+{
+    int step = N;
+    for (int level = 1; level < N; level <<= 1) {
+        int level2 = level << 1;
+        step >>= 1;
+        for (int tidx = 0; tidx < N / 2; tidx++)
+            fft_step(xr, xi, step, level, level2, tidx, N, ws);
+    }
+}
+
+/* SimpleFFT.computeIFFTStep :255-320 */
+static void ifft_step(float *xr, float *xi, int step, int level, int level2,
+                      int tidx, int N, const float *ws)
+{
+    int k = step * (tidx % level);
+    int i = (tidx / level) * level2 + (tidx % level);
+    int j = i + level;
+    float wr;
+    if (N / 4 - k >= 0)
+        wr = ws[N / 4 - k];
+    else
+        wr = ws[N + (N / 4 - k)];
+    float wi = ws[k];
+    float tur = xr[i] - (wr * xr[j]) + (wi * xi[j]);
+    float tui = xi[i] - (wr * xi[j]) - (wi * xr[j]);
+    xr[i] = xr[i] + (wr * xr[j]) - (wi * xi[j]);
+    xi[i] = xi[i] + (wr * xi[j]) + (wi * xr[j]);
+    xr[j] = tur;
+    xi[j] = tui;
+}
+
+/* SimpleFFT.computeIFFTSerial :229-242 (divideArray :333-337 first) */
+void or_ifft_serial(float *xr, float *xi, int N, const float *ws)
+{
+    float fN = (float)N;
+    for (int q = 0; q < N; q++) xr[q] /= fN;
+    for (int q = 0; q < N; q++) xi[q] /= fN;
+    int step = N;
+    for (int level = 1; level < N; level <<= 1) {
+        int level2 = level << 1;
+        step >>= 1;
+        for (int tidx = 0; tidx < N / 2; tidx++)
+            ifft_step(xr, xi, step, level, level2, tidx, N, ws);
+    }
+}
+
+/* One 1D pass over every row of an M x M pair of arrays:
+ * shuffle(xr row), shuffle(xi row), then the transform.
+ * kind 0: forward (computeFFTSerial), 1: CPU inverse (computeIFFTSerial),
+ * 2: GPU-kernel inverse = forward transform on the (xi, xr) swapped pair
+ *    (CrossCorrelationRealFFTParallelKernel.java:606-623, 672-688). */
+static void rows_pass(float *xr, float *xi, int M, const float *ws, int kind)
+{
+    for (int r = 0; r < M; r++) {
+        float *a = xr + (size_t)r * M, *b = xi + (size_t)r * M;
+        or_perfect_shuffle(a, M);
+        or_perfect_shuffle(b, M);
+        if (kind == 0)
+            or_fft_serial(a, b, M, ws);
+        else if (kind == 1)
+            or_ifft_serial(a, b, M, ws);
+        else
+            or_fft_serial(b, a, M, ws);
+    }
+}
+
+static void transpose_sq(float *x, int M)
+{
+    for (int i = 0; i < M; i++)
+        for (int j = i + 1; j < M; j++) {
+            float t = x[(size_t)i * M + j];
+            x[(size_t)i * M + j] = x[(size_t)j * M + i];
+            x[(size_t)j * M + i] = t;
+        }
+}
+
+/* SimpleFFT.computeFFT2DSerial :379-432: rows, transpose, rows, transpose back. */
+void or_fft2d(float *xr, float *xi, int M, int kind)
+{
+    float *ws = (float *)malloc(sizeof(float) * M);
+    or_sine_table(M, ws); /* computeFFT2D :435-436 rebuilds the table per call */
+    rows_pass(xr, xi, M, ws, kind);
+    transpose_sq(xr, M);
+    transpose_sq(xi, M);
+    rows_pass(xr, xi, M, ws, kind);
+    transpose_sq(xr, M);
+    transpose_sq(xi, M);
+    free(ws);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Cross-correlation                                                         */
+/* ------------------------------------------------------------------------ */
+
+/* Matrix.copyMirroredMatrixToArrayNormalizeAndOffset (Matrix.java:282-288),
+ * Matrix.copyMatrixToArrayAndNormalizeAndOffset (:175-181) and the inline CPU
+ * variant (FastRealFFTXCorr.java:48-60): v / max * 16.0f + 1.0f, F mirrored,
+ * zero padded to 2N x 2N. */
+static void pack_pair(const float *A, const float *B, float maxF, float maxG,
+                      int N, float *tr, float *ti)
+{
+    int M = 2 * N;
+    memset(tr, 0, sizeof(float) * M * M);
+    memset(ti, 0, sizeof(float) * M * M);
+    for (int i = 0; i < N; i++)
+        for (int j = 0; j < N; j++) {
+            tr[i * M + j] = A[(N - 1 - i) * N + (N - 1 - j)] / maxF * 16.0f + 1.0f;
+            ti[i * M + j] = B[i * N + j] / maxG * 16.0f + 1.0f;
+        }
+}
+
+/* FastRealFFTXCorr.commonRealFFTXCorr :70-117 unpack + product loop, in place. */
+static void unpack_product_cpu(float *tr, float *ti, int M)
+{
+#define TR(a, b) tr[(a) * M + (b)]
+#define TI(a, b) ti[(a) * M + (b)]
+    for (int i = 0; i <= M / 2; i++) {
+        for (int j = 0; j <= M / 2; j++) {
+            int k1 = i, Nk1 = (M - k1) % M, k2 = j, Nk2 = (M - k2) % M;
+            float Gr = 0.5f * (TI(k1, k2) + TI(Nk1, Nk2));
+            float Gi = -0.5f * (TR(k1, k2) - TR(Nk1, Nk2));
+            float Fr = 0.5f * (TR(k1, k2) + TR(Nk1, Nk2));
+            float Fi = 0.5f * (TI(k1, k2) - TI(Nk1, Nk2));
+            TR(k1, k2) = Fr * Gr - Fi * Gi;
+            TI(k1, k2) = Fr * Gi + Fi * Gr;
+            TR(Nk1, Nk2) = TR(k1, k2);
+            TI(Nk1, Nk2) = -TI(k1, k2);
+            if (k1 != 0 && k2 != 0 && k1 < M / 2 && k2 < M / 2) {
+                k1 = i + M / 2;
+                Nk1 = (M - k1) % M;
+                float Gr2 = 0.5f * (TI(k1, k2) + TI(Nk1, Nk2));
+                float Gi2 = -0.5f * (TR(k1, k2) - TR(Nk1, Nk2));
+                float Fr2 = 0.5f * (TR(k1, k2) + TR(Nk1, Nk2));
+                float Fi2 = 0.5f * (TI(k1, k2) - TI(Nk1, Nk2));
+                TR(k1, k2) = Fr2 * Gr2 - Fi2 * Gi2;
+                TI(k1, k2) = Fr2 * Gi2 + Fi2 * Gr2;
+                TR(Nk1, Nk2) = TR(k1, k2);
+                TI(Nk1, Nk2) = -TI(k1, k2);
+            }
+        }
+    }
+#undef TR
+#undef TI
+}
+
+/* CrossCorrelationRealFFTParallelKernel.run :458-527 unpack + product with the
+ * single /(2N)^2 scaling, restated serially (the kernel's per-row work-items
+ * touch disjoint element pairs, so the serial order gives the same values). */
+static void unpack_product_gpu(float *xr, float *xi, int M)
+{
+    const float scale = (float)(M * M);
+    for (int k1 = 0; k1 <= M / 2; k1++) {
+        int Nk1 = (M - k1) % M;
+        int lk1 = Nk1, Nlk1 = k1;
+        for (int k2 = 0; k2 <= M / 2; k2++) {
+            int Nk2 = (M - k2) % M;
+            float x = xr[k1 * M + k2], Nx = xr[Nk1 * M + Nk2];
+            float y = xi[k1 * M + k2], Ny = xi[Nk1 * M + Nk2];
+            float fr = 0.5f * (x + Nx);
+            float fi = 0.5f * (y - Ny);
+            float gr = 0.5f * (y + Ny);
+            float gi = -0.5f * (x - Nx);
+            float re = (fr * gr - fi * gi) / scale;
+            float im = (fr * gi + fi * gr) / scale;
+            xr[k1 * M + k2] = re;
+            xi[k1 * M + k2] = im;
+            xr[Nk1 * M + Nk2] = re;
+            xi[Nk1 * M + Nk2] = -im;
+        }
+        for (int k2 = 0; k2 < M; k2++) {
+            int Nk2 = (M - k2) % M;
+            if (k1 != 0 && k2 != 0 && k1 < M / 2 && k2 < M / 2) {
+                float x = xr[lk1 * M + k2], Nx = xr[Nlk1 * M + Nk2];
+                float y = xi[lk1 * M + k2], Ny = xi[Nlk1 * M + Nk2];
+                float fr = 0.5f * (x + Nx);
+                float fi = 0.5f * (y - Ny);
+                float gr = 0.5f * (y + Ny);
+                float gi = -0.5f * (x - Nx);
+                float re = (fr * gr - fi * gi) / scale;
+                float im = (fr * gi + fi * gr) / scale;
+                xr[lk1 * M + k2] = re;
+                xi[lk1 * M + k2] = im;
+                xr[Nlk1 * M + Nk2] = re;
+                xi[Nlk1 * M + Nk2] = -im;
+            }
+        }
+    }
+}
+
+/* FastRealFFTXCorr.computeXCorr(Matrix, Matrix) :44-68 -- the Java CPU path
+ * (CrossCorrelationSoftRealFFTJob.compute :94-121): both tiles normalised by
+ * A's max, IFFT dividing by 2N in each 1D pass.  A, B: N x N row-major floats
+ * (tile pixel values); out: (2N-1) x (2N-1). */
+void or_xcorr_javacpu(const float *A, const float *B, float maxA, int N, float *out)
+{
+    int M = 2 * N;
+    float *tr = (float *)malloc(sizeof(float) * M * M);
+    float *ti = (float *)malloc(sizeof(float) * M * M);
+    pack_pair(A, B, maxA, maxA, N, tr, ti);
+    or_fft2d(tr, ti, M, 0);
+    unpack_product_cpu(tr, ti, M);
+    or_fft2d(tr, ti, M, 1);
+    for (int i = 0; i < M - 1; i++)
+        for (int j = 0; j < M - 1; j++)
+            out[i * (M - 1) + j] = tr[i * M + j];
+    free(tr);
+    free(ti);
+}
+
+/* FastRealFFTXCorr.computeXCorr(float[][], float[][]) :15-42 -- the
+ * un-normalised overload used by FastRealFFTXCorrTests (A mirrored, B as is). */
+void or_xcorr_floats(const float *A, const float *B, int N, float *out)
+{
+    int M = 2 * N;
+    float *tr = (float *)calloc((size_t)M * M, sizeof(float));
+    float *ti = (float *)calloc((size_t)M * M, sizeof(float));
+    for (int i = 0; i < N; i++)
+        for (int j = 0; j < N; j++) {
+            tr[i * M + j] = A[(N - 1 - i) * N + (N - 1 - j)];
+            ti[i * M + j] = B[i * N + j];
+        }
+    or_fft2d(tr, ti, M, 0);
+    unpack_product_cpu(tr, ti, M);
+    or_fft2d(tr, ti, M, 1);
+    for (int i = 0; i < M - 1; i++)
+        for (int j = 0; j < M - 1; j++)
+            out[i * (M - 1) + j] = tr[i * M + j];
+    free(tr);
+    free(ti);
+}
+
+/* CrossCorrelationRealFFTParStdJob.analyzeTemplate :91-136 packing (F by its own
+ * max, G by its own max) + CrossCorrelationRealFFTParallelKernel.run :214-713
+ * arithmetic as executed with Java float semantics (Aparapi JTP). */
+void or_xcorr_javagpu(const float *A, const float *B, float maxA, float maxB, int N,
+                      float *out)
+{
+    int M = 2 * N;
+    float *xr = (float *)malloc(sizeof(float) * M * M);
+    float *xi = (float *)malloc(sizeof(float) * M * M);
+    pack_pair(A, B, maxA, maxB, N, xr, xi);
+    or_fft2d(xr, xi, M, 0);
+    unpack_product_gpu(xr, xi, M);
+    or_fft2d(xr, xi, M, 2);
+    for (int i = 0; i < M - 1; i++)
+        for (int j = 0; j < M - 1; j++)
+            out[i * (M - 1) + j] = xr[i * M + j];
+    free(xr);
+    free(xi);
+}
+
+/* Batched helpers over u8 tiles (for tests and the CPU baseline):
+ * mode 0 = Java CPU path (A max for both), mode 1 = GPU job path (own maxima). */
+void or_xcorr_batch_u8(int n, int N, const uint8_t *tA, const uint8_t *tB,
+                       const float *maxA, const float *maxB, int mode, float *planes)
+{
+    int D = 2 * N - 1;
+    float *A = (float *)malloc(sizeof(float) * N * N);
+    float *B = (float *)malloc(sizeof(float) * N * N);
+    for (int t = 0; t < n; t++) {
+        for (int q = 0; q < N * N; q++) {
+            A[q] = (float)tA[(size_t)t * N * N + q];
+            B[q] = (float)tB[(size_t)t * N * N + q];
+        }
+        if (mode == 0)
+            or_xcorr_javacpu(A, B, maxA[t], N, planes + (size_t)t * D * D);
+        else
+            or_xcorr_javagpu(A, B, maxA[t], maxB[t], N, planes + (size_t)t * D * D);
+    }
+    free(A);
+    free(B);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Peak search                                                               */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    float i, j, value;
+} or_cand;
+
+/* MaxCrossResult.computeDistance :48-51 */
+static float cand_dist(const or_cand *c, short ci, short cj)
+{
+    float di = c->i - ci, dj = c->j - cj;
+    return (float)sqrt((double)(di * di + dj * dj));
+}
+
+/* FindMaximumFromCenter.findMaximum :47-183 with updateMaxResults :33-44.
+ * out[0..3] = peak i, peak j, peak value, minFloor. */
+void or_find_max_from_center(const float *m, int dimI, int dimJ, float *out)
+{
+#define EL(a, b) m[(a) * dimJ + (b)]
+    float maximum = 0.0f;
+    or_cand best = {0.0f, 0.0f, 0.0f};
+    short ci = (short)(dimI / 2 + 1), cj = (short)(dimJ / 2 + 1);
+#define UPDATE(II, JJ, VV)                                                    \
+    do {                                                                      \
+        or_cand nw = {(float)(II), (float)(JJ), (VV)};                        \
+        if (nw.value > maximum) {                                             \
+            best = nw;                                                        \
+            maximum = nw.value;                                               \
+        } else if (nw.value == maximum) {                                     \
+            if (cand_dist(&nw, ci, cj) < cand_dist(&best, ci, cj)) best = nw; \
+        }                                                                     \
+    } while (0)
+    UPDATE(dimI / 2 + 1, dimJ / 2 + 1, EL(dimI / 2 + 1, dimJ / 2 + 1));
+    float minValue = 3.40282347e+38f;
+    for (int i = 0; i < dimI / 2; i++) {
+        float v1 = EL(dimI / 2 - i, dimJ / 2 + 1), v2 = EL(dimI / 2 + 1 + i, dimJ / 2 + 1);
+        if (v1 < minValue) minValue = v1;
+        if (v2 < minValue) minValue = v2;
+        if (v1 >= maximum) UPDATE(dimI / 2 - i, dimJ / 2 + 1, v1);
+        if (v2 >= maximum) UPDATE(dimI / 2 + 1 + i, dimJ / 2 + 1, v2);
+    }
+    for (int j = 0; j < dimJ / 2; j++) {
+        float v1 = EL(dimI / 2 + 1, dimJ / 2 - j), v2 = EL(dimI / 2 + 1, dimJ / 2 + 1 + j);
+        if (v1 < minValue) minValue = v1;
+        if (v2 < minValue) minValue = v2;
+        if (v1 >= maximum) UPDATE(dimI / 2 + 1, dimJ / 2 - j, v1);
+        if (v2 >= maximum) UPDATE(dimI / 2 + 1, dimJ / 2 + 1 + j, v2);
+    }
+    for (int i = 0; i < dimI / 2; i++) {
+        for (int j = 0; j < dimJ / 2; j++) {
+            float v1 = EL(dimI / 2 - i, dimJ / 2 - j);
+            float v2 = EL(dimI / 2 + 1 + i, dimJ / 2 - j);
+            float v3 = EL(dimI / 2 - i, dimJ / 2 + 1 + j);
+            float v4 = EL(dimI / 2 + 1 + i, dimJ / 2 + 1 + j);
+            if (v1 < minValue) minValue = v1;
+            if (v2 < minValue) minValue = v2;
+            if (v3 < minValue) minValue = v3;
+            if (v4 < minValue) minValue = v4;
+            if (v1 >= maximum) UPDATE(dimI / 2 - i, dimJ / 2 - j, v1);
+            if (v2 >= maximum) UPDATE(dimI / 2 + 1 + i, dimJ / 2 - j, v2);
+            if (v3 >= maximum) UPDATE(dimI / 2 - i, dimJ / 2 + 1 + j, v3);
+            if (v4 >= maximum) UPDATE(dimI / 2 + 1 + i, dimJ / 2 + 1 + j, v4);
+        }
+    }
+#undef UPDATE
+#undef EL
+    out[0] = best.i;
+    out[1] = best.j;
+    out[2] = best.value;
+    out[3] = minValue;
+}
+
+/* util/SimpleFixedLengthFloatLinkedList.java, restated with a node pool and
+ * next indices that follow the Java object links step for step. */
+#define OR_LIST_MAXP 64
+typedef struct {
+    short i, j;
+    float value;
+    int next;
+} or_node;
+typedef struct {
+    int size, maxSize;
+    float kernelDistance;
+    int first, last, removed;
+    or_node pool[OR_LIST_MAXP + 2];
+    int npool;
+    int freelist[OR_LIST_MAXP + 2];
+    int nfree;
+} or_list;
+
+void or_list_init(or_list *L, int maxSize, int kernelWidth)
+{
+    memset(L, 0, sizeof(*L));
+    L->maxSize = maxSize;
+    L->kernelDistance = (float)((kernelWidth - 1) * kernelWidth); /* ctor :69-76 */
+    L->first = L->last = L->removed = -1;
+}
+
+static int list_alloc(or_list *L)
+{
+    int f = L->removed; /* getFreeElement :78-86 */
+    L->removed = -1;
+    if (f < 0) {
+        if (L->nfree > 0)
+            f = L->freelist[--L->nfree];
+        else
+            f = L->npool++;
+    }
+    return f;
+}
+
+static void list_release(or_list *L, int idx)
+{
+    /* a node dropped from 'removed' by a later removal becomes garbage in Java */
+    if (idx >= 0) L->freelist[L->nfree++] = idx;
+}
+
+/* computeDistance :49-51 */
+static float list_dist(short i, short j, const or_node *e)
+{
+    return (float)(pow((double)(i - e->i), 2) + pow((double)(j - e->j), 2));
+}
+
+/* remove :239-258 (drops the smallest, i.e. the first entry) */
+float or_list_remove(or_list *L)
+{
+    float v = L->pool[L->first].value;
+    if (L->removed >= 0) list_release(L, L->removed);
+    L->removed = L->first;
+    if (L->first == L->last)
+        L->first = L->last = -1;
+    else
+        L->first = L->pool[L->first].next;
+    L->pool[L->removed].next = -1;
+    L->size--;
+    return v;
+}
+
+/* addInAscendOrder :98-237 */
+int or_list_add_ascend(or_list *L, float value, short i, short j)
+{
+    if (L->first < 0 || L->last < 0) {
+        int e = list_alloc(L);
+        L->pool[e].value = value;
+        L->pool[e].i = i;
+        L->pool[e].j = j;
+        L->pool[e].next = -1;
+        L->first = L->last = e;
+        L->size = 1;
+        return 1;
+    }
+    int previous = -1, iter = L->first;
+    int delTo[OR_LIST_MAXP + 2], delPrev[OR_LIST_MAXP + 2];
+    int deleteSize = 0;
+    while (iter >= 0 && L->pool[iter].value < value) {
+        const or_node *e = &L->pool[iter];
+        if (list_dist(i, j, e) < L->kernelDistance && value > e->value) { /* isToRemoveAdjacent :61-67 */
+            delPrev[deleteSize] = previous;
+            delTo[deleteSize] = iter;
+            deleteSize++;
+        }
+        previous = iter;
+        iter = L->pool[iter].next;
+    }
+    if (L->size == L->maxSize && iter == L->first) return 0;
+    while (iter >= 0) {
+        const or_node *e = &L->pool[iter];
+        if (list_dist(i, j, e) < L->kernelDistance && value <= e->value) return 0; /* :53-59 */
+        iter = L->pool[iter].next;
+    }
+    for (int d = 0; d < deleteSize; d++) {
+        int deleted = delTo[d], deletedPrevious = delPrev[d];
+        if (deleted == L->first) {
+            if (L->first == L->last) {
+                L->first = L->last = -1;
+                previous = -1;
+            } else {
+                L->first = L->pool[L->first].next;
+                if (previous == deleted) previous = deletedPrevious;
+            }
+            if (L->removed >= 0 && L->removed != deleted) list_release(L, L->removed);
+            L->removed = deleted;
+        } else {
+            L->pool[deletedPrevious].next = L->pool[deleted].next;
+            if (L->last == deleted) L->last = deletedPrevious;
+            if (previous == deleted) previous = deletedPrevious;
+            list_release(L, deleted);
+        }
+        if (d + 1 < deleteSize && deleted == delPrev[d + 1]) delPrev[d + 1] = deletedPrevious;
+        L->pool[deleted].i = -1;
+        L->pool[deleted].j = -1;
+        L->pool[deleted].value = 0.0f;
+        L->pool[deleted].next = -1;
+        L->size--;
+    }
+    if (previous >= 0) {
+        int ne = list_alloc(L);
+        L->pool[ne].value = value;
+        L->pool[ne].i = i;
+        L->pool[ne].j = j;
+        L->pool[ne].next = L->pool[previous].next;
+        L->pool[previous].next = ne;
+        if (previous == L->last) L->last = ne;
+        L->size++;
+        if (L->size > L->maxSize) or_list_remove(L);
+    } else {
+        if (L->size == L->maxSize) return 0;
+        int e = list_alloc(L);
+        L->pool[e].value = value;
+        L->pool[e].i = i;
+        L->pool[e].j = j;
+        L->pool[e].next = -1;
+        if (L->first < 0 || L->last < 0) {
+            L->first = L->last = e;
+        } else {
+            L->pool[e].next = L->first;
+            L->first = e;
+        }
+        L->size++;
+    }
+    return 1;
+}
+
+/* Dump the list in iteration order (ascending): out = [size, (value,i,j)*] */
+int or_list_dump(const or_list *L, float *out)
+{
+    int n = 0;
+    for (int it = L->first; it >= 0; it = L->pool[it].next) {
+        out[3 * n + 0] = L->pool[it].value;
+        out[3 * n + 1] = L->pool[it].i;
+        out[3 * n + 2] = L->pool[it].j;
+        n++;
+    }
+    return n;
+}
+
+/* Run a scripted sequence of operations (for the list KATs):
+ * ops[k] = (opcode, value, i, j) with opcode 0 = addInAscendOrder, 1 = remove,
+ * 2 = clear.  Returns final size and the dumped entries; accepted[k] holds the
+ * boolean result of each add. */
+int or_list_script(int maxSize, int kernelWidth, int nops, const float *ops, int *accepted,
+                   float *out)
+{
+    or_list L;
+    or_list_init(&L, maxSize, kernelWidth);
+    for (int k = 0; k < nops; k++) {
+        int op = (int)ops[4 * k];
+        if (op == 0)
+            accepted[k] = or_list_add_ascend(&L, ops[4 * k + 1], (short)ops[4 * k + 2],
+                                             (short)ops[4 * k + 3]);
+        else if (op == 1)
+            or_list_remove(&L), accepted[k] = 1;
+        else {
+            while (L.size > 0) or_list_remove(&L); /* clear :367-371 */
+            accepted[k] = 1;
+        }
+    }
+    return or_list_dump(&L, out);
+}
+
+/* FindMaximumMultiPeaks.findMaximum :42-162.  out: P x (i, j, value) in
+ * descending order (peak 0 is the main peak), entries beyond the found count
+ * are zero; returns peaksFound; *minFloor receives the global minimum. */
+int or_find_max_multipeaks(const float *m, int dimI, int dimJ, int numberOfPeaks,
+                           int kernelWidth, float *out, float *minFloor)
+{
+#define EL(a, b) m[(a) * dimJ + (b)]
+    short k = (short)kernelWidth;
+    short iSlices = (short)(dimI / k);
+    if (iSlices * k != dimI) iSlices += 1;
+    short iSlicesBottom = (short)(iSlices / 2), iSlicesTop = (short)(iSlices - iSlicesBottom);
+    short jSlices = (short)(dimJ / k);
+    if (jSlices * k != dimJ) jSlices += 1;
+    short jSlicesRight = (short)(jSlices / 2), jSlicesLeft = (short)(jSlices - jSlicesRight);
+    (void)iSlicesBottom;
+    (void)jSlicesRight;
+    or_list L;
+    or_list_init(&L, numberOfPeaks, kernelWidth);
+    float maxs[4] = {0, 0, 0, 0};
+    short maxsI[4] = {0, 0, 0, 0}, maxsJ[4] = {0, 0, 0, 0};
+    float mn = 3.40282347e+38f;
+    for (short si = 0; si < iSlicesTop; si++) {
+        for (short sj = 0; sj < jSlicesLeft; sj++) {
+            short top = (short)(si * k), left = (short)(sj * k);
+            short bottom = (short)(si * k + iSlicesTop * k), right = (short)(sj * k + jSlicesLeft * k);
+            for (int q = 0; q < 4; q++) maxs[q] = 0.0f;
+            for (short oi = 0; oi < k; oi++) {
+                for (short oj = 0; oj < k; oj++) {
+                    short pI1 = (short)(top + oi), pJ1 = (short)(left + oj);
+                    float v1 = EL(pI1, pJ1);
+                    if (v1 > maxs[0]) { maxs[0] = v1; maxsI[0] = pI1; maxsJ[0] = pJ1; }
+                    if (v1 < mn) mn = v1;
+                    short pI2 = (short)(top + oi), pJ2 = (short)(right + oj);
+                    if (pJ2 < dimJ) {
+                        float v2 = EL(pI2, pJ2);
+                        if (v2 > maxs[1]) { maxs[1] = v2; maxsI[1] = pI2; maxsJ[1] = pJ2; }
+                        if (v2 < mn) mn = v2;
+                    }
+                    short pI3 = (short)(bottom + oi), pJ3 = (short)(left + oj);
+                    if (pI3 < dimI) {
+                        float v3 = EL(pI3, pJ3);
+                        if (v3 > maxs[2]) { maxs[2] = v3; maxsI[2] = pI3; maxsJ[2] = pJ3; }
+                        if (v3 < mn) mn = v3;
+                    }
+                    short pI4 = (short)(bottom + oi), pJ4 = (short)(right + oj);
+                    if (pI4 < dimI && pJ4 < dimJ) {
+                        float v4 = EL(pI4, pJ4);
+                        if (v4 > maxs[3]) { maxs[3] = v4; maxsI[3] = pI4; maxsJ[3] = pJ4; }
+                        if (v4 < mn) mn = v4;
+                    }
+                }
+            }
+            for (int q = 0; q < 4; q++) or_list_add_ascend(&L, maxs[q], maxsI[q], maxsJ[q]);
+        }
+    }
+#undef EL
+    float tmp[3 * (OR_LIST_MAXP + 2)];
+    int found = or_list_dump(&L, tmp);
+    for (int p = 0; p < numberOfPeaks; p++) out[3 * p] = out[3 * p + 1] = out[3 * p + 2] = 0.0f;
+    int index = found - 1;
+    for (int e = 0; e < found; e++, index--) {
+        out[3 * index + 0] = tmp[3 * e + 1];
+        out[3 * index + 1] = tmp[3 * e + 2];
+        out[3 * index + 2] = tmp[3 * e + 0];
+    }
+    *minFloor = mn;
+    return found;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Sub-pixel fits.  Peak records are (i, j, value); outputs (i', j', value',    */
+/* status) with status bit0 = I fit accepted, bit1 = J fit accepted,          */
+/* bit2 = skipped (border or (0,0)).                                          */
+/* ------------------------------------------------------------------------ */
+
+/* Closed-form 3-point Gaussian: the exact interpolant that the Levenberg-
+ * Marquardt GaussianCurveFitter of Gaussian1DInterpolator (:78-94) converges to
+ * for 3 samples. */
+static double gauss3_mean(double x0, double ym, double y0, double yp)
+{
+    double lm = log(ym), l0 = log(y0), lp = log(yp);
+    return x0 + (lm - lp) / (2.0 * lm - 4.0 * l0 + 2.0 * lp);
+}
+
+/* Least-squares Gaussian y = a exp(-(x-b)^2/(2c^2)) through n samples by
+ * damped Gauss-Newton (Levenberg-Marquardt) in double, started from the
+ * 3-point interpolant: the optimum that GaussianCurveFitter converges to. */
+static int gauss_lsq(int n, const double *x, const double *y, double *par)
+{
+    int c = n / 2;
+    double ymax = y[c];
+    double b = gauss3_mean(x[c], y[c - 1], y[c], y[c + 1]);
+    double l = log(y[c - 1]) - 2 * log(y[c]) + log(y[c + 1]);
+    double s = (l < 0) ? sqrt(-1.0 / l) : 1.0;
+    if (!isfinite(b) || fabs(b - x[c]) > 1.0) b = x[c];
+    if (!isfinite(s) || s <= 0) s = 1.0;
+    double a = ymax / exp(-(x[c] - b) * (x[c] - b) / (2 * s * s));
+    double p[3] = {a, b, s};
+    double lambda = 1e-3;
+    double cost = 0;
+    for (int k = 0; k < n; k++) {
+        double r = y[k] - p[0] * exp(-(x[k] - p[1]) * (x[k] - p[1]) / (2 * p[2] * p[2]));
+        cost += r * r;
+    }
+    for (int it = 0; it < 200; it++) {
+        double JtJ[3][3] = {{0}}, Jtr[3] = {0};
+        for (int k = 0; k < n; k++) {
+            double d = x[k] - p[1];
+            double e = exp(-d * d / (2 * p[2] * p[2]));
+            double f = p[0] * e;
+            double J[3] = {e, f * d / (p[2] * p[2]), f * d * d / (p[2] * p[2] * p[2])};
+            double r = y[k] - f;
+            for (int u = 0; u < 3; u++) {
+                Jtr[u] += J[u] * r;
+                for (int v = 0; v < 3; v++) JtJ[u][v] += J[u] * J[v];
+            }
+        }
+        int improved = 0;
+        for (int tries = 0; tries < 30 && !improved; tries++) {
+            double Am[3][4];
+            for (int u = 0; u < 3; u++) {
+                for (int v = 0; v < 3; v++) Am[u][v] = JtJ[u][v] + (u == v ? lambda * JtJ[u][u] : 0.0);
+                Am[u][3] = Jtr[u];
+            }
+            for (int col = 0; col < 3; col++) { /* Gaussian elimination, partial pivot */
+                int piv = col;
+                for (int r2 = col + 1; r2 < 3; r2++)
+                    if (fabs(Am[r2][col]) > fabs(Am[piv][col])) piv = r2;
+                for (int q = 0; q < 4; q++) {
+                    double t = Am[col][q]; Am[col][q] = Am[piv][q]; Am[piv][q] = t;
+                }
+                for (int r2 = col + 1; r2 < 3; r2++) {
+                    double fct = Am[r2][col] / Am[col][col];
+                    for (int q = col; q < 4; q++) Am[r2][q] -= fct * Am[col][q];
+                }
+            }
+            double dp[3];
+            for (int r2 = 2; r2 >= 0; r2--) {
+                double sum = Am[r2][3];
+                for (int q = r2 + 1; q < 3; q++) sum -= Am[r2][q] * dp[q];
+                dp[r2] = sum / Am[r2][r2];
+            }
+            double np[3] = {p[0] + dp[0], p[1] + dp[1], p[2] + dp[2]};
+            double nc = 0;
+            for (int k = 0; k < n; k++) {
+                double r = y[k] - np[0] * exp(-(x[k] - np[1]) * (x[k] - np[1]) / (2 * np[2] * np[2]));
+                nc += r * r;
+            }
+            if (isfinite(nc) && nc <= cost) {
+                double rel = (cost - nc) / (cost > 0 ? cost : 1);
+                p[0] = np[0]; p[1] = np[1]; p[2] = np[2];
+                cost = nc;
+                lambda *= 0.3;
+                improved = 1;
+                if (rel < 1e-15 && fabs(dp[1]) < 1e-13) it = 1000;
+            } else {
+                lambda *= 10;
+            }
+        }
+        if (!improved) break;
+    }
+    par[0] = p[0];
+    par[1] = p[1];
+    par[2] = fabs(p[2]);
+    return 0;
+}
+
+/* Gaussian1DInterpolator.interpolate :36-114 for one peak. */
+void or_gauss1d(const float *m, int dimI, int dimJ, int pixels, float minFloor,
+                const float *peak, float *res)
+{
+    int maxI = (int)peak[0], maxJ = (int)peak[1];
+    res[0] = peak[0];
+    res[1] = peak[1];
+    res[2] = peak[2];
+    res[3] = 4.0f;
+    if (maxI == 0.0f && maxJ == 0.0f) return;
+    if (maxI - pixels / 2 < 0 || maxI + pixels / 2 >= dimI) return;
+    if (maxJ - pixels / 2 < 0 || maxJ + pixels / 2 >= dimJ) return;
+    if (minFloor < -3.15f) { res[3] = 8.0f; return; } /* InterpolateException */
+    float add = 0.0f;
+    if (minFloor < 0.0f) add = -minFloor + 1e-8f;
+    double xs[16], ys[16], par[3];
+    int n = 0;
+    for (int i = maxI - pixels / 2; i <= maxI + pixels / 2; i++, n++) {
+        xs[n] = i;
+        ys[n] = (double)(m[i * dimJ + maxJ] + add);
+    }
+    double meanI = (pixels == 3) ? gauss3_mean(xs[0] + 1, ys[0], ys[1], ys[2])
+                                 : (gauss_lsq(n, xs, ys, par), par[1]);
+    n = 0;
+    for (int j = maxJ - pixels / 2; j <= maxJ + pixels / 2; j++, n++) {
+        xs[n] = j;
+        ys[n] = (double)(m[maxI * dimJ + j] + add);
+    }
+    double meanJ = (pixels == 3) ? gauss3_mean(xs[0] + 1, ys[0], ys[1], ys[2])
+                                 : (gauss_lsq(n, xs, ys, par), par[1]);
+    int st = 0;
+    if (fabs(meanI - peak[0]) < 2.0f) { res[0] = (float)meanI; st |= 1; }
+    if (fabs(meanJ - peak[1]) < 2.0f) { res[1] = (float)meanJ; st |= 2; }
+    res[3] = (float)st;
+}
+
+/* commons-math3 3.6.1 LUDecomposition (partial pivoting, singularity
+ * threshold 1e-11) and its solver, used by Gaussian1DHongweiGuoInterpolator
+ * .estimateGaussianParameters :113-116.  Returns 0 when singular. */
+static int lu_solve3(const double A[3][3], const double b[3], double x[3])
+{
+    double lu[3][3];
+    int piv[3] = {0, 1, 2};
+    memcpy(lu, A, sizeof(lu));
+    for (int col = 0; col < 3; col++) {
+        for (int row = 0; row < col; row++) {
+            double sum = lu[row][col];
+            for (int i = 0; i < row; i++) sum -= lu[row][i] * lu[i][col];
+            lu[row][col] = sum;
+        }
+        int max = col;
+        double largest = -INFINITY;
+        for (int row = col; row < 3; row++) {
+            double sum = lu[row][col];
+            for (int i = 0; i < col; i++) sum -= lu[row][i] * lu[i][col];
+            lu[row][col] = sum;
+            if (fabs(sum) > largest) { largest = fabs(sum); max = row; }
+        }
+        if (fabs(lu[max][col]) < 1e-11) return 0;
+        if (max != col) {
+            for (int i = 0; i < 3; i++) {
+                double t = lu[max][i]; lu[max][i] = lu[col][i]; lu[col][i] = t;
+            }
+            int t = piv[max]; piv[max] = piv[col]; piv[col] = t;
+        }
+        double d = lu[col][col];
+        for (int row = col + 1; row < 3; row++) lu[row][col] /= d;
+    }
+    double bp[3];
+    for (int r = 0; r < 3; r++) bp[r] = b[piv[r]];
+    for (int col = 0; col < 3; col++) {
+        double bc = bp[col];
+        for (int i = col + 1; i < 3; i++) bp[i] -= bc * lu[i][col];
+    }
+    for (int col = 2; col >= 0; col--) {
+        bp[col] /= lu[col][col];
+        double bc = bp[col];
+        for (int i = 0; i < col; i++) bp[i] -= bc * lu[i][col];
+    }
+    for (int r = 0; r < 3; r++) x[r] = bp[r];
+    return 1;
+}
+
+/* Pseudo-inverse solve of the symmetric normal matrix through a cyclic Jacobi
+ * eigen-decomposition: the SingularValueDecomposition fallback (:117-122).
+ * Parity unpinned (commons-math's SVD iteration is not restated). */
+static void pinv_solve3(const double A[3][3], const double b[3], double x[3])
+{
+    double a[3][3], v[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    memcpy(a, A, sizeof(a));
+    for (int sweep = 0; sweep < 50; sweep++) {
+        double off = fabs(a[0][1]) + fabs(a[0][2]) + fabs(a[1][2]);
+        if (off < 1e-300) break;
+        for (int p = 0; p < 2; p++)
+            for (int q = p + 1; q < 3; q++) {
+                if (a[p][q] == 0.0) continue;
+                double th = (a[q][q] - a[p][p]) / (2 * a[p][q]);
+                double t = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1));
+                double c = 1 / sqrt(t * t + 1), s = t * c;
+                for (int k = 0; k < 3; k++) {
+                    double akp = a[k][p], akq = a[k][q];
+                    a[k][p] = c * akp - s * akq;
+                    a[k][q] = s * akp + c * akq;
+                }
+                for (int k = 0; k < 3; k++) {
+                    double apk = a[p][k], aqk = a[q][k];
+                    a[p][k] = c * apk - s * aqk;
+                    a[q][k] = s * apk + c * aqk;
+                }
+                for (int k = 0; k < 3; k++) {
+                    double vkp = v[k][p], vkq = v[k][q];
+                    v[k][p] = c * vkp - s * vkq;
+                    v[k][q] = s * vkp + c * vkq;
+                }
+            }
+    }
+    double smax = 0;
+    for (int k = 0; k < 3; k++) smax = fmax(smax, fabs(a[k][k]));
+    double tol = 3 * smax * 2.220446049250313e-16;
+    for (int r = 0; r < 3; r++) x[r] = 0;
+    for (int k = 0; k < 3; k++) {
+        double s = a[k][k];
+        if (fabs(s) <= tol) continue;
+        double proj = 0;
+        for (int r = 0; r < 3; r++) proj += v[r][k] * b[r];
+        for (int r = 0; r < 3; r++) x[r] += v[r][k] * proj / s;
+    }
+}
+
+/* Gaussian1DHongweiGuoInterpolator.initMatrices :47-70 / updateMatrices :72-102
+ * / estimateGaussianParameters :104-130.  xs, ys are float arrays (Java float[]);
+ * the y^2 of the data term is a float product, everything else double. */
+static void guo_estimate(int iterations, int n, const float *xs, const float *ys, float *mean,
+                         float *amp)
+{
+    double A[3][3] = {{0}}, b[3] = {0}, coefs[3] = {0, 0, 0};
+    for (int i = 0; i < n; i++) {
+        double ySquare = (double)(ys[i] * ys[i]);
+        double xSquare = (double)(xs[i] * xs[i]);
+        double xCubic = xSquare * xs[i];
+        double xFourth = xCubic * xs[i];
+        double logy = log((double)ys[i]);
+        A[0][0] += ySquare;
+        A[0][1] += xs[i] * ySquare;
+        A[0][2] += xSquare * ySquare;
+        b[0] += ySquare * logy;
+        A[1][0] += xs[i] * ySquare;
+        A[1][1] += xSquare * ySquare;
+        A[1][2] += xCubic * ySquare;
+        b[1] += xs[i] * ySquare * logy;
+        A[2][0] += xSquare * ySquare;
+        A[2][1] += xCubic * ySquare;
+        A[2][2] += xFourth * ySquare;
+        b[2] += xSquare * ySquare * logy;
+    }
+    for (int it = 0; it < iterations; it++) {
+        if (!lu_solve3(A, b, coefs)) pinv_solve3(A, b, coefs);
+        memset(A, 0, sizeof(A));
+        memset(b, 0, sizeof(b));
+        for (int i = 0; i < n; i++) {
+            double yk = exp(coefs[0] + coefs[1] * xs[i] + coefs[2] * xs[i] * xs[i]);
+            double ySquare = yk * yk;
+            double xSquare = (double)(xs[i] * xs[i]);
+            double xCubic = xSquare * xs[i];
+            double xFourth = xCubic * xs[i];
+            double logy = log((double)ys[i]);
+            A[0][0] += ySquare;
+            A[0][1] += xs[i] * ySquare;
+            A[0][2] += xSquare * ySquare;
+            b[0] += ySquare * logy;
+            A[1][0] += xs[i] * ySquare;
+            A[1][1] += xSquare * ySquare;
+            A[1][2] += xCubic * ySquare;
+            b[1] += xs[i] * ySquare * logy;
+            A[2][0] += xSquare * ySquare;
+            A[2][1] += xCubic * ySquare;
+            A[2][2] += xFourth * ySquare;
+            b[2] += xSquare * ySquare * logy;
+        }
+    }
+    *mean = (float)(-coefs[1] / (2 * coefs[2]));
+    *amp = (float)(exp(coefs[0] - coefs[1] * coefs[1] / (4 * coefs[2])));
+}
+
+/* Gaussian1DHongweiGuoInterpolator.interpolate :133-248 for one peak. */
+void or_guo(const float *m, int dimI, int dimJ, int pixels, int iterations, float minFloor,
+            const float *peak, float *res)
+{
+    int maxI = (int)peak[0], maxJ = (int)peak[1];
+    res[0] = peak[0];
+    res[1] = peak[1];
+    res[2] = peak[2];
+    res[3] = 4.0f;
+    if (maxI == 0.0f && maxJ == 0.0f) return;
+    if (maxI - pixels / 2 < 0 || maxI + pixels / 2 >= dimI) return;
+    if (maxJ - pixels / 2 < 0 || maxJ + pixels / 2 >= dimJ) return;
+    if (minFloor < -5.15f) { res[3] = 8.0f; return; }
+    float add = 0.0f;
+    if (minFloor < 0.0f) add = -minFloor + 1e-8f;
+    float xs[16], ys[16];
+    int n = 0;
+    for (int mi = maxI - pixels / 2; mi <= maxI + pixels / 2; mi++, n++) {
+        xs[n] = (float)mi;
+        ys[n] = m[mi * dimJ + maxJ] + add;
+    }
+    float yMean, yAmp, xMean, xAmp;
+    guo_estimate(iterations, n, xs, ys, &yMean, &yAmp);
+    n = 0;
+    for (int mj = maxJ - pixels / 2; mj <= maxJ + pixels / 2; mj++, n++) {
+        xs[n] = (float)mj;
+        ys[n] = m[maxI * dimJ + mj] + add;
+    }
+    guo_estimate(iterations, n, xs, ys, &xMean, &xAmp);
+    float finalY = yMean, finalX = xMean;
+    int yValid = 1, xValid = 1;
+    if (fabsf(yMean - (float)maxI) > pixels / 2.0f) { finalY = (float)maxI; yValid = 0; }
+    if (fabsf(xMean - (float)maxJ) > pixels / 2.0f) { finalX = (float)maxJ; xValid = 0; }
+    /* NaN means fail the '>' test and are kept, exactly as in Java */
+    float mx = peak[2];
+    if (yValid) {
+        mx = yAmp;
+        if (xValid && xAmp > mx) mx = xAmp;
+    } else if (xValid) {
+        mx = xAmp;
+    }
+    res[0] = finalY;
+    res[1] = finalX;
+    res[2] = mx;
+    res[3] = (float)(yValid | (xValid << 1));
+}
+
+/* Gaussian1DPolynomialInterpolator.interpolate :25-106 for one peak. */
+void or_poly(const float *m, int dimI, int dimJ, const float *peak, float *res)
+{
+    int maxI = (int)peak[0], maxJ = (int)peak[1];
+    res[0] = peak[0];
+    res[1] = peak[1];
+    res[2] = peak[2];
+    res[3] = 4.0f;
+    if (maxI == 0.0f && maxJ == 0.0f) return;
+    if (maxI - 1 < 0 || maxI + 1 >= dimI) return;
+    if (maxJ - 1 < 0 || maxJ + 1 >= dimJ) return;
+    int doGaussian = 1;
+    float vy[3], vx[3];
+    for (int i = -1; i < 2; i++)
+        for (int j = -1; j < 2; j++) {
+            float value = m[(maxI + i) * dimJ + (j + maxJ)];
+            if (j == 0) {
+                if (value < 0.0f) doGaussian = 0;
+                vy[i + 1] = value + 1e-8f;
+            }
+            if (i == 0) {
+                if (value < 0.0f) doGaussian = 0;
+                vx[j + 1] = value + 1e-8f;
+            }
+        }
+    float peakI = peak[0], peakJ = peak[1];
+    int st = 0;
+    if (doGaussian) {
+        float ti = maxI + (float)(log(vy[0]) - log(vy[2])) /
+                              (float)(2.0f * log(vy[0]) - 4.0f * log(vy[1]) + 2.0f * log(vy[2]));
+        if (isfinite(ti) && fabsf(ti - maxI) <= 1.0f) { peakI = ti; st |= 1; }
+        float tj = maxJ + (float)(log(vx[0]) - log(vx[2])) /
+                              (float)(2.0f * log(vx[0]) - 4.0f * log(vx[1]) + 2.0f * log(vx[2]));
+        if (isfinite(tj) && fabsf(tj - maxJ) <= 1.0f) { peakJ = tj; st |= 2; }
+    } else {
+        float ti = maxI + (float)(vy[0] - vy[2]) / (float)(2.0f * vy[0] - 4.0f * vy[1] + 2.0f * vy[2]);
+        if (isfinite(ti) && fabsf(ti - maxI) <= 1.0f) { peakI = ti; st |= 1; }
+        float tj = maxJ + (float)(vx[0] - vx[2]) / (float)(2.0f * vx[0] - 4.0f * vx[1] + 2.0f * vx[2]);
+        if (isfinite(tj) && fabsf(tj - maxJ) <= 1.0f) { peakJ = tj; st |= 2; }
+    }
+    res[0] = peakI;
+    res[1] = peakJ;
+    res[3] = (float)st;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Tiles: clipping and bilinear micro warping                               */
+/* ------------------------------------------------------------------------ */
+
+/* Image.clipImageMatrix :386-494 (new matrix, partial clipping allowed) with the
+ * MatrixByte maximum (clamped >= 16; MatrixByte.java:16-39, :278-317).  Returns
+ * the tile max. */
+float or_clip_tile(const uint8_t *img, int H, int W, int top, int left, int N, uint8_t *tile)
+{
+    int maxv = 0;
+    for (int i = 0; i < N; i++)
+        for (int j = 0; j < N; j++) {
+            int y = top + i, x = left + j;
+            uint8_t v = 0;
+            if (y >= 0 && y < H && x >= 0 && x < W) {
+                v = img[(size_t)y * W + x];
+                if (v > maxv) maxv = v;
+            }
+            tile[i * N + j] = v;
+        }
+    if (maxv < 16) maxv = 16;
+    return (float)maxv;
+}
+
+/* FastMath.round(float) = (int)floor(x + 0.5f) (commons-math3 3.6.1). */
+static int fm_round(float x) { return (int)floorf(x + 0.5f); }
+
+/* BiLinearImageMicroAndMiniWarpingStrategy.warpPixelOnImage :336-407,
+ * SecondImage order (source = target + displacement). */
+static float warp_pixel(const uint8_t *img, int H, int W, float u, float v, short i, short j)
+{
+    float sI = i + u, sJ = j + v;
+    short pI = (short)fm_round(sI), pJ = (short)fm_round(sJ);
+    if (pI < 0) { sI = 0; pI = 0; }
+    else if (pI >= H) { sI = H - 1; pI = (short)(H - 1); }
+    if (pJ < 0) { sJ = 0; pJ = 0; }
+    else if (pJ >= W) { sJ = W - 1; pJ = (short)(W - 1); }
+    float dU = sI - pI, dV = sJ - pJ;
+    short oI = dU >= 0 ? 1 : -1, oJ = dV >= 0 ? 1 : -1;
+    if (oI + pI < 0 || oI + pI >= H) oI = 0;
+    if (oJ + pJ < 0 || oJ + pJ >= W) oJ = 0;
+    float wU = fabsf(dU), wV = fabsf(dV);
+    float tl = img[(size_t)pI * W + pJ];
+    float tr = img[(size_t)pI * W + pJ + oJ];
+    float bl = img[(size_t)(pI + oI) * W + pJ];
+    float br = img[(size_t)(pI + oI) * W + pJ + oJ];
+    float f = (1.0f - wU) * (1.0f - wV) * tl + (1.0f - wU) * wV * tr + wU * (1.0f - wV) * bl +
+              wU * wV * br;
+    return (float)fm_round(f);
+}
+
+/* Micro warp of tile B in SecondImage mode (warpAndClipImage :132-148,
+ * createMicroDisplacements :188-201, doBiLinearWarp :292-334): the window is
+ * shifted by floor(u), floor(v) and every pixel sampled at the fractional part.
+ * Output tile values are integers 0..255; the reference gives the warped tile
+ * max = 255 (image.getMaximumValue(), :298). */
+void or_warp_micro(const uint8_t *img, int H, int W, int top, int left, int N, float dispU,
+                   float dispV, uint8_t *tile)
+{
+    short tTop = (short)(top + floor(dispU));
+    short tLeft = (short)(left + floor(dispV));
+    float fu = dispU - (float)floor(dispU);
+    float fv = dispV - (float)floor(dispV);
+    for (short i = 0; i < N; i++)
+        for (short j = 0; j < N; j++)
+            tile[i * N + j] =
+                (uint8_t)warp_pixel(img, H, W, fu, fv, (short)(i + tTop), (short)(j + tLeft));
+}

@@ -1,0 +1,346 @@
+"""Pin the CPU oracle (the C restatement of the reference's Java path) against the
+reference's golden fixtures, its unit-test KATs and what the mathematics fixes.
+
+These run on the CPU only; they are the reason the GPU parity tests may trust
+the oracle.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import gaussian_plane, load_json
+
+
+# --------------------------------------------------------------------------
+# FFT building blocks
+# --------------------------------------------------------------------------
+
+@pytest.mark.parametrize("m", range(1, 9))
+def test_perfect_shuffle_is_bit_reversal(m):
+    n = 1 << m
+    out = O.perfect_shuffle(np.arange(n, dtype=np.float32)).astype(int)
+    br = [int(format(i, "0%db" % m)[::-1], 2) for i in range(n)] if m else [0]
+    assert list(out) == br
+
+
+@pytest.mark.parametrize("M", [8, 16, 32, 64, 128, 256])
+def test_sine_table_robust_to_last_ulp_of_double_sin(M):
+    """The reference builds its twiddles as (float)FastMath.sin(double).  FastMath.sin
+    and libm's sin may differ in the last double ulp; the float table is the same
+    for both whenever no entry lies within a few double ulps of a float rounding
+    boundary.  Checked here for every table size used."""
+    ws = O.sine_table(M)
+    for k in range(M):
+        d = math.sin(2.0 * math.pi * k / M)
+        assert ws[k] == np.float32(d)
+        if d == 0.0:
+            continue
+        f = np.float32(d)
+        # distance of d to the nearest midpoint between adjacent floats
+        lo = np.nextafter(f, np.float32(-np.inf))
+        hi = np.nextafter(f, np.float32(np.inf))
+        mid_lo = (float(f) + float(lo)) / 2
+        mid_hi = (float(f) + float(hi)) / 2
+        margin = min(abs(d - mid_lo), abs(d - mid_hi))
+        assert margin > 16 * np.spacing(abs(d)), (M, k, margin)
+
+
+def test_sin_pi_is_not_special_cased():
+    ws = O.sine_table(64)
+    assert ws[32] != 0.0 and abs(ws[32]) < 1e-15  # sin(pi) in double, cast to float
+
+
+# --------------------------------------------------------------------------
+# Cross-correlation
+# --------------------------------------------------------------------------
+
+def test_fastreal_kat_against_direct_sum():
+    """FastRealFFTXCorrTests.java:31-51,195-199: un-normalised 4x4 tiles, FFT result
+    equals the direct sum within 1e-3."""
+    kat = load_json("fastreal_kat.json")
+    F = np.array(kat["F"], np.float32)
+    G = np.array(kat["G"], np.float32)
+    got = O.xcorr_floats(F, G)
+    ref = O.xcorr_direct64(F[::-1, ::-1], G)
+    np.testing.assert_allclose(got, ref, atol=kat["tol"], rtol=0)
+
+
+@pytest.mark.parametrize("N", [2, 4, 8, 16])
+@pytest.mark.parametrize("mode", [0, 1])
+def test_xcorr_restatement_matches_direct_sum(N, mode):
+    rng = np.random.default_rng(100 + N + 7 * mode)
+    A = rng.integers(0, 256, (N, N)).astype(np.float32)
+    B = rng.integers(0, 256, (N, N)).astype(np.float32)
+    mA, mB = max(16.0, A.max()), max(16.0, B.max())
+    if mode == 0:
+        got = O.xcorr_javacpu(A, B, mA)
+        F, G = O.normalise_pair(A, B, mA, mA)
+    else:
+        got = O.xcorr_javagpu(A, B, mA, mB)
+        F, G = O.normalise_pair(A, B, mA, mB)
+    ref = O.xcorr_direct64(F, G)
+    assert np.max(np.abs(got - ref)) <= 2e-6 * np.max(ref)
+    # the argmax of the plane is the brute-force one
+    assert np.unravel_index(np.argmax(got), got.shape) == np.unravel_index(np.argmax(ref), ref.shape)
+
+
+def test_xcorr_modes_differ_only_by_normalisation_and_inverse_twiddles():
+    """With A's max used for G too, the GPU-job arithmetic and the CPU path compute
+    the same plane up to float rounding of the inverse twiddles."""
+    rng = np.random.default_rng(5)
+    A = rng.integers(0, 256, (32, 32)).astype(np.float32)
+    B = rng.integers(0, 256, (32, 32)).astype(np.float32)
+    c = O.xcorr_javacpu(A, B, 200.0)
+    g = O.xcorr_javagpu(A, B, 200.0, 200.0)
+    assert np.max(np.abs(c - g)) <= 1e-6 * np.max(c)
+
+
+def test_shift_of_ten_pixels_lands_at_expected_peak():
+    """ImageCrossCorrelationRealFFTParStdJobTests.java:232-306 recreated with a
+    synthetic image: B's window sits 10 px further into the image than A's, so
+    the content moved by -10 px and the plane maximum sits at (dim/2 - 10,
+    dim/2 - 10), as the reference's test asserts."""
+    rng = np.random.default_rng(3)
+    img = np.where(rng.random((128, 128)) < 0.05, rng.integers(150, 256, (128, 128)), 0)
+    img = img.astype(np.uint8)  # sparse particles, as in a PIV image
+    N = 32
+    A = img[40:40 + N, 40:40 + N].astype(np.float32)
+    B = img[50:50 + N, 50:50 + N].astype(np.float32)
+    p = O.xcorr_javagpu(A, B, A.max(), B.max())
+    i, j = np.unravel_index(np.argmax(p), p.shape)
+    assert (i, j) == (N - 1 - 10, N - 1 - 10)
+
+
+def _rankine_tiles(rankine_pair, N=16):
+    a, b = rankine_pair
+    n = 512 // N
+    tA = a.reshape(n, N, n, N).transpose(0, 2, 1, 3).reshape(-1, N, N)
+    tB = b.reshape(n, N, n, N).transpose(0, 2, 1, 3).reshape(-1, N, N)
+    mA = np.maximum(tA.reshape(len(tA), -1).max(1), 16).astype(np.float32)
+    mB = np.maximum(tB.reshape(len(tB), -1).max(1), 16).astype(np.float32)
+    return np.ascontiguousarray(tA), np.ascontiguousarray(tB), mA, mB
+
+
+def test_golden_gpu_planes(rankine_pair, golden_planes):
+    """The reference's own device self-test (GpuFFTBenchmarkJob.java:161-245,316-334):
+    16 x 16 IAs without overlap over the 512^2 Rankine pair, every element within
+    1e-2 of the stored planes, with per-tile normalisation."""
+    tA, tB, mA, mB = _rankine_tiles(rankine_pair)
+    planes = O.xcorr_batch_u8(tA, tB, mA, mB, 1)
+    assert planes.shape == golden_planes.shape == (1024, 31, 31)
+    err = np.abs(planes - golden_planes)
+    assert err.max() <= 1e-2, err.max()
+    am = planes.reshape(1024, -1).argmax(1)
+    ag = golden_planes.reshape(1024, -1).argmax(1)
+    assert np.all(am == ag)
+    # the CPU-path normalisation does not reproduce the fixture (F5 of the survey)
+    cpu = O.xcorr_batch_u8(tA[:64], tB[:64], mA[:64], mA[:64], 0)
+    assert np.abs(cpu - golden_planes[:64]).max() > 1.0
+
+
+# --------------------------------------------------------------------------
+# Peak search
+# --------------------------------------------------------------------------
+
+def from_center_brute(plane):
+    """Sequential semantics of FindMaximumFromCenter restated as a key order:
+    largest value over rows/cols 1..D-1, then smallest distance to
+    (D/2+1, D/2+1), then earliest visit; (0,0) with value 0 when nothing > 0."""
+    D = plane.shape[0]
+    h = D // 2
+    c = h + 1
+    visit = {}
+    idx = 0
+
+    def mark(i, j):
+        nonlocal idx
+        if (i, j) not in visit:
+            visit[(i, j)] = idx
+        idx += 1
+    mark(c, c)
+    for i in range(h):
+        mark(h - i, c)
+        mark(h + 1 + i, c)
+    for j in range(h):
+        mark(c, h - j)
+        mark(c, h + 1 + j)
+    for i in range(h):
+        for j in range(h):
+            mark(h - i, h - j)
+            mark(h + 1 + i, h - j)
+            mark(h - i, h + 1 + j)
+            mark(h + 1 + i, h + 1 + j)
+    best = None
+    for (i, j), v in visit.items():
+        val = plane[i, j]
+        if not (val >= 0):
+            continue
+        key = (-val, (i - c) ** 2 + (j - c) ** 2, v)
+        if best is None or key < best[0]:
+            best = (key, i, j, val)
+    mn = min(plane[i, j] for (i, j) in visit)
+    if best is None:
+        return (0, 0, 0.0, mn)
+    return (best[1], best[2], best[3], mn)
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_from_center_vs_brute_force(seed):
+    rng = np.random.default_rng(seed)
+    D = [7, 15, 31, 63][seed % 4]
+    plane = rng.integers(0, 6, (D, D)).astype(np.float32)  # many ties
+    if seed % 2:
+        plane[0, :] = 99.0  # row 0 / col 0 are never visited
+        plane[:, 0] = 99.0
+    got = O.find_max_from_center(plane)
+    exp = from_center_brute(plane)
+    assert (int(got[0]), int(got[1]), float(got[2])) == (exp[0], exp[1], float(exp[2]))
+    assert got[3] == exp[3]
+
+
+def test_from_center_all_negative_keeps_origin():
+    plane = -np.ones((15, 15), np.float32)
+    got = O.find_max_from_center(plane)
+    assert tuple(got[:3]) == (0.0, 0.0, 0.0) and got[3] == -1.0
+
+
+def test_multipeak_kat():
+    kat = load_json("multipeak_kat.json")
+    plane = np.array(kat["plane"], np.float32)
+    peaks, found, _ = O.find_max_multipeaks(plane, kat["numberOfPeaks"], kat["kernelWidth"])
+    for p, e in enumerate(kat["expect"]):
+        if e[0] is not None:
+            assert (peaks[p, 0], peaks[p, 1]) == (e[0], e[1])
+        assert abs(peaks[p, 2] - e[2]) <= kat["tol"]
+
+
+@pytest.mark.parametrize("case", load_json("list_kats.json"), ids=lambda c: c["source"].split()[0])
+def test_list_kats(case):
+    lst, _ = O.list_script(case["maxSize"], case["kernelWidth"], case["ops"])
+    assert [list(map(float, e)) for e in lst] == [list(map(float, e)) for e in case["expect"]]
+
+
+def test_list_against_set_semantics_fuzz():
+    """Random insert sequences: the list always holds at most P entries, ascending,
+    and no two entries within the suppression distance."""
+    rng = np.random.default_rng(11)
+    for _ in range(200):
+        P, k = int(rng.integers(1, 6)), int(rng.integers(2, 6))
+        ops = [[0, float(rng.integers(0, 50)), int(rng.integers(0, 20)), int(rng.integers(0, 20))]
+               for _ in range(int(rng.integers(1, 40)))]
+        lst, _ = O.list_script(P, k, ops)
+        assert len(lst) <= P
+        vals = [e[0] for e in lst]
+        assert vals == sorted(vals)
+        for a in range(len(lst)):
+            for b in range(a + 1, len(lst)):
+                d2 = (lst[a][1] - lst[b][1]) ** 2 + (lst[a][2] - lst[b][2]) ** 2
+                assert d2 >= (k - 1) * k
+
+
+# --------------------------------------------------------------------------
+# Sub-pixel fits
+# --------------------------------------------------------------------------
+
+GAUSS_CASES = load_json("gauss1d_cases.json")
+
+
+@pytest.mark.parametrize("case", GAUSS_CASES, ids=lambda c: c["file"][28:-9] + "." + c["test"])
+def test_reference_gaussian_cases(case):
+    """The 567 Gaussian1DInterpolatorAdvanced* cases: noise-free Gaussians, the fit
+    must recover the centre within the reference's tolerance (8e-3 px)."""
+    dim = case["dim"]
+    plane = gaussian_plane(dim, case["uX"], case["sigmaX"], case["uY"], case["sigmaY"])
+    peak = np.array([dim // 2, dim // 2, 1000.0], np.float32)
+    if case["kind"] == "guo":
+        res = O.guo(plane, case["pixels"], 20, 0.0, peak)
+    else:
+        res = O.gauss1d(plane, case["pixels"], 0.0, peak)
+    assert abs(res[0] - (case["uY"] + dim // 2)) <= case["tol"]
+    assert abs(res[1] - (case["uX"] + dim // 2)) <= case["tol"]
+
+
+def test_gauss3_closed_form_exact():
+    """Three samples of an exact Gaussian: the interpolant recovers the centre to
+    double precision (the LM fitter converges to the same interpolant)."""
+    for mu in (-0.45, -0.2, 0.0, 0.13, 0.49):
+        for s in (0.6, 1.0, 2.5):
+            x = np.arange(29, 34, dtype=np.float64)
+            y = (5e4 * np.exp(-(x - 31 - mu) ** 2 / (2 * s * s))).astype(np.float32)
+            plane = np.ones((63, 63), np.float32)
+            plane[29:34, 31] = y
+            plane[31, 29:34] = y
+            r = O.gauss1d(plane, 3, 0.0, np.array([31, 31, y[2]], np.float32))
+            assert abs(r[0] - (31 + mu)) < 2e-5 and abs(r[1] - (31 + mu)) < 2e-5
+
+
+def test_guo_exact_on_noise_free_gaussian_and_amplitude():
+    for pixels in (3, 5, 7):
+        x = np.arange(0, 63, dtype=np.float64)
+        mu, s, a = 30.3, 1.3, 7e4
+        y = (a * np.exp(-(x - mu) ** 2 / (2 * s * s))).astype(np.float32)
+        plane = np.ones((63, 63), np.float32)
+        plane[:, 30] = y
+        plane[30, :] = y
+        r = O.guo(plane, pixels, 20, 0.0, np.array([30, 30, y[30]], np.float32))
+        assert abs(r[0] - mu) < 1e-4 and abs(r[1] - mu) < 1e-4
+        assert abs(r[2] - a) / a < 1e-4  # the peak value is replaced by the amplitude
+
+
+def test_guo_rejects_far_fit_and_border():
+    plane = np.ones((31, 31), np.float32)
+    r = O.guo(plane, 3, 20, 0.0, np.array([0, 15, 1.0], np.float32))
+    assert r[3] == 4.0 and r[0] == 0.0  # border: integer peak kept
+
+
+def test_poly_closed_form():
+    y = np.array([np.exp(-(1.3) ** 2 / 2), np.exp(-(0.3) ** 2 / 2), np.exp(-(0.7) ** 2 / 2)]) * 1e4
+    plane = np.ones((9, 9), np.float32)
+    plane[3:6, 4] = y
+    plane[4, 3:6] = y
+    r = O.poly(plane, np.array([4, 4, y[1]], np.float32))
+    assert abs(r[0] - 4.3) < 1e-4 and abs(r[1] - 4.3) < 1e-4
+
+
+# --------------------------------------------------------------------------
+# Tiles
+# --------------------------------------------------------------------------
+
+def test_clip_tile_zero_fill_and_max_clamp():
+    rng = np.random.default_rng(1)
+    img = rng.integers(0, 12, (40, 50)).astype(np.uint8)
+    t, mx = O.clip_tile(img, -3, 45, 16)
+    ref = np.zeros((16, 16), np.uint8)
+    ref[3:, :5] = img[0:13, 45:50]
+    assert np.array_equal(t, ref) and mx == 16.0
+    img[5, 5] = 200
+    t, mx = O.clip_tile(img, 0, 0, 16)
+    assert np.array_equal(t, img[:16, :16]) and mx == 200.0
+
+
+def test_warp_micro_invariants():
+    rng = np.random.default_rng(2)
+    img = rng.integers(0, 256, (96, 96)).astype(np.uint8)
+    # zero displacement: identity; integer displacement: pure shift
+    assert np.array_equal(O.warp_micro(img, 20, 30, 16, 0.0, 0.0), img[20:36, 30:46])
+    assert np.array_equal(O.warp_micro(img, 20, 30, 16, 3.0, -2.0), img[23:39, 28:44])
+    # constant fraction: separable bilinear with round-half-up, away from borders
+    du, dv = 2.25, 1.5
+    got = O.warp_micro(img, 20, 30, 16, du, dv).astype(np.float64)
+    f = img.astype(np.float64)
+    # frac 0.25 rounds to p = i (delta +0.25), frac 0.5 rounds up (delta -0.5)
+    ref = np.zeros((16, 16))
+    for i in range(16):
+        for j in range(16):
+            si, sj = 22 + i + 0.25, 31 + j + 0.5
+            pi, pj = int(np.floor(si + 0.5)), int(np.floor(sj + 0.5))
+            di, dj = si - pi, sj - pj
+            oi, oj = (1 if di >= 0 else -1), (1 if dj >= 0 else -1)
+            wi, wj = abs(di), abs(dj)
+            v = ((1 - wi) * (1 - wj) * f[pi, pj] + (1 - wi) * wj * f[pi, pj + oj] +
+                 wi * (1 - wj) * f[pi + oi, pj] + wi * wj * f[pi + oi, pj + oj])
+            ref[i, j] = np.floor(v + 0.5)
+    assert np.max(np.abs(got - ref)) <= 1.0  # float vs double rounding at x.5 only
+    assert np.mean(got == ref) > 0.98
