@@ -1,0 +1,157 @@
+/*
+ * qlpiv.h -- C ABI of the MI355X PIV correlation library (libqlpiv.so).
+ *
+ * Drop-in for QuickLabPIV-ng's correlation hot path.  Paths in the citations
+ * are relative to the reference's src/main/java/pt/quickLabPIV/.  The Java
+ * side binds these entry points through JNI or the FFM API (see
+ * INTEGRATION.md); nothing here uses C++ or PyTorch types.
+ *
+ * Conventions
+ *   - Every entry point returns an int status: QLP_OK (0) or a QLP_ERR_* code;
+ *     qlp_last_error() returns the message of the calling thread's last error.
+ *     QLP_ERR_ARG corresponds to the reference's RuntimeException on invalid
+ *     tile lists (CrossCorrelationFFTTemplate.analyzeTilesHelper :594-642),
+ *     QLP_ERR_DEVICE to JobAnalyzeException, QLP_ERR_COMPUTE to
+ *     JobComputeException.  Per-IA fit fallbacks are not errors: they are
+ *     reported in the status field of each peak record.
+ *   - Entry points without a _dev suffix take host buffers, copy them to the
+ *     device, run, copy results back and synchronise.  _dev entry points take
+ *     device pointers and enqueue on the context's stream without
+ *     synchronising (qlp_ctx_synchronize waits).
+ *   - A context owns one HIP stream.  Calls on one context are serialised by
+ *     the caller (one context per calling thread, as the reference keeps one
+ *     xcorr job per worker thread: OpenClGpuManager.compute :124-140).
+ *   - Tiles are N x N row-major uint8 (IA pixels); N is 16, 32 or 64.
+ *     Correlation planes are (2N-1) x (2N-1) row-major float32, element
+ *     [i][j] = sum_nm F'[n][m] G'[n+i-(N-1)][m+j-(N-1)], zero displacement at
+ *     [N-1][N-1] (MaxCrossResult.java:141-149).
+ */
+#ifndef QLPIV_H
+#define QLPIV_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QLP_OK 0
+#define QLP_ERR_ARG 1         /* invalid shapes, counts or options */
+#define QLP_ERR_DEVICE 2      /* HIP device/runtime failure or missing device */
+#define QLP_ERR_COMPUTE 3     /* failure while computing */
+#define QLP_ERR_BUDGET 4      /* request exceeds the context's HBM budget */
+#define QLP_ERR_UNSUPPORTED 5 /* valid in the reference, not provided here */
+
+/* Normalisation / arithmetic form of the correlation (the two cross-correlation
+ * jobs the reference pipeline selects, OpenClGpuManager.java:124-140). */
+#define QLP_NORM_PER_TILE 0 /* CrossCorrelationRealFFTParStdJob (GPU job): F' = F/max(F)*16+1,
+                               G' = G/max(G)*16+1, product / (2N)^2, inverse by re/im swap */
+#define QLP_NORM_A_MAX 1    /* CrossCorrelationSoftRealFFTJob (Java CPU job): both tiles by
+                               max(A), inverse twiddles, /2N per 1D pass */
+
+/* Peak finders (maximum/MaximumFinderFactoryEnum.java:17-41). */
+#define QLP_PEAK_FROM_CENTER 0 /* FindMaximumFromCenter */
+#define QLP_PEAK_MULTI 1       /* FindMaximumMultiPeaks(numberOfPeaks, kernelWidth) */
+
+/* Sub-pixel interpolators (interpolators/CrossCorrelationInterpolatorFactoryEnum.java). */
+#define QLP_SUBPIX_GAUSS1D 0 /* Gaussian1DInterpolator(pixels) */
+#define QLP_SUBPIX_GUO 1     /* Gaussian1DHongweiGuoInterpolator(pixels, iterations) */
+#define QLP_SUBPIX_POLY 2    /* Gaussian1DPolynomialInterpolator */
+
+/* Status bits of a sub-pixel result record (i, j, value, status). */
+#define QLP_FIT_I 1        /* row fit accepted */
+#define QLP_FIT_J 2        /* column fit accepted */
+#define QLP_FIT_SKIPPED 4  /* peak at (0,0) or too close to the border: integer peak kept */
+#define QLP_FIT_FLOOR 8    /* plane floor below the fitter's limit (reference throws) */
+
+typedef struct qlp_ctx qlp_ctx;
+
+/* ---- context ------------------------------------------------------------ */
+
+/* Create a context on HIP device 'hip_device' with an HBM budget in bytes for
+ * the library's own allocations (0 = no limit).  Replaces the device discovery
+ * and per-device geometry of device/DeviceManager.java:25-136 and
+ * CrossCorrelationFFTTemplate.computeLocalGeometry :307-438. */
+int qlp_ctx_create(int hip_device, size_t hbm_budget_bytes, qlp_ctx **out);
+/* Release a context (Job.dispose, LocalPIVOpenCLGpuJob.java:438-443). */
+int qlp_ctx_destroy(qlp_ctx *ctx);
+/* Wait for the work enqueued on the context's stream. */
+int qlp_ctx_synchronize(qlp_ctx *ctx);
+/* The context's hipStream_t, for callers that order their own work with it. */
+void *qlp_ctx_stream(qlp_ctx *ctx);
+/* Message of the calling thread's last error ("" if none). */
+const char *qlp_last_error(void);
+/* Library version string. */
+const char *qlp_version(void);
+
+/* ---- cross-correlation ---------------------------------------------------- */
+
+/* Correlate n IA pairs.  Replaces Job<List<Tile>, XCorrelationResults>:
+ * setInputParameters(JOB_RESULT_CLIPPED_TILES_A / _B) + analyze() + compute() +
+ * getJobResult(JOB_RESULT_CROSS_MATRICES) of CrossCorrelationRealFFTParStdJob
+ * (:35-292; kernel CrossCorrelationRealFFTParallelKernel.run :214-713) for
+ * mode QLP_NORM_PER_TILE, or of CrossCorrelationSoftRealFFTJob.compute :94-121
+ * (FastRealFFTXCorr.computeXCorr :44-68) for QLP_NORM_A_MAX.
+ *   tilesA, tilesB : n x N x N uint8 (Tile.getMatrix() bytes; warped tiles hold
+ *                    integer values 0..255 and are passed as uint8 too)
+ *   maxA, maxB     : n floats, Matrix.getMaxValue() of each tile (maxB unused
+ *                    in QLP_NORM_A_MAX)
+ *   planes_out     : n x (2N-1) x (2N-1) floats, same order as the inputs. */
+int qlp_xcorr_batch(qlp_ctx *ctx, int n, int N, const uint8_t *tilesA, const uint8_t *tilesB,
+                    const float *maxA, const float *maxB, int mode, float *planes_out);
+int qlp_xcorr_batch_dev(qlp_ctx *ctx, int n, int N, const uint8_t *tilesA, const uint8_t *tilesB,
+                        const float *maxA, const float *maxB, int mode, float *planes_out);
+
+/* ---- integer peak search -------------------------------------------------- */
+
+/* IMaximumFinder.findMaximum(Matrix) (maximum/IMaximumFinder.java:16-17) over n
+ * planes of D x D.  kind QLP_PEAK_FROM_CENTER (FindMaximumFromCenter.findMaximum
+ * :47-183; n_peaks must be 1) or QLP_PEAK_MULTI (FindMaximumMultiPeaks.findMaximum
+ * :42-162 with numberOfPeaks = n_peaks <= 16 and kernelWidth).
+ *   peaks     : n x n_peaks x 3 floats (i, j, value), peak 0 the main peak;
+ *               missing peaks are zero
+ *   min_floor : n floats (MaxCrossResult.getMinFloor) */
+int qlp_find_peaks(qlp_ctx *ctx, int kind, int n, int D, const float *planes, int n_peaks,
+                   int kernel_width, float *peaks, float *min_floor);
+int qlp_find_peaks_dev(qlp_ctx *ctx, int kind, int n, int D, const float *planes, int n_peaks,
+                       int kernel_width, float *peaks, float *min_floor);
+
+/* ---- sub-pixel refinement ------------------------------------------------- */
+
+/* ICrossCorrelationInterpolator.interpolate(List<MaxCrossResult>)
+ * (interpolators/ICrossCorrelationInterpolator.java:14-15, per-IA loop of
+ * BasicInterpolatorAdapter.interpolate :26-39) for every peak of n planes.
+ *   peaks_in  : n x n_peaks x 3 (i, j, value) from qlp_find_peaks
+ *   peaks_out : n x n_peaks x 4 (i', j', value', status); the displacement is
+ *               u = i' - (D/2), v = j' - (D/2) with integer D/2 (MaxCrossResult
+ *               .setNthRelativeDisplacementFromPeak :304-326). */
+int qlp_subpixel(qlp_ctx *ctx, int kind, int pixels, int iterations, int n, int D, int n_peaks,
+                 const float *planes, const float *min_floor, const float *peaks_in,
+                 float *peaks_out);
+int qlp_subpixel_dev(qlp_ctx *ctx, int kind, int pixels, int iterations, int n, int D,
+                     int n_peaks, const float *planes, const float *min_floor,
+                     const float *peaks_in, float *peaks_out);
+
+/* ---- tiles from images ---------------------------------------------------- */
+
+/* Image.clipImageMatrix (images/Image.java:386-494) for n windows of N x N at
+ * (top[k], left[k]) of an H x W uint8 image; pixels outside the image are 0;
+ * max_out[k] = max over in-image pixels clamped to >= 16 (MatrixByte.java:16-39). */
+int qlp_clip_tiles_dev(qlp_ctx *ctx, const uint8_t *img, int H, int W, int n, int N,
+                       const int *top, const int *left, uint8_t *tiles, float *max_out);
+
+/* Bilinear micro warp of second-image tiles
+ * (images/BiLinearImageMicroAndMiniWarpingStrategy.warpAndClipImage :132-148,
+ * SecondImage mode with createMicroDisplacements): window at
+ * (top + floor(u), left + floor(v)), fractional remainder resampled;
+ * max_out[k] = 255 (may be NULL). */
+int qlp_warp_micro_tiles_dev(qlp_ctx *ctx, const uint8_t *img, int H, int W, int n, int N,
+                             const int *top, const int *left, const float *u, const float *v,
+                             uint8_t *tiles, float *max_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QLPIV_H */

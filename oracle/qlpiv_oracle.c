@@ -1138,3 +1138,35 @@ void or_warp_micro(const uint8_t *img, int H, int W, int top, int left, int N, f
             tile[i * N + j] =
                 (uint8_t)warp_pixel(img, H, W, fu, fv, (short)(i + tTop), (short)(j + tLeft));
 }
+
+/* ------------------------------------------------------------------------ */
+/* The reference CPU path per IA, for the timed CPU baseline:               */
+/* CrossCorrelationSoftRealFFTJob.compute :94-121 (FastRealFFTXCorr),        */
+/* FindMaximumFromCenter.findMaximum :47-183, then the sub-pixel fit         */
+/* (fit 0 = Gaussian1D 3 px, 1 = Guo 3 px / 20 iterations) on one thread.    */
+/* out: n x 4 (i', j', value', status).                                      */
+/* ------------------------------------------------------------------------ */
+void or_cpu_path_batch(int n, int N, const uint8_t *tA, const uint8_t *tB, const float *maxA,
+                       int fit, float *out)
+{
+    int D = 2 * N - 1;
+    float *A = (float *)malloc(sizeof(float) * N * N);
+    float *B = (float *)malloc(sizeof(float) * N * N);
+    float *plane = (float *)malloc(sizeof(float) * D * D);
+    for (int t = 0; t < n; t++) {
+        for (int q = 0; q < N * N; q++) {
+            A[q] = (float)tA[(size_t)t * N * N + q];
+            B[q] = (float)tB[(size_t)t * N * N + q];
+        }
+        or_xcorr_javacpu(A, B, maxA[t], N, plane);
+        float pk[4];
+        or_find_max_from_center(plane, D, D, pk);
+        if (fit == 0)
+            or_gauss1d(plane, D, D, 3, pk[3], pk, out + 4 * (size_t)t);
+        else
+            or_guo(plane, D, D, 3, 20, pk[3], pk, out + 4 * (size_t)t);
+    }
+    free(A);
+    free(B);
+    free(plane);
+}

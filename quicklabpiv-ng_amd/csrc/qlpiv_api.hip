@@ -1,0 +1,291 @@
+// C ABI of libqlpiv: contexts, errors, host-buffer wrappers.
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "qlpiv_internal.h"
+
+namespace qlpiv {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string &msg) { g_last_error = msg; }
+
+int fail(int code, const std::string &msg)
+{
+    set_error(msg);
+    return code;
+}
+
+int check_hip(hipError_t e, const char *what)
+{
+    if (e == hipSuccess) return QLP_OK;
+    return fail(QLP_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// ws[n] = (float)sin(2*pi*n/M) in double (SimpleFFT.initEulerTable :323-330,
+// CrossCorrelationFFTTemplate.initEulerTable :268-277), arranged as the
+// (c, s) pairs of the three butterfly forms.
+static void build_tables(int M, std::vector<float2> out[3])
+{
+    std::vector<float> ws(M);
+    for (int k = 0; k < M; k++) ws[k] = (float)sin(2.0 * M_PI * (double)(float)k / (double)(float)M);
+    for (int t = 0; t < 3; t++) out[t].resize(M / 2);
+    for (int k = 0; k < M / 2; k++) {
+        out[TW_FWD][k] = make_float2(ws[k + M / 4], -ws[k]);
+        const int ic = (M / 4 - k >= 0) ? (M / 4 - k) : (M + (M / 4 - k));
+        out[TW_INV_CPU][k] = make_float2(ws[ic], ws[k]);
+        out[TW_INV_GPU][k] = make_float2(ws[k + M / 4], ws[k]);
+    }
+}
+
+static int ensure_scratch(Context *ctx, size_t bytes)
+{
+    if (bytes <= ctx->scratch_bytes) return QLP_OK;
+    if (ctx->hbm_budget && ctx->hbm_in_use - ctx->scratch_bytes + bytes > ctx->hbm_budget)
+        return fail(QLP_ERR_BUDGET, "request exceeds the context's HBM budget");
+    if (ctx->scratch) {
+        (void)hipFree(ctx->scratch);
+        ctx->hbm_in_use -= ctx->scratch_bytes;
+        ctx->scratch = nullptr;
+        ctx->scratch_bytes = 0;
+    }
+    int rc = check_hip(hipMalloc(&ctx->scratch, bytes), "hipMalloc scratch");
+    if (rc) return rc;
+    ctx->scratch_bytes = bytes;
+    ctx->hbm_in_use += bytes;
+    return QLP_OK;
+}
+
+static int sync_ctx(Context *ctx) { return check_hip(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize"); }
+
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+}  // namespace qlpiv
+
+using namespace qlpiv;
+
+struct qlp_ctx {
+    Context c;
+};
+
+extern "C" {
+
+const char *qlp_last_error(void) { return g_last_error.c_str(); }
+
+const char *qlp_version(void) { return "qlpiv-mi355x 0.1 (gfx950)"; }
+
+int qlp_ctx_create(int hip_device, size_t hbm_budget_bytes, qlp_ctx **out)
+{
+    if (!out) return fail(QLP_ERR_ARG, "qlp_ctx_create: null output pointer");
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return fail(QLP_ERR_DEVICE, "no HIP device available");
+    if (hip_device < 0 || hip_device >= ndev)
+        return fail(QLP_ERR_ARG, "qlp_ctx_create: device index out of range");
+    int rc = check_hip(hipSetDevice(hip_device), "hipSetDevice");
+    if (rc) return rc;
+    qlp_ctx *h = new qlp_ctx();
+    Context *ctx = &h->c;
+    ctx->device = hip_device;
+    ctx->hbm_budget = hbm_budget_bytes;
+    rc = check_hip(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "hipStreamCreate");
+    if (rc) {
+        delete h;
+        return rc;
+    }
+    for (int li = 0; li < 3; li++) {
+        const int M = 32 << li;
+        std::vector<float2> tabs[3];
+        build_tables(M, tabs);
+        for (int t = 0; t < 3; t++) {
+            void *p = nullptr;
+            rc = check_hip(hipMalloc(&p, sizeof(float2) * (M / 2)), "hipMalloc tables");
+            if (!rc)
+                rc = check_hip(hipMemcpy(p, tabs[t].data(), sizeof(float2) * (M / 2),
+                                         hipMemcpyHostToDevice),
+                               "upload tables");
+            if (rc) {
+                qlp_ctx_destroy(h);
+                return rc;
+            }
+            ctx->tables.tw[li][t] = (float2 *)p;
+            ctx->hbm_in_use += sizeof(float2) * (M / 2);
+        }
+    }
+    *out = h;
+    return QLP_OK;
+}
+
+int qlp_ctx_destroy(qlp_ctx *h)
+{
+    if (!h) return QLP_OK;
+    Context *ctx = &h->c;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (int li = 0; li < 3; li++)
+        for (int t = 0; t < 3; t++)
+            if (ctx->tables.tw[li][t]) (void)hipFree(ctx->tables.tw[li][t]);
+    if (ctx->scratch) (void)hipFree(ctx->scratch);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete h;
+    return QLP_OK;
+}
+
+int qlp_ctx_synchronize(qlp_ctx *h)
+{
+    if (!h) return fail(QLP_ERR_ARG, "null context");
+    return sync_ctx(&h->c);
+}
+
+void *qlp_ctx_stream(qlp_ctx *h) { return h ? (void *)h->c.stream : nullptr; }
+
+static int xcorr_args(int n, int N, int mode)
+{
+    if (n < 0) return fail(QLP_ERR_ARG, "xcorr: negative IA count");
+    if (N != 16 && N != 32 && N != 64)
+        return fail(N == 128 ? QLP_ERR_UNSUPPORTED : QLP_ERR_ARG,
+                    "xcorr: IA size must be 16, 32 or 64 pixels");
+    if (mode != QLP_NORM_PER_TILE && mode != QLP_NORM_A_MAX)
+        return fail(QLP_ERR_ARG, "xcorr: unknown normalisation mode");
+    return QLP_OK;
+}
+
+int qlp_xcorr_batch_dev(qlp_ctx *h, int n, int N, const uint8_t *tA, const uint8_t *tB,
+                        const float *maxA, const float *maxB, int mode, float *planes)
+{
+    if (!h) return fail(QLP_ERR_ARG, "null context");
+    int rc = xcorr_args(n, N, mode);
+    if (rc) return rc;
+    if (n > 0 && (!tA || !tB || !maxA || !planes || (mode == QLP_NORM_PER_TILE && !maxB)))
+        return fail(QLP_ERR_ARG, "xcorr: null buffer");
+    return launch_xcorr(&h->c, n, N, tA, tB, maxA, maxB, mode, planes, -1, nullptr);
+}
+
+int qlp_xcorr_batch(qlp_ctx *h, int n, int N, const uint8_t *tA, const uint8_t *tB,
+                    const float *maxA, const float *maxB, int mode, float *planes)
+{
+    if (!h) return fail(QLP_ERR_ARG, "null context");
+    int rc = xcorr_args(n, N, mode);
+    if (rc || n == 0) return rc;
+    if (!tA || !tB || !maxA || !planes || (mode == QLP_NORM_PER_TILE && !maxB))
+        return fail(QLP_ERR_ARG, "xcorr: null buffer");
+    Context *ctx = &h->c;
+    const int D = 2 * N - 1;
+    const size_t bt = align256((size_t)n * N * N), bm = align256(sizeof(float) * n),
+                 bp = align256(sizeof(float) * (size_t)n * D * D);
+    rc = ensure_scratch(ctx, 2 * bt + 2 * bm + bp);
+    if (rc) return rc;
+    char *base = (char *)ctx->scratch;
+    uint8_t *dA = (uint8_t *)base, *dB = (uint8_t *)(base + bt);
+    float *dmA = (float *)(base + 2 * bt), *dmB = (float *)(base + 2 * bt + bm);
+    float *dP = (float *)(base + 2 * bt + 2 * bm);
+    hipStream_t s = ctx->stream;
+    if ((rc = check_hip(hipMemcpyAsync(dA, tA, (size_t)n * N * N, hipMemcpyHostToDevice, s), "copy A")) ||
+        (rc = check_hip(hipMemcpyAsync(dB, tB, (size_t)n * N * N, hipMemcpyHostToDevice, s), "copy B")) ||
+        (rc = check_hip(hipMemcpyAsync(dmA, maxA, sizeof(float) * n, hipMemcpyHostToDevice, s), "copy maxA")))
+        return rc;
+    if (maxB && (rc = check_hip(hipMemcpyAsync(dmB, maxB, sizeof(float) * n, hipMemcpyHostToDevice, s), "copy maxB")))
+        return rc;
+    rc = launch_xcorr(ctx, n, N, dA, dB, dmA, maxB ? dmB : dmA, mode, dP, -1, nullptr);
+    if (rc) return rc;
+    rc = check_hip(hipMemcpyAsync(planes, dP, sizeof(float) * (size_t)n * D * D, hipMemcpyDeviceToHost, s),
+                   "copy planes");
+    if (rc) return rc;
+    return sync_ctx(ctx);
+}
+
+int qlp_find_peaks_dev(qlp_ctx *h, int kind, int n, int D, const float *planes, int n_peaks,
+                       int kernel_width, float *peaks, float *min_floor)
+{
+    if (!h) return fail(QLP_ERR_ARG, "null context");
+    if (n < 0) return fail(QLP_ERR_ARG, "find_peaks: negative count");
+    if (kind == QLP_PEAK_FROM_CENTER && n_peaks != 1)
+        return fail(QLP_ERR_ARG, "find_peaks: FromCenter finds exactly one peak");
+    return launch_find_peaks(&h->c, kind, n, D, planes, n_peaks, kernel_width, peaks, min_floor);
+}
+
+int qlp_find_peaks(qlp_ctx *h, int kind, int n, int D, const float *planes, int n_peaks,
+                   int kernel_width, float *peaks, float *min_floor)
+{
+    if (!h) return fail(QLP_ERR_ARG, "null context");
+    if (n < 0 || n_peaks < 1) return fail(QLP_ERR_ARG, "find_peaks: bad counts");
+    if (n == 0) return QLP_OK;
+    Context *ctx = &h->c;
+    const size_t bp = align256(sizeof(float) * (size_t)n * D * D),
+                 bk = align256(sizeof(float) * 3 * (size_t)n * n_peaks), bf = align256(sizeof(float) * n);
+    int rc = ensure_scratch(ctx, bp + bk + bf);
+    if (rc) return rc;
+    char *base = (char *)ctx->scratch;
+    float *dP = (float *)base, *dK = (float *)(base + bp), *dF = (float *)(base + bp + bk);
+    hipStream_t s = ctx->stream;
+    if ((rc = check_hip(hipMemcpyAsync(dP, planes, sizeof(float) * (size_t)n * D * D, hipMemcpyHostToDevice, s), "copy planes")))
+        return rc;
+    rc = qlp_find_peaks_dev(h, kind, n, D, dP, n_peaks, kernel_width, dK, dF);
+    if (rc) return rc;
+    if ((rc = check_hip(hipMemcpyAsync(peaks, dK, sizeof(float) * 3 * (size_t)n * n_peaks, hipMemcpyDeviceToHost, s), "copy peaks")) ||
+        (rc = check_hip(hipMemcpyAsync(min_floor, dF, sizeof(float) * n, hipMemcpyDeviceToHost, s), "copy floor")))
+        return rc;
+    return sync_ctx(ctx);
+}
+
+int qlp_subpixel_dev(qlp_ctx *h, int kind, int pixels, int iterations, int n, int D, int n_peaks,
+                     const float *planes, const float *min_floor, const float *peaks_in,
+                     float *peaks_out)
+{
+    if (!h) return fail(QLP_ERR_ARG, "null context");
+    if (n < 0 || n_peaks < 1) return fail(QLP_ERR_ARG, "subpixel: bad counts");
+    return launch_subpixel(&h->c, kind, pixels, iterations, n, D, n_peaks, planes, min_floor,
+                           peaks_in, peaks_out);
+}
+
+int qlp_subpixel(qlp_ctx *h, int kind, int pixels, int iterations, int n, int D, int n_peaks,
+                 const float *planes, const float *min_floor, const float *peaks_in,
+                 float *peaks_out)
+{
+    if (!h) return fail(QLP_ERR_ARG, "null context");
+    if (n < 0 || n_peaks < 1) return fail(QLP_ERR_ARG, "subpixel: bad counts");
+    if (n == 0) return QLP_OK;
+    Context *ctx = &h->c;
+    const size_t bp = align256(sizeof(float) * (size_t)n * D * D), bf = align256(sizeof(float) * n),
+                 bi = align256(sizeof(float) * 3 * (size_t)n * n_peaks),
+                 bo = align256(sizeof(float) * 4 * (size_t)n * n_peaks);
+    int rc = ensure_scratch(ctx, bp + bf + bi + bo);
+    if (rc) return rc;
+    char *base = (char *)ctx->scratch;
+    float *dP = (float *)base, *dF = (float *)(base + bp), *dI = (float *)(base + bp + bf),
+          *dO = (float *)(base + bp + bf + bi);
+    hipStream_t s = ctx->stream;
+    if ((rc = check_hip(hipMemcpyAsync(dP, planes, sizeof(float) * (size_t)n * D * D, hipMemcpyHostToDevice, s), "copy planes")) ||
+        (rc = check_hip(hipMemcpyAsync(dF, min_floor, sizeof(float) * n, hipMemcpyHostToDevice, s), "copy floor")) ||
+        (rc = check_hip(hipMemcpyAsync(dI, peaks_in, sizeof(float) * 3 * (size_t)n * n_peaks, hipMemcpyHostToDevice, s), "copy peaks")))
+        return rc;
+    rc = launch_subpixel(ctx, kind, pixels, iterations, n, D, n_peaks, dP, dF, dI, dO);
+    if (rc) return rc;
+    rc = check_hip(hipMemcpyAsync(peaks_out, dO, sizeof(float) * 4 * (size_t)n * n_peaks, hipMemcpyDeviceToHost, s), "copy out");
+    if (rc) return rc;
+    return sync_ctx(ctx);
+}
+
+int qlp_clip_tiles_dev(qlp_ctx *h, const uint8_t *img, int H, int W, int n, int N, const int *top,
+                       const int *left, uint8_t *tiles, float *max_out)
+{
+    if (!h) return fail(QLP_ERR_ARG, "null context");
+    if (n < 0 || N < 1 || H < 1 || W < 1) return fail(QLP_ERR_ARG, "clip: bad sizes");
+    return launch_clip(&h->c, img, H, W, n, N, top, left, tiles, max_out);
+}
+
+int qlp_warp_micro_tiles_dev(qlp_ctx *h, const uint8_t *img, int H, int W, int n, int N,
+                             const int *top, const int *left, const float *u, const float *v,
+                             uint8_t *tiles, float *max_out)
+{
+    if (!h) return fail(QLP_ERR_ARG, "null context");
+    if (n < 0 || N < 1 || H < 1 || W < 1) return fail(QLP_ERR_ARG, "warp: bad sizes");
+    return launch_warp_micro(&h->c, img, H, W, n, N, top, left, u, v, tiles, max_out);
+}
+
+}  // extern "C"

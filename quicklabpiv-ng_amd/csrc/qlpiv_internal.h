@@ -1,0 +1,65 @@
+// Shared declarations of the MI355X (gfx950) PIV correlation library.
+//
+// Every kernel of this library is compiled with -ffp-contract=off: the
+// reference's Java arithmetic never fuses a multiply into an add, and the
+// parity contract (integer peaks bit-exact, planes value-identical) depends on
+// doing the same operations in the same order.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "../../include/qlpiv.h"
+
+namespace qlpiv {
+
+// Twiddle tables, one per transform length M = 2N in {32, 64, 128}, in the
+// form (c[k], s[k]) consumed by the generic radix-2 butterfly
+//   lo = ((a_r + c*b_r) - s*b_i, (a_i + c*b_i) + s*b_r)
+//   hi = ((a_r - c*b_r) + s*b_i, (a_i - c*b_i) - s*b_r)
+// for k in [0, M/2):
+//   TW_FWD     : c = ws[k + M/4],         s = -ws[k]   (SimpleFFT.computeFFTStep :173-174)
+//   TW_INV_CPU : c = ws[(M/4 - k) mod M], s =  ws[k]   (SimpleFFT.computeIFFTStep :300-307)
+//   TW_INV_GPU : c = ws[k + M/4],         s =  ws[k]   (the kernel's re/im swap trick,
+//                CrossCorrelationRealFFTParallelKernel.java:619,684, which is the
+//                forward butterfly on swapped data)
+// with ws[n] = (float)sin(2*pi*n/M) evaluated in double.
+enum TwKind { TW_FWD = 0, TW_INV_CPU = 1, TW_INV_GPU = 2 };
+
+struct DeviceTables {
+    float2 *tw[3][3];  // [log2M - 5][kind] -> M/2 entries
+};
+
+struct Context {
+    int device = 0;
+    size_t hbm_budget = 0;
+    size_t hbm_in_use = 0;
+    hipStream_t stream = nullptr;
+    DeviceTables tables{};
+    // scratch buffers of the host-pointer entry points (grown on demand)
+    void *scratch = nullptr;
+    size_t scratch_bytes = 0;
+};
+
+void set_error(const std::string &msg);
+int fail(int code, const std::string &msg);
+int check_hip(hipError_t e, const char *what);
+
+// Launchers (qlpiv_xcorr.hip, qlpiv_peaks.hip, qlpiv_subpixel.hip, qlpiv_warp.hip).
+int launch_xcorr(Context *ctx, int n, int N, const uint8_t *tA, const uint8_t *tB,
+                 const float *maxA, const float *maxB, int mode, float *planes,
+                 int peak_kind, float *peaks_out);
+int launch_find_peaks(Context *ctx, int kind, int n, int D, const float *planes, int n_peaks,
+                      int kernel_width, float *peaks, float *min_floor);
+int launch_subpixel(Context *ctx, int kind, int pixels, int iterations, int n, int D, int n_peaks,
+                    const float *planes, const float *min_floor, const float *peaks_in,
+                    float *peaks_out);
+int launch_clip(Context *ctx, const uint8_t *img, int H, int W, int n, int N, const int *top,
+                const int *left, uint8_t *tiles, float *maxv);
+int launch_warp_micro(Context *ctx, const uint8_t *img, int H, int W, int n, int N, const int *top,
+                      const int *left, const float *u, const float *v, uint8_t *tiles,
+                      float *maxv);
+
+}  // namespace qlpiv

@@ -1,0 +1,130 @@
+// Interrogation-area tiles cut from device-resident 8-bit images.
+//
+// CLIP (images/Image.java clipImageMatrix :386-494 with the MatrixByte maximum,
+//   MatrixByte.java:16-39, :278-317): the N x N window at (top, left), zero
+//   outside the image; the tile maximum over the in-image pixels, clamped to
+//   >= 16.
+// WARP_MICRO (images/BiLinearImageMicroAndMiniWarpingStrategy.java, SecondImage
+//   order: warpAndClipImage :132-148, createMicroDisplacements :188-201,
+//   doBiLinearWarp :292-334, warpPixelOnImage :336-407): the window moves by
+//   floor(u), floor(v) and every pixel is resampled bilinearly at the
+//   fractional remainder, with round-half-up of the source position and of the
+//   value (commons-math3 FastMath.round(float) = floor(x + 0.5f)).  Values stay
+//   integers 0..255, so the tile is stored as u8; its maximum is the image
+//   maximum 255 (doBiLinearWarp :298).
+// Rows of the image are read by consecutive lanes (coalesced); one work-group
+// per tile.
+#include "qlpiv_internal.h"
+
+namespace qlpiv {
+
+__global__ __launch_bounds__(256) void clip_kernel(const uint8_t *__restrict__ img, int H, int W,
+                                                  int n, int N, const int *__restrict__ top,
+                                                  const int *__restrict__ left,
+                                                  uint8_t *__restrict__ tiles,
+                                                  float *__restrict__ maxv)
+{
+    __shared__ int s_max[4];
+    const int t = blockIdx.x;
+    if (t >= n) return;
+    const int y0 = top[t], x0 = left[t];
+    int mx = 0;
+    for (int idx = threadIdx.x; idx < N * N; idx += blockDim.x) {
+        const int i = idx / N, j = idx - (idx / N) * N;
+        const int y = y0 + i, x = x0 + j;
+        uint8_t v = 0;
+        if (y >= 0 && y < H && x >= 0 && x < W) {
+            v = img[(size_t)y * W + x];
+            mx = max(mx, (int)v);
+        }
+        tiles[(size_t)t * N * N + idx] = v;
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) mx = max(mx, __shfl_xor(mx, o, 64));
+    if ((threadIdx.x & 63) == 0) s_max[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int m = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++) m = max(m, s_max[w]);
+        maxv[t] = (float)(m < 16 ? 16 : m);
+    }
+}
+
+__device__ __forceinline__ int fm_round(float x) { return (int)floorf(x + 0.5f); }
+
+// warpPixelOnImage :336-407 (SecondImage: source = target + displacement)
+__device__ __forceinline__ float warp_pixel(const uint8_t *__restrict__ img, int H, int W, float u,
+                                            float v, int i, int j)
+{
+    float sI = (float)i + u, sJ = (float)j + v;
+    int pI = (short)fm_round(sI), pJ = (short)fm_round(sJ);
+    if (pI < 0) {
+        sI = 0.0f;
+        pI = 0;
+    } else if (pI >= H) {
+        sI = (float)(H - 1);
+        pI = H - 1;
+    }
+    if (pJ < 0) {
+        sJ = 0.0f;
+        pJ = 0;
+    } else if (pJ >= W) {
+        sJ = (float)(W - 1);
+        pJ = W - 1;
+    }
+    const float dU = sI - (float)pI, dV = sJ - (float)pJ;
+    int oI = dU >= 0.0f ? 1 : -1, oJ = dV >= 0.0f ? 1 : -1;
+    if (oI + pI < 0 || oI + pI >= H) oI = 0;
+    if (oJ + pJ < 0 || oJ + pJ >= W) oJ = 0;
+    const float wU = fabsf(dU), wV = fabsf(dV);
+    const float tl = img[(size_t)pI * W + pJ];
+    const float tr = img[(size_t)pI * W + pJ + oJ];
+    const float bl = img[(size_t)(pI + oI) * W + pJ];
+    const float br = img[(size_t)(pI + oI) * W + pJ + oJ];
+    const float f = (1.0f - wU) * (1.0f - wV) * tl + (1.0f - wU) * wV * tr +
+                    wU * (1.0f - wV) * bl + wU * wV * br;
+    return (float)fm_round(f);
+}
+
+__global__ __launch_bounds__(256) void warp_micro_kernel(const uint8_t *__restrict__ img, int H,
+                                                        int W, int n, int N,
+                                                        const int *__restrict__ top,
+                                                        const int *__restrict__ left,
+                                                        const float *__restrict__ du,
+                                                        const float *__restrict__ dv,
+                                                        uint8_t *__restrict__ tiles,
+                                                        float *__restrict__ maxv)
+{
+    const int t = blockIdx.x;
+    if (t >= n) return;
+    const float u = du[t], v = dv[t];
+    const float fu = floorf(u), fv = floorf(v);
+    const int tTop = (short)(top[t] + (int)fu);
+    const int tLeft = (short)(left[t] + (int)fv);
+    const float ru = u - fu, rv = v - fv;  // createMicroDisplacements :195-196
+    for (int idx = threadIdx.x; idx < N * N; idx += blockDim.x) {
+        const int i = idx / N, j = idx - (idx / N) * N;
+        tiles[(size_t)t * N * N + idx] = (uint8_t)warp_pixel(img, H, W, ru, rv, tTop + i, tLeft + j);
+    }
+    if (threadIdx.x == 0 && maxv) maxv[t] = 255.0f;
+}
+
+int launch_clip(Context *ctx, const uint8_t *img, int H, int W, int n, int N, const int *top,
+                const int *left, uint8_t *tiles, float *maxv)
+{
+    if (n == 0) return QLP_OK;
+    hipLaunchKernelGGL(clip_kernel, dim3(n), dim3(256), 0, ctx->stream, img, H, W, n, N, top, left,
+                       tiles, maxv);
+    return check_hip(hipGetLastError(), "clip kernel launch");
+}
+
+int launch_warp_micro(Context *ctx, const uint8_t *img, int H, int W, int n, int N, const int *top,
+                      const int *left, const float *u, const float *v, uint8_t *tiles, float *maxv)
+{
+    if (n == 0) return QLP_OK;
+    hipLaunchKernelGGL(warp_micro_kernel, dim3(n), dim3(256), 0, ctx->stream, img, H, W, n, N, top,
+                       left, u, v, tiles, maxv);
+    return check_hip(hipGetLastError(), "warp kernel launch");
+}
+
+}  // namespace qlpiv

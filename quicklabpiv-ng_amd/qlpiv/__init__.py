@@ -1,0 +1,235 @@
+"""qlpiv -- MI355X-native PIV correlation hot path of QuickLabPIV-ng.
+
+Host-side Python mirror of the reference's correlation plug-in interfaces over
+the C ABI of libqlpiv.so (include/qlpiv.h):
+
+* ``Context``            -- one HIP stream and the device tables (qlp_ctx)
+* ``CrossCorrelationJob`` -- Job<List<Tile>, XCorrelationResults>
+  (jobs/Job.java:14-67; CrossCorrelationRealFFTParStdJob / SoftRealFFTJob)
+* ``find_maximum`` / ``interpolate`` -- IMaximumFinder / ICrossCorrelationInterpolator
+
+Device buffers are torch tensors on ``cuda``; host buffers are numpy arrays.
+Every call runs the HIP kernels: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+from enum import Enum
+
+import numpy as np
+
+from . import _lib
+from ._lib import (QLP_NORM_A_MAX, QLP_NORM_PER_TILE, QLP_PEAK_FROM_CENTER, QLP_PEAK_MULTI,
+                   QLP_SUBPIX_GAUSS1D, QLP_SUBPIX_GUO, QLP_SUBPIX_POLY, QlpError, build, check)
+
+__all__ = [
+    "Context", "CrossCorrelationJob", "JobResultEnum", "QlpError", "build",
+    "QLP_NORM_PER_TILE", "QLP_NORM_A_MAX", "QLP_PEAK_FROM_CENTER", "QLP_PEAK_MULTI",
+    "QLP_SUBPIX_GAUSS1D", "QLP_SUBPIX_GUO", "QLP_SUBPIX_POLY",
+]
+
+
+def _ptr(x):
+    """Raw pointer of a torch tensor (device) or numpy array (host); None -> NULL."""
+    if x is None:
+        return None
+    if isinstance(x, np.ndarray):
+        assert x.flags["C_CONTIGUOUS"], "host buffers must be C-contiguous"
+        return x.ctypes.data
+    assert x.is_contiguous(), "device buffers must be contiguous"
+    return x.data_ptr()
+
+
+class Context:
+    """qlp_ctx: a HIP stream on one device plus the twiddle tables."""
+
+    def __init__(self, device: int = 0, hbm_budget: int = 0):
+        self.lib = _lib.load()
+        h = ctypes.c_void_p()
+        check(self.lib.qlp_ctx_create(int(device), int(hbm_budget), ctypes.byref(h)))
+        self.h = h
+        self.device = device
+
+    def close(self):
+        if self.h:
+            self.lib.qlp_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def stream_handle(self) -> int:
+        return self.lib.qlp_ctx_stream(self.h)
+
+    def torch_stream(self):
+        """The context's stream as a torch stream (to order torch work with ours)."""
+        import torch
+
+        return torch.cuda.ExternalStream(self.stream_handle, device=self.device)
+
+    def synchronize(self):
+        check(self.lib.qlp_ctx_synchronize(self.h))
+
+    # ---- correlation -----------------------------------------------------
+    def xcorr(self, tilesA, tilesB, maxA, maxB=None, mode=QLP_NORM_PER_TILE, out=None):
+        """Correlate n IA pairs; numpy inputs use the host entry point, torch
+        CUDA tensors the device one (asynchronous on this context's stream)."""
+        n, N = int(tilesA.shape[0]), int(tilesA.shape[-1])
+        D = 2 * N - 1
+        if isinstance(tilesA, np.ndarray):
+            tilesA = np.ascontiguousarray(tilesA, np.uint8)
+            tilesB = np.ascontiguousarray(tilesB, np.uint8)
+            maxA = np.ascontiguousarray(maxA, np.float32)
+            maxB = None if maxB is None else np.ascontiguousarray(maxB, np.float32)
+            if out is None:
+                out = np.empty((n, D, D), np.float32)
+            check(self.lib.qlp_xcorr_batch(self.h, n, N, _ptr(tilesA), _ptr(tilesB), _ptr(maxA),
+                                           _ptr(maxB), int(mode), _ptr(out)))
+            return out
+        import torch
+
+        if out is None:
+            out = torch.empty((n, D, D), dtype=torch.float32, device=tilesA.device)
+        check(self.lib.qlp_xcorr_batch_dev(self.h, n, N, _ptr(tilesA), _ptr(tilesB), _ptr(maxA),
+                                           _ptr(maxB), int(mode), _ptr(out)))
+        return out
+
+    # ---- peaks -------------------------------------------------------------
+    def find_peaks(self, planes, kind=QLP_PEAK_FROM_CENTER, n_peaks=1, kernel_width=3,
+                   out=None, min_floor=None):
+        n, D = int(planes.shape[0]), int(planes.shape[-1])
+        if isinstance(planes, np.ndarray):
+            planes = np.ascontiguousarray(planes, np.float32)
+            peaks = np.empty((n, n_peaks, 3), np.float32)
+            mf = np.empty(n, np.float32)
+            check(self.lib.qlp_find_peaks(self.h, int(kind), n, D, _ptr(planes), int(n_peaks),
+                                          int(kernel_width), _ptr(peaks), _ptr(mf)))
+            return peaks, mf
+        import torch
+
+        peaks = out if out is not None else torch.empty((n, n_peaks, 3), dtype=torch.float32,
+                                                        device=planes.device)
+        mf = min_floor if min_floor is not None else torch.empty(n, dtype=torch.float32,
+                                                                 device=planes.device)
+        check(self.lib.qlp_find_peaks_dev(self.h, int(kind), n, D, _ptr(planes), int(n_peaks),
+                                          int(kernel_width), _ptr(peaks), _ptr(mf)))
+        return peaks, mf
+
+    # ---- sub-pixel -----------------------------------------------------------
+    def subpixel(self, planes, min_floor, peaks, kind=QLP_SUBPIX_GAUSS1D, pixels=3,
+                 iterations=20, out=None):
+        n, D = int(planes.shape[0]), int(planes.shape[-1])
+        P = int(peaks.shape[1])
+        if isinstance(planes, np.ndarray):
+            res = np.empty((n, P, 4), np.float32)
+            check(self.lib.qlp_subpixel(self.h, int(kind), int(pixels), int(iterations), n, D, P,
+                                        _ptr(np.ascontiguousarray(planes, np.float32)),
+                                        _ptr(np.ascontiguousarray(min_floor, np.float32)),
+                                        _ptr(np.ascontiguousarray(peaks, np.float32)), _ptr(res)))
+            return res
+        import torch
+
+        res = out if out is not None else torch.empty((n, P, 4), dtype=torch.float32,
+                                                      device=planes.device)
+        check(self.lib.qlp_subpixel_dev(self.h, int(kind), int(pixels), int(iterations), n, D, P,
+                                        _ptr(planes), _ptr(min_floor), _ptr(peaks), _ptr(res)))
+        return res
+
+    # ---- tiles ---------------------------------------------------------------
+    def clip_tiles(self, img, top, left, N):
+        import torch
+
+        n = int(top.shape[0])
+        H, W = int(img.shape[0]), int(img.shape[1])
+        tiles = torch.empty((n, N, N), dtype=torch.uint8, device=img.device)
+        mx = torch.empty(n, dtype=torch.float32, device=img.device)
+        check(self.lib.qlp_clip_tiles_dev(self.h, _ptr(img), H, W, n, N, _ptr(top), _ptr(left),
+                                          _ptr(tiles), _ptr(mx)))
+        return tiles, mx
+
+    def warp_micro_tiles(self, img, top, left, u, v, N):
+        import torch
+
+        n = int(top.shape[0])
+        H, W = int(img.shape[0]), int(img.shape[1])
+        tiles = torch.empty((n, N, N), dtype=torch.uint8, device=img.device)
+        mx = torch.empty(n, dtype=torch.float32, device=img.device)
+        check(self.lib.qlp_warp_micro_tiles_dev(self.h, _ptr(img), H, W, n, N, _ptr(top),
+                                                _ptr(left), _ptr(u), _ptr(v), _ptr(tiles),
+                                                _ptr(mx)))
+        return tiles, mx
+
+
+class JobResultEnum(Enum):
+    """The keys of the reference's job protocol used on this path
+    (jobs/JobResultEnum.java:14-15)."""
+    JOB_RESULT_CLIPPED_TILES_A = "clipped_tiles_a"
+    JOB_RESULT_CLIPPED_TILES_B = "clipped_tiles_b"
+    JOB_RESULT_CROSS_MATRICES = "cross_matrices"
+
+
+class CrossCorrelationJob:
+    """Job<List<Tile>, XCorrelationResults> over libqlpiv (jobs/Job.java:14-67).
+
+    Tiles are (pixels uint8 N x N, max value) pairs, i.e. Tile.getMatrix() data
+    and Matrix.getMaxValue().  ``mode`` selects the reference job whose
+    arithmetic is reproduced: QLP_NORM_PER_TILE for
+    CrossCorrelationRealFFTParStdJob, QLP_NORM_A_MAX for
+    CrossCorrelationSoftRealFFTJob.  Errors follow the reference: mismatched
+    or non-square tile lists raise RuntimeError in analyze()
+    (CrossCorrelationFFTTemplate.analyzeTilesHelper :594-642)."""
+
+    def __init__(self, ctx: Context, mode=QLP_NORM_PER_TILE):
+        self.ctx = ctx
+        self.mode = mode
+        self.inputs = {}
+        self.results = {}
+        self._analyzed = None
+
+    def set_input_parameters(self, key: JobResultEnum, tiles):
+        self.inputs[key] = tiles
+        self._analyzed = None
+
+    def analyze(self):
+        ta = self.inputs.get(JobResultEnum.JOB_RESULT_CLIPPED_TILES_A)
+        tb = self.inputs.get(JobResultEnum.JOB_RESULT_CLIPPED_TILES_B)
+        if ta is None or tb is None:
+            raise RuntimeError("Tiles A and tiles B must be set before analyze()")
+        if len(ta) != len(tb):
+            raise RuntimeError("Number of tiles in A differs from the number of tiles in B")
+        if len(ta) == 0:
+            self._analyzed = (0, 0, None, None, None, None)
+            return
+        N = ta[0][0].shape[0]
+        for pix, _ in list(ta) + list(tb):
+            if pix.shape != (N, N):
+                raise RuntimeError("Tiles must be square and all of the same size")
+        A = np.ascontiguousarray(np.stack([p for p, _ in ta]).astype(np.uint8))
+        B = np.ascontiguousarray(np.stack([p for p, _ in tb]).astype(np.uint8))
+        mA = np.array([m for _, m in ta], np.float32)
+        mB = np.array([m for _, m in tb], np.float32)
+        self._analyzed = (len(ta), N, A, B, mA, mB)
+
+    def compute(self):
+        if self._analyzed is None:
+            self.analyze()
+        n, N, A, B, mA, mB = self._analyzed
+        planes = [] if n == 0 else list(self.ctx.xcorr(A, B, mA, mB, self.mode))
+        self.results[JobResultEnum.JOB_RESULT_CROSS_MATRICES] = planes
+
+    def get_job_result(self, key: JobResultEnum):
+        return self.results.get(key)
+
+    def dispose(self):
+        self.inputs.clear()
+        self.results.clear()

@@ -1,0 +1,85 @@
+"""ctypes binding of libqlpiv.so (the C ABI declared in include/qlpiv.h).
+
+The library is built in-tree by ``build()`` (hipcc for gfx950) into
+``quicklabpiv-ng_amd/_build/libqlpiv.so``.  Loading fails loudly when it is
+missing: there is no CPU fallback behind this binding.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CSRC = os.path.join(PKG_ROOT, "csrc")
+LIB_PATH = os.path.join(PKG_ROOT, "_build", "libqlpiv.so")
+
+QLP_OK = 0
+QLP_ERR_ARG = 1
+QLP_ERR_DEVICE = 2
+QLP_ERR_COMPUTE = 3
+QLP_ERR_BUDGET = 4
+QLP_ERR_UNSUPPORTED = 5
+
+QLP_NORM_PER_TILE = 0
+QLP_NORM_A_MAX = 1
+QLP_PEAK_FROM_CENTER = 0
+QLP_PEAK_MULTI = 1
+QLP_SUBPIX_GAUSS1D = 0
+QLP_SUBPIX_GUO = 1
+QLP_SUBPIX_POLY = 2
+
+# every symbol include/qlpiv.h declares, with (restype, argtypes)
+_vp, _i, _sz = ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t
+SIGNATURES = {
+    "qlp_ctx_create": (_i, [_i, _sz, ctypes.POINTER(_vp)]),
+    "qlp_ctx_destroy": (_i, [_vp]),
+    "qlp_ctx_synchronize": (_i, [_vp]),
+    "qlp_ctx_stream": (_vp, [_vp]),
+    "qlp_last_error": (ctypes.c_char_p, []),
+    "qlp_version": (ctypes.c_char_p, []),
+    "qlp_xcorr_batch": (_i, [_vp, _i, _i, _vp, _vp, _vp, _vp, _i, _vp]),
+    "qlp_xcorr_batch_dev": (_i, [_vp, _i, _i, _vp, _vp, _vp, _vp, _i, _vp]),
+    "qlp_find_peaks": (_i, [_vp, _i, _i, _i, _vp, _i, _i, _vp, _vp]),
+    "qlp_find_peaks_dev": (_i, [_vp, _i, _i, _i, _vp, _i, _i, _vp, _vp]),
+    "qlp_subpixel": (_i, [_vp, _i, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp]),
+    "qlp_subpixel_dev": (_i, [_vp, _i, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp]),
+    "qlp_clip_tiles_dev": (_i, [_vp, _vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp]),
+    "qlp_warp_micro_tiles_dev": (_i, [_vp, _vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
+}
+
+_lib = None
+
+
+class QlpError(RuntimeError):
+    """A non-zero status from the library; .code holds the QLP_ERR_* value."""
+
+    def __init__(self, code, msg):
+        super().__init__(f"qlpiv error {code}: {msg}")
+        self.code = code
+
+
+def build(jobs=8):
+    """Compile libqlpiv.so for gfx950 (cross-compiles without a GPU)."""
+    subprocess.check_call(["make", "-s", f"-j{jobs}", "-C", CSRC])
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"libqlpiv.so not found at {LIB_PATH}: build it first (__graft_entry__.build())")
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def check(rc):
+    if rc != QLP_OK:
+        raise QlpError(rc, load().qlp_last_error().decode())
+    return rc
