@@ -67,6 +67,13 @@ def build(jobs=8):
 def load():
     global _lib
     if _lib is None:
+        # One HIP runtime per process: PyTorch ships its own libamdhip64.so.7.  When
+        # torch is importable it is loaded first, so that libqlpiv's dependency on
+        # the same soname resolves to torch's copy instead of a second runtime.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(
                 f"libqlpiv.so not found at {LIB_PATH}: build it first (__graft_entry__.build())")

@@ -133,7 +133,7 @@ def planes_for_peaks(rng, n, D):
     p[4, D // 2 + 1, D // 2 + 1] = 500
     p[5, 3, 4] = p[5, D - 2, D - 3] = 700  # equal maxima, different distances
     p[6, 7, 7] = p[6, D - 7 + 1, D - 7 + 1] = 700  # equal maxima, equal distances
-    p[7, 10, 12] = -0.0
+    p[7, min(10, D - 1), min(12, D - 1)] = -0.0
     return p
 
 
