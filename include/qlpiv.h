@@ -150,6 +150,64 @@ int qlp_warp_micro_tiles_dev(qlp_ctx *ctx, const uint8_t *img, int H, int W, int
                              const int *top, const int *left, const float *u, const float *v,
                              uint8_t *tiles, float *max_out);
 
+/* ---- correlation pass straight from device-resident images ---------------- */
+
+#define QLP_WARP_NONE 0  /* NoImageWarpingStrategy: B window at round(top+u), round(left+v) */
+#define QLP_WARP_MICRO 1 /* BiLinearImageMicroAndMiniWarpingStrategy (micro, SecondImage) */
+
+/* Clip/warp, correlate and FromCenter-search n IAs of one frame pair in a
+ * single kernel (the fused form of qlp_clip_tiles_dev + qlp_warp_micro_tiles_dev
+ * + qlp_xcorr_batch_dev + qlp_find_peaks_dev(FROM_CENTER)).  top/left: IA
+ * origins; du/dv: B displacement per IA (NULL = none).  Tile maxima are computed
+ * in the kernel (>= 16 clamp; 255 for warped tiles).  Outputs may be NULL:
+ * planes (n x D x D), peaks (n x 3: i, j, value), min_floor (n), nbr (n x 2 x
+ * (2*nbr_half+1): the plane column then row through the peak, for the fits). */
+int qlp_xcorr_images_dev(qlp_ctx *ctx, int n, int N, int mode, const uint8_t *imgA,
+                         const uint8_t *imgB, int H, int W, const int *top, const int *left,
+                         const float *du, const float *dv, int warp_mode, float *planes,
+                         float *peaks, float *min_floor, float *nbr, int nbr_half);
+
+/* ---- multi-pass PIV over device-resident frame pairs ---------------------- */
+
+/* The per-frame adaptive loop of jobs/LocalPIVOpenCLGpuJob.call :188-409 with
+ * the super-position IA strategy (iareas/AdaptiveInterAreaStrategySuperPosition
+ * .java:31-197), Direct velocity inheritance
+ * (AdaptiveInterVelocityInheritanceStrategyDirect.java), the chosen warping,
+ * FromCenter peak search and sub-pixel fit, and the displacement update
+ * (IterationStepTiles.updateDisplacementsFromMaxCrossResults :317-371). */
+typedef struct qlp_piv_config {
+    int ia_start, ia_end;     /* IA size of the first and of the last level (halving) */
+    float overlap_factor;     /* 1 - overlap: 0.5 for 50 % overlap */
+    int margin_top, margin_bottom, margin_left, margin_right;
+    int norm_mode;            /* QLP_NORM_* */
+    int warp_mode;            /* QLP_WARP_* */
+    int subpix_kind;          /* QLP_SUBPIX_* or -1 for none */
+    int subpix_pixels, subpix_iterations;
+    int interpolator_start_step; /* first level that is interpolated */
+    int keep_planes;          /* keep the correlation planes of the last level */
+} qlp_piv_config;
+
+typedef struct qlp_piv_plan qlp_piv_plan;
+
+/* Build the level geometry for H x W frames and allocate the device state of
+ * 'pairs' frame pairs processed together (inside the context's HBM budget). */
+int qlp_piv_plan_create(qlp_ctx *ctx, int H, int W, int pairs, const qlp_piv_config *cfg,
+                        qlp_piv_plan **out);
+int qlp_piv_plan_destroy(qlp_piv_plan *plan);
+/* Number of levels; IA size and grid of one level; the origins (host arrays of
+ * nI * nJ ints, row-major) of its IAs. */
+int qlp_piv_plan_levels(qlp_piv_plan *plan, int *n_levels);
+int qlp_piv_plan_level(qlp_piv_plan *plan, int level, int *ia, int *nI, int *nJ, int *top,
+                       int *left);
+/* Run every level for the frames imgsA, imgsB (device, pairs x H x W uint8),
+ * asynchronously on the context's stream. */
+int qlp_piv_run_dev(qlp_piv_plan *plan, const uint8_t *imgsA, const uint8_t *imgsB);
+/* Device pointers of a level's results (valid after the run has completed):
+ * u, v (pairs x n displacements, vertical then horizontal), peaks (pairs x n x 4:
+ * i', j', value', status), planes (last level only, when kept; else NULL). */
+int qlp_piv_result_dev(qlp_piv_plan *plan, int level, const float **u, const float **v,
+                       const float **peaks, const float **planes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -72,6 +72,8 @@ struct qlp_ctx {
     Context c;
 };
 
+Context *qlpiv::ctx_of(qlp_ctx *h) { return &h->c; }
+
 extern "C" {
 
 const char *qlp_last_error(void) { return g_last_error.c_str(); }
@@ -163,7 +165,7 @@ int qlp_xcorr_batch_dev(qlp_ctx *h, int n, int N, const uint8_t *tA, const uint8
     if (rc) return rc;
     if (n > 0 && (!tA || !tB || !maxA || !planes || (mode == QLP_NORM_PER_TILE && !maxB)))
         return fail(QLP_ERR_ARG, "xcorr: null buffer");
-    return launch_xcorr(&h->c, n, N, tA, tB, maxA, maxB, mode, planes, -1, nullptr);
+    return launch_xcorr(&h->c, n, N, tA, tB, maxA, maxB, mode, planes);
 }
 
 int qlp_xcorr_batch(qlp_ctx *h, int n, int N, const uint8_t *tA, const uint8_t *tB,
@@ -191,7 +193,7 @@ int qlp_xcorr_batch(qlp_ctx *h, int n, int N, const uint8_t *tA, const uint8_t *
         return rc;
     if (maxB && (rc = check_hip(hipMemcpyAsync(dmB, maxB, sizeof(float) * n, hipMemcpyHostToDevice, s), "copy maxB")))
         return rc;
-    rc = launch_xcorr(ctx, n, N, dA, dB, dmA, maxB ? dmB : dmA, mode, dP, -1, nullptr);
+    rc = launch_xcorr(ctx, n, N, dA, dB, dmA, maxB ? dmB : dmA, mode, dP);
     if (rc) return rc;
     rc = check_hip(hipMemcpyAsync(planes, dP, sizeof(float) * (size_t)n * D * D, hipMemcpyDeviceToHost, s),
                    "copy planes");
@@ -286,6 +288,40 @@ int qlp_warp_micro_tiles_dev(qlp_ctx *h, const uint8_t *img, int H, int W, int n
     if (!h) return fail(QLP_ERR_ARG, "null context");
     if (n < 0 || N < 1 || H < 1 || W < 1) return fail(QLP_ERR_ARG, "warp: bad sizes");
     return launch_warp_micro(&h->c, img, H, W, n, N, top, left, u, v, tiles, max_out);
+}
+
+int qlp_xcorr_images_dev(qlp_ctx *h, int n, int N, int mode, const uint8_t *imgA,
+                         const uint8_t *imgB, int H, int W, const int *top, const int *left,
+                         const float *du, const float *dv, int warp_mode, float *planes,
+                         float *peaks, float *min_floor, float *nbr, int nbr_half)
+{
+    if (!h) return fail(QLP_ERR_ARG, "null context");
+    int rc = xcorr_args(n, N, mode);
+    if (rc) return rc;
+    if (n > 0 && (!imgA || !imgB || !top || !left)) return fail(QLP_ERR_ARG, "xcorr_images: null buffer");
+    if (H < 1 || W < 1) return fail(QLP_ERR_ARG, "xcorr_images: bad image size");
+    if ((du == nullptr) != (dv == nullptr)) return fail(QLP_ERR_ARG, "xcorr_images: du and dv go together");
+    if (warp_mode != QLP_WARP_NONE && warp_mode != QLP_WARP_MICRO)
+        return fail(QLP_ERR_ARG, "xcorr_images: unknown warp mode");
+    PassParams p;
+    p.n = n;
+    p.N = N;
+    p.mode = mode;
+    p.imgA = imgA;
+    p.imgB = imgB;
+    p.H = H;
+    p.W = W;
+    p.top = top;
+    p.left = left;
+    p.du = du;
+    p.dv = dv;
+    p.bmode = warp_mode == QLP_WARP_MICRO ? B_WARP_MICRO : B_CLIP_SHIFT;
+    p.planes = planes;
+    p.peaks = peaks;
+    p.min_floor = min_floor;
+    p.nbr = nbr;
+    p.nbr_half = nbr_half;
+    return launch_pass(&h->c, p);
 }
 
 }  // extern "C"

@@ -46,16 +46,50 @@ struct Context {
 void set_error(const std::string &msg);
 int fail(int code, const std::string &msg);
 int check_hip(hipError_t e, const char *what);
+Context *ctx_of(qlp_ctx *h);
+
+// B-tile formation of a correlation pass whose tiles come from images.
+enum BMode {
+    B_CLIP = 0,        // window at (top, left)
+    B_CLIP_SHIFT = 1,  // window at (round(top + u), round(left + v)): NoImageWarpingStrategy :79-81
+    B_WARP_MICRO = 2,  // bilinear micro warp, SecondImage order
+};
+
+// One correlation pass over n IA pairs.  Tiles come either from u8 tile arrays
+// (tA/tB/maxA/maxB) or straight from two device-resident images (imgA/imgB with
+// per-IA origins top/left and B displacement du/dv).  Outputs are optional:
+// planes (n x D x D), FromCenter peaks (n x 3: i, j, value), min_floor (n) and
+// the peak neighbourhood (n x 2 x (2*nbr_half+1): the column through the peak,
+// then the row) used by the sub-pixel fits.
+struct PassParams {
+    int n = 0, N = 0, mode = QLP_NORM_PER_TILE;
+    const uint8_t *tA = nullptr, *tB = nullptr;
+    const float *maxA = nullptr, *maxB = nullptr;
+    const uint8_t *imgA = nullptr, *imgB = nullptr;
+    int H = 0, W = 0;
+    int per_pair = 0;  // IAs per image pair (0: all n IAs belong to one pair);
+                       // pair k uses images imgA/imgB + k*H*W and origins top/left[ia % per_pair]
+    const int *top = nullptr, *left = nullptr;
+    const float *du = nullptr, *dv = nullptr;
+    int bmode = B_CLIP;
+    float *planes = nullptr;
+    float *peaks = nullptr, *min_floor = nullptr;
+    float *nbr = nullptr;
+    int nbr_half = 0;
+};
 
 // Launchers (qlpiv_xcorr.hip, qlpiv_peaks.hip, qlpiv_subpixel.hip, qlpiv_warp.hip).
 int launch_xcorr(Context *ctx, int n, int N, const uint8_t *tA, const uint8_t *tB,
-                 const float *maxA, const float *maxB, int mode, float *planes,
-                 int peak_kind, float *peaks_out);
+                 const float *maxA, const float *maxB, int mode, float *planes);
+int launch_pass(Context *ctx, const PassParams &p);
 int launch_find_peaks(Context *ctx, int kind, int n, int D, const float *planes, int n_peaks,
                       int kernel_width, float *peaks, float *min_floor);
 int launch_subpixel(Context *ctx, int kind, int pixels, int iterations, int n, int D, int n_peaks,
                     const float *planes, const float *min_floor, const float *peaks_in,
                     float *peaks_out);
+int launch_subpixel_nbr(Context *ctx, int kind, int pixels, int iterations, int n, int D,
+                        int nbr_half, const float *nbr, const float *min_floor,
+                        const float *peaks_in, float *peaks_out, float *du, float *dv);
 int launch_clip(Context *ctx, const uint8_t *img, int H, int W, int n, int N, const int *top,
                 const int *left, uint8_t *tiles, float *maxv);
 int launch_warp_micro(Context *ctx, const uint8_t *img, int H, int W, int n, int N, const int *top,

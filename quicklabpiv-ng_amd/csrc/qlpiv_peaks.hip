@@ -19,28 +19,10 @@
 // position when a block holds no value above 0.
 #include <float.h>
 
+#include "qlpiv_device.h"
 #include "qlpiv_internal.h"
 
 namespace qlpiv {
-
-__device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int m)
-{
-    const int lo = __shfl_xor((int)(uint32_t)v, m, 64);
-    const int hi = __shfl_xor((int)(uint32_t)(v >> 32), m, 64);
-    return ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
-}
-
-// first-visit order of cell (i, j) in FindMaximumFromCenter's scan
-__device__ __forceinline__ int visit_order(int i, int j, int h)
-{
-    const int c = h + 1;
-    if (i == c && j == c) return 0;
-    if (j == c) return (i <= h) ? 1 + 2 * (h - i) : 2 + 2 * (i - h - 1);
-    if (i == c) return (j <= h) ? 1 + 2 * h + 2 * (h - j) : 2 + 2 * h + 2 * (j - h - 1);
-    const int ii = (i <= h) ? h - i : i - h - 1, qi = (i <= h) ? 0 : 1;
-    const int jj = (j <= h) ? h - j : j - h - 1, qj = (j <= h) ? 0 : 1;
-    return 1 + 4 * h + 4 * (ii * h + jj) + qi + 2 * qj;
-}
 
 __global__ __launch_bounds__(256) void peak_from_center_kernel(int n, int D,
                                                                const float *__restrict__ planes,
@@ -51,7 +33,7 @@ __global__ __launch_bounds__(256) void peak_from_center_kernel(int n, int D,
     const int ia = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (ia >= n) return;
     const float *m = planes + (size_t)ia * D * D;
-    const int h = D / 2, c = h + 1;
+    const int h = D / 2;
     uint64_t best = 0;
     int bestCell = 0;
     float mn = FLT_MAX;
@@ -60,11 +42,7 @@ __global__ __launch_bounds__(256) void peak_from_center_kernel(int n, int D,
         if (i == 0 || j == 0) continue;
         const float v = m[idx];
         if (v < mn) mn = v;
-        if (!(v >= 0.0f)) continue;  // never selected: the running maximum starts at 0
-        const uint32_t vb = (v == 0.0f) ? 0u : __float_as_uint(v);  // -0 ties with +0
-        const int d2 = (i - c) * (i - c) + (j - c) * (j - c);
-        const uint64_t key = ((uint64_t)vb << 32) | ((uint64_t)(0x7fff - d2) << 16) |
-                             (uint64_t)(0xffff - visit_order(i, j, h));
+        const uint64_t key = from_center_key(v, i, j, h);
         if (key > best) {
             best = key;
             bestCell = idx;

@@ -1,0 +1,308 @@
+// Multi-pass PIV over device-resident frame pairs.
+//
+// Host side: the IA geometry of every adaptive level
+// (iareas/AdaptiveInterAreaStrategySuperPosition.java:31-197, evaluated in the
+// reference's float arithmetic) and the per-level loop of
+// jobs/LocalPIVOpenCLGpuJob.call :188-409.  Device side, per level:
+//   1. displacement state: zero at level 0, else Direct inheritance from the
+//      parent level (AdaptiveInterVelocityInheritanceStrategyDirect: child
+//      (2I+a, 2J+b) takes parent (I, J); children without a parent keep 0;
+//      NoImageWarping rounds the inherited value);
+//   2. one fused pass: clip A, clip/shift/micro-warp B, correlate, FromCenter,
+//      export the peak neighbourhood;
+//   3. the sub-pixel fit and the displacement update u += i' - D/2.
+// Everything stays in HBM between levels; the host only enqueues launches.
+#include <math.h>
+
+#include <vector>
+
+#include "qlpiv_internal.h"
+
+namespace qlpiv {
+
+struct Level {
+    int N = 0, nI = 0, nJ = 0, n = 0;
+    std::vector<int> top_h, left_h;
+    int *top = nullptr, *left = nullptr;  // n
+    float *u = nullptr, *v = nullptr;     // pairs * n
+    float *peaks = nullptr;               // pairs * n * 3 (integer search)
+    float *mf = nullptr;                  // pairs * n
+    float *nbr = nullptr;                 // pairs * n * 2 * NBR_L
+    float *sub = nullptr;                 // pairs * n * 4
+    float *planes = nullptr;              // pairs * n * D * D (last level, optional)
+};
+
+constexpr int NBR_HALF = 7;
+constexpr int NBR_L = 2 * NBR_HALF + 1;
+
+static inline float f32(double x) { return (float)x; }
+
+static int round_half_up(float x) { return (int)floorf(x + 0.5f); }
+
+// One level of AdaptiveInterAreaStrategySuperPosition (level0: positions
+// margin + round(k * inc); later levels accumulate the float position).
+static void level_grid(int H, int W, int tile, float of, const int m[4], bool level0, Level &L)
+{
+    const int mt = m[0], mb = m[1], ml = m[2], mr = m[3];
+    const float hinc = of * (float)tile, winc = of * (float)tile;
+    const int nJ = (int)(0.01f + ((float)(W - (ml + mr)) - ((float)tile - winc)) / winc);
+    const int nI = (int)(0.01f + ((float)(H - (mt + mb)) - ((float)tile - hinc)) / hinc);
+    const int remI = H - (int)ceilf((float)(nI - 1) * hinc + (float)tile) - (mt + mb);
+    const int remJ = W - (int)ceilf((float)(nJ - 1) * winc + (float)tile) - (ml + mr);
+    const int top0 = mt + (int)floor(remI / 2.0);  // FastMath.floorDiv
+    const int left0 = ml + (int)floor(remJ / 2.0);
+    L.N = tile;
+    L.nI = nI;
+    L.nJ = nJ;
+    L.n = nI * nJ;
+    L.top_h.resize(L.n);
+    L.left_h.resize(L.n);
+    if (level0) {
+        for (int i = 0; i < nI; i++)
+            for (int j = 0; j < nJ; j++) {
+                L.top_h[i * nJ + j] = top0 + round_half_up((float)i * hinc);
+                L.left_h[i * nJ + j] = left0 + round_half_up((float)j * winc);
+            }
+    } else {
+        float tp = (float)top0;
+        for (int i = 0; i < nI; i++, tp += hinc) {
+            float lp = (float)left0;
+            for (int j = 0; j < nJ; j++, lp += winc) {
+                L.top_h[i * nJ + j] = round_half_up(tp);
+                L.left_h[i * nJ + j] = round_half_up(lp);
+            }
+        }
+    }
+}
+
+__global__ void fill_kernel(float *__restrict__ x, float value, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = value;
+}
+
+__global__ void inherit_direct_kernel(int pairs, int nI, int nJ, int pI, int pJ, int round_it,
+                                      const float *__restrict__ pu, const float *__restrict__ pv,
+                                      float *__restrict__ u, float *__restrict__ v)
+{
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = nI * nJ;
+    if (gid >= pairs * n) return;
+    const int pair = gid / n, t = gid - pair * n;
+    const int ci = t / nJ, cj = t - (t / nJ) * nJ;
+    const int qi = ci >> 1, qj = cj >> 1;
+    float iu = 0.0f, iv = 0.0f;
+    if (qi < pI && qj < pJ) {
+        const float a = pu[(size_t)pair * pI * pJ + qi * pJ + qj];
+        const float b = pv[(size_t)pair * pI * pJ + qi * pJ + qj];
+        // accumulateDisplacement onto a reset tile: 0 + increment
+        iu = round_it ? (float)(int)floorf(a + 0.5f) : a;
+        iv = round_it ? (float)(int)floorf(b + 0.5f) : b;
+        iu = 0.0f + iu;
+        iv = 0.0f + iv;
+    }
+    u[gid] = iu;
+    v[gid] = iv;
+}
+
+}  // namespace qlpiv
+
+using namespace qlpiv;
+
+struct qlp_piv_plan {
+    Context *ctx = nullptr;
+    int H = 0, W = 0, pairs = 0;
+    qlp_piv_config cfg{};
+    std::vector<Level> levels;
+    size_t bytes = 0;
+};
+
+static void plan_free(qlp_piv_plan *p)
+{
+    for (auto &L : p->levels) {
+        void *bufs[] = {L.top, L.left, L.u, L.v, L.peaks, L.mf, L.nbr, L.sub, L.planes};
+        for (void *b : bufs)
+            if (b) (void)hipFree(b);
+    }
+    if (p->ctx) p->ctx->hbm_in_use -= p->bytes;
+    p->levels.clear();
+}
+
+extern "C" {
+
+int qlp_piv_plan_create(qlp_ctx *h, int H, int W, int pairs, const qlp_piv_config *cfg,
+                        qlp_piv_plan **out)
+{
+    if (!h || !cfg || !out) return fail(QLP_ERR_ARG, "piv_plan_create: null argument");
+    *out = nullptr;
+    Context *ctx = ctx_of(h);
+    if (H < 1 || W < 1 || pairs < 1) return fail(QLP_ERR_ARG, "piv_plan_create: bad frame size");
+    if (cfg->overlap_factor < 0.1f || cfg->overlap_factor > 0.9f)
+        return fail(QLP_ERR_ARG, "Overlap factor must greater than 1/10 and less than 9/10");
+    if (cfg->ia_end < 1 || cfg->ia_start % cfg->ia_end)
+        return fail(QLP_ERR_ARG, "Interrogation Area start pixels is not a multiple of Interrogation Area end pixels");
+    int steps = cfg->ia_start / cfg->ia_end, nlev = 1;
+    while ((1 << (nlev - 1)) < steps) nlev++;
+    if ((1 << (nlev - 1)) != steps)
+        return fail(QLP_ERR_ARG, "Total number of steps could not be properly computed since step sizes ratio is not a power of two");
+    if (cfg->warp_mode != QLP_WARP_NONE && cfg->warp_mode != QLP_WARP_MICRO)
+        return fail(QLP_ERR_ARG, "piv_plan_create: unknown warp mode");
+    if (cfg->norm_mode != QLP_NORM_PER_TILE && cfg->norm_mode != QLP_NORM_A_MAX)
+        return fail(QLP_ERR_ARG, "piv_plan_create: unknown normalisation mode");
+    for (int tile = cfg->ia_start; tile >= cfg->ia_end; tile /= 2)
+        if (tile != 16 && tile != 32 && tile != 64)
+            return fail(QLP_ERR_UNSUPPORTED, "piv_plan_create: IA sizes must be 16, 32 or 64");
+    if (cfg->subpix_kind >= 0) {
+        const int half = cfg->subpix_kind == QLP_SUBPIX_POLY ? 1 : cfg->subpix_pixels / 2;
+        if (half > NBR_HALF || (cfg->subpix_kind != QLP_SUBPIX_POLY && cfg->subpix_pixels % 2 == 0))
+            return fail(QLP_ERR_ARG, "piv_plan_create: sub-pixel window must be odd and <= 15");
+    }
+    qlp_piv_plan *p = new qlp_piv_plan();
+    p->ctx = ctx;
+    p->H = H;
+    p->W = W;
+    p->pairs = pairs;
+    p->cfg = *cfg;
+    const int m[4] = {cfg->margin_top, cfg->margin_bottom, cfg->margin_left, cfg->margin_right};
+    size_t need = 0;
+    for (int l = 0, tile = cfg->ia_start; l < nlev; l++, tile /= 2) {
+        Level L;
+        level_grid(H, W, tile, cfg->overlap_factor, m, l == 0, L);
+        if (L.nI < 1 || L.nJ < 1) {
+            delete p;
+            return fail(QLP_ERR_ARG, "piv_plan_create: no IA fits the frame");
+        }
+        const size_t n = L.n, pn = (size_t)pairs * n, D = 2 * tile - 1;
+        need += n * 8 + pn * (8 + 12 + 4 + 4 * 2 * NBR_L + 16);
+        if (cfg->keep_planes && l == nlev - 1) need += pn * D * D * 4;
+        p->levels.push_back(std::move(L));
+    }
+    if (ctx->hbm_budget && ctx->hbm_in_use + need > ctx->hbm_budget) {
+        delete p;
+        return fail(QLP_ERR_BUDGET, "piv plan exceeds the context's HBM budget");
+    }
+    int rc = QLP_OK;
+    for (size_t l = 0; l < p->levels.size() && !rc; l++) {
+        Level &L = p->levels[l];
+        const size_t n = L.n, pn = (size_t)pairs * n, D = 2 * L.N - 1;
+        rc = check_hip(hipMalloc(&L.top, n * 4), "hipMalloc plan");
+        if (!rc) rc = check_hip(hipMalloc(&L.left, n * 4), "hipMalloc plan");
+        if (!rc) rc = check_hip(hipMalloc(&L.u, pn * 4), "hipMalloc plan");
+        if (!rc) rc = check_hip(hipMalloc(&L.v, pn * 4), "hipMalloc plan");
+        if (!rc) rc = check_hip(hipMalloc(&L.peaks, pn * 12), "hipMalloc plan");
+        if (!rc) rc = check_hip(hipMalloc(&L.mf, pn * 4), "hipMalloc plan");
+        if (!rc) rc = check_hip(hipMalloc(&L.nbr, pn * 4 * 2 * NBR_L), "hipMalloc plan");
+        if (!rc) rc = check_hip(hipMalloc(&L.sub, pn * 16), "hipMalloc plan");
+        if (!rc && cfg->keep_planes && l + 1 == p->levels.size())
+            rc = check_hip(hipMalloc(&L.planes, pn * D * D * 4), "hipMalloc plan");
+        if (!rc)
+            rc = check_hip(hipMemcpy(L.top, L.top_h.data(), n * 4, hipMemcpyHostToDevice), "upload grid");
+        if (!rc)
+            rc = check_hip(hipMemcpy(L.left, L.left_h.data(), n * 4, hipMemcpyHostToDevice), "upload grid");
+    }
+    p->bytes = need;
+    ctx->hbm_in_use += need;
+    if (rc) {
+        plan_free(p);
+        delete p;
+        return rc;
+    }
+    *out = p;
+    return QLP_OK;
+}
+
+int qlp_piv_plan_destroy(qlp_piv_plan *p)
+{
+    if (!p) return QLP_OK;
+    if (p->ctx) (void)hipStreamSynchronize(p->ctx->stream);
+    plan_free(p);
+    delete p;
+    return QLP_OK;
+}
+
+int qlp_piv_plan_levels(qlp_piv_plan *p, int *n_levels)
+{
+    if (!p || !n_levels) return fail(QLP_ERR_ARG, "null argument");
+    *n_levels = (int)p->levels.size();
+    return QLP_OK;
+}
+
+int qlp_piv_plan_level(qlp_piv_plan *p, int level, int *ia, int *nI, int *nJ, int *top, int *left)
+{
+    if (!p) return fail(QLP_ERR_ARG, "null plan");
+    if (level < 0 || level >= (int)p->levels.size()) return fail(QLP_ERR_ARG, "level out of range");
+    const Level &L = p->levels[level];
+    if (ia) *ia = L.N;
+    if (nI) *nI = L.nI;
+    if (nJ) *nJ = L.nJ;
+    if (top)
+        for (int k = 0; k < L.n; k++) top[k] = L.top_h[k];
+    if (left)
+        for (int k = 0; k < L.n; k++) left[k] = L.left_h[k];
+    return QLP_OK;
+}
+
+int qlp_piv_run_dev(qlp_piv_plan *p, const uint8_t *imgsA, const uint8_t *imgsB)
+{
+    if (!p || !imgsA || !imgsB) return fail(QLP_ERR_ARG, "piv_run: null argument");
+    Context *ctx = p->ctx;
+    const qlp_piv_config &c = p->cfg;
+    for (size_t l = 0; l < p->levels.size(); l++) {
+        Level &L = p->levels[l];
+        const int pn = p->pairs * L.n;
+        const int D = 2 * L.N - 1;
+        if (l == 0) {
+            const size_t blocks = ((size_t)pn + 255) / 256;
+            hipLaunchKernelGGL(fill_kernel, dim3(blocks), dim3(256), 0, ctx->stream, L.u, 0.0f, (size_t)pn);
+            hipLaunchKernelGGL(fill_kernel, dim3(blocks), dim3(256), 0, ctx->stream, L.v, 0.0f, (size_t)pn);
+        } else {
+            const Level &P = p->levels[l - 1];
+            hipLaunchKernelGGL(inherit_direct_kernel, dim3((pn + 255) / 256), dim3(256), 0,
+                               ctx->stream, p->pairs, L.nI, L.nJ, P.nI, P.nJ,
+                               c.warp_mode == QLP_WARP_NONE ? 1 : 0, P.u, P.v, L.u, L.v);
+        }
+        int rc = check_hip(hipGetLastError(), "state kernel launch");
+        if (rc) return rc;
+        PassParams q;
+        q.n = pn;
+        q.N = L.N;
+        q.mode = c.norm_mode;
+        q.imgA = imgsA;
+        q.imgB = imgsB;
+        q.H = p->H;
+        q.W = p->W;
+        q.per_pair = L.n;
+        q.top = L.top;
+        q.left = L.left;
+        q.du = (l == 0) ? nullptr : L.u;
+        q.dv = (l == 0) ? nullptr : L.v;
+        q.bmode = c.warp_mode == QLP_WARP_MICRO ? B_WARP_MICRO : B_CLIP_SHIFT;
+        q.planes = L.planes;
+        q.peaks = L.peaks;
+        q.min_floor = L.mf;
+        q.nbr = L.nbr;
+        q.nbr_half = NBR_HALF;
+        rc = launch_pass(ctx, q);
+        if (rc) return rc;
+        const int kind = ((int)l >= c.interpolator_start_step) ? c.subpix_kind : -1;
+        rc = launch_subpixel_nbr(ctx, kind, c.subpix_pixels, c.subpix_iterations, pn, D, NBR_HALF,
+                                 L.nbr, L.mf, L.peaks, L.sub, L.u, L.v);
+        if (rc) return rc;
+    }
+    return QLP_OK;
+}
+
+int qlp_piv_result_dev(qlp_piv_plan *p, int level, const float **u, const float **v,
+                       const float **peaks, const float **planes)
+{
+    if (!p) return fail(QLP_ERR_ARG, "null plan");
+    if (level < 0 || level >= (int)p->levels.size()) return fail(QLP_ERR_ARG, "level out of range");
+    const Level &L = p->levels[level];
+    if (u) *u = L.u;
+    if (v) *v = L.v;
+    if (peaks) *peaks = L.sub;
+    if (planes) *planes = L.planes;
+    return QLP_OK;
+}
+
+}  // extern "C"

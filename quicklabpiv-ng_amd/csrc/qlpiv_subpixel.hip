@@ -265,20 +265,14 @@ __device__ void guo_estimate(int iterations, int n, const float *xs, const float
     amp = (float)(exp(coefs[0] - coefs[1] * coefs[1] / (4.0 * coefs[2])));
 }
 
-__global__ __launch_bounds__(256) void subpixel_kernel(int kind, int pixels, int iterations, int n,
-                                                      int D, int P, const float *__restrict__ planes,
-                                                      const float *__restrict__ min_floor,
-                                                      const float *__restrict__ peaks_in,
-                                                      float *__restrict__ peaks_out)
+constexpr int NBR_MAX_HALF = 7;
+
+// Fit one peak.  col[NBR_MAX_HALF + d] = plane[maxI + d][maxJ] and
+// row[NBR_MAX_HALF + d] = plane[maxI][maxJ + d] for |d| <= half; out = (i', j',
+// value', status).  The border and (0,0) checks use the plane size D.
+__device__ void fit_peak(int kind, int pixels, int iterations, int D, float pI, float pJ, float pV,
+                         float mf, const float *col, const float *row, float *out)
 {
-    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-    if (gid >= n * P) return;
-    const int ia = gid / P;
-    const float *m = planes + (size_t)ia * D * D;
-    const float pI = peaks_in[(size_t)gid * 3 + 0];
-    const float pJ = peaks_in[(size_t)gid * 3 + 1];
-    const float pV = peaks_in[(size_t)gid * 3 + 2];
-    float *out = peaks_out + (size_t)gid * 4;
     out[0] = pI;
     out[1] = pJ;
     out[2] = pV;
@@ -288,13 +282,13 @@ __global__ __launch_bounds__(256) void subpixel_kernel(int kind, int pixels, int
     if (maxI == 0 && maxJ == 0) return;
     if (maxI - half < 0 || maxI + half >= D) return;
     if (maxJ - half < 0 || maxJ + half >= D) return;
-    const float mf = min_floor[ia];
+    const float *cy = col + NBR_MAX_HALF, *rx = row + NBR_MAX_HALF;
 
     if (kind == QLP_SUBPIX_POLY) {
         float vy[3], vx[3];
         bool gauss = true;
         for (int d = -1; d <= 1; d++) {
-            const float a = m[(maxI + d) * D + maxJ], c = m[maxI * D + maxJ + d];
+            const float a = cy[d], c = rx[d];
             if (a < 0.0f || c < 0.0f) gauss = false;
             vy[d + 1] = a + 1e-8f;
             vx[d + 1] = c + 1e-8f;
@@ -331,16 +325,15 @@ __global__ __launch_bounds__(256) void subpixel_kernel(int kind, int pixels, int
         const float add = (mf < 0.0f) ? (-mf + 1e-8f) : 0.0f;
         double mi, mj;
         if (pixels == 3) {
-            const float ym = m[(maxI - 1) * D + maxJ] + add, y0 = m[maxI * D + maxJ] + add,
-                        yp = m[(maxI + 1) * D + maxJ] + add;
-            const float xm = m[maxI * D + maxJ - 1] + add, xp = m[maxI * D + maxJ + 1] + add;
+            const float ym = cy[-1] + add, y0 = cy[0] + add, yp = cy[1] + add;
+            const float xm = rx[-1] + add, x0 = rx[0] + add, xp = rx[1] + add;
             mi = (double)((float)maxI + gauss3_offset(ym, y0, yp));
-            mj = (double)((float)maxJ + gauss3_offset(xm, y0, xp));
+            mj = (double)((float)maxJ + gauss3_offset(xm, x0, xp));
         } else {
-            double yv[15], xv[15];
+            double yv[2 * NBR_MAX_HALF + 1], xv[2 * NBR_MAX_HALF + 1];
             for (int d = -half; d <= half; d++) {
-                yv[d + half] = (double)(m[(maxI + d) * D + maxJ] + add);
-                xv[d + half] = (double)(m[maxI * D + maxJ + d] + add);
+                yv[d + half] = (double)(cy[d] + add);
+                xv[d + half] = (double)(rx[d] + add);
             }
             mi = (double)maxI + gauss_lsq_mean(pixels, yv);
             mj = (double)maxJ + gauss_lsq_mean(pixels, xv);
@@ -364,16 +357,16 @@ __global__ __launch_bounds__(256) void subpixel_kernel(int kind, int pixels, int
         return;
     }
     const float add = (mf < 0.0f) ? (-mf + 1e-8f) : 0.0f;
-    float xs[15], ys[15];
+    float xs[2 * NBR_MAX_HALF + 1], ys[2 * NBR_MAX_HALF + 1];
     for (int d = -half; d <= half; d++) {
         xs[d + half] = (float)(maxI + d);
-        ys[d + half] = m[(maxI + d) * D + maxJ] + add;
+        ys[d + half] = cy[d] + add;
     }
     float yMean, yAmp, xMean, xAmp;
     guo_estimate(iterations, pixels, xs, ys, yMean, yAmp);
     for (int d = -half; d <= half; d++) {
         xs[d + half] = (float)(maxJ + d);
-        ys[d + half] = m[maxI * D + maxJ + d] + add;
+        ys[d + half] = rx[d] + add;
     }
     guo_estimate(iterations, pixels, xs, ys, xMean, xAmp);
     float fy = yMean, fx = xMean;
@@ -399,20 +392,117 @@ __global__ __launch_bounds__(256) void subpixel_kernel(int kind, int pixels, int
     out[3] = (float)((yValid ? ST_I : 0) | (xValid ? ST_J : 0));
 }
 
+// Fits for every (IA, peak) of planes stored in HBM.
+__global__ __launch_bounds__(256) void subpixel_kernel(int kind, int pixels, int iterations, int n,
+                                                      int D, int P, const float *__restrict__ planes,
+                                                      const float *__restrict__ min_floor,
+                                                      const float *__restrict__ peaks_in,
+                                                      float *__restrict__ peaks_out)
+{
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= n * P) return;
+    const int ia = gid / P;
+    const float *m = planes + (size_t)ia * D * D;
+    const float pI = peaks_in[(size_t)gid * 3 + 0];
+    const float pJ = peaks_in[(size_t)gid * 3 + 1];
+    const float pV = peaks_in[(size_t)gid * 3 + 2];
+    const int maxI = (int)pI, maxJ = (int)pJ;
+    const int half = (kind == QLP_SUBPIX_POLY) ? 1 : pixels / 2;
+    float col[2 * NBR_MAX_HALF + 1], row[2 * NBR_MAX_HALF + 1];
+    for (int d = -half; d <= half; d++) {
+        const int ri = maxI + d, cj = maxJ + d;
+        col[NBR_MAX_HALF + d] = (ri >= 0 && ri < D && maxJ >= 0 && maxJ < D) ? m[ri * D + maxJ] : 0.0f;
+        row[NBR_MAX_HALF + d] = (cj >= 0 && cj < D && maxI >= 0 && maxI < D) ? m[maxI * D + cj] : 0.0f;
+    }
+    fit_peak(kind, pixels, iterations, D, pI, pJ, pV, min_floor[ia], col, row,
+             peaks_out + (size_t)gid * 4);
+}
+
+// Fits from the neighbourhoods exported by the fused correlation pass, and the
+// displacement update of the multi-pass loop: u += i' - D/2, v += j' - D/2
+// (MaxCrossResult.getNthDisplacementU :141-149, Tile.accumulateDisplacement
+// :103-136 with out-of-bound clipping allowed).
+__global__ __launch_bounds__(256) void subpixel_nbr_kernel(int kind, int pixels, int iterations,
+                                                          int n, int D, int nbr_half,
+                                                          const float *__restrict__ nbr,
+                                                          const float *__restrict__ min_floor,
+                                                          const float *__restrict__ peaks_in,
+                                                          float *__restrict__ peaks_out,
+                                                          float *__restrict__ du,
+                                                          float *__restrict__ dv)
+{
+    const int ia = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ia >= n) return;
+    const int L = 2 * nbr_half + 1;
+    const float *nb = nbr + (size_t)ia * 2 * L;
+    float col[2 * NBR_MAX_HALF + 1], row[2 * NBR_MAX_HALF + 1];
+    for (int d = -nbr_half; d <= nbr_half; d++) {
+        col[NBR_MAX_HALF + d] = nb[d + nbr_half];
+        row[NBR_MAX_HALF + d] = nb[L + d + nbr_half];
+    }
+    float res[4];
+    const float pI = peaks_in[(size_t)ia * 3 + 0], pJ = peaks_in[(size_t)ia * 3 + 1];
+    if (kind < 0) {  // no interpolator: the integer peak
+        res[0] = pI;
+        res[1] = pJ;
+        res[2] = peaks_in[(size_t)ia * 3 + 2];
+        res[3] = (float)ST_SKIP;
+    } else {
+        fit_peak(kind, pixels, iterations, D, pI, pJ, peaks_in[(size_t)ia * 3 + 2], min_floor[ia],
+                 col, row, res);
+    }
+    if (peaks_out) {
+        float *o = peaks_out + (size_t)ia * 4;
+        o[0] = res[0];
+        o[1] = res[1];
+        o[2] = res[2];
+        o[3] = res[3];
+    }
+    if (du) {
+        const int c = D / 2;
+        du[ia] = du[ia] + (res[0] - (float)c);
+        dv[ia] = dv[ia] + (res[1] - (float)c);
+    }
+}
+
+static int check_subpixel_args(int kind, int pixels, int iterations)
+{
+    if (kind != QLP_SUBPIX_GAUSS1D && kind != QLP_SUBPIX_GUO && kind != QLP_SUBPIX_POLY)
+        return fail(QLP_ERR_ARG, "subpixel: unknown interpolator kind");
+    if (kind != QLP_SUBPIX_POLY && (pixels < 3 || pixels > 2 * NBR_MAX_HALF + 1 || (pixels & 1) == 0))
+        return fail(QLP_ERR_ARG, "subpixel: pixels must be odd, 3..15");
+    if (kind == QLP_SUBPIX_GUO && iterations < 1)
+        return fail(QLP_ERR_ARG, "subpixel: Guo iterations must be >= 1");
+    return QLP_OK;
+}
+
 int launch_subpixel(Context *ctx, int kind, int pixels, int iterations, int n, int D, int n_peaks,
                     const float *planes, const float *min_floor, const float *peaks_in,
                     float *peaks_out)
 {
     if (n == 0) return QLP_OK;
-    if (kind != QLP_SUBPIX_GAUSS1D && kind != QLP_SUBPIX_GUO && kind != QLP_SUBPIX_POLY)
-        return fail(QLP_ERR_ARG, "subpixel: unknown interpolator kind");
-    if (kind != QLP_SUBPIX_POLY && (pixels < 3 || pixels > 15 || (pixels & 1) == 0))
-        return fail(QLP_ERR_ARG, "subpixel: pixels must be odd, 3..15");
-    if (kind == QLP_SUBPIX_GUO && iterations < 1)
-        return fail(QLP_ERR_ARG, "subpixel: Guo iterations must be >= 1");
+    int rc = check_subpixel_args(kind, pixels, iterations);
+    if (rc) return rc;
     const int total = n * n_peaks;
     hipLaunchKernelGGL(subpixel_kernel, dim3((total + 255) / 256), dim3(256), 0, ctx->stream, kind,
                        pixels, iterations, n, D, n_peaks, planes, min_floor, peaks_in, peaks_out);
+    return check_hip(hipGetLastError(), "subpixel kernel launch");
+}
+
+int launch_subpixel_nbr(Context *ctx, int kind, int pixels, int iterations, int n, int D,
+                        int nbr_half, const float *nbr, const float *min_floor,
+                        const float *peaks_in, float *peaks_out, float *du, float *dv)
+{
+    if (n == 0) return QLP_OK;
+    if (kind >= 0) {
+        int rc = check_subpixel_args(kind, pixels, iterations);
+        if (rc) return rc;
+        const int half = (kind == QLP_SUBPIX_POLY) ? 1 : pixels / 2;
+        if (half > nbr_half) return fail(QLP_ERR_ARG, "subpixel: neighbourhood too small");
+    }
+    hipLaunchKernelGGL(subpixel_nbr_kernel, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, kind,
+                       pixels, iterations, n, D, nbr_half, nbr, min_floor, peaks_in, peaks_out, du,
+                       dv);
     return check_hip(hipGetLastError(), "subpixel kernel launch");
 }
 

@@ -14,6 +14,7 @@
 //   maximum 255 (doBiLinearWarp :298).
 // Rows of the image are read by consecutive lanes (coalesced); one work-group
 // per tile.
+#include "qlpiv_device.h"
 #include "qlpiv_internal.h"
 
 namespace qlpiv {
@@ -48,42 +49,6 @@ __global__ __launch_bounds__(256) void clip_kernel(const uint8_t *__restrict__ i
         for (int w = 0; w < (int)(blockDim.x >> 6); w++) m = max(m, s_max[w]);
         maxv[t] = (float)(m < 16 ? 16 : m);
     }
-}
-
-__device__ __forceinline__ int fm_round(float x) { return (int)floorf(x + 0.5f); }
-
-// warpPixelOnImage :336-407 (SecondImage: source = target + displacement)
-__device__ __forceinline__ float warp_pixel(const uint8_t *__restrict__ img, int H, int W, float u,
-                                            float v, int i, int j)
-{
-    float sI = (float)i + u, sJ = (float)j + v;
-    int pI = (short)fm_round(sI), pJ = (short)fm_round(sJ);
-    if (pI < 0) {
-        sI = 0.0f;
-        pI = 0;
-    } else if (pI >= H) {
-        sI = (float)(H - 1);
-        pI = H - 1;
-    }
-    if (pJ < 0) {
-        sJ = 0.0f;
-        pJ = 0;
-    } else if (pJ >= W) {
-        sJ = (float)(W - 1);
-        pJ = W - 1;
-    }
-    const float dU = sI - (float)pI, dV = sJ - (float)pJ;
-    int oI = dU >= 0.0f ? 1 : -1, oJ = dV >= 0.0f ? 1 : -1;
-    if (oI + pI < 0 || oI + pI >= H) oI = 0;
-    if (oJ + pJ < 0 || oJ + pJ >= W) oJ = 0;
-    const float wU = fabsf(dU), wV = fabsf(dV);
-    const float tl = img[(size_t)pI * W + pJ];
-    const float tr = img[(size_t)pI * W + pJ + oJ];
-    const float bl = img[(size_t)(pI + oI) * W + pJ];
-    const float br = img[(size_t)(pI + oI) * W + pJ + oJ];
-    const float f = (1.0f - wU) * (1.0f - wV) * tl + (1.0f - wU) * wV * tr +
-                    wU * (1.0f - wV) * bl + wU * wV * br;
-    return (float)fm_round(f);
 }
 
 __global__ __launch_bounds__(256) void warp_micro_kernel(const uint8_t *__restrict__ img, int H,

@@ -24,6 +24,7 @@
 //   false : CrossCorrelationRealFFTParStdJob (each tile by its own max; product
 //           divided by M^2 once; inverse through the re/im swap of the forward
 //           transform)
+#include "qlpiv_device.h"
 #include "qlpiv_internal.h"
 
 namespace qlpiv {
@@ -200,48 +201,129 @@ __device__ __forceinline__ void unpack_product(float *__restrict__ re, float *__
     __syncthreads();
 }
 
-template <int LOGM, bool MODE_CPU>
-__global__ __launch_bounds__(XcShape<LOGM>::THREADS) void xcorr_exact_kernel(
-    int n, const uint8_t *__restrict__ tA, const uint8_t *__restrict__ tB,
-    const float *__restrict__ maxA, const float *__restrict__ maxB,
-    const float2 *__restrict__ tw_fwd, const float2 *__restrict__ tw_inv,
-    float *__restrict__ planes)
+// Block-wide maximum of an int (tile maxima).
+template <int THREADS>
+__device__ __forceinline__ int block_max_int(int v, int *s_red)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    int m = s_red[0];
+#pragma unroll
+    for (int w = 1; w < THREADS / 64; w++) m = max(m, s_red[w]);
+    __syncthreads();
+    return m;
+}
+
+// One correlation pass: tile formation, exact FFT correlation, and the
+// optional epilogues (plane store, FromCenter peak with its neighbourhood).
+//   FROM_IMAGE : tiles are cut from the images (A: clip at (top, left);
+//                B: clip, shifted clip or micro warp by (du, dv)), the tile
+//                maxima are computed here (>= 16 clamp; 255 for a warped B)
+//   PEAK       : FromCenter on the plane while it is still in LDS
+template <int LOGM, bool MODE_CPU, bool FROM_IMAGE, bool PEAK>
+__global__ __launch_bounds__(XcShape<LOGM>::THREADS) void xcorr_pass_kernel(
+    PassParams p, const float2 *__restrict__ tw_fwd, const float2 *__restrict__ tw_inv)
 {
     using X = XcShape<LOGM>;
-    constexpr int M = X::M, N = X::N, S = X::S, D = X::D;
+    constexpr int M = X::M, N = X::N, S = X::S, D = X::D, NT = X::THREADS;
+    constexpr int PIX = N * N / NT;  // pixels per thread (2, 4 or 8)
     __shared__ float s_re[M * S];
     __shared__ float s_im[M * S];
     __shared__ float2 s_twf[M / 2], s_twi[M / 2];
+    __shared__ int s_red[NT / 64];
+    __shared__ uint64_t s_key[NT / 64];
+    __shared__ int s_cell[NT / 64];
+    __shared__ float s_min[NT / 64];
 
     const int ia = blockIdx.x;
-    if (ia >= n) return;
-    for (int k = threadIdx.x; k < M / 2; k += X::THREADS) {
+    if (ia >= p.n) return;
+    for (int k = threadIdx.x; k < M / 2; k += NT) {
         s_twf[k] = tw_fwd[k];
         s_twi[k] = tw_inv[k];
     }
-    // normalised, mirrored, zero-padded pack (only the N x N data block is
-    // written; the transforms never read the padding)
-    const float mA = maxA[ia];
-    const float mB = MODE_CPU ? mA : maxB[ia];
-    const uint32_t *a32 = reinterpret_cast<const uint32_t *>(tA + (size_t)ia * N * N);
-    const uint32_t *b32 = reinterpret_cast<const uint32_t *>(tB + (size_t)ia * N * N);
-    for (int w = threadIdx.x; w < N * N / 4; w += X::THREADS) {
-        const uint32_t va = a32[w], vb = b32[w];
-#pragma unroll
-        for (int e = 0; e < 4; e++) {
-            const int idx = 4 * w + e;  // natural index of the pixel
-            const float fa = (float)((va >> (8 * e)) & 0xffu);
-            const float fb = (float)((vb >> (8 * e)) & 0xffu);
-            // F' lands mirrored: F'[r][c] = A[N-1-r][N-1-c]
-            const int mi = N * N - 1 - idx;
-            const int fr = mi / N, fc = mi % N;
-            s_re[fr * S + fc] = fa / mA * 16.0f + 1.0f;
-            const int gr = idx / N, gc = idx % N;
-            s_im[gr * S + gc] = fb / mB * 16.0f + 1.0f;
+
+    // ---- tile formation + normalisation: v / max * 16 + 1, F mirrored -------
+    // (Matrix.copyMirroredMatrixToArrayNormalizeAndOffset :282-288,
+    //  copyMatrixToArrayAndNormalizeAndOffset :175-181; only the N x N data
+    //  block is written, the transforms never read the zero padding)
+    float va[PIX], vb[PIX];
+    float mA, mB;
+    if (FROM_IMAGE) {
+        const int per_pair = p.per_pair > 0 ? p.per_pair : p.n;
+        const int pair = ia / per_pair, t = ia - pair * per_pair;
+        const size_t img_off = (size_t)pair * p.H * p.W;
+        const uint8_t *__restrict__ imA = p.imgA + img_off;
+        const uint8_t *__restrict__ imB = p.imgB + img_off;
+        const int y0 = p.top[t], x0 = p.left[t];
+        int by0 = y0, bx0 = x0;
+        float fu = 0.0f, fv = 0.0f;
+        const bool warp = (p.bmode == B_WARP_MICRO) && p.du != nullptr;
+        if (p.du != nullptr) {
+            const float u = p.du[ia], v = p.dv[ia];  // per IA of the batch
+            if (p.bmode == B_CLIP_SHIFT) {
+                by0 = fm_round((float)y0 + u);
+                bx0 = fm_round((float)x0 + v);
+            } else if (warp) {
+                const float flu = floorf(u), flv = floorf(v);
+                by0 = (short)(y0 + (int)flu);
+                bx0 = (short)(x0 + (int)flv);
+                fu = u - flu;  // createMicroDisplacements :195-196
+                fv = v - flv;
+            }
         }
+        int amax = 0, bmax = 0;
+#pragma unroll
+        for (int k = 0; k < PIX; k++) {
+            const int idx = threadIdx.x + k * NT;
+            const int r = idx / N, c = idx - (idx / N) * N;
+            int a = 0, b = 0;
+            const int ya = y0 + r, xa = x0 + c;
+            if (ya >= 0 && ya < p.H && xa >= 0 && xa < p.W) a = imA[(size_t)ya * p.W + xa];
+            if (warp) {
+                b = (int)warp_pixel(imB, p.H, p.W, fu, fv, by0 + r, bx0 + c);
+            } else {
+                const int yb = by0 + r, xb = bx0 + c;
+                if (yb >= 0 && yb < p.H && xb >= 0 && xb < p.W) {
+                    b = imB[(size_t)yb * p.W + xb];
+                    bmax = max(bmax, b);
+                }
+            }
+            amax = max(amax, a);
+            va[k] = (float)a;
+            vb[k] = (float)b;
+        }
+        amax = block_max_int<NT>(amax, s_red);
+        mA = (float)(amax < 16 ? 16 : amax);
+        if (warp) {
+            mB = 255.0f;  // warped tiles carry the image maximum (doBiLinearWarp :298)
+        } else {
+            bmax = block_max_int<NT>(bmax, s_red);
+            mB = (float)(bmax < 16 ? 16 : bmax);
+        }
+        if (MODE_CPU) mB = mA;
+    } else {
+        mA = p.maxA[ia];
+        mB = MODE_CPU ? mA : p.maxB[ia];
+        const uint8_t *a8 = p.tA + (size_t)ia * N * N, *b8 = p.tB + (size_t)ia * N * N;
+#pragma unroll
+        for (int k = 0; k < PIX; k++) {
+            const int idx = threadIdx.x + k * NT;
+            va[k] = (float)a8[idx];
+            vb[k] = (float)b8[idx];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < PIX; k++) {
+        const int idx = threadIdx.x + k * NT;
+        const int mi = N * N - 1 - idx;  // F'[r][c] = A[N-1-r][N-1-c]
+        s_re[(mi / N) * S + (mi % N)] = va[k] / mA * 16.0f + 1.0f;
+        s_im[(idx / N) * S + (idx % N)] = vb[k] / mB * 16.0f + 1.0f;
     }
     __syncthreads();
 
+    // ---- transforms ----------------------------------------------------------
     // forward: rows (only the N data rows are non-zero), then all columns
     fft_lines<LOGM, true, false>(s_re, s_im, S, 1, N, s_twf);
     fft_lines<LOGM, true, false>(s_re, s_im, 1, S, M, s_twf);
@@ -250,45 +332,145 @@ __global__ __launch_bounds__(XcShape<LOGM>::THREADS) void xcorr_exact_kernel(
     fft_lines<LOGM, false, MODE_CPU>(s_re, s_im, S, 1, M, s_twi);
     fft_lines<LOGM, false, MODE_CPU>(s_re, s_im, 1, S, D, s_twi);
 
-    float *out = planes + (size_t)ia * D * D;
-    for (int idx = threadIdx.x; idx < D * D; idx += X::THREADS) {
-        const int i = idx / D, j = idx - (idx / D) * D;
-        out[idx] = s_re[i * S + j];
+    // ---- epilogues -------------------------------------------------------------
+    if (p.planes) {
+        float *out = p.planes + (size_t)ia * D * D;
+        for (int idx = threadIdx.x; idx < D * D; idx += NT) {
+            const int i = idx / D, j = idx - (idx / D) * D;
+            out[idx] = s_re[i * S + j];
+        }
+    }
+    if (PEAK) {
+        constexpr int h = D / 2;
+        uint64_t best = 0;
+        int bestCell = 0;
+        float mn = 3.40282347e+38f;
+        for (int idx = threadIdx.x; idx < D * D; idx += NT) {
+            const int i = idx / D, j = idx - (idx / D) * D;
+            if (i == 0 || j == 0) continue;
+            const float v = s_re[i * S + j];
+            if (v < mn) mn = v;
+            const uint64_t key = from_center_key(v, i, j, h);
+            if (key > best) {
+                best = key;
+                bestCell = idx;
+            }
+        }
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) {
+            const uint64_t ob = shfl_xor_u64(best, o);
+            const int oc = __shfl_xor(bestCell, o, 64);
+            const float om = __shfl_xor(mn, o, 64);
+            if (ob > best) {
+                best = ob;
+                bestCell = oc;
+            }
+            if (om < mn) mn = om;
+        }
+        if ((threadIdx.x & 63) == 0) {
+            s_key[threadIdx.x >> 6] = best;
+            s_cell[threadIdx.x >> 6] = bestCell;
+            s_min[threadIdx.x >> 6] = mn;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int w = 1; w < NT / 64; w++) {
+                if (s_key[w] > best) {
+                    best = s_key[w];
+                    bestCell = s_cell[w];
+                }
+                if (s_min[w] < mn) mn = s_min[w];
+            }
+            // key 0: no visited value >= 0, the reference keeps (0, 0, 0)
+            int pi = 0, pj = 0;
+            float pv = 0.0f;
+            if (best != 0) {
+                pi = bestCell / D;
+                pj = bestCell % D;
+                pv = s_re[pi * S + pj];
+            }
+            if (p.peaks) {
+                p.peaks[(size_t)ia * 3 + 0] = (float)pi;
+                p.peaks[(size_t)ia * 3 + 1] = (float)pj;
+                p.peaks[(size_t)ia * 3 + 2] = pv;
+            }
+            if (p.min_floor) p.min_floor[ia] = mn;
+            if (p.nbr) {
+                const int L = 2 * p.nbr_half + 1;
+                float *nb = p.nbr + (size_t)ia * 2 * L;
+                for (int d = -p.nbr_half; d <= p.nbr_half; d++) {
+                    const int ri = pi + d, cj = pj + d;
+                    nb[d + p.nbr_half] = (ri >= 0 && ri < D) ? s_re[ri * S + pj] : 0.0f;
+                    nb[L + d + p.nbr_half] = (cj >= 0 && cj < D) ? s_re[pi * S + cj] : 0.0f;
+                }
+            }
+        }
+    }
+}
+
+template <int LOGM, bool MODE_CPU>
+static void launch_pass_mc(Context *ctx, const PassParams &p, bool from_image, bool peak)
+{
+    using X = XcShape<LOGM>;
+    const int li = LOGM - 5;
+    const float2 *twf = ctx->tables.tw[li][TW_FWD];
+    const float2 *twi = ctx->tables.tw[li][MODE_CPU ? TW_INV_CPU : TW_INV_GPU];
+    dim3 grid(p.n), block(X::THREADS);
+    if (from_image) {
+        if (peak)
+            hipLaunchKernelGGL((xcorr_pass_kernel<LOGM, MODE_CPU, true, true>), grid, block, 0,
+                               ctx->stream, p, twf, twi);
+        else
+            hipLaunchKernelGGL((xcorr_pass_kernel<LOGM, MODE_CPU, true, false>), grid, block, 0,
+                               ctx->stream, p, twf, twi);
+    } else {
+        if (peak)
+            hipLaunchKernelGGL((xcorr_pass_kernel<LOGM, MODE_CPU, false, true>), grid, block, 0,
+                               ctx->stream, p, twf, twi);
+        else
+            hipLaunchKernelGGL((xcorr_pass_kernel<LOGM, MODE_CPU, false, false>), grid, block, 0,
+                               ctx->stream, p, twf, twi);
     }
 }
 
 template <int LOGM>
-static int launch_xcorr_m(Context *ctx, int n, const uint8_t *tA, const uint8_t *tB,
-                          const float *maxA, const float *maxB, int mode, float *planes)
+static void launch_pass_m(Context *ctx, const PassParams &p, bool from_image, bool peak)
 {
-    using X = XcShape<LOGM>;
-    const int li = LOGM - 5;
-    dim3 grid(n), block(X::THREADS);
-    if (mode == QLP_NORM_A_MAX) {
-        hipLaunchKernelGGL((xcorr_exact_kernel<LOGM, true>), grid, block, 0, ctx->stream, n, tA,
-                           tB, maxA, maxB, ctx->tables.tw[li][TW_FWD],
-                           ctx->tables.tw[li][TW_INV_CPU], planes);
-    } else {
-        hipLaunchKernelGGL((xcorr_exact_kernel<LOGM, false>), grid, block, 0, ctx->stream, n, tA,
-                           tB, maxA, maxB, ctx->tables.tw[li][TW_FWD],
-                           ctx->tables.tw[li][TW_INV_GPU], planes);
+    if (p.mode == QLP_NORM_A_MAX)
+        launch_pass_mc<LOGM, true>(ctx, p, from_image, peak);
+    else
+        launch_pass_mc<LOGM, false>(ctx, p, from_image, peak);
+}
+
+int launch_pass(Context *ctx, const PassParams &p)
+{
+    if (p.n == 0) return QLP_OK;
+    const bool from_image = p.imgA != nullptr;
+    const bool peak = p.peaks != nullptr || p.min_floor != nullptr || p.nbr != nullptr;
+    if (p.nbr && (p.nbr_half < 0 || p.nbr_half > 7))
+        return fail(QLP_ERR_ARG, "pass: neighbourhood half-width must be 0..7");
+    switch (p.N) {
+    case 16: launch_pass_m<5>(ctx, p, from_image, peak); break;
+    case 32: launch_pass_m<6>(ctx, p, from_image, peak); break;
+    case 64: launch_pass_m<7>(ctx, p, from_image, peak); break;
+    default: return fail(QLP_ERR_UNSUPPORTED, "xcorr: IA size N must be 16, 32 or 64");
     }
-    return check_hip(hipGetLastError(), "xcorr kernel launch");
+    return check_hip(hipGetLastError(), "xcorr pass launch");
 }
 
 int launch_xcorr(Context *ctx, int n, int N, const uint8_t *tA, const uint8_t *tB,
-                 const float *maxA, const float *maxB, int mode, float *planes, int peak_kind,
-                 float *peaks_out)
+                 const float *maxA, const float *maxB, int mode, float *planes)
 {
-    (void)peak_kind;
-    (void)peaks_out;
-    if (n == 0) return QLP_OK;
-    switch (N) {
-    case 16: return launch_xcorr_m<5>(ctx, n, tA, tB, maxA, maxB, mode, planes);
-    case 32: return launch_xcorr_m<6>(ctx, n, tA, tB, maxA, maxB, mode, planes);
-    case 64: return launch_xcorr_m<7>(ctx, n, tA, tB, maxA, maxB, mode, planes);
-    default: return fail(QLP_ERR_UNSUPPORTED, "xcorr: IA size N must be 16, 32 or 64");
-    }
+    PassParams p;
+    p.n = n;
+    p.N = N;
+    p.mode = mode;
+    p.tA = tA;
+    p.tB = tB;
+    p.maxA = maxA;
+    p.maxB = maxB;
+    p.planes = planes;
+    return launch_pass(ctx, p);
 }
 
 }  // namespace qlpiv

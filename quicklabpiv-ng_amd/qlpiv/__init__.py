@@ -157,6 +157,27 @@ class Context:
                                           _ptr(tiles), _ptr(mx)))
         return tiles, mx
 
+    def xcorr_images(self, imgA, imgB, top, left, N, du=None, dv=None, warp_mode=1,
+                     mode=QLP_NORM_PER_TILE, planes=True, nbr_half=0):
+        """Fused pass straight from device images: returns (planes or None, peaks
+        n x 3, min_floor n, nbr or None)."""
+        import torch
+
+        n = int(top.shape[0])
+        H, W = int(imgA.shape[-2]), int(imgA.shape[-1])
+        D = 2 * N - 1
+        dev = imgA.device
+        pl = torch.empty((n, D, D), dtype=torch.float32, device=dev) if planes else None
+        pk = torch.empty((n, 3), dtype=torch.float32, device=dev)
+        mf = torch.empty(n, dtype=torch.float32, device=dev)
+        nb = (torch.empty((n, 2, 2 * nbr_half + 1), dtype=torch.float32, device=dev)
+              if nbr_half else None)
+        check(self.lib.qlp_xcorr_images_dev(self.h, n, N, int(mode), _ptr(imgA), _ptr(imgB), H, W,
+                                            _ptr(top), _ptr(left), _ptr(du), _ptr(dv),
+                                            int(warp_mode), _ptr(pl), _ptr(pk), _ptr(mf), _ptr(nb),
+                                            int(nbr_half)))
+        return pl, pk, mf, nb
+
     def warp_micro_tiles(self, img, top, left, u, v, N):
         import torch
 
@@ -233,3 +254,102 @@ class CrossCorrelationJob:
     def dispose(self):
         self.inputs.clear()
         self.results.clear()
+
+
+class PivPlan:
+    """Multi-pass PIV over device-resident frame pairs (qlp_piv_plan): the
+    adaptive per-level loop of jobs/LocalPIVOpenCLGpuJob.call :188-409 with
+    device-resident displacement state.
+
+    config keys: ia_start, ia_end, overlap_factor (1 - overlap), margins
+    (top, bottom, left, right), norm_mode, warp_mode (QLP_WARP_NONE/MICRO),
+    subpix_kind (QLP_SUBPIX_* or -1), subpix_pixels, subpix_iterations,
+    interpolator_start_step, keep_planes."""
+
+    def __init__(self, ctx: Context, H: int, W: int, pairs: int = 1, **config):
+        cfg = _lib.PivConfig()
+        cfg.ia_start = config.get("ia_start", 64)
+        cfg.ia_end = config.get("ia_end", 16)
+        cfg.overlap_factor = config.get("overlap_factor", 0.5)
+        m = config.get("margins", (0, 0, 0, 0))
+        cfg.margin_top, cfg.margin_bottom, cfg.margin_left, cfg.margin_right = m
+        cfg.norm_mode = config.get("norm_mode", QLP_NORM_PER_TILE)
+        cfg.warp_mode = config.get("warp_mode", _lib.QLP_WARP_MICRO)
+        cfg.subpix_kind = config.get("subpix_kind", QLP_SUBPIX_GUO)
+        cfg.subpix_pixels = config.get("subpix_pixels", 3)
+        cfg.subpix_iterations = config.get("subpix_iterations", 20)
+        cfg.interpolator_start_step = config.get("interpolator_start_step", 0)
+        cfg.keep_planes = int(config.get("keep_planes", False))
+        self.ctx, self.H, self.W, self.pairs, self.cfg = ctx, H, W, pairs, cfg
+        h = ctypes.c_void_p()
+        check(ctx.lib.qlp_piv_plan_create(ctx.h, H, W, pairs, ctypes.byref(cfg), ctypes.byref(h)))
+        self.h = h
+        nl = ctypes.c_int()
+        check(ctx.lib.qlp_piv_plan_levels(h, ctypes.byref(nl)))
+        self.levels = []
+        for lv in range(nl.value):
+            ia, nI, nJ = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+            check(ctx.lib.qlp_piv_plan_level(h, lv, ctypes.byref(ia), ctypes.byref(nI),
+                                             ctypes.byref(nJ), None, None))
+            top = np.empty(nI.value * nJ.value, np.int32)
+            left = np.empty_like(top)
+            check(ctx.lib.qlp_piv_plan_level(h, lv, None, None, None, _ptr(top), _ptr(left)))
+            self.levels.append(dict(ia=ia.value, nI=nI.value, nJ=nJ.value,
+                                    top=top.reshape(nI.value, nJ.value),
+                                    left=left.reshape(nI.value, nJ.value)))
+
+    def close(self):
+        if self.h:
+            self.ctx.lib.qlp_piv_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, imgsA, imgsB):
+        """Enqueue all levels (device tensors pairs x H x W uint8)."""
+        check(self.ctx.lib.qlp_piv_run_dev(self.h, _ptr(imgsA), _ptr(imgsB)))
+
+    def result(self, level=-1):
+        """Host copies (after synchronising) of a level's u, v (pairs x nI x nJ)
+        and peak records (pairs x nI x nJ x 4)."""
+        import torch
+
+        lv = level % len(self.levels)
+        L = self.levels[lv]
+        u, v, pk, pl = (ctypes.c_void_p() for _ in range(4))
+        self.ctx.synchronize()
+        check(self.ctx.lib.qlp_piv_result_dev(self.h, lv, ctypes.byref(u), ctypes.byref(v),
+                                              ctypes.byref(pk), ctypes.byref(pl)))
+        n = L["nI"] * L["nJ"] * self.pairs
+
+        def fetch(ptr, count):
+            out = np.empty(count, np.float32)
+            _hip_copy_to_host(out, ptr.value)
+            return out
+
+        del torch
+        shp = (self.pairs, L["nI"], L["nJ"])
+        return (fetch(u, n).reshape(shp), fetch(v, n).reshape(shp),
+                fetch(pk, 4 * n).reshape(shp + (4,)))
+
+
+_hip_rt = None
+
+
+def _hip_copy_to_host(out: np.ndarray, src: int):
+    """Device-to-host copy with the process's HIP runtime (the one torch loaded)."""
+    global _hip_rt
+    if _hip_rt is None:
+        import torch  # noqa: F401  (loads libamdhip64.so.7 first)
+
+        rt = ctypes.CDLL("libamdhip64.so.7")
+        rt.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        rt.hipMemcpy.restype = ctypes.c_int
+        _hip_rt = rt
+    rc = _hip_rt.hipMemcpy(out.ctypes.data, ctypes.c_void_p(src), out.nbytes, 2)  # D2H
+    if rc != 0:
+        raise QlpError(_lib.QLP_ERR_DEVICE, f"hipMemcpy failed ({rc})")

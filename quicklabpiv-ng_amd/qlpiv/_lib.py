@@ -46,7 +46,32 @@ SIGNATURES = {
     "qlp_subpixel_dev": (_i, [_vp, _i, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp]),
     "qlp_clip_tiles_dev": (_i, [_vp, _vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp]),
     "qlp_warp_micro_tiles_dev": (_i, [_vp, _vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "qlp_xcorr_images_dev": (_i, [_vp, _i, _i, _i, _vp, _vp, _i, _i, _vp, _vp, _vp, _vp, _i,
+                                  _vp, _vp, _vp, _vp, _i]),
+    "qlp_piv_plan_create": (_i, [_vp, _i, _i, _i, _vp, ctypes.POINTER(_vp)]),
+    "qlp_piv_plan_destroy": (_i, [_vp]),
+    "qlp_piv_plan_levels": (_i, [_vp, ctypes.POINTER(_i)]),
+    "qlp_piv_plan_level": (_i, [_vp, _i, ctypes.POINTER(_i), ctypes.POINTER(_i),
+                                ctypes.POINTER(_i), _vp, _vp]),
+    "qlp_piv_run_dev": (_i, [_vp, _vp, _vp]),
+    "qlp_piv_result_dev": (_i, [_vp, _i, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                ctypes.POINTER(_vp), ctypes.POINTER(_vp)]),
 }
+
+QLP_WARP_NONE = 0
+QLP_WARP_MICRO = 1
+
+
+class PivConfig(ctypes.Structure):
+    """qlp_piv_config (include/qlpiv.h)."""
+    _fields_ = [("ia_start", ctypes.c_int), ("ia_end", ctypes.c_int),
+                ("overlap_factor", ctypes.c_float),
+                ("margin_top", ctypes.c_int), ("margin_bottom", ctypes.c_int),
+                ("margin_left", ctypes.c_int), ("margin_right", ctypes.c_int),
+                ("norm_mode", ctypes.c_int), ("warp_mode", ctypes.c_int),
+                ("subpix_kind", ctypes.c_int), ("subpix_pixels", ctypes.c_int),
+                ("subpix_iterations", ctypes.c_int), ("interpolator_start_step", ctypes.c_int),
+                ("keep_planes", ctypes.c_int)]
 
 _lib = None
 
