@@ -184,7 +184,8 @@ typedef struct qlp_piv_config {
     int subpix_kind;          /* QLP_SUBPIX_* or -1 for none */
     int subpix_pixels, subpix_iterations;
     int interpolator_start_step; /* first level that is interpolated */
-    int keep_planes;          /* keep the correlation planes of the last level */
+    int keep_planes;          /* 0: no planes; 1: store the last level's planes; 2: store
+                                 every level's planes (what the Java job returns) */
 } qlp_piv_config;
 
 typedef struct qlp_piv_plan qlp_piv_plan;
@@ -204,9 +205,16 @@ int qlp_piv_plan_level(qlp_piv_plan *plan, int level, int *ia, int *nI, int *nJ,
 int qlp_piv_run_dev(qlp_piv_plan *plan, const uint8_t *imgsA, const uint8_t *imgsB);
 /* Device pointers of a level's results (valid after the run has completed):
  * u, v (pairs x n displacements, vertical then horizontal), peaks (pairs x n x 4:
- * i', j', value', status), planes (last level only, when kept; else NULL). */
+ * i', j', value', status), planes (pairs x n x D x D when stored for that
+ * level; else NULL). */
 int qlp_piv_result_dev(qlp_piv_plan *plan, int level, const float **u, const float **v,
                        const float **peaks, const float **planes);
+/* Per-level device timing of later runs (HIP events on the context's stream):
+ * the displacement-state kernel plus the correlation pass, and the sub-pixel
+ * fit with the displacement update, in milliseconds, of the last run.  Plays
+ * the role of the reference's per-job wall-clock logging (ExecutionStatus). */
+int qlp_piv_plan_set_timing(qlp_piv_plan *plan, int enable);
+int qlp_piv_plan_timing(qlp_piv_plan *plan, int level, float *state_and_pass_ms, float *fit_ms);
 
 #ifdef __cplusplus
 }

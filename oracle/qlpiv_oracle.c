@@ -1170,3 +1170,36 @@ void or_cpu_path_batch(int n, int N, const uint8_t *tA, const uint8_t *tB, const
     free(B);
     free(plane);
 }
+
+/* One adaptive level of the reference CPU pipeline per IA, for the timed CPU
+ * baseline of the multi-pass workload: clip tile A, micro-warp tile B by the
+ * inherited displacement (SecondImage), correlate with the Java CPU job
+ * (A-max normalisation), FromCenter, Guo 3 px / 20 iterations.
+ * out: n x 4 (i', j', value', status). */
+void or_cpu_level_batch(const uint8_t *imgA, const uint8_t *imgB, int H, int W, int n, int N,
+                        const int *top, const int *left, const float *u, const float *v,
+                        float *out)
+{
+    int D = 2 * N - 1;
+    uint8_t *ta = (uint8_t *)malloc((size_t)N * N), *tb = (uint8_t *)malloc((size_t)N * N);
+    float *A = (float *)malloc(sizeof(float) * N * N);
+    float *B = (float *)malloc(sizeof(float) * N * N);
+    float *plane = (float *)malloc(sizeof(float) * D * D);
+    for (int t = 0; t < n; t++) {
+        float mA = or_clip_tile(imgA, H, W, top[t], left[t], N, ta);
+        or_warp_micro(imgB, H, W, top[t], left[t], N, u[t], v[t], tb);
+        for (int q = 0; q < N * N; q++) {
+            A[q] = (float)ta[q];
+            B[q] = (float)tb[q];
+        }
+        or_xcorr_javacpu(A, B, mA, N, plane);
+        float pk[4];
+        or_find_max_from_center(plane, D, D, pk);
+        or_guo(plane, D, D, 3, 20, pk[3], pk, out + 4 * (size_t)t);
+    }
+    free(ta);
+    free(tb);
+    free(A);
+    free(B);
+    free(plane);
+}

@@ -115,6 +115,10 @@ struct qlp_piv_plan {
     qlp_piv_config cfg{};
     std::vector<Level> levels;
     size_t bytes = 0;
+    // optional per-level timing: events before the state kernel, after the
+    // correlation pass and after the fit of every level
+    bool timing = false;
+    std::vector<hipEvent_t> ev;
 };
 
 static void plan_free(qlp_piv_plan *p)
@@ -124,6 +128,8 @@ static void plan_free(qlp_piv_plan *p)
         for (void *b : bufs)
             if (b) (void)hipFree(b);
     }
+    for (hipEvent_t e : p->ev) (void)hipEventDestroy(e);
+    p->ev.clear();
     if (p->ctx) p->ctx->hbm_in_use -= p->bytes;
     p->levels.clear();
 }
@@ -174,7 +180,8 @@ int qlp_piv_plan_create(qlp_ctx *h, int H, int W, int pairs, const qlp_piv_confi
         }
         const size_t n = L.n, pn = (size_t)pairs * n, D = 2 * tile - 1;
         need += n * 8 + pn * (8 + 12 + 4 + 4 * 2 * NBR_L + 16);
-        if (cfg->keep_planes && l == nlev - 1) need += pn * D * D * 4;
+        if (cfg->keep_planes >= 2 || (cfg->keep_planes == 1 && l == nlev - 1))
+            need += pn * D * D * 4;
         p->levels.push_back(std::move(L));
     }
     if (ctx->hbm_budget && ctx->hbm_in_use + need > ctx->hbm_budget) {
@@ -193,7 +200,7 @@ int qlp_piv_plan_create(qlp_ctx *h, int H, int W, int pairs, const qlp_piv_confi
         if (!rc) rc = check_hip(hipMalloc(&L.mf, pn * 4), "hipMalloc plan");
         if (!rc) rc = check_hip(hipMalloc(&L.nbr, pn * 4 * 2 * NBR_L), "hipMalloc plan");
         if (!rc) rc = check_hip(hipMalloc(&L.sub, pn * 16), "hipMalloc plan");
-        if (!rc && cfg->keep_planes && l + 1 == p->levels.size())
+        if (!rc && (cfg->keep_planes >= 2 || (cfg->keep_planes == 1 && l + 1 == p->levels.size())))
             rc = check_hip(hipMalloc(&L.planes, pn * D * D * 4), "hipMalloc plan");
         if (!rc)
             rc = check_hip(hipMemcpy(L.top, L.top_h.data(), n * 4, hipMemcpyHostToDevice), "upload grid");
@@ -251,6 +258,7 @@ int qlp_piv_run_dev(qlp_piv_plan *p, const uint8_t *imgsA, const uint8_t *imgsB)
         Level &L = p->levels[l];
         const int pn = p->pairs * L.n;
         const int D = 2 * L.N - 1;
+        if (p->timing) (void)hipEventRecord(p->ev[3 * l], ctx->stream);
         if (l == 0) {
             const size_t blocks = ((size_t)pn + 255) / 256;
             hipLaunchKernelGGL(fill_kernel, dim3(blocks), dim3(256), 0, ctx->stream, L.u, 0.0f, (size_t)pn);
@@ -284,11 +292,43 @@ int qlp_piv_run_dev(qlp_piv_plan *p, const uint8_t *imgsA, const uint8_t *imgsB)
         q.nbr_half = NBR_HALF;
         rc = launch_pass(ctx, q);
         if (rc) return rc;
+        if (p->timing) (void)hipEventRecord(p->ev[3 * l + 1], ctx->stream);
         const int kind = ((int)l >= c.interpolator_start_step) ? c.subpix_kind : -1;
         rc = launch_subpixel_nbr(ctx, kind, c.subpix_pixels, c.subpix_iterations, pn, D, NBR_HALF,
                                  L.nbr, L.mf, L.peaks, L.sub, L.u, L.v);
         if (rc) return rc;
+        if (p->timing) (void)hipEventRecord(p->ev[3 * l + 2], ctx->stream);
     }
+    return QLP_OK;
+}
+
+int qlp_piv_plan_set_timing(qlp_piv_plan *p, int enable)
+{
+    if (!p) return fail(QLP_ERR_ARG, "null plan");
+    if (enable && p->ev.empty()) {
+        p->ev.resize(3 * p->levels.size());
+        for (auto &e : p->ev) {
+            int rc = check_hip(hipEventCreate(&e), "hipEventCreate");
+            if (rc) return rc;
+        }
+    }
+    p->timing = enable != 0;
+    return QLP_OK;
+}
+
+int qlp_piv_plan_timing(qlp_piv_plan *p, int level, float *state_and_pass_ms, float *fit_ms)
+{
+    if (!p) return fail(QLP_ERR_ARG, "null plan");
+    if (!p->timing) return fail(QLP_ERR_ARG, "timing is not enabled on this plan");
+    if (level < 0 || level >= (int)p->levels.size()) return fail(QLP_ERR_ARG, "level out of range");
+    int rc = check_hip(hipEventSynchronize(p->ev[3 * level + 2]), "hipEventSynchronize");
+    if (rc) return rc;
+    float a = 0.0f, b = 0.0f;
+    rc = check_hip(hipEventElapsedTime(&a, p->ev[3 * level], p->ev[3 * level + 1]), "hipEventElapsedTime");
+    if (!rc) rc = check_hip(hipEventElapsedTime(&b, p->ev[3 * level + 1], p->ev[3 * level + 2]), "hipEventElapsedTime");
+    if (rc) return rc;
+    if (state_and_pass_ms) *state_and_pass_ms = a;
+    if (fit_ms) *fit_ms = b;
     return QLP_OK;
 }
 

@@ -313,6 +313,18 @@ class PivPlan:
         """Enqueue all levels (device tensors pairs x H x W uint8)."""
         check(self.ctx.lib.qlp_piv_run_dev(self.h, _ptr(imgsA), _ptr(imgsB)))
 
+    def set_timing(self, enable=True):
+        check(self.ctx.lib.qlp_piv_plan_set_timing(self.h, int(enable)))
+
+    def timing(self):
+        """[(pass_ms, fit_ms)] per level of the last run (waits for it)."""
+        out = []
+        for lv in range(len(self.levels)):
+            a, b = ctypes.c_float(), ctypes.c_float()
+            check(self.ctx.lib.qlp_piv_plan_timing(self.h, lv, ctypes.byref(a), ctypes.byref(b)))
+            out.append((a.value, b.value))
+        return out
+
     def result(self, level=-1):
         """Host copies (after synchronising) of a level's u, v (pairs x nI x nJ)
         and peak records (pairs x nI x nJ x 4)."""

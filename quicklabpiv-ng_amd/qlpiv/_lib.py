@@ -56,6 +56,9 @@ SIGNATURES = {
     "qlp_piv_run_dev": (_i, [_vp, _vp, _vp]),
     "qlp_piv_result_dev": (_i, [_vp, _i, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
                                 ctypes.POINTER(_vp), ctypes.POINTER(_vp)]),
+    "qlp_piv_plan_set_timing": (_i, [_vp, _i]),
+    "qlp_piv_plan_timing": (_i, [_vp, _i, ctypes.POINTER(ctypes.c_float),
+                                 ctypes.POINTER(ctypes.c_float)]),
 }
 
 QLP_WARP_NONE = 0
