@@ -95,6 +95,8 @@ int qlp_ctx_create(int hip_device, size_t hbm_budget_bytes, qlp_ctx **out)
     Context *ctx = &h->c;
     ctx->device = hip_device;
     ctx->hbm_budget = hbm_budget_bytes;
+    const char *form = getenv("QLPIV_XCORR_FORM");
+    ctx->xcorr_form = (form && strcmp(form, "lds") == 0) ? FORM_LDS : FORM_LANE;
     rc = check_hip(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "hipStreamCreate");
     if (rc) {
         delete h;

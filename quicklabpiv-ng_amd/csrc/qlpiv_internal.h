@@ -32,8 +32,15 @@ struct DeviceTables {
     float2 *tw[3][3];  // [log2M - 5][kind] -> M/2 entries
 };
 
+// Two implementations of the same correlation arithmetic: one lane per line of
+// the transform (default) and the earlier LDS-exchange form (8/16 elements per
+// lane, two LDS round trips per 1D transform).  QLPIV_XCORR_FORM=lds selects the
+// latter when a context is created.
+enum XcorrForm { FORM_LANE = 0, FORM_LDS = 1 };
+
 struct Context {
     int device = 0;
+    int xcorr_form = FORM_LANE;
     size_t hbm_budget = 0;
     size_t hbm_in_use = 0;
     hipStream_t stream = nullptr;
@@ -82,6 +89,7 @@ struct PassParams {
 int launch_xcorr(Context *ctx, int n, int N, const uint8_t *tA, const uint8_t *tB,
                  const float *maxA, const float *maxB, int mode, float *planes);
 int launch_pass(Context *ctx, const PassParams &p);
+int launch_pass_lane(Context *ctx, const PassParams &p, bool from_image, bool peak);
 int launch_find_peaks(Context *ctx, int kind, int n, int D, const float *planes, int n_peaks,
                       int kernel_width, float *peaks, float *min_floor);
 int launch_subpixel(Context *ctx, int kind, int pixels, int iterations, int n, int D, int n_peaks,

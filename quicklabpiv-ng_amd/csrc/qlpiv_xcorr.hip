@@ -449,6 +449,7 @@ int launch_pass(Context *ctx, const PassParams &p)
     const bool peak = p.peaks != nullptr || p.min_floor != nullptr || p.nbr != nullptr;
     if (p.nbr && (p.nbr_half < 0 || p.nbr_half > 7))
         return fail(QLP_ERR_ARG, "pass: neighbourhood half-width must be 0..7");
+    if (ctx->xcorr_form == FORM_LANE) return launch_pass_lane(ctx, p, from_image, peak);
     switch (p.N) {
     case 16: launch_pass_m<5>(ctx, p, from_image, peak); break;
     case 32: launch_pass_m<6>(ctx, p, from_image, peak); break;
