@@ -123,8 +123,18 @@ __device__ __forceinline__ void fft_lines(float *__restrict__ re, float *__restr
 #pragma unroll
                 for (int q = 0; q < E; q++) {
                     if (q & L) continue;
-                    const float2 w = tw[(M >> (s + 1)) * (q & (L - 1))];
-                    bfly(a[q], b[q], a[q + L], b[q + L], w);
+                    const int k = (M >> (s + 1)) * (q & (L - 1));
+                    if (k == 0) {
+                        // c = 1, s = -0/+0 in all three tables: a plain add/sub, same values
+                        const float lr = a[q] + a[q + L], li = b[q] + b[q + L];
+                        const float ur = a[q] - a[q + L], ui = b[q] - b[q + L];
+                        a[q] = lr;
+                        b[q] = li;
+                        a[q + L] = ur;
+                        b[q + L] = ui;
+                    } else {
+                        bfly(a[q], b[q], a[q + L], b[q + L], tw[k]);
+                    }
                 }
             }
 #pragma unroll
@@ -449,7 +459,11 @@ int launch_pass(Context *ctx, const PassParams &p)
     const bool peak = p.peaks != nullptr || p.min_floor != nullptr || p.nbr != nullptr;
     if (p.nbr && (p.nbr_half < 0 || p.nbr_half > 7))
         return fail(QLP_ERR_ARG, "pass: neighbourhood half-width must be 0..7");
-    if (ctx->xcorr_form == FORM_LANE) return launch_pass_lane(ctx, p, from_image, peak);
+    // measured on MI355X (C2 multipass): the lane-per-line form is faster for
+    // N = 16 (0.293 vs 0.323 ms for 65,025 IAs), the LDS-exchange form for
+    // N = 32 (0.381 vs 0.436 ms for 16,129 IAs) and N = 64, where more
+    // wavefronts per IA hide latency better
+    if (ctx->xcorr_form == FORM_LANE && p.N == 16) return launch_pass_lane(ctx, p, from_image, peak);
     switch (p.N) {
     case 16: launch_pass_m<5>(ctx, p, from_image, peak); break;
     case 32: launch_pass_m<6>(ctx, p, from_image, peak); break;

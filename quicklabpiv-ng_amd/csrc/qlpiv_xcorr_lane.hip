@@ -31,7 +31,7 @@ struct Lx {
     static constexpr int N = M / 2;
     static constexpr int D = M - 1;
     static constexpr int S = M + 1;                   // LDS row stride (float2)
-    static constexpr int LPL = (LOGM == 7) ? 2 : 1;   // lanes per line
+    static constexpr int LPL = (LOGM >= 6) ? 2 : 1;   // lanes per line
     static constexpr int E = M / LPL;                 // elements per lane
     static constexpr int LOGE = LOGM - (LPL == 2 ? 1 : 0);
     static constexpr int IAS = (LOGM == 5) ? 2 : 1;   // IAs per work-group
