@@ -167,14 +167,76 @@ int qlp_xcorr_images_dev(qlp_ctx *ctx, int n, int N, int mode, const uint8_t *im
                          const float *du, const float *dv, int warp_mode, float *planes,
                          float *peaks, float *min_floor, float *nbr, int nbr_half);
 
+/* ---- vector validation and replacement ------------------------------------ */
+
+#define QLP_VALIDATOR_MEDIAN 0     /* iareas/validation/NormalizedMedianValidator.java */
+#define QLP_VALIDATOR_DIFFERENCE 1 /* iareas/validation/DifferenceValidator.java */
+#define QLP_REPLACE_NONE 0         /* no replacement (VectorReplacementFactoryEnum.None) */
+#define QLP_REPLACE_SECONDARY_PEAK 1 /* iareas/replacement/SecondaryPeakReplacement.java */
+#define QLP_REPLACE_BILINEAR 2     /* iareas/replacement/BilinearReplacement.java */
+#define QLP_VALID_MAX_STAGES 8
+/* Upper bound on validation sweeps when iterating until no more corrections
+ * (the reference has none: a field whose corrections oscillate would not end). */
+#define QLP_VALID_MAX_SWEEPS 256
+
+/* One CombinedValidatorAndReplacementConfiguration
+ * (iareas/validation/CombinedValidatorAndReplacementConfiguration.java:12-42). */
+typedef struct qlp_valid_stage {
+    int validator;    /* QLP_VALIDATOR_* */
+    float threshold;  /* distance threshold, pixels */
+    float epsilon0;   /* normalised median only */
+    int replacement;  /* QLP_REPLACE_* */
+    int peak_index;   /* QLP_REPLACE_SECONDARY_PEAK: peak used (1 = first secondary) */
+} qlp_valid_stage;
+
+/* VectorValidatorConfiguration (jobs/validator/VectorValidatorConfiguration.java)
+ * plus the stage array of the CombinedValidator; n_stages = 0 turns validation
+ * off.  The multi-peak normalised median model of the reference UI
+ * (PIVConfigurationConverterVisitor.visit(...MultiPeakNormalizedMedianModel)
+ * :607-641) is P stages: median + secondary peak 1 .. P-1, then median +
+ * bilinear. */
+typedef struct qlp_valid_config {
+    int n_stages;
+    int max_iterations;  /* maxCorrectionIterations */
+    int until_no_more;   /* iterateUntilNoMoreCorrections */
+    qlp_valid_stage stage[QLP_VALID_MAX_STAGES];
+} qlp_valid_config;
+
+/* VectorValidatorJob.compute (jobs/validator/VectorValidatorJob.java:243-405):
+ * validate, replace and re-validate the displacement field of 'pairs' grids of
+ * nI x nJ tiles in the reference's centre-out spiral visit order, then set the
+ * vectors still invalid to (0, 0).
+ *   top, left  : host arrays of nI*nJ tile origins (Tile.getTopPixel/getLeftPixel)
+ *   u, v       : device, pairs x nI*nJ accumulated displacements (in/out)
+ *   peak_uv    : device, pairs x nI*nJ x n_peaks x 2 displacement increments
+ *                (MaxCrossResult.getNthDisplacementU/V)
+ *   peak_count : device, pairs x nI*nJ (MaxCrossResult.getTotalPeaks) or NULL
+ *                (every tile holds n_peaks peaks)
+ *   invalid    : device, pairs x nI*nJ bytes, Tile.isInvalidDisplacement()
+ *                after the job (may be NULL)
+ *   stats      : device, pairs x 4 ints: sweeps, corrected vectors, vectors
+ *                left invalid, flags (1: sweep limit reached, 2: a bilinear
+ *                replacement met a grid the reference rejects with
+ *                VectorReplacementException) (may be NULL) */
+int qlp_validate_vectors_dev(qlp_ctx *ctx, int pairs, int nI, int nJ, const int *top,
+                             const int *left, float *u, float *v, int n_peaks,
+                             const float *peak_uv, const int *peak_count,
+                             const qlp_valid_config *cfg, uint8_t *invalid, int *stats);
+/* The same with every array in host memory (synchronous). */
+int qlp_validate_vectors(qlp_ctx *ctx, int pairs, int nI, int nJ, const int *top,
+                         const int *left, float *u, float *v, int n_peaks, const float *peak_uv,
+                         const int *peak_count, const qlp_valid_config *cfg, uint8_t *invalid,
+                         int *stats);
+
 /* ---- multi-pass PIV over device-resident frame pairs ---------------------- */
 
 /* The per-frame adaptive loop of jobs/LocalPIVOpenCLGpuJob.call :188-409 with
  * the super-position IA strategy (iareas/AdaptiveInterAreaStrategySuperPosition
  * .java:31-197), Direct velocity inheritance
  * (AdaptiveInterVelocityInheritanceStrategyDirect.java), the chosen warping,
- * FromCenter peak search and sub-pixel fit, and the displacement update
- * (IterationStepTiles.updateDisplacementsFromMaxCrossResults :317-371). */
+ * peak search and sub-pixel fit, the displacement update
+ * (IterationStepTiles.updateDisplacementsFromMaxCrossResults :317-371) and the
+ * optional vector validation (:395-403). */
 typedef struct qlp_piv_config {
     int ia_start, ia_end;     /* IA size of the first and of the last level (halving) */
     float overlap_factor;     /* 1 - overlap: 0.5 for 50 % overlap */
@@ -186,6 +248,11 @@ typedef struct qlp_piv_config {
     int interpolator_start_step; /* first level that is interpolated */
     int keep_planes;          /* 0: no planes; 1: store the last level's planes; 2: store
                                  every level's planes (what the Java job returns) */
+    int peak_kind;            /* QLP_PEAK_FROM_CENTER (fused into the pass) or
+                                 QLP_PEAK_MULTI (planes are then stored at every level) */
+    int n_peaks;              /* MultiPeaks numberOfPeaks (1 for FromCenter) */
+    int peak_kernel_width;    /* MultiPeaks kernelWidth */
+    qlp_valid_config valid;   /* vector validation after every level (n_stages 0: off) */
 } qlp_piv_config;
 
 typedef struct qlp_piv_plan qlp_piv_plan;
@@ -204,17 +271,24 @@ int qlp_piv_plan_level(qlp_piv_plan *plan, int level, int *ia, int *nI, int *nJ,
  * asynchronously on the context's stream. */
 int qlp_piv_run_dev(qlp_piv_plan *plan, const uint8_t *imgsA, const uint8_t *imgsB);
 /* Device pointers of a level's results (valid after the run has completed):
- * u, v (pairs x n displacements, vertical then horizontal), peaks (pairs x n x 4:
- * i', j', value', status), planes (pairs x n x D x D when stored for that
- * level; else NULL). */
+ * u, v (pairs x n displacements, vertical then horizontal), peaks (pairs x n x
+ * n_peaks x 4: i', j', value', status), planes (pairs x n x D x D when stored
+ * for that level; else NULL). */
 int qlp_piv_result_dev(qlp_piv_plan *plan, int level, const float **u, const float **v,
                        const float **peaks, const float **planes);
+/* Device pointers of a level's validation outcome (NULL when validation is
+ * off): invalid (pairs x n bytes) and stats (pairs x 4 ints), as written by
+ * qlp_validate_vectors_dev. */
+int qlp_piv_validation_dev(qlp_piv_plan *plan, int level, const uint8_t **invalid,
+                           const int **stats);
 /* Per-level device timing of later runs (HIP events on the context's stream):
- * the displacement-state kernel plus the correlation pass, and the sub-pixel
- * fit with the displacement update, in milliseconds, of the last run.  Plays
- * the role of the reference's per-job wall-clock logging (ExecutionStatus). */
+ * the displacement-state kernel plus the correlation pass; the peak search (when
+ * not fused), sub-pixel fit and displacement update; and the vector validation,
+ * in milliseconds, of the last run.  Plays the role of the reference's per-job
+ * wall-clock logging (ExecutionStatus). */
 int qlp_piv_plan_set_timing(qlp_piv_plan *plan, int enable);
-int qlp_piv_plan_timing(qlp_piv_plan *plan, int level, float *state_and_pass_ms, float *fit_ms);
+int qlp_piv_plan_timing(qlp_piv_plan *plan, int level, float *state_and_pass_ms, float *fit_ms,
+                        float *valid_ms);
 
 #ifdef __cplusplus
 }

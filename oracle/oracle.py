@@ -57,8 +57,56 @@ def lib():
         L.or_clip_tile.argtypes = [_u8p, c_int, c_int, c_int, c_int, c_int, _u8p]
         L.or_clip_tile.restype = c_float
         L.or_warp_micro.argtypes = [_u8p, c_int, c_int, c_int, c_int, c_int, c_float, c_float, _u8p]
+        L.or_spiral_order.argtypes = [c_int, c_int, _i32p]
+        L.or_spiral_order.restype = c_int
+        L.or_validate.argtypes = [c_int, c_int, _i32p, _i32p, _f32p, _f32p, c_int, _f32p,
+                                  ctypes.c_void_p, c_int, ctypes.c_void_p, c_int, c_int, c_int,
+                                  _u8p, _i32p]
         _lib = L
     return _lib
+
+
+class VStage(ctypes.Structure):
+    """or_vstage: one validator + replacement stage."""
+
+    _fields_ = [("validator", ctypes.c_int), ("threshold", ctypes.c_float),
+                ("epsilon0", ctypes.c_float), ("replacement", ctypes.c_int),
+                ("peak_index", ctypes.c_int)]
+
+
+def spiral_order(nI, nJ):
+    """Visit order of VectorValidatorJob.circularVisitor; None on a double visit."""
+    order = np.full(nI * nJ, -1, np.int32)
+    got = lib().or_spiral_order(nI, nJ, order)
+    return None if got != nI * nJ else order
+
+
+def validate(top, left, u, v, stages, peak_uv=None, peak_count=None, max_iterations=1,
+             until_no_more=False, max_sweeps=256):
+    """VectorValidatorJob.compute over one grid.
+
+    top, left: nI x nJ tile origins; u, v: nI x nJ displacements; stages: list
+    of (validator, threshold, epsilon0, replacement, peak_index); peak_uv:
+    nI x nJ x P x 2 increments.  Returns (u, v, invalid, stats)."""
+    top = np.ascontiguousarray(top, np.int32)
+    nI, nJ = top.shape
+    left = np.ascontiguousarray(left, np.int32)
+    u = np.array(u, np.float32, copy=True, order="C")
+    v = np.array(v, np.float32, copy=True, order="C")
+    if peak_uv is None:
+        peak_uv = np.zeros((nI, nJ, 1, 2), np.float32)
+    peak_uv = np.ascontiguousarray(peak_uv, np.float32)
+    P = peak_uv.shape[2]
+    cnt = None if peak_count is None else np.ascontiguousarray(peak_count, np.int32)
+    arr = (VStage * len(stages))(*[VStage(int(a), float(b), float(c), int(d), int(e))
+                                   for a, b, c, d, e in stages])
+    invalid = np.zeros((nI, nJ), np.uint8)
+    stats = np.zeros(4, np.int32)
+    lib().or_validate(nI, nJ, top, left, u, v, P, peak_uv,
+                      None if cnt is None else cnt.ctypes.data, len(stages),
+                      ctypes.cast(arr, ctypes.c_void_p), int(max_iterations), int(until_no_more),
+                      int(max_sweeps), invalid, stats)
+    return u, v, invalid, stats
 
 
 # --------------------------------------------------------------------------

@@ -1203,3 +1203,498 @@ void or_cpu_level_batch(const uint8_t *imgA, const uint8_t *imgB, int H, int W, 
     free(B);
     free(plane);
 }
+
+/* ------------------------------------------------------------------------ */
+/* Vector validation: jobs/validator/VectorValidatorJob.java,               */
+/* iareas/validation/{NormalizedMedian,Difference}Validator.java,           */
+/* iareas/replacement/{SecondaryPeak,Bilinear}Replacement.java, Tile.java   */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    int validator;   /* 0 normalised median, 1 difference */
+    float threshold;
+    float epsilon0;
+    int replacement; /* 0 none, 1 secondary peak, 2 bilinear */
+    int peak_index;
+} or_vstage;
+
+typedef struct {
+    int nI, nJ;
+    const int *top, *left;
+    float *u, *v, *bu, *bv;
+    unsigned char *inv, *locked;
+    int P;
+    const float *puv;  /* nI*nJ x P x 2 */
+    const int *pcount; /* or NULL */
+    int err;
+} or_vgrid;
+
+/* VectorValidatorJob.circularVisitor :105-196 + visitTileHelper :198-213.
+ * order[k] = row-major index of the k-th visited tile.  Returns the number of
+ * tiles visited, or -1 when a tile would be visited twice (the reference
+ * throws VectorValidatorException). */
+int or_spiral_order(int nI, int nJ, int *order)
+{
+    int total = nI * nJ;
+    int startI = nI / 2, startJ = nJ / 2;
+    int sideI = 2, sideJ = 2;
+    if (nI != startI * 2) sideI = 1; else startI--;
+    if (nJ != startJ * 2) sideJ = 1; else startJ--;
+    unsigned char *seen = (unsigned char *)calloc((size_t)total, 1);
+    int visited = 0, bad = 0;
+    int moves[4];
+#define OR_VISIT(ii, jj)                                                     \
+    do {                                                                     \
+        int vi_ = (ii), vj_ = (jj);                                          \
+        if (vi_ >= 0 && vi_ < nI && vj_ >= 0 && vj_ < nJ) {                  \
+            if (seen[vi_ * nJ + vj_]) bad = 1;                               \
+            seen[vi_ * nJ + vj_] = 1;                                        \
+            if (visited < total) order[visited] = vi_ * nJ + vj_;            \
+            visited++;                                                       \
+        }                                                                    \
+    } while (0)
+    while (visited < total && !bad) {
+        moves[0] = sideJ > 1 ? sideJ - 1 : 0;
+        moves[1] = sideI > 1 ? sideI - 1 : 0;
+        moves[2] = sideJ > 1 ? sideJ - 1 : 0;
+        moves[3] = sideI > 2 ? sideI - 2 : 0;
+        int j = startJ, i = startI;
+        OR_VISIT(i, j);
+        for (int o = 0; o < moves[0]; o++) { ++j; OR_VISIT(i, j); }
+        for (int o = 0; o < moves[1]; o++) { ++i; OR_VISIT(i, j); }
+        if (moves[1] > 0) {
+            for (int o = 0; o < moves[2]; o++) { --j; OR_VISIT(i, j); }
+            for (int o = 0; o < moves[3]; o++) { --i; OR_VISIT(i, j); }
+        }
+        sideI += 2;
+        sideJ += 2;
+        startJ--;
+        startI--;
+    }
+#undef OR_VISIT
+    free(seen);
+    return bad ? -1 : visited;
+}
+
+/* Tile.setInvalidDisplacement :204-209 */
+static void vt_set_invalid(or_vgrid *g, int t, int state)
+{
+    if (g->locked[t]) return;
+    g->inv[t] = (unsigned char)state;
+}
+
+/* VectorValidatorJob.getNeighbors :215-238: adj[a][b] = tile index or -1 */
+static void vt_neighbors(const or_vgrid *g, int i, int j, int adj[3][3])
+{
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) {
+            int ii = i + (a - 1), jj = j + (b - 1);
+            adj[a][b] = (ii < 0 || ii >= g->nI || jj < 0 || jj >= g->nJ) ? -1 : ii * g->nJ + jj;
+        }
+}
+
+/* distanceComparator of NormalizedMedianValidator :19-35 */
+static int vt_cmp(const or_vgrid *g, int t1, int t2)
+{
+    float u1 = g->u[t1], v1 = g->v[t1], u2 = g->u[t2], v2 = g->v[t2];
+    float n1 = u1 * u1 + v1 * v1;
+    float n2 = u2 * u2 + v2 * v2;
+    if (n1 > n2) return 1;
+    if (n1 < n2) return -1;
+    return 0;
+}
+
+/* java.util.TimSort (OpenJDK 8) for fewer than 32 elements:
+ * countRunAndMakeAscending, then binarySort (Collections.sort -> ArrayList.sort
+ * -> Arrays.sort(T[], lo, hi, Comparator)). */
+static void vt_timsort_small(const or_vgrid *g, int *a, int n)
+{
+    if (n < 2) return;
+    int runHi = 1;
+    if (vt_cmp(g, a[runHi++], a[0]) < 0) {
+        while (runHi < n && vt_cmp(g, a[runHi], a[runHi - 1]) < 0) runHi++;
+        for (int lo = 0, hi = runHi - 1; lo < hi; lo++, hi--) {
+            int t = a[lo]; a[lo] = a[hi]; a[hi] = t;
+        }
+    } else {
+        while (runHi < n && vt_cmp(g, a[runHi], a[runHi - 1]) >= 0) runHi++;
+    }
+    for (int start = runHi; start < n; start++) {
+        int pivot = a[start];
+        int left = 0, right = start;
+        while (left < right) {
+            int mid = (int)((unsigned)(left + right) >> 1);
+            if (vt_cmp(g, pivot, a[mid]) < 0) right = mid; else left = mid + 1;
+        }
+        memmove(a + left + 1, a + left, sizeof(int) * (size_t)(start - left));
+        a[left] = pivot;
+    }
+}
+
+/* Arrays.sort(float[]): NaN after every number */
+static int vt_fcmp(const void *x, const void *y)
+{
+    float a = *(const float *)x, b = *(const float *)y;
+    int na = isnan(a), nb = isnan(b);
+    if (na || nb) return na - nb;
+    return (a > b) - (a < b);
+}
+
+/* NormalizedMedianValidator.validateVector :63-136 */
+static void vt_validate_median(or_vgrid *g, int t, int adj[3][3], float threshold, float epsilon0)
+{
+    int sorted[8], cnt = 0;
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) {
+            if (a == 1 && b == 1) continue;
+            if (adj[a][b] < 0) continue;
+            sorted[cnt++] = adj[a][b];
+        }
+    vt_timsort_small(g, sorted, cnt);
+    float medianU = 0.0f, medianV = 0.0f;
+    if (cnt > 0) {
+        if (cnt % 2 == 0) {
+            medianU = (g->u[sorted[cnt / 2 - 1]] + g->u[sorted[cnt / 2]]) / 2.0f;
+            medianV = (g->v[sorted[cnt / 2 - 1]] + g->v[sorted[cnt / 2]]) / 2.0f;
+        } else {
+            medianU = g->u[sorted[cnt / 2]];
+            medianV = g->v[sorted[cnt / 2]];
+        }
+    }
+    float rIs[8];
+    for (int k = 0; k < 8; k++) rIs[k] = 3.40282347e+38f;
+    int rIndex = 0;
+    for (int k = 0; k < cnt; k++) {
+        float dU = medianU - g->u[sorted[k]];
+        float dV = medianV - g->v[sorted[k]];
+        float rI = dU * dU + dV * dV;
+        rIs[rIndex++] = rI;
+    }
+    qsort(rIs, 8, sizeof(float), vt_fcmp);
+    float rMed;
+    if (rIndex - 1 == 0) {
+        rMed = 1.0f;
+    } else if ((rIndex - 1) % 2 == 0) {
+        rMed = (float)sqrt((double)((rIs[(rIndex - 1) / 2] + rIs[(rIndex - 1) / 2 + 1]) / 2.0f));
+    } else {
+        rMed = (float)sqrt((double)rIs[(rIndex - 1) / 2 + 1]);
+    }
+    float dU = medianU - g->u[t];
+    float dV = medianV - g->v[t];
+    float distance = (float)sqrt((double)(dU * dU + dV * dV)) / (rMed + epsilon0);
+    vt_set_invalid(g, t, distance < threshold ? 0 : 1);
+}
+
+/* DifferenceValidator.validateVector :34-63 (its loops stop at index 1, so it
+ * consults the upper-left, upper and left neighbours only) */
+static void vt_validate_difference(or_vgrid *g, int t, int adj[3][3], float distanceThreshold)
+{
+    float threshold = distanceThreshold * distanceThreshold;
+    for (int a = 0; a < 2; a++)
+        for (int b = 0; b < 2; b++) {
+            if (a == 1 && b == 1) continue;
+            int n = adj[a][b];
+            if (n < 0 || g->inv[n]) continue;
+            float du = g->u[n] - g->u[t];
+            float dv = g->v[n] - g->v[t];
+            float distance = du * du + dv * dv;
+            vt_set_invalid(g, t, distance < threshold ? 0 : 1);
+        }
+}
+
+static void vt_validate(or_vgrid *g, const or_vstage *s, int t, int adj[3][3])
+{
+    if (s->validator == 0) vt_validate_median(g, t, adj, s->threshold, s->epsilon0);
+    else vt_validate_difference(g, t, adj, s->threshold);
+}
+
+/* SecondaryPeakReplacement.replaceVector :41-75 */
+static void vt_replace_secondary(or_vgrid *g, int t, int peakIndex)
+{
+    if (!g->inv[t]) return;
+    int total = g->pcount ? g->pcount[t] : g->P;
+    if (total <= peakIndex) return;
+    const float *pk = g->puv + (size_t)t * g->P * 2;
+    float u = g->u[t], v = g->v[t];
+    float uOld = pk[0], vOld = pk[1];
+    u -= uOld;
+    v -= vOld;
+    float uNew = pk[2 * peakIndex], vNew = pk[2 * peakIndex + 1];
+    u += uNew;
+    v += vNew;
+    g->u[t] = u; /* Tile.replaceDisplacement, out-of-bound clipping allowed */
+    g->v[t] = v;
+    vt_set_invalid(g, t, 0);
+}
+
+/* BilinearReplacement.interpV :31-80 (horizontal, left pixels) */
+static float vt_interpV(or_vgrid *g, int t1, int t2, int ti)
+{
+    float v1 = g->v[t1], v2 = g->v[t2];
+    float leftPixel1 = (float)g->left[t1], leftPixel2 = (float)g->left[t2];
+    float leftPixelInterp = (float)g->left[ti];
+    float weightedV1 = 0.0f, weightedV2 = 0.0f;
+    if ((leftPixel2 - leftPixelInterp) > 1e-6f && (leftPixelInterp - leftPixel1) > 1e-6f &&
+        (leftPixel2 - leftPixel1) > 1e-6f) {
+        float weightV1 = (leftPixel2 - leftPixelInterp) / (leftPixel2 - leftPixel1);
+        if (weightV1 < 0.0f || weightV1 > 1.0f) g->err = 1;
+        weightedV1 = weightV1 * v1;
+        float weightV2 = (leftPixelInterp - leftPixel1) / (leftPixel2 - leftPixel1);
+        if (weightV2 < 0.0f || weightV2 > 1.0f) g->err = 1;
+        weightedV2 = weightV2 * v2;
+    } else {
+        int count = 0;
+        if (fabsf(leftPixelInterp - leftPixel2) < 1e-6f) { weightedV2 = v2; count++; }
+        if (fabsf(leftPixelInterp - leftPixel1) < 1e-6f) { weightedV1 = v1; count++; }
+        if (count == 0) g->err = 1;
+        if (count > 1) {
+            weightedV1 /= (float)count;
+            weightedV2 /= (float)count;
+        }
+    }
+    return weightedV1 + weightedV2;
+}
+
+/* BilinearReplacement.interpU :82-135 (vertical, top pixels) */
+static float vt_interpU(or_vgrid *g, int t1, int t2, int ti)
+{
+    float u1 = g->u[t1], u2 = g->u[t2];
+    float topPixel1 = (float)g->top[t1], topPixel2 = (float)g->top[t2];
+    float topPixelInterp = (float)g->top[ti];
+    float weightedU1 = 0.0f, weightedU2 = 0.0f;
+    if ((topPixel2 - topPixelInterp) > 1e-6f && (topPixelInterp - topPixel1) > 1e-6f &&
+        (topPixel2 - topPixel1) > 1e-6f) {
+        float weightU1 = (topPixel2 - topPixelInterp) / (topPixel2 - topPixel1);
+        if (weightU1 < 0.0f || weightU1 > 1.0f) g->err = 1;
+        weightedU1 = weightU1 * u1;
+        float weightU2 = (topPixelInterp - topPixel1) / (topPixel2 - topPixel1);
+        if (weightU2 < 0.0f || weightU2 > 1.0f) g->err = 1;
+        weightedU2 = weightU2 * u2;
+    } else {
+        int count = 0;
+        if (fabsf(topPixelInterp - topPixel2) < 1e-6f) { weightedU2 = u2; count++; }
+        if (fabsf(topPixelInterp - topPixel1) < 1e-6f) { weightedU1 = u1; count++; }
+        if (count == 0) g->err = 1;
+        if (count > 1) {
+            weightedU1 /= (float)count;
+            weightedU2 /= (float)count;
+        }
+    }
+    return weightedU1 + weightedU2;
+}
+
+#define VT_OK(x) ((x) >= 0 && !g->inv[x])
+
+/* BilinearReplacement.replaceVector :145-427 */
+static void vt_replace_bilinear(or_vgrid *g, int t, int adj[3][3])
+{
+    if (!g->inv[t]) return;
+    int left = adj[1][0], right = adj[1][2], top = adj[0][1], bottom = adj[2][1];
+    int topLeft = adj[0][0], bottomRight = adj[2][2], bottomLeft = adj[2][0], topRight = adj[0][2];
+    float uV[4] = {0, 0, 0, 0}, uU[4] = {0, 0, 0, 0};
+    int classificationU[4] = {0, 0, 0, 0}, classificationV[4] = {0, 0, 0, 0};
+    int count = 0;
+    if (topLeft >= 0 && bottomLeft >= 0 && (float)g->left[topLeft] - (float)g->left[bottomLeft] > 1.0e-6f) g->err = 1;
+    if (topRight >= 0 && bottomRight >= 0 && (float)g->left[topRight] - (float)g->left[bottomRight] > 1.0e-6f) g->err = 1;
+    if (topLeft >= 0 && topRight >= 0 && (float)g->top[topLeft] - (float)g->top[topRight] > 1.0e-6f) g->err = 1;
+    if (bottomLeft >= 0 && bottomRight >= 0 && (float)g->top[bottomLeft] - (float)g->top[bottomRight] > 1.0e-6f) g->err = 1;
+    if (VT_OK(left) && VT_OK(right)) {
+        uV[0] = vt_interpV(g, left, right, t);
+        uU[0] = vt_interpU(g, left, right, t);
+        classificationV[0] = 2;
+        classificationU[0] = 1;
+        count++;
+    }
+    if (VT_OK(bottom) && VT_OK(top)) {
+        uV[1] = vt_interpV(g, top, bottom, t);
+        uU[1] = vt_interpU(g, top, bottom, t);
+        classificationV[1] = 1;
+        classificationU[1] = 2;
+        count++;
+    }
+    if (VT_OK(topLeft) && VT_OK(bottomRight) && VT_OK(bottomLeft) && VT_OK(topRight)) {
+        float wTR = (float)(g->left[t] - g->left[topLeft]) / (float)(g->left[topRight] - g->left[topLeft]);
+        if (wTR < 0.0f || wTR > 1.0f) g->err = 1;
+        float wTL = (float)(g->left[topRight] - g->left[t]) / (float)(g->left[topRight] - g->left[topLeft]);
+        if (wTL < 0.0f || wTL > 1.0f) g->err = 1;
+        float wBR = (float)(g->left[t] - g->left[bottomLeft]) / (float)(g->left[bottomRight] - g->left[bottomLeft]);
+        if (wBR < 0.0f || wBR > 1.0f) g->err = 1;
+        float wBL = (float)(g->left[bottomRight] - g->left[t]) / (float)(g->left[bottomRight] - g->left[bottomLeft]);
+        if (wBL < 0.0f || wBL > 1.0f) g->err = 1;
+        float wT = (float)(g->top[bottomLeft] - g->top[t]) / (float)(g->top[bottomLeft] - g->top[topLeft]);
+        if (wT < 0.0f || wT > 1.0f) g->err = 1;
+        float wB = (float)(g->top[t] - g->top[topLeft]) / (float)(g->top[bottomLeft] - g->top[topLeft]);
+        if (wB < 0.0f || wB > 1.0f) g->err = 1;
+        if (fabs((double)(1.0f - wTR - wTL)) > 1.0e-6) g->err = 1;
+        if (fabs((double)(1.0f - wBR - wBL)) > 1.0e-6) g->err = 1;
+        if (fabs((double)(1.0f - wT - wB)) > 1.0e-6) g->err = 1;
+        uV[2] = wT * (wTR * g->v[topRight] + wTL * g->v[topLeft]) +
+                wB * (wBR * g->v[bottomRight] + wBL * g->v[bottomLeft]);
+        uU[2] = wT * (wTR * g->u[topRight] + wTL * g->u[topLeft]) +
+                wB * (wBR * g->u[bottomRight] + wBL * g->u[bottomLeft]);
+        classificationV[2] = 3;
+        classificationU[2] = 3;
+        count++;
+    } else {
+        if (VT_OK(topLeft) && VT_OK(bottomRight)) {
+            uV[2] = vt_interpV(g, topLeft, bottomRight, t);
+            uU[2] = vt_interpU(g, topLeft, bottomRight, t);
+            classificationV[2] = 1;
+            classificationU[2] = 1;
+            count++;
+        }
+        if (VT_OK(bottomLeft) && VT_OK(topRight)) {
+            uV[3] = vt_interpV(g, bottomLeft, topRight, t);
+            uU[3] = vt_interpU(g, topRight, bottomLeft, t);
+            classificationV[3] = 1;
+            classificationU[3] = 1;
+            count++;
+        }
+    }
+    if (count == 0) {
+        for (int i = 0; i < 3; i++) {
+            for (int j = 0; j < 3; j++) {
+                if (i == 1 && j == 1) continue;
+                if (VT_OK(adj[i][j])) {
+                    uV[0] = g->v[adj[i][j]];
+                    uU[0] = g->u[adj[i][j]];
+                    count = 1;
+                    break;
+                }
+            }
+        }
+        if (count == 0) {
+            vt_set_invalid(g, t, 1);
+        } else {
+            g->u[t] = uU[0];
+            g->v[t] = uV[0];
+            vt_set_invalid(g, t, 0);
+        }
+    } else {
+        float accumV = 0.0f, accumU = 0.0f;
+        int weightU = 0, weightV = 0;
+        for (int i = 0; i < 4; i++) {
+            accumV += (float)classificationV[i] * uV[i];
+            weightV += classificationV[i];
+            accumU += (float)classificationU[i] * uU[i];
+            weightU += classificationU[i];
+        }
+        accumV /= (float)weightV;
+        accumU /= (float)weightU;
+        g->u[t] = accumU;
+        g->v[t] = accumV;
+        vt_set_invalid(g, t, 0);
+    }
+}
+#undef VT_OK
+
+/* VectorValidatorJob.compute :243-405 over one nI x nJ grid.
+ *   u, v    : tile displacements (in/out)
+ *   puv     : nI*nJ x P x 2 peak displacement increments (peak 0 the main peak)
+ *   pcount  : MaxCrossResult.getTotalPeaks per tile, or NULL (= P)
+ *   invalid : Tile.isInvalidDisplacement after the job
+ *   stats   : sweeps, corrected vectors, vectors left invalid,
+ *             flags (1 sweep limit, 2 replacement exception, 4 spiral error)
+ * The sweep limit (max_sweeps) bounds the loop the reference leaves open. */
+void or_validate(int nI, int nJ, const int *top, const int *left, float *u, float *v, int P,
+                 const float *puv, const int *pcount, int n_stages, const or_vstage *stages,
+                 int max_iterations, int until_no_more, int max_sweeps, unsigned char *invalid,
+                 int *stats)
+{
+    int n = nI * nJ;
+    or_vgrid G;
+    or_vgrid *g = &G;
+    g->nI = nI;
+    g->nJ = nJ;
+    g->top = top;
+    g->left = left;
+    g->u = u;
+    g->v = v;
+    g->bu = (float *)malloc(sizeof(float) * n);
+    g->bv = (float *)malloc(sizeof(float) * n);
+    g->inv = (unsigned char *)malloc((size_t)n);
+    g->locked = (unsigned char *)calloc((size_t)n, 1);
+    g->P = P;
+    g->puv = puv;
+    g->pcount = pcount;
+    g->err = 0;
+    int *order = (int *)malloc(sizeof(int) * n);
+    int flags = 0;
+    if (or_spiral_order(nI, nJ, order) != n) {
+        if (stats) {
+            stats[0] = stats[1] = stats[2] = 0;
+            stats[3] = 4;
+        }
+        free(order);
+        free(g->bu);
+        free(g->bv);
+        free(g->inv);
+        free(g->locked);
+        return;
+    }
+    for (int t = 0; t < n; t++) {
+        g->inv[t] = 1; /* Tile() / resetDisplacements */
+        g->bu[t] = u[t]; /* backupDisplacement */
+        g->bv[t] = v[t];
+    }
+    int adj[3][3];
+    /* initializer visitor :268-281 */
+    for (int k = 0; k < n; k++) {
+        int t = order[k];
+        vt_neighbors(g, t / nJ, t % nJ, adj);
+        vt_validate(g, &stages[0], t, adj);
+        if (!g->inv[t]) g->locked[t] = 1;
+    }
+    int sweeps = 0, total = 0;
+    for (;;) {
+        int corrected = 0;
+        for (int k = 0; k < n; k++) {
+            int t = order[k];
+            vt_neighbors(g, t / nJ, t % nJ, adj);
+            vt_validate(g, &stages[0], t, adj);
+            for (int index = 0; index < n_stages && g->inv[t]; index++) {
+                if (stages[index].replacement != 0) {
+                    g->u[t] = g->bu[t]; /* restoreDisplacement */
+                    g->v[t] = g->bv[t];
+                    if (stages[index].replacement == 1) vt_replace_secondary(g, t, stages[index].peak_index);
+                    else vt_replace_bilinear(g, t, adj);
+                }
+                vt_validate(g, &stages[index], t, adj);
+                if (!g->inv[t]) {
+                    corrected++;
+                    break;
+                }
+            }
+        }
+        sweeps++;
+        total += corrected;
+        /* shouldDoAnotherAttempt :84-103 */
+        if (corrected == 0) break;
+        if (!until_no_more && sweeps >= max_iterations) break;
+        if (sweeps >= max_sweeps) {
+            flags |= 1;
+            break;
+        }
+    }
+    int invalids = 0;
+    for (int t = 0; t < n; t++) {
+        if (g->inv[t]) {
+            invalids++;
+            u[t] = 0.0f;
+            v[t] = 0.0f;
+        }
+        if (invalid) invalid[t] = g->inv[t];
+    }
+    if (g->err) flags |= 2;
+    if (stats) {
+        stats[0] = sweeps;
+        stats[1] = total;
+        stats[2] = invalids;
+        stats[3] = flags;
+    }
+    free(order);
+    free(g->bu);
+    free(g->bv);
+    free(g->inv);
+    free(g->locked);
+}

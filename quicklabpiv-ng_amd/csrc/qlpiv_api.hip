@@ -135,6 +135,7 @@ int qlp_ctx_destroy(qlp_ctx *h)
         for (int t = 0; t < 3; t++)
             if (ctx->tables.tw[li][t]) (void)hipFree(ctx->tables.tw[li][t]);
     if (ctx->scratch) (void)hipFree(ctx->scratch);
+    if (ctx->vwork) (void)hipFree(ctx->vwork);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete h;
     return QLP_OK;
@@ -243,6 +244,7 @@ int qlp_subpixel_dev(qlp_ctx *h, int kind, int pixels, int iterations, int n, in
 {
     if (!h) return fail(QLP_ERR_ARG, "null context");
     if (n < 0 || n_peaks < 1) return fail(QLP_ERR_ARG, "subpixel: bad counts");
+    if (kind < 0) return fail(QLP_ERR_ARG, "subpixel: unknown interpolator kind");
     return launch_subpixel(&h->c, kind, pixels, iterations, n, D, n_peaks, planes, min_floor,
                            peaks_in, peaks_out);
 }
@@ -253,6 +255,7 @@ int qlp_subpixel(qlp_ctx *h, int kind, int pixels, int iterations, int n, int D,
 {
     if (!h) return fail(QLP_ERR_ARG, "null context");
     if (n < 0 || n_peaks < 1) return fail(QLP_ERR_ARG, "subpixel: bad counts");
+    if (kind < 0) return fail(QLP_ERR_ARG, "subpixel: unknown interpolator kind");
     if (n == 0) return QLP_OK;
     Context *ctx = &h->c;
     const size_t bp = align256(sizeof(float) * (size_t)n * D * D), bf = align256(sizeof(float) * n),

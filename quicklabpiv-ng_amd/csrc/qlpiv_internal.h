@@ -48,6 +48,9 @@ struct Context {
     // scratch buffers of the host-pointer entry points (grown on demand)
     void *scratch = nullptr;
     size_t scratch_bytes = 0;
+    // workspace of qlp_validate_vectors_dev (grown on demand)
+    void *vwork = nullptr;
+    size_t vwork_bytes = 0;
 };
 
 void set_error(const std::string &msg);
@@ -91,7 +94,7 @@ int launch_xcorr(Context *ctx, int n, int N, const uint8_t *tA, const uint8_t *t
 int launch_pass(Context *ctx, const PassParams &p);
 int launch_pass_lane(Context *ctx, const PassParams &p, bool from_image, bool peak);
 int launch_find_peaks(Context *ctx, int kind, int n, int D, const float *planes, int n_peaks,
-                      int kernel_width, float *peaks, float *min_floor);
+                      int kernel_width, float *peaks, float *min_floor, int *count = nullptr);
 int launch_subpixel(Context *ctx, int kind, int pixels, int iterations, int n, int D, int n_peaks,
                     const float *planes, const float *min_floor, const float *peaks_in,
                     float *peaks_out);
@@ -103,5 +106,27 @@ int launch_clip(Context *ctx, const uint8_t *img, int H, int W, int n, int N, co
 int launch_warp_micro(Context *ctx, const uint8_t *img, int H, int W, int n, int N, const int *top,
                       const int *left, const float *u, const float *v, uint8_t *tiles,
                       float *maxv);
+
+// Vector validation (qlpiv_validate.hip).  The device arrays of one launch:
+// inputs, outputs and the workspace carved by valid_work_carve.
+struct ValidLaunch {
+    int pairs = 0, nI = 0, nJ = 0, P = 1;
+    const int *top = nullptr, *left = nullptr, *rank = nullptr;  // nI*nJ
+    float *u = nullptr, *v = nullptr;                             // pairs*n, in/out
+    const float *pk = nullptr;  // peak displacement increments: (u_k, v_k) =
+    int pk_stride = 2;          //   pk[(t*P + k)*pk_stride + {0,1}] - pk_off
+    float pk_off = 0.0f;
+    const int *pcount = nullptr;  // pairs*n or null
+    uint8_t *invalid = nullptr;   // pairs*n or null
+    int *stats = nullptr;         // pairs*4 or null
+    float *bu = nullptr, *bv = nullptr;
+    int *flags = nullptr, *done = nullptr, *act = nullptr, *act_count = nullptr;
+};
+int check_valid_config(const qlp_valid_config &cfg);
+void spiral_rank(int nI, int nJ, int *rank);
+size_t valid_work_bytes(int pairs, int n);
+void valid_work_carve(void *base, int pairs, int n, ValidLaunch &vl);
+int launch_validate(Context *ctx, const qlp_valid_config &cfg, const ValidLaunch &vl);
+int launch_accumulate(Context *ctx, int n, int P, int D, const float *sub, float *u, float *v);
 
 }  // namespace qlpiv

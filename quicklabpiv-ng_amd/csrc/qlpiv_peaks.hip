@@ -146,7 +146,8 @@ __device__ void mp_insert(MpList &L, int P, int kd, float value, short pi, short
 __global__ __launch_bounds__(64) void peak_multi_kernel(int n, int D, int P, int k,
                                                         const float *__restrict__ planes,
                                                         float *__restrict__ peaks,
-                                                        float *__restrict__ min_floor)
+                                                        float *__restrict__ min_floor,
+                                                        int *__restrict__ count)
 {
     __shared__ float s_v[MP_CHUNK];
     __shared__ short s_i[MP_CHUNK], s_j[MP_CHUNK];
@@ -233,11 +234,14 @@ __global__ __launch_bounds__(64) void peak_multi_kernel(int n, int D, int P, int
             out[3 * idx + 2] = L.v[e];
         }
         min_floor[ia] = mn;
+        // MaxCrossResult.getTotalPeaks :109-115: the secondary arrays exist
+        // only once a peak index >= 1 was stored
+        if (count) count[ia] = L.size > 1 ? L.size : 1;
     }
 }
 
 int launch_find_peaks(Context *ctx, int kind, int n, int D, const float *planes, int n_peaks,
-                      int kernel_width, float *peaks, float *min_floor)
+                      int kernel_width, float *peaks, float *min_floor, int *count)
 {
     if (n == 0) return QLP_OK;
     if (D < 3 || D > 255) return fail(QLP_ERR_ARG, "find_peaks: plane size out of range 3..255");
@@ -253,7 +257,7 @@ int launch_find_peaks(Context *ctx, int kind, int n, int D, const float *planes,
         if ((iSlices - iSlices / 2) * kernel_width > D)
             return fail(QLP_ERR_ARG, "find_peaks: kernel width too large for the plane");
         hipLaunchKernelGGL(peak_multi_kernel, dim3(n), dim3(64), 0, ctx->stream, n, D, n_peaks,
-                           kernel_width, planes, peaks, min_floor);
+                           kernel_width, planes, peaks, min_floor, count);
     } else {
         return fail(QLP_ERR_ARG, "find_peaks: unknown peak finder kind");
     }

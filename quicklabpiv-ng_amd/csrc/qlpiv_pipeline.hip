@@ -9,8 +9,12 @@
 //      (2I+a, 2J+b) takes parent (I, J); children without a parent keep 0;
 //      NoImageWarping rounds the inherited value);
 //   2. one fused pass: clip A, clip/shift/micro-warp B, correlate, FromCenter,
-//      export the peak neighbourhood;
-//   3. the sub-pixel fit and the displacement update u += i' - D/2.
+//      export the peak neighbourhood (with MultiPeaks: the pass stores the
+//      planes and the peak search runs over them);
+//   3. the sub-pixel fit of every peak and the displacement update
+//      u += i' - D/2 from the main peak;
+//   4. optionally the vector validation with secondary-peak and bilinear
+//      replacement (jobs/validator/VectorValidatorJob.java).
 // Everything stays in HBM between levels; the host only enqueues launches.
 #include <math.h>
 
@@ -25,11 +29,16 @@ struct Level {
     std::vector<int> top_h, left_h;
     int *top = nullptr, *left = nullptr;  // n
     float *u = nullptr, *v = nullptr;     // pairs * n
-    float *peaks = nullptr;               // pairs * n * 3 (integer search)
+    float *peaks = nullptr;               // pairs * n * P * 3 (integer search)
     float *mf = nullptr;                  // pairs * n
     float *nbr = nullptr;                 // pairs * n * 2 * NBR_L
-    float *sub = nullptr;                 // pairs * n * 4
-    float *planes = nullptr;              // pairs * n * D * D (last level, optional)
+    float *sub = nullptr;                 // pairs * n * P * 4
+    float *planes = nullptr;              // pairs * n * D * D (optional)
+    int *count = nullptr;                 // pairs * n peaks found (MultiPeaks)
+    int *rank = nullptr;                  // n, spiral visit order of the validation
+    void *vwork = nullptr;                // validation workspace
+    uint8_t *invalid = nullptr;           // pairs * n
+    int *stats = nullptr;                 // pairs * 4
 };
 
 constexpr int NBR_HALF = 7;
@@ -112,11 +121,12 @@ using namespace qlpiv;
 struct qlp_piv_plan {
     Context *ctx = nullptr;
     int H = 0, W = 0, pairs = 0;
+    int P = 1;  // peaks per IA
     qlp_piv_config cfg{};
     std::vector<Level> levels;
     size_t bytes = 0;
     // optional per-level timing: events before the state kernel, after the
-    // correlation pass and after the fit of every level
+    // correlation pass, after the fit and after the validation of every level
     bool timing = false;
     std::vector<hipEvent_t> ev;
 };
@@ -124,7 +134,8 @@ struct qlp_piv_plan {
 static void plan_free(qlp_piv_plan *p)
 {
     for (auto &L : p->levels) {
-        void *bufs[] = {L.top, L.left, L.u, L.v, L.peaks, L.mf, L.nbr, L.sub, L.planes};
+        void *bufs[] = {L.top, L.left, L.u, L.v, L.peaks, L.mf, L.nbr, L.sub, L.planes,
+                        L.count, L.rank, L.vwork, L.invalid, L.stats};
         for (void *b : bufs)
             if (b) (void)hipFree(b);
     }
@@ -163,12 +174,32 @@ int qlp_piv_plan_create(qlp_ctx *h, int H, int W, int pairs, const qlp_piv_confi
         if (half > NBR_HALF || (cfg->subpix_kind != QLP_SUBPIX_POLY && cfg->subpix_pixels % 2 == 0))
             return fail(QLP_ERR_ARG, "piv_plan_create: sub-pixel window must be odd and <= 15");
     }
+    const bool multi = cfg->peak_kind == QLP_PEAK_MULTI;
+    if (cfg->peak_kind != QLP_PEAK_FROM_CENTER && !multi)
+        return fail(QLP_ERR_ARG, "piv_plan_create: unknown peak finder kind");
+    if (!multi && cfg->n_peaks > 1)
+        return fail(QLP_ERR_ARG, "find_peaks: FromCenter finds exactly one peak");
+    const int P = multi ? cfg->n_peaks : 1;
+    if (multi) {
+        if (P < 1 || P > 16) return fail(QLP_ERR_ARG, "find_peaks: number of peaks must be 1..16");
+        const int k = cfg->peak_kernel_width;
+        for (int tile = cfg->ia_start; tile >= cfg->ia_end; tile /= 2) {
+            const int D = 2 * tile - 1;
+            const int iSlices = k >= 1 ? (D + k - 1) / k : 0;
+            if (k < 1 || k > D || (iSlices - iSlices / 2) * k > D)
+                return fail(QLP_ERR_ARG, "find_peaks: kernel width out of range");
+        }
+    }
+    int rc = check_valid_config(cfg->valid);
+    if (rc) return rc;
+    const bool valid = cfg->valid.n_stages > 0;
     qlp_piv_plan *p = new qlp_piv_plan();
     p->ctx = ctx;
     p->H = H;
     p->W = W;
     p->pairs = pairs;
     p->cfg = *cfg;
+    p->P = P;
     const int m[4] = {cfg->margin_top, cfg->margin_bottom, cfg->margin_left, cfg->margin_right};
     size_t need = 0;
     for (int l = 0, tile = cfg->ia_start; l < nlev; l++, tile /= 2) {
@@ -179,16 +210,17 @@ int qlp_piv_plan_create(qlp_ctx *h, int H, int W, int pairs, const qlp_piv_confi
             return fail(QLP_ERR_ARG, "piv_plan_create: no IA fits the frame");
         }
         const size_t n = L.n, pn = (size_t)pairs * n, D = 2 * tile - 1;
-        need += n * 8 + pn * (8 + 12 + 4 + 4 * 2 * NBR_L + 16);
-        if (cfg->keep_planes >= 2 || (cfg->keep_planes == 1 && l == nlev - 1))
+        need += n * 8 + pn * (8 + 12 * P + 4 + 4 * 2 * NBR_L + 16 * P);
+        if (multi || cfg->keep_planes >= 2 || (cfg->keep_planes == 1 && l == nlev - 1))
             need += pn * D * D * 4;
+        if (multi) need += pn * 4;
+        if (valid) need += n * 4 + valid_work_bytes(pairs, (int)n) + pn + (size_t)pairs * 16;
         p->levels.push_back(std::move(L));
     }
     if (ctx->hbm_budget && ctx->hbm_in_use + need > ctx->hbm_budget) {
         delete p;
         return fail(QLP_ERR_BUDGET, "piv plan exceeds the context's HBM budget");
     }
-    int rc = QLP_OK;
     for (size_t l = 0; l < p->levels.size() && !rc; l++) {
         Level &L = p->levels[l];
         const size_t n = L.n, pn = (size_t)pairs * n, D = 2 * L.N - 1;
@@ -196,12 +228,24 @@ int qlp_piv_plan_create(qlp_ctx *h, int H, int W, int pairs, const qlp_piv_confi
         if (!rc) rc = check_hip(hipMalloc(&L.left, n * 4), "hipMalloc plan");
         if (!rc) rc = check_hip(hipMalloc(&L.u, pn * 4), "hipMalloc plan");
         if (!rc) rc = check_hip(hipMalloc(&L.v, pn * 4), "hipMalloc plan");
-        if (!rc) rc = check_hip(hipMalloc(&L.peaks, pn * 12), "hipMalloc plan");
+        if (!rc) rc = check_hip(hipMalloc(&L.peaks, pn * 12 * P), "hipMalloc plan");
         if (!rc) rc = check_hip(hipMalloc(&L.mf, pn * 4), "hipMalloc plan");
         if (!rc) rc = check_hip(hipMalloc(&L.nbr, pn * 4 * 2 * NBR_L), "hipMalloc plan");
-        if (!rc) rc = check_hip(hipMalloc(&L.sub, pn * 16), "hipMalloc plan");
-        if (!rc && (cfg->keep_planes >= 2 || (cfg->keep_planes == 1 && l + 1 == p->levels.size())))
+        if (!rc) rc = check_hip(hipMalloc(&L.sub, pn * 16 * P), "hipMalloc plan");
+        if (!rc && (multi || cfg->keep_planes >= 2 || (cfg->keep_planes == 1 && l + 1 == p->levels.size())))
             rc = check_hip(hipMalloc(&L.planes, pn * D * D * 4), "hipMalloc plan");
+        if (!rc && multi) rc = check_hip(hipMalloc(&L.count, pn * 4), "hipMalloc plan");
+        if (!rc && valid) {
+            rc = check_hip(hipMalloc(&L.rank, n * 4), "hipMalloc plan");
+            if (!rc) rc = check_hip(hipMalloc(&L.vwork, valid_work_bytes(pairs, (int)n)), "hipMalloc plan");
+            if (!rc) rc = check_hip(hipMalloc(&L.invalid, pn), "hipMalloc plan");
+            if (!rc) rc = check_hip(hipMalloc(&L.stats, (size_t)pairs * 16), "hipMalloc plan");
+            if (!rc) {
+                std::vector<int> rank(n);
+                spiral_rank(L.nI, L.nJ, rank.data());
+                rc = check_hip(hipMemcpy(L.rank, rank.data(), n * 4, hipMemcpyHostToDevice), "upload grid");
+            }
+        }
         if (!rc)
             rc = check_hip(hipMemcpy(L.top, L.top_h.data(), n * 4, hipMemcpyHostToDevice), "upload grid");
         if (!rc)
@@ -254,11 +298,12 @@ int qlp_piv_run_dev(qlp_piv_plan *p, const uint8_t *imgsA, const uint8_t *imgsB)
     if (!p || !imgsA || !imgsB) return fail(QLP_ERR_ARG, "piv_run: null argument");
     Context *ctx = p->ctx;
     const qlp_piv_config &c = p->cfg;
+    const bool multi = c.peak_kind == QLP_PEAK_MULTI;
     for (size_t l = 0; l < p->levels.size(); l++) {
         Level &L = p->levels[l];
         const int pn = p->pairs * L.n;
         const int D = 2 * L.N - 1;
-        if (p->timing) (void)hipEventRecord(p->ev[3 * l], ctx->stream);
+        if (p->timing) (void)hipEventRecord(p->ev[4 * l], ctx->stream);
         if (l == 0) {
             const size_t blocks = ((size_t)pn + 255) / 256;
             hipLaunchKernelGGL(fill_kernel, dim3(blocks), dim3(256), 0, ctx->stream, L.u, 0.0f, (size_t)pn);
@@ -286,18 +331,51 @@ int qlp_piv_run_dev(qlp_piv_plan *p, const uint8_t *imgsA, const uint8_t *imgsB)
         q.dv = (l == 0) ? nullptr : L.v;
         q.bmode = c.warp_mode == QLP_WARP_MICRO ? B_WARP_MICRO : B_CLIP_SHIFT;
         q.planes = L.planes;
-        q.peaks = L.peaks;
-        q.min_floor = L.mf;
-        q.nbr = L.nbr;
-        q.nbr_half = NBR_HALF;
+        if (!multi) {  // FromCenter and the fit neighbourhood fused into the pass
+            q.peaks = L.peaks;
+            q.min_floor = L.mf;
+            q.nbr = L.nbr;
+            q.nbr_half = NBR_HALF;
+        }
         rc = launch_pass(ctx, q);
         if (rc) return rc;
-        if (p->timing) (void)hipEventRecord(p->ev[3 * l + 1], ctx->stream);
+        if (p->timing) (void)hipEventRecord(p->ev[4 * l + 1], ctx->stream);
         const int kind = ((int)l >= c.interpolator_start_step) ? c.subpix_kind : -1;
-        rc = launch_subpixel_nbr(ctx, kind, c.subpix_pixels, c.subpix_iterations, pn, D, NBR_HALF,
-                                 L.nbr, L.mf, L.peaks, L.sub, L.u, L.v);
+        if (multi) {
+            rc = launch_find_peaks(ctx, QLP_PEAK_MULTI, pn, D, L.planes, p->P, c.peak_kernel_width,
+                                   L.peaks, L.mf, L.count);
+            if (!rc)
+                rc = launch_subpixel(ctx, kind, c.subpix_pixels, c.subpix_iterations, pn, D, p->P,
+                                     L.planes, L.mf, L.peaks, L.sub);
+            if (!rc) rc = launch_accumulate(ctx, pn, p->P, D, L.sub, L.u, L.v);
+        } else {
+            rc = launch_subpixel_nbr(ctx, kind, c.subpix_pixels, c.subpix_iterations, pn, D,
+                                     NBR_HALF, L.nbr, L.mf, L.peaks, L.sub, L.u, L.v);
+        }
         if (rc) return rc;
-        if (p->timing) (void)hipEventRecord(p->ev[3 * l + 2], ctx->stream);
+        if (p->timing) (void)hipEventRecord(p->ev[4 * l + 2], ctx->stream);
+        if (c.valid.n_stages > 0) {
+            ValidLaunch vl;
+            vl.pairs = p->pairs;
+            vl.nI = L.nI;
+            vl.nJ = L.nJ;
+            vl.P = p->P;
+            vl.top = L.top;
+            vl.left = L.left;
+            vl.rank = L.rank;
+            vl.u = L.u;
+            vl.v = L.v;
+            vl.pk = L.sub;  // (i', j', value, status) per peak: u_k = i' - D/2
+            vl.pk_stride = 4;
+            vl.pk_off = (float)(D / 2);
+            vl.pcount = L.count;
+            vl.invalid = L.invalid;
+            vl.stats = L.stats;
+            valid_work_carve(L.vwork, p->pairs, L.n, vl);
+            rc = launch_validate(ctx, c.valid, vl);
+            if (rc) return rc;
+        }
+        if (p->timing) (void)hipEventRecord(p->ev[4 * l + 3], ctx->stream);
     }
     return QLP_OK;
 }
@@ -306,7 +384,7 @@ int qlp_piv_plan_set_timing(qlp_piv_plan *p, int enable)
 {
     if (!p) return fail(QLP_ERR_ARG, "null plan");
     if (enable && p->ev.empty()) {
-        p->ev.resize(3 * p->levels.size());
+        p->ev.resize(4 * p->levels.size());
         for (auto &e : p->ev) {
             int rc = check_hip(hipEventCreate(&e), "hipEventCreate");
             if (rc) return rc;
@@ -316,19 +394,23 @@ int qlp_piv_plan_set_timing(qlp_piv_plan *p, int enable)
     return QLP_OK;
 }
 
-int qlp_piv_plan_timing(qlp_piv_plan *p, int level, float *state_and_pass_ms, float *fit_ms)
+int qlp_piv_plan_timing(qlp_piv_plan *p, int level, float *state_and_pass_ms, float *fit_ms,
+                        float *valid_ms)
 {
     if (!p) return fail(QLP_ERR_ARG, "null plan");
     if (!p->timing) return fail(QLP_ERR_ARG, "timing is not enabled on this plan");
     if (level < 0 || level >= (int)p->levels.size()) return fail(QLP_ERR_ARG, "level out of range");
-    int rc = check_hip(hipEventSynchronize(p->ev[3 * level + 2]), "hipEventSynchronize");
+    hipEvent_t *e = &p->ev[4 * level];
+    int rc = check_hip(hipEventSynchronize(e[3]), "hipEventSynchronize");
     if (rc) return rc;
-    float a = 0.0f, b = 0.0f;
-    rc = check_hip(hipEventElapsedTime(&a, p->ev[3 * level], p->ev[3 * level + 1]), "hipEventElapsedTime");
-    if (!rc) rc = check_hip(hipEventElapsedTime(&b, p->ev[3 * level + 1], p->ev[3 * level + 2]), "hipEventElapsedTime");
+    float a = 0.0f, b = 0.0f, c = 0.0f;
+    rc = check_hip(hipEventElapsedTime(&a, e[0], e[1]), "hipEventElapsedTime");
+    if (!rc) rc = check_hip(hipEventElapsedTime(&b, e[1], e[2]), "hipEventElapsedTime");
+    if (!rc) rc = check_hip(hipEventElapsedTime(&c, e[2], e[3]), "hipEventElapsedTime");
     if (rc) return rc;
     if (state_and_pass_ms) *state_and_pass_ms = a;
     if (fit_ms) *fit_ms = b;
+    if (valid_ms) *valid_ms = c;
     return QLP_OK;
 }
 
@@ -342,6 +424,16 @@ int qlp_piv_result_dev(qlp_piv_plan *p, int level, const float **u, const float 
     if (v) *v = L.v;
     if (peaks) *peaks = L.sub;
     if (planes) *planes = L.planes;
+    return QLP_OK;
+}
+
+int qlp_piv_validation_dev(qlp_piv_plan *p, int level, const uint8_t **invalid, const int **stats)
+{
+    if (!p) return fail(QLP_ERR_ARG, "null plan");
+    if (level < 0 || level >= (int)p->levels.size()) return fail(QLP_ERR_ARG, "level out of range");
+    const Level &L = p->levels[level];
+    if (invalid) *invalid = L.invalid;
+    if (stats) *stats = L.stats;
     return QLP_OK;
 }
 

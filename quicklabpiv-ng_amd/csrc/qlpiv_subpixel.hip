@@ -406,6 +406,14 @@ __global__ __launch_bounds__(256) void subpixel_kernel(int kind, int pixels, int
     const float pI = peaks_in[(size_t)gid * 3 + 0];
     const float pJ = peaks_in[(size_t)gid * 3 + 1];
     const float pV = peaks_in[(size_t)gid * 3 + 2];
+    if (kind < 0) {  // level before the interpolator start step: the integer peak
+        float *o = peaks_out + (size_t)gid * 4;
+        o[0] = pI;
+        o[1] = pJ;
+        o[2] = pV;
+        o[3] = (float)ST_SKIP;
+        return;
+    }
     const int maxI = (int)pI, maxJ = (int)pJ;
     const int half = (kind == QLP_SUBPIX_POLY) ? 1 : pixels / 2;
     float col[2 * NBR_MAX_HALF + 1], row[2 * NBR_MAX_HALF + 1];
@@ -481,8 +489,10 @@ int launch_subpixel(Context *ctx, int kind, int pixels, int iterations, int n, i
                     float *peaks_out)
 {
     if (n == 0) return QLP_OK;
-    int rc = check_subpixel_args(kind, pixels, iterations);
-    if (rc) return rc;
+    if (kind >= 0) {  // kind < 0: integer peaks copied (multi-pass levels before the start step)
+        int rc = check_subpixel_args(kind, pixels, iterations);
+        if (rc) return rc;
+    }
     const int total = n * n_peaks;
     hipLaunchKernelGGL(subpixel_kernel, dim3((total + 255) / 256), dim3(256), 0, ctx->stream, kind,
                        pixels, iterations, n, D, n_peaks, planes, min_floor, peaks_in, peaks_out);

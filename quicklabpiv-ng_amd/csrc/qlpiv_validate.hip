@@ -1,0 +1,779 @@
+// Vector validation and replacement of displacement fields
+// (jobs/validator/VectorValidatorJob.java:105-405) on the GPU.
+//
+// The reference visits the tiles one at a time in a centre-out spiral
+// (circularVisitor :105-196), and a visit may change the tile's displacement,
+// which the later visits of its neighbours read.  Two facts make the job
+// parallel without changing any result:
+//  * The initializer visit (:268-281) only validates.  The normalised median
+//    verdict of a tile depends on its neighbours' displacements, never on their
+//    validity flags, so every tile is validated at once; tiles found valid are
+//    locked (Tile.setLockedValidationState) and never change again.  With the
+//    difference validator every tile stays invalid: all tiles start invalid
+//    (Tile :56, resetDisplacements :74-81) and that validator only updates a
+//    tile from a valid neighbour, so by induction over the visit order no
+//    tile ever sees one.
+//  * A visit of the validator loop (:283-353) reads and writes only its 3x3
+//    neighbourhood.  Order matters only between two unlocked ("active") tiles
+//    that are neighbours: the later one in spiral order must see the earlier
+//    one's new state, the earlier one the later one's old state.  A sweep
+//    therefore runs in rounds: an active tile is processed once every active
+//    neighbour earlier in spiral order was processed in an earlier round of
+//    the same sweep.  Two neighbours never share a round, so every visit reads
+//    exactly what the sequential visit reads.
+// The sweep kernel is one workgroup per frame pair; rounds are separated by a
+// barrier, and the state other lanes wrote is read with device-coherent loads.
+#include <float.h>
+#include <math.h>
+
+#include <vector>
+
+#include "qlpiv_internal.h"
+
+namespace qlpiv {
+
+enum { F_INVALID = 1, F_LOCKED = 2 };
+
+struct VStage {
+    int validator;
+    float threshold, eps;
+    int replacement, peak_index;
+};
+
+struct VParams {
+    int nI, nJ, n, P, n_stages, max_iter, until;
+    VStage st[QLP_VALID_MAX_STAGES];
+    const int *top, *left, *rank;
+    float *u, *v, *bu, *bv;
+    const float *pk;
+    int pk_stride;
+    float pk_off;
+    const int *pcount;
+    int *flags, *done, *act, *act_count;
+    uint8_t *invalid;
+    int *stats;
+};
+
+__device__ __forceinline__ float ldc(const float *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int ldc(const int *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Comparator of NormalizedMedianValidator.distanceComparator :19-35.
+__device__ __forceinline__ int jcmp(float a, float b) { return a > b ? 1 : (a < b ? -1 : 0); }
+
+// Collections.sort of at most 8 tiles: java.util.TimSort (OpenJDK 8) takes its
+// short-array path, countRunAndMakeAscending followed by binarySort.  For
+// keys without NaN every stable sort gives this order; the exact path also
+// fixes the order when a displacement is NaN.
+__device__ void timsort_small(float *key, float *a, float *b, int n)
+{
+    if (n < 2) return;
+    int runHi = 2;
+    if (jcmp(key[1], key[0]) < 0) {
+        while (runHi < n && jcmp(key[runHi], key[runHi - 1]) < 0) runHi++;
+        for (int lo = 0, hi = runHi - 1; lo < hi; lo++, hi--) {
+            float t = key[lo]; key[lo] = key[hi]; key[hi] = t;
+            t = a[lo]; a[lo] = a[hi]; a[hi] = t;
+            t = b[lo]; b[lo] = b[hi]; b[hi] = t;
+        }
+    } else {
+        while (runHi < n && jcmp(key[runHi], key[runHi - 1]) >= 0) runHi++;
+    }
+    for (int start = runHi; start < n; start++) {
+        const float pk = key[start], pa = a[start], pb = b[start];
+        int left = 0, right = start;
+        while (left < right) {
+            const int mid = (left + right) >> 1;
+            if (jcmp(pk, key[mid]) < 0) right = mid;
+            else left = mid + 1;
+        }
+        for (int m = start; m > left; m--) {
+            key[m] = key[m - 1];
+            a[m] = a[m - 1];
+            b[m] = b[m - 1];
+        }
+        key[left] = pk;
+        a[left] = pa;
+        b[left] = pb;
+    }
+}
+
+// Arrays.sort(float[8]): ascending with NaN last.
+__device__ __forceinline__ bool flt_before(float x, float y)
+{
+    if (isnan(x)) return false;
+    if (isnan(y)) return true;
+    return x < y;
+}
+
+// NormalizedMedianValidator.validateVector :63-136: the new invalid state.
+__device__ bool median_invalid(const VParams &p, size_t base, int i, int j, float tu, float tv,
+                               float thr, float eps)
+{
+    float nu[8], nv[8], key[8];
+    int cnt = 0;
+    for (int di = -1; di <= 1; di++)
+        for (int dj = -1; dj <= 1; dj++) {
+            if (di == 0 && dj == 0) continue;
+            const int ii = i + di, jj = j + dj;
+            if (ii < 0 || ii >= p.nI || jj < 0 || jj >= p.nJ) continue;
+            const size_t o = base + (size_t)ii * p.nJ + jj;
+            nu[cnt] = ldc(p.u + o);
+            nv[cnt] = ldc(p.v + o);
+            key[cnt] = nu[cnt] * nu[cnt] + nv[cnt] * nv[cnt];
+            cnt++;
+        }
+    timsort_small(key, nu, nv, cnt);
+    float mu = 0.0f, mv = 0.0f;
+    if (cnt > 0) {
+        if (cnt % 2 == 0) {
+            mu = (nu[cnt / 2 - 1] + nu[cnt / 2]) / 2.0f;
+            mv = (nv[cnt / 2 - 1] + nv[cnt / 2]) / 2.0f;
+        } else {
+            mu = nu[cnt / 2];
+            mv = nv[cnt / 2];
+        }
+    }
+    float r[8];
+    for (int k = 0; k < 8; k++) r[k] = FLT_MAX;
+    for (int k = 0; k < cnt; k++) {
+        const float dU = mu - nu[k], dV = mv - nv[k];
+        r[k] = dU * dU + dV * dV;
+    }
+    for (int a = 1; a < 8; a++) {  // insertion sort
+        const float x = r[a];
+        int b = a;
+        while (b > 0 && flt_before(x, r[b - 1])) {
+            r[b] = r[b - 1];
+            b--;
+        }
+        r[b] = x;
+    }
+    // the reference's index arithmetic, kept as is (C and Java both truncate
+    // -1 / 2 to 0 and give -1 % 2 == -1)
+    const int ri = cnt;
+    float rMed;
+    if (ri - 1 == 0) rMed = 1.0f;
+    else if ((ri - 1) % 2 == 0) rMed = (float)sqrt((double)((r[(ri - 1) / 2] + r[(ri - 1) / 2 + 1]) / 2.0f));
+    else rMed = (float)sqrt((double)r[(ri - 1) / 2 + 1]);
+    const float dU = mu - tu, dV = mv - tv;
+    const float distance = (float)sqrt((double)(dU * dU + dV * dV)) / (rMed + eps);
+    return !(distance < thr);
+}
+
+// DifferenceValidator.validateVector :34-63: the tile's own state is kept when
+// none of the three neighbours it consults (upper-left, upper, left) is valid.
+__device__ bool difference_invalid(const VParams &p, size_t base, int i, int j, float tu, float tv,
+                                   float thr, bool cur)
+{
+    const float thr2 = thr * thr;
+    bool inv = cur;
+    for (int a = 0; a < 2; a++)
+        for (int b = 0; b < 2; b++) {
+            if (a == 1 && b == 1) continue;
+            const int ii = i + a - 1, jj = j + b - 1;
+            if (ii < 0 || ii >= p.nI || jj < 0 || jj >= p.nJ) continue;
+            const size_t o = base + (size_t)ii * p.nJ + jj;
+            if (ldc(p.flags + o) & F_INVALID) continue;
+            const float du = ldc(p.u + o) - tu, dv = ldc(p.v + o) - tv;
+            const float distance = du * du + dv * dv;
+            inv = !(distance < thr2);
+        }
+    return inv;
+}
+
+__device__ __forceinline__ bool stage_invalid(const VParams &p, int s, size_t base, int i, int j,
+                                              float tu, float tv, bool cur)
+{
+    const VStage &st = p.st[s];
+    if (st.validator == QLP_VALIDATOR_MEDIAN) return median_invalid(p, base, i, j, tu, tv, st.threshold, st.eps);
+    return difference_invalid(p, base, i, j, tu, tv, st.threshold, cur);
+}
+
+// Neighbour of the 3x3 window (a, b in 0..2) or -1 outside the grid.
+__device__ __forceinline__ int nb_index(const VParams &p, int i, int j, int a, int b)
+{
+    const int ii = i + a - 1, jj = j + b - 1;
+    if (ii < 0 || ii >= p.nI || jj < 0 || jj >= p.nJ) return -1;
+    return ii * p.nJ + jj;
+}
+
+struct NbTile {
+    bool ok;  // exists and is valid
+    float u, v, top, left;
+};
+
+__device__ __forceinline__ NbTile nb_tile(const VParams &p, size_t base, int i, int j, int a, int b)
+{
+    NbTile t{false, 0.0f, 0.0f, 0.0f, 0.0f};
+    const int k = nb_index(p, i, j, a, b);
+    if (k < 0) return t;
+    t.top = (float)p.top[k];
+    t.left = (float)p.left[k];
+    if (ldc(p.flags + base + k) & F_INVALID) return t;
+    t.ok = true;
+    t.u = ldc(p.u + base + k);
+    t.v = ldc(p.v + base + k);
+    return t;
+}
+
+// BilinearReplacement.interpV :31-80 (horizontal, over left pixels); interpU
+// :82-135 is the same over top pixels.  err receives the cases the reference
+// rejects with VectorReplacementException.
+__device__ float interp_1d(float x1, float x2, float xi, float a1, float a2, int &err)
+{
+    float w1v = 0.0f, w2v = 0.0f;
+    if ((x2 - xi) > 1e-6f && (xi - x1) > 1e-6f && (x2 - x1) > 1e-6f) {
+        const float w1 = (x2 - xi) / (x2 - x1);
+        if (w1 < 0.0f || w1 > 1.0f) err = 1;
+        w1v = w1 * a1;
+        const float w2 = (xi - x1) / (x2 - x1);
+        if (w2 < 0.0f || w2 > 1.0f) err = 1;
+        w2v = w2 * a2;
+    } else {
+        int count = 0;
+        if (fabsf(xi - x2) < 1e-6f) {
+            w2v = a2;
+            count++;
+        }
+        if (fabsf(xi - x1) < 1e-6f) {
+            w1v = a1;
+            count++;
+        }
+        if (count == 0) err = 1;
+        if (count > 1) {
+            w1v /= (float)count;
+            w2v /= (float)count;
+        }
+    }
+    return w1v + w2v;
+}
+
+__device__ __forceinline__ float weight(float num, float den, int &err)
+{
+    const float w = num / den;
+    if (w < 0.0f || w > 1.0f) err = 1;
+    return w;
+}
+
+// BilinearReplacement.replaceVector :145-427 on an invalid tile.
+__device__ void replace_bilinear(const VParams &p, size_t base, int i, int j, float ttop,
+                                 float tleft, float &tu, float &tv, bool &inv, int &err)
+{
+    const NbTile left = nb_tile(p, base, i, j, 1, 0), right = nb_tile(p, base, i, j, 1, 2);
+    const NbTile top = nb_tile(p, base, i, j, 0, 1), bottom = nb_tile(p, base, i, j, 2, 1);
+    const NbTile tl = nb_tile(p, base, i, j, 0, 0), br = nb_tile(p, base, i, j, 2, 2);
+    const NbTile bl = nb_tile(p, base, i, j, 2, 0), tr = nb_tile(p, base, i, j, 0, 2);
+    const bool tlx = nb_index(p, i, j, 0, 0) >= 0, brx = nb_index(p, i, j, 2, 2) >= 0;
+    const bool blx = nb_index(p, i, j, 2, 0) >= 0, trx = nb_index(p, i, j, 0, 2) >= 0;
+    // tile alignment checks :174-189
+    if (tlx && blx && tl.left - bl.left > 1.0e-6f) err = 1;
+    if (trx && brx && tr.left - br.left > 1.0e-6f) err = 1;
+    if (tlx && trx && tl.top - tr.top > 1.0e-6f) err = 1;
+    if (blx && brx && bl.top - br.top > 1.0e-6f) err = 1;
+    float uV[4] = {0.0f, 0.0f, 0.0f, 0.0f}, uU[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    int cV[4] = {0, 0, 0, 0}, cU[4] = {0, 0, 0, 0};
+    int count = 0;
+    if (left.ok && right.ok) {
+        uV[0] = interp_1d(left.left, right.left, tleft, left.v, right.v, err);
+        uU[0] = interp_1d(left.top, right.top, ttop, left.u, right.u, err);
+        cV[0] = 2;
+        cU[0] = 1;
+        count++;
+    }
+    if (bottom.ok && top.ok) {
+        uV[1] = interp_1d(top.left, bottom.left, tleft, top.v, bottom.v, err);
+        uU[1] = interp_1d(top.top, bottom.top, ttop, top.u, bottom.u, err);
+        cV[1] = 1;
+        cU[1] = 2;
+        count++;
+    }
+    if (tl.ok && br.ok && bl.ok && tr.ok) {
+        const float wTR = weight(tleft - tl.left, tr.left - tl.left, err);
+        const float wTL = weight(tr.left - tleft, tr.left - tl.left, err);
+        const float wBR = weight(tleft - bl.left, br.left - bl.left, err);
+        const float wBL = weight(br.left - tleft, br.left - bl.left, err);
+        const float wT = weight(bl.top - ttop, bl.top - tl.top, err);
+        const float wB = weight(ttop - tl.top, bl.top - tl.top, err);
+        if (fabs((double)(1.0f - wTR - wTL)) > 1.0e-6) err = 1;
+        if (fabs((double)(1.0f - wBR - wBL)) > 1.0e-6) err = 1;
+        if (fabs((double)(1.0f - wT - wB)) > 1.0e-6) err = 1;
+        uV[2] = wT * (wTR * tr.v + wTL * tl.v) + wB * (wBR * br.v + wBL * bl.v);
+        uU[2] = wT * (wTR * tr.u + wTL * tl.u) + wB * (wBR * br.u + wBL * bl.u);
+        cV[2] = 3;
+        cU[2] = 3;
+        count++;
+    } else {
+        if (tl.ok && br.ok) {
+            uV[2] = interp_1d(tl.left, br.left, tleft, tl.v, br.v, err);
+            uU[2] = interp_1d(tl.top, br.top, ttop, tl.u, br.u, err);
+            cV[2] = 1;
+            cU[2] = 1;
+            count++;
+        }
+        if (bl.ok && tr.ok) {
+            uV[3] = interp_1d(bl.left, tr.left, tleft, bl.v, tr.v, err);
+            uU[3] = interp_1d(tr.top, bl.top, ttop, tr.u, bl.u, err);
+            cV[3] = 1;
+            cU[3] = 1;
+            count++;
+        }
+    }
+    if (count == 0) {
+        // :276-301: the first valid neighbour of each window row overwrites
+        // the candidate (the break leaves only the inner loop)
+        bool found = false;
+        float ru = 0.0f, rv = 0.0f;
+        for (int a = 0; a < 3; a++)
+            for (int b = 0; b < 3; b++) {
+                if (a == 1 && b == 1) continue;
+                const NbTile t = nb_tile(p, base, i, j, a, b);
+                if (t.ok) {
+                    ru = t.u;
+                    rv = t.v;
+                    found = true;
+                    break;
+                }
+            }
+        if (found) {
+            tu = ru;
+            tv = rv;
+            inv = false;
+        }
+        return;
+    }
+    float accV = 0.0f, accU = 0.0f;
+    int wU = 0, wV = 0;
+    for (int k = 0; k < 4; k++) {
+        accV += (float)cV[k] * uV[k];
+        wV += cV[k];
+        accU += (float)cU[k] * uU[k];
+        wU += cU[k];
+    }
+    accV /= (float)wV;
+    accU /= (float)wU;
+    tu = accU;
+    tv = accV;
+    inv = false;
+}
+
+// SecondaryPeakReplacement.replaceVector :41-75: undo the main peak's
+// increment, add the chosen peak's.
+__device__ __forceinline__ void replace_secondary(const VParams &p, size_t gt, int k, float &tu,
+                                                  float &tv, bool &inv)
+{
+    const int total = p.pcount ? p.pcount[gt] : p.P;
+    if (total <= k || k >= p.P) return;
+    const float *m = p.pk + gt * p.P * p.pk_stride;
+    const float uOld = m[0] - p.pk_off, vOld = m[1] - p.pk_off;
+    const float uNew = m[(size_t)k * p.pk_stride] - p.pk_off;
+    const float vNew = m[(size_t)k * p.pk_stride + 1] - p.pk_off;
+    float u = tu, v = tv;
+    u -= uOld;
+    v -= vOld;
+    u += uNew;
+    v += vNew;
+    tu = u;
+    tv = v;
+    inv = false;
+}
+
+// Backup, initializer visit (:243-282) and the active list.
+__global__ __launch_bounds__(256) void valid_init_kernel(VParams p, int pairs)
+{
+    const size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= (size_t)pairs * p.n) return;
+    const int pair = (int)(gid / p.n), t = (int)(gid - (size_t)pair * p.n);
+    const size_t base = (size_t)pair * p.n;
+    const float tu = p.u[gid], tv = p.v[gid];
+    p.bu[gid] = tu;
+    p.bv[gid] = tv;
+    p.done[gid] = -1;
+    bool inv = true;
+    if (p.st[0].validator == QLP_VALIDATOR_MEDIAN)
+        inv = median_invalid(p, base, t / p.nJ, t % p.nJ, tu, tv, p.st[0].threshold, p.st[0].eps);
+    p.flags[gid] = inv ? F_INVALID : F_LOCKED;
+    if (inv) {
+        const int slot = atomicAdd(p.act_count + pair, 1);
+        p.act[base + slot] = t;
+    }
+}
+
+// Visit of one active tile by the validator loop (:283-353).
+__device__ void visit(const VParams &p, size_t base, int t, int &corrected, int &err)
+{
+    const int i = t / p.nJ, j = t - (t / p.nJ) * p.nJ;
+    const size_t gt = base + t;
+    float tu = p.u[gt], tv = p.v[gt];
+    bool inv = (p.flags[gt] & F_INVALID) != 0;
+    inv = stage_invalid(p, 0, base, i, j, tu, tv, inv);
+    const float ttop = (float)p.top[t], tleft = (float)p.left[t];
+    for (int s = 0; s < p.n_stages && inv; s++) {
+        const int rep = p.st[s].replacement;
+        if (rep != QLP_REPLACE_NONE) {
+            tu = p.bu[gt];  // Tile.restoreDisplacement
+            tv = p.bv[gt];
+            if (rep == QLP_REPLACE_SECONDARY_PEAK) replace_secondary(p, gt, p.st[s].peak_index, tu, tv, inv);
+            else replace_bilinear(p, base, i, j, ttop, tleft, tu, tv, inv, err);
+        }
+        inv = stage_invalid(p, s, base, i, j, tu, tv, inv);
+        if (!inv) corrected++;
+    }
+    p.u[gt] = tu;
+    p.v[gt] = tv;
+    p.flags[gt] = inv ? F_INVALID : 0;
+}
+
+__device__ bool ready(const VParams &p, size_t base, int t, int sweep_start, int round)
+{
+    const int i = t / p.nJ, j = t - (t / p.nJ) * p.nJ;
+    const int rt = p.rank[t];
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) {
+            if (a == 1 && b == 1) continue;
+            const int k = nb_index(p, i, j, a, b);
+            if (k < 0 || p.rank[k] > rt) continue;
+            if (ldc(p.flags + base + k) & F_LOCKED) continue;
+            const int d = ldc(p.done + base + k);
+            if (d < sweep_start || d >= round) return false;
+        }
+    return true;
+}
+
+constexpr int SWEEP_THREADS = 1024;
+
+__global__ __launch_bounds__(SWEEP_THREADS) void valid_sweep_kernel(VParams p)
+{
+    __shared__ int s_prog, s_corr, s_err, s_inv;
+    const int pair = blockIdx.x;
+    const size_t base = (size_t)pair * p.n;
+    const int cnt = p.act_count[pair];
+    const int *act = p.act + base;
+    int round = 0, sweeps = 0, total_corr = 0, capped = 0, err = 0;
+    for (;;) {
+        const int sweep_start = round;
+        if (threadIdx.x == 0) s_corr = 0;
+        __syncthreads();
+        int processed = 0, corrected = 0;
+        while (processed < cnt) {
+            if (threadIdx.x == 0) s_prog = 0;
+            __syncthreads();
+            int mine = 0;
+            for (int a = threadIdx.x; a < cnt; a += SWEEP_THREADS) {
+                const int t = act[a];
+                if (p.done[base + t] >= sweep_start) continue;
+                if (!ready(p, base, t, sweep_start, round)) continue;
+                visit(p, base, t, corrected, err);
+                p.done[base + t] = round;
+                mine++;
+            }
+            if (mine) atomicAdd(&s_prog, mine);
+            __threadfence();
+            __syncthreads();
+            processed += s_prog;
+            round++;
+            __syncthreads();
+        }
+        if (corrected) atomicAdd(&s_corr, corrected);
+        __syncthreads();
+        const int corr = s_corr;
+        __syncthreads();
+        sweeps++;
+        total_corr += corr;
+        // VectorValidatorJob.shouldDoAnotherAttempt :84-103
+        if (corr == 0) break;
+        if (!p.until && sweeps >= p.max_iter) break;
+        if (sweeps >= QLP_VALID_MAX_SWEEPS) {
+            capped = 1;
+            break;
+        }
+    }
+    // :372-383: vectors still invalid become (0, 0)
+    if (threadIdx.x == 0) {
+        s_inv = 0;
+        s_err = 0;
+    }
+    __syncthreads();
+    int inv = 0;
+    for (int a = threadIdx.x; a < cnt; a += SWEEP_THREADS) {
+        const size_t gt = base + act[a];
+        if (p.flags[gt] & F_INVALID) {
+            p.u[gt] = 0.0f;
+            p.v[gt] = 0.0f;
+            inv++;
+        }
+    }
+    if (inv) atomicAdd(&s_inv, inv);
+    if (err) atomicOr(&s_err, 2);
+    __syncthreads();
+    if (p.invalid)
+        for (int t = threadIdx.x; t < p.n; t += SWEEP_THREADS)
+            p.invalid[base + t] = (p.flags[base + t] & F_INVALID) ? 1 : 0;
+    if (p.stats && threadIdx.x == 0) {
+        int *st = p.stats + (size_t)pair * 4;
+        st[0] = sweeps;
+        st[1] = total_corr;
+        st[2] = s_inv;
+        st[3] = capped | s_err;
+    }
+}
+
+__global__ void zero_int_kernel(int *x, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = 0;
+}
+
+// Displacement update of the multi-pass loop from the main peak of a plane
+// fit: u += i' - D/2, v += j' - D/2 (IterationStepTiles
+// .updateDisplacementsFromMaxCrossResults :317-371 via Tile.accumulateDisplacement).
+__global__ void accumulate_kernel(int n, int P, int D, const float *__restrict__ sub,
+                                  float *__restrict__ u, float *__restrict__ v)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const float c = (float)(D / 2);
+    const float *s = sub + (size_t)t * P * 4;
+    u[t] = u[t] + (s[0] - c);
+    v[t] = v[t] + (s[1] - c);
+}
+
+// VectorValidatorJob.circularVisitor :105-196: position of every tile in the
+// visit order.  The rings grow by one tile on every side per turn, so the walk
+// reaches every tile; each tile is ranked once.
+void spiral_rank(int nI, int nJ, int *rank)
+{
+    const int total = nI * nJ;
+    for (int k = 0; k < total; k++) rank[k] = -1;
+    int startI = nI / 2, startJ = nJ / 2;
+    int sideI = 2, sideJ = 2;
+    if (nI != startI * 2) sideI = 1;
+    else startI--;
+    if (nJ != startJ * 2) sideJ = 1;
+    else startJ--;
+    int visited = 0;
+    auto visit_at = [&](int i, int j) {
+        if (i >= 0 && i < nI && j >= 0 && j < nJ && rank[i * nJ + j] < 0) rank[i * nJ + j] = visited++;
+    };
+    while (visited < total) {
+        const int mR = sideJ > 1 ? sideJ - 1 : 0, mD = sideI > 1 ? sideI - 1 : 0;
+        const int mL = sideJ > 1 ? sideJ - 1 : 0, mU = sideI > 2 ? sideI - 2 : 0;
+        int i = startI, j = startJ;
+        visit_at(i, j);
+        for (int o = 0; o < mR; o++) visit_at(i, ++j);
+        for (int o = 0; o < mD; o++) visit_at(++i, j);
+        if (mD > 0) {
+            for (int o = 0; o < mL; o++) visit_at(i, --j);
+            for (int o = 0; o < mU; o++) visit_at(--i, j);
+        }
+        sideI += 2;
+        sideJ += 2;
+        startJ--;
+        startI--;
+    }
+}
+
+int check_valid_config(const qlp_valid_config &c)
+{
+    if (c.n_stages < 0 || c.n_stages > QLP_VALID_MAX_STAGES)
+        return fail(QLP_ERR_ARG, "validation: number of stages must be 0..8");
+    for (int s = 0; s < c.n_stages; s++) {
+        const qlp_valid_stage &st = c.stage[s];
+        if (st.validator != QLP_VALIDATOR_MEDIAN && st.validator != QLP_VALIDATOR_DIFFERENCE)
+            return fail(QLP_ERR_ARG, "Validator strategy cannot be null");
+        if (st.replacement != QLP_REPLACE_NONE && st.replacement != QLP_REPLACE_SECONDARY_PEAK &&
+            st.replacement != QLP_REPLACE_BILINEAR)
+            return fail(QLP_ERR_ARG, "Unknown vector replacement method");
+        if (st.replacement == QLP_REPLACE_SECONDARY_PEAK && st.peak_index < 1)
+            return fail(QLP_ERR_ARG, "validation: secondary peak index must be >= 1");
+    }
+    return QLP_OK;
+}
+
+static size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+size_t valid_work_bytes(int pairs, int n)
+{
+    const size_t pn = (size_t)pairs * n;
+    return 2 * al256(pn * 4) + 3 * al256(pn * 4) + al256((size_t)pairs * 4);
+}
+
+void valid_work_carve(void *base, int pairs, int n, ValidLaunch &vl)
+{
+    const size_t pn = (size_t)pairs * n;
+    char *b = (char *)base;
+    vl.bu = (float *)b;
+    b += al256(pn * 4);
+    vl.bv = (float *)b;
+    b += al256(pn * 4);
+    vl.flags = (int *)b;
+    b += al256(pn * 4);
+    vl.done = (int *)b;
+    b += al256(pn * 4);
+    vl.act = (int *)b;
+    b += al256(pn * 4);
+    vl.act_count = (int *)b;
+}
+
+int launch_validate(Context *ctx, const qlp_valid_config &cfg, const ValidLaunch &vl)
+{
+    if (cfg.n_stages == 0 || vl.pairs == 0) return QLP_OK;
+    VParams p;
+    p.nI = vl.nI;
+    p.nJ = vl.nJ;
+    p.n = vl.nI * vl.nJ;
+    p.P = vl.P;
+    p.n_stages = cfg.n_stages;
+    p.max_iter = cfg.max_iterations;
+    p.until = cfg.until_no_more;
+    for (int s = 0; s < QLP_VALID_MAX_STAGES; s++) {
+        const qlp_valid_stage &st = cfg.stage[s < cfg.n_stages ? s : 0];
+        p.st[s] = VStage{st.validator, st.threshold, st.epsilon0, st.replacement, st.peak_index};
+    }
+    p.top = vl.top;
+    p.left = vl.left;
+    p.rank = vl.rank;
+    p.u = vl.u;
+    p.v = vl.v;
+    p.bu = vl.bu;
+    p.bv = vl.bv;
+    p.pk = vl.pk;
+    p.pk_stride = vl.pk_stride;
+    p.pk_off = vl.pk_off;
+    p.pcount = vl.pcount;
+    p.flags = vl.flags;
+    p.done = vl.done;
+    p.act = vl.act;
+    p.act_count = vl.act_count;
+    p.invalid = vl.invalid;
+    p.stats = vl.stats;
+    hipLaunchKernelGGL(zero_int_kernel, dim3((vl.pairs + 255) / 256), dim3(256), 0, ctx->stream,
+                       vl.act_count, vl.pairs);
+    const size_t total = (size_t)vl.pairs * p.n;
+    hipLaunchKernelGGL(valid_init_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+                       ctx->stream, p, vl.pairs);
+    hipLaunchKernelGGL(valid_sweep_kernel, dim3(vl.pairs), dim3(SWEEP_THREADS), 0, ctx->stream, p);
+    return check_hip(hipGetLastError(), "validation kernel launch");
+}
+
+int launch_accumulate(Context *ctx, int n, int P, int D, const float *sub, float *u, float *v)
+{
+    if (n == 0) return QLP_OK;
+    hipLaunchKernelGGL(accumulate_kernel, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, n, P, D,
+                       sub, u, v);
+    return check_hip(hipGetLastError(), "accumulate kernel launch");
+}
+
+}  // namespace qlpiv
+
+using namespace qlpiv;
+
+extern "C" {
+
+int qlp_validate_vectors_dev(qlp_ctx *h, int pairs, int nI, int nJ, const int *top,
+                             const int *left, float *u, float *v, int n_peaks,
+                             const float *peak_uv, const int *peak_count,
+                             const qlp_valid_config *cfg, uint8_t *invalid, int *stats)
+{
+    if (!h || !cfg || !top || !left || !u || !v) return fail(QLP_ERR_ARG, "validate: null argument");
+    if (pairs < 0 || nI < 1 || nJ < 1) return fail(QLP_ERR_ARG, "validate: bad grid");
+    if (n_peaks < 1 || n_peaks > 16) return fail(QLP_ERR_ARG, "validate: number of peaks must be 1..16");
+    int rc = check_valid_config(*cfg);
+    if (rc) return rc;
+    for (int s = 0; s < cfg->n_stages; s++)
+        if (cfg->stage[s].replacement == QLP_REPLACE_SECONDARY_PEAK && !peak_uv)
+            return fail(QLP_ERR_ARG, "validate: secondary peak replacement needs the peak displacements");
+    if (pairs == 0) return QLP_OK;
+    Context *ctx = ctx_of(h);
+    const int n = nI * nJ;
+    const size_t grid = al256((size_t)n * 4);
+    const size_t need = 3 * grid + valid_work_bytes(pairs, n);
+    if (need > ctx->vwork_bytes) {
+        if (ctx->hbm_budget && ctx->hbm_in_use - ctx->vwork_bytes + need > ctx->hbm_budget)
+            return fail(QLP_ERR_BUDGET, "request exceeds the context's HBM budget");
+        if (ctx->vwork) {
+            rc = check_hip(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+            if (rc) return rc;
+            (void)hipFree(ctx->vwork);
+            ctx->hbm_in_use -= ctx->vwork_bytes;
+            ctx->vwork = nullptr;
+            ctx->vwork_bytes = 0;
+        }
+        rc = check_hip(hipMalloc(&ctx->vwork, need), "hipMalloc validation workspace");
+        if (rc) return rc;
+        ctx->vwork_bytes = need;
+        ctx->hbm_in_use += need;
+    }
+    std::vector<int> rank(n);
+    spiral_rank(nI, nJ, rank.data());
+    char *b = (char *)ctx->vwork;
+    ValidLaunch vl;
+    vl.pairs = pairs;
+    vl.nI = nI;
+    vl.nJ = nJ;
+    vl.P = n_peaks;
+    vl.top = (const int *)b;
+    vl.left = (const int *)(b + grid);
+    vl.rank = (const int *)(b + 2 * grid);
+    // the grid arrays are pageable host memory: synchronous uploads, so the
+    // workspace is not in use by an earlier call when it is overwritten
+    rc = check_hip(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    if (!rc) rc = check_hip(hipMemcpy(b, top, (size_t)n * 4, hipMemcpyHostToDevice), "upload grid");
+    if (!rc) rc = check_hip(hipMemcpy(b + grid, left, (size_t)n * 4, hipMemcpyHostToDevice), "upload grid");
+    if (!rc) rc = check_hip(hipMemcpy(b + 2 * grid, rank.data(), (size_t)n * 4, hipMemcpyHostToDevice), "upload grid");
+    if (rc) return rc;
+    valid_work_carve(b + 3 * grid, pairs, n, vl);
+    vl.u = u;
+    vl.v = v;
+    vl.pk = peak_uv;
+    vl.pk_stride = 2;
+    vl.pk_off = 0.0f;
+    vl.pcount = peak_count;
+    vl.invalid = invalid;
+    vl.stats = stats;
+    return launch_validate(ctx, *cfg, vl);
+}
+
+int qlp_validate_vectors(qlp_ctx *h, int pairs, int nI, int nJ, const int *top, const int *left,
+                         float *u, float *v, int n_peaks, const float *peak_uv,
+                         const int *peak_count, const qlp_valid_config *cfg, uint8_t *invalid,
+                         int *stats)
+{
+    if (!h || !cfg || !top || !left || !u || !v) return fail(QLP_ERR_ARG, "validate: null argument");
+    if (pairs < 0 || nI < 1 || nJ < 1) return fail(QLP_ERR_ARG, "validate: bad grid");
+    if (n_peaks < 1 || n_peaks > 16) return fail(QLP_ERR_ARG, "validate: number of peaks must be 1..16");
+    if (pairs == 0) return QLP_OK;
+    Context *ctx = ctx_of(h);
+    const size_t pn = (size_t)pairs * nI * nJ;
+    const size_t bu = al256(pn * 4), bk = al256(pn * n_peaks * 8), bc = al256(pn * 4),
+                 bi = al256(pn), bs = al256((size_t)pairs * 16);
+    void *d = nullptr;
+    int rc = check_hip(hipMalloc(&d, 2 * bu + bk + bc + bi + bs), "hipMalloc validation buffers");
+    if (rc) return rc;
+    char *b = (char *)d;
+    float *du = (float *)b, *dv = (float *)(b + bu), *dk = (float *)(b + 2 * bu);
+    int *dc = (int *)(b + 2 * bu + bk);
+    uint8_t *di = (uint8_t *)(b + 2 * bu + bk + bc);
+    int *ds = (int *)(b + 2 * bu + bk + bc + bi);
+    rc = check_hip(hipMemcpy(du, u, pn * 4, hipMemcpyHostToDevice), "copy u");
+    if (!rc) rc = check_hip(hipMemcpy(dv, v, pn * 4, hipMemcpyHostToDevice), "copy v");
+    if (!rc && peak_uv) rc = check_hip(hipMemcpy(dk, peak_uv, pn * n_peaks * 8, hipMemcpyHostToDevice), "copy peaks");
+    if (!rc && peak_count) rc = check_hip(hipMemcpy(dc, peak_count, pn * 4, hipMemcpyHostToDevice), "copy counts");
+    if (!rc)
+        rc = qlp_validate_vectors_dev(h, pairs, nI, nJ, top, left, du, dv, n_peaks,
+                                      peak_uv ? dk : nullptr, peak_count ? dc : nullptr, cfg, di, ds);
+    if (!rc) rc = check_hip(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    if (!rc) rc = check_hip(hipMemcpy(u, du, pn * 4, hipMemcpyDeviceToHost), "copy u");
+    if (!rc) rc = check_hip(hipMemcpy(v, dv, pn * 4, hipMemcpyDeviceToHost), "copy v");
+    if (!rc && invalid) rc = check_hip(hipMemcpy(invalid, di, pn, hipMemcpyDeviceToHost), "copy invalid");
+    if (!rc && stats) rc = check_hip(hipMemcpy(stats, ds, (size_t)pairs * 16, hipMemcpyDeviceToHost), "copy stats");
+    (void)hipFree(d);
+    return rc;
+}
+
+}  // extern "C"

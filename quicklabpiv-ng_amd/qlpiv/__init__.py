@@ -23,7 +23,8 @@ from ._lib import (QLP_NORM_A_MAX, QLP_NORM_PER_TILE, QLP_PEAK_FROM_CENTER, QLP_
                    QLP_SUBPIX_GAUSS1D, QLP_SUBPIX_GUO, QLP_SUBPIX_POLY, QlpError, build, check)
 
 __all__ = [
-    "Context", "CrossCorrelationJob", "JobResultEnum", "QlpError", "build",
+    "Context", "CrossCorrelationJob", "JobResultEnum", "PivPlan", "QlpError",
+    "VectorValidatorJob", "build",
     "QLP_NORM_PER_TILE", "QLP_NORM_A_MAX", "QLP_PEAK_FROM_CENTER", "QLP_PEAK_MULTI",
     "QLP_SUBPIX_GAUSS1D", "QLP_SUBPIX_GUO", "QLP_SUBPIX_POLY",
 ]
@@ -190,13 +191,110 @@ class Context:
                                                 _ptr(mx)))
         return tiles, mx
 
+    # ---- vector validation ---------------------------------------------------
+    def validate_vectors(self, top, left, u, v, stages, peak_uv=None, peak_count=None,
+                         max_iterations=1, until_no_more=False):
+        """VectorValidatorJob.compute over 'pairs' grids.
+
+        top, left: nI x nJ tile origins (host).  u, v: pairs x nI x nJ (or
+        nI x nJ) displacements; peak_uv: ... x P x 2 peak displacement
+        increments; peak_count: ... per-tile peak totals (None: all P).  numpy
+        inputs use the host entry point and return new arrays; torch CUDA
+        tensors are updated in place on the context's stream.  Returns
+        (u, v, invalid, stats)."""
+        top = np.ascontiguousarray(top, np.int32)
+        left = np.ascontiguousarray(left, np.int32)
+        nI, nJ = top.shape
+        cfg = _lib.valid_config(stages, max_iterations, until_no_more)
+        pairs = int(np.prod(u.shape)) // (nI * nJ)
+        P = 1 if peak_uv is None else int(peak_uv.shape[-2])
+        if isinstance(u, np.ndarray):
+            u = np.array(u, np.float32, copy=True, order="C")
+            v = np.array(v, np.float32, copy=True, order="C")
+            pk = None if peak_uv is None else np.ascontiguousarray(peak_uv, np.float32)
+            pc = None if peak_count is None else np.ascontiguousarray(peak_count, np.int32)
+            invalid = np.empty(u.shape, np.uint8)
+            stats = np.empty((pairs, 4), np.int32)
+            check(self.lib.qlp_validate_vectors(self.h, pairs, nI, nJ, _ptr(top), _ptr(left),
+                                                _ptr(u), _ptr(v), P, _ptr(pk), _ptr(pc),
+                                                ctypes.byref(cfg), _ptr(invalid), _ptr(stats)))
+            return u, v, invalid.astype(bool), stats
+        import torch
+
+        invalid = torch.empty(u.shape, dtype=torch.uint8, device=u.device)
+        stats = torch.empty((pairs, 4), dtype=torch.int32, device=u.device)
+        check(self.lib.qlp_validate_vectors_dev(self.h, pairs, nI, nJ, _ptr(top), _ptr(left),
+                                                _ptr(u), _ptr(v), P, _ptr(peak_uv),
+                                                _ptr(peak_count), ctypes.byref(cfg),
+                                                _ptr(invalid), _ptr(stats)))
+        return u, v, invalid, stats
+
 
 class JobResultEnum(Enum):
     """The keys of the reference's job protocol used on this path
-    (jobs/JobResultEnum.java:14-15)."""
+    (jobs/JobResultEnum.java:14-22)."""
     JOB_RESULT_CLIPPED_TILES_A = "clipped_tiles_a"
     JOB_RESULT_CLIPPED_TILES_B = "clipped_tiles_b"
     JOB_RESULT_CROSS_MATRICES = "cross_matrices"
+    JOB_RESULT_VALIDATOR_DATA = "validator_data"
+    JOB_RESULT_VALIDATED_VECTORS = "validated_vectors"
+
+
+class VectorValidatorJob:
+    """Job<VectorValidatorJobParameters, IterationStepTiles> over libqlpiv
+    (jobs/validator/VectorValidatorJob.java:43-410).
+
+    Input (JOB_RESULT_VALIDATOR_DATA): a dict with the tile grid ``top``,
+    ``left`` (nI x nJ), displacements ``u``, ``v`` (nI x nJ), the peak
+    displacement increments ``peak_uv`` (nI x nJ x P x 2, peak 0 the main
+    peak) and optionally ``peak_count`` (nI x nJ).  Output
+    (JOB_RESULT_VALIDATED_VECTORS): a dict with the validated ``u``, ``v``,
+    the ``invalid`` flags and the job ``stats``.  Configuration: the stage list
+    of the combined validator ((validator, threshold, epsilon0, replacement,
+    peak_index) tuples), maxCorrectionIterations and
+    iterateUntilNoMoreCorrections.  analyze() raises the reference's
+    JobAnalyzeException cases as RuntimeError (:52-74)."""
+
+    def __init__(self, ctx: Context, stages, max_iterations=0, until_no_more=False):
+        self.ctx = ctx
+        self.stages = list(stages) if stages is not None else None
+        self.max_iterations = max_iterations
+        self.until_no_more = until_no_more
+        self.inputs = {}
+        self.results = {}
+
+    def set_input_parameters(self, key: JobResultEnum, params):
+        self.inputs[key] = params
+
+    def analyze(self):
+        if self.stages is None:
+            raise RuntimeError("Cannot have a null vector validator or null vector replacement "
+                               "strategy defined")
+        for st in self.stages:
+            if st is None or st[0] is None:
+                raise RuntimeError("Validator strategy cannot be null")
+        if JobResultEnum.JOB_RESULT_VALIDATOR_DATA not in self.inputs:
+            raise RuntimeError("Validator input parameters are not set")
+
+    def compute(self):
+        p = self.inputs[JobResultEnum.JOB_RESULT_VALIDATOR_DATA]
+        if not self.stages:
+            self.results[JobResultEnum.JOB_RESULT_VALIDATED_VECTORS] = dict(p)
+            return
+        u, v, inv, st = self.ctx.validate_vectors(p["top"], p["left"], p["u"], p["v"],
+                                                  self.stages, p.get("peak_uv"),
+                                                  p.get("peak_count"), self.max_iterations,
+                                                  self.until_no_more)
+        out = dict(p)
+        out.update(u=u, v=v, invalid=inv, stats=st)
+        self.results[JobResultEnum.JOB_RESULT_VALIDATED_VECTORS] = out
+
+    def get_job_result(self, key: JobResultEnum):
+        return self.results.get(key)
+
+    def dispose(self):
+        self.inputs.clear()
+        self.results.clear()
 
 
 class CrossCorrelationJob:
@@ -264,7 +362,9 @@ class PivPlan:
     config keys: ia_start, ia_end, overlap_factor (1 - overlap), margins
     (top, bottom, left, right), norm_mode, warp_mode (QLP_WARP_NONE/MICRO),
     subpix_kind (QLP_SUBPIX_* or -1), subpix_pixels, subpix_iterations,
-    interpolator_start_step, keep_planes."""
+    interpolator_start_step, keep_planes, peak_kind (QLP_PEAK_*), n_peaks,
+    peak_kernel_width, validation (list of stage tuples, see
+    VectorValidatorJob), valid_max_iterations, valid_until_no_more."""
 
     def __init__(self, ctx: Context, H: int, W: int, pairs: int = 1, **config):
         cfg = _lib.PivConfig()
@@ -280,6 +380,13 @@ class PivPlan:
         cfg.subpix_iterations = config.get("subpix_iterations", 20)
         cfg.interpolator_start_step = config.get("interpolator_start_step", 0)
         cfg.keep_planes = int(config.get("keep_planes", False))
+        cfg.peak_kind = config.get("peak_kind", QLP_PEAK_FROM_CENTER)
+        cfg.n_peaks = config.get("n_peaks", 1)
+        cfg.peak_kernel_width = config.get("peak_kernel_width", 3)
+        cfg.valid = _lib.valid_config(config.get("validation") or [],
+                                      config.get("valid_max_iterations", 0),
+                                      config.get("valid_until_no_more", False))
+        self.P = cfg.n_peaks if cfg.peak_kind == QLP_PEAK_MULTI else 1
         self.ctx, self.H, self.W, self.pairs, self.cfg = ctx, H, W, pairs, cfg
         h = ctypes.c_void_p()
         check(ctx.lib.qlp_piv_plan_create(ctx.h, H, W, pairs, ctypes.byref(cfg), ctypes.byref(h)))
@@ -317,19 +424,19 @@ class PivPlan:
         check(self.ctx.lib.qlp_piv_plan_set_timing(self.h, int(enable)))
 
     def timing(self):
-        """[(pass_ms, fit_ms)] per level of the last run (waits for it)."""
+        """[(pass_ms, fit_ms, valid_ms)] per level of the last run (waits for it)."""
         out = []
         for lv in range(len(self.levels)):
-            a, b = ctypes.c_float(), ctypes.c_float()
-            check(self.ctx.lib.qlp_piv_plan_timing(self.h, lv, ctypes.byref(a), ctypes.byref(b)))
-            out.append((a.value, b.value))
+            a, b, c = ctypes.c_float(), ctypes.c_float(), ctypes.c_float()
+            check(self.ctx.lib.qlp_piv_plan_timing(self.h, lv, ctypes.byref(a), ctypes.byref(b),
+                                                   ctypes.byref(c)))
+            out.append((a.value, b.value, c.value))
         return out
 
     def result(self, level=-1):
         """Host copies (after synchronising) of a level's u, v (pairs x nI x nJ)
-        and peak records (pairs x nI x nJ x 4)."""
-        import torch
-
+        and peak records (pairs x nI x nJ x 4, or pairs x nI x nJ x P x 4 with
+        several peaks)."""
         lv = level % len(self.levels)
         L = self.levels[lv]
         u, v, pk, pl = (ctypes.c_void_p() for _ in range(4))
@@ -337,16 +444,31 @@ class PivPlan:
         check(self.ctx.lib.qlp_piv_result_dev(self.h, lv, ctypes.byref(u), ctypes.byref(v),
                                               ctypes.byref(pk), ctypes.byref(pl)))
         n = L["nI"] * L["nJ"] * self.pairs
-
-        def fetch(ptr, count):
-            out = np.empty(count, np.float32)
-            _hip_copy_to_host(out, ptr.value)
-            return out
-
-        del torch
         shp = (self.pairs, L["nI"], L["nJ"])
-        return (fetch(u, n).reshape(shp), fetch(v, n).reshape(shp),
-                fetch(pk, 4 * n).reshape(shp + (4,)))
+        pshape = shp + ((4,) if self.P == 1 else (self.P, 4))
+        return (_fetch(u, n, np.float32).reshape(shp), _fetch(v, n, np.float32).reshape(shp),
+                _fetch(pk, 4 * n * self.P, np.float32).reshape(pshape))
+
+    def validation(self, level=-1):
+        """Host copies of a level's validation outcome: invalid flags
+        (pairs x nI x nJ bool) and stats (pairs x 4: sweeps, corrected, left
+        invalid, flags); None when validation is off."""
+        lv = level % len(self.levels)
+        L = self.levels[lv]
+        inv, st = ctypes.c_void_p(), ctypes.c_void_p()
+        self.ctx.synchronize()
+        check(self.ctx.lib.qlp_piv_validation_dev(self.h, lv, ctypes.byref(inv), ctypes.byref(st)))
+        if not inv.value:
+            return None
+        n = L["nI"] * L["nJ"] * self.pairs
+        return (_fetch(inv, n, np.uint8).reshape(self.pairs, L["nI"], L["nJ"]).astype(bool),
+                _fetch(st, 4 * self.pairs, np.int32).reshape(self.pairs, 4))
+
+
+def _fetch(ptr, count, dtype):
+    out = np.empty(count, dtype)
+    _hip_copy_to_host(out, ptr.value)
+    return out
 
 
 _hip_rt = None

@@ -56,13 +56,65 @@ SIGNATURES = {
     "qlp_piv_run_dev": (_i, [_vp, _vp, _vp]),
     "qlp_piv_result_dev": (_i, [_vp, _i, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
                                 ctypes.POINTER(_vp), ctypes.POINTER(_vp)]),
+    "qlp_piv_validation_dev": (_i, [_vp, _i, ctypes.POINTER(_vp), ctypes.POINTER(_vp)]),
     "qlp_piv_plan_set_timing": (_i, [_vp, _i]),
     "qlp_piv_plan_timing": (_i, [_vp, _i, ctypes.POINTER(ctypes.c_float),
+                                 ctypes.POINTER(ctypes.c_float),
                                  ctypes.POINTER(ctypes.c_float)]),
+    "qlp_validate_vectors_dev": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _vp, _vp, _vp,
+                                      _vp, _vp]),
+    "qlp_validate_vectors": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _vp, _vp, _vp,
+                                  _vp, _vp]),
 }
 
 QLP_WARP_NONE = 0
 QLP_WARP_MICRO = 1
+
+
+QLP_VALIDATOR_MEDIAN = 0
+QLP_VALIDATOR_DIFFERENCE = 1
+QLP_REPLACE_NONE = 0
+QLP_REPLACE_SECONDARY_PEAK = 1
+QLP_REPLACE_BILINEAR = 2
+QLP_VALID_MAX_STAGES = 8
+QLP_VALID_MAX_SWEEPS = 256
+
+
+class ValidStage(ctypes.Structure):
+    """qlp_valid_stage (include/qlpiv.h)."""
+    _fields_ = [("validator", ctypes.c_int), ("threshold", ctypes.c_float),
+                ("epsilon0", ctypes.c_float), ("replacement", ctypes.c_int),
+                ("peak_index", ctypes.c_int)]
+
+
+class ValidConfig(ctypes.Structure):
+    """qlp_valid_config (include/qlpiv.h)."""
+    _fields_ = [("n_stages", ctypes.c_int), ("max_iterations", ctypes.c_int),
+                ("until_no_more", ctypes.c_int),
+                ("stage", ValidStage * QLP_VALID_MAX_STAGES)]
+
+
+def valid_config(stages, max_iterations=1, until_no_more=False):
+    """ValidConfig from (validator, threshold, epsilon0, replacement, peak_index) tuples."""
+    if len(stages) > QLP_VALID_MAX_STAGES:
+        raise ValueError("at most 8 validation stages")
+    c = ValidConfig()
+    c.n_stages = len(stages)
+    c.max_iterations = int(max_iterations)
+    c.until_no_more = int(bool(until_no_more))
+    for k, (val, thr, eps, rep, pk) in enumerate(stages):
+        c.stage[k] = ValidStage(int(val), float(thr), float(eps), int(rep), int(pk))
+    return c
+
+
+def multipeak_median_stages(n_peaks, threshold, epsilon0):
+    """The stages of the reference UI's multi-peak normalised median model
+    (PIVConfigurationConverterVisitor.visit(...MultiPeakNormalizedMedianModel)
+    :607-641): median + secondary peak 1 .. P-1, then median + bilinear."""
+    st = [(QLP_VALIDATOR_MEDIAN, threshold, epsilon0, QLP_REPLACE_SECONDARY_PEAK, k + 1)
+          for k in range(n_peaks - 1)]
+    st.append((QLP_VALIDATOR_MEDIAN, threshold, epsilon0, QLP_REPLACE_BILINEAR, 0))
+    return st
 
 
 class PivConfig(ctypes.Structure):
@@ -74,7 +126,9 @@ class PivConfig(ctypes.Structure):
                 ("norm_mode", ctypes.c_int), ("warp_mode", ctypes.c_int),
                 ("subpix_kind", ctypes.c_int), ("subpix_pixels", ctypes.c_int),
                 ("subpix_iterations", ctypes.c_int), ("interpolator_start_step", ctypes.c_int),
-                ("keep_planes", ctypes.c_int)]
+                ("keep_planes", ctypes.c_int), ("peak_kind", ctypes.c_int),
+                ("n_peaks", ctypes.c_int), ("peak_kernel_width", ctypes.c_int),
+                ("valid", ValidConfig)]
 
 _lib = None
 
