@@ -7,15 +7,18 @@ the adaptive multi-pass loop 64 -> 32 -> 16 px at 50 % overlap (3,969 + 16,129
 + 65,025 = 85,123 IAs): per level the Direct velocity inheritance, clip of A,
 bilinear micro-warp of B (SecondImage), FFT cross-correlation with the
 reference GPU job's arithmetic (per-tile normalisation), storing every plane
-(the drop-in returns them), FromCenter integer peak, Guo 3 px / 20 iteration
-fit and the displacement update.  The frames live in HBM before the timed
-region.  The vector validation with secondary-peak substitution of C2 is not
-part of this loop yet.
+(the drop-in returns them), MultiPeaks integer search (3 peaks, 3 x 3 blocks),
+Guo 3 px / 20 iteration fits of every peak, the displacement update and the
+vector validation of the reference UI's multi-peak normalised median model
+(threshold 2.0, epsilon0 0.1: median + secondary peak 1, median + secondary
+peak 2, median + bilinear; one sweep).  The frames live in HBM before the
+timed region.
 
 Prints one JSON line (the driver's contract).  The roofline entry is the
 correlation pass of the 32 x 32 level, timed with HIP events on the library's
 stream; the CPU baseline is the oracle's C restatement of the reference's Java
-CPU path, on this host, over the 16,129 IAs of the 32 x 32 level.
+CPU path (correlation, MultiPeaks, Guo fits, validation), on this host, over the
+16,129 IAs of the 32 x 32 level.
 """
 import argparse
 import json
@@ -34,15 +37,17 @@ H = W = 2048
 
 def pass_bytes(n_ia, N):
     """Algorithmic HBM bytes of one plane-writing correlation pass: the A and B
-    tile pixels (2 N^2), the (2N-1)^2 float32 plane and a 32-byte record (peak,
-    floor, displacement)."""
-    return n_ia * (2 * N * N + 4 * (2 * N - 1) ** 2 + 32)
+    tile pixels (2 N^2), the (2N-1)^2 float32 plane and the 8-byte inherited
+    displacement.  (With MultiPeaks the peak search reads the stored planes in
+    its own kernel.)"""
+    return n_ia * (2 * N * N + 4 * (2 * N - 1) ** 2 + 8)
 
 
-def cpu_baseline(a, b, top, left, u, v, N):
+def cpu_baseline(a, b, top, left, u, v, N, nI, nJ, stages):
     """Reference CPU path per IA (clip A, micro-warp B, CrossCorrelationSoftRealFFTJob
-    arithmetic, FindMaximumFromCenter, Guo 3 px) restated in C by the oracle;
-    one thread, over one level's IAs of the same pair."""
+    arithmetic, FindMaximumMultiPeaks, Guo 3 px of every peak) and the level's
+    vector validation, restated in C by the oracle; one thread, over one
+    level's IAs of the same pair."""
     import ctypes
 
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -50,18 +55,28 @@ def cpu_baseline(a, b, top, left, u, v, N):
 
     L = O.lib()
     i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
-    L.or_cpu_level_batch.argtypes = [O._u8p, O._u8p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
-                                     ctypes.c_int, i32p, i32p, O._f32p, O._f32p, O._f32p]
+    L.or_cpu_level_batch_multi.argtypes = [O._u8p, O._u8p, ctypes.c_int, ctypes.c_int,
+                                           ctypes.c_int, ctypes.c_int, i32p, i32p, O._f32p,
+                                           O._f32p, ctypes.c_int, ctypes.c_int, O._f32p, i32p]
     n = top.size
-    out = np.zeros((n, 4), np.float32)
-    L.or_cpu_level_batch(a, b, H, W, 64, N, top[:64], left[:64], u[:64], v[:64], out[:64])
+    P = 3
+    out = np.zeros((n, P, 4), np.float32)
+    cnt = np.zeros(n, np.int32)
+    L.or_cpu_level_batch_multi(a, b, H, W, 64, N, top[:64], left[:64], u[:64], v[:64], P, 3,
+                               out[:64], cnt[:64])
     t0 = time.perf_counter()
-    L.or_cpu_level_batch(a, b, H, W, n, N, top, left, u, v, out)
+    L.or_cpu_level_batch_multi(a, b, H, W, n, N, top, left, u, v, P, 3, out, cnt)
+    c = np.float32(N - 1)
+    inc = (out[..., :2] - c).astype(np.float32)
+    up = (u + inc[:, 0, 0]).astype(np.float32).reshape(nI, nJ)
+    vp = (v + inc[:, 0, 1]).astype(np.float32).reshape(nI, nJ)
+    O.validate(top.reshape(nI, nJ), left.reshape(nI, nJ), up, vp, stages,
+               inc.reshape(nI, nJ, P, 2), cnt.reshape(nI, nJ), 0, False)
     dt = time.perf_counter() - t0
     return {"value": round(n / dt, 1), "unit": "IAs/s", "cores": 1, "kind": "port",
             "sample": f"the {n} IAs of {N}x{N} of one 2048x2048 pair: clip, micro-warp, Java CPU "
-                      f"xcorr, FromCenter, Guo 3px (C restatement, -O2 -ffp-contract=off, "
-                      f"1 thread), {dt:.2f} s"}
+                      f"xcorr, MultiPeaks 3x3/3, Guo 3px per peak, multi-peak median "
+                      f"validation (C restatement, -O2 -ffp-contract=off, 1 thread), {dt:.2f} s"}
 
 
 def main():
@@ -88,10 +103,12 @@ def main():
     da, db = torch.from_numpy(a).to(dev), torch.from_numpy(b).to(dev)
 
     ctx = qlpiv.Context(0)
+    stages = qlpiv._lib.multipeak_median_stages(3, 2.0, 0.1)
     plan = qlpiv.PivPlan(ctx, H, W, 1, ia_start=64, ia_end=16, overlap_factor=0.5,
                          warp_mode=qlpiv._lib.QLP_WARP_MICRO, norm_mode=qlpiv.QLP_NORM_PER_TILE,
                          subpix_kind=qlpiv.QLP_SUBPIX_GUO, subpix_pixels=3, subpix_iterations=20,
-                         keep_planes=2)
+                         keep_planes=2, peak_kind=qlpiv.QLP_PEAK_MULTI, n_peaks=3,
+                         peak_kernel_width=3, validation=stages, valid_max_iterations=0)
     n_levels = [L["nI"] * L["nJ"] for L in plan.levels]
     n_step = sum(n_levels)
     s = ctx.torch_stream()
@@ -121,7 +138,7 @@ def main():
         plan.run(da, db)
         per_level.append(plan.timing())
     plan.set_timing(False)
-    per_level = np.mean(np.array(per_level), axis=0)  # levels x (pass, fit)
+    per_level = np.mean(np.array(per_level), axis=0)  # levels x (pass, fit, validation)
     lv32 = [L["ia"] for L in plan.levels].index(32)
     pass32_ms = float(per_level[lv32][0])
     achieved = pass_bytes(n_levels[lv32], 32) / (pass32_ms * 1e-3) / 1e9
@@ -148,7 +165,7 @@ def main():
                     vi[i, j] = v[0, i // 2, j // 2]
         cpu = cpu_baseline(a, b, np.ascontiguousarray(L["top"].ravel(), np.int32),
                            np.ascontiguousarray(L["left"].ravel(), np.int32), ui.ravel(), vi.ravel(),
-                           32)
+                           32, L["nI"], L["nJ"], stages)
     plan.close()
     ctx.close()
     value = n_step * args.steps / dt
@@ -167,16 +184,18 @@ def main():
         "data": "synthetic",
         "config": {"workload": "C2 plain FP32 form: one 2048x2048 pair, multipass 64->32->16 at "
                                "50% overlap, micro warp of B, per-tile-normalised FFT xcorr "
-                               "storing every plane, FromCenter, Guo 3px/20it; no vector "
-                               "validation yet",
+                               "storing every plane, MultiPeaks 3 peaks/3x3, Guo 3px/20it on "
+                               "every peak, multi-peak normalised median validation (2.0, 0.1) "
+                               "with secondary-peak and bilinear replacement, 1 sweep",
                    "ias_per_step": n_step, "ias_per_level": n_levels,
                    "pair_latency_ms": {"p50": round(float(np.percentile(step_ms, 50)), 3),
                                        "p99": round(float(np.percentile(step_ms, 99)), 3)},
                    "level_ms": {str(plan.levels[k]["ia"]): {"pass": round(float(per_level[k][0]), 4),
-                                                             "fit": round(float(per_level[k][1]), 4)}
+                                                             "peaks_fit": round(float(per_level[k][1]), 4),
+                                                             "validation": round(float(per_level[k][2]), 4)}
                                 for k in range(len(n_levels))},
                    "ias_per_s_at_32x32": round(n_levels[lv32] / (float(per_level[lv32].sum()) * 1e-3), 1)},
-        "roofline": {"bound": "hbm", "kernel": "xcorr_pass_kernel<6,false,true,true> (32x32 level)",
+        "roofline": {"bound": "hbm", "kernel": "xcorr_pass_kernel<6,false,true,false> (32x32 level)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "avg_launch_ms": round(pass32_ms, 4),

@@ -1176,9 +1176,24 @@ void or_cpu_path_batch(int n, int N, const uint8_t *tA, const uint8_t *tB, const
  * inherited displacement (SecondImage), correlate with the Java CPU job
  * (A-max normalisation), FromCenter, Guo 3 px / 20 iterations.
  * out: n x 4 (i', j', value', status). */
+void or_cpu_level_batch_multi(const uint8_t *imgA, const uint8_t *imgB, int H, int W, int n,
+                              int N, const int *top, const int *left, const float *u,
+                              const float *v, int P, int k, float *out, int *count);
+
 void or_cpu_level_batch(const uint8_t *imgA, const uint8_t *imgB, int H, int W, int n, int N,
                         const int *top, const int *left, const float *u, const float *v,
                         float *out)
+{
+    or_cpu_level_batch_multi(imgA, imgB, H, W, n, N, top, left, u, v, 1, 3, out, NULL);
+}
+
+/* The same with FindMaximumMultiPeaks (numberOfPeaks = P > 1, kernelWidth k)
+ * and the Guo fit of every peak found (Gaussian1DHongweiGuoInterpolator
+ * .interpolate :134-170); P = 1 keeps FindMaximumFromCenter.
+ * out: n x P x 4; count: n peaks found (MaxCrossResult.getTotalPeaks), may be NULL. */
+void or_cpu_level_batch_multi(const uint8_t *imgA, const uint8_t *imgB, int H, int W, int n,
+                              int N, const int *top, const int *left, const float *u,
+                              const float *v, int P, int k, float *out, int *count)
 {
     int D = 2 * N - 1;
     uint8_t *ta = (uint8_t *)malloc((size_t)N * N), *tb = (uint8_t *)malloc((size_t)N * N);
@@ -1193,9 +1208,24 @@ void or_cpu_level_batch(const uint8_t *imgA, const uint8_t *imgB, int H, int W, 
             B[q] = (float)tb[q];
         }
         or_xcorr_javacpu(A, B, mA, N, plane);
-        float pk[4];
-        or_find_max_from_center(plane, D, D, pk);
-        or_guo(plane, D, D, 3, 20, pk[3], pk, out + 4 * (size_t)t);
+        if (P <= 1) {
+            float pk[4];
+            or_find_max_from_center(plane, D, D, pk);
+            or_guo(plane, D, D, 3, 20, pk[3], pk, out + 4 * (size_t)t);
+            if (count) count[t] = 1;
+        } else {
+            float pk[3 * OR_LIST_MAXP], mf;
+            int found = or_find_max_multipeaks(plane, D, D, P, k, pk, &mf);
+            for (int q = 0; q < P; q++) {
+                float *o = out + 4 * ((size_t)t * P + q);
+                if (q < found) {
+                    or_guo(plane, D, D, 3, 20, mf, pk + 3 * q, o);
+                } else {
+                    o[0] = o[1] = o[2] = o[3] = 0.0f;
+                }
+            }
+            if (count) count[t] = found > 1 ? found : 1;
+        }
     }
     free(ta);
     free(tb);

@@ -1,4 +1,4 @@
-// Sub-pixel peak refinement, one thread per (IA, peak).
+// Sub-pixel peak refinement, one thread per (IA, peak) (two for Guo).
 //
 // Input peak records are (i, j, value) triples as produced by the peak search;
 // outputs are (i', j', value', status) with status bit0 = row fit accepted,
@@ -126,22 +126,31 @@ __device__ double gauss_lsq_mean(int n, const double *y)
 }
 
 // commons-math3 LUDecomposition + solver, 3x3 (singularity threshold 1e-11).
-__device__ bool lu_solve3(const double A[3][3], const double b[3], double x[3])
+// Fully unrolled: the pivot row is the only run-time index, resolved by
+// branches, so the factors stay in registers.
+__device__ __forceinline__ bool lu_solve3(const double A[3][3], const double b[3], double x[3])
 {
     double lu[3][3];
     int piv[3] = {0, 1, 2};
+#pragma unroll
     for (int r = 0; r < 3; r++)
+#pragma unroll
         for (int c = 0; c < 3; c++) lu[r][c] = A[r][c];
+#pragma unroll
     for (int col = 0; col < 3; col++) {
+#pragma unroll
         for (int row = 0; row < col; row++) {
             double sum = lu[row][col];
+#pragma unroll
             for (int i = 0; i < row; i++) sum -= lu[row][i] * lu[i][col];
             lu[row][col] = sum;
         }
         int mx = col;
         double largest = -DBL_MAX;
+#pragma unroll
         for (int row = col; row < 3; row++) {
             double sum = lu[row][col];
+#pragma unroll
             for (int i = 0; i < col; i++) sum -= lu[row][i] * lu[i][col];
             lu[row][col] = sum;
             if (fabs(sum) > largest) {
@@ -149,42 +158,64 @@ __device__ bool lu_solve3(const double A[3][3], const double b[3], double x[3])
                 mx = row;
             }
         }
-        if (fabs(lu[mx][col]) < 1e-11) return false;
-        if (mx != col) {
-            for (int i = 0; i < 3; i++) {
-                const double t = lu[mx][i];
-                lu[mx][i] = lu[col][i];
-                lu[col][i] = t;
+        double pv = lu[col][col];
+#pragma unroll
+        for (int r = col + 1; r < 3; r++)
+            if (mx == r) pv = lu[r][col];
+        if (fabs(pv) < 1e-11) return false;
+#pragma unroll
+        for (int r = col + 1; r < 3; r++)
+            if (mx == r) {
+#pragma unroll
+                for (int i = 0; i < 3; i++) {
+                    const double t = lu[r][i];
+                    lu[r][i] = lu[col][i];
+                    lu[col][i] = t;
+                }
+                const int t = piv[r];
+                piv[r] = piv[col];
+                piv[col] = t;
             }
-            const int t = piv[mx];
-            piv[mx] = piv[col];
-            piv[col] = t;
-        }
         const double d = lu[col][col];
+#pragma unroll
         for (int row = col + 1; row < 3; row++) lu[row][col] /= d;
     }
     double bp[3];
-    for (int r = 0; r < 3; r++) bp[r] = b[piv[r]];
+#pragma unroll
+    for (int r = 0; r < 3; r++) bp[r] = piv[r] == 0 ? b[0] : (piv[r] == 1 ? b[1] : b[2]);
+#pragma unroll
     for (int col = 0; col < 3; col++) {
         const double bc = bp[col];
+#pragma unroll
         for (int i = col + 1; i < 3; i++) bp[i] -= bc * lu[i][col];
     }
+#pragma unroll
     for (int col = 2; col >= 0; col--) {
         bp[col] /= lu[col][col];
         const double bc = bp[col];
+#pragma unroll
         for (int i = 0; i < col; i++) bp[i] -= bc * lu[i][col];
     }
+#pragma unroll
     for (int r = 0; r < 3; r++) x[r] = bp[r];
     return true;
 }
 
 // Minimum-norm least-squares solve of the symmetric normal matrix by a cyclic
 // Jacobi eigen-decomposition (the reference's SVD fallback for a singular LU).
-__device__ void pinv_solve3(const double Ain[3][3], const double b[3], double x[3])
+// Out of line: rarely taken.
+struct Sol3 {
+    double x0, x1, x2;
+};
+
+__device__ __attribute__((noinline)) Sol3 pinv_solve3(double a00, double a01, double a02,
+                                                      double a10, double a11, double a12,
+                                                      double a20, double a21, double a22,
+                                                      double b0, double b1, double b2)
 {
-    double a[3][3], v[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
-    for (int r = 0; r < 3; r++)
-        for (int c = 0; c < 3; c++) a[r][c] = Ain[r][c];
+    double a[3][3] = {{a00, a01, a02}, {a10, a11, a12}, {a20, a21, a22}};
+    const double b[3] = {b0, b1, b2};
+    double v[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
     for (int sweep = 0; sweep < 50; sweep++) {
         const double off = fabs(a[0][1]) + fabs(a[0][2]) + fabs(a[1][2]);
         if (off < 1e-300) break;
@@ -214,7 +245,7 @@ __device__ void pinv_solve3(const double Ain[3][3], const double b[3], double x[
     double smax = 0.0;
     for (int k = 0; k < 3; k++) smax = fmax(smax, fabs(a[k][k]));
     const double tol = 3.0 * smax * 2.220446049250313e-16;
-    for (int r = 0; r < 3; r++) x[r] = 0.0;
+    double x[3] = {0.0, 0.0, 0.0};
     for (int k = 0; k < 3; k++) {
         const double s = a[k][k];
         if (fabs(s) <= tol) continue;
@@ -222,14 +253,18 @@ __device__ void pinv_solve3(const double Ain[3][3], const double b[3], double x[
         for (int r = 0; r < 3; r++) proj += v[r][k] * b[r];
         for (int r = 0; r < 3; r++) x[r] += v[r][k] * proj / s;
     }
+    return Sol3{x[0], x[1], x[2]};
 }
 
-__device__ void guo_accumulate(double A[3][3], double b[3], float xs, double ySquare, float ys)
+// One sample of the weighted normal equations
+// (Gaussian1DHongweiGuoInterpolator.estimateGaussianParameters :104-130);
+// logy = ln(y) of the sample, the same in every iteration.
+__device__ __forceinline__ void guo_accumulate(double A[3][3], double b[3], float xs,
+                                               double ySquare, double logy)
 {
     const double xSquare = (double)(xs * xs);
     const double xCubic = xSquare * xs;
     const double xFourth = xCubic * xs;
-    const double logy = log((double)ys);
     A[0][0] += ySquare;
     A[0][1] += xs * ySquare;
     A[0][2] += xSquare * ySquare;
@@ -242,27 +277,6 @@ __device__ void guo_accumulate(double A[3][3], double b[3], float xs, double ySq
     A[2][1] += xCubic * ySquare;
     A[2][2] += xFourth * ySquare;
     b[2] += xSquare * ySquare * logy;
-}
-
-// estimateGaussianParameters :104-130
-__device__ void guo_estimate(int iterations, int n, const float *xs, const float *ys, float &mean,
-                             float &amp)
-{
-    double A[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, b[3] = {0, 0, 0}, coefs[3] = {0, 0, 0};
-    for (int i = 0; i < n; i++) guo_accumulate(A, b, xs[i], (double)(ys[i] * ys[i]), ys[i]);
-    for (int it = 0; it < iterations; it++) {
-        if (!lu_solve3(A, b, coefs)) pinv_solve3(A, b, coefs);
-        for (int r = 0; r < 3; r++) {
-            b[r] = 0.0;
-            for (int c = 0; c < 3; c++) A[r][c] = 0.0;
-        }
-        for (int i = 0; i < n; i++) {
-            const double yk = exp(coefs[0] + coefs[1] * xs[i] + coefs[2] * xs[i] * xs[i]);
-            guo_accumulate(A, b, xs[i], yk * yk, ys[i]);
-        }
-    }
-    mean = (float)(-coefs[1] / (2.0 * coefs[2]));
-    amp = (float)(exp(coefs[0] - coefs[1] * coefs[1] / (4.0 * coefs[2])));
 }
 
 constexpr int NBR_MAX_HALF = 7;
@@ -351,45 +365,151 @@ __device__ void fit_peak(int kind, int pixels, int iterations, int D, float pI, 
         return;
     }
 
-    // GUO
-    if (mf < -5.15f) {
-        out[3] = (float)ST_FLOOR;
-        return;
-    }
-    const float add = (mf < 0.0f) ? (-mf + 1e-8f) : 0.0f;
+}
+
+// One axis of Gaussian1DHongweiGuoInterpolator (estimateGaussianParameters
+// :104-130 and its use :190-240): samples s[NBR_MAX_HALF + d] for |d| <= half
+// at positions c + d, shifted by 'add'; returns the fitted mean and amplitude.
+// HM is a compile-time bound on half: the sample loops run over |d| <= HM
+// with a guard, so the arrays stay in registers.  The logarithms of the
+// samples are taken once (the reference recomputes the same values in every
+// iteration).
+template <int HM>
+__device__ void guo_axis(int iterations, int half, int c, const float *s, float add, float &mean,
+                         float &amp)
+{
     float xs[2 * NBR_MAX_HALF + 1], ys[2 * NBR_MAX_HALF + 1];
-    for (int d = -half; d <= half; d++) {
-        xs[d + half] = (float)(maxI + d);
-        ys[d + half] = cy[d] + add;
+    double ly[2 * NBR_MAX_HALF + 1];
+#pragma unroll
+    for (int d = -HM; d <= HM; d++) {
+        const int k = d + NBR_MAX_HALF;
+        xs[k] = (float)(c + d);
+        ys[k] = s[k] + add;
+        ly[k] = (d >= -half && d <= half) ? log((double)ys[k]) : 0.0;
     }
-    float yMean, yAmp, xMean, xAmp;
-    guo_estimate(iterations, pixels, xs, ys, yMean, yAmp);
-    for (int d = -half; d <= half; d++) {
-        xs[d + half] = (float)(maxJ + d);
-        ys[d + half] = rx[d] + add;
+    double A[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, b[3] = {0, 0, 0}, coefs[3] = {0, 0, 0};
+#pragma unroll
+    for (int d = -HM; d <= HM; d++) {
+        const int k = d + NBR_MAX_HALF;
+        if (d >= -half && d <= half) guo_accumulate(A, b, xs[k], (double)(ys[k] * ys[k]), ly[k]);
     }
-    guo_estimate(iterations, pixels, xs, ys, xMean, xAmp);
-    float fy = yMean, fx = xMean;
-    bool yValid = true, xValid = true;
-    if (fabsf(yMean - (float)maxI) > pixels / 2.0f) {
-        fy = (float)maxI;
-        yValid = false;
+    for (int it = 0; it < iterations; it++) {
+        if (!lu_solve3(A, b, coefs)) {
+            const Sol3 x = pinv_solve3(A[0][0], A[0][1], A[0][2], A[1][0], A[1][1], A[1][2],
+                                       A[2][0], A[2][1], A[2][2], b[0], b[1], b[2]);
+            coefs[0] = x.x0;
+            coefs[1] = x.x1;
+            coefs[2] = x.x2;
+        }
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+            b[r] = 0.0;
+#pragma unroll
+            for (int q = 0; q < 3; q++) A[r][q] = 0.0;
+        }
+#pragma unroll
+        for (int d = -HM; d <= HM; d++) {
+            const int k = d + NBR_MAX_HALF;
+            if (d >= -half && d <= half) {
+                const double yk = exp(coefs[0] + coefs[1] * xs[k] + coefs[2] * xs[k] * xs[k]);
+                guo_accumulate(A, b, xs[k], yk * yk, ly[k]);
+            }
+        }
     }
-    if (fabsf(xMean - (float)maxJ) > pixels / 2.0f) {
-        fx = (float)maxJ;
-        xValid = false;
+    mean = (float)(-coefs[1] / (2.0 * coefs[2]));
+    amp = (float)(exp(coefs[0] - coefs[1] * coefs[1] / (4.0 * coefs[2])));
+}
+
+// Guo fits, two lanes per peak: lane 2q fits the row axis (i), lane 2q+1 the
+// column axis (j); the pair combines through a lane exchange.  Peaks come
+// from planes in HBM (FROM_NBR false; items = n * P) or from the
+// neighbourhoods exported by the fused correlation pass (FROM_NBR true; items
+// = n, followed by the displacement update u += i' - D/2, v += j' - D/2 of
+// the multi-pass loop, Tile.accumulateDisplacement :103-136).
+template <bool FROM_NBR, int HM>
+__global__ __launch_bounds__(256) void guo_kernel(int pixels, int iterations, int items, int D,
+                                                  int P, const float *__restrict__ src,
+                                                  int nbr_half, const float *__restrict__ min_floor,
+                                                  const float *__restrict__ peaks_in,
+                                                  float *__restrict__ peaks_out,
+                                                  float *__restrict__ du, float *__restrict__ dv)
+{
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int item = gid >> 1, axis = gid & 1;
+    const bool in = item < items;
+    const int ia = FROM_NBR ? item : item / (P > 0 ? P : 1);
+    float pI = 0.0f, pJ = 0.0f, pV = 0.0f, mf = 0.0f;
+    if (in) {
+        pI = peaks_in[(size_t)item * 3 + 0];
+        pJ = peaks_in[(size_t)item * 3 + 1];
+        pV = peaks_in[(size_t)item * 3 + 2];
+        mf = min_floor[ia];
     }
-    float mx = pV;
-    if (yValid) {
-        mx = yAmp;
-        if (xValid && xAmp > mx) mx = xAmp;
-    } else if (xValid) {
-        mx = xAmp;
+    const int maxI = (int)pI, maxJ = (int)pJ;
+    const int half = pixels / 2;
+    const bool skip = (maxI == 0 && maxJ == 0) || maxI - half < 0 || maxI + half >= D ||
+                      maxJ - half < 0 || maxJ + half >= D;
+    const bool low = mf < -5.15f;
+    float mean = 0.0f, amp = 0.0f;
+    if (in && !skip && !low) {
+        const float add = (mf < 0.0f) ? (-mf + 1e-8f) : 0.0f;
+        float smp[2 * NBR_MAX_HALF + 1];
+        if (FROM_NBR) {
+            const int L = 2 * nbr_half + 1;
+            const float *nb = src + (size_t)ia * 2 * L + axis * L + nbr_half;
+#pragma unroll
+            for (int d = -HM; d <= HM; d++)
+                smp[d + NBR_MAX_HALF] = (d >= -half && d <= half) ? nb[d] : 0.0f;
+        } else {
+            const float *m = src + (size_t)ia * D * D + (size_t)maxI * D + maxJ;
+            const int stride = axis ? 1 : D;
+#pragma unroll
+            for (int d = -HM; d <= HM; d++)
+                smp[d + NBR_MAX_HALF] = (d >= -half && d <= half) ? m[d * stride] : 0.0f;
+        }
+        guo_axis<HM>(iterations, half, axis ? maxJ : maxI, smp, add, mean, amp);
     }
-    out[0] = fy;
-    out[1] = fx;
-    out[2] = mx;
-    out[3] = (float)((yValid ? ST_I : 0) | (xValid ? ST_J : 0));
+    const float xMean = __shfl_xor(mean, 1, 64), xAmp = __shfl_xor(amp, 1, 64);
+    if (!in || axis) return;
+    float res[4] = {pI, pJ, pV, (float)ST_SKIP};
+    if (!skip && low) {
+        res[3] = (float)ST_FLOOR;
+    } else if (!skip) {
+        const float yMean = mean, yAmp = amp;
+        float fy = yMean, fx = xMean;
+        bool yValid = true, xValid = true;
+        if (fabsf(yMean - (float)maxI) > pixels / 2.0f) {
+            fy = (float)maxI;
+            yValid = false;
+        }
+        if (fabsf(xMean - (float)maxJ) > pixels / 2.0f) {
+            fx = (float)maxJ;
+            xValid = false;
+        }
+        float mx = pV;
+        if (yValid) {
+            mx = yAmp;
+            if (xValid && xAmp > mx) mx = xAmp;
+        } else if (xValid) {
+            mx = xAmp;
+        }
+        res[0] = fy;
+        res[1] = fx;
+        res[2] = mx;
+        res[3] = (float)((yValid ? ST_I : 0) | (xValid ? ST_J : 0));
+    }
+    if (peaks_out) {
+        float *o = peaks_out + (size_t)item * 4;
+        o[0] = res[0];
+        o[1] = res[1];
+        o[2] = res[2];
+        o[3] = res[3];
+    }
+    if (FROM_NBR && du) {
+        const int c = D / 2;
+        du[item] = du[item] + (res[0] - (float)c);
+        dv[item] = dv[item] + (res[1] - (float)c);
+    }
 }
 
 // Fits for every (IA, peak) of planes stored in HBM.
@@ -494,8 +614,16 @@ int launch_subpixel(Context *ctx, int kind, int pixels, int iterations, int n, i
         if (rc) return rc;
     }
     const int total = n * n_peaks;
-    hipLaunchKernelGGL(subpixel_kernel, dim3((total + 255) / 256), dim3(256), 0, ctx->stream, kind,
-                       pixels, iterations, n, D, n_peaks, planes, min_floor, peaks_in, peaks_out);
+    if (kind == QLP_SUBPIX_GUO) {
+        auto k = pixels == 3 ? guo_kernel<false, 1> : pixels == 5 ? guo_kernel<false, 2>
+               : pixels == 7 ? guo_kernel<false, 3> : guo_kernel<false, NBR_MAX_HALF>;
+        hipLaunchKernelGGL(k, dim3((2 * total + 255) / 256), dim3(256), 0, ctx->stream, pixels,
+                           iterations, total, D, n_peaks, planes, 0, min_floor, peaks_in,
+                           peaks_out, nullptr, nullptr);
+    } else
+        hipLaunchKernelGGL(subpixel_kernel, dim3((total + 255) / 256), dim3(256), 0, ctx->stream,
+                           kind, pixels, iterations, n, D, n_peaks, planes, min_floor, peaks_in,
+                           peaks_out);
     return check_hip(hipGetLastError(), "subpixel kernel launch");
 }
 
@@ -510,9 +638,16 @@ int launch_subpixel_nbr(Context *ctx, int kind, int pixels, int iterations, int 
         const int half = (kind == QLP_SUBPIX_POLY) ? 1 : pixels / 2;
         if (half > nbr_half) return fail(QLP_ERR_ARG, "subpixel: neighbourhood too small");
     }
-    hipLaunchKernelGGL(subpixel_nbr_kernel, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, kind,
-                       pixels, iterations, n, D, nbr_half, nbr, min_floor, peaks_in, peaks_out, du,
-                       dv);
+    if (kind == QLP_SUBPIX_GUO) {
+        auto k = pixels == 3 ? guo_kernel<true, 1> : pixels == 5 ? guo_kernel<true, 2>
+               : pixels == 7 ? guo_kernel<true, 3> : guo_kernel<true, NBR_MAX_HALF>;
+        hipLaunchKernelGGL(k, dim3((2 * n + 255) / 256), dim3(256), 0, ctx->stream, pixels,
+                           iterations, n, D, 1, nbr, nbr_half, min_floor, peaks_in, peaks_out, du,
+                           dv);
+    } else
+        hipLaunchKernelGGL(subpixel_nbr_kernel, dim3((n + 255) / 256), dim3(256), 0, ctx->stream,
+                           kind, pixels, iterations, n, D, nbr_half, nbr, min_floor, peaks_in,
+                           peaks_out, du, dv);
     return check_hip(hipGetLastError(), "subpixel kernel launch");
 }
 

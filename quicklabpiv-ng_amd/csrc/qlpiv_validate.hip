@@ -67,9 +67,10 @@ __device__ __forceinline__ int ldc(const int *p)
 __device__ __forceinline__ int jcmp(float a, float b) { return a > b ? 1 : (a < b ? -1 : 0); }
 
 // Collections.sort of at most 8 tiles: java.util.TimSort (OpenJDK 8) takes its
-// short-array path, countRunAndMakeAscending followed by binarySort.  For
-// keys without NaN every stable sort gives this order; the exact path also
-// fixes the order when a displacement is NaN.
+// short-array path, countRunAndMakeAscending followed by binarySort.  Only
+// needed when a key is NaN (the comparator then calls NaN equal to
+// everything and the order depends on the algorithm); otherwise every stable
+// sort gives the same order and the rank form below is used.
 __device__ void timsort_small(float *key, float *a, float *b, int n)
 {
     if (n < 2) return;
@@ -103,7 +104,40 @@ __device__ void timsort_small(float *key, float *a, float *b, int n)
     }
 }
 
-// Arrays.sort(float[8]): ascending with NaN last.
+// Median of the norm-sorted neighbours through the exact TimSort path.  Kept
+// out of line and reloading the window, so that its arrays (private memory)
+// stay off the register-resident common path.
+__device__ __attribute__((noinline)) float2 median_timsort(const float *u, const float *v, int nI,
+                                                           int nJ, size_t base, int i, int j)
+{
+    float nu[8], nv[8], key[8];
+    int cnt = 0;
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) {
+            if (a == 1 && b == 1) continue;
+            const int ii = i + a - 1, jj = j + b - 1;
+            if (ii < 0 || ii >= nI || jj < 0 || jj >= nJ) continue;
+            const size_t o = base + (size_t)ii * nJ + jj;
+            nu[cnt] = ldc(u + o);
+            nv[cnt] = ldc(v + o);
+            key[cnt] = nu[cnt] * nu[cnt] + nv[cnt] * nv[cnt];
+            cnt++;
+        }
+    timsort_small(key, nu, nv, cnt);
+    float2 m = make_float2(0.0f, 0.0f);
+    if (cnt > 0) {
+        if (cnt % 2 == 0) {
+            m.x = (nu[cnt / 2 - 1] + nu[cnt / 2]) / 2.0f;
+            m.y = (nv[cnt / 2 - 1] + nv[cnt / 2]) / 2.0f;
+        } else {
+            m.x = nu[cnt / 2];
+            m.y = nv[cnt / 2];
+        }
+    }
+    return m;
+}
+
+// Arrays.sort(float[]) order: ascending with NaN after every number.
 __device__ __forceinline__ bool flt_before(float x, float y)
 {
     if (isnan(x)) return false;
@@ -112,55 +146,89 @@ __device__ __forceinline__ bool flt_before(float x, float y)
 }
 
 // NormalizedMedianValidator.validateVector :63-136: the new invalid state.
+// The eight window slots stay in registers; sorted positions are found by
+// ranks (ties by window order, which is the insertion order of the
+// reference's list, so the stable sort's order).
 __device__ bool median_invalid(const VParams &p, size_t base, int i, int j, float tu, float tv,
                                float thr, float eps)
 {
     float nu[8], nv[8], key[8];
+    bool present[8];
     int cnt = 0;
-    for (int di = -1; di <= 1; di++)
-        for (int dj = -1; dj <= 1; dj++) {
-            if (di == 0 && dj == 0) continue;
-            const int ii = i + di, jj = j + dj;
-            if (ii < 0 || ii >= p.nI || jj < 0 || jj >= p.nJ) continue;
+    bool anynan = false;
+#pragma unroll
+    for (int s = 0; s < 8; s++) {
+        const int a = s < 4 ? s / 3 : (s + 1) / 3, b = s < 4 ? s % 3 : (s + 1) % 3;
+        const int ii = i + a - 1, jj = j + b - 1;
+        present[s] = ii >= 0 && ii < p.nI && jj >= 0 && jj < p.nJ;
+        nu[s] = nv[s] = 0.0f;
+        if (present[s]) {
             const size_t o = base + (size_t)ii * p.nJ + jj;
-            nu[cnt] = ldc(p.u + o);
-            nv[cnt] = ldc(p.v + o);
-            key[cnt] = nu[cnt] * nu[cnt] + nv[cnt] * nv[cnt];
+            nu[s] = ldc(p.u + o);
+            nv[s] = ldc(p.v + o);
             cnt++;
         }
-    timsort_small(key, nu, nv, cnt);
+        key[s] = nu[s] * nu[s] + nv[s] * nv[s];
+        anynan |= present[s] && isnan(key[s]);
+    }
     float mu = 0.0f, mv = 0.0f;
-    if (cnt > 0) {
+    if (anynan) {
+        const float2 m = median_timsort(p.u, p.v, p.nI, p.nJ, base, i, j);
+        mu = m.x;
+        mv = m.y;
+    } else if (cnt > 0) {
+        const int hi = cnt / 2, lo = cnt / 2 - 1;
+        float au = 0.0f, av = 0.0f, bu = 0.0f, bv = 0.0f;
+#pragma unroll
+        for (int s = 0; s < 8; s++) {
+            int rank = 0;
+#pragma unroll
+            for (int t = 0; t < 8; t++)
+                if (t != s && present[t] && (key[t] < key[s] || (key[t] == key[s] && t < s))) rank++;
+            if (present[s] && rank == lo) {
+                au = nu[s];
+                av = nv[s];
+            }
+            if (present[s] && rank == hi) {
+                bu = nu[s];
+                bv = nv[s];
+            }
+        }
         if (cnt % 2 == 0) {
-            mu = (nu[cnt / 2 - 1] + nu[cnt / 2]) / 2.0f;
-            mv = (nv[cnt / 2 - 1] + nv[cnt / 2]) / 2.0f;
+            mu = (au + bu) / 2.0f;
+            mv = (av + bv) / 2.0f;
         } else {
-            mu = nu[cnt / 2];
-            mv = nv[cnt / 2];
+            mu = bu;
+            mv = bv;
         }
     }
+    // residuals in the reference's 8-slot array, unused slots Float.MAX_VALUE
     float r[8];
-    for (int k = 0; k < 8; k++) r[k] = FLT_MAX;
-    for (int k = 0; k < cnt; k++) {
-        const float dU = mu - nu[k], dV = mv - nv[k];
-        r[k] = dU * dU + dV * dV;
-    }
-    for (int a = 1; a < 8; a++) {  // insertion sort
-        const float x = r[a];
-        int b = a;
-        while (b > 0 && flt_before(x, r[b - 1])) {
-            r[b] = r[b - 1];
-            b--;
-        }
-        r[b] = x;
+#pragma unroll
+    for (int s = 0; s < 8; s++) {
+        const float dU = mu - nu[s], dV = mv - nv[s];
+        r[s] = present[s] ? dU * dU + dV * dV : FLT_MAX;
     }
     // the reference's index arithmetic, kept as is (C and Java both truncate
     // -1 / 2 to 0 and give -1 % 2 == -1)
     const int ri = cnt;
+    const int t1 = (ri - 1) / 2, t2 = (ri - 1) / 2 + 1;
+    float r1 = 0.0f, r2 = 0.0f;
+#pragma unroll
+    for (int s = 0; s < 8; s++) {
+        int rank = 0;
+#pragma unroll
+        for (int t = 0; t < 8; t++)
+            if (t != s && (flt_before(r[t], r[s]) ||
+                           (t < s && !flt_before(r[s], r[t]) && !flt_before(r[t], r[s]))))
+                rank++;
+        if (rank == t1) r1 = r[s];
+        if (rank == t2) r2 = r[s];
+    }
     float rMed;
     if (ri - 1 == 0) rMed = 1.0f;
-    else if ((ri - 1) % 2 == 0) rMed = (float)sqrt((double)((r[(ri - 1) / 2] + r[(ri - 1) / 2 + 1]) / 2.0f));
-    else rMed = (float)sqrt((double)r[(ri - 1) / 2 + 1]);
+    else if ((ri - 1) % 2 == 0) rMed = (float)sqrt((double)((r1 + r2) / 2.0f));
+    else rMed = (float)sqrt((double)r2);
     const float dU = mu - tu, dV = mv - tv;
     const float distance = (float)sqrt((double)(dU * dU + dV * dV)) / (rMed + eps);
     return !(distance < thr);
@@ -445,7 +513,9 @@ __device__ bool ready(const VParams &p, size_t base, int t, int sweep_start, int
     return true;
 }
 
-constexpr int SWEEP_THREADS = 1024;
+// 256 lanes: the active tiles are a few percent of the grid, and a smaller
+// group leaves the visit its registers (no spills)
+constexpr int SWEEP_THREADS = 256;
 
 __global__ __launch_bounds__(SWEEP_THREADS) void valid_sweep_kernel(VParams p)
 {

@@ -151,7 +151,8 @@ def test_from_center_bit_exact(ctx, D):
         assert mf[k] == ref[3], k
 
 
-@pytest.mark.parametrize("D,P,kw", [(31, 3, 3), (63, 3, 3), (63, 4, 5), (127, 5, 3), (10, 4, 3)])
+@pytest.mark.parametrize("D,P,kw", [(31, 3, 3), (63, 3, 3), (63, 4, 5), (127, 5, 3), (10, 4, 3),
+                                     (63, 16, 3), (127, 3, 1), (31, 1, 3)])
 def test_multipeaks_bit_exact(ctx, D, P, kw):
     import qlpiv
 
@@ -204,7 +205,8 @@ def gaussian_peak_planes(rng, n, D):
 
 
 @pytest.mark.parametrize("kind,pixels", [("gauss", 3), ("gauss", 5), ("gauss", 7), ("guo", 3),
-                                         ("guo", 5), ("guo", 7), ("poly", 3)])
+                                         ("guo", 5), ("guo", 7), ("guo", 9), ("guo", 15),
+                                         ("poly", 3)])
 def test_subpixel_vs_oracle(ctx, kind, pixels):
     import qlpiv
 
@@ -228,9 +230,12 @@ def test_subpixel_vs_oracle(ctx, kind, pixels):
         else:
             ref = O.poly(planes[k], peaks[k, 0])
         assert res[k, 0, 3] == ref[3], (k, res[k, 0], ref)
-        assert abs(res[k, 0, 0] - ref[0]) <= SUBPIX_TOL, (k, res[k, 0], ref)
-        assert abs(res[k, 0, 1] - ref[1]) <= SUBPIX_TOL, (k, res[k, 0], ref)
-        if kind == "guo":
+        # wide windows can take the log of a non-positive sample: both sides
+        # then carry NaN through the fit
+        for c in (0, 1):
+            assert (np.isnan(res[k, 0, c]) and np.isnan(ref[c])) or \
+                abs(res[k, 0, c] - ref[c]) <= SUBPIX_TOL, (k, res[k, 0], ref)
+        if kind == "guo" and not np.isnan(ref[2]):
             assert abs(res[k, 0, 2] - ref[2]) <= 1e-4 * abs(ref[2]) + 1e-3
 
 
