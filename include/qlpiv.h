@@ -228,6 +228,28 @@ int qlp_validate_vectors(qlp_ctx *ctx, int pairs, int nI, int nJ, const int *top
                          const int *peak_count, const qlp_valid_config *cfg, uint8_t *invalid,
                          int *stats);
 
+/* ---- velocity inheritance between adaptive levels ------------------------- */
+
+#define QLP_INHERIT_DIRECT 0   /* AdaptiveInterVelocityInheritanceStrategyDirect */
+#define QLP_INHERIT_AREA 1     /* AdaptiveInterVelocityInheritanceStrategyArea */
+#define QLP_INHERIT_DISTANCE 2 /* AdaptiveInterVelocityInheritanceStrategyDistanceGeneric */
+#define QLP_INHERIT_BICUBIC 3  /* AdaptiveInterVelocityInheritanceStrategyBiCubicSpline */
+
+/* IInterAreaVelocityInheritanceStrategy.reuseIterationStepTilesParameters
+ * (iareas/IInterAreaVelocityInheritanceStrategy.java) for 'pairs' frames: the
+ * child level's tiles are reset and take the parent level's displacements.
+ *   parent grid : pnI x pnJ tiles of pH x pW pixels, origins ptop/pleft (host,
+ *                 row-major, pnI*pnJ each)
+ *   child grid  : cnI x cnJ tiles of cH x cW pixels, origins ctop/cleft (host)
+ *   round       : 1 for warping modes that need integer displacements
+ *                 (WarpingModeFactoryEnum.isRequiresRounding)
+ *   pu, pv      : device, pairs x pnI*pnJ parent displacements
+ *   u, v        : device, pairs x cnI*cnJ child displacements (out) */
+int qlp_inherit_velocity_dev(qlp_ctx *ctx, int mode, int pairs, int pnI, int pnJ, int pH, int pW,
+                             const int *ptop, const int *pleft, int cnI, int cnJ, int cH, int cW,
+                             const int *ctop, const int *cleft, int round, const float *pu,
+                             const float *pv, float *u, float *v);
+
 /* ---- multi-pass PIV over device-resident frame pairs ---------------------- */
 
 /* The per-frame adaptive loop of jobs/LocalPIVOpenCLGpuJob.call :188-409 with
@@ -253,6 +275,7 @@ typedef struct qlp_piv_config {
     int n_peaks;              /* MultiPeaks numberOfPeaks (1 for FromCenter) */
     int peak_kernel_width;    /* MultiPeaks kernelWidth */
     qlp_valid_config valid;   /* vector validation after every level (n_stages 0: off) */
+    int inherit_mode;         /* QLP_INHERIT_* between levels (0: Direct) */
 } qlp_piv_config;
 
 typedef struct qlp_piv_plan qlp_piv_plan;

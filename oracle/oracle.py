@@ -24,6 +24,7 @@ _lib = None
 _f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
 _u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
 _i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
 
 
 def build():
@@ -62,8 +63,47 @@ def lib():
         L.or_validate.argtypes = [c_int, c_int, _i32p, _i32p, _f32p, _f32p, c_int, _f32p,
                                   ctypes.c_void_p, c_int, ctypes.c_void_p, c_int, c_int, c_int,
                                   _u8p, _i32p]
+        for fn in ("or_inherit_area", "or_inherit_distance", "or_inherit_bicubic"):
+            f = getattr(L, fn)
+            f.argtypes = [c_int, c_int, c_int, c_int, _i32p, _i32p, _f32p, _f32p, c_int, c_int,
+                          c_int, c_int, _i32p, _i32p, c_int, _f32p, _f32p]
+        L.or_inherit_area.restype = c_int
+        L.or_inherit_distance.restype = c_int
+        L.or_inherit_bicubic.restype = None
+        L.or_piecewise_bicubic.argtypes = [_f64p, c_int, _f64p, c_int, _f64p, ctypes.c_double,
+                                           ctypes.c_double]
+        L.or_piecewise_bicubic.restype = ctypes.c_double
         _lib = L
     return _lib
+
+
+def inherit(kind, parent, child, pu, pv, round_it=False):
+    """Velocity inheritance of one frame.  kind: "area", "distance" or
+    "bicubic"; parent / child: (top, left, tile_size) with nI x nJ origin
+    arrays; pu, pv: parent displacements (pnI x pnJ).  Returns (u, v, rc)."""
+    ptop, pleft, psz = parent
+    ctop, cleft, csz = child
+    ptop = np.ascontiguousarray(ptop, np.int32)
+    pleft = np.ascontiguousarray(pleft, np.int32)
+    ctop = np.ascontiguousarray(ctop, np.int32)
+    cleft = np.ascontiguousarray(cleft, np.int32)
+    pnI, pnJ = ptop.shape
+    cnI, cnJ = ctop.shape
+    u = np.zeros((cnI, cnJ), np.float32)
+    v = np.zeros((cnI, cnJ), np.float32)
+    f = getattr(lib(), "or_inherit_" + kind)
+    rc = f(pnI, pnJ, int(psz), int(psz), ptop, pleft, np.ascontiguousarray(pu, np.float32),
+           np.ascontiguousarray(pv, np.float32), cnI, cnJ, int(csz), int(csz), ctop, cleft,
+           int(bool(round_it)), u, v)
+    return u, v, (0 if rc is None else rc)
+
+
+def piecewise_bicubic(xval, yval, f, x, y):
+    """commons-math3 PiecewiseBicubicSplineInterpolatingFunction.value(x, y)."""
+    xval = np.ascontiguousarray(xval, np.float64)
+    yval = np.ascontiguousarray(yval, np.float64)
+    f = np.ascontiguousarray(f, np.float64)
+    return lib().or_piecewise_bicubic(xval, len(xval), yval, len(yval), f, float(x), float(y))
 
 
 class VStage(ctypes.Structure):

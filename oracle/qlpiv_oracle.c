@@ -1728,3 +1728,436 @@ void or_validate(int nI, int nJ, const int *top, const int *left, float *u, floa
     free(g->inv);
     free(g->locked);
 }
+
+/* ------------------------------------------------------------------------ */
+/* Velocity inheritance between adaptive levels                             */
+/* ------------------------------------------------------------------------ */
+
+/* Tile.accumulateDisplacement :103-136 on a reset tile (clipping allowed),
+ * with the rounding of warping modes that need integer displacements. */
+static void inherit_store(float du, float dv, int round_it, float *u, float *v)
+{
+    if (round_it) {
+        du = (float)fm_round(du);
+        dv = (float)fm_round(dv);
+    }
+    *u = 0.0f + du;
+    *v = 0.0f + dv;
+}
+
+typedef struct {
+    int parent;
+    float overlapFactor, overlapArea;
+    int moveUp, moveDown, moveLeft, moveRight;
+} or_overlap;
+
+/* AdaptiveInterVelocityInheritanceStrategyArea.checkOverlappingFactor :224-299 */
+static void or_check_overlap(int pTop, int pLeft, int pH, int pW, int cTop, int cLeft, int cH,
+                             int cW, or_overlap *r)
+{
+    int moveDown = 0, moveUp = 0, moveRight = 0, moveLeft = 0;
+    int vert = 0, horiz = 0;
+    if (pTop <= cTop) {
+        if (pTop + pH > cTop) {
+            int end = cTop + cH;
+            if (end - 1 >= pTop + pH) end = pTop + pH;
+            vert = end - cTop;
+        } else {
+            moveDown = 1;
+        }
+    } else {
+        if (cTop + cH > pTop) vert = cTop + cH - pTop;
+        else moveUp = 1;
+    }
+    if (pLeft <= cLeft) {
+        if (pLeft + pW > cLeft) {
+            int end = cLeft + cW;
+            if (end - 1 >= pLeft + pW) end = pLeft + pW;
+            horiz = end - cLeft;
+        } else {
+            moveRight = 1;
+        }
+    } else {
+        if (cLeft + cW > pLeft) horiz = cLeft + cW - pLeft;
+        else moveLeft = 1;
+    }
+    r->overlapArea = (float)(vert * horiz);
+    r->overlapFactor = r->overlapArea / (float)(cW * cH);
+    r->moveUp = moveUp;
+    r->moveDown = moveDown;
+    r->moveLeft = moveLeft;
+    r->moveRight = moveRight;
+}
+
+/* AdaptiveInterVelocityInheritanceStrategyArea.reuseIterationStepTilesParameters
+ * :47-214 for one frame.  Returns 0, or -1 where the reference throws
+ * IllegalStateException. */
+int or_inherit_area(int pnI, int pnJ, int pH, int pW, const int *ptop, const int *pleft,
+                    const float *pu, const float *pv, int cnI, int cnJ, int cH, int cW,
+                    const int *ctop, const int *cleft, int round_it, float *u, float *v)
+{
+    int parentIndexI = 0, parentIndexJ = 0;
+    int cap = 64;
+    or_overlap *areas = (or_overlap *)malloc(sizeof(or_overlap) * cap);
+#define PT(I, J) ((I) * pnJ + (J))
+    for (int indexI = 0; indexI < cnI; indexI++) {
+        parentIndexJ = 0;
+        int ref = PT(parentIndexI, parentIndexJ);
+        for (int indexJ = 0; indexJ < cnJ; indexJ++) {
+            int cur = indexI * cnJ + indexJ;
+            or_overlap ra;
+            or_check_overlap(ptop[ref], pleft[ref], pH, pW, ctop[cur], cleft[cur], cH, cW, &ra);
+            if (ra.moveUp || ra.moveLeft) goto fail;
+            if ((ra.moveRight && ra.moveLeft) || (ra.moveUp && ra.moveDown)) goto fail;
+            if (ra.moveRight) {
+                if (ra.moveDown) goto fail;
+                if (parentIndexJ + 1 == pnJ) goto fail;
+                parentIndexJ++;
+                ref = PT(parentIndexI, parentIndexJ);
+            }
+            if (ra.moveDown) {
+                if (parentIndexI + 1 == pnI) goto fail;
+                parentIndexI++;
+                ref = PT(parentIndexI, parentIndexJ);
+            }
+            float accumulatedArea = 0.0f;
+            int offsetI = 0, offsetJ = 0, maxTilesInJ = 0, endCycles = 0, resultIndex = 0;
+            do {
+                int pt = PT(parentIndexI + offsetI, parentIndexJ + offsetJ);
+                if (resultIndex >= cap) {
+                    cap *= 2;
+                    areas = (or_overlap *)realloc(areas, sizeof(or_overlap) * cap);
+                }
+                or_overlap *ca = &areas[resultIndex];
+                or_check_overlap(ptop[pt], pleft[pt], pH, pW, ctop[cur], cleft[cur], cH, cW, ca);
+                ca->parent = pt;
+                if (ca->overlapFactor == 0) {
+                    if ((ca->moveRight && ca->moveLeft) || (ca->moveUp && ca->moveDown)) goto fail;
+                    if (ca->moveRight) goto fail;
+                    if (ca->moveDown) goto fail;
+                    if (ca->moveLeft) {
+                        maxTilesInJ = offsetJ;
+                        offsetJ = 0;
+                        offsetI++;
+                        if (parentIndexI + offsetI == pnI) endCycles = 1;
+                    } else if (ca->moveUp) {
+                        if (offsetJ != 0) goto fail;
+                        endCycles = 1;
+                    }
+                } else {
+                    accumulatedArea += ca->overlapArea;
+                    resultIndex++;
+                    offsetJ++;
+                    int incrementI = 0;
+                    if (parentIndexJ + offsetJ == pnJ) incrementI = 1;
+                    if (maxTilesInJ > 0 && offsetJ == maxTilesInJ) incrementI = 1;
+                    if (incrementI) {
+                        offsetI++;
+                        offsetJ = 0;
+                        if (parentIndexI + offsetI == pnI) endCycles = 1;
+                    }
+                }
+            } while (!endCycles);
+            float du = 0.0f, dv = 0.0f;
+            for (int k = 0; k < resultIndex; k++) {
+                float weight = areas[k].overlapArea / accumulatedArea;
+                du += weight * pu[areas[k].parent];
+                dv += weight * pv[areas[k].parent];
+            }
+            inherit_store(du, dv, round_it, &u[cur], &v[cur]);
+        }
+    }
+#undef PT
+    free(areas);
+    return 0;
+fail:
+    free(areas);
+    return -1;
+}
+
+typedef struct {
+    int parent; /* -1: null */
+    float distance;
+    int isUp, isDown, isLeft, isRight;
+} or_dist;
+
+/* AdaptiveInterVelocityInheritanceStrategyDistanceGeneric.checkDistance :220-265 */
+static void or_check_distance(int pTop, int pLeft, int pH, int pW, int cTop, int cLeft, int cH,
+                              int cW, or_dist *r)
+{
+    float centerY = (float)pTop + (float)pH / 2.0f;
+    float centerX = (float)pLeft + (float)pW / 2.0f;
+    float currentCenterY = (float)cTop + (float)cH / 2.0f;
+    float currentCenterX = (float)cLeft + (float)cW / 2.0f;
+    r->isUp = centerY <= currentCenterY;
+    r->isDown = !r->isUp;
+    r->isLeft = centerX <= currentCenterX;
+    r->isRight = !r->isLeft;
+    double dx = (double)(currentCenterX - centerX), dy = (double)(currentCenterY - centerY);
+    /* FastMath.pow(d, 2) of a float operand: the exact square (representable in
+     * double, and FastMath.pow errs by less than 1 ulp) */
+    r->distance = (float)sqrt(dx * dx + dy * dy);
+}
+
+/* AdaptiveInterVelocityInheritanceStrategyDistanceGeneric.reuseIterationStepTilesParameters
+ * :30-208 for one frame.  Returns 0, or -1 where the reference throws. */
+int or_inherit_distance(int pnI, int pnJ, int pH, int pW, const int *ptop, const int *pleft,
+                        const float *pu, const float *pv, int cnI, int cnJ, int cH, int cW,
+                        const int *ctop, const int *cleft, int round_it, float *u, float *v)
+{
+    int parentIndexI = 0, parentIndexJ = 0;
+    or_dist ca[4];
+#define PT(I, J) ((I) * pnJ + (J))
+    for (int indexI = 0; indexI < cnI; indexI++) {
+        parentIndexJ = 0;
+        for (int indexJ = 0; indexJ < cnJ; indexJ++) {
+            int cur = indexI * cnJ + indexJ;
+            int more = 1;
+            while (more) {
+                int adjusted = 0;
+                int ref = PT(parentIndexI, parentIndexJ);
+                or_check_distance(ptop[ref], pleft[ref], pH, pW, ctop[cur], cleft[cur], cH, cW, &ca[0]);
+                ca[0].parent = ref;
+                if (parentIndexI + 1 < pnI && parentIndexJ + 1 < pnJ) {
+                    int dr = PT(parentIndexI + 1, parentIndexJ + 1);
+                    or_check_distance(ptop[dr], pleft[dr], pH, pW, ctop[cur], cleft[cur], cH, cW, &ca[3]);
+                    ca[3].parent = dr;
+                    if (!ca[3].isRight) {
+                        parentIndexJ++;
+                        adjusted = 1;
+                    }
+                    if (!ca[3].isDown) {
+                        parentIndexI++;
+                        adjusted = 1;
+                    }
+                    if (ca[3].isDown && ca[3].isRight) more = 0;
+                    else continue;
+                } else {
+                    ca[3].parent = -1;
+                }
+                if (parentIndexJ + 1 < pnJ) {
+                    int rt = PT(parentIndexI, parentIndexJ + 1);
+                    or_check_distance(ptop[rt], pleft[rt], pH, pW, ctop[cur], cleft[cur], cH, cW, &ca[1]);
+                    ca[1].parent = rt;
+                    if (!ca[1].isRight) {
+                        if (!more) return -1;
+                        parentIndexJ++;
+                        adjusted = 1;
+                        continue;
+                    } else {
+                        more = 0;
+                    }
+                } else {
+                    ca[1].parent = -1;
+                }
+                if (parentIndexI + 1 < pnI) {
+                    int dn = PT(parentIndexI + 1, parentIndexJ);
+                    or_check_distance(ptop[dn], pleft[dn], pH, pW, ctop[cur], cleft[cur], cH, cW, &ca[2]);
+                    ca[2].parent = dn;
+                    if (!ca[2].isDown) {
+                        if (!more) return -1;
+                        adjusted = 1;
+                        parentIndexI++;
+                        continue;
+                    } else {
+                        more = 0;
+                    }
+                } else {
+                    ca[2].parent = -1;
+                }
+                if (more && !adjusted) more = 0;
+            }
+            if (ca[0].isRight) {
+                ca[1].parent = -1;
+                ca[3].parent = -1;
+            }
+            if (ca[0].isDown) {
+                ca[2].parent = -1;
+                ca[3].parent = -1;
+            }
+            float total = 0.0f;
+            int zero = -1;
+            for (int k = 0; k < 4; k++) {
+                if (ca[k].parent >= 0) {
+                    if (ca[k].distance == 0.0f) {
+                        zero = k;
+                        break;
+                    }
+                    total += 1.0f / ca[k].distance;
+                }
+            }
+            float du = 0.0f, dv = 0.0f;
+            if (zero < 0) {
+                for (int k = 0; k < 4; k++) {
+                    if (ca[k].parent >= 0) {
+                        float weight = (1.0f / ca[k].distance) / total;
+                        du += weight * pu[ca[k].parent];
+                        dv += weight * pv[ca[k].parent];
+                    }
+                }
+            } else {
+                du = pu[ca[zero].parent];
+                dv = pv[ca[zero].parent];
+            }
+            inherit_store(du, dv, round_it, &u[cur], &v[cur]);
+        }
+    }
+#undef PT
+    return 0;
+}
+
+/* ---- commons-math3 3.6.1 AkimaSplineInterpolator + PolynomialSplineFunction,
+ *      restated from the library's published algorithm ---------------------- */
+
+/* Precision.equals(x, 0.0) (one ulp) for x >= 0 */
+static int cm_equals_zero(double x)
+{
+    uint64_t b;
+    memcpy(&b, &x, 8);
+    return b <= 1;
+}
+
+static double cm_diff3(const double *xv, const double *yv, int d, int i0, int i1, int i2)
+{
+    double x0 = yv[i0], x1 = yv[i1], x2 = yv[i2];
+    double t = xv[d] - xv[i0];
+    double t1 = xv[i1] - xv[i0];
+    double t2 = xv[i2] - xv[i0];
+    double a = (x2 - x0 - (t2 / t1 * (x1 - x0))) / (t2 * t2 - t1 * t2);
+    double b = (x1 - x0 - a * t1 * t1) / t1;
+    return (2 * a * t) + b;
+}
+
+/* Java's Arrays.binarySearch over sorted doubles (no NaN, no signed zeros in use) */
+static int cm_bsearch(const double *a, int n, double key)
+{
+    int lo = 0, hi = n - 1;
+    while (lo <= hi) {
+        int mid = (int)((unsigned)(lo + hi) >> 1);
+        double mv = a[mid];
+        if (mv < key) lo = mid + 1;
+        else if (mv > key) hi = mid - 1;
+        else return mid;
+    }
+    return -(lo + 1);
+}
+
+/* PolynomialFunction.value with the constructor's trailing-zero trim */
+static double cm_poly(const double *c, int n, double x)
+{
+    while (n > 1 && c[n - 1] == 0) n--;
+    double r = c[n - 1];
+    for (int j = n - 2; j >= 0; j--) r = x * r + c[j];
+    return r;
+}
+
+/* AkimaSplineInterpolator.interpolate(x, y) over 5 points, evaluated at v
+ * (PolynomialSplineFunction.value). */
+static double cm_akima5(const double *xv, const double *yv, double v)
+{
+    const int n = 5;
+    double diff[4], w[4], fd[5];
+    for (int i = 0; i < 4; i++) diff[i] = (yv[i + 1] - yv[i]) / (xv[i + 1] - xv[i]);
+    w[0] = 0.0;
+    for (int i = 1; i < 4; i++) w[i] = fabs(diff[i] - diff[i - 1]);
+    for (int i = 2; i < n - 2; i++) {
+        double wP = w[i + 1], wM = w[i - 1];
+        if (cm_equals_zero(wP) && cm_equals_zero(wM)) {
+            double xvv = xv[i], xvP = xv[i + 1], xvM = xv[i - 1];
+            fd[i] = (((xvP - xvv) * diff[i - 1]) + ((xvv - xvM) * diff[i])) / (xvP - xvM);
+        } else {
+            fd[i] = ((wP * diff[i - 1]) + (wM * diff[i])) / (wP + wM);
+        }
+    }
+    fd[0] = cm_diff3(xv, yv, 0, 0, 1, 2);
+    fd[1] = cm_diff3(xv, yv, 1, 0, 1, 2);
+    fd[n - 2] = cm_diff3(xv, yv, n - 2, n - 3, n - 2, n - 1);
+    fd[n - 1] = cm_diff3(xv, yv, n - 1, n - 3, n - 2, n - 1);
+    /* PolynomialSplineFunction.value: the piece holding v */
+    int i = cm_bsearch(xv, n, v);
+    if (i < 0) i = -i - 2;
+    if (i >= n - 1) i--;
+    double wd = xv[i + 1] - xv[i];
+    double w2 = wd * wd;
+    double yvv = yv[i], yvP = yv[i + 1], f0 = fd[i], fP = fd[i + 1];
+    double c[4];
+    c[0] = yvv;
+    c[1] = fd[i];
+    c[2] = (3 * (yvP - yvv) / wd - 2 * f0 - fP) / wd;
+    c[3] = (2 * (yvv - yvP) / wd + f0 + fP) / w2;
+    return cm_poly(c, 4, v - xv[i]);
+}
+
+/* PiecewiseBicubicSplineInterpolatingFunction.searchIndex (offset 2, count 5) */
+static int cm_search5(double c, const double *val, int len)
+{
+    int r = cm_bsearch(val, len, c);
+    if (r < 0) r = -r - 2 - 1;
+    else r -= 2;
+    if (r < 0) r = 0;
+    if (r + 5 >= len) r = len - 5;
+    return r;
+}
+
+/* PiecewiseBicubicSplineInterpolatingFunction.value(x, y): f[i][j] at (xval[i], yval[j]) */
+double or_piecewise_bicubic(const double *xval, int xLen, const double *yval, int yLen,
+                            const double *f, double x, double y)
+{
+    int i = cm_search5(x, xval, xLen);
+    int j = cm_search5(y, yval, yLen);
+    double xa[5], ya[5], za[5], interp[5];
+    for (int k = 0; k < 5; k++) {
+        xa[k] = xval[i + k];
+        ya[k] = yval[j + k];
+    }
+    for (int zi = 0; zi < 5; zi++) {
+        for (int k = 0; k < 5; k++) za[k] = f[(size_t)(i + k) * yLen + (j + zi)];
+        interp[zi] = cm_akima5(xa, za, x);
+    }
+    return cm_akima5(ya, interp, y);
+}
+
+/* AdaptiveInterVelocityInheritanceStrategyBiCubicSpline.reuseIterationStepTilesParameters
+ * :29-118 for one frame: parent displacements on the grid of parent tile
+ * centres, PiecewiseBicubicSplineInterpolator (5 x 5 Akima windows), child
+ * centres clamped to the parent range.  With fewer than 5 parent tiles in a
+ * direction the reference falls back to a bilinear function whose values it
+ * then never stores: the children keep zero displacement. */
+void or_inherit_bicubic(int pnI, int pnJ, int pH, int pW, const int *ptop, const int *pleft,
+                        const float *pu, const float *pv, int cnI, int cnJ, int cH, int cW,
+                        const int *ctop, const int *cleft, int round_it, float *u, float *v)
+{
+    int n = cnI * cnJ;
+    if (pnI < 5 || pnJ < 5) {
+        for (int t = 0; t < n; t++) u[t] = v[t] = 0.0f;
+        return;
+    }
+    double *us = (double *)malloc(sizeof(double) * pnI * pnJ);
+    double *vs = (double *)malloc(sizeof(double) * pnI * pnJ);
+    double *xs = (double *)malloc(sizeof(double) * pnJ);
+    double *ys = (double *)malloc(sizeof(double) * pnI);
+    for (int i = 0; i < pnI; i++)
+        for (int j = 0; j < pnJ; j++) {
+            us[i * pnJ + j] = pu[i * pnJ + j];
+            vs[i * pnJ + j] = pv[i * pnJ + j];
+            if (i == 0) xs[j] = (double)((float)pleft[j] + (float)pW / 2.0f - 0.5f);
+            if (j == 0) ys[i] = (double)((float)ptop[i * pnJ] + (float)pH / 2.0f - 0.5f);
+        }
+    float Xmin = (float)xs[0], Xmax = (float)xs[pnJ - 1];
+    float Ymin = (float)ys[0], Ymax = (float)ys[pnI - 1];
+    for (int t = 0; t < n; t++) {
+        float x = (float)cleft[t] + (float)cW / 2.0f - 0.5f;
+        float y = (float)ctop[t] + (float)cH / 2.0f - 0.5f;
+        if (x < Xmin) x = Xmin;
+        else if (x > Xmax) x = Xmax;
+        if (y < Ymin) y = Ymin;
+        else if (y > Ymax) y = Ymax;
+        float iu = (float)or_piecewise_bicubic(ys, pnI, xs, pnJ, us, (double)y, (double)x);
+        float iv = (float)or_piecewise_bicubic(ys, pnI, xs, pnJ, vs, (double)y, (double)x);
+        inherit_store(iu, iv, round_it, &u[t], &v[t]);
+    }
+    free(us);
+    free(vs);
+    free(xs);
+    free(ys);
+}

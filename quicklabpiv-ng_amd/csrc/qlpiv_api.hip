@@ -60,6 +60,24 @@ static int ensure_scratch(Context *ctx, size_t bytes)
     return QLP_OK;
 }
 
+int ensure_iwork(Context *ctx, size_t bytes)
+{
+    if (bytes <= ctx->iwork_bytes) return QLP_OK;
+    if (ctx->hbm_budget && ctx->hbm_in_use - ctx->iwork_bytes + bytes > ctx->hbm_budget)
+        return fail(QLP_ERR_BUDGET, "request exceeds the context's HBM budget");
+    if (ctx->iwork) {
+        (void)hipFree(ctx->iwork);
+        ctx->hbm_in_use -= ctx->iwork_bytes;
+        ctx->iwork = nullptr;
+        ctx->iwork_bytes = 0;
+    }
+    int rc = check_hip(hipMalloc(&ctx->iwork, bytes), "hipMalloc inheritance workspace");
+    if (rc) return rc;
+    ctx->iwork_bytes = bytes;
+    ctx->hbm_in_use += bytes;
+    return QLP_OK;
+}
+
 static int sync_ctx(Context *ctx) { return check_hip(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize"); }
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -136,6 +154,7 @@ int qlp_ctx_destroy(qlp_ctx *h)
             if (ctx->tables.tw[li][t]) (void)hipFree(ctx->tables.tw[li][t]);
     if (ctx->scratch) (void)hipFree(ctx->scratch);
     if (ctx->vwork) (void)hipFree(ctx->vwork);
+    if (ctx->iwork) (void)hipFree(ctx->iwork);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete h;
     return QLP_OK;

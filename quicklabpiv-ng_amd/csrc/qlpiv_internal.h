@@ -10,6 +10,7 @@
 #include <stdint.h>
 
 #include <string>
+#include <vector>
 
 #include "../../include/qlpiv.h"
 
@@ -51,6 +52,9 @@ struct Context {
     // workspace of qlp_validate_vectors_dev (grown on demand)
     void *vwork = nullptr;
     size_t vwork_bytes = 0;
+    // workspace of qlp_inherit_velocity_dev (grown on demand)
+    void *iwork = nullptr;
+    size_t iwork_bytes = 0;
 };
 
 void set_error(const std::string &msg);
@@ -128,5 +132,25 @@ size_t valid_work_bytes(int pairs, int n);
 void valid_work_carve(void *base, int pairs, int n, ValidLaunch &vl);
 int launch_validate(Context *ctx, const qlp_valid_config &cfg, const ValidLaunch &vl);
 int launch_accumulate(Context *ctx, int n, int P, int D, const float *sub, float *u, float *v);
+
+// Velocity inheritance (qlpiv_inherit.hip, qlpiv_pipeline.hip).  A tile grid:
+// nI x nJ tiles of H x W pixels at host-side origins top/left (row-major).
+struct InheritGrid {
+    int nI, nJ, H, W;
+    const int *top, *left;
+};
+int build_inherit_stencil(int mode, const InheritGrid &pg, const InheritGrid &cg,
+                          std::vector<int> &idx, std::vector<float> &w, int &K);
+int build_inherit_bicubic(const InheritGrid &pg, const InheritGrid &cg, std::vector<double> &ys,
+                          std::vector<double> &xs, std::vector<float> &cy, std::vector<float> &cx);
+int launch_inherit_stencil(Context *ctx, int pairs, int cn, int pn, int K, const int *idx,
+                           const float *w, int round_it, const float *pu, const float *pv, float *u,
+                           float *v);
+int launch_inherit_bicubic(Context *ctx, int pairs, int pnI, int pnJ, int cn, const double *ys,
+                           const double *xs, const float *cy, const float *cx, int round_it,
+                           const float *pu, const float *pv, float *u, float *v);
+int launch_inherit_direct(Context *ctx, int pairs, int nI, int nJ, int pI, int pJ, int round_it,
+                          const float *pu, const float *pv, float *u, float *v);
+int ensure_iwork(Context *ctx, size_t bytes);
 
 }  // namespace qlpiv

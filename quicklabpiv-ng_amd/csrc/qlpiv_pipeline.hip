@@ -4,10 +4,11 @@
 // (iareas/AdaptiveInterAreaStrategySuperPosition.java:31-197, evaluated in the
 // reference's float arithmetic) and the per-level loop of
 // jobs/LocalPIVOpenCLGpuJob.call :188-409.  Device side, per level:
-//   1. displacement state: zero at level 0, else Direct inheritance from the
-//      parent level (AdaptiveInterVelocityInheritanceStrategyDirect: child
-//      (2I+a, 2J+b) takes parent (I, J); children without a parent keep 0;
-//      NoImageWarping rounds the inherited value);
+//   1. displacement state: zero at level 0, else inheritance from the parent
+//      level: Direct (AdaptiveInterVelocityInheritanceStrategyDirect: child
+//      (2I+a, 2J+b) takes parent (I, J); children without a parent keep 0),
+//      Area, Distance or BiCubicSpline (qlpiv_inherit.hip); NoImageWarping
+//      rounds the inherited value;
 //   2. one fused pass: clip A, clip/shift/micro-warp B, correlate, FromCenter,
 //      export the peak neighbourhood (with MultiPeaks: the pass stores the
 //      planes and the peak search runs over them);
@@ -39,6 +40,12 @@ struct Level {
     void *vwork = nullptr;                // validation workspace
     uint8_t *invalid = nullptr;           // pairs * n
     int *stats = nullptr;                 // pairs * 4
+    // inheritance from the parent level (Area / Distance stencil, BiCubicSpline grid)
+    int K = 0;
+    int *sidx = nullptr;                  // n * K
+    float *sw = nullptr;                  // n * K
+    double *bys = nullptr, *bxs = nullptr;  // parent centres
+    float *bcy = nullptr, *bcx = nullptr;   // clamped child centres (n)
 };
 
 constexpr int NBR_HALF = 7;
@@ -135,7 +142,8 @@ static void plan_free(qlp_piv_plan *p)
 {
     for (auto &L : p->levels) {
         void *bufs[] = {L.top, L.left, L.u, L.v, L.peaks, L.mf, L.nbr, L.sub, L.planes,
-                        L.count, L.rank, L.vwork, L.invalid, L.stats};
+                        L.count, L.rank, L.vwork, L.invalid, L.stats, L.sidx, L.sw,
+                        L.bys, L.bxs, L.bcy, L.bcx};
         for (void *b : bufs)
             if (b) (void)hipFree(b);
     }
@@ -143,6 +151,49 @@ static void plan_free(qlp_piv_plan *p)
     p->ev.clear();
     if (p->ctx) p->ctx->hbm_in_use -= p->bytes;
     p->levels.clear();
+}
+
+// Host-side inheritance set-up of one level from its parent (stencil or spline grid).
+static int prepare_inherit(int mode, const Level &P, Level &L, size_t &bytes)
+{
+    InheritGrid pg{P.nI, P.nJ, P.N, P.N, P.top_h.data(), P.left_h.data()};
+    InheritGrid cg{L.nI, L.nJ, L.N, L.N, L.top_h.data(), L.left_h.data()};
+    const size_t n = L.n;
+    int rc;
+    if (mode == QLP_INHERIT_BICUBIC) {
+        std::vector<double> ys, xs;
+        std::vector<float> cy, cx;
+        rc = build_inherit_bicubic(pg, cg, ys, xs, cy, cx);
+        if (!rc) rc = check_hip(hipMalloc(&L.bys, ys.size() * 8), "hipMalloc plan");
+        if (!rc) rc = check_hip(hipMalloc(&L.bxs, xs.size() * 8), "hipMalloc plan");
+        if (!rc) rc = check_hip(hipMalloc(&L.bcy, n * 4), "hipMalloc plan");
+        if (!rc) rc = check_hip(hipMalloc(&L.bcx, n * 4), "hipMalloc plan");
+        if (!rc) rc = check_hip(hipMemcpy(L.bys, ys.data(), ys.size() * 8, hipMemcpyHostToDevice), "upload grid");
+        if (!rc) rc = check_hip(hipMemcpy(L.bxs, xs.data(), xs.size() * 8, hipMemcpyHostToDevice), "upload grid");
+        if (!rc) rc = check_hip(hipMemcpy(L.bcy, cy.data(), n * 4, hipMemcpyHostToDevice), "upload grid");
+        if (!rc) rc = check_hip(hipMemcpy(L.bcx, cx.data(), n * 4, hipMemcpyHostToDevice), "upload grid");
+        bytes += (ys.size() + xs.size()) * 8 + n * 8;
+        return rc;
+    }
+    std::vector<int> idx;
+    std::vector<float> w;
+    rc = build_inherit_stencil(mode, pg, cg, idx, w, L.K);
+    if (!rc) rc = check_hip(hipMalloc(&L.sidx, idx.size() * 4), "hipMalloc plan");
+    if (!rc) rc = check_hip(hipMalloc(&L.sw, w.size() * 4), "hipMalloc plan");
+    if (!rc) rc = check_hip(hipMemcpy(L.sidx, idx.data(), idx.size() * 4, hipMemcpyHostToDevice), "upload stencil");
+    if (!rc) rc = check_hip(hipMemcpy(L.sw, w.data(), w.size() * 4, hipMemcpyHostToDevice), "upload stencil");
+    bytes += idx.size() * 8;
+    return rc;
+}
+
+int qlpiv::launch_inherit_direct(Context *ctx, int pairs, int nI, int nJ, int pI, int pJ,
+                                 int round_it, const float *pu, const float *pv, float *u, float *v)
+{
+    const int pn = pairs * nI * nJ;
+    if (pn == 0) return QLP_OK;
+    hipLaunchKernelGGL(inherit_direct_kernel, dim3((pn + 255) / 256), dim3(256), 0, ctx->stream,
+                       pairs, nI, nJ, pI, pJ, round_it, pu, pv, u, v);
+    return check_hip(hipGetLastError(), "inherit kernel launch");
 }
 
 extern "C" {
@@ -192,6 +243,8 @@ int qlp_piv_plan_create(qlp_ctx *h, int H, int W, int pairs, const qlp_piv_confi
     }
     int rc = check_valid_config(cfg->valid);
     if (rc) return rc;
+    if (cfg->inherit_mode < QLP_INHERIT_DIRECT || cfg->inherit_mode > QLP_INHERIT_BICUBIC)
+        return fail(QLP_ERR_ARG, "piv_plan_create: unknown velocity inheritance strategy");
     const bool valid = cfg->valid.n_stages > 0;
     qlp_piv_plan *p = new qlp_piv_plan();
     p->ctx = ctx;
@@ -250,6 +303,8 @@ int qlp_piv_plan_create(qlp_ctx *h, int H, int W, int pairs, const qlp_piv_confi
             rc = check_hip(hipMemcpy(L.top, L.top_h.data(), n * 4, hipMemcpyHostToDevice), "upload grid");
         if (!rc)
             rc = check_hip(hipMemcpy(L.left, L.left_h.data(), n * 4, hipMemcpyHostToDevice), "upload grid");
+        if (!rc && l > 0 && cfg->inherit_mode != QLP_INHERIT_DIRECT)
+            rc = prepare_inherit(cfg->inherit_mode, p->levels[l - 1], L, need);
     }
     p->bytes = need;
     ctx->hbm_in_use += need;
@@ -308,14 +363,23 @@ int qlp_piv_run_dev(qlp_piv_plan *p, const uint8_t *imgsA, const uint8_t *imgsB)
             const size_t blocks = ((size_t)pn + 255) / 256;
             hipLaunchKernelGGL(fill_kernel, dim3(blocks), dim3(256), 0, ctx->stream, L.u, 0.0f, (size_t)pn);
             hipLaunchKernelGGL(fill_kernel, dim3(blocks), dim3(256), 0, ctx->stream, L.v, 0.0f, (size_t)pn);
-        } else {
-            const Level &P = p->levels[l - 1];
-            hipLaunchKernelGGL(inherit_direct_kernel, dim3((pn + 255) / 256), dim3(256), 0,
-                               ctx->stream, p->pairs, L.nI, L.nJ, P.nI, P.nJ,
-                               c.warp_mode == QLP_WARP_NONE ? 1 : 0, P.u, P.v, L.u, L.v);
         }
         int rc = check_hip(hipGetLastError(), "state kernel launch");
         if (rc) return rc;
+        if (l > 0) {
+            const Level &P = p->levels[l - 1];
+            const int round_it = c.warp_mode == QLP_WARP_NONE ? 1 : 0;
+            if (c.inherit_mode == QLP_INHERIT_DIRECT)
+                rc = launch_inherit_direct(ctx, p->pairs, L.nI, L.nJ, P.nI, P.nJ, round_it, P.u,
+                                           P.v, L.u, L.v);
+            else if (c.inherit_mode == QLP_INHERIT_BICUBIC)
+                rc = launch_inherit_bicubic(ctx, p->pairs, P.nI, P.nJ, L.n, L.bys, L.bxs, L.bcy,
+                                            L.bcx, round_it, P.u, P.v, L.u, L.v);
+            else
+                rc = launch_inherit_stencil(ctx, p->pairs, L.n, P.n, L.K, L.sidx, L.sw, round_it,
+                                            P.u, P.v, L.u, L.v);
+            if (rc) return rc;
+        }
         PassParams q;
         q.n = pn;
         q.N = L.N;

@@ -230,6 +230,31 @@ class Context:
         return u, v, invalid, stats
 
 
+def inherit_velocity(ctx: Context, mode, parent, child, pu, pv, round_it=False):
+    """IInterAreaVelocityInheritanceStrategy.reuseIterationStepTilesParameters on
+    device tensors.  parent / child: (top, left, tile_size) with nI x nJ origin
+    arrays; pu, pv: pairs x pnI x pnJ (or pnI x pnJ) CUDA tensors.  Returns the
+    child displacements (u, v) as CUDA tensors."""
+    import torch
+
+    ptop, pleft, psz = parent
+    ctop, cleft, csz = child
+    ptop = np.ascontiguousarray(ptop, np.int32)
+    pleft = np.ascontiguousarray(pleft, np.int32)
+    ctop = np.ascontiguousarray(ctop, np.int32)
+    cleft = np.ascontiguousarray(cleft, np.int32)
+    pnI, pnJ = ptop.shape
+    cnI, cnJ = ctop.shape
+    pairs = int(pu.numel()) // (pnI * pnJ)
+    u = torch.empty((pairs, cnI, cnJ), dtype=torch.float32, device=pu.device)
+    v = torch.empty_like(u)
+    check(ctx.lib.qlp_inherit_velocity_dev(ctx.h, int(mode), pairs, pnI, pnJ, int(psz), int(psz),
+                                           _ptr(ptop), _ptr(pleft), cnI, cnJ, int(csz), int(csz),
+                                           _ptr(ctop), _ptr(cleft), int(bool(round_it)),
+                                           _ptr(pu), _ptr(pv), _ptr(u), _ptr(v)))
+    return u, v
+
+
 class JobResultEnum(Enum):
     """The keys of the reference's job protocol used on this path
     (jobs/JobResultEnum.java:14-22)."""
@@ -364,7 +389,8 @@ class PivPlan:
     subpix_kind (QLP_SUBPIX_* or -1), subpix_pixels, subpix_iterations,
     interpolator_start_step, keep_planes, peak_kind (QLP_PEAK_*), n_peaks,
     peak_kernel_width, validation (list of stage tuples, see
-    VectorValidatorJob), valid_max_iterations, valid_until_no_more."""
+    VectorValidatorJob), valid_max_iterations, valid_until_no_more,
+    inherit_mode (QLP_INHERIT_*)."""
 
     def __init__(self, ctx: Context, H: int, W: int, pairs: int = 1, **config):
         cfg = _lib.PivConfig()
@@ -386,6 +412,7 @@ class PivPlan:
         cfg.valid = _lib.valid_config(config.get("validation") or [],
                                       config.get("valid_max_iterations", 0),
                                       config.get("valid_until_no_more", False))
+        cfg.inherit_mode = config.get("inherit_mode", _lib.QLP_INHERIT_DIRECT)
         self.P = cfg.n_peaks if cfg.peak_kind == QLP_PEAK_MULTI else 1
         self.ctx, self.H, self.W, self.pairs, self.cfg = ctx, H, W, pairs, cfg
         h = ctypes.c_void_p()

@@ -65,11 +65,18 @@ SIGNATURES = {
                                       _vp, _vp]),
     "qlp_validate_vectors": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _vp, _vp, _vp,
                                   _vp, _vp]),
+    "qlp_inherit_velocity_dev": (_i, [_vp, _i, _i, _i, _i, _i, _i, _vp, _vp, _i, _i, _i, _i,
+                                      _vp, _vp, _i, _vp, _vp, _vp, _vp]),
 }
 
 QLP_WARP_NONE = 0
 QLP_WARP_MICRO = 1
 
+
+QLP_INHERIT_DIRECT = 0
+QLP_INHERIT_AREA = 1
+QLP_INHERIT_DISTANCE = 2
+QLP_INHERIT_BICUBIC = 3
 
 QLP_VALIDATOR_MEDIAN = 0
 QLP_VALIDATOR_DIFFERENCE = 1
@@ -128,7 +135,7 @@ class PivConfig(ctypes.Structure):
                 ("subpix_iterations", ctypes.c_int), ("interpolator_start_step", ctypes.c_int),
                 ("keep_planes", ctypes.c_int), ("peak_kind", ctypes.c_int),
                 ("n_peaks", ctypes.c_int), ("peak_kernel_width", ctypes.c_int),
-                ("valid", ValidConfig)]
+                ("valid", ValidConfig), ("inherit_mode", ctypes.c_int)]
 
 _lib = None
 
