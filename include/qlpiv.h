@@ -228,6 +228,17 @@ int qlp_validate_vectors(qlp_ctx *ctx, int pairs, int nI, int nJ, const int *top
                          const int *peak_count, const qlp_valid_config *cfg, uint8_t *invalid,
                          int *stats);
 
+/* ---- image pre-filter ----------------------------------------------------- */
+
+/* GaussianFilter2D.applyFilter (images/filters/GaussianFilter2D.java:82-114,
+ * GaussianFilter2DInternal.applyFilter :31-171, DirectConvolution.convolve1D)
+ * on n_img images of H x W (device; uint8, or float when in_is_float): a
+ * separable Gaussian of kernel_px taps (<= 31) and the given sigma, rows then
+ * columns; tmp and out are device float images of the same size (tmp holds
+ * the row pass).  sigma < 1e-5 copies the input. */
+int qlp_gaussian_filter_dev(qlp_ctx *ctx, int n_img, int H, int W, const void *in,
+                            int in_is_float, float sigma, int kernel_px, float *tmp, float *out);
+
 /* ---- velocity inheritance between adaptive levels ------------------------- */
 
 #define QLP_INHERIT_DIRECT 0   /* AdaptiveInterVelocityInheritanceStrategyDirect */

@@ -73,6 +73,9 @@ def lib():
         L.or_piecewise_bicubic.argtypes = [_f64p, c_int, _f64p, c_int, _f64p, ctypes.c_double,
                                            ctypes.c_double]
         L.or_piecewise_bicubic.restype = ctypes.c_double
+        L.or_gauss_kernel.argtypes = [c_float, c_int, _f32p]
+        L.or_gaussian_filter2d.argtypes = [_f32p, c_int, c_int, _f32p, c_int, _f32p]
+        L.or_convolve1d.argtypes = [_f32p, c_int, _f32p, c_int, _f32p]
         _lib = L
     return _lib
 
@@ -104,6 +107,30 @@ def piecewise_bicubic(xval, yval, f, x, y):
     yval = np.ascontiguousarray(yval, np.float64)
     f = np.ascontiguousarray(f, np.float64)
     return lib().or_piecewise_bicubic(xval, len(xval), yval, len(yval), f, float(x), float(y))
+
+
+def gauss_kernel(sigma, kernel_px):
+    k = np.zeros(kernel_px, np.float32)
+    lib().or_gauss_kernel(float(sigma), int(kernel_px), k)
+    return k
+
+
+def convolve1d(a, b):
+    """DirectConvolution.convolve1D(a, |a|, b, |b|): the longer vector's length."""
+    a = np.ascontiguousarray(a, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    out = np.zeros(max(a.size, b.size), np.float32)
+    lib().or_convolve1d(a, a.size, b, b.size, out)
+    return out
+
+
+def gaussian_filter2d(img, sigma, kernel_px):
+    """GaussianFilter2D.applyFilter on a float image (rows, then columns)."""
+    img = np.ascontiguousarray(img, np.float32)
+    out = np.zeros_like(img)
+    k = gauss_kernel(sigma, kernel_px)
+    lib().or_gaussian_filter2d(img, img.shape[0], img.shape[1], k, int(kernel_px), out)
+    return out
 
 
 class VStage(ctypes.Structure):

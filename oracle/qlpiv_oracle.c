@@ -2161,3 +2161,83 @@ void or_inherit_bicubic(int pnI, int pnJ, int pH, int pW, const int *ptop, const
     free(xs);
     free(ys);
 }
+
+/* ------------------------------------------------------------------------ */
+/* Gaussian pre-filter: images/filters/GaussianFilter2D.java,               */
+/* GaussianFilter2DInternal.java, DirectConvolution.java                    */
+/* ------------------------------------------------------------------------ */
+
+/* GaussianFilter2D.createKernel :36-50 (FastMath.sqrt/exp/PI in double) */
+void or_gauss_kernel(float sigma, int kernelPx, float *k)
+{
+    double kd[64];
+    int half = kernelPx / 2;
+    double sum = 0.0;
+    for (int i = 0; i < kernelPx; i++) {
+        double x = i - half;
+        kd[i] = 1.0 / sqrt(2.0 * M_PI * sigma * sigma) * exp(-x * x / (2.0 * sigma * sigma));
+        sum += kd[i];
+    }
+    for (int i = 0; i < kernelPx; i++) k[i] = (float)(kd[i] / sum);
+}
+
+/* DirectConvolution.convolve1D :15-49 */
+void or_convolve1d(const float *a, int sizeA, const float *b, int sizeB, float *result)
+{
+    int minSize = sizeA, otherSize = sizeB;
+    const float *minVec = a, *otherVec = b;
+    if (sizeB < minSize) {
+        minSize = sizeB;
+        otherSize = sizeA;
+        minVec = b;
+        otherVec = a;
+    }
+    for (int i = 0; i < otherSize; i++) result[i] = 0.0f;
+    int halfSize = minSize / 2;
+    for (int i = 0; i < sizeA + sizeB - 1; i++) {
+        for (int j = 0; j < minSize; j++) {
+            if (i - j >= 0 && i - j < otherSize) {
+                if (halfSize * 2 == minSize) {
+                    if (i >= halfSize - 1 && i < otherSize + halfSize - 1)
+                        result[i - (halfSize - 1)] += otherVec[i - j] * minVec[j];
+                } else {
+                    if (i >= halfSize && i < otherSize + halfSize)
+                        result[i - halfSize] += otherVec[i - j] * minVec[j];
+                }
+            }
+        }
+    }
+}
+
+/* GaussianFilter2DInternal.applyFilter(float[][], float[][]) :120-171: rows,
+ * then columns of the row result; borders: mirrored on the leading side, the
+ * last half-kernel samples repeated in order on the trailing side. */
+void or_gaussian_filter2d(const float *in, int H, int W, const float *kernel, int K, float *out)
+{
+    int half = K / 2;
+    int nmax = (H > W ? H : W) + K - 1;
+    float *buf = (float *)malloc(sizeof(float) * nmax);
+    float *res = (float *)malloc(sizeof(float) * nmax);
+    int bs = W + K - 1;
+    for (int i = 0; i < H; i++) {
+        for (int j = 0; j < W; j++) buf[j + half] = in[(size_t)i * W + j];
+        for (int j = 0; j < half; j++) {
+            buf[half - 1 - j] = in[(size_t)i * W + j];
+            buf[bs - 1 - j] = in[(size_t)i * W + (W - 1 - j)];
+        }
+        or_convolve1d(buf, bs, kernel, K, res);
+        for (int j = 0; j < W; j++) out[(size_t)i * W + j] = res[j + half];
+    }
+    bs = H + K - 1;
+    for (int j = 0; j < W; j++) {
+        for (int i = 0; i < H; i++) buf[i + half] = out[(size_t)i * W + j];
+        for (int i = 0; i < half; i++) {
+            buf[half - 1 - i] = out[(size_t)i * W + j];
+            buf[bs - 1 - i] = out[(size_t)(H - 1 - i) * W + j];
+        }
+        or_convolve1d(buf, bs, kernel, K, res);
+        for (int i = 0; i < H; i++) out[(size_t)i * W + j] = res[i + half];
+    }
+    free(buf);
+    free(res);
+}

@@ -191,6 +191,22 @@ class Context:
                                                 _ptr(mx)))
         return tiles, mx
 
+    # ---- image pre-filter ----------------------------------------------------
+    def gaussian_filter(self, imgs, sigma, kernel_px):
+        """GaussianFilter2D.applyFilter on device images (n x H x W or H x W,
+        uint8 or float32 CUDA tensors); returns float32 images."""
+        import torch
+
+        shape = imgs.shape
+        H, W = int(shape[-2]), int(shape[-1])
+        n = int(imgs.numel()) // (H * W)
+        out = torch.empty(shape, dtype=torch.float32, device=imgs.device)
+        tmp = torch.empty(shape, dtype=torch.float32, device=imgs.device)
+        check(self.lib.qlp_gaussian_filter_dev(self.h, n, H, W, _ptr(imgs),
+                                               int(imgs.dtype == torch.float32), float(sigma),
+                                               int(kernel_px), _ptr(tmp), _ptr(out)))
+        return out
+
     # ---- vector validation ---------------------------------------------------
     def validate_vectors(self, top, left, u, v, stages, peak_uv=None, peak_count=None,
                          max_iterations=1, until_no_more=False):

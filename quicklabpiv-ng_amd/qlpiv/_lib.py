@@ -65,6 +65,7 @@ SIGNATURES = {
                                       _vp, _vp]),
     "qlp_validate_vectors": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _vp, _vp, _vp,
                                   _vp, _vp]),
+    "qlp_gaussian_filter_dev": (_i, [_vp, _i, _i, _i, _vp, _i, ctypes.c_float, _i, _vp, _vp]),
     "qlp_inherit_velocity_dev": (_i, [_vp, _i, _i, _i, _i, _i, _i, _vp, _vp, _i, _i, _i, _i,
                                       _vp, _vp, _i, _vp, _vp, _vp, _vp]),
 }

@@ -21,6 +21,10 @@ Fixtures (data only -- inputs and expected outputs):
 * multipeak_kat.json      -- the 10 x 10 plane and expectations of
                              tests/.../maximum/FindMaximumMultiPeakTests.java:25-160.
 * fastreal_kat.json       -- the 4 x 4 tiles of tests/.../xcorr/FastRealFFTXCorrTests.java:31-51.
+* gauss_filter_golden.npz -- testFiles/GaussianFilter2D_{3px_2.0f,5px_0.48f}_rankine_vortex01_*.matFloat
+                             (GaussianFilter2DTest.java:31-85, tolerances 4e-5 / 1e-4): the
+                             filtered rankine_vortex01_0/1 images, kept as the 16-pixel border
+                             strips (where the padding rules act) and a central 64 x 64 block.
 
 GPL-3.0 data from QuickLabPIV-ng (Luis Mendes); attribution kept here.
 """
@@ -154,6 +158,18 @@ def main(ref):
         json.dump(multipeak_kat(), f)
     with open(os.path.join(HERE, "fastreal_kat.json"), "w") as f:
         json.dump(fastreal_kat(), f)
+    gauss = {}
+    for key, fn in (("g3_2p0", "GaussianFilter2D_3px_2.0f_rankine_vortex01_0.matFloat"),
+                    ("g5_0p48", "GaussianFilter2D_5px_0.48f_rankine_vortex01_0_1.matFloat")):
+        _, mats = read_matfloat(os.path.join(ref, "testFiles", fn))
+        for k, m in enumerate(mats):
+            assert m.shape == (512, 512)
+            gauss[f"{key}_{k}_top"] = m[:16]
+            gauss[f"{key}_{k}_bottom"] = m[-16:]
+            gauss[f"{key}_{k}_left"] = m[:, :16]
+            gauss[f"{key}_{k}_right"] = m[:, -16:]
+            gauss[f"{key}_{k}_centre"] = m[224:288, 224:288]
+    np.savez_compressed(os.path.join(HERE, "gauss_filter_golden.npz"), **gauss)
     print("wrote fixtures to", HERE)
 
 
