@@ -265,8 +265,8 @@ int qlp_inherit_velocity_dev(qlp_ctx *ctx, int mode, int pairs, int pnI, int pnJ
 
 /* The per-frame adaptive loop of jobs/LocalPIVOpenCLGpuJob.call :188-409 with
  * the super-position IA strategy (iareas/AdaptiveInterAreaStrategySuperPosition
- * .java:31-197), Direct velocity inheritance
- * (AdaptiveInterVelocityInheritanceStrategyDirect.java), the chosen warping,
+ * .java:31-197), the chosen velocity inheritance between levels (Direct by
+ * default, AdaptiveInterVelocityInheritanceStrategyDirect.java), warping,
  * peak search and sub-pixel fit, the displacement update
  * (IterationStepTiles.updateDisplacementsFromMaxCrossResults :317-371) and the
  * optional vector validation (:395-403). */
