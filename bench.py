@@ -79,12 +79,51 @@ def cpu_baseline(a, b, top, left, u, v, N, nI, nJ, stages):
                       f"validation (C restatement, -O2 -ffp-contract=off, 1 thread), {dt:.2f} s"}
 
 
+def time_plan(ctx, plan, da, db, steps, warmup):
+    """Run warmup untimed steps, then time exactly `steps` steps (host clock
+    around the whole run, HIP events per step on the library's stream), then
+    per-level kernel times over five more runs (the plan's own events)."""
+    import torch
+
+    s = ctx.torch_stream()
+    for _ in range(warmup):
+        plan.run(da, db)
+    ctx.synchronize()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for e0, e1 in evs:
+        e0.record(s)
+        plan.run(da, db)
+        e1.record(s)
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    step_ms = np.array([e0.elapsed_time(e1) for e0, e1 in evs])
+    plan.set_timing(True)
+    per_level = []
+    for _ in range(5):
+        plan.run(da, db)
+        per_level.append(plan.timing())
+    plan.set_timing(False)
+    return dt, step_ms, np.mean(np.array(per_level), axis=0)  # levels x (pass, fit, validation)
+
+
+def level_ms(plan, per_level):
+    return {str(plan.levels[k]["ia"]): {"pass": round(float(per_level[k][0]), 4),
+                                        "peaks_fit": round(float(per_level[k][1]), 4),
+                                        "validation": round(float(per_level[k][2]), 4)}
+            for k in range(len(plan.levels))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-mfma", action="store_true", help="skip the matrix-core form's line")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (one MI355X per box); "
@@ -104,44 +143,41 @@ def main():
 
     ctx = qlpiv.Context(0)
     stages = qlpiv._lib.multipeak_median_stages(3, 2.0, 0.1)
-    plan = qlpiv.PivPlan(ctx, H, W, 1, ia_start=64, ia_end=16, overlap_factor=0.5,
-                         warp_mode=qlpiv._lib.QLP_WARP_MICRO, norm_mode=qlpiv.QLP_NORM_PER_TILE,
-                         subpix_kind=qlpiv.QLP_SUBPIX_GUO, subpix_pixels=3, subpix_iterations=20,
-                         keep_planes=2, peak_kind=qlpiv.QLP_PEAK_MULTI, n_peaks=3,
-                         peak_kernel_width=3, validation=stages, valid_max_iterations=0)
+    c2 = dict(ia_start=64, ia_end=16, overlap_factor=0.5, warp_mode=qlpiv._lib.QLP_WARP_MICRO,
+              norm_mode=qlpiv.QLP_NORM_PER_TILE, subpix_kind=qlpiv.QLP_SUBPIX_GUO,
+              subpix_pixels=3, subpix_iterations=20, keep_planes=2,
+              peak_kind=qlpiv.QLP_PEAK_MULTI, n_peaks=3, peak_kernel_width=3,
+              validation=stages, valid_max_iterations=0)
+    plan = qlpiv.PivPlan(ctx, H, W, 1, **c2)
     n_levels = [L["nI"] * L["nJ"] for L in plan.levels]
     n_step = sum(n_levels)
-    s = ctx.torch_stream()
-
-    for _ in range(args.warmup):
-        plan.run(da, db)
-    ctx.synchronize()
-    print(f"[bench] warmup done; timing {args.steps} steps of {n_step} IAs", file=sys.stderr,
-          flush=True)
-    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-           for _ in range(args.steps)]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for e0, e1 in evs:
-        e0.record(s)
-        plan.run(da, db)
-        e1.record(s)
-    ctx.synchronize()
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    step_ms = np.array([e0.elapsed_time(e1) for e0, e1 in evs])
-
-    # per-level kernel time over a few more runs (event-synchronised)
-    plan.set_timing(True)
-    per_level = []
-    for _ in range(5):
-        plan.run(da, db)
-        per_level.append(plan.timing())
-    plan.set_timing(False)
-    per_level = np.mean(np.array(per_level), axis=0)  # levels x (pass, fit, validation)
+    print(f"[bench] exact FP32 form: {args.warmup} warmup + {args.steps} timed steps of "
+          f"{n_step} IAs", file=sys.stderr, flush=True)
+    dt, step_ms, per_level = time_plan(ctx, plan, da, db, args.steps, args.warmup)
     lv32 = [L["ia"] for L in plan.levels].index(32)
     pass32_ms = float(per_level[lv32][0])
     achieved = pass_bytes(n_levels[lv32], 32) / (pass32_ms * 1e-3) / 1e9
+    ue, ve, _ = plan.result(len(plan.levels) - 1)
+
+    mfma = None
+    if not args.no_mfma:
+        print("[bench] MFMA form (fp16 hi+lo, 16/32 px levels)", file=sys.stderr, flush=True)
+        plan_m = qlpiv.PivPlan(ctx, H, W, 1, xcorr_form=qlpiv.QLP_XCORR_MFMA_F16, **c2)
+        dt_m, step_m, per_m = time_plan(ctx, plan_m, da, db, args.steps, args.warmup)
+        um, vm, _ = plan_m.result(len(plan_m.levels) - 1)
+        d = np.hypot(um - ue, vm - ve).ravel()
+        plan_m.close()
+        mfma = {"form": "MFMA fp16 hi+lo operands, fp32 accumulation (64 px level exact)",
+                "value": round(n_step * args.steps / dt_m, 1), "unit": "IAs/s",
+                "ms_per_step": round(dt_m / args.steps * 1e3, 3),
+                "pair_latency_ms": {"p50": round(float(np.percentile(step_m, 50)), 3),
+                                    "p99": round(float(np.percentile(step_m, 99)), 3)},
+                "level_ms": level_ms(plan, per_m),
+                "ias_per_s_at_32x32": round(n_levels[lv32] / (float(per_m[lv32].sum()) * 1e-3), 1),
+                "speedup_vs_exact": round(dt / dt_m, 3),
+                "final_field_vs_exact_px": {"median": round(float(np.median(d)), 6),
+                                            "p99": round(float(np.percentile(d, 99)), 6),
+                                            "frac_within_0.01": round(float((d <= 0.01).mean()), 5)}}
 
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "xcorr_pass_c2_traffic.json")
@@ -190,11 +226,9 @@ def main():
                    "ias_per_step": n_step, "ias_per_level": n_levels,
                    "pair_latency_ms": {"p50": round(float(np.percentile(step_ms, 50)), 3),
                                        "p99": round(float(np.percentile(step_ms, 99)), 3)},
-                   "level_ms": {str(plan.levels[k]["ia"]): {"pass": round(float(per_level[k][0]), 4),
-                                                             "peaks_fit": round(float(per_level[k][1]), 4),
-                                                             "validation": round(float(per_level[k][2]), 4)}
-                                for k in range(len(n_levels))},
+                   "level_ms": level_ms(plan, per_level),
                    "ias_per_s_at_32x32": round(n_levels[lv32] / (float(per_level[lv32].sum()) * 1e-3), 1)},
+        "mfma_form": mfma,
         "roofline": {"bound": "hbm", "kernel": "xcorr_pass_kernel<6,false,true,false> (32x32 level)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,

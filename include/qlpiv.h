@@ -103,6 +103,24 @@ int qlp_xcorr_batch(qlp_ctx *ctx, int n, int N, const uint8_t *tilesA, const uin
 int qlp_xcorr_batch_dev(qlp_ctx *ctx, int n, int N, const uint8_t *tilesA, const uint8_t *tilesB,
                         const float *maxA, const float *maxB, int mode, float *planes_out);
 
+/* Arithmetic form of the correlation.  QLP_XCORR_EXACT repeats the reference's
+ * float FFT operations (planes value-identical to the Java job).  The MFMA forms
+ * compute the same correlation as dense DFT-basis products on the matrix cores
+ * in fp16 or bf16 with fp32 accumulation, with split hi + lo operands (or single
+ * operands): the throughput form, accurate to the stated error against an FP64
+ * correlation but not value-identical.  IA sizes 16 and 32. */
+#define QLP_XCORR_EXACT 0
+#define QLP_XCORR_MFMA_F16 1         /* fp16 hi + lo operands */
+#define QLP_XCORR_MFMA_BF16 2        /* bf16 hi + lo operands */
+#define QLP_XCORR_MFMA_F16_SINGLE 3  /* fp16 operands */
+#define QLP_XCORR_MFMA_BF16_SINGLE 4 /* bf16 operands */
+
+/* qlp_xcorr_batch_dev in the chosen form (same arguments and layouts).  No
+ * counterpart in the reference: the matrix-core form of the correlation job. */
+int qlp_xcorr_form_dev(qlp_ctx *ctx, int form, int n, int N, const uint8_t *tilesA,
+                       const uint8_t *tilesB, const float *maxA, const float *maxB, int mode,
+                       float *planes_out);
+
 /* ---- integer peak search -------------------------------------------------- */
 
 /* IMaximumFinder.findMaximum(Matrix) (maximum/IMaximumFinder.java:16-17) over n
@@ -287,6 +305,8 @@ typedef struct qlp_piv_config {
     int peak_kernel_width;    /* MultiPeaks kernelWidth */
     qlp_valid_config valid;   /* vector validation after every level (n_stages 0: off) */
     int inherit_mode;         /* QLP_INHERIT_* between levels (0: Direct) */
+    int xcorr_form;           /* QLP_XCORR_* arithmetic of the 16 and 32 px levels (64 px
+                                 levels always use QLP_XCORR_EXACT) */
 } qlp_piv_config;
 
 typedef struct qlp_piv_plan qlp_piv_plan;

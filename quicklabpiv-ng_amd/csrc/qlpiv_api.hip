@@ -155,6 +155,9 @@ int qlp_ctx_destroy(qlp_ctx *h)
     if (ctx->scratch) (void)hipFree(ctx->scratch);
     if (ctx->vwork) (void)hipFree(ctx->vwork);
     if (ctx->iwork) (void)hipFree(ctx->iwork);
+    for (int a = 0; a < 2; a++)
+        for (int b = 0; b < 2; b++)
+            if (ctx->mfma_tab[a][b]) (void)hipFree(ctx->mfma_tab[a][b]);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete h;
     return QLP_OK;
@@ -188,6 +191,30 @@ int qlp_xcorr_batch_dev(qlp_ctx *h, int n, int N, const uint8_t *tA, const uint8
     if (n > 0 && (!tA || !tB || !maxA || !planes || (mode == QLP_NORM_PER_TILE && !maxB)))
         return fail(QLP_ERR_ARG, "xcorr: null buffer");
     return launch_xcorr(&h->c, n, N, tA, tB, maxA, maxB, mode, planes);
+}
+
+int qlp_xcorr_form_dev(qlp_ctx *h, int form, int n, int N, const uint8_t *tA, const uint8_t *tB,
+                       const float *maxA, const float *maxB, int mode, float *planes)
+{
+    if (form == QLP_XCORR_EXACT) return qlp_xcorr_batch_dev(h, n, N, tA, tB, maxA, maxB, mode, planes);
+    if (!h) return fail(QLP_ERR_ARG, "null context");
+    int rc = xcorr_args(n, N, mode);
+    if (rc) return rc;
+    if (form < QLP_XCORR_MFMA_F16 || form > QLP_XCORR_MFMA_BF16_SINGLE)
+        return fail(QLP_ERR_ARG, "xcorr: unknown arithmetic form");
+    if (N != 16 && N != 32) return fail(QLP_ERR_UNSUPPORTED, "xcorr: the MFMA forms take IA sizes 16 and 32");
+    if (n > 0 && (!tA || !tB || !maxA || !planes || (mode == QLP_NORM_PER_TILE && !maxB)))
+        return fail(QLP_ERR_ARG, "xcorr: null buffer");
+    PassParams p;
+    p.n = n;
+    p.N = N;
+    p.mode = mode;
+    p.tA = tA;
+    p.tB = tB;
+    p.maxA = maxA;
+    p.maxB = mode == QLP_NORM_PER_TILE ? maxB : maxA;
+    p.planes = planes;
+    return launch_pass_mfma(&h->c, p, form, false, false);
 }
 
 int qlp_xcorr_batch(qlp_ctx *h, int n, int N, const uint8_t *tA, const uint8_t *tB,

@@ -55,6 +55,8 @@ struct Context {
     // workspace of qlp_inherit_velocity_dev (grown on demand)
     void *iwork = nullptr;
     size_t iwork_bytes = 0;
+    // constant DFT fragments of the MFMA correlation form, [N 16/32][fp16/bf16]
+    void *mfma_tab[2][2] = {};
 };
 
 void set_error(const std::string &msg);
@@ -90,6 +92,7 @@ struct PassParams {
     float *peaks = nullptr, *min_floor = nullptr;
     float *nbr = nullptr;
     int nbr_half = 0;
+    int arith = QLP_XCORR_EXACT;  // QLP_XCORR_*: arithmetic form of the correlation
 };
 
 // Launchers (qlpiv_xcorr.hip, qlpiv_peaks.hip, qlpiv_subpixel.hip, qlpiv_warp.hip).
@@ -97,6 +100,8 @@ int launch_xcorr(Context *ctx, int n, int N, const uint8_t *tA, const uint8_t *t
                  const float *maxA, const float *maxB, int mode, float *planes);
 int launch_pass(Context *ctx, const PassParams &p);
 int launch_pass_lane(Context *ctx, const PassParams &p, bool from_image, bool peak);
+// The throughput form (qlpiv_xcorr_mfma.hip): form is QLP_XCORR_MFMA_*.
+int launch_pass_mfma(Context *ctx, const PassParams &p, int form, bool from_image, bool peak);
 int launch_find_peaks(Context *ctx, int kind, int n, int D, const float *planes, int n_peaks,
                       int kernel_width, float *peaks, float *min_floor, int *count = nullptr);
 int launch_subpixel(Context *ctx, int kind, int pixels, int iterations, int n, int D, int n_peaks,

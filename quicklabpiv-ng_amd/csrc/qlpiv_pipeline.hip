@@ -245,6 +245,8 @@ int qlp_piv_plan_create(qlp_ctx *h, int H, int W, int pairs, const qlp_piv_confi
     if (rc) return rc;
     if (cfg->inherit_mode < QLP_INHERIT_DIRECT || cfg->inherit_mode > QLP_INHERIT_BICUBIC)
         return fail(QLP_ERR_ARG, "piv_plan_create: unknown velocity inheritance strategy");
+    if (cfg->xcorr_form < QLP_XCORR_EXACT || cfg->xcorr_form > QLP_XCORR_MFMA_BF16_SINGLE)
+        return fail(QLP_ERR_ARG, "piv_plan_create: unknown correlation form");
     const bool valid = cfg->valid.n_stages > 0;
     qlp_piv_plan *p = new qlp_piv_plan();
     p->ctx = ctx;
@@ -395,6 +397,7 @@ int qlp_piv_run_dev(qlp_piv_plan *p, const uint8_t *imgsA, const uint8_t *imgsB)
         q.dv = (l == 0) ? nullptr : L.v;
         q.bmode = c.warp_mode == QLP_WARP_MICRO ? B_WARP_MICRO : B_CLIP_SHIFT;
         q.planes = L.planes;
+        q.arith = c.xcorr_form;
         if (!multi) {  // FromCenter and the fit neighbourhood fused into the pass
             q.peaks = L.peaks;
             q.min_floor = L.mf;

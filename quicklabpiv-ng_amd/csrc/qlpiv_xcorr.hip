@@ -463,6 +463,9 @@ int launch_pass(Context *ctx, const PassParams &p)
     // N = 16 (0.293 vs 0.323 ms for 65,025 IAs), the LDS-exchange form for
     // N = 32 (0.381 vs 0.436 ms for 16,129 IAs) and N = 64, where more
     // wavefronts per IA hide latency better
+    // the matrix-core forms cover N = 16 and 32; N = 64 keeps the exact form
+    if (p.arith != QLP_XCORR_EXACT && (p.N == 16 || p.N == 32))
+        return launch_pass_mfma(ctx, p, p.arith, from_image, peak);
     if (ctx->xcorr_form == FORM_LANE && p.N == 16) return launch_pass_lane(ctx, p, from_image, peak);
     switch (p.N) {
     case 16: launch_pass_m<5>(ctx, p, from_image, peak); break;

@@ -1,0 +1,761 @@
+// Correlation on the matrix cores: the throughput form of the correlation pass.
+//
+// The plane of one IA pair is the linear cross-correlation of the normalised
+// tiles a, g (N x N):  R(p, q) = sum_x a(x) g(x + (p - (N-1), q - (N-1))),
+// which the reference obtains by zero-padded FFTs of length M = 2N
+// (CrossCorrelationRealFFTParallelKernel.java:214-713).  Here it is computed as
+// dense products with a real trigonometric basis of M functions
+//     h_r(x) = cos(2 pi r x / M)            r <  M/2
+//     h_r(x) = cos(pi x)                    r == M/2
+//     h_r(x) = sin(2 pi (r - M/2) x / M)   r >  M/2
+// forward  E  = Hf a Hf^T   (Hf: M x N, the basis at x = 0 .. N-1),
+// spectrum C  = the real-basis coefficients of conj(A^) G^ / M^2, formed
+//               elementwise from the four (cos/sin x cos/sin) terms of every
+//               frequency pair (k, l), (k, -l) of E_a and E_g,
+// inverse  R  = Hout^T C Hout  (Hout: M x M, the basis at the lags p - (N-1)).
+// That is 2 (M N^2 + M^2 N) + 2 M^3 multiply-adds per IA (0.92 M at N = 32),
+// on MFMA in fp16 or bf16 with fp32 accumulation.  Split precision writes every
+// operand as hi + lo (lo = x - hi, itself rounded) and accumulates
+// hi*hi + hi*lo + lo*hi.  Power-of-two scalings keep fp16 operands in range;
+// they are exact and undone on the fp32 result.
+//
+// One wavefront owns one IA.  The chain is laid out so that an MFMA result
+// feeds the next product as its B operand straight from registers: the K index
+// of every product follows the row order of the accumulator layout (kphys),
+// and the constant A operands (Hf, Hout^T) are stored in that order.  Only the
+// two transposes of the 2D transforms (after Hf a, after Hout^T C^T) go
+// through LDS.  The constant fragments are staged in LDS once per work-group.
+// Operand fragments are kept as four packed 32-bit words (two 16-bit values
+// each), converted in pairs.
+#include "qlpiv_device.h"
+#include "qlpiv_internal.h"
+
+#include <math.h>
+#include <string.h>
+
+#include <type_traits>
+#include <vector>
+
+namespace qlpiv {
+
+typedef _Float16 mh8 __attribute__((ext_vector_type(8)));
+typedef __bf16 mb8 __attribute__((ext_vector_type(8)));
+typedef float mf16 __attribute__((ext_vector_type(16)));
+typedef float mf4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u4 __attribute__((ext_vector_type(4)));  // one operand fragment
+typedef unsigned int u2 __attribute__((ext_vector_type(2)));
+
+constexpr int MF_WAVES = 4;  // wavefronts (IAs in flight) per work-group
+
+template <int N>
+struct Mf {
+    static constexpr int M = 2 * N, D = M - 1;
+    static constexpr int MT = (M == 32) ? 16 : 32;  // MFMA tile: 16x16x32 or 32x32x16
+    static constexpr int KB = (MT == 32) ? 16 : 32;  // K of one MFMA
+    static constexpr int NS = MT * MT / 64;          // accumulator slots per lane
+    static constexpr int RT = M / MT;                // tiles per matrix side (2)
+    static constexpr int KN = (N + KB - 1) / KB;     // K blocks over the tile size
+    static constexpr int KM = M / KB;                // K blocks over the transform size
+    static constexpr int SV = 36;                    // LDS row stride of a V tile (halves)
+    static constexpr int SY = (MT == 32) ? 20 : 36;  // LDS row stride of a Y block
+    static constexpr int YCT = (MT == 32) ? 1 : 2;   // Y column tiles per K block
+    static constexpr int XS = (MT * SV > M * SY) ? MT * SV : M * SY;  // per-wave LDS (halves)
+    static constexpr int NF_HF = RT * KN;            // Hf fragments
+    static constexpr int NF_HO = RT * KM;            // Hout^T fragments
+    static constexpr int NF = NF_HF + NF_HO;
+    // fp16 range: |a| <= 17, |E| <= 17 N^2 (kept in fp32), |C| <= 1156 N^2
+    // before 2^-E1, |Y| <= M max|C| before 2^-E2: both stay below 2^15
+    // (bounds of the normalised data).  Scaling C by the product of both
+    // instead pushes its small coefficients into fp16 subnormals (measured:
+    // 2e-6 instead of 7e-7 of the plane maximum).
+    static constexpr int E1 = (N == 16) ? 4 : 6;
+    static constexpr int E2 = (M == 32) ? 5 : 6;
+    static_assert(RT == 2, "the quad layout needs 2 x 2 tiles");
+};
+
+template <int MT>
+using Acc = typename std::conditional<MT == 32, mf16, mf4>::type;
+
+// accumulator row of slot v for lane group g
+template <int MT>
+__host__ __device__ constexpr int mrow(int v, int g)
+{
+    return MT == 32 ? 8 * (v >> 2) + 4 * g + (v & 3) : 4 * g + v;
+}
+
+// K index held in element s of a B (and A) fragment of K block kb, lane group g
+template <int MT>
+__host__ __device__ constexpr int kphys(int kb, int g, int s)
+{
+    return MT == 32 ? 16 * kb + 8 * (s >> 2) + 4 * g + (s & 3)
+                    : 32 * kb + 16 * (s >> 2) + 4 * g + (s & 3);
+}
+
+// An accumulator tile used as a B operand: slot v of row tile rt is element s
+// of K block kb (the row it holds is kphys(kb, g, s)).
+template <int MT>
+__host__ __device__ constexpr int acc_kb(int rt, int v)
+{
+    return MT == 32 ? 2 * rt + (v >> 3) : rt >> 1;
+}
+template <int MT>
+__host__ __device__ constexpr int acc_s(int rt, int v)
+{
+    return MT == 32 ? (v & 7) : 4 * (rt & 1) + v;
+}
+
+template <bool BF, int MT>
+__device__ __forceinline__ Acc<MT> mma(u4 a, u4 b, Acc<MT> c)
+{
+    if constexpr (MT == 32) {
+        if constexpr (BF)
+            return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(mb8, a),
+                                                           __builtin_bit_cast(mb8, b), c, 0, 0, 0);
+        else
+            return __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(mh8, a),
+                                                          __builtin_bit_cast(mh8, b), c, 0, 0, 0);
+    } else {
+        if constexpr (BF)
+            return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(mb8, a),
+                                                           __builtin_bit_cast(mb8, b), c, 0, 0, 0);
+        else
+            return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(mh8, a),
+                                                          __builtin_bit_cast(mh8, b), c, 0, 0, 0);
+    }
+}
+
+// x = hi + lo in the operand type (lo = x - hi, rounded), as 16-bit patterns
+template <bool BF>
+__device__ __forceinline__ void split1(float x, unsigned &h, unsigned &l)
+{
+    if constexpr (BF) {
+        const __bf16 hh = (__bf16)x;
+        h = __builtin_bit_cast(unsigned short, hh);
+        l = __builtin_bit_cast(unsigned short, (__bf16)(x - (float)hh));
+    } else {
+        const _Float16 hh = (_Float16)x;
+        h = __builtin_bit_cast(unsigned short, hh);
+        l = __builtin_bit_cast(unsigned short, (_Float16)(x - (float)hh));
+    }
+}
+
+typedef _Float16 mh2 __attribute__((ext_vector_type(2)));
+typedef __bf16 mb2 __attribute__((ext_vector_type(2)));
+typedef float mf2 __attribute__((ext_vector_type(2)));
+
+// two values into packed hi and lo words (x0 in the low half): packed
+// conversions, three instructions per value
+template <bool BF>
+__device__ __forceinline__ void split2(float x0, float x1, unsigned &h, unsigned &l)
+{
+    const mf2 x = {x0, x1};
+    if constexpr (BF) {
+        const mb2 hi = __builtin_convertvector(x, mb2);
+        const mb2 lo = __builtin_convertvector(x - __builtin_convertvector(hi, mf2), mb2);
+        h = __builtin_bit_cast(unsigned, hi);
+        l = __builtin_bit_cast(unsigned, lo);
+    } else {
+        const mh2 hi = __builtin_convertvector(x, mh2);
+        const mh2 lo = __builtin_convertvector(x - __builtin_convertvector(hi, mf2), mh2);
+        h = __builtin_bit_cast(unsigned, hi);
+        l = __builtin_bit_cast(unsigned, lo);
+    }
+}
+
+// acc += A B (split: Ah Bh + Ah Bl + Al Bh)
+template <bool BF, bool SPLIT, int MT>
+__device__ __forceinline__ Acc<MT> mma3(u4 ah, u4 al, u4 bh, u4 bl, Acc<MT> c)
+{
+    if constexpr (SPLIT) {
+        c = mma<BF, MT>(al, bh, c);
+        c = mma<BF, MT>(ah, bl, c);
+    }
+    return mma<BF, MT>(ah, bh, c);
+}
+
+// B fragment from an LDS matrix stored [j][k] (row j = the fragment column,
+// stride S halves): element s holds column kphys(kb, g, s) - koff; a group of
+// four columns at or beyond kmax is 0.  The four columns of a group, for every
+// lane group g, stay inside one aligned block of 16 (MT 16) or 8 (MT 32)
+// columns and kmax is a multiple of 16, so whether a group is in range does
+// not depend on g: the test is made on compile-time values.
+template <int MT, int S, int KB_, int KOFF, int KMAX>
+__device__ __forceinline__ u4 bfrag_from_lds(const unsigned short *__restrict__ base, int row,
+                                             int g)
+{
+    constexpr int b0 = kphys<MT>(KB_, 0, 0) - KOFF, b1 = kphys<MT>(KB_, 0, 4) - KOFF;
+    u2 x0 = {0u, 0u}, x1 = {0u, 0u};
+    if constexpr (b0 < KMAX) x0 = *(const u2 *)(base + row * S + b0 + 4 * g);
+    if constexpr (b1 < KMAX) x1 = *(const u2 *)(base + row * S + b1 + 4 * g);
+    return (u4){x0[0], x0[1], x1[0], x1[1]};
+}
+
+// Orders this wavefront's LDS writes before its later LDS reads by other lanes
+// of the same wavefront.  A wavefront's LDS instructions execute in issue
+// order, so only the compiler has to be kept from moving accesses across (the
+// fences); the per-wave LDS regions need no work-group barrier.
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// f(integral_constant<I>) for I = B .. E-1
+template <int B, int E, typename F>
+__device__ __forceinline__ void static_for(F &&f)
+{
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>{});
+        static_for<B + 1, E>(f);
+    }
+}
+
+struct MfmaParams {
+    PassParams p;
+    const u4 *frags;  // NF fragments x {hi, lo} x 64 lanes
+};
+
+template <int N, bool BF, bool SPLIT, bool FROM_IMAGE, bool PEAK>
+__global__ __launch_bounds__(64 * MF_WAVES, 3) void xcorr_mfma_kernel(MfmaParams mp, int n_iter)
+{
+    using X = Mf<N>;
+    constexpr int M = X::M, D = X::D, MT = X::MT, NS = X::NS;
+    const PassParams &p = mp.p;
+    __shared__ u4 s_frag[X::NF * 2 * 64];
+    __shared__ __attribute__((aligned(16))) unsigned short s_xh[MF_WAVES][X::XS];
+    __shared__ __attribute__((aligned(16))) unsigned short s_xl[MF_WAVES][X::XS];
+
+    for (int k = threadIdx.x; k < X::NF * 2 * 64; k += 64 * MF_WAVES) s_frag[k] = mp.frags[k];
+    __syncthreads();
+
+    // wave-uniform (scalar) wave index: the IA index and its global base
+    // pointers are then scalar, and loads/stores take 32-bit lane offsets
+    const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+    unsigned short *__restrict__ xh = s_xh[wave];
+    unsigned short *__restrict__ xl = s_xl[wave];
+
+    for (int it = 0; it < n_iter; it++) {
+        // The lane coordinates are made opaque once per iteration: everything
+        // derived from them (LDS addresses of every slot, fragment offsets,
+        // plane cells and their FromCenter order) is then recomputed from one
+        // register plus constants instead of being hoisted out of the loop as
+        // dozens of loop-invariant registers.
+        int lane = threadIdx.x & 63;
+        asm volatile("" : "+v"(lane));
+        const int j = lane % MT, g = lane / MT;
+        auto frag = [&](int f, int lo) -> u4 { return s_frag[(f * 2 + lo) * 64 + lane]; };
+        const int ia = (it * (int)gridDim.x + (int)blockIdx.x) * MF_WAVES + wave;
+        const bool valid = ia < p.n;
+
+        // ---- tiles in the B layout of Hf a: element (kb, s) = row kphys, column j
+        float ta[X::KN][8], tg[X::KN][8];
+        float mA = 16.0f, mB = 16.0f;
+        if (FROM_IMAGE) {
+            int amax = 0, bmax = 0;
+            bool warp = false;
+            if (valid) {
+                const int per_pair = p.per_pair > 0 ? p.per_pair : p.n;
+                const int pair = ia / per_pair, tt = ia - pair * per_pair;
+                const size_t img_off = (size_t)pair * p.H * p.W;
+                const uint8_t *__restrict__ imA = p.imgA + img_off;
+                const uint8_t *__restrict__ imB = p.imgB + img_off;
+                const int y0 = p.top[tt], x0 = p.left[tt];
+                int by0 = y0, bx0 = x0;
+                float fu = 0.0f, fv = 0.0f;
+                warp = (p.bmode == B_WARP_MICRO) && p.du != nullptr;
+                if (p.du != nullptr) {
+                    const float u = p.du[ia], v = p.dv[ia];
+                    if (p.bmode == B_CLIP_SHIFT) {
+                        by0 = fm_round((float)y0 + u);
+                        bx0 = fm_round((float)x0 + v);
+                    } else if (warp) {
+                        const float flu = floorf(u), flv = floorf(v);
+                        by0 = (short)(y0 + (int)flu);
+                        bx0 = (short)(x0 + (int)flv);
+                        fu = u - flu;  // createMicroDisplacements :195-196
+                        fv = v - flv;
+                    }
+                }
+#pragma unroll
+                for (int kb = 0; kb < X::KN; kb++)
+#pragma unroll
+                    for (int s = 0; s < 8; s++) {
+                        const int r = kphys<MT>(kb, g, s);
+                        int a = 0, b = 0;
+                        if (r < N) {
+                            const int ya = y0 + r, xa = x0 + j;
+                            if (ya >= 0 && ya < p.H && xa >= 0 && xa < p.W)
+                                a = imA[(size_t)ya * p.W + xa];
+                            if (warp) {
+                                b = (int)warp_pixel(imB, p.H, p.W, fu, fv, by0 + r, bx0 + j);
+                            } else {
+                                const int yb = by0 + r, xb = bx0 + j;
+                                if (yb >= 0 && yb < p.H && xb >= 0 && xb < p.W) {
+                                    b = imB[(size_t)yb * p.W + xb];
+                                    bmax = max(bmax, b);
+                                }
+                            }
+                            amax = max(amax, a);
+                        }
+                        ta[kb][s] = (float)a;
+                        tg[kb][s] = (float)b;
+                    }
+            } else {
+#pragma unroll
+                for (int kb = 0; kb < X::KN; kb++)
+#pragma unroll
+                    for (int s = 0; s < 8; s++) ta[kb][s] = tg[kb][s] = 0.0f;
+            }
+#pragma unroll
+            for (int o = 32; o >= 1; o >>= 1) {
+                amax = max(amax, __shfl_xor(amax, o, 64));
+                bmax = max(bmax, __shfl_xor(bmax, o, 64));
+            }
+            mA = (float)(amax < 16 ? 16 : amax);
+            mB = warp ? 255.0f : (float)(bmax < 16 ? 16 : bmax);
+        } else {
+            if (valid) {
+                mA = p.maxA[ia];
+                mB = p.maxB[ia];
+            }
+            const uint8_t *a8 = p.tA + (size_t)(valid ? ia : 0) * N * N;
+            const uint8_t *b8 = p.tB + (size_t)(valid ? ia : 0) * N * N;
+#pragma unroll
+            for (int kb = 0; kb < X::KN; kb++)
+#pragma unroll
+                for (int s = 0; s < 8; s++) {
+                    const int r = kphys<MT>(kb, g, s);
+                    const bool in = valid && r < N;
+                    ta[kb][s] = in ? (float)a8[(unsigned)(r * N + j)] : 0.0f;
+                    tg[kb][s] = in ? (float)b8[(unsigned)(r * N + j)] : 0.0f;
+                }
+        }
+        if (p.mode == QLP_NORM_A_MAX) mB = mA;
+
+        // ---- forward: W = Hf (Hf x)^T = E^T for x = a, g ------------------------
+        Acc<MT> W[2][2][2];  // [image][rt][ct]
+#pragma unroll
+        for (int im = 0; im < 2; im++) {
+            u4 bh[X::KN], bl[X::KN];
+            // x / max * 16 + 1 (Matrix.copyMatrixToArrayAndNormalizeAndOffset
+            // :175-181) as x * (16 / max) + 1: one division per tile; within an
+            // ulp of the reference's value, far below the form's error
+            const float sx = 16.0f / (im == 0 ? mA : mB);
+#pragma unroll
+            for (int kb = 0; kb < X::KN; kb++)
+#pragma unroll
+                for (int s = 0; s < 8; s += 2) {
+                    float y[2];
+#pragma unroll
+                    for (int e = 0; e < 2; e++) {
+                        const int r = kphys<MT>(kb, g, s + e);
+                        const float x = im == 0 ? ta[kb][s + e] : tg[kb][s + e];
+                        y[e] = r < N ? x * sx + 1.0f : 0.0f;
+                    }
+                    unsigned h, l;
+                    split2<BF>(y[0], y[1], h, l);
+                    bh[kb][s >> 1] = h;
+                    bl[kb][s >> 1] = l;
+                }
+            // V = Hf x (M x N) one row tile ct at a time, to LDS as [rho - MT ct][m];
+            // W = Hf V^T (M x M), column tile ct, reads it transposed
+#pragma unroll
+            for (int ct = 0; ct < 2; ct++) {
+                Acc<MT> acc = {};
+#pragma unroll
+                for (int kb = 0; kb < X::KN; kb++)
+                    acc = mma3<BF, SPLIT, MT>(frag(ct * X::KN + kb, 0), frag(ct * X::KN + kb, 1),
+                                              bh[kb], bl[kb], acc);
+#pragma unroll
+                for (int v = 0; v < NS; v += 2) {
+                    unsigned h, l;
+                    split2<BF>(acc[v], acc[v + 1], h, l);
+                    const int r = mrow<MT>(v, g);  // rows r and r + 1
+                    xh[r * X::SV + j] = (unsigned short)h;
+                    xl[r * X::SV + j] = (unsigned short)l;
+                    xh[(r + 1) * X::SV + j] = (unsigned short)(h >> 16);
+                    xl[(r + 1) * X::SV + j] = (unsigned short)(l >> 16);
+                }
+                wave_lds_sync();
+#pragma unroll
+                for (int rt = 0; rt < 2; rt++) {
+                    Acc<MT> w = {};
+                    static_for<0, X::KN>([&](auto kc) {
+                        constexpr int kb = decltype(kc)::value;
+                        w = mma3<BF, SPLIT, MT>(
+                            frag(rt * X::KN + kb, 0), frag(rt * X::KN + kb, 1),
+                            bfrag_from_lds<MT, X::SV, kb, 0, N>(xh, j, g),
+                            bfrag_from_lds<MT, X::SV, kb, 0, N>(xl, j, g), w);
+                    });
+                    W[im][rt][ct] = w;
+                }
+                wave_lds_sync();
+            }
+        }
+
+        // ---- spectrum: real-basis coefficients of conj(A^) G^ / M^2 ------------
+        // W[.][rt][ct] at (lane, slot) = E[rho][sigma] with rho = MT ct + j
+        // (row frequency k = j) and sigma = MT rt + row (column frequency l):
+        // CC = [0][0], SC = [0][1], CS = [1][0], SS = [1][1].  k = 0 or l = 0
+        // hold the two edge frequencies (0 and M/2) that have no sine partner.
+        constexpr float sg = 4.0f / (float)(M * M) / (float)(1 << X::E1);
+        constexpr float se = 2.0f / (float)(M * M) / (float)(1 << X::E1);
+        constexpr float sc = 1.0f / (float)(M * M) / (float)(1 << X::E1);
+        u4 ch[2][X::KM], cl[2][X::KM];  // C^T as B fragments [ct][kb]
+#pragma unroll
+        for (int v0 = 0; v0 < NS; v0 += 2) {
+            float o[2][2][2];  // [slot v0 + e][rt][ct]
+#pragma unroll
+            for (int e = 0; e < 2; e++) {
+                const int v = v0 + e;
+                const bool kk = (j == 0), ll = (mrow<MT>(v, g) == 0);
+                const float CC = W[0][0][0][v], SC = W[0][0][1][v], CS = W[0][1][0][v],
+                            SS = W[0][1][1][v];
+                const float cc = W[1][0][0][v], scg = W[1][0][1][v], cs = W[1][1][0][v],
+                            ss = W[1][1][1][v];
+                if (!kk && !ll) {
+                    o[e][0][0] = sg * (CC * cc + SS * ss + SC * scg + CS * cs);
+                    o[e][1][1] = sg * (CC * ss + SS * cc - SC * cs - CS * scg);
+                    o[e][0][1] = -sg * (SC * cc - CC * scg + SS * cs - CS * ss);
+                    o[e][1][0] = sg * (CC * cs - CS * cc + SC * ss - SS * scg);
+                } else if (kk && !ll) {
+                    // frequencies k = 0 (ct 0) and k = M/2 (ct 1): CC = [0][ct], CS = [1][ct]
+                    o[e][0][0] = se * (CC * cc + CS * cs);
+                    o[e][1][0] = se * (CC * cs - CS * cc);
+                    o[e][0][1] = se * (SC * scg + SS * ss);
+                    o[e][1][1] = se * (SC * ss - SS * scg);
+                } else if (!kk && ll) {
+                    // frequencies l = 0 (rt 0) and l = M/2 (rt 1): CC = [rt][0], SC = [rt][1]
+                    o[e][0][0] = se * (CC * cc + SC * scg);
+                    o[e][0][1] = -se * (SC * cc - CC * scg);
+                    o[e][1][0] = se * (CS * cs + SS * ss);
+                    o[e][1][1] = -se * (SS * cs - CS * ss);
+                } else {
+                    o[e][0][0] = sc * CC * cc;
+                    o[e][0][1] = sc * SC * scg;
+                    o[e][1][0] = sc * CS * cs;
+                    o[e][1][1] = sc * SS * ss;
+                }
+            }
+#pragma unroll
+            for (int rt = 0; rt < 2; rt++)
+#pragma unroll
+                for (int ct = 0; ct < 2; ct++) {
+                    unsigned h, l;
+                    split2<BF>(o[0][rt][ct], o[1][rt][ct], h, l);
+                    ch[ct][acc_kb<MT>(rt, v0)][acc_s<MT>(rt, v0) >> 1] = h;
+                    cl[ct][acc_kb<MT>(rt, v0)][acc_s<MT>(rt, v0) >> 1] = l;
+                }
+        }
+
+        // ---- inverse: Y = Hout^T C^T, then R = Hout^T Y^T ------------------------
+        // R accumulates over the K blocks of rho.  For each K block the Y columns
+        // it needs (16 rho of one Y column tile for MT 32, all 32 for MT 16) go
+        // to LDS as [p][rho - KB kb], where the second product reads them
+        // transposed; a Y column tile is computed once, before its first K block.
+        Acc<MT> R[2][2] = {};
+        Acc<MT> Yt[2][X::YCT];  // Y tiles [rt][column tile of the current block]
+        static_for<0, X::KM>([&](auto kc) {
+            constexpr int kb = decltype(kc)::value;
+            constexpr int ct0 = (MT == 32) ? (kb >> 1) : 0, half = (MT == 32) ? (kb & 1) : 0;
+            if constexpr (half == 0) {
+#pragma unroll
+                for (int rt = 0; rt < 2; rt++)
+#pragma unroll
+                    for (int c2 = 0; c2 < X::YCT; c2++) {
+                        Acc<MT> acc = {};
+#pragma unroll
+                        for (int k = 0; k < X::KM; k++) {
+                            const int f = X::NF_HF + rt * X::KM + k;
+                            acc = mma3<BF, SPLIT, MT>(frag(f, 0), frag(f, 1), ch[ct0 + c2][k],
+                                                      cl[ct0 + c2][k], acc);
+                        }
+                        Yt[rt][c2] = acc * (1.0f / (float)(1 << X::E2));
+                    }
+            }
+            // the lanes holding this block's rho write them (all lanes for MT 16)
+            if (MT == 16 || (j >> 4) == half) {
+#pragma unroll
+                for (int rt = 0; rt < 2; rt++)
+#pragma unroll
+                    for (int c2 = 0; c2 < X::YCT; c2++)
+#pragma unroll
+                        for (int v = 0; v < NS; v += 2) {
+                            unsigned h, l;
+                            split2<BF>(Yt[rt][c2][v], Yt[rt][c2][v + 1], h, l);
+                            const int r = MT * rt + mrow<MT>(v, g);  // rows r and r + 1
+                            const int col = MT * c2 + j - 16 * half;
+                            xh[r * X::SY + col] = (unsigned short)h;
+                            xl[r * X::SY + col] = (unsigned short)l;
+                            xh[(r + 1) * X::SY + col] = (unsigned short)(h >> 16);
+                            xl[(r + 1) * X::SY + col] = (unsigned short)(l >> 16);
+                        }
+            }
+            wave_lds_sync();
+#pragma unroll
+            for (int rt = 0; rt < 2; rt++)
+#pragma unroll
+                for (int ct = 0; ct < 2; ct++) {
+                    const int f = X::NF_HF + rt * X::KM + kb;
+                    R[rt][ct] = mma3<BF, SPLIT, MT>(
+                        frag(f, 0), frag(f, 1),
+                        bfrag_from_lds<MT, X::SY, kb, X::KB * kb, X::KB>(xh, MT * ct + j, g),
+                        bfrag_from_lds<MT, X::SY, kb, X::KB * kb, X::KB>(xl, MT * ct + j, g),
+                        R[rt][ct]);
+                }
+            wave_lds_sync();
+        });
+
+        // ---- epilogues: plane store, FromCenter -----------------------------------
+        constexpr float unscale = (float)(1 << (X::E1 + X::E2));
+        if (valid && p.planes) {
+            // row a = D (the last row of row tile 1) is only reached by the last
+            // slot, column b = D only in column tile 1
+            float *__restrict__ out = p.planes + (size_t)ia * D * D;
+#pragma unroll
+            for (int ct = 0; ct < 2; ct++) {
+                const int b = MT * ct + j;
+                if (ct == 0 || b < D) {
+#pragma unroll
+                    for (int rt = 0; rt < 2; rt++)
+#pragma unroll
+                        for (int v = 0; v < NS; v++) {
+                            const int a = MT * rt + mrow<MT>(v, g);
+                            if (rt == 0 || v < NS - 1 || a < D) out[(unsigned)(a * D + b)] = R[rt][ct][v] * unscale;
+                        }
+                }
+            }
+        }
+        if (PEAK) {
+            constexpr int h = D / 2;
+            uint64_t best = 0;
+            int bestCell = 0;
+            float mn = 3.40282347e+38f;
+#pragma unroll
+            for (int rt = 0; rt < 2; rt++)
+#pragma unroll
+                for (int ct = 0; ct < 2; ct++)
+#pragma unroll
+                    for (int v = 0; v < NS; v++) {
+                        const int a = MT * rt + mrow<MT>(v, g), b = MT * ct + j;
+                        if (a == 0 || b == 0 || a >= D || b >= D) continue;
+                        const float val = R[rt][ct][v] * unscale;
+                        if (val < mn) mn = val;
+                        const uint64_t key = from_center_key(val, a, b, h);
+                        if (key > best) {
+                            best = key;
+                            bestCell = a * D + b;
+                        }
+                    }
+#pragma unroll
+            for (int o = 32; o >= 1; o >>= 1) {
+                const uint64_t ob = shfl_xor_u64(best, o);
+                const int oc = __shfl_xor(bestCell, o, 64);
+                const float om = __shfl_xor(mn, o, 64);
+                if (ob > best) {
+                    best = ob;
+                    bestCell = oc;
+                }
+                if (om < mn) mn = om;
+            }
+            int pi = 0, pj = 0;
+            if (best != 0) {
+                pi = bestCell / D;
+                pj = bestCell % D;
+            }
+            float pv = 0.0f;
+            const int L = 2 * p.nbr_half + 1;
+#pragma unroll
+            for (int rt = 0; rt < 2; rt++)
+#pragma unroll
+                for (int ct = 0; ct < 2; ct++)
+#pragma unroll
+                    for (int v = 0; v < NS; v++) {
+                        const int a = MT * rt + mrow<MT>(v, g), b = MT * ct + j;
+                        if (a >= D || b >= D) continue;
+                        const float val = R[rt][ct][v] * unscale;
+                        if (best != 0 && a == pi && b == pj) pv = val;
+                        if (valid && p.nbr) {
+                            float *nb = p.nbr + (size_t)ia * 2 * L;
+                            const int di = a - pi, dj = b - pj;
+                            if (b == pj && di >= -p.nbr_half && di <= p.nbr_half)
+                                nb[di + p.nbr_half] = val;
+                            if (a == pi && dj >= -p.nbr_half && dj <= p.nbr_half)
+                                nb[L + dj + p.nbr_half] = val;
+                        }
+                    }
+            // the peak value: one lane holds it, the others hold 0
+#pragma unroll
+            for (int o = 32; o >= 1; o >>= 1) pv += __shfl_xor(pv, o, 64);
+            if (valid && lane == 0) {
+                if (p.peaks) {
+                    p.peaks[(size_t)ia * 3 + 0] = (float)pi;
+                    p.peaks[(size_t)ia * 3 + 1] = (float)pj;
+                    p.peaks[(size_t)ia * 3 + 2] = pv;
+                }
+                if (p.min_floor) p.min_floor[ia] = mn;
+            }
+            // neighbourhood cells outside the plane are 0 (as in the exact pass)
+            if (valid && p.nbr && lane < L) {
+                float *nb = p.nbr + (size_t)ia * 2 * L;
+                const int d = lane - p.nbr_half;
+                if (pi + d < 0 || pi + d >= D) nb[lane] = 0.0f;
+                if (pj + d < 0 || pj + d >= D) nb[L + lane] = 0.0f;
+            }
+        }
+    }
+}
+
+// ---- host side ---------------------------------------------------------------
+
+static double mf_basis(int M, int r, long x)
+{
+    const long xm = ((x % M) + M) % M;
+    if (r < M / 2) return cos(2.0 * M_PI * (double)((r * xm) % M) / M);
+    if (r == M / 2) return (xm & 1) ? -1.0 : 1.0;
+    return sin(2.0 * M_PI * (double)(((r - M / 2) * xm) % M) / M);
+}
+
+static unsigned short to_bf16_bits(float f)
+{
+    uint32_t b;
+    memcpy(&b, &f, 4);
+    b += 0x7fffu + ((b >> 16) & 1u);  // round to nearest even (finite values)
+    return (unsigned short)(b >> 16);
+}
+
+static float from_bf16_bits(unsigned short h)
+{
+    const uint32_t b = (uint32_t)h << 16;
+    float f;
+    memcpy(&f, &b, 4);
+    return f;
+}
+
+static void host_split(double x, bool bf, unsigned short &h, unsigned short &l)
+{
+    const float xf = (float)x;
+    if (bf) {
+        h = to_bf16_bits(xf);
+        l = to_bf16_bits((float)(x - (double)from_bf16_bits(h)));
+    } else {
+        const _Float16 hh = (_Float16)xf;
+        memcpy(&h, &hh, 2);
+        const _Float16 ll = (_Float16)(float)(x - (double)(float)hh);
+        memcpy(&l, &ll, 2);
+    }
+}
+
+// Fragments of the constant A operands in kphys order: for fragment (rt, kb) of
+// an R x K matrix, lane (i + MT g) element s holds A[MT rt + i][kphys(kb, g, s)].
+template <int N>
+static void build_frags(bool bf, std::vector<unsigned short> &out)
+{
+    using X = Mf<N>;
+    constexpr int M = X::M, MT = X::MT;
+    out.assign((size_t)X::NF * 2 * 64 * 8, 0);
+    auto put = [&](int f, auto A, int rows, int cols, int kblocks, int fbase) {
+        for (int rt = 0; rt < 2; rt++)
+            for (int kb = 0; kb < kblocks; kb++) {
+                const int ff = fbase + rt * kblocks + kb;
+                for (int lane = 0; lane < 64; lane++)
+                    for (int s = 0; s < 8; s++) {
+                        const int i = MT * rt + lane % MT, k = kphys<MT>(kb, lane / MT, s);
+                        const double x = (i < rows && k < cols) ? A(i, k) : 0.0;
+                        unsigned short h, l;
+                        host_split(x, bf, h, l);
+                        out[((size_t)(ff * 2 + 0) * 64 + lane) * 8 + s] = h;
+                        out[((size_t)(ff * 2 + 1) * 64 + lane) * 8 + s] = l;
+                    }
+            }
+        (void)f;
+    };
+    // Hf[rho][n] = h_rho(n)
+    put(0, [&](int r, int nn) { return mf_basis(M, r, nn); }, M, N, X::KN, 0);
+    // Hout^T[p][sigma] = h_sigma(p - (N - 1))
+    put(1, [&](int pp, int sg) { return mf_basis(M, sg, (long)pp - (N - 1)); }, M, M, X::KM,
+        X::NF_HF);
+}
+
+static int mfma_tables(Context *ctx, int N, bool bf, const u4 **out)
+{
+    const int ni = N == 16 ? 0 : 1, bi = bf ? 1 : 0;
+    if (!ctx->mfma_tab[ni][bi]) {
+        std::vector<unsigned short> h;
+        if (N == 16)
+            build_frags<16>(bf, h);
+        else
+            build_frags<32>(bf, h);
+        void *d = nullptr;
+        const size_t bytes = h.size() * sizeof(unsigned short);
+        if (int rc = check_hip(hipMalloc(&d, bytes), "hipMalloc(mfma tables)")) return rc;
+        if (int rc = check_hip(hipMemcpy(d, h.data(), bytes, hipMemcpyHostToDevice),
+                               "hipMemcpy(mfma tables)")) {
+            (void)hipFree(d);
+            return rc;
+        }
+        ctx->mfma_tab[ni][bi] = d;
+    }
+    *out = (const u4 *)ctx->mfma_tab[ni][bi];
+    return QLP_OK;
+}
+
+template <int N, bool BF, bool SPLIT>
+static void launch_mfma_t(Context *ctx, const MfmaParams &mp, bool from_image, bool peak)
+{
+    const int groups = (mp.p.n + MF_WAVES - 1) / MF_WAVES;
+    const int grid = groups < 2048 ? groups : 2048;
+    const int n_iter = (groups + grid - 1) / grid;
+#define QLP_MF_LAUNCH(FI, PK)                                                                 \
+    hipLaunchKernelGGL((xcorr_mfma_kernel<N, BF, SPLIT, FI, PK>), dim3(grid),                 \
+                       dim3(64 * MF_WAVES), 0, ctx->stream, mp, n_iter)
+    if (from_image) {
+        if (peak)
+            QLP_MF_LAUNCH(true, true);
+        else
+            QLP_MF_LAUNCH(true, false);
+    } else {
+        if (peak)
+            QLP_MF_LAUNCH(false, true);
+        else
+            QLP_MF_LAUNCH(false, false);
+    }
+#undef QLP_MF_LAUNCH
+}
+
+int launch_pass_mfma(Context *ctx, const PassParams &p, int form, bool from_image, bool peak)
+{
+    if (p.n <= 0) return QLP_OK;
+    if (p.N != 16 && p.N != 32)
+        return fail(QLP_ERR_UNSUPPORTED, "mfma correlation: IA size must be 16 or 32");
+    const bool bf = (form == QLP_XCORR_MFMA_BF16 || form == QLP_XCORR_MFMA_BF16_SINGLE);
+    const bool split = (form == QLP_XCORR_MFMA_F16 || form == QLP_XCORR_MFMA_BF16);
+    if (form < QLP_XCORR_MFMA_F16 || form > QLP_XCORR_MFMA_BF16_SINGLE)
+        return fail(QLP_ERR_ARG, "mfma correlation: unknown form");
+    MfmaParams mp;
+    mp.p = p;
+    if (int rc = mfma_tables(ctx, p.N, bf, &mp.frags)) return rc;
+#define QLP_MF_N(NN)                                                                          \
+    do {                                                                                      \
+        if (bf) {                                                                             \
+            if (split)                                                                        \
+                launch_mfma_t<NN, true, true>(ctx, mp, from_image, peak);                     \
+            else                                                                              \
+                launch_mfma_t<NN, true, false>(ctx, mp, from_image, peak);                    \
+        } else {                                                                              \
+            if (split)                                                                        \
+                launch_mfma_t<NN, false, true>(ctx, mp, from_image, peak);                    \
+            else                                                                              \
+                launch_mfma_t<NN, false, false>(ctx, mp, from_image, peak);                   \
+        }                                                                                     \
+    } while (0)
+    if (p.N == 16)
+        QLP_MF_N(16);
+    else
+        QLP_MF_N(32);
+#undef QLP_MF_N
+    return check_hip(hipGetLastError(), "mfma correlation launch");
+}
+
+}  // namespace qlpiv

@@ -20,13 +20,17 @@ import numpy as np
 
 from . import _lib
 from ._lib import (QLP_NORM_A_MAX, QLP_NORM_PER_TILE, QLP_PEAK_FROM_CENTER, QLP_PEAK_MULTI,
-                   QLP_SUBPIX_GAUSS1D, QLP_SUBPIX_GUO, QLP_SUBPIX_POLY, QlpError, build, check)
+                   QLP_SUBPIX_GAUSS1D, QLP_SUBPIX_GUO, QLP_SUBPIX_POLY, QLP_XCORR_EXACT,
+                   QLP_XCORR_MFMA_BF16, QLP_XCORR_MFMA_BF16_SINGLE, QLP_XCORR_MFMA_F16,
+                   QLP_XCORR_MFMA_F16_SINGLE, QlpError, build, check)
 
 __all__ = [
     "Context", "CrossCorrelationJob", "JobResultEnum", "PivPlan", "QlpError",
     "VectorValidatorJob", "build",
     "QLP_NORM_PER_TILE", "QLP_NORM_A_MAX", "QLP_PEAK_FROM_CENTER", "QLP_PEAK_MULTI",
-    "QLP_SUBPIX_GAUSS1D", "QLP_SUBPIX_GUO", "QLP_SUBPIX_POLY",
+    "QLP_SUBPIX_GAUSS1D", "QLP_SUBPIX_GUO", "QLP_SUBPIX_POLY", "QLP_XCORR_EXACT",
+    "QLP_XCORR_MFMA_F16", "QLP_XCORR_MFMA_BF16", "QLP_XCORR_MFMA_F16_SINGLE",
+    "QLP_XCORR_MFMA_BF16_SINGLE",
 ]
 
 
@@ -82,11 +86,23 @@ class Context:
         check(self.lib.qlp_ctx_synchronize(self.h))
 
     # ---- correlation -----------------------------------------------------
-    def xcorr(self, tilesA, tilesB, maxA, maxB=None, mode=QLP_NORM_PER_TILE, out=None):
+    def xcorr(self, tilesA, tilesB, maxA, maxB=None, mode=QLP_NORM_PER_TILE, out=None, form=0):
         """Correlate n IA pairs; numpy inputs use the host entry point, torch
-        CUDA tensors the device one (asynchronous on this context's stream)."""
+        CUDA tensors the device one (asynchronous on this context's stream).
+        form: QLP_XCORR_EXACT (the reference's arithmetic) or one of the MFMA
+        forms QLP_XCORR_MFMA_* (device tensors only)."""
         n, N = int(tilesA.shape[0]), int(tilesA.shape[-1])
         D = 2 * N - 1
+        if form != QLP_XCORR_EXACT:
+            if isinstance(tilesA, np.ndarray):
+                raise TypeError("the MFMA forms take device tensors")
+            import torch
+
+            if out is None:
+                out = torch.empty((n, D, D), dtype=torch.float32, device=tilesA.device)
+            check(self.lib.qlp_xcorr_form_dev(self.h, int(form), n, N, _ptr(tilesA), _ptr(tilesB),
+                                              _ptr(maxA), _ptr(maxB), int(mode), _ptr(out)))
+            return out
         if isinstance(tilesA, np.ndarray):
             tilesA = np.ascontiguousarray(tilesA, np.uint8)
             tilesB = np.ascontiguousarray(tilesB, np.uint8)
@@ -406,7 +422,8 @@ class PivPlan:
     interpolator_start_step, keep_planes, peak_kind (QLP_PEAK_*), n_peaks,
     peak_kernel_width, validation (list of stage tuples, see
     VectorValidatorJob), valid_max_iterations, valid_until_no_more,
-    inherit_mode (QLP_INHERIT_*)."""
+    inherit_mode (QLP_INHERIT_*), xcorr_form (QLP_XCORR_*: the arithmetic of the
+    16 and 32 px levels)."""
 
     def __init__(self, ctx: Context, H: int, W: int, pairs: int = 1, **config):
         cfg = _lib.PivConfig()
@@ -429,6 +446,7 @@ class PivPlan:
                                       config.get("valid_max_iterations", 0),
                                       config.get("valid_until_no_more", False))
         cfg.inherit_mode = config.get("inherit_mode", _lib.QLP_INHERIT_DIRECT)
+        cfg.xcorr_form = config.get("xcorr_form", QLP_XCORR_EXACT)
         self.P = cfg.n_peaks if cfg.peak_kind == QLP_PEAK_MULTI else 1
         self.ctx, self.H, self.W, self.pairs, self.cfg = ctx, H, W, pairs, cfg
         h = ctypes.c_void_p()

@@ -28,6 +28,11 @@ QLP_PEAK_MULTI = 1
 QLP_SUBPIX_GAUSS1D = 0
 QLP_SUBPIX_GUO = 1
 QLP_SUBPIX_POLY = 2
+QLP_XCORR_EXACT = 0
+QLP_XCORR_MFMA_F16 = 1
+QLP_XCORR_MFMA_BF16 = 2
+QLP_XCORR_MFMA_F16_SINGLE = 3
+QLP_XCORR_MFMA_BF16_SINGLE = 4
 
 # every symbol include/qlpiv.h declares, with (restype, argtypes)
 _vp, _i, _sz = ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t
@@ -40,6 +45,7 @@ SIGNATURES = {
     "qlp_version": (ctypes.c_char_p, []),
     "qlp_xcorr_batch": (_i, [_vp, _i, _i, _vp, _vp, _vp, _vp, _i, _vp]),
     "qlp_xcorr_batch_dev": (_i, [_vp, _i, _i, _vp, _vp, _vp, _vp, _i, _vp]),
+    "qlp_xcorr_form_dev": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _vp]),
     "qlp_find_peaks": (_i, [_vp, _i, _i, _i, _vp, _i, _i, _vp, _vp]),
     "qlp_find_peaks_dev": (_i, [_vp, _i, _i, _i, _vp, _i, _i, _vp, _vp]),
     "qlp_subpixel": (_i, [_vp, _i, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp]),
@@ -136,7 +142,8 @@ class PivConfig(ctypes.Structure):
                 ("subpix_iterations", ctypes.c_int), ("interpolator_start_step", ctypes.c_int),
                 ("keep_planes", ctypes.c_int), ("peak_kind", ctypes.c_int),
                 ("n_peaks", ctypes.c_int), ("peak_kernel_width", ctypes.c_int),
-                ("valid", ValidConfig), ("inherit_mode", ctypes.c_int)]
+                ("valid", ValidConfig), ("inherit_mode", ctypes.c_int),
+                ("xcorr_form", ctypes.c_int)]
 
 _lib = None
 
