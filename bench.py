@@ -110,6 +110,109 @@ def time_plan(ctx, plan, da, db, steps, warmup):
     return dt, step_ms, np.mean(np.array(per_level), axis=0)  # levels x (pass, fit, validation)
 
 
+def precision_report(total32, total64):
+    """The precision configuration: independent IAs cut from synthetic 2048 x 2048
+    pairs (50 % overlap grids; 16,129 IAs of 32 x 32 or 3,969 of 64 x 64 per
+    pair), correlated in every arithmetic form and compared with an FP64 FFT
+    correlation of the same normalised tiles (torch float64 on the device).
+    Per form: plane error relative to the plane maximum (max and RMS over IAs),
+    FromCenter integer-peak agreement with the FP64 plane (rounded to float32
+    for the same peak finder) on IAs whose FP64 peak ratio P1/P2 of 3 x 3 local
+    maxima exceeds 1.2, and the 3-point Gaussian sub-pixel difference on those
+    IAs where both fits are accepted."""
+    import torch
+    import torch.nn.functional as F
+
+    import qlpiv
+    from qlpiv.geometry import level_grid
+    from qlpiv.synth import particle_pair
+
+    dev = torch.device("cuda:0")
+    ctx = qlpiv.Context(0)
+    forms = {"exact_fp32": qlpiv.QLP_XCORR_EXACT, "mfma_f16_split": qlpiv.QLP_XCORR_MFMA_F16,
+             "mfma_bf16_split": qlpiv.QLP_XCORR_MFMA_BF16, "mfma_f16": qlpiv.QLP_XCORR_MFMA_F16_SINGLE,
+             "mfma_bf16": qlpiv.QLP_XCORR_MFMA_BF16_SINGLE}
+    report = {}
+    for N, total in ((32, total32), (64, total64)):
+        M, D = 2 * N, 2 * N - 1
+        tops, lefts = level_grid(H, W, N, 0.5, level0=True)
+        top_all = torch.from_numpy(np.ascontiguousarray(tops.ravel(), np.int32)).to(dev)
+        left_all = torch.from_numpy(np.ascontiguousarray(lefts.ravel(), np.int32)).to(dev)
+        acc = {k: {"err_max": 0.0, "err_sq": 0.0, "n": 0, "agree": 0, "n_ppr": 0,
+                   "sub_max": 0.0, "sub_sq": 0.0, "n_sub": 0} for k in forms}
+        done, pair = 0, 0
+        while done < total:
+            n = min(total - done, top_all.numel())
+            print(f"[precision] N={N} pair {pair}: {n} IAs", file=sys.stderr, flush=True)
+            a, b, _ = particle_pair(H, W, seed=1400 + pair, shift=(3.3, 3.3), core_radius=128.0,
+                                    u_max=6.0)
+            da, db = torch.from_numpy(a).to(dev), torch.from_numpy(b).to(dev)
+            tA, mA = ctx.clip_tiles(da, top_all[:n].contiguous(), left_all[:n].contiguous(), N)
+            tB, mB = ctx.clip_tiles(db, top_all[:n].contiguous(), left_all[:n].contiguous(), N)
+            ctx.synchronize()
+            # FP64 reference of the normalised tiles (float32 normalisation as the job's)
+            an = (tA.float() / mA[:, None, None]) * 16.0 + 1.0
+            gn = (tB.float() / mB[:, None, None]) * 16.0 + 1.0
+            ref = torch.empty((n, D, D), dtype=torch.float64, device=dev)
+            idx = (torch.arange(D, device=dev) - (N - 1)) % M
+            for s in range(0, n, 8192):
+                e = min(n, s + 8192)
+                A = torch.fft.fft2(an[s:e].double(), s=(M, M))
+                G = torch.fft.fft2(gn[s:e].double(), s=(M, M))
+                r = torch.fft.ifft2(torch.conj(A) * G).real
+                ref[s:e] = r[:, idx][:, :, idx]
+            ref_max = ref.abs().amax(dim=(1, 2))
+            # peak ratio of the FP64 plane: the two largest 3 x 3 local maxima
+            pooled = F.max_pool2d(ref[:, None], 3, stride=1, padding=1)[:, 0]
+            loc = torch.where(ref == pooled, ref, torch.full_like(ref, -float("inf")))
+            top2 = torch.topk(loc.reshape(n, -1), 2, dim=1).values
+            ppr = torch.where(top2[:, 1] > 0, top2[:, 0] / top2[:, 1],
+                              torch.full_like(top2[:, 0], float("inf")))
+            sel = ppr > 1.2
+            ref32 = ref.float().contiguous()
+            torch.cuda.synchronize()  # torch's stream wrote ref32; the library's reads it
+            pk_r, mf_r = ctx.find_peaks(ref32)
+            sub_r = ctx.subpixel(ref32, mf_r, pk_r, kind=qlpiv.QLP_SUBPIX_GAUSS1D, pixels=3)
+            for name, form in forms.items():
+                if form != qlpiv.QLP_XCORR_EXACT and N not in (16, 32):
+                    continue
+                planes = ctx.xcorr(tA, tB, mA, mB, form=form)
+                pk, mf = ctx.find_peaks(planes)
+                sub = ctx.subpixel(planes, mf, pk, kind=qlpiv.QLP_SUBPIX_GAUSS1D, pixels=3)
+                ctx.synchronize()
+                err = (planes.double() - ref).abs().amax(dim=(1, 2)) / ref_max
+                st = acc[name]
+                st["err_max"] = max(st["err_max"], float(err.max()))
+                st["err_sq"] += float((err ** 2).sum())
+                st["n"] += n
+                agree = (pk[:, 0, :2] == pk_r[:, 0, :2]).all(dim=1)
+                st["agree"] += int((agree & sel).sum())
+                st["n_ppr"] += int(sel.sum())
+                ok = sel & agree & (sub[:, 0, 3] == 3) & (sub_r[:, 0, 3] == 3)
+                d = (sub[:, 0, :2] - sub_r[:, 0, :2]).abs().amax(dim=1)[ok]
+                if d.numel():
+                    st["sub_max"] = max(st["sub_max"], float(d.max()))
+                    st["sub_sq"] += float((d.double() ** 2).sum())
+                    st["n_sub"] += int(d.numel())
+                del planes
+            done += n
+            pair += 1
+        report[f"{N}x{N}"] = {
+            "ias": total, "pairs": pair,
+            "forms": {k: {"plane_err_max_rel": st["err_max"],
+                          "plane_err_rms_rel": (st["err_sq"] / st["n"]) ** 0.5,
+                          "int_peak_agreement_ppr_gt_1.2": st["agree"] / max(st["n_ppr"], 1),
+                          "ias_ppr_gt_1.2": st["n_ppr"],
+                          "subpixel_err_max_px": st["sub_max"],
+                          "subpixel_err_rms_px": (st["sub_sq"] / max(st["n_sub"], 1)) ** 0.5,
+                          "fits_compared": st["n_sub"]}
+                      for k, st in acc.items() if st["n"]}}
+    ctx.close()
+    print(json.dumps({"metric": "precision of the correlation forms vs an FP64 correlation",
+                      "config": "independent IAs cut from synthetic 2048x2048 pairs",
+                      "report": report}), flush=True)
+
+
 def level_ms(plan, per_level):
     return {str(plan.levels[k]["ia"]): {"pass": round(float(per_level[k][0]), 4),
                                         "peaks_fit": round(float(per_level[k][1]), 4),
@@ -124,11 +227,18 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mfma", action="store_true", help="skip the matrix-core form's line")
+    ap.add_argument("--precision", action="store_true",
+                    help="print the precision report of the correlation forms instead")
+    ap.add_argument("--precision-ias", type=int, nargs=2, default=(200000, 50000),
+                    metavar=("N32", "N64"), help="IAs of 32x32 and of 64x64 in the report")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (one MI355X per box); "
               "run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.precision:
+        precision_report(*args.precision_ias)
+        return
 
     import torch
 
