@@ -213,6 +213,109 @@ def precision_report(total32, total64):
                       "report": report}), flush=True)
 
 
+SERVE_SIZES = ((512, 512), (1024, 1024), (1024, 2048), (2048, 2048))  # (H, W)
+
+
+def serve_schedule(n, rate, seed):
+    """Poisson arrivals at `rate` req/s; per request a frame size (uniform over
+    SERVE_SIZES), a final IA size from {16, 32, 64} and 1-3 levels (IA sizes
+    halving, the first at most 64 px)."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, n))
+    out = []
+    for k in range(n):
+        size = int(rng.integers(len(SERVE_SIZES)))
+        ia_end = int(rng.choice([16, 32, 64]))
+        levels = int(rng.integers(1, {16: 3, 32: 2, 64: 1}[ia_end] + 1))
+        out.append((float(t[k]), size, ia_end * 2 ** (levels - 1), ia_end))
+    return out
+
+
+def serve_run(qlpiv, frames, schedule, form, budget, slots, saturate):
+    """Submit the schedule (at its arrival times, or all at once when saturate)
+    to a fresh server; returns per-request latencies, IAs and the wall time."""
+    srv = qlpiv.Server(0, hbm_budget=budget, slots=slots)
+    common = dict(overlap_factor=0.5, warp_mode=qlpiv._lib.QLP_WARP_MICRO,
+                  subpix_kind=qlpiv.QLP_SUBPIX_GUO, subpix_pixels=3, subpix_iterations=20,
+                  xcorr_form=form)
+    # warm-up: every (size, IA) combination once, so that each slot's plans exist
+    warm = [srv.submit(*frames[s][0], ia_start=a, ia_end=e, **common)
+            for s in range(len(SERVE_SIZES)) for (a, e) in ((64, 16), (32, 16), (16, 16),
+                                                          (64, 32), (32, 32), (64, 64))
+            for _ in range(slots)]
+    for t in warm:
+        srv.wait(t)
+        srv.result(t)
+    tickets = []
+    t0 = time.perf_counter()
+    for k, (ta, s, a, e) in enumerate(schedule):
+        if not saturate:
+            while time.perf_counter() - t0 < ta:
+                pass
+        tickets.append(srv.submit(*frames[s][k % len(frames[s])], ia_start=a, ia_end=e, **common))
+    lat, ias, first, last = [], 0, None, None
+    for t in tickets:
+        srv.wait(t)
+        _, _, _, tm = srv.result(t)
+        lat.append(tm["done_ms"] - tm["submit_ms"])
+        ias += tm["ias"]
+        first = tm["submit_ms"] if first is None else min(first, tm["submit_ms"])
+        last = tm["done_ms"] if last is None else max(last, tm["done_ms"])
+    st = srv.stats()
+    srv.close()
+    wall = (last - first) * 1e-3
+    lat = np.array(lat)
+    return {"requests": len(tickets), "ias": ias, "wall_s": round(wall, 4),
+            "ias_per_s": round(ias / wall, 1), "req_per_s": round(len(tickets) / wall, 2),
+            "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
+                           "p99": round(float(np.percentile(lat, 99)), 3),
+                           "max": round(float(lat.max()), 3)},
+            "hbm_peak_bytes": st["hbm_peak"], "failed": st["failed"]}
+
+
+def serve_report(n_req, rate, slots):
+    """The serving configuration: a Poisson stream of frame-pair requests of mixed
+    sizes through the request loop (qlp_server) with a 64 GB HBM budget, for the
+    plain FP32 form and the MFMA fp16 hi+lo form, at the offered rate and
+    saturated (all requests queued at once)."""
+    import torch
+
+    import qlpiv
+    from qlpiv.synth import particle_pair
+
+    frames = []
+    for k, (h, w) in enumerate(SERVE_SIZES):
+        pairs = []
+        for r in range(2):
+            a, b, _ = particle_pair(h, w, seed=1500 + 10 * k + r, shift=(3.3, 3.3),
+                                    core_radius=float(np.random.default_rng(k + 10 * r).uniform(32, 256)),
+                                    u_max=float(np.random.default_rng(k + 10 * r + 1).uniform(2, 10)))
+            pa = torch.from_numpy(a).pin_memory().numpy()
+            pb = torch.from_numpy(b).pin_memory().numpy()
+            pairs.append((pa, pb))
+        frames.append(pairs)
+        print(f"[serve] frames {h}x{w} ready", file=sys.stderr, flush=True)
+    schedule = serve_schedule(n_req, rate, seed=1600)
+    budget = 64 << 30
+    out = {}
+    for name, form in (("exact_fp32", qlpiv.QLP_XCORR_EXACT), ("mfma_f16_split", qlpiv.QLP_XCORR_MFMA_F16)):
+        for mode in ("offered", "saturated"):
+            print(f"[serve] {name} {mode}: {n_req} requests", file=sys.stderr, flush=True)
+            out.setdefault(name, {})[mode] = serve_run(qlpiv, frames, schedule, form, budget,
+                                                       slots, mode == "saturated")
+    print(json.dumps({"metric": "serving: throughput and latency of a Poisson stream of mixed "
+                                "frame-pair requests",
+                      "config": {"arrival_rate_req_per_s": rate, "requests": n_req,
+                                 "sizes_HxW": SERVE_SIZES, "final_ia": [16, 32, 64],
+                                 "levels": "1-3", "slots": slots, "hbm_budget_bytes": budget,
+                                 "per_request": "micro warp, FromCenter, Guo 3px/20it, direct "
+                                                "inheritance; last level returned to the host"},
+                      "forms": out,
+                      "speedup_saturated": round(out["mfma_f16_split"]["saturated"]["ias_per_s"]
+                                                 / out["exact_fp32"]["saturated"]["ias_per_s"], 3)}),
+          flush=True)
+
+
 def level_ms(plan, per_level):
     return {str(plan.levels[k]["ia"]): {"pass": round(float(per_level[k][0]), 4),
                                         "peaks_fit": round(float(per_level[k][1]), 4),
@@ -231,6 +334,11 @@ def main():
                     help="print the precision report of the correlation forms instead")
     ap.add_argument("--precision-ias", type=int, nargs=2, default=(200000, 50000),
                     metavar=("N32", "N64"), help="IAs of 32x32 and of 64x64 in the report")
+    ap.add_argument("--serve", action="store_true",
+                    help="print the serving report (Poisson stream through the request loop)")
+    ap.add_argument("--serve-requests", type=int, default=2000)
+    ap.add_argument("--serve-rate", type=float, default=400.0)
+    ap.add_argument("--serve-slots", type=int, default=8)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (one MI355X per box); "
@@ -238,6 +346,9 @@ def main():
         sys.exit(2)
     if args.precision:
         precision_report(*args.precision_ias)
+        return
+    if args.serve:
+        serve_report(args.serve_requests, args.serve_rate, args.serve_slots)
         return
 
     import torch

@@ -343,6 +343,64 @@ int qlp_piv_validation_dev(qlp_piv_plan *plan, int level, const uint8_t **invali
 int qlp_piv_plan_set_timing(qlp_piv_plan *plan, int enable);
 int qlp_piv_plan_timing(qlp_piv_plan *plan, int level, float *state_and_pass_ms, float *fit_ms,
                         float *valid_ms);
+/* Device bytes a plan of this configuration allocates (host-only computation;
+ * the budget check of qlp_piv_plan_create and of the server's admission). */
+int qlp_piv_plan_estimate(int H, int W, int pairs, const qlp_piv_config *cfg, size_t *bytes);
+
+/* ---- serving: a request loop over frame pairs of mixed sizes ----------------
+ *
+ * Replaces the reference's pool of per-thread PIV workers over a list of frame
+ * pairs (InputFiles.java:172-224 with OpenClGpuManager.java:124-142: one job
+ * and device queue per worker thread) by one server per GPU.  Requests are
+ * admitted in arrival order as soon as a slot (a context with its own stream)
+ * is free and the HBM they need fits the server's budget (cached plans of idle
+ * slots are evicted, least recently used first); requests on different slots
+ * overlap on the device.  Frames, twiddles and every level's displacement state
+ * stay on the device for the request's whole multi-pass run; the last level's
+ * field is returned in host memory. */
+#define QLP_SERVER_MAX_SLOTS 16
+
+typedef struct qlp_server_config {
+    size_t hbm_budget; /* bytes of device memory the server may hold */
+    int slots;         /* requests in flight at once (1..QLP_SERVER_MAX_SLOTS) */
+} qlp_server_config;
+
+typedef struct qlp_request {
+    int H, W;                   /* frame size */
+    const uint8_t *imgA, *imgB; /* H x W uint8 frames; the caller keeps them until the
+                                   request has finished */
+    int frames_on_device;       /* 1: imgA/imgB are device pointers, 0: host memory */
+    qlp_piv_config cfg;         /* the multi-pass configuration of this request */
+} qlp_request;
+
+typedef struct qlp_request_times {
+    double submit_ms, admit_ms, done_ms; /* since the server was created */
+    double ias;                          /* IAs correlated over all levels */
+} qlp_request_times;
+
+typedef struct qlp_server_stats {
+    uint64_t completed, failed, pending, evictions;
+    size_t hbm_in_use, hbm_peak;
+    double ias_done;
+    int running;
+} qlp_server_stats;
+
+typedef struct qlp_server qlp_server;
+
+int qlp_server_create(int hip_device, const qlp_server_config *cfg, qlp_server **out);
+/* Waits for the requests in flight and the queued ones, then frees everything. */
+int qlp_server_destroy(qlp_server *server);
+/* Queue a request; QLP_ERR_BUDGET if it alone exceeds the budget. */
+int qlp_server_submit(qlp_server *server, const qlp_request *req, uint64_t *ticket);
+/* Wait up to timeout_ms (-1: no limit) for a request; *done = 1 once finished.
+ * Returns the request's own error if it failed. */
+int qlp_server_wait(qlp_server *server, uint64_t ticket, int timeout_ms, int *done);
+/* The last level's field of a finished request (host memory owned by the
+ * server until qlp_server_release): u, v (nI x nJ), its IA size and timings. */
+int qlp_server_result(qlp_server *server, uint64_t ticket, const float **u, const float **v,
+                      int *nI, int *nJ, int *ia, qlp_request_times *times);
+int qlp_server_release(qlp_server *server, uint64_t ticket);
+int qlp_server_get_stats(qlp_server *server, qlp_server_stats *stats);
 
 #ifdef __cplusplus
 }

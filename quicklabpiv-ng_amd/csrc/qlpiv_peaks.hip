@@ -224,7 +224,7 @@ __global__ __launch_bounds__(64) void peak_multi_kernel(int n, int D, int P, int
     const int seg = lane / W, sl = lane % W;
     const int ia = blockIdx.x * S + seg;
     const bool act = ia < n;
-    const uint64_t segmask = (W == 64) ? ~0ull : (((1ull << W) - 1ull) << (seg * W));
+    const uint64_t segmask = (W == 64) ? ~0ull : (((1ull << (W & 63)) - 1ull) << (seg * W));
     const float *m = planes + (size_t)(act ? ia : 0) * D * D;
 
     int iSlices = D / k;

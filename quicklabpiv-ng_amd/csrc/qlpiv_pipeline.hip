@@ -198,12 +198,11 @@ int qlpiv::launch_inherit_direct(Context *ctx, int pairs, int nI, int nJ, int pI
 
 extern "C" {
 
-int qlp_piv_plan_create(qlp_ctx *h, int H, int W, int pairs, const qlp_piv_config *cfg,
-                        qlp_piv_plan **out)
+// Checks a configuration and lays out the level geometry of a plan (host only):
+// fills p->levels, p->P and the device bytes the plan's buffers need.
+static int plan_layout(int H, int W, int pairs, const qlp_piv_config *cfg, qlp_piv_plan *p,
+                       size_t &need)
 {
-    if (!h || !cfg || !out) return fail(QLP_ERR_ARG, "piv_plan_create: null argument");
-    *out = nullptr;
-    Context *ctx = ctx_of(h);
     if (H < 1 || W < 1 || pairs < 1) return fail(QLP_ERR_ARG, "piv_plan_create: bad frame size");
     if (cfg->overlap_factor < 0.1f || cfg->overlap_factor > 0.9f)
         return fail(QLP_ERR_ARG, "Overlap factor must greater than 1/10 and less than 9/10");
@@ -248,22 +247,17 @@ int qlp_piv_plan_create(qlp_ctx *h, int H, int W, int pairs, const qlp_piv_confi
     if (cfg->xcorr_form < QLP_XCORR_EXACT || cfg->xcorr_form > QLP_XCORR_MFMA_BF16_SINGLE)
         return fail(QLP_ERR_ARG, "piv_plan_create: unknown correlation form");
     const bool valid = cfg->valid.n_stages > 0;
-    qlp_piv_plan *p = new qlp_piv_plan();
-    p->ctx = ctx;
     p->H = H;
     p->W = W;
     p->pairs = pairs;
     p->cfg = *cfg;
     p->P = P;
     const int m[4] = {cfg->margin_top, cfg->margin_bottom, cfg->margin_left, cfg->margin_right};
-    size_t need = 0;
+    need = 0;
     for (int l = 0, tile = cfg->ia_start; l < nlev; l++, tile /= 2) {
         Level L;
         level_grid(H, W, tile, cfg->overlap_factor, m, l == 0, L);
-        if (L.nI < 1 || L.nJ < 1) {
-            delete p;
-            return fail(QLP_ERR_ARG, "piv_plan_create: no IA fits the frame");
-        }
+        if (L.nI < 1 || L.nJ < 1) return fail(QLP_ERR_ARG, "piv_plan_create: no IA fits the frame");
         const size_t n = L.n, pn = (size_t)pairs * n, D = 2 * tile - 1;
         need += n * 8 + pn * (8 + 12 * P + 4 + 4 * 2 * NBR_L + 16 * P);
         if (multi || cfg->keep_planes >= 2 || (cfg->keep_planes == 1 && l == nlev - 1))
@@ -272,6 +266,39 @@ int qlp_piv_plan_create(qlp_ctx *h, int H, int W, int pairs, const qlp_piv_confi
         if (valid) need += n * 4 + valid_work_bytes(pairs, (int)n) + pn + (size_t)pairs * 16;
         p->levels.push_back(std::move(L));
     }
+    return QLP_OK;
+}
+
+int qlp_piv_plan_estimate(int H, int W, int pairs, const qlp_piv_config *cfg, size_t *bytes)
+{
+    if (!cfg || !bytes) return fail(QLP_ERR_ARG, "piv_plan_estimate: null argument");
+    qlp_piv_plan p;
+    size_t need = 0;
+    if (int rc = plan_layout(H, W, pairs, cfg, &p, need)) return rc;
+    // the inheritance stencils (at most 9 parents per tile) or spline tables
+    if (cfg->inherit_mode != QLP_INHERIT_DIRECT)
+        for (const Level &L : p.levels) need += L.n * 9 * 8 + 64 * 1024;
+    *bytes = need;
+    return QLP_OK;
+}
+
+int qlp_piv_plan_create(qlp_ctx *h, int H, int W, int pairs, const qlp_piv_config *cfg,
+                        qlp_piv_plan **out)
+{
+    if (!h || !cfg || !out) return fail(QLP_ERR_ARG, "piv_plan_create: null argument");
+    *out = nullptr;
+    Context *ctx = ctx_of(h);
+    qlp_piv_plan *p = new qlp_piv_plan();
+    p->ctx = ctx;
+    size_t need = 0;
+    int rc = plan_layout(H, W, pairs, cfg, p, need);
+    if (rc) {
+        delete p;
+        return rc;
+    }
+    const bool multi = cfg->peak_kind == QLP_PEAK_MULTI;
+    const bool valid = cfg->valid.n_stages > 0;
+    const int P = p->P;
     if (ctx->hbm_budget && ctx->hbm_in_use + need > ctx->hbm_budget) {
         delete p;
         return fail(QLP_ERR_BUDGET, "piv plan exceeds the context's HBM budget");

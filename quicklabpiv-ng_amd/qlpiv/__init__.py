@@ -25,7 +25,8 @@ from ._lib import (QLP_NORM_A_MAX, QLP_NORM_PER_TILE, QLP_PEAK_FROM_CENTER, QLP_
                    QLP_XCORR_MFMA_F16_SINGLE, QlpError, build, check)
 
 __all__ = [
-    "Context", "CrossCorrelationJob", "JobResultEnum", "PivPlan", "QlpError",
+    "Context", "CrossCorrelationJob", "JobResultEnum", "PivPlan", "QlpError", "Server",
+    "piv_config", "plan_estimate",
     "VectorValidatorJob", "build",
     "QLP_NORM_PER_TILE", "QLP_NORM_A_MAX", "QLP_PEAK_FROM_CENTER", "QLP_PEAK_MULTI",
     "QLP_SUBPIX_GAUSS1D", "QLP_SUBPIX_GUO", "QLP_SUBPIX_POLY", "QLP_XCORR_EXACT",
@@ -411,6 +412,116 @@ class CrossCorrelationJob:
         self.results.clear()
 
 
+def piv_config(**config):
+    """qlp_piv_config from keyword arguments (the keys PivPlan documents)."""
+    cfg = _lib.PivConfig()
+    cfg.ia_start = config.get("ia_start", 64)
+    cfg.ia_end = config.get("ia_end", 16)
+    cfg.overlap_factor = config.get("overlap_factor", 0.5)
+    m = config.get("margins", (0, 0, 0, 0))
+    cfg.margin_top, cfg.margin_bottom, cfg.margin_left, cfg.margin_right = m
+    cfg.norm_mode = config.get("norm_mode", QLP_NORM_PER_TILE)
+    cfg.warp_mode = config.get("warp_mode", _lib.QLP_WARP_MICRO)
+    cfg.subpix_kind = config.get("subpix_kind", QLP_SUBPIX_GUO)
+    cfg.subpix_pixels = config.get("subpix_pixels", 3)
+    cfg.subpix_iterations = config.get("subpix_iterations", 20)
+    cfg.interpolator_start_step = config.get("interpolator_start_step", 0)
+    cfg.keep_planes = int(config.get("keep_planes", False))
+    cfg.peak_kind = config.get("peak_kind", QLP_PEAK_FROM_CENTER)
+    cfg.n_peaks = config.get("n_peaks", 1)
+    cfg.peak_kernel_width = config.get("peak_kernel_width", 3)
+    cfg.valid = _lib.valid_config(config.get("validation") or [],
+                                  config.get("valid_max_iterations", 0),
+                                  config.get("valid_until_no_more", False))
+    cfg.inherit_mode = config.get("inherit_mode", _lib.QLP_INHERIT_DIRECT)
+    cfg.xcorr_form = config.get("xcorr_form", QLP_XCORR_EXACT)
+    return cfg
+
+
+def plan_estimate(H, W, pairs=1, **config):
+    """Device bytes a PivPlan of this configuration allocates (qlp_piv_plan_estimate)."""
+    lib = _lib.load()
+    cfg = piv_config(**config)
+    b = ctypes.c_size_t()
+    check(lib.qlp_piv_plan_estimate(int(H), int(W), int(pairs), ctypes.byref(cfg), ctypes.byref(b)))
+    return b.value
+
+
+class Server:
+    """The serving form (qlp_server): a request loop over frame pairs of mixed
+    sizes with HBM-budget admission and requests overlapping on the device.
+    Replaces the reference's worker-thread pool over frame pairs
+    (InputFiles.java:172-224, OpenClGpuManager.java:124-142).
+
+    submit() takes the frames (numpy uint8 host arrays, preferably pinned, or
+    CUDA tensors) and the PivPlan configuration keys; the frames must stay alive
+    until the request has finished (wait())."""
+
+    def __init__(self, device=0, hbm_budget=64 << 30, slots=4):
+        self.lib = _lib.load()
+        cfg = _lib.ServerConfig(int(hbm_budget), int(slots))
+        h = ctypes.c_void_p()
+        check(self.lib.qlp_server_create(int(device), ctypes.byref(cfg), ctypes.byref(h)))
+        self.h = h
+        self._keep = {}
+
+    def submit(self, imgA, imgB, **config):
+        H, W = int(imgA.shape[0]), int(imgA.shape[1])
+        on_dev = not isinstance(imgA, np.ndarray)
+        if not on_dev:
+            imgA = np.ascontiguousarray(imgA, np.uint8)
+            imgB = np.ascontiguousarray(imgB, np.uint8)
+        req = _lib.Request(H, W, _ptr(imgA), _ptr(imgB), int(on_dev), piv_config(**config))
+        t = ctypes.c_uint64()
+        check(self.lib.qlp_server_submit(self.h, ctypes.byref(req), ctypes.byref(t)))
+        self._keep[t.value] = (imgA, imgB)
+        return t.value
+
+    def wait(self, ticket, timeout_ms=-1):
+        done = ctypes.c_int()
+        check(self.lib.qlp_server_wait(self.h, ctypes.c_uint64(ticket), int(timeout_ms),
+                                       ctypes.byref(done)))
+        if done.value:
+            self._keep.pop(ticket, None)
+        return bool(done.value)
+
+    def result(self, ticket, release=True):
+        """(u, v, ia, times) of a finished request: the last level's field
+        (nI x nJ numpy copies), its IA size and the request's timings in ms
+        (submit, admit, done) and IA count."""
+        u, v = ctypes.c_void_p(), ctypes.c_void_p()
+        nI, nJ, ia = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        tm = _lib.RequestTimes()
+        check(self.lib.qlp_server_result(self.h, ctypes.c_uint64(ticket), ctypes.byref(u),
+                                         ctypes.byref(v), ctypes.byref(nI), ctypes.byref(nJ),
+                                         ctypes.byref(ia), ctypes.byref(tm)))
+        n = nI.value * nJ.value
+        fu = np.ctypeslib.as_array((ctypes.c_float * n).from_address(u.value)).reshape(nI.value, nJ.value).copy()
+        fv = np.ctypeslib.as_array((ctypes.c_float * n).from_address(v.value)).reshape(nI.value, nJ.value).copy()
+        times = {"submit_ms": tm.submit_ms, "admit_ms": tm.admit_ms, "done_ms": tm.done_ms,
+                 "ias": int(tm.ias)}
+        if release:
+            check(self.lib.qlp_server_release(self.h, ctypes.c_uint64(ticket)))
+        return fu, fv, ia.value, times
+
+    def stats(self):
+        st = _lib.ServerStats()
+        check(self.lib.qlp_server_get_stats(self.h, ctypes.byref(st)))
+        return {k: getattr(st, k) for k, _ in st._fields_}
+
+    def close(self):
+        if self.h:
+            self.lib.qlp_server_destroy(self.h)
+            self.h = None
+            self._keep.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class PivPlan:
     """Multi-pass PIV over device-resident frame pairs (qlp_piv_plan): the
     adaptive per-level loop of jobs/LocalPIVOpenCLGpuJob.call :188-409 with
@@ -426,27 +537,7 @@ class PivPlan:
     16 and 32 px levels)."""
 
     def __init__(self, ctx: Context, H: int, W: int, pairs: int = 1, **config):
-        cfg = _lib.PivConfig()
-        cfg.ia_start = config.get("ia_start", 64)
-        cfg.ia_end = config.get("ia_end", 16)
-        cfg.overlap_factor = config.get("overlap_factor", 0.5)
-        m = config.get("margins", (0, 0, 0, 0))
-        cfg.margin_top, cfg.margin_bottom, cfg.margin_left, cfg.margin_right = m
-        cfg.norm_mode = config.get("norm_mode", QLP_NORM_PER_TILE)
-        cfg.warp_mode = config.get("warp_mode", _lib.QLP_WARP_MICRO)
-        cfg.subpix_kind = config.get("subpix_kind", QLP_SUBPIX_GUO)
-        cfg.subpix_pixels = config.get("subpix_pixels", 3)
-        cfg.subpix_iterations = config.get("subpix_iterations", 20)
-        cfg.interpolator_start_step = config.get("interpolator_start_step", 0)
-        cfg.keep_planes = int(config.get("keep_planes", False))
-        cfg.peak_kind = config.get("peak_kind", QLP_PEAK_FROM_CENTER)
-        cfg.n_peaks = config.get("n_peaks", 1)
-        cfg.peak_kernel_width = config.get("peak_kernel_width", 3)
-        cfg.valid = _lib.valid_config(config.get("validation") or [],
-                                      config.get("valid_max_iterations", 0),
-                                      config.get("valid_until_no_more", False))
-        cfg.inherit_mode = config.get("inherit_mode", _lib.QLP_INHERIT_DIRECT)
-        cfg.xcorr_form = config.get("xcorr_form", QLP_XCORR_EXACT)
+        cfg = piv_config(**config)
         self.P = cfg.n_peaks if cfg.peak_kind == QLP_PEAK_MULTI else 1
         self.ctx, self.H, self.W, self.pairs, self.cfg = ctx, H, W, pairs, cfg
         h = ctypes.c_void_p()

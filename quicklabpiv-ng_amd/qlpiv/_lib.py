@@ -74,6 +74,15 @@ SIGNATURES = {
     "qlp_gaussian_filter_dev": (_i, [_vp, _i, _i, _i, _vp, _i, ctypes.c_float, _i, _vp, _vp]),
     "qlp_inherit_velocity_dev": (_i, [_vp, _i, _i, _i, _i, _i, _i, _vp, _vp, _i, _i, _i, _i,
                                       _vp, _vp, _i, _vp, _vp, _vp, _vp]),
+    "qlp_piv_plan_estimate": (_i, [_i, _i, _i, _vp, ctypes.POINTER(_sz)]),
+    "qlp_server_create": (_i, [_i, _vp, ctypes.POINTER(_vp)]),
+    "qlp_server_destroy": (_i, [_vp]),
+    "qlp_server_submit": (_i, [_vp, _vp, ctypes.POINTER(ctypes.c_uint64)]),
+    "qlp_server_wait": (_i, [_vp, ctypes.c_uint64, _i, ctypes.POINTER(_i)]),
+    "qlp_server_result": (_i, [_vp, ctypes.c_uint64, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                               ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i), _vp]),
+    "qlp_server_release": (_i, [_vp, ctypes.c_uint64]),
+    "qlp_server_get_stats": (_i, [_vp, _vp]),
 }
 
 QLP_WARP_NONE = 0
@@ -144,6 +153,32 @@ class PivConfig(ctypes.Structure):
                 ("n_peaks", ctypes.c_int), ("peak_kernel_width", ctypes.c_int),
                 ("valid", ValidConfig), ("inherit_mode", ctypes.c_int),
                 ("xcorr_form", ctypes.c_int)]
+
+
+class ServerConfig(ctypes.Structure):
+    """qlp_server_config (include/qlpiv.h)."""
+    _fields_ = [("hbm_budget", ctypes.c_size_t), ("slots", ctypes.c_int)]
+
+
+class Request(ctypes.Structure):
+    """qlp_request (include/qlpiv.h)."""
+    _fields_ = [("H", ctypes.c_int), ("W", ctypes.c_int), ("imgA", ctypes.c_void_p),
+                ("imgB", ctypes.c_void_p), ("frames_on_device", ctypes.c_int),
+                ("cfg", PivConfig)]
+
+
+class RequestTimes(ctypes.Structure):
+    """qlp_request_times (include/qlpiv.h)."""
+    _fields_ = [("submit_ms", ctypes.c_double), ("admit_ms", ctypes.c_double),
+                ("done_ms", ctypes.c_double), ("ias", ctypes.c_double)]
+
+
+class ServerStats(ctypes.Structure):
+    """qlp_server_stats (include/qlpiv.h)."""
+    _fields_ = [("completed", ctypes.c_uint64), ("failed", ctypes.c_uint64),
+                ("pending", ctypes.c_uint64), ("evictions", ctypes.c_uint64),
+                ("hbm_in_use", ctypes.c_size_t), ("hbm_peak", ctypes.c_size_t),
+                ("ias_done", ctypes.c_double), ("running", ctypes.c_int)]
 
 _lib = None
 
