@@ -303,6 +303,12 @@ def serve_report(n_req, rate, slots):
             print(f"[serve] {name} {mode}: {n_req} requests", file=sys.stderr, flush=True)
             out.setdefault(name, {})[mode] = serve_run(qlpiv, frames, schedule, form, budget,
                                                        slots, mode == "saturated")
+    # the plain FP32 form on the same stream: one request at a time (one slot)
+    print(f"[serve] plain FP32 form, one request at a time: {n_req} requests", file=sys.stderr,
+          flush=True)
+    out["plain_fp32_one_at_a_time"] = {"saturated": serve_run(qlpiv, frames, schedule,
+                                                              qlpiv.QLP_XCORR_EXACT, budget, 1,
+                                                              True)}
     print(json.dumps({"metric": "serving: throughput and latency of a Poisson stream of mixed "
                                 "frame-pair requests",
                       "config": {"arrival_rate_req_per_s": rate, "requests": n_req,
@@ -311,8 +317,12 @@ def serve_report(n_req, rate, slots):
                                  "per_request": "micro warp, FromCenter, Guo 3px/20it, direct "
                                                 "inheritance; last level returned to the host"},
                       "forms": out,
-                      "speedup_saturated": round(out["mfma_f16_split"]["saturated"]["ias_per_s"]
-                                                 / out["exact_fp32"]["saturated"]["ias_per_s"], 3)}),
+                      "speedup_mfma_vs_exact_saturated": round(
+                          out["mfma_f16_split"]["saturated"]["ias_per_s"]
+                          / out["exact_fp32"]["saturated"]["ias_per_s"], 3),
+                      "speedup_serving_vs_plain_fp32": round(
+                          out["mfma_f16_split"]["saturated"]["ias_per_s"]
+                          / out["plain_fp32_one_at_a_time"]["saturated"]["ias_per_s"], 3)}),
           flush=True)
 
 
