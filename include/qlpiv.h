@@ -168,10 +168,32 @@ int qlp_warp_micro_tiles_dev(qlp_ctx *ctx, const uint8_t *img, int H, int W, int
                              const int *top, const int *left, const float *u, const float *v,
                              uint8_t *tiles, float *max_out);
 
+/* Bilinear mini warp of the second-image tiles of 'pairs' frames (img: pairs x
+ * H x W) over a grid of nI x nJ tiles of N px (BiLinearImageMicroAndMiniWarping
+ * Strategy.warpAndClipImage :132-148 with createMiniDisplacements :203-290):
+ * window at (top + floor(u), left + floor(v)), each pixel resampled at the
+ * spline field of the tile's 7 x 7 neighbourhood (neighbours' displacements
+ * minus floor of the tile's own; BiCubicSplineInterpolatorWithBiLinearBackup).
+ * top/left: device, nI*nJ origins; u, v: device, pairs x nI*nJ; tiles: pairs x
+ * nI*nJ x N x N; max_out = 255 (may be NULL).  nI, nJ >= 2. */
+int qlp_warp_mini_tiles_dev(qlp_ctx *ctx, const uint8_t *img, int H, int W, int pairs, int nI,
+                            int nJ, int N, const int *top, const int *left, const float *u,
+                            const float *v, uint8_t *tiles, float *max_out);
+/* Full-image bilinear warp of 'pairs' second frames (BiLinearImageWarpingStrategy
+ * .warpAndClipImage :38-141, SecondImage): every pixel resampled at the spline
+ * field of the whole grid (createTileDisplacementInterpolator).  top/left: HOST
+ * arrays of the grid's nI*nJ origins (the knots); u, v: device, pairs x nI*nJ;
+ * out: device, pairs x H x W.  nI, nJ >= 2. */
+int qlp_warp_image_dev(qlp_ctx *ctx, const uint8_t *img, int H, int W, int pairs, int nI, int nJ,
+                       int N, const int *top, const int *left, const float *u, const float *v,
+                       uint8_t *out);
+
 /* ---- correlation pass straight from device-resident images ---------------- */
 
 #define QLP_WARP_NONE 0  /* NoImageWarpingStrategy: B window at round(top+u), round(left+v) */
 #define QLP_WARP_MICRO 1 /* BiLinearImageMicroAndMiniWarpingStrategy (micro, SecondImage) */
+#define QLP_WARP_MINI 2  /* BiLinearImageMicroAndMiniWarpingStrategy (mini, SecondImage) */
+#define QLP_WARP_FULL 3  /* BiLinearImageWarpingStrategy (whole frame B, SecondImage) */
 
 /* Clip/warp, correlate and FromCenter-search n IAs of one frame pair in a
  * single kernel (the fused form of qlp_clip_tiles_dev + qlp_warp_micro_tiles_dev

@@ -76,6 +76,19 @@ def lib():
         L.or_gauss_kernel.argtypes = [c_float, c_int, _f32p]
         L.or_gaussian_filter2d.argtypes = [_f32p, c_int, c_int, _f32p, c_int, _f32p]
         L.or_convolve1d.argtypes = [_f32p, c_int, _f32p, c_int, _f32p]
+        L.or_natural_slopes.argtypes = [c_int, _f64p, _f64p, _f64p]
+        L.or_spline_surface.argtypes = [c_int, _f64p, c_int, _f64p, _f64p, _f64p, _f64p, _f64p]
+        L.or_spline_surface_value.argtypes = [c_int, _f64p, c_int, _f64p, _f64p, _f64p, _f64p,
+                                              _f64p, ctypes.c_double, ctypes.c_double]
+        L.or_spline_surface_value.restype = ctypes.c_double
+        L.or_bilinear_value.argtypes = [c_int, _f64p, c_int, _f64p, _f64p, ctypes.c_double,
+                                        ctypes.c_double]
+        L.or_bilinear_value.restype = ctypes.c_double
+        L.or_warp_mini.argtypes = [_u8p, c_int, c_int, c_int, c_int, _i32p, _i32p, c_int, _f32p,
+                                   _f32p, c_int, c_int, _u8p]
+        L.or_full_field.argtypes = [c_int, c_int, _i32p, _i32p, c_int, c_int, _f32p, _f32p, c_int,
+                                    c_int, _f32p, _f32p]
+        L.or_warp_image.argtypes = [_u8p, c_int, c_int, _f32p, _f32p, _u8p]
         _lib = L
     return _lib
 
@@ -295,6 +308,74 @@ def warp_micro(img, top, left, N, du, dv):
     lib().or_warp_micro(img, img.shape[0], img.shape[1], int(top), int(left), N, float(du),
                         float(dv), tile)
     return tile
+
+
+def natural_slopes(x, y):
+    """Slopes at the knots of the natural cubic spline through (x, y)."""
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.ascontiguousarray(y, np.float64)
+    s = np.zeros_like(x)
+    lib().or_natural_slopes(x.size, x, y, s)
+    return s
+
+
+def spline_surface(xs, ys, f):
+    """(fx, fy, fxy) knot slopes of the bicubic spline surface, f[i, j] at (xs[i], ys[j])."""
+    xs = np.ascontiguousarray(xs, np.float64)
+    ys = np.ascontiguousarray(ys, np.float64)
+    f = np.ascontiguousarray(f, np.float64)
+    fx, fy, fxy = (np.zeros_like(f) for _ in range(3))
+    lib().or_spline_surface(xs.size, xs, ys.size, ys, f, fx, fy, fxy)
+    return fx, fy, fxy
+
+
+def spline_surface_value(xs, ys, f, x, y):
+    xs = np.ascontiguousarray(xs, np.float64)
+    ys = np.ascontiguousarray(ys, np.float64)
+    f = np.ascontiguousarray(f, np.float64)
+    fx, fy, fxy = spline_surface(xs, ys, f)
+    return lib().or_spline_surface_value(xs.size, xs, ys.size, ys, f, fx, fy, fxy, float(x),
+                                         float(y))
+
+
+def bilinear_value(ys, xs, v, y, x):
+    ys = np.ascontiguousarray(ys, np.float64)
+    xs = np.ascontiguousarray(xs, np.float64)
+    v = np.ascontiguousarray(v, np.float64)
+    return lib().or_bilinear_value(ys.size, ys, xs.size, xs, v, float(y), float(x))
+
+
+def warp_mini(img, top, left, N, u, v, ti, tj):
+    """Mini-warped tile B (ti, tj) of a grid (top, left: nI x nJ origins, u, v: the
+    grid's displacements)."""
+    img = np.ascontiguousarray(img, np.uint8)
+    top = np.ascontiguousarray(top, np.int32)
+    left = np.ascontiguousarray(left, np.int32)
+    u = np.ascontiguousarray(u, np.float32)
+    v = np.ascontiguousarray(v, np.float32)
+    tile = np.zeros((N, N), np.uint8)
+    lib().or_warp_mini(img, img.shape[0], img.shape[1], top.shape[0], top.shape[1], top, left, N,
+                       u, v, int(ti), int(tj), tile)
+    return tile
+
+
+def full_field(top, left, N, u, v, H, W):
+    top = np.ascontiguousarray(top, np.int32)
+    left = np.ascontiguousarray(left, np.int32)
+    u = np.ascontiguousarray(u, np.float32)
+    v = np.ascontiguousarray(v, np.float32)
+    fu = np.zeros((H, W), np.float32)
+    fv = np.zeros((H, W), np.float32)
+    lib().or_full_field(top.shape[0], top.shape[1], top, left, N, N, u, v, H, W, fu, fv)
+    return fu, fv
+
+
+def warp_image(img, fu, fv):
+    img = np.ascontiguousarray(img, np.uint8)
+    out = np.zeros_like(img)
+    lib().or_warp_image(img, img.shape[0], img.shape[1], np.ascontiguousarray(fu, np.float32),
+                        np.ascontiguousarray(fv, np.float32), out)
+    return out
 
 
 # --------------------------------------------------------------------------

@@ -2241,3 +2241,257 @@ void or_gaussian_filter2d(const float *in, int H, int W, const float *kernel, in
     free(buf);
     free(res);
 }
+
+/* ------------------------------------------------------------------------ */
+/* Spline displacement fields of the Mini and Full-image warps:              */
+/* iareas/BiCubicSplineInterpolatorWithBiLinearBackup.java over commons-math3 */
+/* 3.6.1 BicubicSplineInterpolator (absent here; restated from its published  */
+/* algorithm: natural cubic splines give the knot slopes, the cross slope is  */
+/* the divided difference over the neighbouring knots, one bicubic Hermite    */
+/* patch per cell), and iareas/BiLinearInterpolatingFunction.java.  FP64.     */
+/* ------------------------------------------------------------------------ */
+
+#define OR_MAX_KNOTS 1024
+
+/* Natural cubic spline through (x[k], y[k]), k < n, n >= 3: its slope at every
+ * knot.  SplineInterpolator.interpolate (Burden & Faires, Numerical Analysis,
+ * Algorithm 3.4: natural end conditions, forward elimination of the tridiagonal
+ * system, back substitution), then PolynomialSplineFunction.derivative() at the
+ * knots: each knot's own piece at 0, the last knot at the end of the last piece. */
+void or_natural_slopes(int n, const double *x, const double *y, double *slope)
+{
+    const int m = n - 1; /* intervals (n <= OR_MAX_KNOTS) */
+    double h[OR_MAX_KNOTS], mu[OR_MAX_KNOTS], z[OR_MAX_KNOTS + 1], b[OR_MAX_KNOTS],
+        c[OR_MAX_KNOTS + 1], d[OR_MAX_KNOTS];
+    for (int i = 0; i < m; i++) h[i] = x[i + 1] - x[i];
+    mu[0] = 0.0;
+    z[0] = 0.0;
+    for (int i = 1; i < m; i++) {
+        const double g = 2.0 * (x[i + 1] - x[i - 1]) - h[i - 1] * mu[i - 1];
+        mu[i] = h[i] / g;
+        z[i] = (3.0 * (y[i + 1] * h[i - 1] - y[i] * (x[i + 1] - x[i - 1]) + y[i - 1] * h[i]) /
+                    (h[i - 1] * h[i]) -
+                h[i - 1] * z[i - 1]) /
+               g;
+    }
+    z[m] = 0.0;
+    c[m] = 0.0;
+    for (int j = m - 1; j >= 0; j--) {
+        c[j] = z[j] - mu[j] * c[j + 1];
+        b[j] = (y[j + 1] - y[j]) / h[j] - h[j] * (c[j + 1] + 2.0 * c[j]) / 3.0;
+        d[j] = (c[j + 1] - c[j]) / (3.0 * h[j]);
+    }
+    for (int j = 0; j < m; j++) slope[j] = b[j];
+    /* derivative polynomial b + 2c t + 3d t^2 of the last piece at t = h (Horner) */
+    slope[m] = (3.0 * d[m - 1] * h[m - 1] + 2.0 * c[m - 1]) * h[m - 1] + b[m - 1];
+}
+
+/* BicubicSplineInterpolator.interpolate(xs, ys, f): f[i * ny + j] at (xs[i], ys[j]);
+ * slopes along x (splines over xs of every column j), along y (splines over ys
+ * of every row i), and the cross slope
+ *   (f[nI][nJ] - f[nI][pJ] - f[pI][nJ] + f[pI][pJ]) / ((xs[nI]-xs[pI]) (ys[nJ]-ys[pJ]))
+ * with nI = i + 1 (i at the last knot), pI = i - 1 (i at the first). */
+void or_spline_surface(int nx, const double *xs, int ny, const double *ys, const double *f,
+                       double *fx, double *fy, double *fxy)
+{
+    double col[OR_MAX_KNOTS], s[OR_MAX_KNOTS];
+    for (int j = 0; j < ny; j++) {
+        for (int i = 0; i < nx; i++) col[i] = f[i * ny + j];
+        or_natural_slopes(nx, xs, col, s);
+        for (int i = 0; i < nx; i++) fx[i * ny + j] = s[i];
+    }
+    for (int i = 0; i < nx; i++) {
+        or_natural_slopes(ny, ys, f + i * ny, s);
+        for (int j = 0; j < ny; j++) fy[i * ny + j] = s[j];
+    }
+    for (int i = 0; i < nx; i++) {
+        const int nI = i + 1 < nx ? i + 1 : i, pI = i - 1 >= 0 ? i - 1 : 0;
+        for (int j = 0; j < ny; j++) {
+            const int nJ = j + 1 < ny ? j + 1 : j, pJ = j - 1 >= 0 ? j - 1 : 0;
+            fxy[i * ny + j] = (f[nI * ny + nJ] - f[nI * ny + pJ] - f[pI * ny + nJ] + f[pI * ny + pJ]) /
+                              ((xs[nI] - xs[pI]) * (ys[nJ] - ys[pJ]));
+        }
+    }
+}
+
+/* commons BicubicSplineInterpolatingFunction.searchIndex: the cell of c */
+static int spline_cell(double c, const double *v, int n)
+{
+    if (c < v[0]) return -1;
+    for (int i = 1; i < n; i++)
+        if (c <= v[i]) return i - 1;
+    return -1;
+}
+
+/* BicubicSplineInterpolatingFunction.value(x, y): the cell's bicubic patch, in
+ * Hermite form on the normalised cell coordinates s, t (the patch matching the
+ * corner values, slopes times the cell size and cross slopes times both sizes). */
+double or_spline_surface_value(int nx, const double *xs, int ny, const double *ys, const double *f,
+                               const double *fx, const double *fy, const double *fxy, double x,
+                               double y)
+{
+    const int i = spline_cell(x, xs, nx), j = spline_cell(y, ys, ny);
+    if (i < 0 || j < 0) return NAN;
+    const double dx = xs[i + 1] - xs[i], dy = ys[j + 1] - ys[j];
+    const double s = (x - xs[i]) / dx, t = (y - ys[j]) / dy;
+    const double hs[2] = {(1.0 + 2.0 * s) * (1.0 - s) * (1.0 - s), s * s * (3.0 - 2.0 * s)};
+    const double gs[2] = {s * (1.0 - s) * (1.0 - s), s * s * (s - 1.0)};
+    const double ht[2] = {(1.0 + 2.0 * t) * (1.0 - t) * (1.0 - t), t * t * (3.0 - 2.0 * t)};
+    const double gt[2] = {t * (1.0 - t) * (1.0 - t), t * t * (t - 1.0)};
+    double r = 0.0;
+    for (int a = 0; a < 2; a++)
+        for (int b = 0; b < 2; b++) {
+            const int k = (i + a) * ny + (j + b);
+            r += f[k] * hs[a] * ht[b] + fx[k] * dx * gs[a] * ht[b] + fy[k] * dy * hs[a] * gt[b] +
+                 fxy[k] * dx * dy * gs[a] * gt[b];
+        }
+    return r;
+}
+
+/* BiLinearInterpolatingFunction.value(y, x) :40-70 (ys rows, xs columns) */
+double or_bilinear_value(int ny, const double *ys, int nx, const double *xs, const double *v,
+                         double y, double x)
+{
+    int I, J;
+    for (I = 1; I < ny; I++)
+        if (y <= ys[I]) break;
+    for (J = 1; J < nx; J++)
+        if (x <= xs[J]) break;
+    if (I >= ny) I = ny - 1;
+    if (J >= nx) J = nx - 1;
+    const double wY = 1.0 - ((ys[I] - y) / (ys[I] - ys[I - 1]));
+    const double wX = 1.0 - ((xs[J] - x) / (xs[J] - xs[J - 1]));
+    return wY * (wX * v[I * nx + J] + (1.0 - wX) * v[I * nx + J - 1]) +
+           (1.0 - wY) * (wX * v[(I - 1) * nx + J] + (1.0 - wX) * v[(I - 1) * nx + J - 1]);
+}
+
+/* A displacement field over knots ys (rows) x xs (columns): the bicubic spline
+ * when both directions hold at least 3 knots (the natural spline's minimum),
+ * otherwise the bilinear backup; queries are clamped to the knot range in
+ * float (interpolateDisplacements :55-104). */
+typedef struct {
+    int ny, nx, spline;
+    double *ys, *xs;
+    double *f[2], *fx[2], *fy[2], *fxy[2];
+    float ymin, ymax, xmin, xmax;
+} or_field;
+
+static void field_alloc(or_field *F, int ny, int nx)
+{
+    F->ny = ny;
+    F->nx = nx;
+    F->ys = (double *)malloc(sizeof(double) * ny);
+    F->xs = (double *)malloc(sizeof(double) * nx);
+    for (int c = 0; c < 2; c++) {
+        F->f[c] = (double *)malloc(sizeof(double) * ny * nx);
+        F->fx[c] = (double *)malloc(sizeof(double) * ny * nx);
+        F->fy[c] = (double *)malloc(sizeof(double) * ny * nx);
+        F->fxy[c] = (double *)malloc(sizeof(double) * ny * nx);
+    }
+}
+
+static void field_free(or_field *F)
+{
+    free(F->ys);
+    free(F->xs);
+    for (int c = 0; c < 2; c++) {
+        free(F->f[c]);
+        free(F->fx[c]);
+        free(F->fy[c]);
+        free(F->fxy[c]);
+    }
+}
+
+static void field_build(or_field *F)
+{
+    F->spline = F->ny >= 3 && F->nx >= 3;
+    if (F->spline)
+        for (int c = 0; c < 2; c++)
+            or_spline_surface(F->ny, F->ys, F->nx, F->xs, F->f[c], F->fx[c], F->fy[c], F->fxy[c]);
+    F->ymin = (float)F->ys[0];
+    F->ymax = (float)F->ys[F->ny - 1];
+    F->xmin = (float)F->xs[0];
+    F->xmax = (float)F->xs[F->nx - 1];
+}
+
+static float field_value(const or_field *F, int c, float y, float x)
+{
+    if (x < F->xmin) x = F->xmin;
+    else if (x > F->xmax) x = F->xmax;
+    if (y < F->ymin) y = F->ymin;
+    else if (y > F->ymax) y = F->ymax;
+    if (F->spline)
+        return (float)or_spline_surface_value(F->ny, F->ys, F->nx, F->xs, F->f[c], F->fx[c],
+                                              F->fy[c], F->fxy[c], (double)y, (double)x);
+    return (float)or_bilinear_value(F->ny, F->ys, F->nx, F->xs, F->f[c], (double)y, (double)x);
+}
+
+/* Mini warp of tile B (ti, tj) in SecondImage mode (warpAndClipImage :132-148,
+ * createMiniDisplacements :203-290, doBiLinearWarp :292-334): the window is
+ * shifted by floor(u), floor(v) of the tile; every pixel is sampled at the
+ * field of the tile's 7 x 7 neighbourhood (cut to the grid), whose knots are the
+ * tile centres and whose values are the neighbours' displacements minus
+ * floor of this tile's.  Grid of nI x nJ tiles of N px, row-major tops/lefts. */
+void or_warp_mini(const uint8_t *img, int H, int W, int nI, int nJ, const int *top,
+                  const int *left, int N, const float *u, const float *v, int ti, int tj,
+                  uint8_t *tile)
+{
+    or_field F;
+    const int t = ti * nJ + tj;
+    const double fu0 = floor((double)u[t]), fv0 = floor((double)v[t]);
+    const short tTop = (short)(top[t] + floor(u[t]));
+    const short tLeft = (short)(left[t] + floor(v[t]));
+    const int i0 = ti - 3 < 0 ? 0 : ti - 3, i1 = ti + 3 > nI - 1 ? nI - 1 : ti + 3;
+    const int j0 = tj - 3 < 0 ? 0 : tj - 3, j1 = tj + 3 > nJ - 1 ? nJ - 1 : tj + 3;
+    field_alloc(&F, i1 - i0 + 1, j1 - j0 + 1);
+    for (int i = i0; i <= i1; i++) F.ys[i - i0] = (double)(top[i * nJ + j0] + N / 2.0f - 0.5f);
+    for (int j = j0; j <= j1; j++) F.xs[j - j0] = (double)(left[i0 * nJ + j] + N / 2.0f - 0.5f);
+    for (int i = i0; i <= i1; i++)
+        for (int j = j0; j <= j1; j++) {
+            F.f[0][(i - i0) * F.nx + (j - j0)] = (double)u[i * nJ + j] - fu0;
+            F.f[1][(i - i0) * F.nx + (j - j0)] = (double)v[i * nJ + j] - fv0;
+        }
+    field_build(&F);
+    for (int i = 0; i < N; i++)
+        for (int j = 0; j < N; j++) {
+            const float y = (float)(tTop + i), x = (float)(tLeft + j);
+            const float du = field_value(&F, 0, y, x), dv = field_value(&F, 1, y, x);
+            tile[i * N + j] =
+                (uint8_t)warp_pixel(img, H, W, du, dv, (short)(i + tTop), (short)(j + tLeft));
+        }
+    field_free(&F);
+}
+
+/* Full-image warp field (BiLinearImageWarpingStrategy.warpAndClipImage :46-57,
+ * createTileDisplacementInterpolator): the field over the whole grid's tile
+ * centres, evaluated at every pixel (i rows, j columns) of an H x W image. */
+void or_full_field(int nI, int nJ, const int *top, const int *left, int tH, int tW,
+                   const float *u, const float *v, int H, int W, float *fu, float *fv)
+{
+    or_field F;
+    field_alloc(&F, nI, nJ);
+    for (int i = 0; i < nI; i++) F.ys[i] = (double)(top[i * nJ] + tH / 2.0f - 0.5f);
+    for (int j = 0; j < nJ; j++) F.xs[j] = (double)(left[j] + tW / 2.0f - 0.5f);
+    for (int k = 0; k < nI * nJ; k++) {
+        F.f[0][k] = (double)u[k];
+        F.f[1][k] = (double)v[k];
+    }
+    field_build(&F);
+    for (int i = 0; i < H; i++)
+        for (int j = 0; j < W; j++) {
+            fu[(size_t)i * W + j] = field_value(&F, 0, (float)i, (float)j);
+            fv[(size_t)i * W + j] = field_value(&F, 1, (float)i, (float)j);
+        }
+    field_free(&F);
+}
+
+/* BiLinearImageWarpingStrategy.doBiLinearWarp :143-183, SecondImage: every
+ * pixel of the image sampled at its own field displacement. */
+void or_warp_image(const uint8_t *img, int H, int W, const float *fu, const float *fv,
+                   uint8_t *out)
+{
+    for (int i = 0; i < H; i++)
+        for (int j = 0; j < W; j++)
+            out[(size_t)i * W + j] = (uint8_t)warp_pixel(img, H, W, fu[(size_t)i * W + j],
+                                                         fv[(size_t)i * W + j], (short)i, (short)j);
+}

@@ -88,6 +88,8 @@ struct PassParams {
     const int *top = nullptr, *left = nullptr;
     const float *du = nullptr, *dv = nullptr;
     int bmode = B_CLIP;
+    bool bmax255 = false;  // B tiles carry the maximum 255 (cut from a warped image:
+                           // a new MatrixByte region, MatrixByte.java:40-45, 366-384)
     float *planes = nullptr;
     float *peaks = nullptr, *min_floor = nullptr;
     float *nbr = nullptr;
@@ -115,6 +117,14 @@ int launch_clip(Context *ctx, const uint8_t *img, int H, int W, int n, int N, co
 int launch_warp_micro(Context *ctx, const uint8_t *img, int H, int W, int n, int N, const int *top,
                       const int *left, const float *u, const float *v, uint8_t *tiles,
                       float *maxv);
+// Spline-field warps (qlpiv_splinewarp.hip).
+int launch_warp_mini(Context *ctx, int pairs, const uint8_t *imgs, int H, int W, int nI, int nJ,
+                     int N, const int *top, const int *left, const float *u, const float *v,
+                     uint8_t *tiles, float *maxv);
+size_t full_warp_work_bytes(int pairs, int nI, int nJ);
+int launch_full_warp(Context *ctx, int pairs, const uint8_t *imgs, int H, int W, int nI, int nJ,
+                     int tH, int tW, const int *top_h, const int *left_h, const float *u,
+                     const float *v, void *work, uint8_t *out);
 
 // Vector validation (qlpiv_validate.hip).  The device arrays of one launch:
 // inputs, outputs and the workspace carved by valid_work_carve.

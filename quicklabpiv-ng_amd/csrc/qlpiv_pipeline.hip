@@ -23,6 +23,8 @@
 
 #include "qlpiv_internal.h"
 
+#include <algorithm>
+
 namespace qlpiv {
 
 struct Level {
@@ -132,6 +134,13 @@ struct qlp_piv_plan {
     qlp_piv_config cfg{};
     std::vector<Level> levels;
     size_t bytes = 0;
+    // warp buffers of the levels after the first: Mini warp A and B tiles and
+    // their maxima; Full-image warp: the warped second frames and the field
+    // workspace (sized for the largest level)
+    uint8_t *tA = nullptr, *tB = nullptr, *wimg = nullptr;
+    float *mA = nullptr, *mB = nullptr;
+    void *fwork = nullptr;
+    size_t tiles_bytes = 0, fwork_bytes = 0;
     // optional per-level timing: events before the state kernel, after the
     // correlation pass, after the fit and after the validation of every level
     bool timing = false;
@@ -147,6 +156,12 @@ static void plan_free(qlp_piv_plan *p)
         for (void *b : bufs)
             if (b) (void)hipFree(b);
     }
+    for (void *b : {(void *)p->tA, (void *)p->tB, (void *)p->wimg, (void *)p->mA, (void *)p->mB,
+                    p->fwork})
+        if (b) (void)hipFree(b);
+    p->tA = p->tB = p->wimg = nullptr;
+    p->mA = p->mB = nullptr;
+    p->fwork = nullptr;
     for (hipEvent_t e : p->ev) (void)hipEventDestroy(e);
     p->ev.clear();
     if (p->ctx) p->ctx->hbm_in_use -= p->bytes;
@@ -212,7 +227,7 @@ static int plan_layout(int H, int W, int pairs, const qlp_piv_config *cfg, qlp_p
     while ((1 << (nlev - 1)) < steps) nlev++;
     if ((1 << (nlev - 1)) != steps)
         return fail(QLP_ERR_ARG, "Total number of steps could not be properly computed since step sizes ratio is not a power of two");
-    if (cfg->warp_mode != QLP_WARP_NONE && cfg->warp_mode != QLP_WARP_MICRO)
+    if (cfg->warp_mode < QLP_WARP_NONE || cfg->warp_mode > QLP_WARP_FULL)
         return fail(QLP_ERR_ARG, "piv_plan_create: unknown warp mode");
     if (cfg->norm_mode != QLP_NORM_PER_TILE && cfg->norm_mode != QLP_NORM_A_MAX)
         return fail(QLP_ERR_ARG, "piv_plan_create: unknown normalisation mode");
@@ -264,8 +279,23 @@ static int plan_layout(int H, int W, int pairs, const qlp_piv_config *cfg, qlp_p
             need += pn * D * D * 4;
         if (multi) need += pn * 4;
         if (valid) need += n * 4 + valid_work_bytes(pairs, (int)n) + pn + (size_t)pairs * 16;
+        if (l > 0 && (cfg->warp_mode == QLP_WARP_MINI || cfg->warp_mode == QLP_WARP_FULL) &&
+            (L.nI < 2 || L.nJ < 2))
+            return fail(QLP_ERR_UNSUPPORTED,
+                        "piv_plan_create: spline warps need 2 tiles in each direction");
+        if (l > 0 && cfg->warp_mode == QLP_WARP_MINI)
+            p->tiles_bytes = std::max(p->tiles_bytes, pn * (size_t)tile * tile);
+        if (l > 0 && cfg->warp_mode == QLP_WARP_FULL)
+            p->fwork_bytes = std::max(p->fwork_bytes, full_warp_work_bytes(pairs, L.nI, L.nJ));
         p->levels.push_back(std::move(L));
     }
+    if (cfg->warp_mode == QLP_WARP_MINI && p->tiles_bytes) {
+        size_t mx = 0;
+        for (const Level &L : p->levels) mx = std::max(mx, (size_t)pairs * L.n);
+        need += 2 * p->tiles_bytes + 2 * mx * 4;
+    }
+    if (cfg->warp_mode == QLP_WARP_FULL && p->fwork_bytes)
+        need += (size_t)pairs * H * W + p->fwork_bytes;
     return QLP_OK;
 }
 
@@ -334,6 +364,18 @@ int qlp_piv_plan_create(qlp_ctx *h, int H, int W, int pairs, const qlp_piv_confi
             rc = check_hip(hipMemcpy(L.left, L.left_h.data(), n * 4, hipMemcpyHostToDevice), "upload grid");
         if (!rc && l > 0 && cfg->inherit_mode != QLP_INHERIT_DIRECT)
             rc = prepare_inherit(cfg->inherit_mode, p->levels[l - 1], L, need);
+    }
+    if (!rc && cfg->warp_mode == QLP_WARP_MINI && p->tiles_bytes) {
+        size_t mx = 0;
+        for (const Level &L : p->levels) mx = std::max(mx, (size_t)pairs * L.n);
+        rc = check_hip(hipMalloc(&p->tA, p->tiles_bytes), "hipMalloc plan");
+        if (!rc) rc = check_hip(hipMalloc(&p->tB, p->tiles_bytes), "hipMalloc plan");
+        if (!rc) rc = check_hip(hipMalloc(&p->mA, mx * 4), "hipMalloc plan");
+        if (!rc) rc = check_hip(hipMalloc(&p->mB, mx * 4), "hipMalloc plan");
+    }
+    if (!rc && cfg->warp_mode == QLP_WARP_FULL && p->fwork_bytes) {
+        rc = check_hip(hipMalloc(&p->wimg, (size_t)pairs * H * W), "hipMalloc plan");
+        if (!rc) rc = check_hip(hipMalloc(&p->fwork, p->fwork_bytes), "hipMalloc plan");
     }
     p->bytes = need;
     ctx->hbm_in_use += need;
@@ -423,6 +465,32 @@ int qlp_piv_run_dev(qlp_piv_plan *p, const uint8_t *imgsA, const uint8_t *imgsB)
         q.du = (l == 0) ? nullptr : L.u;
         q.dv = (l == 0) ? nullptr : L.v;
         q.bmode = c.warp_mode == QLP_WARP_MICRO ? B_WARP_MICRO : B_CLIP_SHIFT;
+        if (l > 0 && c.warp_mode == QLP_WARP_MINI) {
+            // clip A and mini-warp B into tiles, then correlate from the tiles
+            const size_t tN = (size_t)L.N * L.N;
+            for (int k = 0; k < p->pairs && !rc; k++)
+                rc = launch_clip(ctx, imgsA + (size_t)k * p->H * p->W, p->H, p->W, L.n, L.N, L.top,
+                                 L.left, p->tA + (size_t)k * L.n * tN, p->mA + (size_t)k * L.n);
+            if (!rc)
+                rc = launch_warp_mini(ctx, p->pairs, imgsB, p->H, p->W, L.nI, L.nJ, L.N, L.top,
+                                      L.left, L.u, L.v, p->tB, p->mB);
+            if (rc) return rc;
+            q.imgA = q.imgB = nullptr;
+            q.tA = p->tA;
+            q.tB = p->tB;
+            q.maxA = p->mA;
+            q.maxB = p->mB;
+            q.du = q.dv = nullptr;
+        } else if (l > 0 && c.warp_mode == QLP_WARP_FULL) {
+            // warp the whole second frames, then cut B at (top, left) with maximum 255
+            rc = launch_full_warp(ctx, p->pairs, imgsB, p->H, p->W, L.nI, L.nJ, L.N, L.N,
+                                  L.top_h.data(), L.left_h.data(), L.u, L.v, p->fwork, p->wimg);
+            if (rc) return rc;
+            q.imgB = p->wimg;
+            q.du = q.dv = nullptr;
+            q.bmode = B_CLIP;
+            q.bmax255 = true;
+        }
         q.planes = L.planes;
         q.arith = c.xcorr_form;
         if (!multi) {  // FromCenter and the fit neighbourhood fused into the pass

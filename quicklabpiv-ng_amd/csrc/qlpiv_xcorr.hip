@@ -306,7 +306,7 @@ __global__ __launch_bounds__(XcShape<LOGM>::THREADS) void xcorr_pass_kernel(
         }
         amax = block_max_int<NT>(amax, s_red);
         mA = (float)(amax < 16 ? 16 : amax);
-        if (warp) {
+        if (warp || p.bmax255) {
             mB = 255.0f;  // warped tiles carry the image maximum (doBiLinearWarp :298)
         } else {
             bmax = block_max_int<NT>(bmax, s_red);

@@ -221,7 +221,7 @@ __global__ __launch_bounds__(Lx<LOGM>::THREADS) void xcorr_lane_kernel(
             }
         }
         mA = (float)(amax < 16 ? 16 : amax);
-        mB = warp ? 255.0f : (float)(bmax < 16 ? 16 : bmax);  // doBiLinearWarp :298
+        mB = (warp || p.bmax255) ? 255.0f : (float)(bmax < 16 ? 16 : bmax);  // doBiLinearWarp :298
         if (MODE_CPU) mB = mA;
     } else {
         if (valid) {

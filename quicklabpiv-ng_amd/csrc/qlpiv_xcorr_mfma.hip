@@ -313,7 +313,7 @@ __global__ __launch_bounds__(64 * MF_WAVES, 3) void xcorr_mfma_kernel(MfmaParams
                 bmax = max(bmax, __shfl_xor(bmax, o, 64));
             }
             mA = (float)(amax < 16 ? 16 : amax);
-            mB = warp ? 255.0f : (float)(bmax < 16 ? 16 : bmax);
+            mB = (warp || p.bmax255) ? 255.0f : (float)(bmax < 16 ? 16 : bmax);
         } else {
             if (valid) {
                 mA = p.maxA[ia];

@@ -208,6 +208,37 @@ class Context:
                                                 _ptr(mx)))
         return tiles, mx
 
+    def warp_mini_tiles(self, imgs, top, left, u, v, N):
+        """Mini-warped B tiles of every tile of an nI x nJ grid for 'pairs' frames
+        (imgs: pairs x H x W device uint8; top, left: device nI x nJ int32; u, v:
+        device pairs x nI x nJ) -> (pairs x nI*nJ x N x N tiles, maxima)."""
+        import torch
+
+        imgs = imgs if imgs.dim() == 3 else imgs[None]
+        pairs, H, W = (int(x) for x in imgs.shape)
+        nI, nJ = int(top.shape[0]), int(top.shape[1])
+        tiles = torch.empty((pairs, nI * nJ, N, N), dtype=torch.uint8, device=imgs.device)
+        mx = torch.empty((pairs, nI * nJ), dtype=torch.float32, device=imgs.device)
+        check(self.lib.qlp_warp_mini_tiles_dev(self.h, _ptr(imgs), H, W, pairs, nI, nJ, N,
+                                               _ptr(top), _ptr(left), _ptr(u), _ptr(v),
+                                               _ptr(tiles), _ptr(mx)))
+        return tiles, mx
+
+    def warp_image(self, imgs, top, left, u, v, N):
+        """Full-image warp of 'pairs' second frames by the spline field of an
+        nI x nJ grid (top, left: host numpy nI x nJ; u, v: device pairs x nI x nJ)."""
+        import torch
+
+        imgs = imgs if imgs.dim() == 3 else imgs[None]
+        pairs, H, W = (int(x) for x in imgs.shape)
+        top = np.ascontiguousarray(top, np.int32)
+        left = np.ascontiguousarray(left, np.int32)
+        nI, nJ = top.shape
+        out = torch.empty_like(imgs)
+        check(self.lib.qlp_warp_image_dev(self.h, _ptr(imgs), H, W, pairs, nI, nJ, N, _ptr(top),
+                                          _ptr(left), _ptr(u), _ptr(v), _ptr(out)))
+        return out
+
     # ---- image pre-filter ----------------------------------------------------
     def gaussian_filter(self, imgs, sigma, kernel_px):
         """GaussianFilter2D.applyFilter on device images (n x H x W or H x W,

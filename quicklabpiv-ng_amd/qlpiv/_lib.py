@@ -52,6 +52,9 @@ SIGNATURES = {
     "qlp_subpixel_dev": (_i, [_vp, _i, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp]),
     "qlp_clip_tiles_dev": (_i, [_vp, _vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp]),
     "qlp_warp_micro_tiles_dev": (_i, [_vp, _vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "qlp_warp_mini_tiles_dev": (_i, [_vp, _vp, _i, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp,
+                                     _vp]),
+    "qlp_warp_image_dev": (_i, [_vp, _vp, _i, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp]),
     "qlp_xcorr_images_dev": (_i, [_vp, _i, _i, _i, _vp, _vp, _i, _i, _vp, _vp, _vp, _vp, _i,
                                   _vp, _vp, _vp, _vp, _i]),
     "qlp_piv_plan_create": (_i, [_vp, _i, _i, _i, _vp, ctypes.POINTER(_vp)]),
@@ -87,6 +90,8 @@ SIGNATURES = {
 
 QLP_WARP_NONE = 0
 QLP_WARP_MICRO = 1
+QLP_WARP_MINI = 2
+QLP_WARP_FULL = 3
 
 
 QLP_INHERIT_DIRECT = 0
