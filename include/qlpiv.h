@@ -21,7 +21,9 @@
  *   - A context owns one HIP stream.  Calls on one context are serialised by
  *     the caller (one context per calling thread, as the reference keeps one
  *     xcorr job per worker thread: OpenClGpuManager.compute :124-140).
- *   - Tiles are N x N row-major uint8 (IA pixels); N is 16, 32 or 64.
+ *   - Tiles are N x N row-major uint8 (IA pixels); N is 16, 32, 64 or 128
+ *     (128: the correlation keeps its 256 x 256 tile in a context-owned HBM
+ *     workspace of 512 KB per resident work-group, at most 128 MB).
  *     Correlation planes are (2N-1) x (2N-1) row-major float32, element
  *     [i][j] = sum_nm F'[n][m] G'[n+i-(N-1)][m+j-(N-1)], zero displacement at
  *     [N-1][N-1] (MaxCrossResult.java:141-149).

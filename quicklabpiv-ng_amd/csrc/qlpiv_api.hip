@@ -42,40 +42,40 @@ static void build_tables(int M, std::vector<float2> out[3])
     }
 }
 
-static int ensure_scratch(Context *ctx, size_t bytes)
+// Grow a context-owned device buffer to at least 'bytes' (contents are not
+// kept), charging it to the context's HBM budget.
+static int grow_buffer(Context *ctx, void *&buf, size_t &have, size_t bytes, const char *what)
 {
-    if (bytes <= ctx->scratch_bytes) return QLP_OK;
-    if (ctx->hbm_budget && ctx->hbm_in_use - ctx->scratch_bytes + bytes > ctx->hbm_budget)
+    if (bytes <= have) return QLP_OK;
+    if (ctx->hbm_budget && ctx->hbm_in_use - have + bytes > ctx->hbm_budget)
         return fail(QLP_ERR_BUDGET, "request exceeds the context's HBM budget");
-    if (ctx->scratch) {
-        (void)hipFree(ctx->scratch);
-        ctx->hbm_in_use -= ctx->scratch_bytes;
-        ctx->scratch = nullptr;
-        ctx->scratch_bytes = 0;
+    if (buf) {
+        (void)hipFree(buf);
+        ctx->hbm_in_use -= have;
+        buf = nullptr;
+        have = 0;
     }
-    int rc = check_hip(hipMalloc(&ctx->scratch, bytes), "hipMalloc scratch");
+    int rc = check_hip(hipMalloc(&buf, bytes), what);
     if (rc) return rc;
-    ctx->scratch_bytes = bytes;
+    have = bytes;
     ctx->hbm_in_use += bytes;
     return QLP_OK;
 }
 
+static int ensure_scratch(Context *ctx, size_t bytes)
+{
+    return grow_buffer(ctx, ctx->scratch, ctx->scratch_bytes, bytes, "hipMalloc scratch");
+}
+
 int ensure_iwork(Context *ctx, size_t bytes)
 {
-    if (bytes <= ctx->iwork_bytes) return QLP_OK;
-    if (ctx->hbm_budget && ctx->hbm_in_use - ctx->iwork_bytes + bytes > ctx->hbm_budget)
-        return fail(QLP_ERR_BUDGET, "request exceeds the context's HBM budget");
-    if (ctx->iwork) {
-        (void)hipFree(ctx->iwork);
-        ctx->hbm_in_use -= ctx->iwork_bytes;
-        ctx->iwork = nullptr;
-        ctx->iwork_bytes = 0;
-    }
-    int rc = check_hip(hipMalloc(&ctx->iwork, bytes), "hipMalloc inheritance workspace");
-    if (rc) return rc;
-    ctx->iwork_bytes = bytes;
-    ctx->hbm_in_use += bytes;
-    return QLP_OK;
+    return grow_buffer(ctx, ctx->iwork, ctx->iwork_bytes, bytes, "hipMalloc inheritance workspace");
+}
+
+int ensure_bigwork(Context *ctx, int n)
+{
+    return grow_buffer(ctx, ctx->bigwork, ctx->bigwork_bytes, big_work_bytes(n),
+                       "hipMalloc IA-128 workspace");
 }
 
 static int sync_ctx(Context *ctx) { return check_hip(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize"); }
@@ -120,7 +120,7 @@ int qlp_ctx_create(int hip_device, size_t hbm_budget_bytes, qlp_ctx **out)
         delete h;
         return rc;
     }
-    for (int li = 0; li < 3; li++) {
+    for (int li = 0; li < TW_SIZES; li++) {
         const int M = 32 << li;
         std::vector<float2> tabs[3];
         build_tables(M, tabs);
@@ -149,12 +149,13 @@ int qlp_ctx_destroy(qlp_ctx *h)
     Context *ctx = &h->c;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-    for (int li = 0; li < 3; li++)
+    for (int li = 0; li < TW_SIZES; li++)
         for (int t = 0; t < 3; t++)
             if (ctx->tables.tw[li][t]) (void)hipFree(ctx->tables.tw[li][t]);
     if (ctx->scratch) (void)hipFree(ctx->scratch);
     if (ctx->vwork) (void)hipFree(ctx->vwork);
     if (ctx->iwork) (void)hipFree(ctx->iwork);
+    if (ctx->bigwork) (void)hipFree(ctx->bigwork);
     for (int a = 0; a < 2; a++)
         for (int b = 0; b < 2; b++)
             if (ctx->mfma_tab[a][b]) (void)hipFree(ctx->mfma_tab[a][b]);
@@ -174,9 +175,8 @@ void *qlp_ctx_stream(qlp_ctx *h) { return h ? (void *)h->c.stream : nullptr; }
 static int xcorr_args(int n, int N, int mode)
 {
     if (n < 0) return fail(QLP_ERR_ARG, "xcorr: negative IA count");
-    if (N != 16 && N != 32 && N != 64)
-        return fail(N == 128 ? QLP_ERR_UNSUPPORTED : QLP_ERR_ARG,
-                    "xcorr: IA size must be 16, 32 or 64 pixels");
+    if (N != 16 && N != 32 && N != 64 && N != 128)
+        return fail(QLP_ERR_ARG, "xcorr: IA size must be 16, 32, 64 or 128 pixels");
     if (mode != QLP_NORM_PER_TILE && mode != QLP_NORM_A_MAX)
         return fail(QLP_ERR_ARG, "xcorr: unknown normalisation mode");
     return QLP_OK;

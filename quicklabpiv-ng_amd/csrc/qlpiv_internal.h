@@ -16,7 +16,7 @@
 
 namespace qlpiv {
 
-// Twiddle tables, one per transform length M = 2N in {32, 64, 128}, in the
+// Twiddle tables, one per transform length M = 2N in {32, 64, 128, 256}, in the
 // form (c[k], s[k]) consumed by the generic radix-2 butterfly
 //   lo = ((a_r + c*b_r) - s*b_i, (a_i + c*b_i) + s*b_r)
 //   hi = ((a_r - c*b_r) + s*b_i, (a_i - c*b_i) - s*b_r)
@@ -29,8 +29,9 @@ namespace qlpiv {
 // with ws[n] = (float)sin(2*pi*n/M) evaluated in double.
 enum TwKind { TW_FWD = 0, TW_INV_CPU = 1, TW_INV_GPU = 2 };
 
+constexpr int TW_SIZES = 4;
 struct DeviceTables {
-    float2 *tw[3][3];  // [log2M - 5][kind] -> M/2 entries
+    float2 *tw[TW_SIZES][3];  // [log2M - 5][kind] -> M/2 entries
 };
 
 // Two implementations of the same correlation arithmetic: one lane per line of
@@ -55,6 +56,9 @@ struct Context {
     // workspace of qlp_inherit_velocity_dev (grown on demand)
     void *iwork = nullptr;
     size_t iwork_bytes = 0;
+    // per-work-group tile slots of the IA-128 correlation (grown on demand)
+    void *bigwork = nullptr;
+    size_t bigwork_bytes = 0;
     // constant DFT fragments of the MFMA correlation form, [N 16/32][fp16/bf16]
     void *mfma_tab[2][2] = {};
 };
@@ -102,6 +106,11 @@ int launch_xcorr(Context *ctx, int n, int N, const uint8_t *tA, const uint8_t *t
                  const float *maxA, const float *maxB, int mode, float *planes);
 int launch_pass(Context *ctx, const PassParams &p);
 int launch_pass_lane(Context *ctx, const PassParams &p, bool from_image, bool peak);
+// IA 128: the tile lives in an HBM workspace of one 512 KB slot per work-group
+// (at most big_slots(n) slots); ensure_bigwork sizes it for a pass of n IAs.
+int big_slots(int n);
+size_t big_work_bytes(int n);
+int ensure_bigwork(Context *ctx, int n);
 // The throughput form (qlpiv_xcorr_mfma.hip): form is QLP_XCORR_MFMA_*.
 int launch_pass_mfma(Context *ctx, const PassParams &p, int form, bool from_image, bool peak);
 int launch_find_peaks(Context *ctx, int kind, int n, int D, const float *planes, int n_peaks,

@@ -141,6 +141,8 @@ struct qlp_piv_plan {
     float *mA = nullptr, *mB = nullptr;
     void *fwork = nullptr;
     size_t tiles_bytes = 0, fwork_bytes = 0;
+    // IAs of the largest IA-128 pass: sizes the context's tile-slot workspace
+    int big_ias = 0;
     // optional per-level timing: events before the state kernel, after the
     // correlation pass, after the fit and after the validation of every level
     bool timing = false;
@@ -232,8 +234,8 @@ static int plan_layout(int H, int W, int pairs, const qlp_piv_config *cfg, qlp_p
     if (cfg->norm_mode != QLP_NORM_PER_TILE && cfg->norm_mode != QLP_NORM_A_MAX)
         return fail(QLP_ERR_ARG, "piv_plan_create: unknown normalisation mode");
     for (int tile = cfg->ia_start; tile >= cfg->ia_end; tile /= 2)
-        if (tile != 16 && tile != 32 && tile != 64)
-            return fail(QLP_ERR_UNSUPPORTED, "piv_plan_create: IA sizes must be 16, 32 or 64");
+        if (tile != 16 && tile != 32 && tile != 64 && tile != 128)
+            return fail(QLP_ERR_UNSUPPORTED, "piv_plan_create: IA sizes must be 16, 32, 64 or 128");
     if (cfg->subpix_kind >= 0) {
         const int half = cfg->subpix_kind == QLP_SUBPIX_POLY ? 1 : cfg->subpix_pixels / 2;
         if (half > NBR_HALF || (cfg->subpix_kind != QLP_SUBPIX_POLY && cfg->subpix_pixels % 2 == 0))
@@ -283,6 +285,7 @@ static int plan_layout(int H, int W, int pairs, const qlp_piv_config *cfg, qlp_p
             (L.nI < 2 || L.nJ < 2))
             return fail(QLP_ERR_UNSUPPORTED,
                         "piv_plan_create: spline warps need 2 tiles in each direction");
+        if (tile == 128) p->big_ias = std::max(p->big_ias, (int)pn);
         if (l > 0 && cfg->warp_mode == QLP_WARP_MINI)
             p->tiles_bytes = std::max(p->tiles_bytes, pn * (size_t)tile * tile);
         if (l > 0 && cfg->warp_mode == QLP_WARP_FULL)
@@ -308,6 +311,8 @@ int qlp_piv_plan_estimate(int H, int W, int pairs, const qlp_piv_config *cfg, si
     // the inheritance stencils (at most 9 parents per tile) or spline tables
     if (cfg->inherit_mode != QLP_INHERIT_DIRECT)
         for (const Level &L : p.levels) need += L.n * 9 * 8 + 64 * 1024;
+    // the context's IA-128 workspace (shared by the context's plans)
+    if (p.big_ias) need += big_work_bytes(p.big_ias);
     *bytes = need;
     return QLP_OK;
 }
@@ -377,6 +382,7 @@ int qlp_piv_plan_create(qlp_ctx *h, int H, int W, int pairs, const qlp_piv_confi
         rc = check_hip(hipMalloc(&p->wimg, (size_t)pairs * H * W), "hipMalloc plan");
         if (!rc) rc = check_hip(hipMalloc(&p->fwork, p->fwork_bytes), "hipMalloc plan");
     }
+    if (!rc && p->big_ias) rc = ensure_bigwork(ctx, p->big_ias);
     p->bytes = need;
     ctx->hbm_in_use += need;
     if (rc) {

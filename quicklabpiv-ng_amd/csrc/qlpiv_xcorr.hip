@@ -62,23 +62,26 @@ __device__ __forceinline__ void bfly(float &ar, float &ai, float &br, float &bi,
 // One 1D transform over 'nlines' lines of M complex values held in LDS.
 // Element (line l, index o) lives at l*LS + o*PS.  The input is in natural
 // order (the bit-reversal shuffle is folded into the first gather); the output
-// is in natural order, in place.
+// is in natural order, in place.  NT threads share the work; at most MAXLINES
+// lines per call.
 //   ZERO_HALF : indices >= M/2 are zero padding (forward passes)
 //   PRESCALE  : divide every input by M first (the CPU IFFT's divideArray)
-template <int LOGM, bool ZERO_HALF, bool PRESCALE>
+template <int LOGM, bool ZERO_HALF, bool PRESCALE, int NT = XcShape<LOGM>::THREADS,
+          int MAXLINES = XcShape<LOGM>::M>
 __device__ __forceinline__ void fft_lines(float *__restrict__ re, float *__restrict__ im, int LS,
                                           int PS, int nlines, const float2 *__restrict__ tw)
 {
     using X = XcShape<LOGM>;
     constexpr int M = X::M, E = X::E, LOGE = X::LOGE, T = X::T, B = LOGM - LOGE;
+    constexpr int ROUNDS = (MAXLINES * T + NT - 1) / NT;
     const int ntasks = nlines * T;
-    float rr[X::ROUNDS][E], ri[X::ROUNDS][E];
+    float rr[ROUNDS][E], ri[ROUNDS][E];
 
     // pass 1 gather: thread (l, t) takes positions p = t*E + q, which after the
     // perfect shuffle hold the original indices bitrev(p)
 #pragma unroll
-    for (int rd = 0; rd < X::ROUNDS; rd++) {
-        const int task = threadIdx.x + rd * X::THREADS;
+    for (int rd = 0; rd < ROUNDS; rd++) {
+        const int task = threadIdx.x + rd * NT;
         if (task < ntasks) {
             const int l = task % nlines, t = task / nlines;
             const int base = l * LS;
@@ -92,8 +95,8 @@ __device__ __forceinline__ void fft_lines(float *__restrict__ re, float *__restr
     }
     __syncthreads();
 #pragma unroll
-    for (int rd = 0; rd < X::ROUNDS; rd++) {
-        const int task = threadIdx.x + rd * X::THREADS;
+    for (int rd = 0; rd < ROUNDS; rd++) {
+        const int task = threadIdx.x + rd * NT;
         if (task < ntasks) {
             const int l = task % nlines, t = task / nlines;
             const int base = l * LS;
@@ -147,7 +150,7 @@ __device__ __forceinline__ void fft_lines(float *__restrict__ re, float *__restr
     }
     __syncthreads();
     // pass 2: positions p = t + (q << B), stages LOGE..LOGM-1, in place
-    for (int task = threadIdx.x; task < ntasks; task += X::THREADS) {
+    for (int task = threadIdx.x; task < ntasks; task += NT) {
         const int l = task % nlines, t = task / nlines;
         const int base = l * LS;
         float a[E], b[E];
@@ -182,12 +185,12 @@ __device__ __forceinline__ void fft_lines(float *__restrict__ re, float *__restr
 // (k, -k) is owned by one thread that reads both values before writing both.
 // The formulas are symmetric (Fr, Gr) or antisymmetric (Fi, Gi) in the pair, so
 // P[-k] = conj(P[k]) holds bit for bit whichever element owns the pair.
-template <int LOGM, bool MODE_CPU>
+// Row stride S; NT threads.
+template <int LOGM, bool MODE_CPU, int NT = XcShape<LOGM>::THREADS, int S = XcShape<LOGM>::S>
 __device__ __forceinline__ void unpack_product(float *__restrict__ re, float *__restrict__ im)
 {
-    using X = XcShape<LOGM>;
-    constexpr int M = X::M, S = X::S;
-    for (int idx = threadIdx.x; idx < M * M; idx += X::THREADS) {
+    constexpr int M = XcShape<LOGM>::M;
+    for (int idx = threadIdx.x; idx < M * M; idx += NT) {
         const int r = idx >> LOGM, c = idx & (M - 1);
         const int nr = (M - r) & (M - 1), nc = (M - c) & (M - 1);
         if (idx > nr * M + nc) continue;
@@ -224,6 +227,81 @@ __device__ __forceinline__ int block_max_int(int v, int *s_red)
     for (int w = 1; w < THREADS / 64; w++) m = max(m, s_red[w]);
     __syncthreads();
     return m;
+}
+
+// FromCenter candidate of one plane cell (i, j) with linear index cell: row and
+// column 0 are never visited; the running minimum covers the visited cells.
+__device__ __forceinline__ void from_center_visit(float v, int i, int j, int h, int cell,
+                                                  uint64_t &best, int &bestCell, float &mn)
+{
+    if (i == 0 || j == 0) return;
+    if (v < mn) mn = v;
+    const uint64_t key = from_center_key(v, i, j, h);
+    if (key > best) {
+        best = key;
+        bestCell = cell;
+    }
+}
+
+// Block reduction of the per-thread FromCenter candidates, then thread 0
+// writes the peak record (i, j, value), the minimum and the neighbourhood
+// (column through the peak, then the row) of IA ia.  Plane values are read
+// from src with row stride 'stride'.  The keys are unique per cell, so the
+// result does not depend on which thread visited which cell.
+template <int NT, int D>
+__device__ __forceinline__ void peak_epilogue(const PassParams &p, int ia, uint64_t best,
+                                              int bestCell, float mn, const float *src, int stride,
+                                              uint64_t *s_key, int *s_cell, float *s_min)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        const uint64_t ob = shfl_xor_u64(best, o);
+        const int oc = __shfl_xor(bestCell, o, 64);
+        const float om = __shfl_xor(mn, o, 64);
+        if (ob > best) {
+            best = ob;
+            bestCell = oc;
+        }
+        if (om < mn) mn = om;
+    }
+    if ((threadIdx.x & 63) == 0) {
+        s_key[threadIdx.x >> 6] = best;
+        s_cell[threadIdx.x >> 6] = bestCell;
+        s_min[threadIdx.x >> 6] = mn;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < NT / 64; w++) {
+            if (s_key[w] > best) {
+                best = s_key[w];
+                bestCell = s_cell[w];
+            }
+            if (s_min[w] < mn) mn = s_min[w];
+        }
+        // key 0: no visited value >= 0, the reference keeps (0, 0, 0)
+        int pi = 0, pj = 0;
+        float pv = 0.0f;
+        if (best != 0) {
+            pi = bestCell / D;
+            pj = bestCell % D;
+            pv = src[pi * stride + pj];
+        }
+        if (p.peaks) {
+            p.peaks[(size_t)ia * 3 + 0] = (float)pi;
+            p.peaks[(size_t)ia * 3 + 1] = (float)pj;
+            p.peaks[(size_t)ia * 3 + 2] = pv;
+        }
+        if (p.min_floor) p.min_floor[ia] = mn;
+        if (p.nbr) {
+            const int L = 2 * p.nbr_half + 1;
+            float *nb = p.nbr + (size_t)ia * 2 * L;
+            for (int d = -p.nbr_half; d <= p.nbr_half; d++) {
+                const int ri = pi + d, cj = pj + d;
+                nb[d + p.nbr_half] = (ri >= 0 && ri < D) ? src[ri * stride + pj] : 0.0f;
+                nb[L + d + p.nbr_half] = (cj >= 0 && cj < D) ? src[pi * stride + cj] : 0.0f;
+            }
+        }
+    }
 }
 
 // One correlation pass: tile formation, exact FFT correlation, and the
@@ -357,65 +435,230 @@ __global__ __launch_bounds__(XcShape<LOGM>::THREADS) void xcorr_pass_kernel(
         float mn = 3.40282347e+38f;
         for (int idx = threadIdx.x; idx < D * D; idx += NT) {
             const int i = idx / D, j = idx - (idx / D) * D;
-            if (i == 0 || j == 0) continue;
-            const float v = s_re[i * S + j];
-            if (v < mn) mn = v;
-            const uint64_t key = from_center_key(v, i, j, h);
-            if (key > best) {
-                best = key;
-                bestCell = idx;
-            }
+            from_center_visit(s_re[i * S + j], i, j, h, idx, best, bestCell, mn);
         }
-#pragma unroll
-        for (int o = 32; o >= 1; o >>= 1) {
-            const uint64_t ob = shfl_xor_u64(best, o);
-            const int oc = __shfl_xor(bestCell, o, 64);
-            const float om = __shfl_xor(mn, o, 64);
-            if (ob > best) {
-                best = ob;
-                bestCell = oc;
-            }
-            if (om < mn) mn = om;
-        }
-        if ((threadIdx.x & 63) == 0) {
-            s_key[threadIdx.x >> 6] = best;
-            s_cell[threadIdx.x >> 6] = bestCell;
-            s_min[threadIdx.x >> 6] = mn;
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            for (int w = 1; w < NT / 64; w++) {
-                if (s_key[w] > best) {
-                    best = s_key[w];
-                    bestCell = s_cell[w];
-                }
-                if (s_min[w] < mn) mn = s_min[w];
-            }
-            // key 0: no visited value >= 0, the reference keeps (0, 0, 0)
-            int pi = 0, pj = 0;
-            float pv = 0.0f;
-            if (best != 0) {
-                pi = bestCell / D;
-                pj = bestCell % D;
-                pv = s_re[pi * S + pj];
-            }
-            if (p.peaks) {
-                p.peaks[(size_t)ia * 3 + 0] = (float)pi;
-                p.peaks[(size_t)ia * 3 + 1] = (float)pj;
-                p.peaks[(size_t)ia * 3 + 2] = pv;
-            }
-            if (p.min_floor) p.min_floor[ia] = mn;
-            if (p.nbr) {
-                const int L = 2 * p.nbr_half + 1;
-                float *nb = p.nbr + (size_t)ia * 2 * L;
-                for (int d = -p.nbr_half; d <= p.nbr_half; d++) {
-                    const int ri = pi + d, cj = pj + d;
-                    nb[d + p.nbr_half] = (ri >= 0 && ri < D) ? s_re[ri * S + pj] : 0.0f;
-                    nb[L + d + p.nbr_half] = (cj >= 0 && cj < D) ? s_re[pi * S + cj] : 0.0f;
-                }
-            }
-        }
+        peak_epilogue<NT, D>(p, ia, best, bestCell, mn, s_re, S, s_key, s_cell, s_min);
     }
+}
+
+// ---------------------------------------------------------------------------
+// IA 128 (M = 256).  The 256 x 256 complex tile (512 KB) does not fit in LDS,
+// so each work-group keeps it in its own slot of an HBM workspace and runs
+// every 1D pass on batches of BL lines staged through LDS: rows are copied in
+// as they lie, columns are transposed on the way in and out.  Butterflies,
+// twiddle entries, stage order, the unpack and the epilogues are those of the
+// in-LDS kernel above, so the planes are the same values.  Work-groups loop
+// over the IAs (grid = workspace slots).
+namespace big {
+constexpr int LOGM = 8, M = 256, N = 128, D = 255, NT = 512, BL = 32, LS = M + 1;
+constexpr size_t SLOT_FLOATS = 2 * (size_t)M * M;
+constexpr int MAX_SLOTS = 256;  // one resident work-group per CU (8 waves, 68 KB LDS)
+}  // namespace big
+
+int big_slots(int n) { return n < big::MAX_SLOTS ? n : big::MAX_SLOTS; }
+
+size_t big_work_bytes(int n) { return (size_t)big_slots(n) * big::SLOT_FLOATS * sizeof(float); }
+
+template <bool MODE_CPU, bool FROM_IMAGE, bool PEAK>
+__global__ __launch_bounds__(big::NT) void xcorr_big_kernel(PassParams p,
+                                                            const float2 *__restrict__ tw_fwd,
+                                                            const float2 *__restrict__ tw_inv,
+                                                            float *__restrict__ work)
+{
+    using namespace big;
+    __shared__ float s_re[BL * LS];
+    __shared__ float s_im[BL * LS];
+    __shared__ float2 s_twf[M / 2], s_twi[M / 2];
+    __shared__ int s_red[NT / 64];
+    __shared__ uint64_t s_key[NT / 64];
+    __shared__ int s_cell[NT / 64];
+    __shared__ float s_min[NT / 64];
+
+    const int tid = threadIdx.x;
+    for (int k = tid; k < M / 2; k += NT) {
+        s_twf[k] = tw_fwd[k];
+        s_twi[k] = tw_inv[k];
+    }
+    float *__restrict__ wre = work + (size_t)blockIdx.x * SLOT_FLOATS;
+    float *__restrict__ wim = wre + (size_t)M * M;
+
+    for (int ia = blockIdx.x; ia < p.n; ia += gridDim.x) {
+        // ---- pixel sources (as in the in-LDS kernel) and tile maxima ----------
+        const uint8_t *a8 = nullptr, *b8 = nullptr;
+        int y0 = 0, x0 = 0, by0 = 0, bx0 = 0;
+        float fu = 0.0f, fv = 0.0f;
+        bool warp = false;
+        float mA, mB;
+        if (FROM_IMAGE) {
+            const int per_pair = p.per_pair > 0 ? p.per_pair : p.n;
+            const int pair = ia / per_pair, t = ia - pair * per_pair;
+            const size_t img_off = (size_t)pair * p.H * p.W;
+            a8 = p.imgA + img_off;
+            b8 = p.imgB + img_off;
+            y0 = p.top[t];
+            x0 = p.left[t];
+            by0 = y0;
+            bx0 = x0;
+            warp = (p.bmode == B_WARP_MICRO) && p.du != nullptr;
+            if (p.du != nullptr) {
+                const float u = p.du[ia], v = p.dv[ia];
+                if (p.bmode == B_CLIP_SHIFT) {
+                    by0 = fm_round((float)y0 + u);
+                    bx0 = fm_round((float)x0 + v);
+                } else if (warp) {
+                    const float flu = floorf(u), flv = floorf(v);
+                    by0 = (short)(y0 + (int)flu);
+                    bx0 = (short)(x0 + (int)flv);
+                    fu = u - flu;
+                    fv = v - flv;
+                }
+            }
+        } else {
+            a8 = p.tA + (size_t)ia * N * N;
+            b8 = p.tB + (size_t)ia * N * N;
+        }
+        auto pixA = [&](int r, int c) -> int {
+            if (!FROM_IMAGE) return a8[r * N + c];
+            const int y = y0 + r, x = x0 + c;
+            return (y >= 0 && y < p.H && x >= 0 && x < p.W) ? a8[(size_t)y * p.W + x] : 0;
+        };
+        auto pixB = [&](int r, int c) -> int {
+            if (!FROM_IMAGE) return b8[r * N + c];
+            if (warp) return (int)warp_pixel(b8, p.H, p.W, fu, fv, by0 + r, bx0 + c);
+            const int y = by0 + r, x = bx0 + c;
+            return (y >= 0 && y < p.H && x >= 0 && x < p.W) ? b8[(size_t)y * p.W + x] : 0;
+        };
+        if (FROM_IMAGE) {
+            const bool b255 = warp || p.bmax255;
+            int amax = 0, bmax = 0;
+            for (int idx = tid; idx < N * N; idx += NT) {
+                const int r = idx / N, c = idx - (idx / N) * N;
+                amax = max(amax, pixA(r, c));
+                if (!b255) bmax = max(bmax, pixB(r, c));
+            }
+            amax = block_max_int<NT>(amax, s_red);
+            mA = (float)(amax < 16 ? 16 : amax);
+            if (b255) {
+                mB = 255.0f;
+            } else {
+                bmax = block_max_int<NT>(bmax, s_red);
+                mB = (float)(bmax < 16 ? 16 : bmax);
+            }
+            if (MODE_CPU) mB = mA;
+        } else {
+            mA = p.maxA[ia];
+            mB = MODE_CPU ? mA : p.maxB[ia];
+        }
+
+        // ---- forward rows: the N data rows, F'[r][c] = A[N-1-r][N-1-c] -------
+        for (int rb = 0; rb < N; rb += BL) {
+            for (int idx = tid; idx < BL * N; idx += NT) {
+                const int l = idx / N, c = idx - (idx / N) * N, r = rb + l;
+                s_re[l * LS + c] = (float)pixA(N - 1 - r, N - 1 - c) / mA * 16.0f + 1.0f;
+                s_im[l * LS + c] = (float)pixB(r, c) / mB * 16.0f + 1.0f;
+            }
+            __syncthreads();
+            fft_lines<LOGM, true, false, NT, BL>(s_re, s_im, LS, 1, BL, s_twf);
+            for (int idx = tid; idx < BL * M; idx += NT) {
+                const int l = idx / M, c = idx - (idx / M) * M;
+                wre[(size_t)(rb + l) * M + c] = s_re[l * LS + c];
+                wim[(size_t)(rb + l) * M + c] = s_im[l * LS + c];
+            }
+            __syncthreads();
+        }
+        // ---- forward columns: rows >= N are zero padding ---------------------
+        for (int cb = 0; cb < M; cb += BL) {
+            for (int idx = tid; idx < N * BL; idx += NT) {
+                const int r = idx / BL, l = idx - (idx / BL) * BL;
+                s_re[l * LS + r] = wre[(size_t)r * M + cb + l];
+                s_im[l * LS + r] = wim[(size_t)r * M + cb + l];
+            }
+            __syncthreads();
+            fft_lines<LOGM, true, false, NT, BL>(s_re, s_im, LS, 1, BL, s_twf);
+            for (int idx = tid; idx < M * BL; idx += NT) {
+                const int r = idx / BL, l = idx - (idx / BL) * BL;
+                wre[(size_t)r * M + cb + l] = s_re[l * LS + r];
+                wim[(size_t)r * M + cb + l] = s_im[l * LS + r];
+            }
+            __syncthreads();
+        }
+        unpack_product<LOGM, MODE_CPU, NT, M>(wre, wim);
+        // ---- inverse rows ------------------------------------------------------
+        for (int rb = 0; rb < M; rb += BL) {
+            for (int idx = tid; idx < BL * M; idx += NT) {
+                const int l = idx / M, c = idx - (idx / M) * M;
+                s_re[l * LS + c] = wre[(size_t)(rb + l) * M + c];
+                s_im[l * LS + c] = wim[(size_t)(rb + l) * M + c];
+            }
+            __syncthreads();
+            fft_lines<LOGM, false, MODE_CPU, NT, BL>(s_re, s_im, LS, 1, BL, s_twi);
+            for (int idx = tid; idx < BL * M; idx += NT) {
+                const int l = idx / M, c = idx - (idx / M) * M;
+                wre[(size_t)(rb + l) * M + c] = s_re[l * LS + c];
+                wim[(size_t)(rb + l) * M + c] = s_im[l * LS + c];
+            }
+            __syncthreads();
+        }
+        // ---- inverse columns (the D that reach the plane) and epilogues -------
+        // The real parts go to the plane and, for the peak epilogue, back into
+        // the workspace columns just consumed.
+        uint64_t best = 0;
+        int bestCell = 0;
+        float mn = 3.40282347e+38f;
+        float *out = p.planes ? p.planes + (size_t)ia * D * D : nullptr;
+        for (int cb = 0; cb < D; cb += BL) {
+            const int nl = min(BL, D - cb);
+            for (int idx = tid; idx < M * BL; idx += NT) {
+                const int r = idx / BL, l = idx - (idx / BL) * BL;
+                if (l < nl) {
+                    s_re[l * LS + r] = wre[(size_t)r * M + cb + l];
+                    s_im[l * LS + r] = wim[(size_t)r * M + cb + l];
+                }
+            }
+            __syncthreads();
+            fft_lines<LOGM, false, MODE_CPU, NT, BL>(s_re, s_im, LS, 1, nl, s_twi);
+            for (int idx = tid; idx < D * BL; idx += NT) {
+                const int i = idx / BL, l = idx - (idx / BL) * BL, j = cb + l;
+                if (l >= nl) continue;
+                const float v = s_re[l * LS + i];
+                if (out) out[(size_t)i * D + j] = v;
+                if (PEAK) {
+                    wre[(size_t)i * M + j] = v;
+                    from_center_visit(v, i, j, D / 2, i * D + j, best, bestCell, mn);
+                }
+            }
+            __syncthreads();
+        }
+        if (PEAK) peak_epilogue<NT, D>(p, ia, best, bestCell, mn, wre, M, s_key, s_cell, s_min);
+        __syncthreads();  // the slot is rewritten by the next IA
+    }
+}
+
+template <bool MODE_CPU>
+static int launch_pass_big(Context *ctx, const PassParams &p, bool from_image, bool peak)
+{
+    int rc = ensure_bigwork(ctx, p.n);
+    if (rc) return rc;
+    const float2 *twf = ctx->tables.tw[big::LOGM - 5][TW_FWD];
+    const float2 *twi = ctx->tables.tw[big::LOGM - 5][MODE_CPU ? TW_INV_CPU : TW_INV_GPU];
+    float *work = (float *)ctx->bigwork;
+    dim3 grid(big_slots(p.n)), block(big::NT);
+    if (from_image) {
+        if (peak)
+            hipLaunchKernelGGL((xcorr_big_kernel<MODE_CPU, true, true>), grid, block, 0, ctx->stream,
+                               p, twf, twi, work);
+        else
+            hipLaunchKernelGGL((xcorr_big_kernel<MODE_CPU, true, false>), grid, block, 0,
+                               ctx->stream, p, twf, twi, work);
+    } else {
+        if (peak)
+            hipLaunchKernelGGL((xcorr_big_kernel<MODE_CPU, false, true>), grid, block, 0,
+                               ctx->stream, p, twf, twi, work);
+        else
+            hipLaunchKernelGGL((xcorr_big_kernel<MODE_CPU, false, false>), grid, block, 0,
+                               ctx->stream, p, twf, twi, work);
+    }
+    return check_hip(hipGetLastError(), "IA-128 pass launch");
 }
 
 template <int LOGM, bool MODE_CPU>
@@ -463,7 +706,7 @@ int launch_pass(Context *ctx, const PassParams &p)
     // N = 16 (0.293 vs 0.323 ms for 65,025 IAs), the LDS-exchange form for
     // N = 32 (0.381 vs 0.436 ms for 16,129 IAs) and N = 64, where more
     // wavefronts per IA hide latency better
-    // the matrix-core forms cover N = 16 and 32; N = 64 keeps the exact form
+    // the matrix-core forms cover N = 16 and 32; N = 64 and 128 keep the exact form
     if (p.arith != QLP_XCORR_EXACT && (p.N == 16 || p.N == 32))
         return launch_pass_mfma(ctx, p, p.arith, from_image, peak);
     if (ctx->xcorr_form == FORM_LANE && p.N == 16) return launch_pass_lane(ctx, p, from_image, peak);
@@ -471,7 +714,10 @@ int launch_pass(Context *ctx, const PassParams &p)
     case 16: launch_pass_m<5>(ctx, p, from_image, peak); break;
     case 32: launch_pass_m<6>(ctx, p, from_image, peak); break;
     case 64: launch_pass_m<7>(ctx, p, from_image, peak); break;
-    default: return fail(QLP_ERR_UNSUPPORTED, "xcorr: IA size N must be 16, 32 or 64");
+    case 128:
+        return p.mode == QLP_NORM_A_MAX ? launch_pass_big<true>(ctx, p, from_image, peak)
+                                        : launch_pass_big<false>(ctx, p, from_image, peak);
+    default: return fail(QLP_ERR_UNSUPPORTED, "xcorr: IA size N must be 16, 32, 64 or 128");
     }
     return check_hip(hipGetLastError(), "xcorr pass launch");
 }

@@ -292,9 +292,9 @@ def test_job_interface_and_errors(ctx):
     with pytest.raises(RuntimeError):
         job.analyze()
     with pytest.raises(qlpiv.QlpError) as e:
-        ctx.xcorr(np.zeros((2, 128, 128), np.uint8), np.zeros((2, 128, 128), np.uint8),
+        ctx.xcorr(np.zeros((2, 256, 256), np.uint8), np.zeros((2, 256, 256), np.uint8),
                   np.ones(2, np.float32), np.ones(2, np.float32))
-    assert e.value.code == qlpiv._lib.QLP_ERR_UNSUPPORTED
+    assert e.value.code == qlpiv._lib.QLP_ERR_ARG
     with pytest.raises(qlpiv.QlpError) as e:
         ctx.xcorr(np.zeros((2, 24, 24), np.uint8), np.zeros((2, 24, 24), np.uint8),
                   np.ones(2, np.float32), np.ones(2, np.float32))

@@ -100,8 +100,8 @@ def direct_inherit(pu, nI, nJ, round_it=False):
     return out
 
 
-@pytest.mark.parametrize("warp", [1, 0])
-def test_multipass_levels_against_oracle(ctx, pair256, warp):
+@pytest.mark.parametrize("warp,ia_start", [(1, 64), (0, 64), (1, 128)])
+def test_multipass_levels_against_oracle(ctx, pair256, warp, ia_start):
     """Every level of the device loop equals the oracle restatement of that level
     run from the same inherited displacements: integer peaks bit-exact, the
     updated displacements within 1e-4 px (the Guo fit's FP64 log/exp)."""
@@ -110,11 +110,11 @@ def test_multipass_levels_against_oracle(ctx, pair256, warp):
     import qlpiv
 
     a, b, _ = pair256
-    plan = qlpiv.PivPlan(ctx, 256, 256, 1, ia_start=64, ia_end=16, overlap_factor=0.5,
+    plan = qlpiv.PivPlan(ctx, 256, 256, 1, ia_start=ia_start, ia_end=16, overlap_factor=0.5,
                          warp_mode=warp, subpix_kind=qlpiv.QLP_SUBPIX_GUO, subpix_pixels=3,
                          subpix_iterations=20)
     da, db = torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda()
-    say(f"multipass warp={warp}")
+    say(f"multipass warp={warp} ia_start={ia_start}")
     plan.run(da, db)
     prev_u = prev_v = None
     for lv, L in enumerate(plan.levels):

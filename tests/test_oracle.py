@@ -86,6 +86,52 @@ def test_xcorr_restatement_matches_direct_sum(N, mode):
     assert np.unravel_index(np.argmax(got), got.shape) == np.unravel_index(np.argmax(ref), ref.shape)
 
 
+def test_fp64_fft_helper_is_the_direct_sum():
+    """xcorr_fft64 equals the brute-force sum: whole planes for small N, and 40
+    random cells (borders included) of a 128 px plane."""
+    rng = np.random.default_rng(17)
+    for N in (2, 4, 8, 16):
+        F = rng.uniform(1, 17, (N, N)).astype(np.float32)
+        G = rng.uniform(1, 17, (N, N)).astype(np.float32)
+        ref = O.xcorr_direct64(F, G)
+        assert np.abs(O.xcorr_fft64(F, G) - ref).max() <= 1e-9 * ref.max()
+    N = 128
+    Fm = rng.uniform(1, 17, (N, N))
+    G = rng.uniform(1, 17, (N, N))
+    got = O.xcorr_fft64(Fm, G)
+    F = Fm[::-1, ::-1]
+    cells = [(0, 0), (2 * N - 2, 2 * N - 2), (N - 1, N - 1), (0, 2 * N - 2)]
+    cells += [tuple(c) for c in rng.integers(0, 2 * N - 1, (36, 2))]
+    for pi, pj in cells:
+        i, j = pi - (N - 1), pj - (N - 1)
+        n0, n1 = max(0, -i), min(N, N - i)
+        m0, m1 = max(0, -j), min(N, N - j)
+        ref = np.sum(F[n0:n1, m0:m1] * G[n0 + i:n1 + i, m0 + j:m1 + j])
+        assert abs(got[pi, pj] - ref) <= 1e-9 * abs(ref), (pi, pj)
+
+
+@pytest.mark.parametrize("N", [32, 64, 128])
+@pytest.mark.parametrize("mode", [0, 1])
+def test_xcorr_restatement_matches_fp64_fft_at_large_sizes(N, mode):
+    """The IA sizes the direct sum is too slow for, up to 128 (M = 256 twiddles):
+    against an FP64 FFT of the same normalised tiles (itself pinned to the direct
+    sum below).  A wrong twiddle or stage at these sizes moves the plane by far
+    more than float rounding (measured: <= 4.3e-7 of the maximum)."""
+    rng = np.random.default_rng(200 + N + mode)
+    A = rng.integers(0, 256, (N, N)).astype(np.float32)
+    B = rng.integers(0, 256, (N, N)).astype(np.float32)
+    mA, mB = max(16.0, A.max()), max(16.0, B.max())
+    if mode == 0:
+        got = O.xcorr_javacpu(A, B, mA)
+        F, G = O.normalise_pair(A, B, mA, mA)
+    else:
+        got = O.xcorr_javagpu(A, B, mA, mB)
+        F, G = O.normalise_pair(A, B, mA, mB)
+    ref = O.xcorr_fft64(F, G)
+    assert np.max(np.abs(got - ref)) <= 2e-6 * np.max(ref)
+    assert np.unravel_index(np.argmax(got), got.shape) == np.unravel_index(np.argmax(ref), ref.shape)
+
+
 def test_xcorr_modes_differ_only_by_normalisation_and_inverse_twiddles():
     """With A's max used for G too, the GPU-job arithmetic and the CPU path compute
     the same plane up to float rounding of the inverse twiddles."""
