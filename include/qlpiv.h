@@ -371,6 +371,72 @@ int qlp_piv_plan_timing(qlp_piv_plan *plan, int level, float *state_and_pass_ms,
  * the budget check of qlp_piv_plan_create and of the server's admission). */
 int qlp_piv_plan_estimate(int H, int W, int pairs, const qlp_piv_config *cfg, size_t *bytes);
 
+/* ---- batches of frame pairs of mixed sizes in one graph launch --------------
+ *
+ * No counterpart in the reference, which runs its single-level frame job
+ * (jobs/LocalPIVOpenCLGpuJob.call :188-409: clip, correlation, FromCenter peak,
+ * sub-pixel fit) one frame pair at a time.  A batch holds many pairs, each with
+ * its own frame size and IA size (16, 32, 64 or 128), and runs that job for all
+ * of them as ONE launch of a HIP graph built once and replayed for every new set
+ * of frames.  Per IA the results equal those of a one-level plan
+ * (ia_start = ia_end = the pair's IA size) with the same configuration.
+ *
+ * Autotuner: on the first run every IA-size class is timed with each kernel
+ * shape that computes its correlation (QLP_SHAPE_*; the exact shapes give the
+ * same values), the fastest is kept, and the graph is built twice, with the
+ * classes in sequence on one stream and forked onto one stream each; the faster
+ * schedule is kept.  Measured shapes are cached in the context by (IA size, IA
+ * count, arithmetic form, normalisation), so later batches skip the timing. */
+#define QLP_SHAPE_AUTO 0      /* the library's defaults, no measurement */
+#define QLP_SHAPE_LANE 1      /* one lane per transform line (16 px IAs) */
+#define QLP_SHAPE_LDS 2       /* one work-group per IA, default size */
+#define QLP_SHAPE_LDS_HALF 3  /* one work-group per IA, half size */
+#define QLP_SCHEDULE_TUNED (-1)
+#define QLP_SCHEDULE_SERIAL 0
+#define QLP_SCHEDULE_CONCURRENT 1
+
+typedef struct qlp_batch qlp_batch;
+typedef struct {
+    int H, W; /* frame size */
+    int ia;   /* IA size: 16, 32, 64 or 128 */
+} qlp_batch_pair;
+typedef struct {
+    int ia, n_ias;     /* IA size and IAs of the class */
+    int shape;         /* QLP_SHAPE_* in use */
+    int n_candidates;  /* shapes timed (0: fixed or from the context's cache) */
+    int cand_shape[4];
+    float cand_ms[4];  /* best of 3 timed runs of the class with each shape */
+} qlp_batch_class_report;
+typedef struct {
+    int n_classes;
+    qlp_batch_class_report cls[4];
+    int schedule;                    /* QLP_SCHEDULE_SERIAL or _CONCURRENT in use */
+    float serial_ms, concurrent_ms;  /* best of 3 graph launches (0 if not timed) */
+} qlp_batch_report;
+
+/* Lay out the IAs of n_pairs pairs (first-level grid of each frame with the
+ * config's overlap and margins) and allocate their device state.  Used fields
+ * of cfg: overlap_factor, margins, norm_mode, subpix_kind / _pixels /
+ * _iterations, interpolator_start_step, xcorr_form (16 and 32 px classes). */
+int qlp_batch_create(qlp_ctx *ctx, int n_pairs, const qlp_batch_pair *pairs,
+                     const qlp_piv_config *cfg, qlp_batch **out);
+/* Fix the kernel shape of the class of IA size 'ia' (ia 0: every class) and/or
+ * the schedule (QLP_SCHEDULE_TUNED keeps it measured), before the first run. */
+int qlp_batch_fix(qlp_batch *b, int ia, int shape, int schedule);
+/* Correlate every pair: imgsA[k], imgsB[k] are device pointers to pair k's
+ * H x W uint8 frames (any address each run).  Asynchronous on the context's
+ * stream; the first run tunes and builds the graph (synchronising). */
+int qlp_batch_run_dev(qlp_batch *b, const uint8_t *const *imgsA, const uint8_t *const *imgsB);
+/* Grid of pair k and the total IA count of the batch. */
+int qlp_batch_grid(qlp_batch *b, int pair, int *nI, int *nJ, size_t *total_ias);
+/* Device pointers of pair k's results (valid once the run has completed):
+ * u, v (nI*nJ displacements, vertical then horizontal) and sub (nI*nJ x 4:
+ * i', j', value', status of the sub-pixel fit). */
+int qlp_batch_result_dev(qlp_batch *b, int pair, const float **u, const float **v,
+                         const float **sub);
+int qlp_batch_report_get(qlp_batch *b, qlp_batch_report *report);
+int qlp_batch_destroy(qlp_batch *b);
+
 /* ---- serving: a request loop over frame pairs of mixed sizes ----------------
  *
  * Replaces the reference's pool of per-thread PIV workers over a list of frame

@@ -72,10 +72,9 @@ int ensure_iwork(Context *ctx, size_t bytes)
     return grow_buffer(ctx, ctx->iwork, ctx->iwork_bytes, bytes, "hipMalloc inheritance workspace");
 }
 
-int ensure_bigwork(Context *ctx, int n)
+int ensure_bigwork(Context *ctx, size_t bytes)
 {
-    return grow_buffer(ctx, ctx->bigwork, ctx->bigwork_bytes, big_work_bytes(n),
-                       "hipMalloc IA-128 workspace");
+    return grow_buffer(ctx, ctx->bigwork, ctx->bigwork_bytes, bytes, "hipMalloc IA-128 workspace");
 }
 
 static int sync_ctx(Context *ctx) { return check_hip(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize"); }

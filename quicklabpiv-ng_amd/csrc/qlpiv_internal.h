@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <map>
 #include <string>
 #include <vector>
 
@@ -61,6 +62,8 @@ struct Context {
     size_t bigwork_bytes = 0;
     // constant DFT fragments of the MFMA correlation form, [N 16/32][fp16/bf16]
     void *mfma_tab[2][2] = {};
+    // kernel shapes measured by the batch autotuner, by class key
+    std::map<uint64_t, int> tuned_shape;
 };
 
 void set_error(const std::string &msg);
@@ -73,6 +76,17 @@ enum BMode {
     B_CLIP = 0,        // window at (top, left)
     B_CLIP_SHIFT = 1,  // window at (round(top + u), round(left + v)): NoImageWarpingStrategy :79-81
     B_WARP_MICRO = 2,  // bilinear micro warp, SecondImage order
+};
+
+// Kernel shape of an exact correlation pass.  Every shape computes the same
+// values; the batch path's autotuner picks among them.
+enum PassShape {
+    SHAPE_AUTO = 0,      // measured defaults (lane form for N = 16, else in-LDS)
+    SHAPE_LANE = 1,      // one lane per transform line (N = 16 only)
+    SHAPE_LDS = 2,       // one work-group per IA with the default work-group size
+                         // (N = 128: 512 threads, 32-line batches)
+    SHAPE_LDS_HALF = 3,  // half the work-group size, tiles input only
+                         // (N = 128: 256 threads, 16-line batches)
 };
 
 // One correlation pass over n IA pairs.  Tiles come either from u8 tile arrays
@@ -99,6 +113,7 @@ struct PassParams {
     float *nbr = nullptr;
     int nbr_half = 0;
     int arith = QLP_XCORR_EXACT;  // QLP_XCORR_*: arithmetic form of the correlation
+    int shape = 0;                // PassShape of an exact pass
 };
 
 // Launchers (qlpiv_xcorr.hip, qlpiv_peaks.hip, qlpiv_subpixel.hip, qlpiv_warp.hip).
@@ -106,11 +121,15 @@ int launch_xcorr(Context *ctx, int n, int N, const uint8_t *tA, const uint8_t *t
                  const float *maxA, const float *maxB, int mode, float *planes);
 int launch_pass(Context *ctx, const PassParams &p);
 int launch_pass_lane(Context *ctx, const PassParams &p, bool from_image, bool peak);
-// IA 128: the tile lives in an HBM workspace of one 512 KB slot per work-group
-// (at most big_slots(n) slots); ensure_bigwork sizes it for a pass of n IAs.
-int big_slots(int n);
-size_t big_work_bytes(int n);
-int ensure_bigwork(Context *ctx, int n);
+// IA 128: the tile lives in an HBM workspace (ctx->bigwork) of one 512 KB slot
+// per resident work-group.  big_work_need is the workspace of a pass of n IAs
+// with the given kernel (resident work-groups from the occupancy of this
+// device); prepare_big grows the context's workspace to it; big_work_estimate
+// is what n IAs need on a 256-CU device with one resident work-group per CU.
+size_t big_work_need(int n, int mode, bool from_image, bool peak, int shape);
+int prepare_big(Context *ctx, int n, int mode, bool from_image, bool peak, int shape);
+size_t big_work_estimate(int n);
+int ensure_bigwork(Context *ctx, size_t bytes);
 // The throughput form (qlpiv_xcorr_mfma.hip): form is QLP_XCORR_MFMA_*.
 int launch_pass_mfma(Context *ctx, const PassParams &p, int form, bool from_image, bool peak);
 int launch_find_peaks(Context *ctx, int kind, int n, int D, const float *planes, int n_peaks,
@@ -176,5 +195,11 @@ int launch_inherit_bicubic(Context *ctx, int pairs, int pnI, int pnJ, int cn, co
 int launch_inherit_direct(Context *ctx, int pairs, int nI, int nJ, int pI, int pJ, int round_it,
                           const float *pu, const float *pv, float *u, float *v);
 int ensure_iwork(Context *ctx, size_t bytes);
+
+// First-level IA origins of a frame (AdaptiveInterAreaStrategySuperPosition,
+// qlpiv_pipeline.hip); margins = top, bottom, left, right.
+void grid_origins(int H, int W, int tile, float overlap_factor, const int margins[4], int &nI,
+                  int &nJ, std::vector<int> &top, std::vector<int> &left);
+__global__ void fill_kernel(float *__restrict__ x, float value, size_t n);
 
 }  // namespace qlpiv

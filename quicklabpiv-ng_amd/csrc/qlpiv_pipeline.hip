@@ -93,6 +93,17 @@ static void level_grid(int H, int W, int tile, float of, const int m[4], bool le
     }
 }
 
+void grid_origins(int H, int W, int tile, float overlap_factor, const int margins[4], int &nI,
+                  int &nJ, std::vector<int> &top, std::vector<int> &left)
+{
+    Level L;
+    level_grid(H, W, tile, overlap_factor, margins, true, L);
+    nI = L.nI;
+    nJ = L.nJ;
+    top = std::move(L.top_h);
+    left = std::move(L.left_h);
+}
+
 __global__ void fill_kernel(float *__restrict__ x, float value, size_t n)
 {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -312,7 +323,7 @@ int qlp_piv_plan_estimate(int H, int W, int pairs, const qlp_piv_config *cfg, si
     if (cfg->inherit_mode != QLP_INHERIT_DIRECT)
         for (const Level &L : p.levels) need += L.n * 9 * 8 + 64 * 1024;
     // the context's IA-128 workspace (shared by the context's plans)
-    if (p.big_ias) need += big_work_bytes(p.big_ias);
+    if (p.big_ias) need += big_work_estimate(p.big_ias);
     *bytes = need;
     return QLP_OK;
 }
@@ -382,7 +393,8 @@ int qlp_piv_plan_create(qlp_ctx *h, int H, int W, int pairs, const qlp_piv_confi
         rc = check_hip(hipMalloc(&p->wimg, (size_t)pairs * H * W), "hipMalloc plan");
         if (!rc) rc = check_hip(hipMalloc(&p->fwork, p->fwork_bytes), "hipMalloc plan");
     }
-    if (!rc && p->big_ias) rc = ensure_bigwork(ctx, p->big_ias);
+    // the plan's IA-128 passes read the images and find peaks
+    if (!rc && p->big_ias) rc = prepare_big(ctx, p->big_ias, cfg->norm_mode, true, true, SHAPE_AUTO);
     p->bytes = need;
     ctx->hbm_in_use += need;
     if (rc) {

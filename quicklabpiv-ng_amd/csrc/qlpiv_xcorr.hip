@@ -29,7 +29,9 @@
 
 namespace qlpiv {
 
-template <int LOGM>
+// Geometry of the in-LDS kernel for M = 2N = 2^LOGM; NT_ overrides the
+// default work-group size.
+template <int LOGM, int NT_ = 0>
 struct XcShape {
     static constexpr int M = 1 << LOGM;
     static constexpr int N = M / 2;
@@ -38,8 +40,7 @@ struct XcShape {
     static constexpr int E = 1 << LOGE;
     static constexpr int T = M / E;  // threads per line
     static constexpr int S = M + 1;  // LDS row stride in floats (odd: conflict-free columns)
-    static constexpr int THREADS = (LOGM == 5) ? 128 : (LOGM == 6 ? 256 : 512);
-    static constexpr int ROUNDS = (M * T + THREADS - 1) / THREADS;
+    static constexpr int THREADS = NT_ ? NT_ : ((LOGM == 5) ? 128 : (LOGM == 6 ? 256 : 512));
 };
 
 __device__ __forceinline__ int bitrev_n(int x, int bits) { return (int)(__brev((unsigned)x) >> (32 - bits)); }
@@ -310,13 +311,14 @@ __device__ __forceinline__ void peak_epilogue(const PassParams &p, int ia, uint6
 //                B: clip, shifted clip or micro warp by (du, dv)), the tile
 //                maxima are computed here (>= 16 clamp; 255 for a warped B)
 //   PEAK       : FromCenter on the plane while it is still in LDS
-template <int LOGM, bool MODE_CPU, bool FROM_IMAGE, bool PEAK>
-__global__ __launch_bounds__(XcShape<LOGM>::THREADS) void xcorr_pass_kernel(
+//   NT_        : work-group size (0: the default of XcShape)
+template <int LOGM, bool MODE_CPU, bool FROM_IMAGE, bool PEAK, int NT_ = 0>
+__global__ __launch_bounds__((XcShape<LOGM, NT_>::THREADS)) void xcorr_pass_kernel(
     PassParams p, const float2 *__restrict__ tw_fwd, const float2 *__restrict__ tw_inv)
 {
-    using X = XcShape<LOGM>;
+    using X = XcShape<LOGM, NT_>;
     constexpr int M = X::M, N = X::N, S = X::S, D = X::D, NT = X::THREADS;
-    constexpr int PIX = N * N / NT;  // pixels per thread (2, 4 or 8)
+    constexpr int PIX = N * N / NT;  // pixels per thread (2 to 16)
     __shared__ float s_re[M * S];
     __shared__ float s_im[M * S];
     __shared__ float2 s_twf[M / 2], s_twi[M / 2];
@@ -413,12 +415,12 @@ __global__ __launch_bounds__(XcShape<LOGM>::THREADS) void xcorr_pass_kernel(
 
     // ---- transforms ----------------------------------------------------------
     // forward: rows (only the N data rows are non-zero), then all columns
-    fft_lines<LOGM, true, false>(s_re, s_im, S, 1, N, s_twf);
-    fft_lines<LOGM, true, false>(s_re, s_im, 1, S, M, s_twf);
-    unpack_product<LOGM, MODE_CPU>(s_re, s_im);
+    fft_lines<LOGM, true, false, NT>(s_re, s_im, S, 1, N, s_twf);
+    fft_lines<LOGM, true, false, NT>(s_re, s_im, 1, S, M, s_twf);
+    unpack_product<LOGM, MODE_CPU, NT>(s_re, s_im);
     // inverse: all rows, then the D columns that reach the output
-    fft_lines<LOGM, false, MODE_CPU>(s_re, s_im, S, 1, M, s_twi);
-    fft_lines<LOGM, false, MODE_CPU>(s_re, s_im, 1, S, D, s_twi);
+    fft_lines<LOGM, false, MODE_CPU, NT>(s_re, s_im, S, 1, M, s_twi);
+    fft_lines<LOGM, false, MODE_CPU, NT>(s_re, s_im, 1, S, D, s_twi);
 
     // ---- epilogues -------------------------------------------------------------
     if (p.planes) {
@@ -450,20 +452,18 @@ __global__ __launch_bounds__(XcShape<LOGM>::THREADS) void xcorr_pass_kernel(
 // in-LDS kernel above, so the planes are the same values.  Work-groups loop
 // over the IAs (grid = workspace slots).
 namespace big {
-constexpr int LOGM = 8, M = 256, N = 128, D = 255, NT = 512, BL = 32, LS = M + 1;
-constexpr size_t SLOT_FLOATS = 2 * (size_t)M * M;
-constexpr int MAX_SLOTS = 256;  // one resident work-group per CU (8 waves, 68 KB LDS)
+constexpr int LOGM = 8, M = 256, N = 128, D = 255, LS = M + 1;
+constexpr size_t SLOT_BYTES = 2 * (size_t)M * M * sizeof(float);
 }  // namespace big
 
-int big_slots(int n) { return n < big::MAX_SLOTS ? n : big::MAX_SLOTS; }
+size_t big_work_estimate(int n) { return (size_t)(n < 256 ? n : 256) * big::SLOT_BYTES; }
 
-size_t big_work_bytes(int n) { return (size_t)big_slots(n) * big::SLOT_FLOATS * sizeof(float); }
-
-template <bool MODE_CPU, bool FROM_IMAGE, bool PEAK>
-__global__ __launch_bounds__(big::NT) void xcorr_big_kernel(PassParams p,
-                                                            const float2 *__restrict__ tw_fwd,
-                                                            const float2 *__restrict__ tw_inv,
-                                                            float *__restrict__ work)
+// NT threads, batches of BL lines (two shapes: 512 x 32 and 256 x 16).
+template <bool MODE_CPU, bool FROM_IMAGE, bool PEAK, int NT, int BL>
+__global__ __launch_bounds__(NT) void xcorr_big_kernel(PassParams p,
+                                                       const float2 *__restrict__ tw_fwd,
+                                                       const float2 *__restrict__ tw_inv,
+                                                       float *__restrict__ work)
 {
     using namespace big;
     __shared__ float s_re[BL * LS];
@@ -479,7 +479,7 @@ __global__ __launch_bounds__(big::NT) void xcorr_big_kernel(PassParams p,
         s_twf[k] = tw_fwd[k];
         s_twi[k] = tw_inv[k];
     }
-    float *__restrict__ wre = work + (size_t)blockIdx.x * SLOT_FLOATS;
+    float *__restrict__ wre = work + (size_t)blockIdx.x * (SLOT_BYTES / sizeof(float));
     float *__restrict__ wim = wre + (size_t)M * M;
 
     for (int ia = blockIdx.x; ia < p.n; ia += gridDim.x) {
@@ -634,31 +634,69 @@ __global__ __launch_bounds__(big::NT) void xcorr_big_kernel(PassParams p,
     }
 }
 
-template <bool MODE_CPU>
-static int launch_pass_big(Context *ctx, const PassParams &p, bool from_image, bool peak)
+// Work-groups of one kernel that the device holds at once (its workspace slots).
+template <typename K>
+static int resident_groups(K kernel, int threads)
 {
-    int rc = ensure_bigwork(ctx, p.n);
+    int per_cu = 0, dev = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0) != hipSuccess ||
+        hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return 256;
+    return (per_cu < 1 ? 1 : per_cu) * cus;
+}
+
+// need != null: only report the workspace bytes of this pass.
+template <bool MODE_CPU, bool FROM_IMAGE, bool PEAK, int NT, int BL>
+static int launch_big_k(Context *ctx, const PassParams &p, size_t *need)
+{
+    auto k = xcorr_big_kernel<MODE_CPU, FROM_IMAGE, PEAK, NT, BL>;
+    static const int groups = resident_groups(k, NT);
+    const int slots = p.n < groups ? p.n : groups;
+    if (need) {
+        *need = (size_t)slots * big::SLOT_BYTES;
+        return QLP_OK;
+    }
+    int rc = ensure_bigwork(ctx, (size_t)slots * big::SLOT_BYTES);
     if (rc) return rc;
     const float2 *twf = ctx->tables.tw[big::LOGM - 5][TW_FWD];
     const float2 *twi = ctx->tables.tw[big::LOGM - 5][MODE_CPU ? TW_INV_CPU : TW_INV_GPU];
-    float *work = (float *)ctx->bigwork;
-    dim3 grid(big_slots(p.n)), block(big::NT);
-    if (from_image) {
-        if (peak)
-            hipLaunchKernelGGL((xcorr_big_kernel<MODE_CPU, true, true>), grid, block, 0, ctx->stream,
-                               p, twf, twi, work);
-        else
-            hipLaunchKernelGGL((xcorr_big_kernel<MODE_CPU, true, false>), grid, block, 0,
-                               ctx->stream, p, twf, twi, work);
-    } else {
-        if (peak)
-            hipLaunchKernelGGL((xcorr_big_kernel<MODE_CPU, false, true>), grid, block, 0,
-                               ctx->stream, p, twf, twi, work);
-        else
-            hipLaunchKernelGGL((xcorr_big_kernel<MODE_CPU, false, false>), grid, block, 0,
-                               ctx->stream, p, twf, twi, work);
-    }
+    hipLaunchKernelGGL(k, dim3(slots), dim3(NT), 0, ctx->stream, p, twf, twi, (float *)ctx->bigwork);
     return check_hip(hipGetLastError(), "IA-128 pass launch");
+}
+
+template <bool MODE_CPU>
+static int launch_pass_big(Context *ctx, const PassParams &p, bool from_image, bool peak,
+                           size_t *need = nullptr)
+{
+    if (p.shape == SHAPE_LDS_HALF && !from_image)
+        return peak ? launch_big_k<MODE_CPU, false, true, 256, 16>(ctx, p, need)
+                    : launch_big_k<MODE_CPU, false, false, 256, 16>(ctx, p, need);
+    if (from_image)
+        return peak ? launch_big_k<MODE_CPU, true, true, 512, 32>(ctx, p, need)
+                    : launch_big_k<MODE_CPU, true, false, 512, 32>(ctx, p, need);
+    return peak ? launch_big_k<MODE_CPU, false, true, 512, 32>(ctx, p, need)
+                : launch_big_k<MODE_CPU, false, false, 512, 32>(ctx, p, need);
+}
+
+size_t big_work_need(int n, int mode, bool from_image, bool peak, int shape)
+{
+    PassParams p;
+    p.n = n;
+    p.N = 128;
+    p.mode = mode;
+    p.shape = shape;
+    size_t need = 0;
+    if (mode == QLP_NORM_A_MAX)
+        (void)launch_pass_big<true>(nullptr, p, from_image, peak, &need);
+    else
+        (void)launch_pass_big<false>(nullptr, p, from_image, peak, &need);
+    return need;
+}
+
+int prepare_big(Context *ctx, int n, int mode, bool from_image, bool peak, int shape)
+{
+    return ensure_bigwork(ctx, big_work_need(n, mode, from_image, peak, shape));
 }
 
 template <int LOGM, bool MODE_CPU>
@@ -669,6 +707,16 @@ static void launch_pass_mc(Context *ctx, const PassParams &p, bool from_image, b
     const float2 *twf = ctx->tables.tw[li][TW_FWD];
     const float2 *twi = ctx->tables.tw[li][MODE_CPU ? TW_INV_CPU : TW_INV_GPU];
     dim3 grid(p.n), block(X::THREADS);
+    if (p.shape == SHAPE_LDS_HALF && !from_image) {
+        constexpr int H = X::THREADS / 2;
+        if (peak)
+            hipLaunchKernelGGL((xcorr_pass_kernel<LOGM, MODE_CPU, false, true, H>), grid, dim3(H),
+                               0, ctx->stream, p, twf, twi);
+        else
+            hipLaunchKernelGGL((xcorr_pass_kernel<LOGM, MODE_CPU, false, false, H>), grid, dim3(H),
+                               0, ctx->stream, p, twf, twi);
+        return;
+    }
     if (from_image) {
         if (peak)
             hipLaunchKernelGGL((xcorr_pass_kernel<LOGM, MODE_CPU, true, true>), grid, block, 0,
@@ -709,7 +757,8 @@ int launch_pass(Context *ctx, const PassParams &p)
     // the matrix-core forms cover N = 16 and 32; N = 64 and 128 keep the exact form
     if (p.arith != QLP_XCORR_EXACT && (p.N == 16 || p.N == 32))
         return launch_pass_mfma(ctx, p, p.arith, from_image, peak);
-    if (ctx->xcorr_form == FORM_LANE && p.N == 16) return launch_pass_lane(ctx, p, from_image, peak);
+    if (p.N == 16 && (p.shape == SHAPE_LANE || (p.shape == SHAPE_AUTO && ctx->xcorr_form == FORM_LANE)))
+        return launch_pass_lane(ctx, p, from_image, peak);
     switch (p.N) {
     case 16: launch_pass_m<5>(ctx, p, from_image, peak); break;
     case 32: launch_pass_m<6>(ctx, p, from_image, peak); break;

@@ -25,7 +25,7 @@ from ._lib import (QLP_NORM_A_MAX, QLP_NORM_PER_TILE, QLP_PEAK_FROM_CENTER, QLP_
                    QLP_XCORR_MFMA_F16_SINGLE, QlpError, build, check)
 
 __all__ = [
-    "Context", "CrossCorrelationJob", "JobResultEnum", "PivPlan", "QlpError", "Server",
+    "Batch", "Context", "CrossCorrelationJob", "JobResultEnum", "PivPlan", "QlpError", "Server",
     "piv_config", "plan_estimate",
     "VectorValidatorJob", "build",
     "QLP_NORM_PER_TILE", "QLP_NORM_A_MAX", "QLP_PEAK_FROM_CENTER", "QLP_PEAK_MULTI",
@@ -476,6 +476,81 @@ def plan_estimate(H, W, pairs=1, **config):
     b = ctypes.c_size_t()
     check(lib.qlp_piv_plan_estimate(int(H), int(W), int(pairs), ctypes.byref(cfg), ctypes.byref(b)))
     return b.value
+
+
+class Batch:
+    """Frame pairs of mixed frame and IA sizes run as one graph launch
+    (qlp_batch): the single-level frame job (clip, correlation, FromCenter peak,
+    sub-pixel fit) of every pair, with the kernel shapes and the schedule picked
+    by the library's autotuner on the first run.
+
+    pairs: list of (H, W, ia).  config: the PivPlan keys (overlap_factor,
+    margins, norm_mode, subpix_*, interpolator_start_step, xcorr_form)."""
+
+    def __init__(self, ctx, pairs, **config):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.pairs = [tuple(int(x) for x in p) for p in pairs]
+        arr = (_lib.BatchPair * len(self.pairs))(*[_lib.BatchPair(*p) for p in self.pairs])
+        cfg = piv_config(**config)
+        h = ctypes.c_void_p()
+        check(self.lib.qlp_batch_create(ctx.h, len(self.pairs), arr, ctypes.byref(cfg),
+                                        ctypes.byref(h)))
+        self.h = h
+        self.grids = []
+        total = ctypes.c_size_t()
+        for k in range(len(self.pairs)):
+            nI, nJ = ctypes.c_int(), ctypes.c_int()
+            check(self.lib.qlp_batch_grid(self.h, k, ctypes.byref(nI), ctypes.byref(nJ),
+                                          ctypes.byref(total)))
+            self.grids.append((nI.value, nJ.value))
+        self.total_ias = total.value
+
+    def fix(self, ia=0, shape=_lib.QLP_SHAPE_AUTO, schedule=_lib.QLP_SCHEDULE_TUNED):
+        """Fix a class's kernel shape (ia 0: all classes) and/or the schedule."""
+        check(self.lib.qlp_batch_fix(self.h, int(ia), int(shape), int(schedule)))
+
+    def run(self, framesA, framesB):
+        """framesA, framesB: sequences of CUDA uint8 tensors (H x W each), kept
+        alive by the caller until the run has completed."""
+        n = len(self.pairs)
+        pa = (ctypes.c_void_p * n)(*[_ptr(t) for t in framesA])
+        pb = (ctypes.c_void_p * n)(*[_ptr(t) for t in framesB])
+        check(self.lib.qlp_batch_run_dev(self.h, pa, pb))
+
+    def result(self, k):
+        """Host copies (after synchronising) of pair k's u, v (nI x nJ) and
+        sub-pixel records (nI x nJ x 4)."""
+        u, v, s = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+        self.ctx.synchronize()
+        check(self.lib.qlp_batch_result_dev(self.h, int(k), ctypes.byref(u), ctypes.byref(v),
+                                            ctypes.byref(s)))
+        nI, nJ = self.grids[k]
+        n = nI * nJ
+        return (_fetch(u, n, np.float32).reshape(nI, nJ), _fetch(v, n, np.float32).reshape(nI, nJ),
+                _fetch(s, 4 * n, np.float32).reshape(nI, nJ, 4))
+
+    def report(self):
+        r = _lib.BatchReport()
+        check(self.lib.qlp_batch_report_get(self.h, ctypes.byref(r)))
+        cls = []
+        for c in r.cls[:r.n_classes]:
+            cls.append({"ia": c.ia, "n_ias": c.n_ias, "shape": c.shape,
+                        "candidates": {int(c.cand_shape[k]): round(float(c.cand_ms[k]), 4)
+                                       for k in range(c.n_candidates)}})
+        return {"classes": cls, "schedule": r.schedule, "serial_ms": r.serial_ms,
+                "concurrent_ms": r.concurrent_ms}
+
+    def close(self):
+        if self.h:
+            self.lib.qlp_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Server:

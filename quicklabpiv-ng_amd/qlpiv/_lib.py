@@ -86,7 +86,23 @@ SIGNATURES = {
                                ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i), _vp]),
     "qlp_server_release": (_i, [_vp, ctypes.c_uint64]),
     "qlp_server_get_stats": (_i, [_vp, _vp]),
+    "qlp_batch_create": (_i, [_vp, _i, _vp, _vp, ctypes.POINTER(_vp)]),
+    "qlp_batch_fix": (_i, [_vp, _i, _i, _i]),
+    "qlp_batch_run_dev": (_i, [_vp, _vp, _vp]),
+    "qlp_batch_grid": (_i, [_vp, _i, ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_sz)]),
+    "qlp_batch_result_dev": (_i, [_vp, _i, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                  ctypes.POINTER(_vp)]),
+    "qlp_batch_report_get": (_i, [_vp, _vp]),
+    "qlp_batch_destroy": (_i, [_vp]),
 }
+
+QLP_SHAPE_AUTO = 0
+QLP_SHAPE_LANE = 1
+QLP_SHAPE_LDS = 2
+QLP_SHAPE_LDS_HALF = 3
+QLP_SCHEDULE_TUNED = -1
+QLP_SCHEDULE_SERIAL = 0
+QLP_SCHEDULE_CONCURRENT = 1
 
 QLP_WARP_NONE = 0
 QLP_WARP_MICRO = 1
@@ -184,6 +200,26 @@ class ServerStats(ctypes.Structure):
                 ("pending", ctypes.c_uint64), ("evictions", ctypes.c_uint64),
                 ("hbm_in_use", ctypes.c_size_t), ("hbm_peak", ctypes.c_size_t),
                 ("ias_done", ctypes.c_double), ("running", ctypes.c_int)]
+
+
+class BatchPair(ctypes.Structure):
+    """qlp_batch_pair (include/qlpiv.h)."""
+    _fields_ = [("H", ctypes.c_int), ("W", ctypes.c_int), ("ia", ctypes.c_int)]
+
+
+class BatchClassReport(ctypes.Structure):
+    """qlp_batch_class_report (include/qlpiv.h)."""
+    _fields_ = [("ia", ctypes.c_int), ("n_ias", ctypes.c_int), ("shape", ctypes.c_int),
+                ("n_candidates", ctypes.c_int), ("cand_shape", ctypes.c_int * 4),
+                ("cand_ms", ctypes.c_float * 4)]
+
+
+class BatchReport(ctypes.Structure):
+    """qlp_batch_report (include/qlpiv.h)."""
+    _fields_ = [("n_classes", ctypes.c_int), ("cls", BatchClassReport * 4),
+                ("schedule", ctypes.c_int), ("serial_ms", ctypes.c_float),
+                ("concurrent_ms", ctypes.c_float)]
+
 
 _lib = None
 
