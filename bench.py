@@ -326,6 +326,96 @@ def serve_report(n_req, rate, slots):
           flush=True)
 
 
+BATCH_SIZES = [(512, 512), (1024, 1024), (1024, 2048), (2048, 2048)]  # H x W
+
+
+def _batch_frame(spec):
+    k, h, w, _ia, seed, R, U = spec
+    from qlpiv.synth import particle_pair
+
+    a, b, _ = particle_pair(h, w, seed=seed, shift=(0.0, 0.0), core_radius=R, u_max=U)
+    return k, a, b
+
+
+def batch_report(steps, warmup):
+    """The batching configuration: 64 frame pairs, 16 each at 512x512,
+    1024x1024, 1024x2048 and 2048x2048, the IA size of each pair drawn from
+    {16, 32, 64, 128} at 50 % overlap, per-pair Rankine vortex R ~ U(32, 256),
+    U ~ U(2, 10).  One pass per pair (clip, exact correlation with per-tile
+    normalisation, FromCenter, Guo 3 px / 20 iterations) for all 64 pairs in one
+    graph launch (qlp_batch), against the same work as one one-level plan per
+    pair run one after the other."""
+    from multiprocessing import Pool
+
+    import torch
+
+    import qlpiv
+
+    rng = np.random.default_rng(1700)
+    specs = []
+    for (h, w) in BATCH_SIZES:
+        for _ in range(16):
+            specs.append((len(specs), h, w, int(rng.choice([16, 32, 64, 128])), 1700 + len(specs),
+                          float(rng.uniform(32, 256)), float(rng.uniform(2, 10))))
+    print(f"[batch] generating {len(specs)} frame pairs", file=sys.stderr, flush=True)
+    with Pool(8) as pool:
+        got = sorted(pool.map(_batch_frame, specs), key=lambda x: x[0])
+    dA = [torch.from_numpy(a).cuda() for _, a, _ in got]
+    dB = [torch.from_numpy(b).cuda() for _, _, b in got]
+    del got
+    cfg = dict(overlap_factor=0.5, subpix_kind=qlpiv.QLP_SUBPIX_GUO, subpix_pixels=3,
+               subpix_iterations=20)
+    ctx = qlpiv.Context(0)
+    pairs = [(h, w, ia) for (_, h, w, ia, _, _, _) in specs]
+    batch = qlpiv.Batch(ctx, pairs, **cfg)
+    print(f"[batch] {batch.total_ias} IAs; tuning and building the graph", file=sys.stderr, flush=True)
+    batch.run(dA, dB)
+    ctx.synchronize()
+    for _ in range(warmup):
+        batch.run(dA, dB)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        batch.run(dA, dB)
+    ctx.synchronize()
+    ms = (time.perf_counter() - t0) / steps * 1e3
+    report = batch.report()
+    # the same work without the batch: one one-level plan per pair, in sequence
+    print("[batch] one plan per pair, in sequence", file=sys.stderr, flush=True)
+    plans = [qlpiv.PivPlan(ctx, h, w, 1, ia_start=ia, ia_end=ia, warp_mode=0, **cfg)
+             for (h, w, ia) in pairs]
+    for p, a, b in zip(plans, dA, dB):
+        p.run(a, b)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for p, a, b in zip(plans, dA, dB):
+            p.run(a, b)
+    ctx.synchronize()
+    ms_seq = (time.perf_counter() - t0) / steps * 1e3
+    # equal results (spot check of every pair's field)
+    same = all(np.array_equal(batch.result(k)[0], plans[k].result(0)[0][0]) for k in range(len(pairs)))
+    for p in plans:
+        p.close()
+    batch.close()
+    ctx.close()
+    mix = {str(n): sum(1 for p in pairs if p[2] == n) for n in (16, 32, 64, 128)}
+    print(json.dumps({"metric": "batching: correlated IAs/sec of 64 mixed frame pairs in one graph launch",
+                      "value": round(batch.total_ias / ms * 1e3, 1), "unit": "IAs/s",
+                      "ms_per_batch": round(ms, 4), "steps": steps, "warmup": warmup,
+                      "ias": batch.total_ias, "dtype": "f32", "data": "synthetic",
+                      "config": {"workload": "64 pairs: 16 each at 512x512, 1024x1024, 1024x2048, "
+                                             "2048x2048; IA per pair from {16,32,64,128}, 50% "
+                                             "overlap; one pass: clip, exact correlation, "
+                                             "FromCenter, Guo 3px/20it",
+                                 "pairs_per_ia": mix},
+                      "autotuner": report,
+                      "sequential_plans": {"ms_per_batch": round(ms_seq, 4),
+                                           "ias_per_s": round(batch.total_ias / ms_seq * 1e3, 1)},
+                      "speedup_vs_sequential_plans": round(ms_seq / ms, 3),
+                      "results_equal_to_plans": bool(same)}), flush=True)
+
+
 def level_ms(plan, per_level):
     return {str(plan.levels[k]["ia"]): {"pass": round(float(per_level[k][0]), 4),
                                         "peaks_fit": round(float(per_level[k][1]), 4),
@@ -349,6 +439,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=2000)
     ap.add_argument("--serve-rate", type=float, default=400.0)
     ap.add_argument("--serve-slots", type=int, default=8)
+    ap.add_argument("--batch", action="store_true",
+                    help="print the batching report (64 mixed pairs in one graph launch)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (one MI355X per box); "
@@ -359,6 +451,9 @@ def main():
         return
     if args.serve:
         serve_report(args.serve_requests, args.serve_rate, args.serve_slots)
+        return
+    if args.batch:
+        batch_report(args.steps, args.warmup)
         return
 
     import torch

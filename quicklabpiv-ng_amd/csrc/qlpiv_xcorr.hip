@@ -246,13 +246,13 @@ __device__ __forceinline__ void from_center_visit(float v, int i, int j, int h, 
 
 // Block reduction of the per-thread FromCenter candidates, then thread 0
 // writes the peak record (i, j, value), the minimum and the neighbourhood
-// (column through the peak, then the row) of IA ia.  Plane values are read
-// from src with row stride 'stride'.  The keys are unique per cell, so the
+// (column through the peak, then the row) of IA ia.  Plane value (i, j) is
+// read at src[i*rs + j*cs].  The keys are unique per cell, so the
 // result does not depend on which thread visited which cell.
 template <int NT, int D>
 __device__ __forceinline__ void peak_epilogue(const PassParams &p, int ia, uint64_t best,
-                                              int bestCell, float mn, const float *src, int stride,
-                                              uint64_t *s_key, int *s_cell, float *s_min)
+                                              int bestCell, float mn, const float *src, int rs,
+                                              int cs, uint64_t *s_key, int *s_cell, float *s_min)
 {
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) {
@@ -285,7 +285,7 @@ __device__ __forceinline__ void peak_epilogue(const PassParams &p, int ia, uint6
         if (best != 0) {
             pi = bestCell / D;
             pj = bestCell % D;
-            pv = src[pi * stride + pj];
+            pv = src[pi * rs + pj * cs];
         }
         if (p.peaks) {
             p.peaks[(size_t)ia * 3 + 0] = (float)pi;
@@ -298,8 +298,8 @@ __device__ __forceinline__ void peak_epilogue(const PassParams &p, int ia, uint6
             float *nb = p.nbr + (size_t)ia * 2 * L;
             for (int d = -p.nbr_half; d <= p.nbr_half; d++) {
                 const int ri = pi + d, cj = pj + d;
-                nb[d + p.nbr_half] = (ri >= 0 && ri < D) ? src[ri * stride + pj] : 0.0f;
-                nb[L + d + p.nbr_half] = (cj >= 0 && cj < D) ? src[pi * stride + cj] : 0.0f;
+                nb[d + p.nbr_half] = (ri >= 0 && ri < D) ? src[ri * rs + pj * cs] : 0.0f;
+                nb[L + d + p.nbr_half] = (cj >= 0 && cj < D) ? src[pi * rs + cj * cs] : 0.0f;
             }
         }
     }
@@ -439,33 +439,76 @@ __global__ __launch_bounds__((XcShape<LOGM, NT_>::THREADS)) void xcorr_pass_kern
             const int i = idx / D, j = idx - (idx / D) * D;
             from_center_visit(s_re[i * S + j], i, j, h, idx, best, bestCell, mn);
         }
-        peak_epilogue<NT, D>(p, ia, best, bestCell, mn, s_re, S, s_key, s_cell, s_min);
+        peak_epilogue<NT, D>(p, ia, best, bestCell, mn, s_re, S, 1, s_key, s_cell, s_min);
     }
 }
 
 // ---------------------------------------------------------------------------
 // IA 128 (M = 256).  The 256 x 256 complex tile (512 KB) does not fit in LDS,
-// so each work-group keeps it in its own slot of an HBM workspace and runs
-// every 1D pass on batches of BL lines staged through LDS: rows are copied in
-// as they lie, columns are transposed on the way in and out.  Butterflies,
-// twiddle entries, stage order, the unpack and the epilogues are those of the
-// in-LDS kernel above, so the planes are the same values.  Work-groups loop
-// over the IAs (grid = workspace slots).
+// so each work-group keeps it in its own slot of an HBM workspace (complex
+// float2 values, row-major) and runs every 1D pass on batches of BL lines
+// staged through LDS.  Butterflies, twiddle entries, stage order, the unpack
+// and the epilogues are those of the in-LDS kernel above, so the planes are the
+// same values.  The passes, in place in the slot:
+//   forward rows     : the N data rows -> slot rows 0..N-1
+//   forward columns  : batches of columns closed under negation (index k and
+//                      M-k together), so that the two-for-one unpack and the
+//                      product run in LDS; only rows 0..M/2 of the product are
+//                      stored (the others are their conjugate mirror)
+//   inverse rows     : batches of rows closed under negation; a row above M/2
+//                      is read as the conjugate of its mirror row, reversed
+//   inverse columns  : the D columns that reach the plane
+// Work-groups loop over the IAs (grid = workspace slots).
 namespace big {
 constexpr int LOGM = 8, M = 256, N = 128, D = 255, LS = M + 1;
-constexpr size_t SLOT_BYTES = 2 * (size_t)M * M * sizeof(float);
+constexpr size_t SLOT_BYTES = (size_t)M * M * sizeof(float2);
 }  // namespace big
 
 size_t big_work_estimate(int n) { return (size_t)(n < 256 ? n : 256) * big::SLOT_BYTES; }
+
+// Index (row or column) held by line l of batch k of a paired pass with BL
+// lines: batch 0 holds 0 .. BL/2-1, their negatives M-1 .. M-BL/2+1, and M/2;
+// batch k >= 1 holds BL/2*k .. BL/2*k + BL/2-1 and their negatives.
+template <int BL>
+__device__ __forceinline__ int paired_index(int k, int l)
+{
+    constexpr int H = BL / 2, M = big::M;
+    if (k == 0) return l < H ? l : (l < BL - 1 ? M + H - 1 - l : M / 2);
+    return l < H ? H * k + l : M + H - H * k - l;
+}
+
+// Line of the same batch that holds the negative of line l's index.
+template <int BL>
+__device__ __forceinline__ int paired_line(int k, int l)
+{
+    constexpr int H = BL / 2;
+    if (k == 0) return (l == 0 || l == BL - 1) ? l : (l < H ? l + H - 1 : l - (H - 1));
+    return l < H ? l + H : l - H;
+}
+
+// The thread index, hidden from the optimiser: address arithmetic built on it
+// is then recomputed in each batch instead of being hoisted out of the IA loop
+// into registers for every unrolled load.
+__device__ __forceinline__ int opaque_tid()
+{
+    int t = threadIdx.x;
+    asm volatile("" : "+v"(t));
+    return t;
+}
 
 // NT threads, batches of BL lines (two shapes: 512 x 32 and 256 x 16).
 template <bool MODE_CPU, bool FROM_IMAGE, bool PEAK, int NT, int BL>
 __global__ __launch_bounds__(NT) void xcorr_big_kernel(PassParams p,
                                                        const float2 *__restrict__ tw_fwd,
                                                        const float2 *__restrict__ tw_inv,
-                                                       float *__restrict__ work)
+                                                       float2 *__restrict__ work)
 {
     using namespace big;
+    constexpr int NB = M / BL;  // batches of a full pass
+    // loads per thread of one batch; each batch issues all of its loads before
+    // the first LDS write, so that a thread has them in flight together
+    constexpr int IT_HALF = BL * N / NT, IT_FULL = BL * M / NT, IT_MAX = N * N / NT;
+    static_assert(IT_HALF * NT == BL * N && IT_FULL * NT == BL * M && IT_MAX * NT == N * N, "shape");
     __shared__ float s_re[BL * LS];
     __shared__ float s_im[BL * LS];
     __shared__ float2 s_twf[M / 2], s_twi[M / 2];
@@ -479,8 +522,7 @@ __global__ __launch_bounds__(NT) void xcorr_big_kernel(PassParams p,
         s_twf[k] = tw_fwd[k];
         s_twi[k] = tw_inv[k];
     }
-    float *__restrict__ wre = work + (size_t)blockIdx.x * (SLOT_BYTES / sizeof(float));
-    float *__restrict__ wim = wre + (size_t)M * M;
+    float2 *__restrict__ ws = work + (size_t)blockIdx.x * (SLOT_BYTES / sizeof(float2));
 
     for (int ia = blockIdx.x; ia < p.n; ia += gridDim.x) {
         // ---- pixel sources (as in the in-LDS kernel) and tile maxima ----------
@@ -531,7 +573,9 @@ __global__ __launch_bounds__(NT) void xcorr_big_kernel(PassParams p,
         if (FROM_IMAGE) {
             const bool b255 = warp || p.bmax255;
             int amax = 0, bmax = 0;
-            for (int idx = tid; idx < N * N; idx += NT) {
+#pragma unroll 8
+            for (int q = 0; q < IT_MAX; q++) {
+                const int idx = tid + q * NT;
                 const int r = idx / N, c = idx - (idx / N) * N;
                 amax = max(amax, pixA(r, c));
                 if (!b255) bmax = max(bmax, pixB(r, c));
@@ -552,67 +596,129 @@ __global__ __launch_bounds__(NT) void xcorr_big_kernel(PassParams p,
 
         // ---- forward rows: the N data rows, F'[r][c] = A[N-1-r][N-1-c] -------
         for (int rb = 0; rb < N; rb += BL) {
-            for (int idx = tid; idx < BL * N; idx += NT) {
+            const int tid = opaque_tid();
+            int va[IT_HALF], vb[IT_HALF];
+#pragma unroll
+            for (int q = 0; q < IT_HALF; q++) {
+                const int idx = tid + q * NT;
                 const int l = idx / N, c = idx - (idx / N) * N, r = rb + l;
-                s_re[l * LS + c] = (float)pixA(N - 1 - r, N - 1 - c) / mA * 16.0f + 1.0f;
-                s_im[l * LS + c] = (float)pixB(r, c) / mB * 16.0f + 1.0f;
+                va[q] = pixA(N - 1 - r, N - 1 - c);
+                vb[q] = pixB(r, c);
+            }
+#pragma unroll
+            for (int q = 0; q < IT_HALF; q++) {
+                const int idx = tid + q * NT;
+                const int l = idx / N, c = idx - (idx / N) * N;
+                s_re[l * LS + c] = (float)va[q] / mA * 16.0f + 1.0f;
+                s_im[l * LS + c] = (float)vb[q] / mB * 16.0f + 1.0f;
             }
             __syncthreads();
             fft_lines<LOGM, true, false, NT, BL>(s_re, s_im, LS, 1, BL, s_twf);
             for (int idx = tid; idx < BL * M; idx += NT) {
                 const int l = idx / M, c = idx - (idx / M) * M;
-                wre[(size_t)(rb + l) * M + c] = s_re[l * LS + c];
-                wim[(size_t)(rb + l) * M + c] = s_im[l * LS + c];
+                ws[(size_t)(rb + l) * M + c] = make_float2(s_re[l * LS + c], s_im[l * LS + c]);
             }
             __syncthreads();
         }
-        // ---- forward columns: rows >= N are zero padding ---------------------
-        for (int cb = 0; cb < M; cb += BL) {
-            for (int idx = tid; idx < N * BL; idx += NT) {
+        // ---- forward columns (rows >= N are zero padding) + unpack/product ----
+        for (int k = 0; k < NB; k++) {
+            const int tid = opaque_tid();
+            float2 x[IT_HALF];
+#pragma unroll
+            for (int q = 0; q < IT_HALF; q++) {
+                const int idx = tid + q * NT;
                 const int r = idx / BL, l = idx - (idx / BL) * BL;
-                s_re[l * LS + r] = wre[(size_t)r * M + cb + l];
-                s_im[l * LS + r] = wim[(size_t)r * M + cb + l];
+                x[q] = ws[(size_t)r * M + paired_index<BL>(k, l)];
+            }
+#pragma unroll
+            for (int q = 0; q < IT_HALF; q++) {
+                const int idx = tid + q * NT;
+                const int r = idx / BL, l = idx - (idx / BL) * BL;
+                s_re[l * LS + r] = x[q].x;
+                s_im[l * LS + r] = x[q].y;
             }
             __syncthreads();
             fft_lines<LOGM, true, false, NT, BL>(s_re, s_im, LS, 1, BL, s_twf);
-            for (int idx = tid; idx < M * BL; idx += NT) {
+            // unpack (FastRealFFTXCorr.commonRealFFTXCorr :73-112): the owner of
+            // each conjugate pair (line l, row r) / (paired line, -r) writes both
+            for (int idx = tid; idx < BL * M; idx += NT) {
+                const int l = idx / M, r = idx - (idx / M) * M;
+                const int pl = paired_line<BL>(k, l), nr = (M - r) & (M - 1);
+                if (l > pl || (l == pl && r > nr)) continue;
+                const float x = s_re[l * LS + r], y = s_im[l * LS + r];
+                const float Nx = s_re[pl * LS + nr], Ny = s_im[pl * LS + nr];
+                const float fr = 0.5f * (x + Nx);
+                const float fi = 0.5f * (y - Ny);
+                const float gr = 0.5f * (y + Ny);
+                const float gi = -0.5f * (x - Nx);
+                float pre = fr * gr - fi * gi;
+                float pim = fr * gi + fi * gr;
+                if (!MODE_CPU) {
+                    pre = pre / (float)(M * M);
+                    pim = pim / (float)(M * M);
+                }
+                s_re[l * LS + r] = pre;
+                s_im[l * LS + r] = pim;
+                s_re[pl * LS + nr] = pre;
+                s_im[pl * LS + nr] = -pim;
+            }
+            __syncthreads();
+            for (int idx = tid; idx < (M / 2 + 1) * BL; idx += NT) {
                 const int r = idx / BL, l = idx - (idx / BL) * BL;
-                wre[(size_t)r * M + cb + l] = s_re[l * LS + r];
-                wim[(size_t)r * M + cb + l] = s_im[l * LS + r];
+                ws[(size_t)r * M + paired_index<BL>(k, l)] = make_float2(s_re[l * LS + r], s_im[l * LS + r]);
             }
             __syncthreads();
         }
-        unpack_product<LOGM, MODE_CPU, NT, M>(wre, wim);
-        // ---- inverse rows ------------------------------------------------------
-        for (int rb = 0; rb < M; rb += BL) {
-            for (int idx = tid; idx < BL * M; idx += NT) {
+        // ---- inverse rows: rows above M/2 from their conjugate mirror ---------
+        for (int k = 0; k < NB; k++) {
+            const int tid = opaque_tid();
+            float2 x[IT_FULL];
+#pragma unroll
+            for (int q = 0; q < IT_FULL; q++) {
+                const int idx = tid + q * NT;
                 const int l = idx / M, c = idx - (idx / M) * M;
-                s_re[l * LS + c] = wre[(size_t)(rb + l) * M + c];
-                s_im[l * LS + c] = wim[(size_t)(rb + l) * M + c];
+                const int r = paired_index<BL>(k, l);
+                const bool mirror = r > M / 2;
+                x[q] = ws[mirror ? (size_t)(M - r) * M + ((M - c) & (M - 1)) : (size_t)r * M + c];
+                if (mirror) x[q].y = -x[q].y;
+            }
+#pragma unroll
+            for (int q = 0; q < IT_FULL; q++) {
+                const int idx = tid + q * NT;
+                const int l = idx / M, c = idx - (idx / M) * M;
+                s_re[l * LS + c] = x[q].x;
+                s_im[l * LS + c] = x[q].y;
             }
             __syncthreads();
             fft_lines<LOGM, false, MODE_CPU, NT, BL>(s_re, s_im, LS, 1, BL, s_twi);
-            for (int idx = tid; idx < BL * M; idx += NT) {
-                const int l = idx / M, c = idx - (idx / M) * M;
-                wre[(size_t)(rb + l) * M + c] = s_re[l * LS + c];
-                wim[(size_t)(rb + l) * M + c] = s_im[l * LS + c];
+            for (int idx = tid; idx < BL * D; idx += NT) {
+                const int l = idx / D, c = idx - (idx / D) * D;
+                ws[(size_t)paired_index<BL>(k, l) * M + c] = make_float2(s_re[l * LS + c], s_im[l * LS + c]);
             }
             __syncthreads();
         }
         // ---- inverse columns (the D that reach the plane) and epilogues -------
-        // The real parts go to the plane and, for the peak epilogue, back into
-        // the workspace columns just consumed.
         uint64_t best = 0;
         int bestCell = 0;
         float mn = 3.40282347e+38f;
         float *out = p.planes ? p.planes + (size_t)ia * D * D : nullptr;
         for (int cb = 0; cb < D; cb += BL) {
             const int nl = min(BL, D - cb);
-            for (int idx = tid; idx < M * BL; idx += NT) {
+            const int tid = opaque_tid();
+            float2 x[IT_FULL];
+#pragma unroll
+            for (int q = 0; q < IT_FULL; q++) {
+                const int idx = tid + q * NT;
+                const int r = idx / BL, l = idx - (idx / BL) * BL;
+                if (l < nl) x[q] = ws[(size_t)r * M + cb + l];
+            }
+#pragma unroll
+            for (int q = 0; q < IT_FULL; q++) {
+                const int idx = tid + q * NT;
                 const int r = idx / BL, l = idx - (idx / BL) * BL;
                 if (l < nl) {
-                    s_re[l * LS + r] = wre[(size_t)r * M + cb + l];
-                    s_im[l * LS + r] = wim[(size_t)r * M + cb + l];
+                    s_re[l * LS + r] = x[q].x;
+                    s_im[l * LS + r] = x[q].y;
                 }
             }
             __syncthreads();
@@ -623,13 +729,21 @@ __global__ __launch_bounds__(NT) void xcorr_big_kernel(PassParams p,
                 const float v = s_re[l * LS + i];
                 if (out) out[(size_t)i * D + j] = v;
                 if (PEAK) {
-                    wre[(size_t)i * M + j] = v;
+                    // the neighbourhood is read back from the plane, or from the
+                    // slot's column just consumed when no plane is stored
+                    if (!out) ws[(size_t)i * M + j].x = v;
                     from_center_visit(v, i, j, D / 2, i * D + j, best, bestCell, mn);
                 }
             }
             __syncthreads();
         }
-        if (PEAK) peak_epilogue<NT, D>(p, ia, best, bestCell, mn, wre, M, s_key, s_cell, s_min);
+        if (PEAK) {
+            if (out)
+                peak_epilogue<NT, D>(p, ia, best, bestCell, mn, out, D, 1, s_key, s_cell, s_min);
+            else
+                peak_epilogue<NT, D>(p, ia, best, bestCell, mn, (const float *)ws, 2 * M, 2, s_key,
+                                     s_cell, s_min);
+        }
         __syncthreads();  // the slot is rewritten by the next IA
     }
 }
@@ -661,7 +775,7 @@ static int launch_big_k(Context *ctx, const PassParams &p, size_t *need)
     if (rc) return rc;
     const float2 *twf = ctx->tables.tw[big::LOGM - 5][TW_FWD];
     const float2 *twi = ctx->tables.tw[big::LOGM - 5][MODE_CPU ? TW_INV_CPU : TW_INV_GPU];
-    hipLaunchKernelGGL(k, dim3(slots), dim3(NT), 0, ctx->stream, p, twf, twi, (float *)ctx->bigwork);
+    hipLaunchKernelGGL(k, dim3(slots), dim3(NT), 0, ctx->stream, p, twf, twi, (float2 *)ctx->bigwork);
     return check_hip(hipGetLastError(), "IA-128 pass launch");
 }
 
