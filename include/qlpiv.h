@@ -193,9 +193,36 @@ int qlp_warp_image_dev(qlp_ctx *ctx, const uint8_t *img, int H, int W, int pairs
 /* ---- correlation pass straight from device-resident images ---------------- */
 
 #define QLP_WARP_NONE 0  /* NoImageWarpingStrategy: B window at round(top+u), round(left+v) */
-#define QLP_WARP_MICRO 1 /* BiLinearImageMicroAndMiniWarpingStrategy (micro, SecondImage) */
-#define QLP_WARP_MINI 2  /* BiLinearImageMicroAndMiniWarpingStrategy (mini, SecondImage) */
-#define QLP_WARP_FULL 3  /* BiLinearImageWarpingStrategy (whole frame B, SecondImage) */
+#define QLP_WARP_MICRO 1 /* BiLinearImageMicroAndMiniWarpingStrategy (micro) */
+#define QLP_WARP_MINI 2  /* BiLinearImageMicroAndMiniWarpingStrategy (mini) */
+#define QLP_WARP_FULL 3  /* BiLinearImageWarpingStrategy (whole frames) */
+
+/* Warp orders (BiLinearImageWarpingModeEnum; WarpingModeFactoryEnum.create
+ * :37-76 pairs each with Micro, Mini and Full): which frames are warped. */
+#define QLP_WARP_ORDER_SECOND 0 /* B sampled at target + u; A clipped */
+#define QLP_WARP_ORDER_FIRST 1  /* A sampled at target - u; B clipped */
+#define QLP_WARP_ORDER_BOTH 2   /* A at target - u/2, B at target + u/2 (plus the
+                                   reference's rounding offsets for Micro / Mini) */
+
+/* IImageWarpingStrategy.warpAndClipImage of the Micro / Mini strategies at a
+ * level > 0 (images/BiLinearImageMicroAndMiniWarpingStrategy.java:39-186) for
+ * 'pairs' frame pairs on one nI x nJ grid of N px IAs: imgsA, imgsB pairs x H x W;
+ * top, left: nI*nJ origins (device); u, v: pairs x nI*nJ B-tile displacements.
+ * Writes tilesA/tilesB (pairs x nI*nJ x N x N) and their maxima: 255 for a warped
+ * tile (H4), max(tile) >= 16 for a clipped one.  warp_mode QLP_WARP_MICRO or
+ * QLP_WARP_MINI (Mini needs 2 tiles in each direction). */
+int qlp_warp_clip_tiles_dev(qlp_ctx *ctx, int warp_mode, int order, const uint8_t *imgsA,
+                            const uint8_t *imgsB, int H, int W, int pairs, int nI, int nJ, int N,
+                            const int *top, const int *left, const float *u, const float *v,
+                            uint8_t *tilesA, float *maxA, uint8_t *tilesB, float *maxB);
+/* The frames BiLinearImageWarpingStrategy.warpAndClipImage warps at a level > 0
+ * (images/BiLinearImageWarpingStrategy.java:38-183) for one warp order: outA
+ * (FirstImage, BothImages) and outB (SecondImage, BothImages), pairs x H x W
+ * each; unused ones may be NULL.  top, left: HOST arrays of nI*nJ origins; u, v:
+ * device, pairs x nI*nJ. */
+int qlp_warp_images_dev(qlp_ctx *ctx, int order, const uint8_t *imgsA, const uint8_t *imgsB, int H,
+                        int W, int pairs, int nI, int nJ, int N, const int *top, const int *left,
+                        const float *u, const float *v, uint8_t *outA, uint8_t *outB);
 
 /* Clip/warp, correlate and FromCenter-search n IAs of one frame pair in a
  * single kernel (the fused form of qlp_clip_tiles_dev + qlp_warp_micro_tiles_dev
@@ -331,6 +358,7 @@ typedef struct qlp_piv_config {
     int inherit_mode;         /* QLP_INHERIT_* between levels (0: Direct) */
     int xcorr_form;           /* QLP_XCORR_* arithmetic of the 16 and 32 px levels (64 px
                                  levels always use QLP_XCORR_EXACT) */
+    int warp_order;           /* QLP_WARP_ORDER_* of the Micro, Mini and Full warps */
 } qlp_piv_config;
 
 typedef struct qlp_piv_plan qlp_piv_plan;

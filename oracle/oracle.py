@@ -58,6 +58,8 @@ def lib():
         L.or_clip_tile.argtypes = [_u8p, c_int, c_int, c_int, c_int, c_int, _u8p]
         L.or_clip_tile.restype = c_float
         L.or_warp_micro.argtypes = [_u8p, c_int, c_int, c_int, c_int, c_int, c_float, c_float, _u8p]
+        L.or_warp_micro_side.argtypes = [_u8p, c_int, c_int, c_int, c_int, c_int, c_float, c_float,
+                                         c_int, _u8p]
         L.or_spiral_order.argtypes = [c_int, c_int, _i32p]
         L.or_spiral_order.restype = c_int
         L.or_validate.argtypes = [c_int, c_int, _i32p, _i32p, _f32p, _f32p, c_int, _f32p,
@@ -86,9 +88,12 @@ def lib():
         L.or_bilinear_value.restype = ctypes.c_double
         L.or_warp_mini.argtypes = [_u8p, c_int, c_int, c_int, c_int, _i32p, _i32p, c_int, _f32p,
                                    _f32p, c_int, c_int, _u8p]
+        L.or_warp_mini_side.argtypes = [_u8p, c_int, c_int, c_int, c_int, _i32p, _i32p, c_int,
+                                        _f32p, _f32p, c_int, c_int, c_int, _u8p]
         L.or_full_field.argtypes = [c_int, c_int, _i32p, _i32p, c_int, c_int, _f32p, _f32p, c_int,
                                     c_int, _f32p, _f32p]
         L.or_warp_image.argtypes = [_u8p, c_int, c_int, _f32p, _f32p, _u8p]
+        L.or_warp_image_side.argtypes = [_u8p, c_int, c_int, _f32p, _f32p, c_int, _u8p]
         _lib = L
     return _lib
 
@@ -302,11 +307,17 @@ def clip_tile(img, top, left, N):
     return tile, mx
 
 
-def warp_micro(img, top, left, N, du, dv):
+# sides of a tile warp: which frame is sampled in which order
+# (BiLinearImageWarpingModeEnum): B in SecondImage, A in FirstImage, A and B in
+# BothImages
+SIDE_B_SECOND, SIDE_A_FIRST, SIDE_A_BOTH, SIDE_B_BOTH = 0, 1, 2, 3
+
+
+def warp_micro(img, top, left, N, du, dv, side=SIDE_B_SECOND):
     img = np.ascontiguousarray(img, np.uint8)
     tile = np.zeros((N, N), np.uint8)
-    lib().or_warp_micro(img, img.shape[0], img.shape[1], int(top), int(left), N, float(du),
-                        float(dv), tile)
+    lib().or_warp_micro_side(img, img.shape[0], img.shape[1], int(top), int(left), N, float(du),
+                             float(dv), int(side), tile)
     return tile
 
 
@@ -345,17 +356,17 @@ def bilinear_value(ys, xs, v, y, x):
     return lib().or_bilinear_value(ys.size, ys, xs.size, xs, v, float(y), float(x))
 
 
-def warp_mini(img, top, left, N, u, v, ti, tj):
-    """Mini-warped tile B (ti, tj) of a grid (top, left: nI x nJ origins, u, v: the
-    grid's displacements)."""
+def warp_mini(img, top, left, N, u, v, ti, tj, side=SIDE_B_SECOND):
+    """Mini-warped tile (ti, tj) of a grid (top, left: nI x nJ origins, u, v: the
+    grid's displacements) for a side (SIDE_*)."""
     img = np.ascontiguousarray(img, np.uint8)
     top = np.ascontiguousarray(top, np.int32)
     left = np.ascontiguousarray(left, np.int32)
     u = np.ascontiguousarray(u, np.float32)
     v = np.ascontiguousarray(v, np.float32)
     tile = np.zeros((N, N), np.uint8)
-    lib().or_warp_mini(img, img.shape[0], img.shape[1], top.shape[0], top.shape[1], top, left, N,
-                       u, v, int(ti), int(tj), tile)
+    lib().or_warp_mini_side(img, img.shape[0], img.shape[1], top.shape[0], top.shape[1], top, left,
+                            N, u, v, int(ti), int(tj), int(side), tile)
     return tile
 
 
@@ -370,11 +381,11 @@ def full_field(top, left, N, u, v, H, W):
     return fu, fv
 
 
-def warp_image(img, fu, fv):
+def warp_image(img, fu, fv, side=SIDE_B_SECOND):
     img = np.ascontiguousarray(img, np.uint8)
     out = np.zeros_like(img)
-    lib().or_warp_image(img, img.shape[0], img.shape[1], np.ascontiguousarray(fu, np.float32),
-                        np.ascontiguousarray(fv, np.float32), out)
+    lib().or_warp_image_side(img, img.shape[0], img.shape[1], np.ascontiguousarray(fu, np.float32),
+                             np.ascontiguousarray(fv, np.float32), int(side), out)
     return out
 
 

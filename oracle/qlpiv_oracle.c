@@ -1097,11 +1097,17 @@ float or_clip_tile(const uint8_t *img, int H, int W, int top, int left, int N, u
 /* FastMath.round(float) = (int)floor(x + 0.5f) (commons-math3 3.6.1). */
 static int fm_round(float x) { return (int)floorf(x + 0.5f); }
 
-/* BiLinearImageMicroAndMiniWarpingStrategy.warpPixelOnImage :336-407,
- * SecondImage order (source = target + displacement). */
-static float warp_pixel(const uint8_t *img, int H, int W, float u, float v, short i, short j)
+/* BiLinearImageMicroAndMiniWarpingStrategy.warpPixelOnImage :336-407: the
+ * bilinear sample at source = target + displacement (SecondImage order) or
+ * target - displacement (FirstImage order, first != 0), round-half-up. */
+static float warp_pixel_order(const uint8_t *img, int H, int W, float u, float v, short i, short j,
+                              int first)
 {
     float sI = i + u, sJ = j + v;
+    if (first) {
+        sI = i - u;
+        sJ = j - v;
+    }
     short pI = (short)fm_round(sI), pJ = (short)fm_round(sJ);
     if (pI < 0) { sI = 0; pI = 0; }
     else if (pI >= H) { sI = H - 1; pI = (short)(H - 1); }
@@ -1121,22 +1127,91 @@ static float warp_pixel(const uint8_t *img, int H, int W, float u, float v, shor
     return (float)fm_round(f);
 }
 
-/* Micro warp of tile B in SecondImage mode (warpAndClipImage :132-148,
- * createMicroDisplacements :188-201, doBiLinearWarp :292-334): the window is
- * shifted by floor(u), floor(v) and every pixel sampled at the fractional part.
- * Output tile values are integers 0..255; the reference gives the warped tile
- * max = 255 (image.getMaximumValue(), :298). */
+static float warp_pixel(const uint8_t *img, int H, int W, float u, float v, short i, short j)
+{
+    return warp_pixel_order(img, H, W, u, v, i, j, 0);
+}
+
+/* Which frame a tile warp samples, and how (BiLinearImageWarpingModeEnum,
+ * warpAndClipImage :63-167): the B tile in SecondImage order, the A tile in
+ * FirstImage order, and the A and B tiles in BothImages order. */
+enum { OR_SIDE_B_SECOND = 0, OR_SIDE_A_FIRST = 1, OR_SIDE_A_BOTH = 2, OR_SIDE_B_BOTH = 3 };
+
+typedef struct {
+    short top, left;   /* origin of the warped window */
+    float offU, offV;  /* BothImages offsets */
+    int half, first;   /* halve the field and add the offsets; sample at target - field */
+} or_side;
+
+/* warpAndClipImage :65-86 (FirstImage A), :87-123 (BothImages A), :132-148
+ * (SecondImage B), :149-167 (BothImages B), for a tile at (top, left) whose B
+ * tile carries the displacement (dispU, dispV). */
+static or_side side_geometry(int side, int top, int left, float dispU, float dispV)
+{
+    or_side g = {0, 0, 0.0f, 0.0f, 0, 0};
+    switch (side) {
+    case OR_SIDE_A_FIRST:
+        g.top = (short)(top - floor(dispU));
+        g.left = (short)(left - floor(dispV));
+        g.first = 1;
+        break;
+    case OR_SIDE_A_BOTH:
+        g.top = (short)(top - floor(dispU / 2.0f));
+        g.left = (short)(left - floor(dispV / 2.0f));
+        g.offU = (float)(dispU / 2.0f - floor(dispU / 2.0f)) - (float)(dispU - floor(dispU));
+        g.offV = (float)(dispV / 2.0f - floor(dispV / 2.0f)) - (float)(dispV - floor(dispV));
+        g.half = 1;
+        g.first = 1;
+        break;
+    case OR_SIDE_B_BOTH:
+        g.top = (short)(top + floor(dispU / 2.0f));
+        g.left = (short)(left + floor(dispV / 2.0f));
+        g.offU = (float)(dispU / 2.0f - floor(dispU / 2.0f));
+        g.offV = (float)(dispV / 2.0f - floor(dispV / 2.0f));
+        g.half = 1;
+        break;
+    default: /* OR_SIDE_B_SECOND */
+        g.top = (short)(top + floor(dispU));
+        g.left = (short)(left + floor(dispV));
+        break;
+    }
+    return g;
+}
+
+/* doBiLinearWarp :292-334 for one pixel of a side: the field value (u, v) at
+ * the pixel, halved and offset in BothImages order, sampled in the side's order. */
+static uint8_t side_pixel(const uint8_t *img, int H, int W, const or_side *g, float u, float v,
+                          int i, int j)
+{
+    if (g->half) {
+        u = u / 2.0f + g->offU;
+        v = v / 2.0f + g->offV;
+    }
+    return (uint8_t)warp_pixel_order(img, H, W, u, v, (short)(i + g->top), (short)(j + g->left),
+                                     g->first);
+}
+
+/* Micro warp of one tile (warpAndClipImage :39-186, createMicroDisplacements
+ * :188-201, doBiLinearWarp :292-334) for a side (OR_SIDE_*): every pixel takes
+ * the fractional part of the B tile's displacement.  Output tile values are
+ * integers 0..255; the reference gives a warped tile the maximum 255
+ * (image.getMaximumValue(), :298). */
+void or_warp_micro_side(const uint8_t *img, int H, int W, int top, int left, int N, float dispU,
+                        float dispV, int side, uint8_t *tile)
+{
+    const or_side g = side_geometry(side, top, left, dispU, dispV);
+    const float fu = dispU - (float)floor(dispU);
+    const float fv = dispV - (float)floor(dispV);
+    for (int i = 0; i < N; i++)
+        for (int j = 0; j < N; j++) tile[i * N + j] = side_pixel(img, H, W, &g, fu, fv, i, j);
+}
+
+/* SecondImage order: the B tile shifted by floor(u), floor(v), sampled at the
+ * fractional part. */
 void or_warp_micro(const uint8_t *img, int H, int W, int top, int left, int N, float dispU,
                    float dispV, uint8_t *tile)
 {
-    short tTop = (short)(top + floor(dispU));
-    short tLeft = (short)(left + floor(dispV));
-    float fu = dispU - (float)floor(dispU);
-    float fv = dispV - (float)floor(dispV);
-    for (short i = 0; i < N; i++)
-        for (short j = 0; j < N; j++)
-            tile[i * N + j] =
-                (uint8_t)warp_pixel(img, H, W, fu, fv, (short)(i + tTop), (short)(j + tLeft));
+    or_warp_micro_side(img, H, W, top, left, N, dispU, dispV, OR_SIDE_B_SECOND, tile);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -2426,21 +2501,23 @@ static float field_value(const or_field *F, int c, float y, float x)
     return (float)or_bilinear_value(F->ny, F->ys, F->nx, F->xs, F->f[c], (double)y, (double)x);
 }
 
-/* Mini warp of tile B (ti, tj) in SecondImage mode (warpAndClipImage :132-148,
- * createMiniDisplacements :203-290, doBiLinearWarp :292-334): the window is
- * shifted by floor(u), floor(v) of the tile; every pixel is sampled at the
- * field of the tile's 7 x 7 neighbourhood (cut to the grid), whose knots are the
- * tile centres and whose values are the neighbours' displacements minus
- * floor of this tile's.  Grid of nI x nJ tiles of N px, row-major tops/lefts. */
-void or_warp_mini(const uint8_t *img, int H, int W, int nI, int nJ, const int *top,
-                  const int *left, int N, const float *u, const float *v, int ti, int tj,
-                  uint8_t *tile)
+/* Mini warp of tile (ti, tj) for a side (warpAndClipImage :39-186,
+ * createMiniDisplacements :203-290, doBiLinearWarp :292-334): the field of the
+ * tile's 7 x 7 neighbourhood (cut to the grid), whose knots are the tile
+ * centres and whose values are the neighbours' displacements minus floor of
+ * this tile's, is evaluated from the B reference window (top + floor(u),
+ * left + floor(v)) in every order, and applied at the side's window.  Grid of
+ * nI x nJ tiles of N px, row-major tops/lefts. */
+void or_warp_mini_side(const uint8_t *img, int H, int W, int nI, int nJ, const int *top,
+                       const int *left, int N, const float *u, const float *v, int ti, int tj,
+                       int side, uint8_t *tile)
 {
     or_field F;
     const int t = ti * nJ + tj;
     const double fu0 = floor((double)u[t]), fv0 = floor((double)v[t]);
-    const short tTop = (short)(top[t] + floor(u[t]));
-    const short tLeft = (short)(left[t] + floor(v[t]));
+    const short refTop = (short)(top[t] + floor(u[t]));
+    const short refLeft = (short)(left[t] + floor(v[t]));
+    const or_side g = side_geometry(side, top[t], left[t], u[t], v[t]);
     const int i0 = ti - 3 < 0 ? 0 : ti - 3, i1 = ti + 3 > nI - 1 ? nI - 1 : ti + 3;
     const int j0 = tj - 3 < 0 ? 0 : tj - 3, j1 = tj + 3 > nJ - 1 ? nJ - 1 : tj + 3;
     field_alloc(&F, i1 - i0 + 1, j1 - j0 + 1);
@@ -2454,12 +2531,18 @@ void or_warp_mini(const uint8_t *img, int H, int W, int nI, int nJ, const int *t
     field_build(&F);
     for (int i = 0; i < N; i++)
         for (int j = 0; j < N; j++) {
-            const float y = (float)(tTop + i), x = (float)(tLeft + j);
+            const float y = (float)(refTop + i), x = (float)(refLeft + j);
             const float du = field_value(&F, 0, y, x), dv = field_value(&F, 1, y, x);
-            tile[i * N + j] =
-                (uint8_t)warp_pixel(img, H, W, du, dv, (short)(i + tTop), (short)(j + tLeft));
+            tile[i * N + j] = side_pixel(img, H, W, &g, du, dv, i, j);
         }
     field_free(&F);
+}
+
+void or_warp_mini(const uint8_t *img, int H, int W, int nI, int nJ, const int *top,
+                  const int *left, int N, const float *u, const float *v, int ti, int tj,
+                  uint8_t *tile)
+{
+    or_warp_mini_side(img, H, W, nI, nJ, top, left, N, u, v, ti, tj, OR_SIDE_B_SECOND, tile);
 }
 
 /* Full-image warp field (BiLinearImageWarpingStrategy.warpAndClipImage :46-57,
@@ -2485,13 +2568,28 @@ void or_full_field(int nI, int nJ, const int *top, const int *left, int tH, int 
     field_free(&F);
 }
 
-/* BiLinearImageWarpingStrategy.doBiLinearWarp :143-183, SecondImage: every
- * pixel of the image sampled at its own field displacement. */
+/* BiLinearImageWarpingStrategy.doBiLinearWarp :143-183: every pixel of the
+ * image sampled at its own field displacement; side OR_SIDE_B_SECOND (target
+ * + field), OR_SIDE_A_FIRST (target - field), OR_SIDE_A_BOTH / _B_BOTH (half
+ * the field, minus / plus). */
+void or_warp_image_side(const uint8_t *img, int H, int W, const float *fu, const float *fv,
+                        int side, uint8_t *out)
+{
+    const int half = side == OR_SIDE_A_BOTH || side == OR_SIDE_B_BOTH;
+    const int first = side == OR_SIDE_A_FIRST || side == OR_SIDE_A_BOTH;
+    for (int i = 0; i < H; i++)
+        for (int j = 0; j < W; j++) {
+            float u = fu[(size_t)i * W + j], v = fv[(size_t)i * W + j];
+            if (half) {
+                u /= 2.0f;
+                v /= 2.0f;
+            }
+            out[(size_t)i * W + j] = (uint8_t)warp_pixel_order(img, H, W, u, v, (short)i, (short)j, first);
+        }
+}
+
 void or_warp_image(const uint8_t *img, int H, int W, const float *fu, const float *fv,
                    uint8_t *out)
 {
-    for (int i = 0; i < H; i++)
-        for (int j = 0; j < W; j++)
-            out[(size_t)i * W + j] = (uint8_t)warp_pixel(img, H, W, fu[(size_t)i * W + j],
-                                                         fv[(size_t)i * W + j], (short)i, (short)j);
+    or_warp_image_side(img, H, W, fu, fv, OR_SIDE_B_SECOND, out);
 }

@@ -47,6 +47,66 @@ __device__ __forceinline__ float warp_pixel(const uint8_t *__restrict__ img, int
     return (float)fm_round(f);
 }
 
+// Sides of a tile warp (BiLinearImageWarpingModeEnum, warpAndClipImage
+// :63-167): the B tile in SecondImage order, the A tile in FirstImage order,
+// the A and the B tile in BothImages order.
+enum WarpSide { SIDE_B_SECOND = 0, SIDE_A_FIRST = 1, SIDE_A_BOTH = 2, SIDE_B_BOTH = 3 };
+
+struct SideGeom {
+    int top, left;     // origin of the warped window
+    float offu, offv;  // BothImages offsets
+    bool half, first;  // halve the field and add the offsets; sample at target - field
+};
+
+// The window and offsets of a side for a tile at (top, left) whose B tile
+// carries the displacement (u, v): origins shifted by floor(u) (SecondImage),
+// -floor(u) (FirstImage) or +-floor(u/2) (BothImages), the BothImages offsets
+// evaluated in double and cast to float as the reference does (:95-96, :152-153).
+__device__ __forceinline__ SideGeom side_geometry(int side, int top, int left, float u, float v)
+{
+    SideGeom g{0, 0, 0.0f, 0.0f, false, false};
+    const float hu = u / 2.0f, hv = v / 2.0f;
+    if (side == SIDE_A_FIRST) {
+        g.top = (short)(top - (int)floorf(u));
+        g.left = (short)(left - (int)floorf(v));
+        g.first = true;
+    } else if (side == SIDE_A_BOTH) {
+        g.top = (short)(top - (int)floorf(hu));
+        g.left = (short)(left - (int)floorf(hv));
+        g.offu = (float)((double)hu - floor((double)hu)) - (float)((double)u - floor((double)u));
+        g.offv = (float)((double)hv - floor((double)hv)) - (float)((double)v - floor((double)v));
+        g.half = true;
+        g.first = true;
+    } else if (side == SIDE_B_BOTH) {
+        g.top = (short)(top + (int)floorf(hu));
+        g.left = (short)(left + (int)floorf(hv));
+        g.offu = (float)((double)hu - floor((double)hu));
+        g.offv = (float)((double)hv - floor((double)hv));
+        g.half = true;
+    } else {
+        g.top = (short)(top + (int)floorf(u));
+        g.left = (short)(left + (int)floorf(v));
+    }
+    return g;
+}
+
+// Pixel (i, j) of a side's window sampled with the field value (u, v) there
+// (doBiLinearWarp :292-334).  FirstImage samples at target - field, which is
+// target + (-field) exactly in IEEE arithmetic.
+__device__ __forceinline__ float side_sample(const uint8_t *__restrict__ img, int H, int W,
+                                             const SideGeom &g, float u, float v, int i, int j)
+{
+    if (g.half) {
+        u = u / 2.0f + g.offu;
+        v = v / 2.0f + g.offv;
+    }
+    if (g.first) {
+        u = -u;
+        v = -v;
+    }
+    return warp_pixel(img, H, W, u, v, (short)(i + g.top), (short)(j + g.left));
+}
+
 __device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int m)
 {
     const int lo = __shfl_xor((int)(uint32_t)v, m, 64);

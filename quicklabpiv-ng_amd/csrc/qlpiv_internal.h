@@ -142,17 +142,30 @@ int launch_subpixel_nbr(Context *ctx, int kind, int pixels, int iterations, int 
                         const float *peaks_in, float *peaks_out, float *du, float *dv);
 int launch_clip(Context *ctx, const uint8_t *img, int H, int W, int n, int N, const int *top,
                 const int *left, uint8_t *tiles, float *maxv);
+// Tile warps of one side (WarpSide, qlpiv_device.h; SIDE_B_SECOND is the
+// SecondImage B tile).
 int launch_warp_micro(Context *ctx, const uint8_t *img, int H, int W, int n, int N, const int *top,
                       const int *left, const float *u, const float *v, uint8_t *tiles,
-                      float *maxv);
+                      float *maxv, int side = 0);
 // Spline-field warps (qlpiv_splinewarp.hip).
 int launch_warp_mini(Context *ctx, int pairs, const uint8_t *imgs, int H, int W, int nI, int nJ,
                      int N, const int *top, const int *left, const float *u, const float *v,
-                     uint8_t *tiles, float *maxv);
+                     uint8_t *tiles, float *maxv, int side = 0);
+// warpAndClipImage of the Micro / Mini strategies at a level > 0
+// (BiLinearImageMicroAndMiniWarpingStrategy.java:39-186): the A and B tiles
+// and maxima of the nI x nJ IAs of 'pairs' frame pairs in warp order 'order'
+// (QLP_WARP_ORDER_*): a warped tile carries the maximum 255, a clipped one its
+// own (>= 16).  mode: QLP_WARP_MICRO or QLP_WARP_MINI.
+int warp_clip_tiles(Context *ctx, int mode, int order, const uint8_t *imgsA, const uint8_t *imgsB,
+                    int H, int W, int pairs, int nI, int nJ, int N, const int *top, const int *left,
+                    const float *u, const float *v, uint8_t *tA, float *mA, uint8_t *tB, float *mB);
 size_t full_warp_work_bytes(int pairs, int nI, int nJ);
-int launch_full_warp(Context *ctx, int pairs, const uint8_t *imgs, int H, int W, int nI, int nJ,
-                     int tH, int tW, const int *top_h, const int *left_h, const float *u,
-                     const float *v, void *work, uint8_t *out);
+// The field is built once; 'sides' lists the frames to warp with it
+// (outs[k] receives imgs[k] warped for side sides[k]).
+int launch_full_warp(Context *ctx, int pairs, const uint8_t *const *imgs, int H, int W, int nI,
+                     int nJ, int tH, int tW, const int *top_h, const int *left_h, const float *u,
+                     const float *v, void *work, uint8_t *const *outs, const int *sides,
+                     int n_sides);
 
 // Vector validation (qlpiv_validate.hip).  The device arrays of one launch:
 // inputs, outputs and the workspace carved by valid_work_carve.

@@ -19,11 +19,11 @@
 // Everything stays in HBM between levels; the host only enqueues launches.
 #include <math.h>
 
+#include <algorithm>
 #include <vector>
 
+#include "qlpiv_device.h"
 #include "qlpiv_internal.h"
-
-#include <algorithm>
 
 namespace qlpiv {
 
@@ -160,6 +160,22 @@ struct qlp_piv_plan {
     std::vector<hipEvent_t> ev;
 };
 
+// Levels after the first correlate from tile buffers (warp kernels in front of
+// the pass) unless the warp is SecondImage Micro or Full, which the pass reads
+// from the frames itself.
+static bool tiles_route(const qlp_piv_config &c)
+{
+    return c.warp_mode == QLP_WARP_MINI ||
+           ((c.warp_mode == QLP_WARP_MICRO || c.warp_mode == QLP_WARP_FULL) &&
+            c.warp_order != QLP_WARP_ORDER_SECOND);
+}
+
+// Warped whole frames a Full-warp plan keeps: B (SecondImage), A (FirstImage), both.
+static int full_frames(const qlp_piv_config &c)
+{
+    return c.warp_mode != QLP_WARP_FULL ? 0 : (c.warp_order == QLP_WARP_ORDER_BOTH ? 2 : 1);
+}
+
 static void plan_free(qlp_piv_plan *p)
 {
     for (auto &L : p->levels) {
@@ -242,6 +258,8 @@ static int plan_layout(int H, int W, int pairs, const qlp_piv_config *cfg, qlp_p
         return fail(QLP_ERR_ARG, "Total number of steps could not be properly computed since step sizes ratio is not a power of two");
     if (cfg->warp_mode < QLP_WARP_NONE || cfg->warp_mode > QLP_WARP_FULL)
         return fail(QLP_ERR_ARG, "piv_plan_create: unknown warp mode");
+    if (cfg->warp_order < QLP_WARP_ORDER_SECOND || cfg->warp_order > QLP_WARP_ORDER_BOTH)
+        return fail(QLP_ERR_ARG, "piv_plan_create: unknown warp order");
     if (cfg->norm_mode != QLP_NORM_PER_TILE && cfg->norm_mode != QLP_NORM_A_MAX)
         return fail(QLP_ERR_ARG, "piv_plan_create: unknown normalisation mode");
     for (int tile = cfg->ia_start; tile >= cfg->ia_end; tile /= 2)
@@ -297,19 +315,19 @@ static int plan_layout(int H, int W, int pairs, const qlp_piv_config *cfg, qlp_p
             return fail(QLP_ERR_UNSUPPORTED,
                         "piv_plan_create: spline warps need 2 tiles in each direction");
         if (tile == 128) p->big_ias = std::max(p->big_ias, (int)pn);
-        if (l > 0 && cfg->warp_mode == QLP_WARP_MINI)
+        if (l > 0 && tiles_route(*cfg))
             p->tiles_bytes = std::max(p->tiles_bytes, pn * (size_t)tile * tile);
         if (l > 0 && cfg->warp_mode == QLP_WARP_FULL)
             p->fwork_bytes = std::max(p->fwork_bytes, full_warp_work_bytes(pairs, L.nI, L.nJ));
         p->levels.push_back(std::move(L));
     }
-    if (cfg->warp_mode == QLP_WARP_MINI && p->tiles_bytes) {
+    if (tiles_route(*cfg) && p->tiles_bytes) {
         size_t mx = 0;
         for (const Level &L : p->levels) mx = std::max(mx, (size_t)pairs * L.n);
         need += 2 * p->tiles_bytes + 2 * mx * 4;
     }
     if (cfg->warp_mode == QLP_WARP_FULL && p->fwork_bytes)
-        need += (size_t)pairs * H * W + p->fwork_bytes;
+        need += (size_t)full_frames(*cfg) * pairs * H * W + p->fwork_bytes;
     return QLP_OK;
 }
 
@@ -381,7 +399,7 @@ int qlp_piv_plan_create(qlp_ctx *h, int H, int W, int pairs, const qlp_piv_confi
         if (!rc && l > 0 && cfg->inherit_mode != QLP_INHERIT_DIRECT)
             rc = prepare_inherit(cfg->inherit_mode, p->levels[l - 1], L, need);
     }
-    if (!rc && cfg->warp_mode == QLP_WARP_MINI && p->tiles_bytes) {
+    if (!rc && tiles_route(*cfg) && p->tiles_bytes) {
         size_t mx = 0;
         for (const Level &L : p->levels) mx = std::max(mx, (size_t)pairs * L.n);
         rc = check_hip(hipMalloc(&p->tA, p->tiles_bytes), "hipMalloc plan");
@@ -390,7 +408,7 @@ int qlp_piv_plan_create(qlp_ctx *h, int H, int W, int pairs, const qlp_piv_confi
         if (!rc) rc = check_hip(hipMalloc(&p->mB, mx * 4), "hipMalloc plan");
     }
     if (!rc && cfg->warp_mode == QLP_WARP_FULL && p->fwork_bytes) {
-        rc = check_hip(hipMalloc(&p->wimg, (size_t)pairs * H * W), "hipMalloc plan");
+        rc = check_hip(hipMalloc(&p->wimg, (size_t)full_frames(*cfg) * pairs * H * W), "hipMalloc plan");
         if (!rc) rc = check_hip(hipMalloc(&p->fwork, p->fwork_bytes), "hipMalloc plan");
     }
     // the plan's IA-128 passes read the images and find peaks
@@ -483,15 +501,63 @@ int qlp_piv_run_dev(qlp_piv_plan *p, const uint8_t *imgsA, const uint8_t *imgsB)
         q.du = (l == 0) ? nullptr : L.u;
         q.dv = (l == 0) ? nullptr : L.v;
         q.bmode = c.warp_mode == QLP_WARP_MICRO ? B_WARP_MICRO : B_CLIP_SHIFT;
-        if (l > 0 && c.warp_mode == QLP_WARP_MINI) {
-            // clip A and mini-warp B into tiles, then correlate from the tiles
-            const size_t tN = (size_t)L.N * L.N;
-            for (int k = 0; k < p->pairs && !rc; k++)
-                rc = launch_clip(ctx, imgsA + (size_t)k * p->H * p->W, p->H, p->W, L.n, L.N, L.top,
-                                 L.left, p->tA + (size_t)k * L.n * tN, p->mA + (size_t)k * L.n);
-            if (!rc)
-                rc = launch_warp_mini(ctx, p->pairs, imgsB, p->H, p->W, L.nI, L.nJ, L.N, L.top,
-                                      L.left, L.u, L.v, p->tB, p->mB);
+        const int order = c.warp_order;
+        if (l > 0 && c.warp_mode == QLP_WARP_FULL) {
+            // warp the whole frames of the order (A, B or both) with the grid's field
+            const size_t frames = (size_t)p->pairs * p->H * p->W;
+            uint8_t *wA = p->wimg, *wB = p->wimg + (order == QLP_WARP_ORDER_BOTH ? frames : 0);
+            const uint8_t *src[2];
+            uint8_t *dst[2];
+            int sides[2], ns = 0;
+            if (order != QLP_WARP_ORDER_SECOND) {
+                src[ns] = imgsA;
+                dst[ns] = wA;
+                sides[ns++] = order == QLP_WARP_ORDER_FIRST ? SIDE_A_FIRST : SIDE_A_BOTH;
+            }
+            if (order != QLP_WARP_ORDER_FIRST) {
+                src[ns] = imgsB;
+                dst[ns] = wB;
+                sides[ns++] = order == QLP_WARP_ORDER_SECOND ? SIDE_B_SECOND : SIDE_B_BOTH;
+            }
+            rc = launch_full_warp(ctx, p->pairs, src, p->H, p->W, L.nI, L.nJ, L.N, L.N,
+                                  L.top_h.data(), L.left_h.data(), L.u, L.v, p->fwork, dst, sides, ns);
+            if (rc) return rc;
+            q.du = q.dv = nullptr;
+            q.bmode = B_CLIP;
+            if (order == QLP_WARP_ORDER_SECOND) {
+                // the pass cuts B from the warped frames at (top, left), maximum 255
+                q.imgB = wB;
+                q.bmax255 = true;
+            } else {
+                // tiles cut from the warped frames carry the maximum 255; a B tile
+                // clipped from the plain frame (FirstImage) its own
+                const size_t tN = (size_t)L.N * L.N;
+                const uint8_t *cutA = wA, *cutB = order == QLP_WARP_ORDER_BOTH ? wB : imgsB;
+                for (int k = 0; k < p->pairs && !rc; k++) {
+                    rc = launch_clip(ctx, cutA + (size_t)k * p->H * p->W, p->H, p->W, L.n, L.N, L.top,
+                                     L.left, p->tA + (size_t)k * L.n * tN, p->mA + (size_t)k * L.n);
+                    if (!rc)
+                        rc = launch_clip(ctx, cutB + (size_t)k * p->H * p->W, p->H, p->W, L.n, L.N,
+                                         L.top, L.left, p->tB + (size_t)k * L.n * tN,
+                                         p->mB + (size_t)k * L.n);
+                }
+                if (rc) return rc;
+                const unsigned blocks = (unsigned)((pn + 255) / 256);
+                hipLaunchKernelGGL(fill_kernel, dim3(blocks), dim3(256), 0, ctx->stream, p->mA,
+                                   255.0f, (size_t)pn);
+                if (order == QLP_WARP_ORDER_BOTH)
+                    hipLaunchKernelGGL(fill_kernel, dim3(blocks), dim3(256), 0, ctx->stream, p->mB,
+                                       255.0f, (size_t)pn);
+                q.imgA = q.imgB = nullptr;
+                q.tA = p->tA;
+                q.tB = p->tB;
+                q.maxA = p->mA;
+                q.maxB = p->mB;
+            }
+        } else if (l > 0 && tiles_route(c)) {
+            // Micro / Mini warps of the order into tiles, then correlate from the tiles
+            rc = warp_clip_tiles(ctx, c.warp_mode, order, imgsA, imgsB, p->H, p->W, p->pairs, L.nI,
+                                 L.nJ, L.N, L.top, L.left, L.u, L.v, p->tA, p->mA, p->tB, p->mB);
             if (rc) return rc;
             q.imgA = q.imgB = nullptr;
             q.tA = p->tA;
@@ -499,15 +565,6 @@ int qlp_piv_run_dev(qlp_piv_plan *p, const uint8_t *imgsA, const uint8_t *imgsB)
             q.maxA = p->mA;
             q.maxB = p->mB;
             q.du = q.dv = nullptr;
-        } else if (l > 0 && c.warp_mode == QLP_WARP_FULL) {
-            // warp the whole second frames, then cut B at (top, left) with maximum 255
-            rc = launch_full_warp(ctx, p->pairs, imgsB, p->H, p->W, L.nI, L.nJ, L.N, L.N,
-                                  L.top_h.data(), L.left_h.data(), L.u, L.v, p->fwork, p->wimg);
-            if (rc) return rc;
-            q.imgB = p->wimg;
-            q.du = q.dv = nullptr;
-            q.bmode = B_CLIP;
-            q.bmax255 = true;
         }
         q.planes = L.planes;
         q.arith = c.xcorr_form;

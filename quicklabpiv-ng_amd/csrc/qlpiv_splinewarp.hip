@@ -1,6 +1,8 @@
 // Mini and Full-image bilinear warps with spline displacement fields
 // (images/BiLinearImageMicroAndMiniWarpingStrategy.java:203-290 for Mini,
-// images/BiLinearImageWarpingStrategy.java:38-183 for Full, both SecondImage).
+// images/BiLinearImageWarpingStrategy.java:38-183 for Full), in the three warp
+// orders (SecondImage: B; FirstImage: A; BothImages: A and B with half the
+// field, WarpSide in qlpiv_device.h).
 //
 // Both sample frame B at a per-pixel displacement taken from a displacement
 // field over tile centres (iareas/BiCubicSplineInterpolatorWithBiLinearBackup
@@ -12,13 +14,14 @@
 // FP64, as the reference does.
 //   Mini: per tile, the field of its 7 x 7 tile neighbourhood (cut to the grid),
 //         whose values are the neighbours' displacements minus floor of the
-//         tile's own; the window is shifted by floor(u), floor(v).  One
-//         work-group per tile: the neighbourhood's slopes in LDS, then every
-//         pixel.  The warped tile carries the maximum 255.
+//         tile's own, evaluated from the B reference window (shifted by
+//         floor(u), floor(v)) in every order and applied at the side's window.
+//         One work-group per tile: the neighbourhood's slopes in LDS, then
+//         every pixel.  The warped tile carries the maximum 255.
 //   Full: the field of the whole grid at every pixel of the frame: one thread
 //         per knot row or column for the slopes (a global workspace), then one
-//         thread per pixel for the field and the bilinear sample; the pass cuts
-//         the B tiles from the warped frame (maximum 255).
+//         thread per pixel for the field and the bilinear sample of each warped
+//         frame; the pass cuts the tiles from the warped frames (maximum 255).
 #include <math.h>
 
 #include <vector>
@@ -149,7 +152,7 @@ __global__ __launch_bounds__(64) void warp_mini_kernel(int n, int nI, int nJ, in
                                                        const int *__restrict__ top,
                                                        const int *__restrict__ left,
                                                        const float *__restrict__ u,
-                                                       const float *__restrict__ v,
+                                                       const float *__restrict__ v, int side,
                                                        uint8_t *__restrict__ tiles,
                                                        float *__restrict__ maxv)
 {
@@ -162,8 +165,10 @@ __global__ __launch_bounds__(64) void warp_mini_kernel(int n, int nI, int nJ, in
     const uint8_t *__restrict__ img = imgs + (size_t)pair * H * W;
     const float u0 = pu[t], v0 = pv[t];
     const double fu0 = floor((double)u0), fv0 = floor((double)v0);
+    // the field is evaluated from the B reference window in every order
     const int tTop = (short)(top[t] + (int)floorf(u0));
     const int tLeft = (short)(left[t] + (int)floorf(v0));
+    const SideGeom g = side_geometry(side, top[t], left[t], u0, v0);
     const int h = MINI_R / 2;
     const int i0 = ti - h < 0 ? 0 : ti - h, i1 = ti + h > nI - 1 ? nI - 1 : ti + h;
     const int j0 = tj - h < 0 ? 0 : tj - h, j1 = tj + h > nJ - 1 ? nJ - 1 : tj + h;
@@ -198,14 +203,14 @@ __global__ __launch_bounds__(64) void warp_mini_kernel(int n, int nI, int nJ, in
         y = y < ymin ? ymin : (y > ymax ? ymax : y);
         const float du = (float)field_at(s_y, ny, s_x, nx, s_f[0], s_fy[0], s_fx[0], spline, y, x);
         const float dv = (float)field_at(s_y, ny, s_x, nx, s_f[1], s_fy[1], s_fx[1], spline, y, x);
-        out[p] = (uint8_t)warp_pixel(img, H, W, du, dv, (short)(i + tTop), (short)(j + tLeft));
+        out[p] = (uint8_t)side_sample(img, H, W, g, du, dv, i, j);
     }
     if (k == 0 && maxv) maxv[(size_t)pair * n + t] = 255.0f;
 }
 
 int launch_warp_mini(Context *ctx, int pairs, const uint8_t *imgs, int H, int W, int nI, int nJ,
                      int N, const int *top, const int *left, const float *u, const float *v,
-                     uint8_t *tiles, float *maxv)
+                     uint8_t *tiles, float *maxv, int side)
 {
     const int n = nI * nJ;
     if (pairs <= 0 || n <= 0) return QLP_OK;
@@ -214,7 +219,7 @@ int launch_warp_mini(Context *ctx, int pairs, const uint8_t *imgs, int H, int W,
     if (nI < 2 || nJ < 2)
         return fail(QLP_ERR_UNSUPPORTED, "mini warp: the grid needs 2 tiles in each direction");
     hipLaunchKernelGGL(warp_mini_kernel, dim3(pairs * n), dim3(64), 0, ctx->stream, n, nI, nJ, N,
-                       H, W, imgs, top, left, u, v, tiles, maxv);
+                       H, W, imgs, top, left, u, v, side, tiles, maxv);
     return check_hip(hipGetLastError(), "mini warp launch");
 }
 
@@ -251,11 +256,13 @@ __global__ void full_slopes_kernel(int pairs, int nI, int nJ, const double *__re
                        fx + (size_t)pc * n + (size_t)(line - nJ) * nJ, 1, w);
 }
 
+// One thread per pixel: the field there, then the sample of frame imgs for
+// side 'side' (doBiLinearWarp :143-183: BothImages halves the field).
 __global__ void full_warp_kernel(int pairs, int H, int W, int nI, int nJ,
                                  const double *__restrict__ ys, const double *__restrict__ xs,
                                  const double *__restrict__ f, const double *__restrict__ fy,
                                  const double *__restrict__ fx, const uint8_t *__restrict__ imgs,
-                                 uint8_t *__restrict__ out)
+                                 int side, uint8_t *__restrict__ out)
 {
     const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t plane = (size_t)H * W;
@@ -270,8 +277,16 @@ __global__ void full_warp_kernel(int pairs, int H, int W, int nI, int nJ,
     x = x < xmin ? xmin : (x > xmax ? xmax : x);
     y = y < ymin ? ymin : (y > ymax ? ymax : y);
     const size_t b0 = pair * 2 * n, b1 = b0 + n;
-    const float du = (float)field_at(ys, nI, xs, nJ, f + b0, fy + b0, fx + b0, spline, y, x);
-    const float dv = (float)field_at(ys, nI, xs, nJ, f + b1, fy + b1, fx + b1, spline, y, x);
+    float du = (float)field_at(ys, nI, xs, nJ, f + b0, fy + b0, fx + b0, spline, y, x);
+    float dv = (float)field_at(ys, nI, xs, nJ, f + b1, fy + b1, fx + b1, spline, y, x);
+    if (side == SIDE_A_BOTH || side == SIDE_B_BOTH) {
+        du /= 2.0f;
+        dv /= 2.0f;
+    }
+    if (side == SIDE_A_FIRST || side == SIDE_A_BOTH) {  // target - field
+        du = -du;
+        dv = -dv;
+    }
     out[g] = (uint8_t)warp_pixel(imgs + pair * plane, H, W, du, dv, (short)i, (short)j);
 }
 
@@ -282,9 +297,10 @@ size_t full_warp_work_bytes(int pairs, int nI, int nJ)
            (size_t)pairs * 2 * (nI + nJ) * 4 * L * 8;
 }
 
-int launch_full_warp(Context *ctx, int pairs, const uint8_t *imgs, int H, int W, int nI, int nJ,
-                     int tH, int tW, const int *top_h, const int *left_h, const float *u,
-                     const float *v, void *work, uint8_t *out)
+int launch_full_warp(Context *ctx, int pairs, const uint8_t *const *imgs, int H, int W, int nI,
+                     int nJ, int tH, int tW, const int *top_h, const int *left_h, const float *u,
+                     const float *v, void *work, uint8_t *const *outs, const int *sides,
+                     int n_sides)
 {
     if (pairs <= 0) return QLP_OK;
     if (nI < 2 || nJ < 2)
@@ -312,9 +328,41 @@ int launch_full_warp(Context *ctx, int pairs, const uint8_t *imgs, int H, int W,
                            pairs, nI, nJ, ys, xs, f, fy, fx, ws);
     }
     const size_t total = (size_t)pairs * H * W;
-    hipLaunchKernelGGL(full_warp_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
-                       ctx->stream, pairs, H, W, nI, nJ, ys, xs, f, fy, fx, imgs, out);
+    for (int k = 0; k < n_sides; k++)
+        hipLaunchKernelGGL(full_warp_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+                           ctx->stream, pairs, H, W, nI, nJ, ys, xs, f, fy, fx, imgs[k], sides[k],
+                           outs[k]);
     return check_hip(hipGetLastError(), "full warp launch");
+}
+
+int warp_clip_tiles(Context *ctx, int mode, int order, const uint8_t *imgsA, const uint8_t *imgsB,
+                    int H, int W, int pairs, int nI, int nJ, int N, const int *top, const int *left,
+                    const float *u, const float *v, uint8_t *tA, float *mA, uint8_t *tB, float *mB)
+{
+    const int n = nI * nJ;
+    const size_t tN = (size_t)N * N, plane = (size_t)H * W;
+    auto clip = [&](const uint8_t *imgs, uint8_t *t, float *m) {
+        int rc = QLP_OK;
+        for (int k = 0; k < pairs && !rc; k++)
+            rc = launch_clip(ctx, imgs + k * plane, H, W, n, N, top, left, t + k * n * tN, m + k * n);
+        return rc;
+    };
+    auto warp = [&](const uint8_t *imgs, int side, uint8_t *t, float *m) {
+        if (mode == QLP_WARP_MINI)
+            return launch_warp_mini(ctx, pairs, imgs, H, W, nI, nJ, N, top, left, u, v, t, m, side);
+        int rc = QLP_OK;
+        for (int k = 0; k < pairs && !rc; k++)
+            rc = launch_warp_micro(ctx, imgs + k * plane, H, W, n, N, top, left, u + k * n,
+                                   v + k * n, t + k * n * tN, m + k * n, side);
+        return rc;
+    };
+    int rc = order == QLP_WARP_ORDER_SECOND
+                 ? clip(imgsA, tA, mA)
+                 : warp(imgsA, order == QLP_WARP_ORDER_FIRST ? SIDE_A_FIRST : SIDE_A_BOTH, tA, mA);
+    if (rc) return rc;
+    return order == QLP_WARP_ORDER_FIRST
+               ? clip(imgsB, tB, mB)
+               : warp(imgsB, order == QLP_WARP_ORDER_SECOND ? SIDE_B_SECOND : SIDE_B_BOTH, tB, mB);
 }
 
 }  // namespace qlpiv
@@ -322,6 +370,53 @@ int launch_full_warp(Context *ctx, int pairs, const uint8_t *imgs, int H, int W,
 using namespace qlpiv;
 
 extern "C" {
+
+int qlp_warp_clip_tiles_dev(qlp_ctx *h, int warp_mode, int order, const uint8_t *imgsA,
+                            const uint8_t *imgsB, int H, int W, int pairs, int nI, int nJ, int N,
+                            const int *top, const int *left, const float *u, const float *v,
+                            uint8_t *tilesA, float *maxA, uint8_t *tilesB, float *maxB)
+{
+    if (!h || !imgsA || !imgsB || !top || !left || !u || !v || !tilesA || !maxA || !tilesB || !maxB)
+        return fail(QLP_ERR_ARG, "warp_clip_tiles: null argument");
+    if (warp_mode != QLP_WARP_MICRO && warp_mode != QLP_WARP_MINI)
+        return fail(QLP_ERR_ARG, "warp_clip_tiles: warp mode must be Micro or Mini");
+    if (order < QLP_WARP_ORDER_SECOND || order > QLP_WARP_ORDER_BOTH)
+        return fail(QLP_ERR_ARG, "warp_clip_tiles: unknown warp order");
+    if (H < 1 || W < 1 || pairs < 0 || nI < 1 || nJ < 1 || N < 1)
+        return fail(QLP_ERR_ARG, "warp_clip_tiles: bad sizes");
+    return warp_clip_tiles(ctx_of(h), warp_mode, order, imgsA, imgsB, H, W, pairs, nI, nJ, N, top,
+                           left, u, v, tilesA, maxA, tilesB, maxB);
+}
+
+int qlp_warp_images_dev(qlp_ctx *h, int order, const uint8_t *imgsA, const uint8_t *imgsB, int H,
+                        int W, int pairs, int nI, int nJ, int N, const int *top, const int *left,
+                        const float *u, const float *v, uint8_t *outA, uint8_t *outB)
+{
+    if (!h || !top || !left || !u || !v) return fail(QLP_ERR_ARG, "warp_images: null argument");
+    if (order < QLP_WARP_ORDER_SECOND || order > QLP_WARP_ORDER_BOTH)
+        return fail(QLP_ERR_ARG, "warp_images: unknown warp order");
+    if (H < 1 || W < 1 || pairs < 0 || nI < 1 || nJ < 1 || N < 1)
+        return fail(QLP_ERR_ARG, "warp_images: bad sizes");
+    const uint8_t *src[2];
+    uint8_t *dst[2];
+    int sides[2], ns = 0;
+    if (order != QLP_WARP_ORDER_SECOND) {
+        if (!imgsA || !outA) return fail(QLP_ERR_ARG, "warp_images: null first frames");
+        src[ns] = imgsA;
+        dst[ns] = outA;
+        sides[ns++] = order == QLP_WARP_ORDER_FIRST ? SIDE_A_FIRST : SIDE_A_BOTH;
+    }
+    if (order != QLP_WARP_ORDER_FIRST) {
+        if (!imgsB || !outB) return fail(QLP_ERR_ARG, "warp_images: null second frames");
+        src[ns] = imgsB;
+        dst[ns] = outB;
+        sides[ns++] = order == QLP_WARP_ORDER_SECOND ? SIDE_B_SECOND : SIDE_B_BOTH;
+    }
+    Context *ctx = ctx_of(h);
+    if (int rc = ensure_iwork(ctx, full_warp_work_bytes(pairs, nI, nJ))) return rc;
+    return launch_full_warp(ctx, pairs, src, H, W, nI, nJ, N, N, top, left, u, v, ctx->iwork, dst,
+                            sides, ns);
+}
 
 int qlp_warp_mini_tiles_dev(qlp_ctx *h, const uint8_t *img, int H, int W, int pairs, int nI,
                             int nJ, int N, const int *top, const int *left, const float *u,
@@ -344,7 +439,9 @@ int qlp_warp_image_dev(qlp_ctx *h, const uint8_t *img, int H, int W, int pairs, 
         return fail(QLP_ERR_ARG, "warp_image: bad sizes");
     Context *ctx = ctx_of(h);
     if (int rc = ensure_iwork(ctx, full_warp_work_bytes(pairs, nI, nJ))) return rc;
-    return launch_full_warp(ctx, pairs, img, H, W, nI, nJ, N, N, top, left, u, v, ctx->iwork, out);
+    const int side = SIDE_B_SECOND;
+    return launch_full_warp(ctx, pairs, &img, H, W, nI, nJ, N, N, top, left, u, v, ctx->iwork, &out,
+                            &side, 1);
 }
 
 }  // extern "C"

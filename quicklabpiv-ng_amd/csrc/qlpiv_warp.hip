@@ -51,25 +51,26 @@ __global__ __launch_bounds__(256) void clip_kernel(const uint8_t *__restrict__ i
     }
 }
 
+// One work-group per tile: the Micro warp of one side (WarpSide) of tile t,
+// whose B tile carries the displacement (du, dv)[t]; every pixel takes its
+// fractional part (createMicroDisplacements :195-196).
 __global__ __launch_bounds__(256) void warp_micro_kernel(const uint8_t *__restrict__ img, int H,
                                                         int W, int n, int N,
                                                         const int *__restrict__ top,
                                                         const int *__restrict__ left,
                                                         const float *__restrict__ du,
-                                                        const float *__restrict__ dv,
+                                                        const float *__restrict__ dv, int side,
                                                         uint8_t *__restrict__ tiles,
                                                         float *__restrict__ maxv)
 {
     const int t = blockIdx.x;
     if (t >= n) return;
     const float u = du[t], v = dv[t];
-    const float fu = floorf(u), fv = floorf(v);
-    const int tTop = (short)(top[t] + (int)fu);
-    const int tLeft = (short)(left[t] + (int)fv);
-    const float ru = u - fu, rv = v - fv;  // createMicroDisplacements :195-196
+    const SideGeom g = side_geometry(side, top[t], left[t], u, v);
+    const float ru = u - floorf(u), rv = v - floorf(v);
     for (int idx = threadIdx.x; idx < N * N; idx += blockDim.x) {
         const int i = idx / N, j = idx - (idx / N) * N;
-        tiles[(size_t)t * N * N + idx] = (uint8_t)warp_pixel(img, H, W, ru, rv, tTop + i, tLeft + j);
+        tiles[(size_t)t * N * N + idx] = (uint8_t)side_sample(img, H, W, g, ru, rv, i, j);
     }
     if (threadIdx.x == 0 && maxv) maxv[t] = 255.0f;
 }
@@ -84,11 +85,12 @@ int launch_clip(Context *ctx, const uint8_t *img, int H, int W, int n, int N, co
 }
 
 int launch_warp_micro(Context *ctx, const uint8_t *img, int H, int W, int n, int N, const int *top,
-                      const int *left, const float *u, const float *v, uint8_t *tiles, float *maxv)
+                      const int *left, const float *u, const float *v, uint8_t *tiles, float *maxv,
+                      int side)
 {
     if (n == 0) return QLP_OK;
     hipLaunchKernelGGL(warp_micro_kernel, dim3(n), dim3(256), 0, ctx->stream, img, H, W, n, N, top,
-                       left, u, v, tiles, maxv);
+                       left, u, v, side, tiles, maxv);
     return check_hip(hipGetLastError(), "warp kernel launch");
 }
 

@@ -239,6 +239,47 @@ class Context:
                                           _ptr(left), _ptr(u), _ptr(v), _ptr(out)))
         return out
 
+    def warp_clip_tiles(self, imgsA, imgsB, top, left, u, v, N, warp_mode=_lib.QLP_WARP_MICRO,
+                        order=_lib.QLP_WARP_ORDER_SECOND):
+        """warpAndClipImage of the Micro / Mini strategies at a level > 0: the A
+        and B tiles of every IA of an nI x nJ grid for 'pairs' frame pairs in a
+        warp order (imgs: pairs x H x W device uint8; top, left: device nI x nJ
+        int32; u, v: device pairs x nI x nJ) -> (tilesA, maxA, tilesB, maxB)."""
+        import torch
+
+        imgsA = imgsA if imgsA.dim() == 3 else imgsA[None]
+        imgsB = imgsB if imgsB.dim() == 3 else imgsB[None]
+        pairs, H, W = (int(x) for x in imgsA.shape)
+        nI, nJ = int(top.shape[0]), int(top.shape[1])
+        dev = imgsA.device
+        tA = torch.empty((pairs, nI * nJ, N, N), dtype=torch.uint8, device=dev)
+        tB = torch.empty_like(tA)
+        mA = torch.empty((pairs, nI * nJ), dtype=torch.float32, device=dev)
+        mB = torch.empty_like(mA)
+        check(self.lib.qlp_warp_clip_tiles_dev(self.h, int(warp_mode), int(order), _ptr(imgsA),
+                                               _ptr(imgsB), H, W, pairs, nI, nJ, N, _ptr(top),
+                                               _ptr(left), _ptr(u), _ptr(v), _ptr(tA), _ptr(mA),
+                                               _ptr(tB), _ptr(mB)))
+        return tA, mA, tB, mB
+
+    def warp_images(self, imgsA, imgsB, top, left, u, v, N, order=_lib.QLP_WARP_ORDER_SECOND):
+        """The frames the Full-image warp produces in a warp order: (warped A or
+        None, warped B or None); top, left host numpy nI x nJ."""
+        import torch
+
+        imgsA = imgsA if imgsA.dim() == 3 else imgsA[None]
+        imgsB = imgsB if imgsB.dim() == 3 else imgsB[None]
+        pairs, H, W = (int(x) for x in imgsA.shape)
+        top = np.ascontiguousarray(top, np.int32)
+        left = np.ascontiguousarray(left, np.int32)
+        nI, nJ = top.shape
+        outA = None if order == _lib.QLP_WARP_ORDER_SECOND else torch.empty_like(imgsA)
+        outB = None if order == _lib.QLP_WARP_ORDER_FIRST else torch.empty_like(imgsB)
+        check(self.lib.qlp_warp_images_dev(self.h, int(order), _ptr(imgsA), _ptr(imgsB), H, W,
+                                           pairs, nI, nJ, N, _ptr(top), _ptr(left), _ptr(u),
+                                           _ptr(v), _ptr(outA), _ptr(outB)))
+        return outA, outB
+
     # ---- image pre-filter ----------------------------------------------------
     def gaussian_filter(self, imgs, sigma, kernel_px):
         """GaussianFilter2D.applyFilter on device images (n x H x W or H x W,
@@ -466,6 +507,7 @@ def piv_config(**config):
                                   config.get("valid_until_no_more", False))
     cfg.inherit_mode = config.get("inherit_mode", _lib.QLP_INHERIT_DIRECT)
     cfg.xcorr_form = config.get("xcorr_form", QLP_XCORR_EXACT)
+    cfg.warp_order = config.get("warp_order", _lib.QLP_WARP_ORDER_SECOND)
     return cfg
 
 

@@ -86,6 +86,10 @@ SIGNATURES = {
                                ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i), _vp]),
     "qlp_server_release": (_i, [_vp, ctypes.c_uint64]),
     "qlp_server_get_stats": (_i, [_vp, _vp]),
+    "qlp_warp_clip_tiles_dev": (_i, [_vp, _i, _i, _vp, _vp, _i, _i, _i, _i, _i, _i, _vp, _vp, _vp,
+                                     _vp, _vp, _vp, _vp, _vp]),
+    "qlp_warp_images_dev": (_i, [_vp, _i, _vp, _vp, _i, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp,
+                                 _vp, _vp]),
     "qlp_batch_create": (_i, [_vp, _i, _vp, _vp, ctypes.POINTER(_vp)]),
     "qlp_batch_fix": (_i, [_vp, _i, _i, _i]),
     "qlp_batch_run_dev": (_i, [_vp, _vp, _vp]),
@@ -108,6 +112,9 @@ QLP_WARP_NONE = 0
 QLP_WARP_MICRO = 1
 QLP_WARP_MINI = 2
 QLP_WARP_FULL = 3
+QLP_WARP_ORDER_SECOND = 0
+QLP_WARP_ORDER_FIRST = 1
+QLP_WARP_ORDER_BOTH = 2
 
 
 QLP_INHERIT_DIRECT = 0
@@ -173,7 +180,7 @@ class PivConfig(ctypes.Structure):
                 ("keep_planes", ctypes.c_int), ("peak_kind", ctypes.c_int),
                 ("n_peaks", ctypes.c_int), ("peak_kernel_width", ctypes.c_int),
                 ("valid", ValidConfig), ("inherit_mode", ctypes.c_int),
-                ("xcorr_form", ctypes.c_int)]
+                ("xcorr_form", ctypes.c_int), ("warp_order", ctypes.c_int)]
 
 
 class ServerConfig(ctypes.Structure):

@@ -124,7 +124,7 @@ def test_device_frames_and_errors():
     assert ia == 16 and np.array_equal(u, ua) and np.array_equal(v, va)
     # a configuration the library rejects, and a request larger than the budget
     with pytest.raises(qlpiv.QlpError):
-        srv.submit(a, b, ia_start=128, ia_end=16)
+        srv.submit(a, b, ia_start=48, ia_end=16)
     small = qlpiv.Server(0, hbm_budget=1 << 20, slots=1)
     with pytest.raises(qlpiv.QlpError) as e:
         small.submit(a, b, ia_start=64, ia_end=16, keep_planes=2)
