@@ -465,6 +465,37 @@ int qlp_batch_result_dev(qlp_batch *b, int pair, const float **u, const float **
 int qlp_batch_report_get(qlp_batch *b, qlp_batch_report *report);
 int qlp_batch_destroy(qlp_batch *b);
 
+/* ---- result export -----------------------------------------------------------
+ *
+ * The MATLAB Level 5 velocity file of exporter/StructMultiFrameFloatVelocityExporter
+ * .java (openFile :54-113, exportDataToFile(frameNumber, IterationStepTiles)
+ * :131-180, closeFile :115-129), written by host code: "exportDate", the struct
+ * "parameters" and per IA size the struct "velocities<H>x<W>" (iaHeight,
+ * iaWidth, adaptiveStep, margins, u and v as single nI x nJ x total_maps),
+ * uncompressed, in the reference's element layout. */
+typedef struct qlp_exporter qlp_exporter;
+typedef struct {
+    int image_width, image_height;
+    int ia_start_i, ia_end_i, ia_start_j, ia_end_j;
+    float overlap_factor;
+    int swap_uv;               /* PIVMapOptionalConfiguration.isSwapUVOrder */
+    int mark_invalid_nan;      /* invalid tiles exported as NaN */
+    int total_maps;            /* velocity maps (frames) in the file, >= 1 */
+    int multi_volume;          /* the file is one volume of several */
+    const char *next_filename; /* the next volume's file (multi_volume), NULL for the last */
+} qlp_export_params;
+/* Create (truncate) the file and record the parameters; date_text NULL takes the
+ * current time as java.util.Date.toString() prints it. */
+int qlp_export_open(const char *filename, const qlp_export_params *params, const char *date_text,
+                    qlp_exporter **out);
+/* One map (frame index < total_maps) of one level: u, v, invalid (may be NULL)
+ * are HOST nI x nJ row-major arrays; margins = top, left, bottom, right. */
+int qlp_export_frame(qlp_exporter *exporter, int frame, int ia_height, int ia_width,
+                     int adaptive_step, const int margins[4], int nI, int nJ, const float *u,
+                     const float *v, const uint8_t *invalid);
+/* Write the file and release the exporter. */
+int qlp_export_close(qlp_exporter *exporter);
+
 /* ---- serving: a request loop over frame pairs of mixed sizes ----------------
  *
  * Replaces the reference's pool of per-thread PIV workers over a list of frame

@@ -26,6 +26,7 @@ from ._lib import (QLP_NORM_A_MAX, QLP_NORM_PER_TILE, QLP_PEAK_FROM_CENTER, QLP_
 
 __all__ = [
     "Batch", "Context", "CrossCorrelationJob", "JobResultEnum", "PivPlan", "QlpError", "Server",
+    "VelocityExporter",
     "piv_config", "plan_estimate",
     "VectorValidatorJob", "build",
     "QLP_NORM_PER_TILE", "QLP_NORM_A_MAX", "QLP_PEAK_FROM_CENTER", "QLP_PEAK_MULTI",
@@ -518,6 +519,56 @@ def plan_estimate(H, W, pairs=1, **config):
     b = ctypes.c_size_t()
     check(lib.qlp_piv_plan_estimate(int(H), int(W), int(pairs), ctypes.byref(cfg), ctypes.byref(b)))
     return b.value
+
+
+class VelocityExporter:
+    """The MATLAB Level 5 velocity file (qlp_export_*): openFile, per-frame
+    exportDataToFile and closeFile of exporter/StructMultiFrameFloatVelocityExporter.
+
+    image_size = (height, width); ia_start / ia_end = (I, J) IA sizes of the
+    first and last level."""
+
+    def __init__(self, filename, image_size, ia_start, ia_end, overlap_factor, total_maps=1,
+                 swap_uv=False, mark_invalid_nan=False, multi_volume=False, next_filename=None,
+                 date_text=None):
+        self.lib = _lib.load()
+        p = _lib.ExportParams()
+        p.image_height, p.image_width = (int(x) for x in image_size)
+        p.ia_start_i, p.ia_start_j = (int(x) for x in ia_start)
+        p.ia_end_i, p.ia_end_j = (int(x) for x in ia_end)
+        p.overlap_factor = float(overlap_factor)
+        p.swap_uv = int(bool(swap_uv))
+        p.mark_invalid_nan = int(bool(mark_invalid_nan))
+        p.total_maps = int(total_maps)
+        p.multi_volume = int(bool(multi_volume))
+        p.next_filename = next_filename.encode() if next_filename else None
+        h = ctypes.c_void_p()
+        check(self.lib.qlp_export_open(str(filename).encode(), ctypes.byref(p),
+                                       date_text.encode() if date_text else None, ctypes.byref(h)))
+        self.h = h
+
+    def export(self, frame, u, v, ia, adaptive_step, margins=(0, 0, 0, 0), invalid=None):
+        """One level's map of frame 'frame': u, v (nI x nJ host arrays), ia =
+        (height, width), margins = (top, left, bottom, right), invalid flags."""
+        u = np.ascontiguousarray(u, np.float32)
+        v = np.ascontiguousarray(v, np.float32)
+        nI, nJ = u.shape
+        m = (ctypes.c_int * 4)(*[int(x) for x in margins])
+        inv = None if invalid is None else np.ascontiguousarray(invalid, np.uint8)
+        check(self.lib.qlp_export_frame(self.h, int(frame), int(ia[0]), int(ia[1]),
+                                        int(adaptive_step), m, nI, nJ, _ptr(u), _ptr(v),
+                                        _ptr(inv)))
+
+    def close(self):
+        if self.h:
+            h, self.h = self.h, None
+            check(self.lib.qlp_export_close(h))
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
 
 
 class Batch:

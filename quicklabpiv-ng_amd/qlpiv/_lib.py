@@ -90,6 +90,9 @@ SIGNATURES = {
                                      _vp, _vp, _vp, _vp, _vp]),
     "qlp_warp_images_dev": (_i, [_vp, _i, _vp, _vp, _i, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp,
                                  _vp, _vp]),
+    "qlp_export_open": (_i, [ctypes.c_char_p, _vp, ctypes.c_char_p, ctypes.POINTER(_vp)]),
+    "qlp_export_frame": (_i, [_vp, _i, _i, _i, _i, _vp, _i, _i, _vp, _vp, _vp]),
+    "qlp_export_close": (_i, [_vp]),
     "qlp_batch_create": (_i, [_vp, _i, _vp, _vp, ctypes.POINTER(_vp)]),
     "qlp_batch_fix": (_i, [_vp, _i, _i, _i]),
     "qlp_batch_run_dev": (_i, [_vp, _vp, _vp]),
@@ -207,6 +210,16 @@ class ServerStats(ctypes.Structure):
                 ("pending", ctypes.c_uint64), ("evictions", ctypes.c_uint64),
                 ("hbm_in_use", ctypes.c_size_t), ("hbm_peak", ctypes.c_size_t),
                 ("ias_done", ctypes.c_double), ("running", ctypes.c_int)]
+
+
+class ExportParams(ctypes.Structure):
+    """qlp_export_params (include/qlpiv.h)."""
+    _fields_ = [("image_width", ctypes.c_int), ("image_height", ctypes.c_int),
+                ("ia_start_i", ctypes.c_int), ("ia_end_i", ctypes.c_int),
+                ("ia_start_j", ctypes.c_int), ("ia_end_j", ctypes.c_int),
+                ("overlap_factor", ctypes.c_float), ("swap_uv", ctypes.c_int),
+                ("mark_invalid_nan", ctypes.c_int), ("total_maps", ctypes.c_int),
+                ("multi_volume", ctypes.c_int), ("next_filename", ctypes.c_char_p)]
 
 
 class BatchPair(ctypes.Structure):
