@@ -465,6 +465,17 @@ int qlp_batch_result_dev(qlp_batch *b, int pair, const float **u, const float **
 int qlp_batch_report_get(qlp_batch *b, qlp_batch_report *report);
 int qlp_batch_destroy(qlp_batch *b);
 
+/* ---- frame ingest --------------------------------------------------------------
+ *
+ * jobs/ImageReaderJob.compute :80-110 with images/Image.readImageToBuffer
+ * :146-156 for 8-bit frames (the TYPE_BYTE_GRAY raster copied as it is), by
+ * host code: 8-bit grayscale (black is zero) TIFF strips, uncompressed, PackBits,
+ * LZW or Deflate (predictor 1 or 2), and 8-bit grayscale non-interlaced PNG.
+ * Colour, palette, alpha, other depths, tiles and JPEG return
+ * QLP_ERR_UNSUPPORTED (the reference converts them through Java2D). */
+/* Decode 'path' into out (H x W bytes, row-major); out NULL only reports H, W. */
+int qlp_image_read(const char *path, uint8_t *out, size_t out_bytes, int *H, int *W);
+
 /* ---- result export -----------------------------------------------------------
  *
  * The MATLAB Level 5 velocity file of exporter/StructMultiFrameFloatVelocityExporter

@@ -26,7 +26,7 @@ from ._lib import (QLP_NORM_A_MAX, QLP_NORM_PER_TILE, QLP_PEAK_FROM_CENTER, QLP_
 
 __all__ = [
     "Batch", "Context", "CrossCorrelationJob", "JobResultEnum", "PivPlan", "QlpError", "Server",
-    "VelocityExporter",
+    "VelocityExporter", "read_image",
     "piv_config", "plan_estimate",
     "VectorValidatorJob", "build",
     "QLP_NORM_PER_TILE", "QLP_NORM_A_MAX", "QLP_PEAK_FROM_CENTER", "QLP_PEAK_MULTI",
@@ -519,6 +519,18 @@ def plan_estimate(H, W, pairs=1, **config):
     b = ctypes.c_size_t()
     check(lib.qlp_piv_plan_estimate(int(H), int(W), int(pairs), ctypes.byref(cfg), ctypes.byref(b)))
     return b.value
+
+
+def read_image(path):
+    """An 8-bit grayscale TIFF or PNG frame as an H x W uint8 array
+    (qlp_image_read: ImageReaderJob for 8-bit images)."""
+    lib = _lib.load()
+    H, W = ctypes.c_int(), ctypes.c_int()
+    p = str(path).encode()
+    check(lib.qlp_image_read(p, None, 0, ctypes.byref(H), ctypes.byref(W)))
+    out = np.empty((H.value, W.value), np.uint8)
+    check(lib.qlp_image_read(p, _ptr(out), out.nbytes, ctypes.byref(H), ctypes.byref(W)))
+    return out
 
 
 class VelocityExporter:

@@ -90,6 +90,7 @@ SIGNATURES = {
                                      _vp, _vp, _vp, _vp, _vp]),
     "qlp_warp_images_dev": (_i, [_vp, _i, _vp, _vp, _i, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp,
                                  _vp, _vp]),
+    "qlp_image_read": (_i, [ctypes.c_char_p, _vp, _sz, ctypes.POINTER(_i), ctypes.POINTER(_i)]),
     "qlp_export_open": (_i, [ctypes.c_char_p, _vp, ctypes.c_char_p, ctypes.POINTER(_vp)]),
     "qlp_export_frame": (_i, [_vp, _i, _i, _i, _i, _vp, _i, _i, _vp, _vp, _vp]),
     "qlp_export_close": (_i, [_vp]),

@@ -9,6 +9,8 @@ Fixtures (data only -- inputs and expected outputs):
 
 * rankine_pair.npz        -- resources/fftBenchmark/rankine_vortex01_{0,1}.tif decoded
                              to uint8 (512 x 512, 8-bit PackBits TIFFs).
+* rankine_vortex01_{0,1}.tif -- the two TIFF files themselves (input of the image
+                             reader's test).
 * gpu_fft_golden.npz      -- resources/fftBenchmark/gpuFFTBenchmarkValidationData.matFloat:
                              1024 correlation planes of 31 x 31 produced by the
                              reference's GPU job over 16 x 16 IAs without overlap
@@ -31,6 +33,7 @@ GPL-3.0 data from QuickLabPIV-ng (Luis Mendes); attribution kept here.
 import json
 import os
 import re
+import shutil
 import sys
 
 import numpy as np
@@ -133,9 +136,16 @@ def fastreal_kat():
                 tol=1e-3)
 
 
+def copy_tiffs(ref):
+    for k in (0, 1):
+        name = f"rankine_vortex01_{k}.tif"
+        shutil.copyfile(os.path.join(ref, "resources/fftBenchmark", name), os.path.join(HERE, name))
+
+
 def main(ref):
     from PIL import Image
 
+    copy_tiffs(ref)
     fa = os.path.join(ref, "resources/fftBenchmark/rankine_vortex01_0.tif")
     fb = os.path.join(ref, "resources/fftBenchmark/rankine_vortex01_1.tif")
     ia = np.array(Image.open(fa))

@@ -1,0 +1,73 @@
+"""Frame ingest (qlp_image_read) on the CPU: the reference's own test frames
+(resources/fftBenchmark/rankine_vortex01_{0,1}.tif, 8-bit PackBits) against
+their decoded pixels, and frames written by PIL in every layout the reader
+accepts, read back exactly; unsupported layouts are refused."""
+import os
+
+import numpy as np
+import pytest
+from PIL import Image
+
+import qlpiv
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def test_reference_tiffs_decode_to_their_pixels(rankine_pair):
+    a, b = rankine_pair
+    assert np.array_equal(qlpiv.read_image(os.path.join(GOLDEN, "rankine_vortex01_0.tif")), a)
+    assert np.array_equal(qlpiv.read_image(os.path.join(GOLDEN, "rankine_vortex01_1.tif")), b)
+
+
+def frame(seed, H=97, W=131):
+    rng = np.random.default_rng(seed)
+    f = rng.integers(0, 256, (H, W)).astype(np.uint8)
+    f[10:30, 40:90] = 7  # runs for the run-length and dictionary coders
+    f[50] = np.arange(W) % 256
+    return f
+
+
+@pytest.mark.parametrize("compression", [None, "packbits", "tiff_lzw", "tiff_adobe_deflate"])
+@pytest.mark.parametrize("rows_per_strip", [None, 8])
+def test_tiff_layouts_round_trip(tmp_path, compression, rows_per_strip):
+    f = frame(1)
+    path = tmp_path / "f.tif"
+    kw = {}
+    if rows_per_strip:
+        kw["tiffinfo"] = {278: rows_per_strip}
+    Image.fromarray(f).save(path, compression=compression, **kw)
+    assert np.array_equal(qlpiv.read_image(path), f)
+
+
+def test_tiff_horizontal_predictor(tmp_path):
+    f = frame(2)
+    path = tmp_path / "p.tif"
+    Image.fromarray(f).save(path, compression="tiff_lzw", tiffinfo={317: 2})
+    if Image.open(path).tag_v2.get(317, 1) != 2:
+        pytest.skip("PIL wrote no predictor")
+    assert np.array_equal(qlpiv.read_image(path), f)
+
+
+@pytest.mark.parametrize("optimize", [False, True])
+def test_png_round_trip(tmp_path, optimize):
+    f = frame(3, 64, 200)
+    path = tmp_path / "f.png"
+    Image.fromarray(f).save(path, optimize=optimize)
+    assert np.array_equal(qlpiv.read_image(path), f)
+
+
+def test_unsupported_layouts_are_refused(tmp_path):
+    rgb = np.zeros((8, 8, 3), np.uint8)
+    for name in ("c.png", "c.tif"):
+        Image.fromarray(rgb).save(tmp_path / name)
+        with pytest.raises(qlpiv.QlpError) as e:
+            qlpiv.read_image(tmp_path / name)
+        assert e.value.code == qlpiv._lib.QLP_ERR_UNSUPPORTED
+    Image.fromarray(np.zeros((8, 8), np.uint16)).save(tmp_path / "d16.png")
+    with pytest.raises(qlpiv.QlpError):
+        qlpiv.read_image(tmp_path / "d16.png")
+    Image.fromarray(np.zeros((8, 8), np.uint8)).save(tmp_path / "j.jpg")
+    with pytest.raises(qlpiv.QlpError):
+        qlpiv.read_image(tmp_path / "j.jpg")
+    with pytest.raises(qlpiv.QlpError):
+        qlpiv.read_image(tmp_path / "missing.tif")
