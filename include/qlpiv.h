@@ -110,7 +110,7 @@ int qlp_xcorr_batch_dev(qlp_ctx *ctx, int n, int N, const uint8_t *tilesA, const
  * compute the same correlation as dense DFT-basis products on the matrix cores
  * in fp16 or bf16 with fp32 accumulation, with split hi + lo operands (or single
  * operands): the throughput form, accurate to the stated error against an FP64
- * correlation but not value-identical.  IA sizes 16 and 32. */
+ * correlation but not value-identical.  IA sizes 16, 32 and 64. */
 #define QLP_XCORR_EXACT 0
 #define QLP_XCORR_MFMA_F16 1         /* fp16 hi + lo operands */
 #define QLP_XCORR_MFMA_BF16 2        /* bf16 hi + lo operands */
@@ -235,6 +235,12 @@ int qlp_xcorr_images_dev(qlp_ctx *ctx, int n, int N, int mode, const uint8_t *im
                          const uint8_t *imgB, int H, int W, const int *top, const int *left,
                          const float *du, const float *dv, int warp_mode, float *planes,
                          float *peaks, float *min_floor, float *nbr, int nbr_half);
+/* qlp_xcorr_images_dev in the chosen arithmetic form (QLP_XCORR_*).  No
+ * counterpart in the reference: the matrix-core form of the fused pass. */
+int qlp_xcorr_images_form_dev(qlp_ctx *ctx, int form, int n, int N, int mode, const uint8_t *imgA,
+                              const uint8_t *imgB, int H, int W, const int *top, const int *left,
+                              const float *du, const float *dv, int warp_mode, float *planes,
+                              float *peaks, float *min_floor, float *nbr, int nbr_half);
 
 /* ---- vector validation and replacement ------------------------------------ */
 
@@ -356,8 +362,8 @@ typedef struct qlp_piv_config {
     int peak_kernel_width;    /* MultiPeaks kernelWidth */
     qlp_valid_config valid;   /* vector validation after every level (n_stages 0: off) */
     int inherit_mode;         /* QLP_INHERIT_* between levels (0: Direct) */
-    int xcorr_form;           /* QLP_XCORR_* arithmetic of the 16 and 32 px levels (64 px
-                                 levels always use QLP_XCORR_EXACT) */
+    int xcorr_form;           /* QLP_XCORR_* arithmetic of the 16, 32 and 64 px levels
+                                 (128 px levels always use QLP_XCORR_EXACT) */
     int warp_order;           /* QLP_WARP_ORDER_* of the Micro, Mini and Full warps */
 } qlp_piv_config;
 
@@ -445,7 +451,7 @@ typedef struct {
 /* Lay out the IAs of n_pairs pairs (first-level grid of each frame with the
  * config's overlap and margins) and allocate their device state.  Used fields
  * of cfg: overlap_factor, margins, norm_mode, subpix_kind / _pixels /
- * _iterations, interpolator_start_step, xcorr_form (16 and 32 px classes). */
+ * _iterations, interpolator_start_step, xcorr_form (16, 32 and 64 px classes). */
 int qlp_batch_create(qlp_ctx *ctx, int n_pairs, const qlp_batch_pair *pairs,
                      const qlp_piv_config *cfg, qlp_batch **out);
 /* Fix the kernel shape of the class of IA size 'ia' (ia 0: every class) and/or

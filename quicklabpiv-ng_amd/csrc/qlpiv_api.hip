@@ -155,7 +155,7 @@ int qlp_ctx_destroy(qlp_ctx *h)
     if (ctx->vwork) (void)hipFree(ctx->vwork);
     if (ctx->iwork) (void)hipFree(ctx->iwork);
     if (ctx->bigwork) (void)hipFree(ctx->bigwork);
-    for (int a = 0; a < 2; a++)
+    for (int a = 0; a < 3; a++)
         for (int b = 0; b < 2; b++)
             if (ctx->mfma_tab[a][b]) (void)hipFree(ctx->mfma_tab[a][b]);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -201,7 +201,8 @@ int qlp_xcorr_form_dev(qlp_ctx *h, int form, int n, int N, const uint8_t *tA, co
     if (rc) return rc;
     if (form < QLP_XCORR_MFMA_F16 || form > QLP_XCORR_MFMA_BF16_SINGLE)
         return fail(QLP_ERR_ARG, "xcorr: unknown arithmetic form");
-    if (N != 16 && N != 32) return fail(QLP_ERR_UNSUPPORTED, "xcorr: the MFMA forms take IA sizes 16 and 32");
+    if (N != 16 && N != 32 && N != 64)
+        return fail(QLP_ERR_UNSUPPORTED, "xcorr: the MFMA forms take IA sizes 16, 32 and 64");
     if (n > 0 && (!tA || !tB || !maxA || !planes || (mode == QLP_NORM_PER_TILE && !maxB)))
         return fail(QLP_ERR_ARG, "xcorr: null buffer");
     PassParams p;
@@ -340,12 +341,16 @@ int qlp_warp_micro_tiles_dev(qlp_ctx *h, const uint8_t *img, int H, int W, int n
     return launch_warp_micro(&h->c, img, H, W, n, N, top, left, u, v, tiles, max_out);
 }
 
-int qlp_xcorr_images_dev(qlp_ctx *h, int n, int N, int mode, const uint8_t *imgA,
-                         const uint8_t *imgB, int H, int W, const int *top, const int *left,
-                         const float *du, const float *dv, int warp_mode, float *planes,
-                         float *peaks, float *min_floor, float *nbr, int nbr_half)
+int qlp_xcorr_images_form_dev(qlp_ctx *h, int form, int n, int N, int mode, const uint8_t *imgA,
+                              const uint8_t *imgB, int H, int W, const int *top, const int *left,
+                              const float *du, const float *dv, int warp_mode, float *planes,
+                              float *peaks, float *min_floor, float *nbr, int nbr_half)
 {
     if (!h) return fail(QLP_ERR_ARG, "null context");
+    if (form < QLP_XCORR_EXACT || form > QLP_XCORR_MFMA_BF16_SINGLE)
+        return fail(QLP_ERR_ARG, "xcorr: unknown correlation form");
+    if (form != QLP_XCORR_EXACT && N != 16 && N != 32 && N != 64)
+        return fail(QLP_ERR_UNSUPPORTED, "xcorr: the MFMA forms take IA sizes 16, 32 and 64");
     int rc = xcorr_args(n, N, mode);
     if (rc) return rc;
     if (n > 0 && (!imgA || !imgB || !top || !left)) return fail(QLP_ERR_ARG, "xcorr_images: null buffer");
@@ -371,7 +376,17 @@ int qlp_xcorr_images_dev(qlp_ctx *h, int n, int N, int mode, const uint8_t *imgA
     p.min_floor = min_floor;
     p.nbr = nbr;
     p.nbr_half = nbr_half;
+    p.arith = form;
     return launch_pass(&h->c, p);
+}
+
+int qlp_xcorr_images_dev(qlp_ctx *h, int n, int N, int mode, const uint8_t *imgA,
+                         const uint8_t *imgB, int H, int W, const int *top, const int *left,
+                         const float *du, const float *dv, int warp_mode, float *planes,
+                         float *peaks, float *min_floor, float *nbr, int nbr_half)
+{
+    return qlp_xcorr_images_form_dev(h, QLP_XCORR_EXACT, n, N, mode, imgA, imgB, H, W, top, left,
+                                     du, dv, warp_mode, planes, peaks, min_floor, nbr, nbr_half);
 }
 
 }  // extern "C"

@@ -108,7 +108,7 @@ static int subpix_half(const qlp_piv_config &c)
 // Candidate shapes of a class (the first is the library default).
 static int shape_candidates(int N, int arith, int *out)
 {
-    if (arith != QLP_XCORR_EXACT && (N == 16 || N == 32)) {
+    if (arith != QLP_XCORR_EXACT && N <= 64) {
         out[0] = SHAPE_AUTO;  // the matrix-core kernel has one shape
         return 1;
     }
@@ -225,10 +225,10 @@ static int enqueue_class(qlp_batch *b, const BatchClass &c, int shape, hipStream
                              q.nbr_half, c.nbr, c.mf, c.peaks, c.sub, c.u, c.v);
     if (rc) return rc;
     // every buffer was sized before the first enqueue: nothing may have grown
-    if (sub.bigwork != b->bigwork || sub.mfma_tab[0][0] != b->ctx->mfma_tab[0][0] ||
-        sub.mfma_tab[1][0] != b->ctx->mfma_tab[1][0] || sub.mfma_tab[0][1] != b->ctx->mfma_tab[0][1] ||
-        sub.mfma_tab[1][1] != b->ctx->mfma_tab[1][1])
-        return fail(QLP_ERR_DEVICE, "batch: a workspace grew during an enqueue");
+    bool grew = sub.bigwork != b->bigwork;
+    for (int a = 0; a < 3; a++)
+        for (int f = 0; f < 2; f++) grew = grew || sub.mfma_tab[a][f] != b->ctx->mfma_tab[a][f];
+    if (grew) return fail(QLP_ERR_DEVICE, "batch: a workspace grew during an enqueue");
     return QLP_OK;
 }
 
@@ -236,7 +236,7 @@ static int enqueue_class(qlp_batch *b, const BatchClass &c, int shape, hipStream
 static int prepare_class(qlp_batch *b, const BatchClass &c)
 {
     if (c.n == 0) return QLP_OK;
-    if (b->cfg.xcorr_form != QLP_XCORR_EXACT && (c.N == 16 || c.N == 32)) {
+    if (b->cfg.xcorr_form != QLP_XCORR_EXACT && c.N <= 64) {
         // the MFMA tables are uploaded on the form's first launch: one IA
         Context *ctx = b->ctx;
         PassParams q;

@@ -60,8 +60,8 @@ struct Context {
     // per-work-group tile slots of the IA-128 correlation (grown on demand)
     void *bigwork = nullptr;
     size_t bigwork_bytes = 0;
-    // constant DFT fragments of the MFMA correlation form, [N 16/32][fp16/bf16]
-    void *mfma_tab[2][2] = {};
+    // constant DFT fragments of the MFMA correlation form, [N 16/32/64][fp16/bf16]
+    void *mfma_tab[3][2] = {};
     // kernel shapes measured by the batch autotuner, by class key
     std::map<uint64_t, int> tuned_shape;
 };

@@ -28,6 +28,7 @@
 // Operand fragments are kept as four packed 32-bit words (two 16-bit values
 // each), converted in pairs.
 #include "qlpiv_device.h"
+#include "qlpiv_epilogue.h"
 #include "qlpiv_internal.h"
 
 #include <math.h>
@@ -68,9 +69,8 @@ struct Mf {
     // (bounds of the normalised data).  Scaling C by the product of both
     // instead pushes its small coefficients into fp16 subnormals (measured:
     // 2e-6 instead of 7e-7 of the plane maximum).
-    static constexpr int E1 = (N == 16) ? 4 : 6;
-    static constexpr int E2 = (M == 32) ? 5 : 6;
-    static_assert(RT == 2, "the quad layout needs 2 x 2 tiles");
+    static constexpr int E1 = (N == 16) ? 4 : (N == 32 ? 6 : 8);
+    static constexpr int E2 = (M == 32) ? 5 : (M == 64 ? 6 : 7);
 };
 
 template <int MT>
@@ -220,6 +220,7 @@ template <int N, bool BF, bool SPLIT, bool FROM_IMAGE, bool PEAK>
 __global__ __launch_bounds__(64 * MF_WAVES, 3) void xcorr_mfma_kernel(MfmaParams mp, int n_iter)
 {
     using X = Mf<N>;
+    static_assert(X::RT == 2, "one wavefront per IA holds 2 x 2 tiles");
     constexpr int M = X::M, D = X::D, MT = X::MT, NS = X::NS;
     const PassParams &p = mp.p;
     __shared__ u4 s_frag[X::NF * 2 * 64];
@@ -607,6 +608,307 @@ __global__ __launch_bounds__(64 * MF_WAVES, 3) void xcorr_mfma_kernel(MfmaParams
     }
 }
 
+// ---- N = 64: four wavefronts per IA ------------------------------------------
+//
+// At M = 128 every matrix of the chain is 4 x 4 tiles of 32 x 32, too many for
+// one wavefront's registers.  The four wavefronts of a work-group share one IA:
+// wave (sr, sc) owns the forward and spectrum tiles rt in {sr, sr + 2} (column
+// frequency) x ct in {sc, sc + 2} (row frequency).  Basis tile t + 2 holds the
+// sines of tile t's cosines (tile 2 starts with cos(pi x), the partner of
+// k = 0), so each wave holds whole (cos/sin x cos/sin) quads and forms its part
+// of the spectrum alone.  The transposes between the products go through one
+// LDS buffer, in the order of the chain:
+//   V  = Hf x          wave w: row tile w                 -> LDS [rho][m]
+//   W  = Hf V^T        wave (sr, sc): its 4 tiles, x = a then g
+//   C  = spectrum      wave-local                         -> LDS [rho][sigma]
+//   Y  = Hout^T C^T    wave w: row tile w (4 column tiles) -> LDS [p][rho]
+//   R  = Hout^T Y^T    wave w: row tile w                 -> plane, FromCenter
+// The constant fragments (Hf: 4 x 4, Hout^T: 4 x 8) are read from global memory
+// (96 KB, L2-resident).  Work-groups loop over the IAs.
+namespace m64 {
+constexpr int N = 64, M = 128, D = 127, MT = 32, NS = 16, KN = 4, KM = 8;
+constexpr int SV = 72;   // LDS row stride of V (halves)
+constexpr int SC = 136;  // LDS row stride of C and Y (halves)
+constexpr int NT = 256;
+constexpr int NF_HF = 16;  // Hf fragments first, then Hout^T (32)
+}  // namespace m64
+
+template <bool BF, bool SPLIT, bool FROM_IMAGE, bool PEAK>
+__global__ __launch_bounds__(m64::NT, 2) void xcorr_mfma64_kernel(PassParams p,
+                                                                  const u4 *__restrict__ frags)
+{
+    using namespace m64;
+    using X = Mf<64>;
+    // hi and lo halves of V, C or Y; the plane (fp32, stride M) at the end
+    __shared__ __attribute__((aligned(16))) unsigned short s_buf[2][M * SC];
+    __shared__ uint64_t s_key[4];
+    __shared__ int s_cell[4];
+    __shared__ float s_min[4];
+    unsigned short *__restrict__ bh = s_buf[0];
+    unsigned short *__restrict__ bl = s_buf[1];
+    float *__restrict__ s_plane = (float *)&s_buf[0][0];
+
+    const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+    const int sr = wave >> 1, sc = wave & 1;
+
+    for (int ia = blockIdx.x; ia < p.n; ia += gridDim.x) {
+        int lane = threadIdx.x & 63;
+        asm volatile("" : "+v"(lane));  // see xcorr_mfma_kernel
+        const int j = lane % MT, g = lane / MT;
+        auto frag = [&](int f, int lo) -> u4 { return frags[(f * 2 + lo) * 64 + lane]; };
+
+        // ---- pixel sources (as in the other forms) ------------------------------
+        const uint8_t *a8 = nullptr, *b8 = nullptr;
+        int y0 = 0, x0 = 0, by0 = 0, bx0 = 0;
+        float fu = 0.0f, fv = 0.0f;
+        bool warp = false;
+        if (FROM_IMAGE) {
+            const int per_pair = p.per_pair > 0 ? p.per_pair : p.n;
+            const int pair = ia / per_pair, tt = ia - pair * per_pair;
+            const size_t img_off = (size_t)pair * p.H * p.W;
+            a8 = p.imgA + img_off;
+            b8 = p.imgB + img_off;
+            y0 = p.top[tt];
+            x0 = p.left[tt];
+            by0 = y0;
+            bx0 = x0;
+            warp = (p.bmode == B_WARP_MICRO) && p.du != nullptr;
+            if (p.du != nullptr) {
+                const float u = p.du[ia], v = p.dv[ia];
+                if (p.bmode == B_CLIP_SHIFT) {
+                    by0 = fm_round((float)y0 + u);
+                    bx0 = fm_round((float)x0 + v);
+                } else if (warp) {
+                    const float flu = floorf(u), flv = floorf(v);
+                    by0 = (short)(y0 + (int)flu);
+                    bx0 = (short)(x0 + (int)flv);
+                    fu = u - flu;
+                    fv = v - flv;
+                }
+            }
+        } else {
+            a8 = p.tA + (size_t)ia * N * N;
+            b8 = p.tB + (size_t)ia * N * N;
+        }
+        // pixel (r, m) of tile A or B (in-frame test; B: shift or micro warp)
+        auto pix = [&](int im, int r, int m, bool &in) -> int {
+            in = true;
+            if (!FROM_IMAGE) return (im == 0 ? a8 : b8)[r * N + m];
+            if (im == 1 && warp) return (int)warp_pixel(b8, p.H, p.W, fu, fv, by0 + r, bx0 + m);
+            const int y = (im == 0 ? y0 : by0) + r, x = (im == 0 ? x0 : bx0) + m;
+            in = y >= 0 && y < p.H && x >= 0 && x < p.W;
+            return in ? (im == 0 ? a8 : b8)[(size_t)y * p.W + x] : 0;
+        };
+
+        // ---- forward, one image at a time: V = Hf x, then W = Hf V^T ------------
+        Acc<MT> W[2][2][2];  // [image][rt sr / sr+2][ct sc / sc+2]
+        float mA = 16.0f;
+#pragma unroll
+        for (int im = 0; im < 2; im++) {
+            // the tile in the B layout of Hf x: (kb, s) = row kphys, column 32 c + j
+            float t[2][KN][8];
+            int mx = 0;
+#pragma unroll
+            for (int c = 0; c < 2; c++)
+#pragma unroll
+                for (int kb = 0; kb < KN; kb++)
+#pragma unroll
+                    for (int s = 0; s < 8; s++) {
+                        bool in;
+                        const int v = pix(im, kphys<MT>(kb, g, s), MT * c + j, in);
+                        if (in) mx = max(mx, v);
+                        t[c][kb][s] = (float)v;
+                    }
+            float mt;
+            if (FROM_IMAGE) {
+#pragma unroll
+                for (int o = 32; o >= 1; o >>= 1) mx = max(mx, __shfl_xor(mx, o, 64));
+                mt = (im == 1 && (warp || p.bmax255)) ? 255.0f : (float)(mx < 16 ? 16 : mx);
+            } else {
+                mt = im == 0 ? p.maxA[ia] : p.maxB[ia];
+            }
+            if (im == 0) mA = mt;
+            if (im == 1 && p.mode == QLP_NORM_A_MAX) mt = mA;
+            // x / max * 16 + 1 as x * (16 / max) + 1, as in the other sizes
+            const float sx = 16.0f / mt;
+            u4 xh[2][KN], xl[2][KN];
+#pragma unroll
+            for (int c = 0; c < 2; c++)
+#pragma unroll
+                for (int kb = 0; kb < KN; kb++)
+#pragma unroll
+                    for (int s = 0; s < 8; s += 2) {
+                        unsigned h, l;
+                        split2<BF>(t[c][kb][s] * sx + 1.0f, t[c][kb][s + 1] * sx + 1.0f, h, l);
+                        xh[c][kb][s >> 1] = h;
+                        xl[c][kb][s >> 1] = l;
+                    }
+            // V row tile 'wave' (32 basis rows x 64 columns) to LDS [rho][m]
+#pragma unroll
+            for (int c = 0; c < 2; c++) {
+                Acc<MT> acc = {};
+#pragma unroll
+                for (int kb = 0; kb < KN; kb++)
+                    acc = mma3<BF, SPLIT, MT>(frag(wave * KN + kb, 0), frag(wave * KN + kb, 1),
+                                              xh[c][kb], xl[c][kb], acc);
+#pragma unroll
+                for (int v = 0; v < NS; v += 2) {
+                    unsigned h, l;
+                    split2<BF>(acc[v], acc[v + 1], h, l);
+                    const int r = MT * wave + mrow<MT>(v, g), m = MT * c + j;
+                    bh[r * SV + m] = (unsigned short)h;
+                    bl[r * SV + m] = (unsigned short)l;
+                    bh[(r + 1) * SV + m] = (unsigned short)(h >> 16);
+                    bl[(r + 1) * SV + m] = (unsigned short)(l >> 16);
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int ri = 0; ri < 2; ri++)
+#pragma unroll
+                for (int ci = 0; ci < 2; ci++) {
+                    const int rt = sr + 2 * ri, ct = sc + 2 * ci;
+                    Acc<MT> w = {};
+                    static_for<0, KN>([&](auto kc) {
+                        constexpr int kb = decltype(kc)::value;
+                        w = mma3<BF, SPLIT, MT>(frag(rt * KN + kb, 0), frag(rt * KN + kb, 1),
+                                                bfrag_from_lds<MT, SV, kb, 0, N>(bh, MT * ct + j, g),
+                                                bfrag_from_lds<MT, SV, kb, 0, N>(bl, MT * ct + j, g),
+                                                w);
+                    });
+                    W[im][ri][ci] = w;
+                }
+            __syncthreads();  // V is rewritten (next image) or replaced by C
+        }
+
+        // ---- spectrum (the quad formulas of the one-wave kernel) -> LDS [rho][sigma]
+        constexpr float sg = 4.0f / (float)(M * M) / (float)(1 << X::E1);
+        constexpr float se = 2.0f / (float)(M * M) / (float)(1 << X::E1);
+        constexpr float s1 = 1.0f / (float)(M * M) / (float)(1 << X::E1);
+#pragma unroll
+        for (int v0 = 0; v0 < NS; v0 += 2) {
+            float o[2][2][2];  // [slot v0 + e][ri][ci]
+#pragma unroll
+            for (int e = 0; e < 2; e++) {
+                const int v = v0 + e;
+                const bool kk = sc == 0 && j == 0, ll = sr == 0 && mrow<MT>(v, g) == 0;
+                const float CC = W[0][0][0][v], SC_ = W[0][0][1][v], CS = W[0][1][0][v],
+                            SS = W[0][1][1][v];
+                const float cc = W[1][0][0][v], scg = W[1][0][1][v], cs = W[1][1][0][v],
+                            ss = W[1][1][1][v];
+                if (!kk && !ll) {
+                    o[e][0][0] = sg * (CC * cc + SS * ss + SC_ * scg + CS * cs);
+                    o[e][1][1] = sg * (CC * ss + SS * cc - SC_ * cs - CS * scg);
+                    o[e][0][1] = -sg * (SC_ * cc - CC * scg + SS * cs - CS * ss);
+                    o[e][1][0] = sg * (CC * cs - CS * cc + SC_ * ss - SS * scg);
+                } else if (kk && !ll) {
+                    o[e][0][0] = se * (CC * cc + CS * cs);
+                    o[e][1][0] = se * (CC * cs - CS * cc);
+                    o[e][0][1] = se * (SC_ * scg + SS * ss);
+                    o[e][1][1] = se * (SC_ * ss - SS * scg);
+                } else if (!kk && ll) {
+                    o[e][0][0] = se * (CC * cc + SC_ * scg);
+                    o[e][0][1] = -se * (SC_ * cc - CC * scg);
+                    o[e][1][0] = se * (CS * cs + SS * ss);
+                    o[e][1][1] = -se * (SS * cs - CS * ss);
+                } else {
+                    o[e][0][0] = s1 * CC * cc;
+                    o[e][0][1] = s1 * SC_ * scg;
+                    o[e][1][0] = s1 * CS * cs;
+                    o[e][1][1] = s1 * SS * ss;
+                }
+            }
+            // slots v0, v0 + 1 are sigma, sigma + 1 of the same rho: one 32-bit store
+#pragma unroll
+            for (int ri = 0; ri < 2; ri++)
+#pragma unroll
+                for (int ci = 0; ci < 2; ci++) {
+                    unsigned h, l;
+                    split2<BF>(o[0][ri][ci], o[1][ri][ci], h, l);
+                    const int rho = MT * (sc + 2 * ci) + j;
+                    const int sigma = MT * (sr + 2 * ri) + mrow<MT>(v0, g);
+                    *(unsigned *)&bh[rho * SC + sigma] = h;
+                    *(unsigned *)&bl[rho * SC + sigma] = l;
+                }
+        }
+        __syncthreads();
+
+        // ---- inverse: Y = Hout^T C^T (row tile 'wave'), then R = Hout^T Y^T ------
+        u4 hoh[KM], hol[KM];
+#pragma unroll
+        for (int kb = 0; kb < KM; kb++) {
+            hoh[kb] = frag(NF_HF + wave * KM + kb, 0);
+            hol[kb] = frag(NF_HF + wave * KM + kb, 1);
+        }
+        Acc<MT> Y[4];
+#pragma unroll
+        for (int ct = 0; ct < 4; ct++) {
+            Acc<MT> acc = {};
+            static_for<0, KM>([&](auto kc) {
+                constexpr int kb = decltype(kc)::value;
+                acc = mma3<BF, SPLIT, MT>(hoh[kb], hol[kb],
+                                          bfrag_from_lds<MT, SC, kb, 0, M>(bh, MT * ct + j, g),
+                                          bfrag_from_lds<MT, SC, kb, 0, M>(bl, MT * ct + j, g),
+                                          acc);
+            });
+            Y[ct] = acc * (1.0f / (float)(1 << X::E2));
+        }
+        __syncthreads();  // every wave has read C
+#pragma unroll
+        for (int ct = 0; ct < 4; ct++)
+#pragma unroll
+            for (int v = 0; v < NS; v += 2) {
+                unsigned h, l;
+                split2<BF>(Y[ct][v], Y[ct][v + 1], h, l);
+                const int r = MT * wave + mrow<MT>(v, g), rho = MT * ct + j;
+                bh[r * SC + rho] = (unsigned short)h;
+                bl[r * SC + rho] = (unsigned short)l;
+                bh[(r + 1) * SC + rho] = (unsigned short)(h >> 16);
+                bl[(r + 1) * SC + rho] = (unsigned short)(l >> 16);
+            }
+        __syncthreads();
+        Acc<MT> R[4];
+#pragma unroll
+        for (int bt = 0; bt < 4; bt++) {
+            Acc<MT> acc = {};
+            static_for<0, KM>([&](auto kc) {
+                constexpr int kb = decltype(kc)::value;
+                acc = mma3<BF, SPLIT, MT>(hoh[kb], hol[kb],
+                                          bfrag_from_lds<MT, SC, kb, 0, M>(bh, MT * bt + j, g),
+                                          bfrag_from_lds<MT, SC, kb, 0, M>(bl, MT * bt + j, g),
+                                          acc);
+            });
+            R[bt] = acc;
+        }
+        __syncthreads();  // every wave has read Y: the buffer takes the plane
+
+        // ---- epilogues: plane store, FromCenter -------------------------------------
+        constexpr float unscale = (float)(1 << (X::E1 + X::E2));
+        float *__restrict__ out = p.planes ? p.planes + (size_t)ia * D * D : nullptr;
+        uint64_t best = 0;
+        int bestCell = 0;
+        float mn = 3.40282347e+38f;
+#pragma unroll
+        for (int bt = 0; bt < 4; bt++)
+#pragma unroll
+            for (int v = 0; v < NS; v++) {
+                const int a = MT * wave + mrow<MT>(v, g), b = MT * bt + j;
+                if (a >= D || b >= D) continue;
+                const float val = R[bt][v] * unscale;
+                if (out) out[(unsigned)(a * D + b)] = val;
+                if (PEAK) {
+                    s_plane[a * M + b] = val;
+                    from_center_visit(val, a, b, D / 2, a * D + b, best, bestCell, mn);
+                }
+            }
+        if (PEAK) {
+            __syncthreads();
+            peak_epilogue<NT, D>(p, ia, best, bestCell, mn, s_plane, M, 1, s_key, s_cell, s_min);
+        }
+        __syncthreads();  // the buffer is rewritten by the next IA
+    }
+}
+
 // ---- host side ---------------------------------------------------------------
 
 static double mf_basis(int M, int r, long x)
@@ -656,7 +958,7 @@ static void build_frags(bool bf, std::vector<unsigned short> &out)
     constexpr int M = X::M, MT = X::MT;
     out.assign((size_t)X::NF * 2 * 64 * 8, 0);
     auto put = [&](int f, auto A, int rows, int cols, int kblocks, int fbase) {
-        for (int rt = 0; rt < 2; rt++)
+        for (int rt = 0; rt < X::RT; rt++)
             for (int kb = 0; kb < kblocks; kb++) {
                 const int ff = fbase + rt * kblocks + kb;
                 for (int lane = 0; lane < 64; lane++)
@@ -680,13 +982,15 @@ static void build_frags(bool bf, std::vector<unsigned short> &out)
 
 static int mfma_tables(Context *ctx, int N, bool bf, const u4 **out)
 {
-    const int ni = N == 16 ? 0 : 1, bi = bf ? 1 : 0;
+    const int ni = N == 16 ? 0 : (N == 32 ? 1 : 2), bi = bf ? 1 : 0;
     if (!ctx->mfma_tab[ni][bi]) {
         std::vector<unsigned short> h;
         if (N == 16)
             build_frags<16>(bf, h);
-        else
+        else if (N == 32)
             build_frags<32>(bf, h);
+        else
+            build_frags<64>(bf, h);
         void *d = nullptr;
         const size_t bytes = h.size() * sizeof(unsigned short);
         if (int rc = check_hip(hipMalloc(&d, bytes), "hipMalloc(mfma tables)")) return rc;
@@ -724,11 +1028,38 @@ static void launch_mfma_t(Context *ctx, const MfmaParams &mp, bool from_image, b
 #undef QLP_MF_LAUNCH
 }
 
+template <bool BF, bool SPLIT>
+static void launch_mfma64_t(Context *ctx, const PassParams &p, const u4 *frags, bool from_image,
+                            bool peak)
+{
+    // two work-groups of 70 KB LDS per CU; each loops over IAs
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        cus = 256;
+    const int grid = p.n < 2 * cus ? p.n : 2 * cus;
+#define QLP_MF64_LAUNCH(FI, PK)                                                                \
+    hipLaunchKernelGGL((xcorr_mfma64_kernel<BF, SPLIT, FI, PK>), dim3(grid), dim3(m64::NT), 0,   \
+                       ctx->stream, p, frags)
+    if (from_image) {
+        if (peak)
+            QLP_MF64_LAUNCH(true, true);
+        else
+            QLP_MF64_LAUNCH(true, false);
+    } else {
+        if (peak)
+            QLP_MF64_LAUNCH(false, true);
+        else
+            QLP_MF64_LAUNCH(false, false);
+    }
+#undef QLP_MF64_LAUNCH
+}
+
 int launch_pass_mfma(Context *ctx, const PassParams &p, int form, bool from_image, bool peak)
 {
     if (p.n <= 0) return QLP_OK;
-    if (p.N != 16 && p.N != 32)
-        return fail(QLP_ERR_UNSUPPORTED, "mfma correlation: IA size must be 16 or 32");
+    if (p.N != 16 && p.N != 32 && p.N != 64)
+        return fail(QLP_ERR_UNSUPPORTED, "mfma correlation: IA size must be 16, 32 or 64");
     const bool bf = (form == QLP_XCORR_MFMA_BF16 || form == QLP_XCORR_MFMA_BF16_SINGLE);
     const bool split = (form == QLP_XCORR_MFMA_F16 || form == QLP_XCORR_MFMA_BF16);
     if (form < QLP_XCORR_MFMA_F16 || form > QLP_XCORR_MFMA_BF16_SINGLE)
@@ -750,10 +1081,21 @@ int launch_pass_mfma(Context *ctx, const PassParams &p, int form, bool from_imag
                 launch_mfma_t<NN, false, false>(ctx, mp, from_image, peak);                   \
         }                                                                                     \
     } while (0)
-    if (p.N == 16)
+    if (p.N == 16) {
         QLP_MF_N(16);
-    else
+    } else if (p.N == 32) {
         QLP_MF_N(32);
+    } else if (bf) {
+        if (split)
+            launch_mfma64_t<true, true>(ctx, p, mp.frags, from_image, peak);
+        else
+            launch_mfma64_t<true, false>(ctx, p, mp.frags, from_image, peak);
+    } else {
+        if (split)
+            launch_mfma64_t<false, true>(ctx, p, mp.frags, from_image, peak);
+        else
+            launch_mfma64_t<false, false>(ctx, p, mp.frags, from_image, peak);
+    }
 #undef QLP_MF_N
     return check_hip(hipGetLastError(), "mfma correlation launch");
 }

@@ -57,6 +57,8 @@ SIGNATURES = {
     "qlp_warp_image_dev": (_i, [_vp, _vp, _i, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp]),
     "qlp_xcorr_images_dev": (_i, [_vp, _i, _i, _i, _vp, _vp, _i, _i, _vp, _vp, _vp, _vp, _i,
                                   _vp, _vp, _vp, _vp, _i]),
+    "qlp_xcorr_images_form_dev": (_i, [_vp, _i, _i, _i, _i, _vp, _vp, _i, _i, _vp, _vp, _vp, _vp,
+                                       _i, _vp, _vp, _vp, _vp, _i]),
     "qlp_piv_plan_create": (_i, [_vp, _i, _i, _i, _vp, ctypes.POINTER(_vp)]),
     "qlp_piv_plan_destroy": (_i, [_vp]),
     "qlp_piv_plan_levels": (_i, [_vp, ctypes.POINTER(_i)]),
