@@ -1,0 +1,263 @@
+// MultiPeaks (maximum/FindMaximumMultiPeaks.java:42-162) over one plane per
+// segment of W lanes of a wavefront, shared by the stand-alone peak kernel and
+// the correlation passes that still hold their planes in LDS.
+//
+// Block maxima of kernelWidth^2 blocks in the reference's four-quadrant visit
+// order feed a bounded ascending list with neighbour suppression
+// (util/SimpleFixedLengthFloatLinkedList.addInAscendOrder :98-237).  The block
+// maxima are found by all lanes of the segment in parallel; the insertions that
+// can change the list are replayed in visit order, including the reference's
+// carry-over of the previous block's position when a block holds no value
+// above 0.
+#pragma once
+
+#include <float.h>
+
+#include "qlpiv_device.h"
+
+namespace qlpiv {
+
+constexpr int MP_MAX_PEAKS = 16;
+
+struct MpList {
+    int size;
+    float v[MP_MAX_PEAKS + 1];
+    short i[MP_MAX_PEAKS + 1], j[MP_MAX_PEAKS + 1];
+};
+
+// addInAscendOrder :98-237 on a sorted array: the same accept/reject and
+// delete decisions as the linked list, in the same order.
+__device__ __forceinline__ void mp_insert(MpList &L, int P, int kd, float value, short pi, short pj)
+{
+    if (L.size == 0) {
+        L.v[0] = value;
+        L.i[0] = pi;
+        L.j[0] = pj;
+        L.size = 1;
+        return;
+    }
+    int pos = 0;
+    uint32_t delmask = 0;
+    while (pos < L.size && L.v[pos] < value) {
+        const int di = pi - L.i[pos], dj = pj - L.j[pos];
+        if ((float)(di * di + dj * dj) < (float)kd && value > L.v[pos]) delmask |= 1u << pos;
+        pos++;
+    }
+    if (L.size == P && pos == 0) return;
+    for (int e = pos; e < L.size; e++) {
+        const int di = pi - L.i[e], dj = pj - L.j[e];
+        if ((float)(di * di + dj * dj) < (float)kd && value <= L.v[e]) return;
+    }
+    // compact away the deleted entries before the insertion point
+    int w = 0;
+    for (int e = 0; e < L.size; e++) {
+        if (e < pos && (delmask >> e) & 1u) continue;
+        L.v[w] = L.v[e];
+        L.i[w] = L.i[e];
+        L.j[w] = L.j[e];
+        w++;
+    }
+    const int ndel = L.size - w;
+    L.size = w;
+    const int ins = pos - ndel;
+    if (ins == 0 && L.size == P) return;
+    for (int e = L.size; e > ins; e--) {
+        L.v[e] = L.v[e - 1];
+        L.i[e] = L.i[e - 1];
+        L.j[e] = L.j[e - 1];
+    }
+    L.v[ins] = value;
+    L.i[ins] = pi;
+    L.j[ins] = pj;
+    L.size++;
+    if (ins > 0 && L.size > P) {  // remove() drops the smallest (first) entry
+        for (int e = 0; e + 1 < L.size; e++) {
+            L.v[e] = L.v[e + 1];
+            L.i[e] = L.i[e + 1];
+            L.j[e] = L.j[e + 1];
+        }
+        L.size--;
+    }
+}
+
+// Maximum (strict >, from 0, row-major first) and minimum of one block; rows
+// and columns past the plane edge are not part of the block.  Cell (r, c) is
+// m[r * rs + c * cs].  KW > 0 fixes the kernel width at compile time so that
+// all KW^2 loads are in flight together.
+template <int KW>
+__device__ __forceinline__ void block_max(const float *__restrict__ m, int rs, int cs, int D, int k,
+                                          int top, int left, float &bv, int &bi, int &bj, int &upd,
+                                          float &mn)
+{
+    if (KW > 0) {
+        float val[KW > 0 ? KW * KW : 1];
+#pragma unroll
+        for (int oi = 0; oi < KW; oi++)
+#pragma unroll
+            for (int oj = 0; oj < KW; oj++) {
+                const int r = top + oi, cc = left + oj;
+                val[oi * KW + oj] = (r < D && cc < D) ? m[r * rs + cc * cs] : 0.0f;
+            }
+#pragma unroll
+        for (int oi = 0; oi < KW; oi++)
+#pragma unroll
+            for (int oj = 0; oj < KW; oj++) {
+                const int r = top + oi, cc = left + oj;
+                if (r < D && cc < D) {
+                    const float v = val[oi * KW + oj];
+                    if (v > bv) {
+                        bv = v;
+                        bi = r;
+                        bj = cc;
+                        upd = 1;
+                    }
+                    if (v < mn) mn = v;
+                }
+            }
+    } else {
+        for (int oi = 0; oi < k; oi++) {
+            const int r = top + oi;
+            if (r >= D) break;
+            for (int oj = 0; oj < k; oj++) {
+                const int cc = left + oj;
+                if (cc >= D) break;
+                const float v = m[r * rs + cc * cs];
+                if (v > bv) {
+                    bv = v;
+                    bi = r;
+                    bj = cc;
+                    upd = 1;
+                }
+                if (v < mn) mn = v;
+            }
+        }
+    }
+}
+
+// Orders a wavefront's LDS writes before its later LDS reads by other lanes of
+// the same wavefront (its LDS instructions execute in issue order; the fences
+// keep the compiler from moving accesses across).
+__device__ __forceinline__ void mp_wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// One plane per segment of W lanes (64 / W segments per wavefront; every lane
+// of the wavefront calls this with its own segment's plane m and list L in
+// LDS).  The plane's entries (blocks in visit order) are processed W at a time:
+//  * every lane finds the maximum of one block (strict >, starting at 0) and
+//    the minimum of its cells; a block without a value above 0 is inserted as
+//    0 at the last position recorded for its quadrant, found with a ballot over
+//    the earlier lanes of the same quadrant in the segment or carried from
+//    earlier chunks;
+//  * the insertions are replayed in visit order, but only for entries that can
+//    change the list: once the list holds P entries, a value not above its
+//    smallest one is rejected without any other effect (addInAscendOrder
+//    :127-129), so a ballot finds each segment's next entry that is either
+//    above that minimum or meets a list with free room, and the segment's first
+//    lane inserts it.  Entries skipped between two insertions are exactly those
+//    the list would reject.
+// The segment's first lane writes peaks (P x 3: i, j, value, descending),
+// *min_floor and *count when act.
+template <int KW, int W>
+__device__ __forceinline__ void multipeak_segment(const float *__restrict__ m, int rs, int cs,
+                                                  int D, int P, int k, bool act, MpList &L,
+                                                  float *__restrict__ peaks, float *min_floor,
+                                                  int *count)
+{
+    const int lane = threadIdx.x & 63;
+    const int seg = lane / W, sl = lane % W;
+    const uint64_t segmask = (W == 64) ? ~0ull : (((1ull << (W & 63)) - 1ull) << (seg * W));
+
+    int iSlices = D / k;
+    if (iSlices * k != D) iSlices += 1;
+    const int iTop = iSlices - iSlices / 2;
+    int jSlices = D / k;
+    if (jSlices * k != D) jSlices += 1;
+    const int jLeft = jSlices - jSlices / 2;
+    const int nentries = iTop * jLeft * 4;
+    const int q = sl & 3;  // chunks start at multiples of W (a multiple of 4)
+    const uint64_t qlanes = 0x1111111111111111ull << (lane & 3);
+
+    if (sl == 0) L.size = 0;
+    mp_wave_sync();
+    float mn = FLT_MAX;
+    int staleI[4] = {0, 0, 0, 0}, staleJ[4] = {0, 0, 0, 0};
+    for (int base = 0; base < nentries; base += W) {
+        const int e = base + sl;
+        const bool valid = act && e < nentries;
+        float bv = 0.0f;
+        int bi = 0, bj = 0, upd = 0;
+        if (valid) {
+            const int blk = e >> 2;
+            const int si = blk / jLeft, sj = blk - (blk / jLeft) * jLeft;
+            const int top = si * k + ((q & 2) ? iTop * k : 0);
+            const int left = sj * k + ((q & 1) ? jLeft * k : 0);
+            block_max<KW>(m, rs, cs, D, k, top, left, bv, bi, bj, upd, mn);
+        }
+        // position carried by blocks without an update (FindMaximumMultiPeaks :66-139
+        // keeps maxsI/maxsJ of the quadrant from its previous block)
+        const uint64_t updm = __ballot(valid && upd);
+        const uint64_t prior = updm & qlanes & segmask & ((1ull << lane) - 1ull);
+        const int src = prior ? 63 - __clzll(prior) : lane;
+        const int ci = __shfl(bi, src, 64), cj = __shfl(bj, src, 64);
+        int carryI = staleI[0], carryJ = staleJ[0];
+#pragma unroll
+        for (int qq = 1; qq < 4; qq++)
+            if (q == qq) {
+                carryI = staleI[qq];
+                carryJ = staleJ[qq];
+            }
+        const int pi = upd ? bi : (prior ? ci : carryI);
+        const int pj = upd ? bj : (prior ? cj : carryJ);
+        const float pv = upd ? bv : 0.0f;
+#pragma unroll
+        for (int qq = 0; qq < 4; qq++) {
+            const uint64_t mq = updm & (0x1111111111111111ull << qq) & segmask;
+            const int last = mq ? 63 - __clzll(mq) : lane;
+            const int li = __shfl(bi, last, 64), lj = __shfl(bj, last, 64);
+            if (mq) {
+                staleI[qq] = li;
+                staleJ[qq] = lj;
+            }
+        }
+        // replay the insertions that can change the lists
+        uint64_t pending = __ballot(valid);
+        for (;;) {
+            const int size = L.size;
+            const float lo = L.v[0];
+            const uint64_t cand = __ballot(size < P || pv > lo) & pending;
+            if (!cand) break;
+            const uint64_t mine = cand & segmask;
+            const int c = mine ? __ffsll((unsigned long long)mine) - 1 : lane;
+            const float cv = __shfl(pv, c, 64);
+            const int cI = __shfl(pi, c, 64), cJ = __shfl(pj, c, 64);
+            if (sl == 0 && mine) mp_insert(L, P, (k - 1) * k, cv, (short)cI, (short)cJ);
+            mp_wave_sync();
+            pending &= ~__ballot(mine && lane <= c);
+        }
+    }
+#pragma unroll
+    for (int o = W / 2; o >= 1; o >>= 1) {
+        const float om = __shfl_xor(mn, o, 64);
+        if (om < mn) mn = om;
+    }
+    if (sl == 0 && act) {
+        for (int p = 0; p < P; p++) peaks[3 * p] = peaks[3 * p + 1] = peaks[3 * p + 2] = 0.0f;
+        // descending: peak 0 is the largest
+        for (int e = 0; e < L.size; e++) {
+            const int idx = L.size - 1 - e;
+            peaks[3 * idx + 0] = (float)L.i[e];
+            peaks[3 * idx + 1] = (float)L.j[e];
+            peaks[3 * idx + 2] = L.v[e];
+        }
+        *min_floor = mn;
+        // MaxCrossResult.getTotalPeaks :109-115: the secondary arrays exist
+        // only once a peak index >= 1 was stored
+        if (count) *count = L.size > 1 ? L.size : 1;
+    }
+}
+
+}  // namespace qlpiv
