@@ -105,6 +105,20 @@ int qlp_xcorr_batch(qlp_ctx *ctx, int n, int N, const uint8_t *tilesA, const uin
 int qlp_xcorr_batch_dev(qlp_ctx *ctx, int n, int N, const uint8_t *tilesA, const uint8_t *tilesB,
                         const float *maxA, const float *maxB, int mode, float *planes_out);
 
+/* The same job on float tiles: the MatrixFloat tiles of frames filtered before
+ * warping (images/ImageFloat.java clipImageMatrix :278-384, fed by
+ * GaussianFilter2D.applyFilter(IImage, IImage) :116-126 through
+ * JavaImageFilterJob.compute :50-53), normalised as v / max * 16 + 1 by
+ * Matrix.copyMatrixToArrayAndNormalizeAndOffset like the 8-bit ones.
+ *   tilesA, tilesB : n x N x N floats
+ *   maxA, maxB     : Matrix.getMaxValue() of each tile (qlp_clip_tiles_f32_dev,
+ *                    qlp_warp_micro_tiles_f32_dev)
+ * Exact form only; IA sizes 16, 32, 64 and 128. */
+int qlp_xcorr_f32(qlp_ctx *ctx, int n, int N, const float *tilesA, const float *tilesB,
+                  const float *maxA, const float *maxB, int mode, float *planes_out);
+int qlp_xcorr_f32_dev(qlp_ctx *ctx, int n, int N, const float *tilesA, const float *tilesB,
+                      const float *maxA, const float *maxB, int mode, float *planes_out);
+
 /* Arithmetic form of the correlation.  QLP_XCORR_EXACT repeats the reference's
  * float FFT operations (planes value-identical to the Java job).  The MFMA forms
  * compute the same correlation as dense DFT-basis products on the matrix cores
@@ -169,6 +183,33 @@ int qlp_clip_tiles_dev(qlp_ctx *ctx, const uint8_t *img, int H, int W, int n, in
 int qlp_warp_micro_tiles_dev(qlp_ctx *ctx, const uint8_t *img, int H, int W, int n, int N,
                              const int *top, const int *left, const float *u, const float *v,
                              uint8_t *tiles, float *max_out);
+
+/* Tiles of a float frame (images/ImageFloat.java, e.g. a frame filtered by
+ * GaussianFilter2D :116-126).  ImageFloat.clipImageMatrix :278-384: n windows
+ * of N x N at (top[k], left[k]), or at (round(top[k] + u[k]), round(left[k] +
+ * v[k])) when u and v are given (NoImageWarpingStrategy :80-81), zero outside;
+ * max_out[k] = max over the tile, 0 -> 1 (MatrixFloat.java:45-60,
+ * Matrix.setMaxValue :21-29; no >= 16 clamp). */
+int qlp_clip_tiles_f32_dev(qlp_ctx *ctx, const float *img, int H, int W, int n, int N,
+                           const int *top, const int *left, const float *u, const float *v,
+                           float *tiles, float *max_out);
+
+/* Sides of a Micro tile warp (BiLinearImageMicroAndMiniWarpingStrategy
+ * .warpAndClipImage :63-167): the B tile in SecondImage order, the A tile in
+ * FirstImage order, and the A or B tile in BothImages order. */
+#define QLP_SIDE_B_SECOND 0
+#define QLP_SIDE_A_FIRST 1
+#define QLP_SIDE_A_BOTH 2
+#define QLP_SIDE_B_BOTH 3
+
+/* The Micro warp of one side of n tiles of a float frame, whose B tiles carry
+ * the displacements (u, v) (doBiLinearWarp :292-334 reading ImageFloat
+ * .readPixel :531-533).  Values are the rounded samples (they may exceed 255);
+ * max_out[k] = 65535, the ImageFloat maximum (getMaximumValue :541-543,
+ * doBiLinearWarp :298); max_out may be NULL. */
+int qlp_warp_micro_tiles_f32_dev(qlp_ctx *ctx, const float *img, int H, int W, int n, int N,
+                                 const int *top, const int *left, const float *u, const float *v,
+                                 int side, float *tiles, float *max_out);
 
 /* Bilinear mini warp of the second-image tiles of 'pairs' frames (img: pairs x
  * H x W) over a grid of nI x nJ tiles of N px (BiLinearImageMicroAndMiniWarping
@@ -365,7 +406,15 @@ typedef struct qlp_piv_config {
     int xcorr_form;           /* QLP_XCORR_* arithmetic of the 16, 32 and 64 px levels
                                  (128 px levels always use QLP_XCORR_EXACT) */
     int warp_order;           /* QLP_WARP_ORDER_* of the Micro, Mini and Full warps */
+    int frame_type;           /* QLP_FRAME_U8 (qlp_piv_run_dev) or QLP_FRAME_F32
+                                 (qlp_piv_run_f32_dev) */
 } qlp_piv_config;
+
+/* Pixel type of a plan's frames: 8-bit (images/Image.java) or float
+ * (images/ImageFloat.java, what GaussianFilter2D.applyFilter(IImage, IImage)
+ * :116-126 writes when frames are filtered before warping). */
+#define QLP_FRAME_U8 0
+#define QLP_FRAME_F32 1
 
 typedef struct qlp_piv_plan qlp_piv_plan;
 
@@ -382,6 +431,12 @@ int qlp_piv_plan_level(qlp_piv_plan *plan, int level, int *ia, int *nI, int *nJ,
 /* Run every level for the frames imgsA, imgsB (device, pairs x H x W uint8),
  * asynchronously on the context's stream. */
 int qlp_piv_run_dev(qlp_piv_plan *plan, const uint8_t *imgsA, const uint8_t *imgsB);
+/* The same loop for float frames (plan created with frame_type QLP_FRAME_F32;
+ * imgsA, imgsB device, pairs x H x W floats): every level cuts float tiles
+ * (ImageFloat.clipImageMatrix :278-384; the Micro warp of the order's sides,
+ * maximum 65535) and correlates them in the exact form.  Warp modes
+ * QLP_WARP_NONE and QLP_WARP_MICRO (every order). */
+int qlp_piv_run_f32_dev(qlp_piv_plan *plan, const float *imgsA, const float *imgsB);
 /* Device pointers of a level's results (valid after the run has completed):
  * u, v (pairs x n displacements, vertical then horizontal), peaks (pairs x n x
  * n_peaks x 4: i', j', value', status), planes (pairs x n x D x D when stored

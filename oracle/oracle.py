@@ -57,6 +57,11 @@ def lib():
         L.or_poly.argtypes = [_f32p, c_int, c_int, _f32p, _f32p]
         L.or_clip_tile.argtypes = [_u8p, c_int, c_int, c_int, c_int, c_int, _u8p]
         L.or_clip_tile.restype = c_float
+        L.or_clip_tile_f32.argtypes = [_f32p, c_int, c_int, c_int, c_int, c_int, _f32p]
+        L.or_clip_tile_f32.restype = c_float
+        L.or_warp_micro_side_f32.argtypes = [_f32p, c_int, c_int, c_int, c_int, c_int, c_float,
+                                             c_float, c_int, _f32p]
+        L.or_warp_micro_side_f32.restype = c_float
         L.or_warp_micro.argtypes = [_u8p, c_int, c_int, c_int, c_int, c_int, c_float, c_float, _u8p]
         L.or_warp_micro_side.argtypes = [_u8p, c_int, c_int, c_int, c_int, c_int, c_float, c_float,
                                          c_int, _u8p]
@@ -319,6 +324,24 @@ def warp_micro(img, top, left, N, du, dv, side=SIDE_B_SECOND):
     lib().or_warp_micro_side(img, img.shape[0], img.shape[1], int(top), int(left), N, float(du),
                              float(dv), int(side), tile)
     return tile
+
+
+def clip_tile_f32(img, top, left, N):
+    """N x N window of a float frame, zero outside; returns (tile, max), the
+    max over the tile with 0 -> 1 (no >= 16 clamp for float tiles)."""
+    img = np.ascontiguousarray(img, np.float32)
+    tile = np.zeros((N, N), np.float32)
+    mx = lib().or_clip_tile_f32(img, img.shape[0], img.shape[1], int(top), int(left), N, tile)
+    return tile, mx
+
+
+def warp_micro_f32(img, top, left, N, du, dv, side=SIDE_B_SECOND):
+    """Micro warp of a float frame's tile; returns (tile, 65535)."""
+    img = np.ascontiguousarray(img, np.float32)
+    tile = np.zeros((N, N), np.float32)
+    mx = lib().or_warp_micro_side_f32(img, img.shape[0], img.shape[1], int(top), int(left), N,
+                                      float(du), float(dv), int(side), tile)
+    return tile, mx
 
 
 def natural_slopes(x, y):

@@ -1100,7 +1100,25 @@ static int fm_round(float x) { return (int)floorf(x + 0.5f); }
 /* BiLinearImageMicroAndMiniWarpingStrategy.warpPixelOnImage :336-407: the
  * bilinear sample at source = target + displacement (SecondImage order) or
  * target - displacement (FirstImage order, first != 0), round-half-up. */
+/* A frame of 8-bit pixels (Image) or of floats (ImageFloat): readPixel. */
+typedef struct {
+    const uint8_t *u8;
+    const float *f32;
+} or_frame;
+
+static float frame_px(const or_frame *fr, size_t k) { return fr->f32 ? fr->f32[k] : (float)fr->u8[k]; }
+
+static float warp_pixel_frame(const or_frame *img, int H, int W, float u, float v, short i, short j,
+                              int first);
+
 static float warp_pixel_order(const uint8_t *img, int H, int W, float u, float v, short i, short j,
+                              int first)
+{
+    const or_frame fr = {img, NULL};
+    return warp_pixel_frame(&fr, H, W, u, v, i, j, first);
+}
+
+static float warp_pixel_frame(const or_frame *img, int H, int W, float u, float v, short i, short j,
                               int first)
 {
     float sI = i + u, sJ = j + v;
@@ -1118,10 +1136,10 @@ static float warp_pixel_order(const uint8_t *img, int H, int W, float u, float v
     if (oI + pI < 0 || oI + pI >= H) oI = 0;
     if (oJ + pJ < 0 || oJ + pJ >= W) oJ = 0;
     float wU = fabsf(dU), wV = fabsf(dV);
-    float tl = img[(size_t)pI * W + pJ];
-    float tr = img[(size_t)pI * W + pJ + oJ];
-    float bl = img[(size_t)(pI + oI) * W + pJ];
-    float br = img[(size_t)(pI + oI) * W + pJ + oJ];
+    float tl = frame_px(img, (size_t)pI * W + pJ);
+    float tr = frame_px(img, (size_t)pI * W + pJ + oJ);
+    float bl = frame_px(img, (size_t)(pI + oI) * W + pJ);
+    float br = frame_px(img, (size_t)(pI + oI) * W + pJ + oJ);
     float f = (1.0f - wU) * (1.0f - wV) * tl + (1.0f - wU) * wV * tr + wU * (1.0f - wV) * bl +
               wU * wV * br;
     return (float)fm_round(f);
@@ -1204,6 +1222,50 @@ void or_warp_micro_side(const uint8_t *img, int H, int W, int top, int left, int
     const float fv = dispV - (float)floor(dispV);
     for (int i = 0; i < N; i++)
         for (int j = 0; j < N; j++) tile[i * N + j] = side_pixel(img, H, W, &g, fu, fv, i, j);
+}
+
+/* ImageFloat.clipImageMatrix :278-384 (new matrix, partial clipping allowed)
+ * with the MatrixFloat maximum over the tile (MatrixFloat.java:45-60,
+ * :375-400; a maximum of 0 becomes 1, Matrix.setMaxValue :21-29).  Returns the
+ * tile max. */
+float or_clip_tile_f32(const float *img, int H, int W, int top, int left, int N, float *tile)
+{
+    float maxv = 0.0f;
+    for (int i = 0; i < N; i++)
+        for (int j = 0; j < N; j++) {
+            int y = top + i, x = left + j;
+            float v = 0.0f;
+            if (y >= 0 && y < H && x >= 0 && x < W) {
+                v = img[(size_t)y * W + x];
+                if (v > maxv) maxv = v;
+            }
+            tile[i * N + j] = v;
+        }
+    return maxv == 0.0f ? 1.0f : maxv;
+}
+
+/* Micro warp of one tile of a float frame (the same strategy as
+ * or_warp_micro_side, reading ImageFloat.readPixel :531-533).  Values are the
+ * rounded samples; the reference gives the tile the image maximum 65535
+ * (ImageFloat.getMaximumValue :541-543, doBiLinearWarp :298), returned here. */
+float or_warp_micro_side_f32(const float *img, int H, int W, int top, int left, int N, float dispU,
+                             float dispV, int side, float *tile)
+{
+    const or_frame fr = {NULL, img};
+    const or_side g = side_geometry(side, top, left, dispU, dispV);
+    const float fu = dispU - (float)floor(dispU);
+    const float fv = dispV - (float)floor(dispV);
+    for (int i = 0; i < N; i++)
+        for (int j = 0; j < N; j++) {
+            float u = fu, v = fv;
+            if (g.half) {
+                u = u / 2.0f + g.offU;
+                v = v / 2.0f + g.offV;
+            }
+            tile[i * N + j] = warp_pixel_frame(&fr, H, W, u, v, (short)(i + g.top), (short)(j + g.left),
+                                               g.first);
+        }
+    return 65535.0f;
 }
 
 /* SecondImage order: the B tile shifted by floor(u), floor(v), sampled at the

@@ -192,6 +192,60 @@ int qlp_xcorr_batch_dev(qlp_ctx *h, int n, int N, const uint8_t *tA, const uint8
     return launch_xcorr(&h->c, n, N, tA, tB, maxA, maxB, mode, planes);
 }
 
+int qlp_xcorr_f32_dev(qlp_ctx *h, int n, int N, const float *tA, const float *tB, const float *maxA,
+                      const float *maxB, int mode, float *planes)
+{
+    if (!h) return fail(QLP_ERR_ARG, "null context");
+    int rc = xcorr_args(n, N, mode);
+    if (rc || n == 0) return rc;
+    if (!tA || !tB || !maxA || !planes || (mode == QLP_NORM_PER_TILE && !maxB))
+        return fail(QLP_ERR_ARG, "xcorr: null buffer");
+    PassParams p;
+    p.n = n;
+    p.N = N;
+    p.mode = mode;
+    p.tAf = tA;
+    p.tBf = tB;
+    p.maxA = maxA;
+    p.maxB = maxB ? maxB : maxA;
+    p.planes = planes;
+    return launch_pass(&h->c, p);
+}
+
+int qlp_xcorr_f32(qlp_ctx *h, int n, int N, const float *tA, const float *tB, const float *maxA,
+                  const float *maxB, int mode, float *planes)
+{
+    if (!h) return fail(QLP_ERR_ARG, "null context");
+    int rc = xcorr_args(n, N, mode);
+    if (rc || n == 0) return rc;
+    if (!tA || !tB || !maxA || !planes || (mode == QLP_NORM_PER_TILE && !maxB))
+        return fail(QLP_ERR_ARG, "xcorr: null buffer");
+    Context *ctx = &h->c;
+    const int D = 2 * N - 1;
+    const size_t tbytes = sizeof(float) * (size_t)n * N * N;
+    const size_t bt = align256(tbytes), bm = align256(sizeof(float) * n),
+                 bp = align256(sizeof(float) * (size_t)n * D * D);
+    rc = ensure_scratch(ctx, 2 * bt + 2 * bm + bp);
+    if (rc) return rc;
+    char *base = (char *)ctx->scratch;
+    float *dA = (float *)base, *dB = (float *)(base + bt);
+    float *dmA = (float *)(base + 2 * bt), *dmB = (float *)(base + 2 * bt + bm);
+    float *dP = (float *)(base + 2 * bt + 2 * bm);
+    hipStream_t s = ctx->stream;
+    if ((rc = check_hip(hipMemcpyAsync(dA, tA, tbytes, hipMemcpyHostToDevice, s), "copy A")) ||
+        (rc = check_hip(hipMemcpyAsync(dB, tB, tbytes, hipMemcpyHostToDevice, s), "copy B")) ||
+        (rc = check_hip(hipMemcpyAsync(dmA, maxA, sizeof(float) * n, hipMemcpyHostToDevice, s), "copy maxA")))
+        return rc;
+    if (maxB && (rc = check_hip(hipMemcpyAsync(dmB, maxB, sizeof(float) * n, hipMemcpyHostToDevice, s), "copy maxB")))
+        return rc;
+    rc = qlp_xcorr_f32_dev(h, n, N, dA, dB, dmA, maxB ? dmB : dmA, mode, dP);
+    if (rc) return rc;
+    rc = check_hip(hipMemcpyAsync(planes, dP, sizeof(float) * (size_t)n * D * D, hipMemcpyDeviceToHost, s),
+                   "copy planes");
+    if (rc) return rc;
+    return sync_ctx(ctx);
+}
+
 int qlp_xcorr_form_dev(qlp_ctx *h, int form, int n, int N, const uint8_t *tA, const uint8_t *tB,
                        const float *maxA, const float *maxB, int mode, float *planes)
 {
@@ -339,6 +393,27 @@ int qlp_warp_micro_tiles_dev(qlp_ctx *h, const uint8_t *img, int H, int W, int n
     if (!h) return fail(QLP_ERR_ARG, "null context");
     if (n < 0 || N < 1 || H < 1 || W < 1) return fail(QLP_ERR_ARG, "warp: bad sizes");
     return launch_warp_micro(&h->c, img, H, W, n, N, top, left, u, v, tiles, max_out);
+}
+
+int qlp_clip_tiles_f32_dev(qlp_ctx *h, const float *img, int H, int W, int n, int N, const int *top,
+                           const int *left, const float *u, const float *v, float *tiles,
+                           float *max_out)
+{
+    if (!h) return fail(QLP_ERR_ARG, "null context");
+    if (n < 0 || N < 1 || H < 1 || W < 1) return fail(QLP_ERR_ARG, "clip: bad sizes");
+    if (n > 0 && (!img || !top || !left || !tiles || !max_out)) return fail(QLP_ERR_ARG, "clip: null buffer");
+    return launch_clip_f32(&h->c, img, H, W, n, N, top, left, u, v, tiles, max_out);
+}
+
+int qlp_warp_micro_tiles_f32_dev(qlp_ctx *h, const float *img, int H, int W, int n, int N,
+                                 const int *top, const int *left, const float *u, const float *v,
+                                 int side, float *tiles, float *max_out)
+{
+    if (!h) return fail(QLP_ERR_ARG, "null context");
+    if (n < 0 || N < 1 || H < 1 || W < 1) return fail(QLP_ERR_ARG, "warp: bad sizes");
+    if (side < QLP_SIDE_B_SECOND || side > QLP_SIDE_B_BOTH) return fail(QLP_ERR_ARG, "warp: unknown side");
+    if (n > 0 && (!img || !top || !left || !u || !v || !tiles)) return fail(QLP_ERR_ARG, "warp: null buffer");
+    return launch_warp_micro_f32(&h->c, img, H, W, n, N, top, left, u, v, tiles, max_out, side);
 }
 
 int qlp_xcorr_images_form_dev(qlp_ctx *h, int form, int n, int N, int mode, const uint8_t *imgA,

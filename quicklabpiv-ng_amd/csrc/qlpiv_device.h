@@ -14,7 +14,8 @@ __device__ __forceinline__ int fm_round(float x) { return (int)floorf(x + 0.5f);
 // the image (which zeroes the fractional offset there), neighbour on the side
 // of the offset, bilinear weights evaluated left to right in float, value
 // rounded half up.
-__device__ __forceinline__ float warp_pixel(const uint8_t *__restrict__ img, int H, int W, float u,
+template <typename T>
+__device__ __forceinline__ float warp_pixel(const T *__restrict__ img, int H, int W, float u,
                                             float v, int i, int j)
 {
     float sI = (float)i + u, sJ = (float)j + v;
@@ -93,7 +94,8 @@ __device__ __forceinline__ SideGeom side_geometry(int side, int top, int left, f
 // Pixel (i, j) of a side's window sampled with the field value (u, v) there
 // (doBiLinearWarp :292-334).  FirstImage samples at target - field, which is
 // target + (-field) exactly in IEEE arithmetic.
-__device__ __forceinline__ float side_sample(const uint8_t *__restrict__ img, int H, int W,
+template <typename T>
+__device__ __forceinline__ float side_sample(const T *__restrict__ img, int H, int W,
                                              const SideGeom &g, float u, float v, int i, int j)
 {
     if (g.half) {

@@ -98,6 +98,7 @@ enum PassShape {
 struct PassParams {
     int n = 0, N = 0, mode = QLP_NORM_PER_TILE;
     const uint8_t *tA = nullptr, *tB = nullptr;
+    const float *tAf = nullptr, *tBf = nullptr;  // float tiles (MatrixFloat), in place of tA / tB
     const float *maxA = nullptr, *maxB = nullptr;
     const uint8_t *imgA = nullptr, *imgB = nullptr;
     int H = 0, W = 0;
@@ -147,6 +148,14 @@ int launch_clip(Context *ctx, const uint8_t *img, int H, int W, int n, int N, co
 int launch_warp_micro(Context *ctx, const uint8_t *img, int H, int W, int n, int N, const int *top,
                       const int *left, const float *u, const float *v, uint8_t *tiles,
                       float *maxv, int side = 0);
+// Tiles of float frames (ImageFloat): a clip (at (top, left), or shifted by the
+// rounded (u, v) when u and v are given) with the MatrixFloat maximum, and the
+// Micro warp of one side with the maximum 65535.
+int launch_clip_f32(Context *ctx, const float *img, int H, int W, int n, int N, const int *top,
+                    const int *left, const float *du, const float *dv, float *tiles, float *maxv);
+int launch_warp_micro_f32(Context *ctx, const float *img, int H, int W, int n, int N, const int *top,
+                          const int *left, const float *u, const float *v, float *tiles, float *maxv,
+                          int side = 0);
 // Spline-field warps (qlpiv_splinewarp.hip).
 int launch_warp_mini(Context *ctx, int pairs, const uint8_t *imgs, int H, int W, int nI, int nJ,
                      int N, const int *top, const int *left, const float *u, const float *v,

@@ -152,6 +152,9 @@ struct qlp_piv_plan {
     float *mA = nullptr, *mB = nullptr;
     void *fwork = nullptr;
     size_t tiles_bytes = 0, fwork_bytes = 0;
+    // float frames: the A and B float tiles of every level (sized for the largest)
+    float *fA = nullptr, *fB = nullptr;
+    size_t ftiles_bytes = 0;
     // IAs of the largest IA-128 pass: sizes the context's tile-slot workspace
     int big_ias = 0;
     // optional per-level timing: events before the state kernel, after the
@@ -186,10 +189,10 @@ static void plan_free(qlp_piv_plan *p)
             if (b) (void)hipFree(b);
     }
     for (void *b : {(void *)p->tA, (void *)p->tB, (void *)p->wimg, (void *)p->mA, (void *)p->mB,
-                    p->fwork})
+                    p->fwork, (void *)p->fA, (void *)p->fB})
         if (b) (void)hipFree(b);
     p->tA = p->tB = p->wimg = nullptr;
-    p->mA = p->mB = nullptr;
+    p->mA = p->mB = p->fA = p->fB = nullptr;
     p->fwork = nullptr;
     for (hipEvent_t e : p->ev) (void)hipEventDestroy(e);
     p->ev.clear();
@@ -292,6 +295,13 @@ static int plan_layout(int H, int W, int pairs, const qlp_piv_config *cfg, qlp_p
         return fail(QLP_ERR_ARG, "piv_plan_create: unknown velocity inheritance strategy");
     if (cfg->xcorr_form < QLP_XCORR_EXACT || cfg->xcorr_form > QLP_XCORR_MFMA_BF16_SINGLE)
         return fail(QLP_ERR_ARG, "piv_plan_create: unknown correlation form");
+    if (cfg->frame_type != QLP_FRAME_U8 && cfg->frame_type != QLP_FRAME_F32)
+        return fail(QLP_ERR_ARG, "piv_plan_create: unknown frame type");
+    const bool f32 = cfg->frame_type == QLP_FRAME_F32;
+    if (f32 && cfg->warp_mode != QLP_WARP_NONE && cfg->warp_mode != QLP_WARP_MICRO)
+        return fail(QLP_ERR_UNSUPPORTED, "piv_plan_create: float frames take no warp or the Micro warp");
+    if (f32 && cfg->xcorr_form != QLP_XCORR_EXACT)
+        return fail(QLP_ERR_UNSUPPORTED, "piv_plan_create: float frames take the exact correlation form");
     const bool valid = cfg->valid.n_stages > 0;
     p->H = H;
     p->W = W;
@@ -315,16 +325,17 @@ static int plan_layout(int H, int W, int pairs, const qlp_piv_config *cfg, qlp_p
             return fail(QLP_ERR_UNSUPPORTED,
                         "piv_plan_create: spline warps need 2 tiles in each direction");
         if (tile == 128) p->big_ias = std::max(p->big_ias, (int)pn);
-        if (l > 0 && tiles_route(*cfg))
+        if (l > 0 && tiles_route(*cfg) && !f32)
             p->tiles_bytes = std::max(p->tiles_bytes, pn * (size_t)tile * tile);
+        if (f32) p->ftiles_bytes = std::max(p->ftiles_bytes, pn * (size_t)tile * tile * sizeof(float));
         if (l > 0 && cfg->warp_mode == QLP_WARP_FULL)
             p->fwork_bytes = std::max(p->fwork_bytes, full_warp_work_bytes(pairs, L.nI, L.nJ));
         p->levels.push_back(std::move(L));
     }
-    if (tiles_route(*cfg) && p->tiles_bytes) {
+    if ((tiles_route(*cfg) && p->tiles_bytes) || f32) {
         size_t mx = 0;
         for (const Level &L : p->levels) mx = std::max(mx, (size_t)pairs * L.n);
-        need += 2 * p->tiles_bytes + 2 * mx * 4;
+        need += 2 * p->tiles_bytes + 2 * p->ftiles_bytes + 2 * mx * 4;
     }
     if (cfg->warp_mode == QLP_WARP_FULL && p->fwork_bytes)
         need += (size_t)full_frames(*cfg) * pairs * H * W + p->fwork_bytes;
@@ -399,11 +410,18 @@ int qlp_piv_plan_create(qlp_ctx *h, int H, int W, int pairs, const qlp_piv_confi
         if (!rc && l > 0 && cfg->inherit_mode != QLP_INHERIT_DIRECT)
             rc = prepare_inherit(cfg->inherit_mode, p->levels[l - 1], L, need);
     }
-    if (!rc && tiles_route(*cfg) && p->tiles_bytes) {
+    const bool f32 = cfg->frame_type == QLP_FRAME_F32;
+    if (!rc && ((tiles_route(*cfg) && p->tiles_bytes) || f32)) {
         size_t mx = 0;
         for (const Level &L : p->levels) mx = std::max(mx, (size_t)pairs * L.n);
-        rc = check_hip(hipMalloc(&p->tA, p->tiles_bytes), "hipMalloc plan");
-        if (!rc) rc = check_hip(hipMalloc(&p->tB, p->tiles_bytes), "hipMalloc plan");
+        if (p->tiles_bytes) {
+            rc = check_hip(hipMalloc(&p->tA, p->tiles_bytes), "hipMalloc plan");
+            if (!rc) rc = check_hip(hipMalloc(&p->tB, p->tiles_bytes), "hipMalloc plan");
+        }
+        if (!rc && f32) {
+            rc = check_hip(hipMalloc(&p->fA, p->ftiles_bytes), "hipMalloc plan");
+            if (!rc) rc = check_hip(hipMalloc(&p->fB, p->ftiles_bytes), "hipMalloc plan");
+        }
         if (!rc) rc = check_hip(hipMalloc(&p->mA, mx * 4), "hipMalloc plan");
         if (!rc) rc = check_hip(hipMalloc(&p->mB, mx * 4), "hipMalloc plan");
     }
@@ -412,7 +430,7 @@ int qlp_piv_plan_create(qlp_ctx *h, int H, int W, int pairs, const qlp_piv_confi
         if (!rc) rc = check_hip(hipMalloc(&p->fwork, p->fwork_bytes), "hipMalloc plan");
     }
     // the plan's IA-128 passes read the images and find peaks
-    if (!rc && p->big_ias) rc = prepare_big(ctx, p->big_ias, cfg->norm_mode, true, true, SHAPE_AUTO);
+    if (!rc && p->big_ias) rc = prepare_big(ctx, p->big_ias, cfg->norm_mode, !f32, true, SHAPE_AUTO);
     p->bytes = need;
     ctx->hbm_in_use += need;
     if (rc) {
@@ -455,9 +473,62 @@ int qlp_piv_plan_level(qlp_piv_plan *p, int level, int *ia, int *nI, int *nJ, in
     return QLP_OK;
 }
 
+// Float tiles of one level of a float-frame plan (ImageFloat frames): A and B
+// clipped (B shifted by the rounded displacement with no warp), or the Micro
+// warp of the order's sides, for every pair.
+static int float_tiles(qlp_piv_plan *p, const Level &L, size_t l, const float *imgsA,
+                       const float *imgsB)
+{
+    Context *ctx = p->ctx;
+    const qlp_piv_config &c = p->cfg;
+    const int order = c.warp_order;
+    const bool micro = l > 0 && c.warp_mode == QLP_WARP_MICRO;
+    const size_t tN = (size_t)L.N * L.N, frame = (size_t)p->H * p->W;
+    int rc = QLP_OK;
+    for (int k = 0; k < p->pairs && !rc; k++) {
+        const float *a = imgsA + k * frame, *b = imgsB + k * frame;
+        const size_t o = (size_t)k * L.n;
+        float *ta = p->fA + o * tN, *tb = p->fB + o * tN, *ma = p->mA + o, *mb = p->mB + o;
+        const float *u = L.u + o, *v = L.v + o;
+        if (micro && order != QLP_WARP_ORDER_SECOND)
+            rc = launch_warp_micro_f32(ctx, a, p->H, p->W, L.n, L.N, L.top, L.left, u, v, ta, ma,
+                                       order == QLP_WARP_ORDER_FIRST ? SIDE_A_FIRST : SIDE_A_BOTH);
+        else
+            rc = launch_clip_f32(ctx, a, p->H, p->W, L.n, L.N, L.top, L.left, nullptr, nullptr, ta, ma);
+        if (rc) break;
+        if (micro && order != QLP_WARP_ORDER_FIRST)
+            rc = launch_warp_micro_f32(ctx, b, p->H, p->W, L.n, L.N, L.top, L.left, u, v, tb, mb,
+                                       order == QLP_WARP_ORDER_SECOND ? SIDE_B_SECOND : SIDE_B_BOTH);
+        else if (l > 0 && c.warp_mode == QLP_WARP_NONE)
+            rc = launch_clip_f32(ctx, b, p->H, p->W, L.n, L.N, L.top, L.left, u, v, tb, mb);
+        else
+            rc = launch_clip_f32(ctx, b, p->H, p->W, L.n, L.N, L.top, L.left, nullptr, nullptr, tb, mb);
+    }
+    return rc;
+}
+
+static int run_levels(qlp_piv_plan *p, const uint8_t *imgsA, const uint8_t *imgsB,
+                      const float *fimgsA, const float *fimgsB);
+
 int qlp_piv_run_dev(qlp_piv_plan *p, const uint8_t *imgsA, const uint8_t *imgsB)
 {
     if (!p || !imgsA || !imgsB) return fail(QLP_ERR_ARG, "piv_run: null argument");
+    if (p->cfg.frame_type != QLP_FRAME_U8)
+        return fail(QLP_ERR_ARG, "piv_run: the plan was made for float frames (qlp_piv_run_f32_dev)");
+    return run_levels(p, imgsA, imgsB, nullptr, nullptr);
+}
+
+int qlp_piv_run_f32_dev(qlp_piv_plan *p, const float *imgsA, const float *imgsB)
+{
+    if (!p || !imgsA || !imgsB) return fail(QLP_ERR_ARG, "piv_run: null argument");
+    if (p->cfg.frame_type != QLP_FRAME_F32)
+        return fail(QLP_ERR_ARG, "piv_run: the plan was made for 8-bit frames (qlp_piv_run_dev)");
+    return run_levels(p, nullptr, nullptr, imgsA, imgsB);
+}
+
+static int run_levels(qlp_piv_plan *p, const uint8_t *imgsA, const uint8_t *imgsB,
+                      const float *fimgsA, const float *fimgsB)
+{
     Context *ctx = p->ctx;
     const qlp_piv_config &c = p->cfg;
     const bool multi = c.peak_kind == QLP_PEAK_MULTI;
@@ -502,7 +573,16 @@ int qlp_piv_run_dev(qlp_piv_plan *p, const uint8_t *imgsA, const uint8_t *imgsB)
         q.dv = (l == 0) ? nullptr : L.v;
         q.bmode = c.warp_mode == QLP_WARP_MICRO ? B_WARP_MICRO : B_CLIP_SHIFT;
         const int order = c.warp_order;
-        if (l > 0 && c.warp_mode == QLP_WARP_FULL) {
+        if (fimgsA) {
+            rc = float_tiles(p, L, l, fimgsA, fimgsB);
+            if (rc) return rc;
+            q.imgA = q.imgB = nullptr;
+            q.tAf = p->fA;
+            q.tBf = p->fB;
+            q.maxA = p->mA;
+            q.maxB = p->mB;
+            q.du = q.dv = nullptr;
+        } else if (l > 0 && c.warp_mode == QLP_WARP_FULL) {
             // warp the whole frames of the order (A, B or both) with the grid's field
             const size_t frames = (size_t)p->pairs * p->H * p->W;
             uint8_t *wA = p->wimg, *wB = p->wimg + (order == QLP_WARP_ORDER_BOTH ? frames : 0);

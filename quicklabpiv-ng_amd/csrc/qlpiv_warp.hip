@@ -75,6 +75,100 @@ __global__ __launch_bounds__(256) void warp_micro_kernel(const uint8_t *__restri
     if (threadIdx.x == 0 && maxv) maxv[t] = 255.0f;
 }
 
+// Float frames (images/ImageFloat.java: the frames the Gaussian pre-filter
+// writes, GaussianFilter2D.applyFilter(IImage, IImage) :116-126).
+// CLIP_F32 (ImageFloat.clipImageMatrix :278-384): the N x N window, zero
+// outside, at (top, left), or with a displacement at (round(top + u),
+// round(left + v)) (NoImageWarpingStrategy :80-81); the tile maximum is the
+// MatrixFloat one, over the tile with 0 -> 1 (MatrixFloat.java:45-60,
+// Matrix.setMaxValue :21-29; no >= 16 clamp).
+__global__ __launch_bounds__(256) void clip_f32_kernel(const float *__restrict__ img, int H, int W,
+                                                      int n, int N, const int *__restrict__ top,
+                                                      const int *__restrict__ left,
+                                                      const float *__restrict__ du,
+                                                      const float *__restrict__ dv,
+                                                      float *__restrict__ tiles,
+                                                      float *__restrict__ maxv)
+{
+    __shared__ float s_max[4];
+    const int t = blockIdx.x;
+    if (t >= n) return;
+    int y0 = top[t], x0 = left[t];
+    if (du) {
+        y0 = fm_round((float)y0 + du[t]);
+        x0 = fm_round((float)x0 + dv[t]);
+    }
+    float mx = 0.0f;
+    for (int idx = threadIdx.x; idx < N * N; idx += blockDim.x) {
+        const int i = idx / N, j = idx - (idx / N) * N;
+        const int y = y0 + i, x = x0 + j;
+        float v = 0.0f;
+        if (y >= 0 && y < H && x >= 0 && x < W) {
+            v = img[(size_t)y * W + x];
+            if (v > mx) mx = v;
+        }
+        tiles[(size_t)t * N * N + idx] = v;
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        const float om = __shfl_xor(mx, o, 64);
+        if (om > mx) mx = om;
+    }
+    if ((threadIdx.x & 63) == 0) s_max[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float m = 0.0f;
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++)
+            if (s_max[w] > m) m = s_max[w];
+        maxv[t] = m == 0.0f ? 1.0f : m;
+    }
+}
+
+// WARP_MICRO_F32: the Micro warp of one side of a float frame's tile, sampled
+// as for 8-bit frames (readPixel of ImageFloat :531-533); the rounded values
+// can exceed 255, and the tile carries the ImageFloat maximum 65535
+// (getMaximumValue :541-543, doBiLinearWarp :298).
+__global__ __launch_bounds__(256) void warp_micro_f32_kernel(const float *__restrict__ img, int H,
+                                                            int W, int n, int N,
+                                                            const int *__restrict__ top,
+                                                            const int *__restrict__ left,
+                                                            const float *__restrict__ du,
+                                                            const float *__restrict__ dv, int side,
+                                                            float *__restrict__ tiles,
+                                                            float *__restrict__ maxv)
+{
+    const int t = blockIdx.x;
+    if (t >= n) return;
+    const float u = du[t], v = dv[t];
+    const SideGeom g = side_geometry(side, top[t], left[t], u, v);
+    const float ru = u - floorf(u), rv = v - floorf(v);
+    for (int idx = threadIdx.x; idx < N * N; idx += blockDim.x) {
+        const int i = idx / N, j = idx - (idx / N) * N;
+        tiles[(size_t)t * N * N + idx] = side_sample(img, H, W, g, ru, rv, i, j);
+    }
+    if (threadIdx.x == 0 && maxv) maxv[t] = 65535.0f;
+}
+
+int launch_clip_f32(Context *ctx, const float *img, int H, int W, int n, int N, const int *top,
+                    const int *left, const float *du, const float *dv, float *tiles, float *maxv)
+{
+    if (n == 0) return QLP_OK;
+    if ((du == nullptr) != (dv == nullptr)) return fail(QLP_ERR_ARG, "clip: u and v go together");
+    hipLaunchKernelGGL(clip_f32_kernel, dim3(n), dim3(256), 0, ctx->stream, img, H, W, n, N, top, left,
+                       du, dv, tiles, maxv);
+    return check_hip(hipGetLastError(), "float clip kernel launch");
+}
+
+int launch_warp_micro_f32(Context *ctx, const float *img, int H, int W, int n, int N, const int *top,
+                          const int *left, const float *u, const float *v, float *tiles, float *maxv,
+                          int side)
+{
+    if (n == 0) return QLP_OK;
+    hipLaunchKernelGGL(warp_micro_f32_kernel, dim3(n), dim3(256), 0, ctx->stream, img, H, W, n, N,
+                       top, left, u, v, side, tiles, maxv);
+    return check_hip(hipGetLastError(), "float warp kernel launch");
+}
+
 int launch_clip(Context *ctx, const uint8_t *img, int H, int W, int n, int N, const int *top,
                 const int *left, uint8_t *tiles, float *maxv)
 {

@@ -322,12 +322,13 @@ __global__ __launch_bounds__((XcShape<LOGM, NT_>::THREADS)) void xcorr_pass_kern
     } else {
         mA = p.maxA[ia];
         mB = MODE_CPU ? mA : p.maxB[ia];
-        const uint8_t *a8 = p.tA + (size_t)ia * N * N, *b8 = p.tB + (size_t)ia * N * N;
+        // 8-bit tiles (MatrixByte) or float tiles (MatrixFloat)
+        const size_t at = (size_t)ia * N * N;
 #pragma unroll
         for (int k = 0; k < PIX; k++) {
             const int idx = threadIdx.x + k * NT;
-            va[k] = (float)a8[idx];
-            vb[k] = (float)b8[idx];
+            va[k] = p.tAf ? p.tAf[at + idx] : (float)p.tA[at + idx];
+            vb[k] = p.tBf ? p.tBf[at + idx] : (float)p.tB[at + idx];
         }
     }
 #pragma unroll
@@ -485,16 +486,19 @@ __global__ __launch_bounds__(NT) void xcorr_big_kernel(PassParams p,
             a8 = p.tA + (size_t)ia * N * N;
             b8 = p.tB + (size_t)ia * N * N;
         }
-        auto pixA = [&](int r, int c) -> int {
-            if (!FROM_IMAGE) return a8[r * N + c];
+        // 8-bit tiles (MatrixByte) or float tiles (MatrixFloat)
+        const float *af = p.tAf ? p.tAf + (size_t)ia * N * N : nullptr;
+        const float *bf = p.tBf ? p.tBf + (size_t)ia * N * N : nullptr;
+        auto pixA = [&](int r, int c) -> float {
+            if (!FROM_IMAGE) return af ? af[r * N + c] : (float)a8[r * N + c];
             const int y = y0 + r, x = x0 + c;
-            return (y >= 0 && y < p.H && x >= 0 && x < p.W) ? a8[(size_t)y * p.W + x] : 0;
+            return (y >= 0 && y < p.H && x >= 0 && x < p.W) ? (float)a8[(size_t)y * p.W + x] : 0.0f;
         };
-        auto pixB = [&](int r, int c) -> int {
-            if (!FROM_IMAGE) return b8[r * N + c];
-            if (warp) return (int)warp_pixel(b8, p.H, p.W, fu, fv, by0 + r, bx0 + c);
+        auto pixB = [&](int r, int c) -> float {
+            if (!FROM_IMAGE) return bf ? bf[r * N + c] : (float)b8[r * N + c];
+            if (warp) return warp_pixel(b8, p.H, p.W, fu, fv, by0 + r, bx0 + c);
             const int y = by0 + r, x = bx0 + c;
-            return (y >= 0 && y < p.H && x >= 0 && x < p.W) ? b8[(size_t)y * p.W + x] : 0;
+            return (y >= 0 && y < p.H && x >= 0 && x < p.W) ? (float)b8[(size_t)y * p.W + x] : 0.0f;
         };
         if (FROM_IMAGE) {
             const bool b255 = warp || p.bmax255;
@@ -503,8 +507,8 @@ __global__ __launch_bounds__(NT) void xcorr_big_kernel(PassParams p,
             for (int q = 0; q < IT_MAX; q++) {
                 const int idx = tid + q * NT;
                 const int r = idx / N, c = idx - (idx / N) * N;
-                amax = max(amax, pixA(r, c));
-                if (!b255) bmax = max(bmax, pixB(r, c));
+                amax = max(amax, (int)pixA(r, c));
+                if (!b255) bmax = max(bmax, (int)pixB(r, c));
             }
             amax = block_max_int<NT>(amax, s_red);
             mA = (float)(amax < 16 ? 16 : amax);
@@ -523,7 +527,7 @@ __global__ __launch_bounds__(NT) void xcorr_big_kernel(PassParams p,
         // ---- forward rows: the N data rows, F'[r][c] = A[N-1-r][N-1-c] -------
         for (int rb = 0; rb < N; rb += BL) {
             const int tid = opaque_tid();
-            int va[IT_HALF], vb[IT_HALF];
+            float va[IT_HALF], vb[IT_HALF];
 #pragma unroll
             for (int q = 0; q < IT_HALF; q++) {
                 const int idx = tid + q * NT;
@@ -535,8 +539,8 @@ __global__ __launch_bounds__(NT) void xcorr_big_kernel(PassParams p,
             for (int q = 0; q < IT_HALF; q++) {
                 const int idx = tid + q * NT;
                 const int l = idx / N, c = idx - (idx / N) * N;
-                s_re[l * LS + c] = (float)va[q] / mA * 16.0f + 1.0f;
-                s_im[l * LS + c] = (float)vb[q] / mB * 16.0f + 1.0f;
+                s_re[l * LS + c] = va[q] / mA * 16.0f + 1.0f;
+                s_im[l * LS + c] = vb[q] / mB * 16.0f + 1.0f;
             }
             __syncthreads();
             fft_lines<LOGM, true, false, NT, BL>(s_re, s_im, LS, 1, BL, s_twf);
@@ -795,9 +799,14 @@ int launch_pass(Context *ctx, const PassParams &p)
     // N = 32 (0.381 vs 0.436 ms for 16,129 IAs) and N = 64, where more
     // wavefronts per IA hide latency better
     // the matrix-core forms cover N = 16, 32 and 64; N = 128 keeps the exact form
+    // float tiles (MatrixFloat) run the exact form in the in-LDS or IA-128 kernels
+    const bool ftiles = p.tAf != nullptr || p.tBf != nullptr;
+    if (ftiles && (from_image || p.arith != QLP_XCORR_EXACT))
+        return fail(QLP_ERR_UNSUPPORTED, "xcorr: float tiles take the exact form from tiles");
     if (p.arith != QLP_XCORR_EXACT && p.N <= 64)
         return launch_pass_mfma(ctx, p, p.arith, from_image, peak);
-    if (p.N == 16 && (p.shape == SHAPE_LANE || (p.shape == SHAPE_AUTO && ctx->xcorr_form == FORM_LANE)))
+    if (!ftiles && p.N == 16 &&
+        (p.shape == SHAPE_LANE || (p.shape == SHAPE_AUTO && ctx->xcorr_form == FORM_LANE)))
         return launch_pass_lane(ctx, p, from_image, peak);
     switch (p.N) {
     case 16: launch_pass_m<5>(ctx, p, from_image, peak); break;

@@ -123,6 +123,30 @@ class Context:
                                            _ptr(maxB), int(mode), _ptr(out)))
         return out
 
+    def xcorr_f32(self, tilesA, tilesB, maxA, maxB=None, mode=QLP_NORM_PER_TILE, out=None):
+        """Correlate n pairs of float tiles (MatrixFloat, e.g. cut from filtered
+        frames) in the exact form; numpy inputs use the host entry point, torch
+        CUDA tensors the device one."""
+        n, N = int(tilesA.shape[0]), int(tilesA.shape[-1])
+        D = 2 * N - 1
+        if isinstance(tilesA, np.ndarray):
+            tilesA = np.ascontiguousarray(tilesA, np.float32)
+            tilesB = np.ascontiguousarray(tilesB, np.float32)
+            maxA = np.ascontiguousarray(maxA, np.float32)
+            maxB = None if maxB is None else np.ascontiguousarray(maxB, np.float32)
+            if out is None:
+                out = np.empty((n, D, D), np.float32)
+            check(self.lib.qlp_xcorr_f32(self.h, n, N, _ptr(tilesA), _ptr(tilesB), _ptr(maxA),
+                                         _ptr(maxB), int(mode), _ptr(out)))
+            return out
+        import torch
+
+        if out is None:
+            out = torch.empty((n, D, D), dtype=torch.float32, device=tilesA.device)
+        check(self.lib.qlp_xcorr_f32_dev(self.h, n, N, _ptr(tilesA), _ptr(tilesB), _ptr(maxA),
+                                         _ptr(maxB), int(mode), _ptr(out)))
+        return out
+
     # ---- peaks -------------------------------------------------------------
     def find_peaks(self, planes, kind=QLP_PEAK_FROM_CENTER, n_peaks=1, kernel_width=3,
                    out=None, min_floor=None):
@@ -207,6 +231,33 @@ class Context:
         check(self.lib.qlp_warp_micro_tiles_dev(self.h, _ptr(img), H, W, n, N, _ptr(top),
                                                 _ptr(left), _ptr(u), _ptr(v), _ptr(tiles),
                                                 _ptr(mx)))
+        return tiles, mx
+
+    def clip_tiles_f32(self, img, top, left, N, u=None, v=None):
+        """Float tiles of a device float frame at (top, left), or shifted by the
+        rounded (u, v): (n x N x N tiles, maxima)."""
+        import torch
+
+        n = int(top.shape[0])
+        H, W = int(img.shape[0]), int(img.shape[1])
+        tiles = torch.empty((n, N, N), dtype=torch.float32, device=img.device)
+        mx = torch.empty(n, dtype=torch.float32, device=img.device)
+        check(self.lib.qlp_clip_tiles_f32_dev(self.h, _ptr(img), H, W, n, N, _ptr(top), _ptr(left),
+                                              _ptr(u), _ptr(v), _ptr(tiles), _ptr(mx)))
+        return tiles, mx
+
+    def warp_micro_tiles_f32(self, img, top, left, u, v, N, side=_lib.QLP_SIDE_B_SECOND):
+        """Micro warp of one side of n tiles of a device float frame:
+        (n x N x N float tiles, maxima 65535)."""
+        import torch
+
+        n = int(top.shape[0])
+        H, W = int(img.shape[0]), int(img.shape[1])
+        tiles = torch.empty((n, N, N), dtype=torch.float32, device=img.device)
+        mx = torch.empty(n, dtype=torch.float32, device=img.device)
+        check(self.lib.qlp_warp_micro_tiles_f32_dev(self.h, _ptr(img), H, W, n, N, _ptr(top),
+                                                    _ptr(left), _ptr(u), _ptr(v), int(side),
+                                                    _ptr(tiles), _ptr(mx)))
         return tiles, mx
 
     def warp_mini_tiles(self, imgs, top, left, u, v, N):
@@ -509,6 +560,7 @@ def piv_config(**config):
     cfg.inherit_mode = config.get("inherit_mode", _lib.QLP_INHERIT_DIRECT)
     cfg.xcorr_form = config.get("xcorr_form", QLP_XCORR_EXACT)
     cfg.warp_order = config.get("warp_order", _lib.QLP_WARP_ORDER_SECOND)
+    cfg.frame_type = config.get("frame_type", _lib.QLP_FRAME_U8)
     return cfg
 
 
@@ -780,8 +832,12 @@ class PivPlan:
             pass
 
     def run(self, imgsA, imgsB):
-        """Enqueue all levels (device tensors pairs x H x W uint8)."""
-        check(self.ctx.lib.qlp_piv_run_dev(self.h, _ptr(imgsA), _ptr(imgsB)))
+        """Enqueue all levels (device tensors pairs x H x W: uint8, or float32
+        for a plan made with frame_type QLP_FRAME_F32)."""
+        if self.cfg.frame_type == _lib.QLP_FRAME_F32:
+            check(self.ctx.lib.qlp_piv_run_f32_dev(self.h, _ptr(imgsA), _ptr(imgsB)))
+        else:
+            check(self.ctx.lib.qlp_piv_run_dev(self.h, _ptr(imgsA), _ptr(imgsB)))
 
     def set_timing(self, enable=True):
         check(self.ctx.lib.qlp_piv_plan_set_timing(self.h, int(enable)))

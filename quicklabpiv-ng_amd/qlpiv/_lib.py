@@ -45,6 +45,11 @@ SIGNATURES = {
     "qlp_version": (ctypes.c_char_p, []),
     "qlp_xcorr_batch": (_i, [_vp, _i, _i, _vp, _vp, _vp, _vp, _i, _vp]),
     "qlp_xcorr_batch_dev": (_i, [_vp, _i, _i, _vp, _vp, _vp, _vp, _i, _vp]),
+    "qlp_xcorr_f32": (_i, [_vp, _i, _i, _vp, _vp, _vp, _vp, _i, _vp]),
+    "qlp_xcorr_f32_dev": (_i, [_vp, _i, _i, _vp, _vp, _vp, _vp, _i, _vp]),
+    "qlp_clip_tiles_f32_dev": (_i, [_vp, _vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "qlp_warp_micro_tiles_f32_dev": (_i, [_vp, _vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _vp,
+                                          _vp]),
     "qlp_xcorr_form_dev": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _vp]),
     "qlp_find_peaks": (_i, [_vp, _i, _i, _i, _vp, _i, _i, _vp, _vp]),
     "qlp_find_peaks_dev": (_i, [_vp, _i, _i, _i, _vp, _i, _i, _vp, _vp]),
@@ -65,6 +70,7 @@ SIGNATURES = {
     "qlp_piv_plan_level": (_i, [_vp, _i, ctypes.POINTER(_i), ctypes.POINTER(_i),
                                 ctypes.POINTER(_i), _vp, _vp]),
     "qlp_piv_run_dev": (_i, [_vp, _vp, _vp]),
+    "qlp_piv_run_f32_dev": (_i, [_vp, _vp, _vp]),
     "qlp_piv_result_dev": (_i, [_vp, _i, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
                                 ctypes.POINTER(_vp), ctypes.POINTER(_vp)]),
     "qlp_piv_validation_dev": (_i, [_vp, _i, ctypes.POINTER(_vp), ctypes.POINTER(_vp)]),
@@ -121,6 +127,8 @@ QLP_WARP_FULL = 3
 QLP_WARP_ORDER_SECOND = 0
 QLP_WARP_ORDER_FIRST = 1
 QLP_WARP_ORDER_BOTH = 2
+QLP_SIDE_B_SECOND, QLP_SIDE_A_FIRST, QLP_SIDE_A_BOTH, QLP_SIDE_B_BOTH = 0, 1, 2, 3
+QLP_FRAME_U8, QLP_FRAME_F32 = 0, 1
 
 
 QLP_INHERIT_DIRECT = 0
@@ -186,7 +194,8 @@ class PivConfig(ctypes.Structure):
                 ("keep_planes", ctypes.c_int), ("peak_kind", ctypes.c_int),
                 ("n_peaks", ctypes.c_int), ("peak_kernel_width", ctypes.c_int),
                 ("valid", ValidConfig), ("inherit_mode", ctypes.c_int),
-                ("xcorr_form", ctypes.c_int), ("warp_order", ctypes.c_int)]
+                ("xcorr_form", ctypes.c_int), ("warp_order", ctypes.c_int),
+                ("frame_type", ctypes.c_int)]
 
 
 class ServerConfig(ctypes.Structure):

@@ -366,6 +366,49 @@ def test_clip_tile_zero_fill_and_max_clamp():
     assert np.array_equal(t, img[:16, :16]) and mx == 200.0
 
 
+def test_float_clip_max_semantics():
+    """MatrixFloat tiles: zero fill outside the frame, maximum over the tile
+    with no >= 16 clamp and 0 -> 1; integer-valued frames give the 8-bit clip's
+    tiles."""
+    rng = np.random.default_rng(3)
+    img = (rng.random((40, 50)) * 3.0).astype(np.float32)
+    t, mx = O.clip_tile_f32(img, -3, 45, 16)
+    ref = np.zeros((16, 16), np.float32)
+    ref[3:, :5] = img[0:13, 45:50]
+    assert np.array_equal(t, ref) and mx == np.float32(img[0:13, 45:50].max())
+    t, mx = O.clip_tile_f32(img, 100, 100, 16)  # wholly outside: zeros, maximum 1
+    assert not t.any() and mx == 1.0
+    t, mx = O.clip_tile_f32(-img, 0, 0, 16)  # no value above 0: maximum 1
+    assert mx == 1.0
+    img8 = rng.integers(0, 256, (40, 50)).astype(np.uint8)
+    for top, left in [(0, 0), (-5, 40), (30, -7)]:
+        a, ma = O.clip_tile(img8, top, left, 16)
+        b, mb = O.clip_tile_f32(img8.astype(np.float32), top, left, 16)
+        assert np.array_equal(a.astype(np.float32), b)
+        assert mb == max(float(a.max()), 1.0) and ma == max(float(a.max()), 16.0)
+
+
+def test_float_warp_micro_reads_float_frames():
+    """The float-frame Micro warp: on integer-valued frames it equals the 8-bit
+    warp (every side); with integer displacements it is the shifted frame
+    rounded half up; on a constant frame c it is round(c) everywhere inside;
+    its maximum is 65535."""
+    rng = np.random.default_rng(4)
+    img8 = rng.integers(0, 256, (80, 90)).astype(np.uint8)
+    f = img8.astype(np.float32)
+    for side in range(4):
+        for top, left, du, dv in [(10, 12, 1.3, -2.7), (-4, 70, -0.4, 3.9), (60, -3, 2.5, 0.5)]:
+            a = O.warp_micro(img8, top, left, 16, du, dv, side)
+            b, m = O.warp_micro_f32(f, top, left, 16, du, dv, side)
+            assert np.array_equal(a.astype(np.float32), b) and m == 65535.0
+    g = (rng.random((80, 90)) * 300.0).astype(np.float32)
+    b, _ = O.warp_micro_f32(g, 20, 30, 16, 3.0, -2.0)
+    assert np.array_equal(b, np.floor(g[23:39, 28:44] + np.float32(0.5)))
+    c = np.full((80, 90), 37.625, np.float32)
+    b, _ = O.warp_micro_f32(c, 20, 30, 16, 1.7, -0.3)
+    assert np.all(b == 38.0)
+
+
 def test_warp_micro_invariants():
     rng = np.random.default_rng(2)
     img = rng.integers(0, 256, (96, 96)).astype(np.uint8)
