@@ -260,4 +260,157 @@ __device__ __forceinline__ void multipeak_segment(const float *__restrict__ m, i
     }
 }
 
+// The r-th (from 0) set bit of a mask.
+__device__ __forceinline__ int mp_nth_bit(uint64_t m, int r)
+{
+    for (; r > 0; r--) m &= m - 1ull;
+    return m ? __ffsll((unsigned long long)m) - 1 : 0;
+}
+
+// addInAscendOrder :98-237 with the list spread over the lanes of a segment:
+// lane sl holds entry sl (ascending) while sl < size.  Every lane of the
+// segment calls it with the same candidate; the accept/reject and delete
+// decisions are those of mp_insert.
+__device__ __forceinline__ void mp_insert_lanes(float &ev, int &ei, int &ej, int &size, int sl,
+                                                uint64_t segmask, int P, int kd, float value,
+                                                int pi, int pj)
+{
+    if (size == 0) {
+        if (sl == 0) {
+            ev = value;
+            ei = pi;
+            ej = pj;
+        }
+        size = 1;
+        return;
+    }
+    const bool have = sl < size;
+    const int di = pi - ei, dj = pj - ej;
+    const bool near = have && (float)(di * di + dj * dj) < (float)kd;
+    const bool lt = have && ev < value;  // entries below the insertion point
+    const int pos = __popcll(__ballot(lt) & segmask);
+    if (size == P && pos == 0) return;
+    // an entry at or above the insertion point (value <= its value) nearby rejects it
+    if (__ballot(near && have && !lt) & segmask) return;
+    // nearby entries below it are deleted
+    const uint64_t delm = __ballot(near && lt) & segmask;
+    const int ndel = __popcll(delm);
+    const int size1 = size - ndel;
+    const int ins = pos - ndel;
+    if (ins == 0 && size1 == P) return;
+    const int drop = (ins > 0 && size1 + 1 > P) ? 1 : 0;  // remove() of the smallest entry
+    const int nsize = size1 + 1 - drop;
+    const uint64_t surv = __ballot(have) & segmask & ~delm;
+    // target index sl: position u of the list after the insertion, before the drop
+    const int u = sl + drop;
+    const bool fresh = u == ins;
+    const int src = mp_nth_bit(surv, (u < ins) ? u : (u > ins ? u - 1 : 0));
+    const float sv = __shfl(ev, src, 64);
+    const int si = __shfl(ei, src, 64), sj = __shfl(ej, src, 64);
+    if (sl < nsize) {
+        ev = fresh ? value : sv;
+        ei = fresh ? pi : si;
+        ej = fresh ? pj : sj;
+    }
+    size = nsize;
+}
+
+// multipeak_segment with the list in registers (needs P <= W): same visit
+// order, insertion replay and outputs.
+template <int KW, int W>
+__device__ __forceinline__ void multipeak_segment_lanes(const float *__restrict__ m, int rs, int cs,
+                                                        int D, int P, int k, bool act,
+                                                        float *__restrict__ peaks,
+                                                        float *min_floor, int *count)
+{
+    const int lane = threadIdx.x & 63;
+    const int seg = lane / W, sl = lane % W;
+    const uint64_t segmask = (W == 64) ? ~0ull : (((1ull << (W & 63)) - 1ull) << (seg * W));
+
+    int iSlices = D / k;
+    if (iSlices * k != D) iSlices += 1;
+    const int iTop = iSlices - iSlices / 2;
+    int jSlices = D / k;
+    if (jSlices * k != D) jSlices += 1;
+    const int jLeft = jSlices - jSlices / 2;
+    const int nentries = iTop * jLeft * 4;
+    const int q = sl & 3;
+    const uint64_t qlanes = 0x1111111111111111ull << (lane & 3);
+    const int kd = (k - 1) * k;
+
+    float ev = 0.0f;
+    int ei = 0, ej = 0, size = 0;
+    float mn = FLT_MAX;
+    int staleI[4] = {0, 0, 0, 0}, staleJ[4] = {0, 0, 0, 0};
+    for (int base = 0; base < nentries; base += W) {
+        const int e = base + sl;
+        const bool valid = act && e < nentries;
+        float bv = 0.0f;
+        int bi = 0, bj = 0, upd = 0;
+        if (valid) {
+            const int blk = e >> 2;
+            const int si = blk / jLeft, sj = blk - (blk / jLeft) * jLeft;
+            const int top = si * k + ((q & 2) ? iTop * k : 0);
+            const int left = sj * k + ((q & 1) ? jLeft * k : 0);
+            block_max<KW>(m, rs, cs, D, k, top, left, bv, bi, bj, upd, mn);
+        }
+        const uint64_t updm = __ballot(valid && upd);
+        const uint64_t prior = updm & qlanes & segmask & ((1ull << lane) - 1ull);
+        const int psrc = prior ? 63 - __clzll(prior) : lane;
+        const int ci = __shfl(bi, psrc, 64), cj = __shfl(bj, psrc, 64);
+        int carryI = staleI[0], carryJ = staleJ[0];
+#pragma unroll
+        for (int qq = 1; qq < 4; qq++)
+            if (q == qq) {
+                carryI = staleI[qq];
+                carryJ = staleJ[qq];
+            }
+        const int pi = upd ? bi : (prior ? ci : carryI);
+        const int pj = upd ? bj : (prior ? cj : carryJ);
+        const float pv = upd ? bv : 0.0f;
+#pragma unroll
+        for (int qq = 0; qq < 4; qq++) {
+            const uint64_t mq = updm & (0x1111111111111111ull << qq) & segmask;
+            const int last = mq ? 63 - __clzll(mq) : lane;
+            const int li = __shfl(bi, last, 64), lj = __shfl(bj, last, 64);
+            if (mq) {
+                staleI[qq] = li;
+                staleJ[qq] = lj;
+            }
+        }
+        uint64_t pending = __ballot(valid);
+        for (;;) {
+            const float lo = __shfl(ev, seg * W, 64);
+            const uint64_t cand = __ballot(size < P || pv > lo) & pending;
+            if (!cand) break;
+            const uint64_t mine = cand & segmask;
+            const int c = mine ? __ffsll((unsigned long long)mine) - 1 : lane;
+            const float cv = __shfl(pv, c, 64);
+            const int cI = __shfl(pi, c, 64), cJ = __shfl(pj, c, 64);
+            if (mine) mp_insert_lanes(ev, ei, ej, size, sl, segmask, P, kd, cv, cI, cJ);
+            pending &= ~__ballot(mine && lane <= c);
+        }
+    }
+#pragma unroll
+    for (int o = W / 2; o >= 1; o >>= 1) {
+        const float om = __shfl_xor(mn, o, 64);
+        if (om < mn) mn = om;
+    }
+    if (act && sl < P) {
+        // descending: peak 0 is the largest; slots past the list are zero
+        if (sl < size) {
+            const int idx = size - 1 - sl;
+            peaks[3 * idx + 0] = (float)ei;
+            peaks[3 * idx + 1] = (float)ej;
+            peaks[3 * idx + 2] = ev;
+        } else {
+            peaks[3 * sl] = peaks[3 * sl + 1] = peaks[3 * sl + 2] = 0.0f;
+        }
+    }
+    if (act && sl == 0) {
+        *min_floor = mn;
+        if (count) *count = size > 1 ? size : 1;
+    }
+}
+
 }  // namespace qlpiv

@@ -13,6 +13,8 @@
 // MULTI (maximum/FindMaximumMultiPeaks.java:42-162): multipeak_segment
 // (qlpiv_multipeak.h) with 8 to 64 lanes per plane.
 #include <float.h>
+#include <stdlib.h>
+#include <string.h>
 
 #include "qlpiv_device.h"
 #include "qlpiv_internal.h"
@@ -79,7 +81,7 @@ __global__ __launch_bounds__(256) void peak_from_center_kernel(int n, int D,
 // lanes per plane when there are many, so that the sequential insertions of
 // several planes share one instruction stream).
 template <int KW, int W>
-__global__ __launch_bounds__(64) void peak_multi_kernel(int n, int D, int P, int k,
+__global__ __launch_bounds__(64) void peak_multi_kernel(int n, int D, int P, int k, int lanes_list,
                                                         const float *__restrict__ planes,
                                                         float *__restrict__ peaks,
                                                         float *__restrict__ min_floor,
@@ -91,31 +93,45 @@ __global__ __launch_bounds__(64) void peak_multi_kernel(int n, int D, int P, int
     const int ia = blockIdx.x * S + seg;
     const bool act = ia < n;
     const size_t at = act ? (size_t)ia : 0;
-    multipeak_segment<KW, W>(planes + at * D * D, D, 1, D, P, k, act, s_list[seg],
-                             peaks + at * P * 3, min_floor + at, count ? count + at : nullptr);
+    if (P <= W && lanes_list)  // the list in the segment's registers
+        multipeak_segment_lanes<KW, W>(planes + at * D * D, D, 1, D, P, k, act, peaks + at * P * 3,
+                                       min_floor + at, count ? count + at : nullptr);
+    else
+        multipeak_segment<KW, W>(planes + at * D * D, D, 1, D, P, k, act, s_list[seg],
+                                 peaks + at * P * 3, min_floor + at, count ? count + at : nullptr);
 }
 
 template <int KW>
 static void launch_multi(Context *ctx, int W, int n, int D, int P, int k, const float *planes,
                          float *peaks, float *min_floor, int *count)
 {
+    // the list in the segment's registers when it fits (P <= W) and the segment
+    // is at most 32 lanes.  Measured on MI355X (C2 step, P = 3, k = 3; MultiPeaks
+    // + Guo per level, ms, LDS list -> register list): 65,025 x 31^2 at W = 8:
+    // 0.257 -> 0.219; 16,129 x 63^2 at W = 16: 0.191 -> 0.163; 3,969 x 127^2 at
+    // W = 64: 0.140 -> 0.182.  QLPIV_MULTIPEAK_LIST (lds or lanes) overrides it.
+    int lanes_list = W <= 32 ? 1 : 0;
+    if (const char *env = getenv("QLPIV_MULTIPEAK_LIST")) {
+        if (strcmp(env, "lds") == 0) lanes_list = 0;
+        if (strcmp(env, "lanes") == 0) lanes_list = 1;
+    }
     const unsigned grid = (unsigned)((n + 64 / W - 1) / (64 / W));
     switch (W) {
     case 8:
         hipLaunchKernelGGL((peak_multi_kernel<KW, 8>), dim3(grid), dim3(64), 0, ctx->stream, n, D,
-                           P, k, planes, peaks, min_floor, count);
+                           P, k, lanes_list, planes, peaks, min_floor, count);
         break;
     case 16:
         hipLaunchKernelGGL((peak_multi_kernel<KW, 16>), dim3(grid), dim3(64), 0, ctx->stream, n,
-                           D, P, k, planes, peaks, min_floor, count);
+                           D, P, k, lanes_list, planes, peaks, min_floor, count);
         break;
     case 32:
         hipLaunchKernelGGL((peak_multi_kernel<KW, 32>), dim3(grid), dim3(64), 0, ctx->stream, n,
-                           D, P, k, planes, peaks, min_floor, count);
+                           D, P, k, lanes_list, planes, peaks, min_floor, count);
         break;
     default:
         hipLaunchKernelGGL((peak_multi_kernel<KW, 64>), dim3(grid), dim3(64), 0, ctx->stream, n,
-                           D, P, k, planes, peaks, min_floor, count);
+                           D, P, k, lanes_list, planes, peaks, min_floor, count);
     }
 }
 

@@ -4,6 +4,7 @@ reference's Java code), through the C ABI.
 Bars: correlation planes value-identical to the Java arithmetic (float ==),
 integer peaks bit-exact, sub-pixel results within 1e-4 px, tiles bit-exact.
 """
+import os
 import sys
 
 import numpy as np
@@ -165,6 +166,36 @@ def test_multipeaks_bit_exact(ctx, D, P, kw):
         ref, found, rmf = O.find_max_multipeaks(planes[k], P, kw)
         assert np.array_equal(peaks[k], ref), (k, peaks[k], ref)
         assert mf[k] == rmf
+
+
+@pytest.mark.parametrize("W", [8, 16, 32, 64])
+@pytest.mark.parametrize("lst", ["lanes", "lds"])
+def test_multipeaks_every_segment_width_and_list_form(ctx, W, lst):
+    """Both forms of the search's bounded list (in the lanes of a plane's
+    segment, or in LDS) at every segment width, against the oracle: P below,
+    equal to and above the width (the LDS list then), kernel widths 1, 2, 3."""
+    import qlpiv
+
+    saved = {k: os.environ.get(k) for k in ("QLPIV_MULTIPEAK_LANES", "QLPIV_MULTIPEAK_LIST")}
+    os.environ["QLPIV_MULTIPEAK_LANES"] = str(W)
+    os.environ["QLPIV_MULTIPEAK_LIST"] = lst
+    try:
+        for D, P, kw in [(31, 3, 3), (31, 8, 2), (63, 16, 3), (63, 1, 3), (15, 5, 1)]:
+            rng = np.random.default_rng(W + D + P)
+            planes = planes_for_peaks(rng, 40, D)
+            planes[7] = np.round(planes[7] / 10)  # many equal block maxima
+            say(f"multipeaks W={W} list={lst} D={D} P={P} k={kw}")
+            peaks, mf = ctx.find_peaks(planes, qlpiv.QLP_PEAK_MULTI, P, kw)
+            for k in range(len(planes)):
+                ref, found, rmf = O.find_max_multipeaks(planes[k], P, kw)
+                assert np.array_equal(peaks[k], ref), (D, P, kw, k)
+                assert mf[k] == rmf, (D, P, kw, k)
+    finally:
+        for k, v in saved.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
 
 
 def test_multipeaks_reference_kat(ctx):
