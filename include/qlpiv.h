@@ -531,9 +531,12 @@ int qlp_batch_destroy(qlp_batch *b);
  * jobs/ImageReaderJob.compute :80-110 with images/Image.readImageToBuffer
  * :146-156 for 8-bit frames (the TYPE_BYTE_GRAY raster copied as it is), by
  * host code: 8-bit grayscale (black is zero) TIFF strips, uncompressed, PackBits,
- * LZW or Deflate (predictor 1 or 2), and 8-bit grayscale non-interlaced PNG.
- * Colour, palette, alpha, other depths, tiles and JPEG return
- * QLP_ERR_UNSUPPORTED (the reference converts them through Java2D). */
+ * LZW or Deflate (predictor 1 or 2), 8-bit grayscale non-interlaced PNG, and
+ * baseline / extended sequential one-component 8-bit JPEG (with restart
+ * intervals; the integer inverse DCT of IJG-derived decoders such as the one
+ * javax.imageio wraps).  Colour, palette, alpha, other depths, tiles and
+ * progressive or arithmetic-coded JPEG return QLP_ERR_UNSUPPORTED (the
+ * reference converts them through Java2D). */
 /* Decode 'path' into out (H x W bytes, row-major); out NULL only reports H, W. */
 int qlp_image_read(const char *path, uint8_t *out, size_t out_bytes, int *H, int *W);
 

@@ -1,14 +1,22 @@
-// Frame ingest (host code): 8-bit grayscale TIFF and PNG files to uint8 frames.
+// Frame ingest (host code): 8-bit grayscale TIFF, PNG and JPEG files to uint8
+// frames.
 //
 // The reference reads frames with javax.imageio (jobs/ImageReaderJob.compute
 // :80-110) and, for 8-bit images, copies a TYPE_BYTE_GRAY raster as it is
 // (images/Image.readImageToBuffer :146-156).  This reader decodes exactly that
 // case: one 8-bit sample per pixel, black is zero.  Other layouts (colour,
-// palette, alpha, 1/2/4/16-bit, tiles, JPEG) go through Java2D conversions in
-// the reference and are rejected with QLP_ERR_UNSUPPORTED.
+// palette, alpha, 1/2/4/16-bit, tiles, progressive JPEG) go through Java2D
+// conversions or other decoders in the reference and are rejected with
+// QLP_ERR_UNSUPPORTED.
 //   TIFF: baseline strips, little or big endian, compression none, PackBits,
 //         LZW or Deflate, with or without horizontal differencing (predictor 2).
 //   PNG:  colour type 0, bit depth 8, not interlaced; the five row filters.
+//   JPEG: baseline or extended sequential Huffman, 8-bit, one component, with
+//         restart intervals (ITU-T T.81); the inverse DCT is the integer
+//         Loeffler-Ligtenberg-Moschytz factorisation with 13-bit constants and
+//         2 extra bits between the passes that IJG-derived decoders (the one
+//         javax.imageio wraps included) use by default, so the samples are the
+//         ones those decoders produce.
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -302,6 +310,291 @@ int read_png(const std::vector<uint8_t> &b, Image &img)
     return QLP_OK;
 }
 
+// ---- JPEG (ITU-T T.81, baseline / extended sequential Huffman, one component) ----
+
+struct HuffTable {
+    bool present = false;
+    // canonical code of each length (F.2.2.3): codes of length l are
+    // mincode[l] .. maxcode[l], their values start at valptr[l] in val
+    int mincode[17], maxcode[18], valptr[17];
+    uint8_t val[256];
+};
+
+bool build_huffman(const uint8_t *bits, const uint8_t *vals, int nvals, HuffTable &t)
+{
+    int code = 0, k = 0;
+    for (int l = 1; l <= 16; l++) {
+        t.valptr[l] = k;
+        t.mincode[l] = code;
+        code += bits[l - 1];
+        k += bits[l - 1];
+        t.maxcode[l] = bits[l - 1] ? code - 1 : -1;
+        if (code > (1 << l)) return false;  // over-subscribed code lengths
+        code <<= 1;
+    }
+    t.maxcode[17] = 0x7fffffff;
+    if (k != nvals) return false;
+    memcpy(t.val, vals, (size_t)nvals);
+    t.present = true;
+    return true;
+}
+
+struct BitReader {
+    const uint8_t *p, *end;
+    uint32_t acc = 0;
+    int n = 0;
+    bool marker = false;  // a marker was met: the segment's data is exhausted
+    int fill_byte()
+    {
+        if (marker || p >= end) return 0;
+        int b = *p;
+        if (b == 0xff) {
+            if (p + 1 < end && p[1] == 0x00) {
+                p += 2;
+                return 0xff;
+            }
+            marker = true;  // leave p at the marker
+            return 0;
+        }
+        p++;
+        return b;
+    }
+    int bit()
+    {
+        if (n == 0) {
+            acc = (uint32_t)fill_byte();
+            n = 8;
+        }
+        n--;
+        return (int)((acc >> n) & 1u);
+    }
+    int bits(int s)
+    {
+        int v = 0;
+        for (int i = 0; i < s; i++) v = (v << 1) | bit();
+        return v;
+    }
+    void align() { n = 0; }
+};
+
+int huff_decode(BitReader &br, const HuffTable &t)
+{
+    int code = br.bit();
+    int l = 1;
+    while (l <= 16 && code > t.maxcode[l]) {
+        code = (code << 1) | br.bit();
+        l++;
+    }
+    if (l > 16) return -1;
+    return t.val[t.valptr[l] + code - t.mincode[l]];
+}
+
+// EXTEND (F.2.2.1): the s-bit magnitude v as a signed coefficient difference.
+inline int extend(int v, int s) { return (s && v < (1 << (s - 1))) ? v - (1 << s) + 1 : v; }
+
+const int kZigzag[64] = {0,  1,  8,  16, 9,  2,  3,  10, 17, 24, 32, 25, 18, 11, 4,  5,
+                         12, 19, 26, 33, 40, 48, 41, 34, 27, 20, 13, 6,  7,  14, 21, 28,
+                         35, 42, 49, 56, 57, 50, 43, 36, 29, 22, 15, 23, 30, 37, 44, 51,
+                         58, 59, 52, 45, 38, 31, 39, 46, 53, 60, 61, 54, 47, 55, 62, 63};
+
+// Inverse DCT of one dequantised 8 x 8 block: the Loeffler-Ligtenberg-Moschytz
+// factorisation in 32-bit fixed point with 13-bit constants (c = round(x *
+// 2^13) of the rotation factors below), columns first keeping 2 extra bits,
+// then rows, rounding each pass by adding half the discarded weight; the
+// result is offset by 128 and clamped to 0..255.
+void idct8x8(const int *in, uint8_t *out, int stride)
+{
+    constexpr int CB = 13, P1 = 2;
+    auto fix = [](double x) { return (int64_t)(x * (1 << CB) + 0.5); };
+    static const int64_t F0298 = fix(0.298631336), F0390 = fix(0.390180644),
+                         F0541 = fix(0.541196100), F0765 = fix(0.765366865),
+                         F0899 = fix(0.899976223), F1175 = fix(1.175875602),
+                         F1501 = fix(1.501321110), F1847 = fix(1.847759065),
+                         F1961 = fix(1.961570560), F2053 = fix(2.053119869),
+                         F2562 = fix(2.562915447), F3072 = fix(3.072711026);
+    int64_t ws[64];
+    // one 1D transform of x[0..7] (stride sx) into 8 values before scaling
+    auto rot = [&](const int64_t *x, int sx, int64_t *o) {
+        // even part
+        int64_t z2 = x[2 * sx], z3 = x[6 * sx];
+        int64_t z1 = (z2 + z3) * F0541;
+        const int64_t t2 = z1 - z3 * F1847;
+        const int64_t t3 = z1 + z2 * F0765;
+        z2 = x[0];
+        z3 = x[4 * sx];
+        const int64_t t0 = (int64_t)((uint64_t)(z2 + z3) << CB);
+        const int64_t t1 = (int64_t)((uint64_t)(z2 - z3) << CB);
+        const int64_t a10 = t0 + t3, a13 = t0 - t3, a11 = t1 + t2, a12 = t1 - t2;
+        // odd part
+        int64_t b0 = x[7 * sx], b1 = x[5 * sx], b2 = x[3 * sx], b3 = x[1 * sx];
+        int64_t y1 = b0 + b3, y2 = b1 + b2, y3 = b0 + b2, y4 = b1 + b3;
+        const int64_t y5 = (y3 + y4) * F1175;
+        b0 *= F0298;
+        b1 *= F2053;
+        b2 *= F3072;
+        b3 *= F1501;
+        y1 *= -F0899;
+        y2 *= -F2562;
+        y3 *= -F1961;
+        y4 *= -F0390;
+        y3 += y5;
+        y4 += y5;
+        b0 += y1 + y3;
+        b1 += y2 + y4;
+        b2 += y2 + y3;
+        b3 += y1 + y4;
+        o[0] = a10 + b3;
+        o[7] = a10 - b3;
+        o[1] = a11 + b2;
+        o[6] = a11 - b2;
+        o[2] = a12 + b1;
+        o[5] = a12 - b1;
+        o[3] = a13 + b0;
+        o[4] = a13 - b0;
+    };
+    auto descale = [](int64_t x, int n) { return (x + ((int64_t)1 << (n - 1))) >> n; };
+    int64_t col[8], o[8];
+    for (int c = 0; c < 8; c++) {
+        for (int r = 0; r < 8; r++) col[r] = in[r * 8 + c];
+        bool ac = false;
+        for (int r = 1; r < 8; r++) ac |= col[r] != 0;
+        if (!ac) {
+            for (int r = 0; r < 8; r++) ws[r * 8 + c] = (int64_t)((uint64_t)col[0] << P1);
+            continue;
+        }
+        rot(col, 1, o);
+        for (int r = 0; r < 8; r++) ws[r * 8 + c] = descale(o[r], CB - P1);
+    }
+    for (int r = 0; r < 8; r++) {
+        rot(ws + r * 8, 1, o);
+        for (int c = 0; c < 8; c++) {
+            const int64_t v = descale(o[c], CB + P1 + 3) + 128;
+            out[r * stride + c] = (uint8_t)(v < 0 ? 0 : (v > 255 ? 255 : v));
+        }
+    }
+}
+
+int read_jpeg(const std::vector<uint8_t> &b, Image &img)
+{
+    const uint8_t *p = b.data(), *end = b.data() + b.size();
+    uint16_t q[4][64];
+    bool qok[4] = {false, false, false, false};
+    HuffTable dc[4], ac[4];
+    int H = 0, W = 0, qsel = 0, restart = 0;
+    bool frame = false;
+    p += 2;  // SOI
+    auto be16 = [](const uint8_t *x) { return (int)x[0] << 8 | x[1]; };
+    while (p + 4 <= end) {
+        if (p[0] != 0xff) return fail(QLP_ERR_ARG, "jpeg: corrupt marker stream");
+        const int m = p[1];
+        if (m == 0xff) {  // fill byte
+            p++;
+            continue;
+        }
+        if (m == 0xd9) break;  // EOI before a scan
+        const int len = be16(p + 2);
+        const uint8_t *seg = p + 4, *segend = p + 2 + len;
+        if (len < 2 || segend > end) return fail(QLP_ERR_ARG, "jpeg: truncated segment");
+        if (m == 0xc0 || m == 0xc1) {  // baseline / extended sequential, Huffman
+            if (len < 8 || seg[0] != 8)
+                return fail(QLP_ERR_UNSUPPORTED, "jpeg: only 8-bit samples are read");
+            H = be16(seg + 1);
+            W = be16(seg + 3);
+            const int nc = seg[5];
+            if (nc != 1)
+                return fail(QLP_ERR_UNSUPPORTED,
+                            "jpeg: only one-component (grayscale) images are read; the reference "
+                            "converts colour through Java2D");
+            if (len < 8 + 3) return fail(QLP_ERR_ARG, "jpeg: truncated frame header");
+            qsel = seg[8];
+            if (qsel > 3) return fail(QLP_ERR_ARG, "jpeg: bad quantisation table index");
+            if (H < 1 || W < 1) return fail(QLP_ERR_UNSUPPORTED, "jpeg: image height is defined later");
+            frame = true;
+        } else if (m >= 0xc2 && m <= 0xcf && m != 0xc4 && m != 0xc8 && m != 0xcc) {
+            return fail(QLP_ERR_UNSUPPORTED, "jpeg: progressive, lossless and arithmetic-coded images are not read");
+        } else if (m == 0xdb) {  // quantisation tables
+            const uint8_t *x = seg;
+            while (x < segend) {
+                const int pq = x[0] >> 4, tq = x[0] & 15;
+                if (tq > 3 || pq > 1 || x + 1 + 64 * (pq + 1) > segend)
+                    return fail(QLP_ERR_ARG, "jpeg: bad quantisation table");
+                for (int k = 0; k < 64; k++)
+                    q[tq][kZigzag[k]] = pq ? (uint16_t)be16(x + 1 + 2 * k) : x[1 + k];
+                qok[tq] = true;
+                x += 1 + 64 * (pq + 1);
+            }
+        } else if (m == 0xc4) {  // Huffman tables
+            const uint8_t *x = seg;
+            while (x + 17 <= segend) {
+                const int tc = x[0] >> 4, th = x[0] & 15;
+                int nv = 0;
+                for (int l = 0; l < 16; l++) nv += x[1 + l];
+                if (tc > 1 || th > 3 || nv > 256 || x + 17 + nv > segend)
+                    return fail(QLP_ERR_ARG, "jpeg: bad Huffman table");
+                if (!build_huffman(x + 1, x + 17, nv, tc ? ac[th] : dc[th]))
+                    return fail(QLP_ERR_ARG, "jpeg: bad Huffman code lengths");
+                x += 17 + nv;
+            }
+        } else if (m == 0xdd) {  // restart interval
+            if (len < 4) return fail(QLP_ERR_ARG, "jpeg: bad restart interval");
+            restart = be16(seg);
+        } else if (m == 0xda) {  // start of scan
+            if (!frame) return fail(QLP_ERR_ARG, "jpeg: scan before the frame header");
+            if (len < 8 || seg[0] != 1) return fail(QLP_ERR_ARG, "jpeg: bad scan header");
+            const int td = seg[2] >> 4, ta = seg[2] & 15;
+            if (td > 3 || ta > 3 || !dc[td].present || !ac[ta].present || !qok[qsel])
+                return fail(QLP_ERR_ARG, "jpeg: scan refers to a missing table");
+            const int bw = (W + 7) / 8, bh = (H + 7) / 8;
+            std::vector<uint8_t> full((size_t)bw * 8 * bh * 8);
+            BitReader br{segend, end};
+            int pred = 0, coef[64];
+            const int nblocks = bw * bh;
+            for (int blk = 0; blk < nblocks; blk++) {
+                if (restart && blk > 0 && blk % restart == 0) {
+                    // RSTn: byte-align, skip the marker, reset the DC prediction
+                    br.align();
+                    while (br.p + 1 < end && !(br.p[0] == 0xff && br.p[1] >= 0xd0 && br.p[1] <= 0xd7))
+                        br.p++;
+                    if (br.p + 1 >= end) return fail(QLP_ERR_ARG, "jpeg: missing restart marker");
+                    br.p += 2;
+                    br.marker = false;
+                    pred = 0;
+                }
+                for (int k = 0; k < 64; k++) coef[k] = 0;
+                const int s = huff_decode(br, dc[td]);
+                if (s < 0 || s > 11) return fail(QLP_ERR_ARG, "jpeg: corrupt DC code");
+                pred += extend(br.bits(s), s);
+                coef[0] = pred * q[qsel][0];
+                for (int k = 1; k < 64;) {
+                    const int rs = huff_decode(br, ac[ta]);
+                    if (rs < 0) return fail(QLP_ERR_ARG, "jpeg: corrupt AC code");
+                    const int r = rs >> 4, sz = rs & 15;
+                    if (sz == 0) {
+                        if (r != 15) break;  // end of block
+                        k += 16;             // sixteen zeros
+                        continue;
+                    }
+                    k += r;
+                    if (k > 63) return fail(QLP_ERR_ARG, "jpeg: coefficient index past the block");
+                    const int z = kZigzag[k];
+                    coef[z] = extend(br.bits(sz), sz) * q[qsel][z];
+                    k++;
+                }
+                const int by = blk / bw, bx = blk - by * bw;
+                idct8x8(coef, full.data() + (size_t)by * 8 * bw * 8 + bx * 8, bw * 8);
+            }
+            img.H = H;
+            img.W = W;
+            img.px.resize((size_t)H * W);
+            for (int y = 0; y < H; y++)
+                memcpy(img.px.data() + (size_t)y * W, full.data() + (size_t)y * bw * 8, (size_t)W);
+            return QLP_OK;
+        }
+        p = segend;
+    }
+    return fail(QLP_ERR_ARG, "jpeg: no image data");
+}
+
 int decode(const char *path, Image &img)
 {
     if (!path) return fail(QLP_ERR_ARG, "image: null path");
@@ -311,6 +604,7 @@ int decode(const char *path, Image &img)
     if (b.size() >= 8 && !memcmp(b.data(), png_sig, 8)) return read_png(b, img);
     if (b.size() >= 8 && ((b[0] == 'I' && b[1] == 'I') || (b[0] == 'M' && b[1] == 'M')))
         return read_tiff(b, img);
+    if (b.size() >= 4 && b[0] == 0xff && b[1] == 0xd8 && b[2] == 0xff) return read_jpeg(b, img);
     return fail(QLP_ERR_UNSUPPORTED, std::string("image: unsupported file format: ") + path);
 }
 

@@ -56,6 +56,44 @@ def test_png_round_trip(tmp_path, optimize):
     assert np.array_equal(qlpiv.read_image(path), f)
 
 
+def particles(seed, H, W):
+    """A PIV-like frame: Gaussian particles on a dark background."""
+    rng = np.random.default_rng(seed)
+    y, x = np.mgrid[0:H, 0:W]
+    f = np.zeros((H, W))
+    for cy, cx, a in zip(rng.uniform(0, H, 300), rng.uniform(0, W, 300), rng.uniform(80, 255, 300)):
+        f += a * np.exp(-((y - cy) ** 2 + (x - cx) ** 2) / 2.0)
+    return np.clip(f + rng.normal(0, 3, (H, W)), 0, 255).astype(np.uint8)
+
+
+@pytest.mark.parametrize("quality", [5, 50, 75, 95, 100])
+@pytest.mark.parametrize("shape", [(64, 64), (97, 131), (8, 8), (1, 13)])
+def test_jpeg_decodes_as_libjpeg(tmp_path, quality, shape):
+    """Baseline grayscale JPEG: the samples equal those of PIL's decoder
+    (libjpeg-turbo, integer inverse DCT), bit for bit, at every quality and
+    for sizes that are not multiples of 8."""
+    f = particles(quality + shape[1], *shape)
+    path = tmp_path / "f.jpg"
+    Image.fromarray(f).save(path, quality=quality)
+    ref = np.asarray(Image.open(path))
+    assert np.array_equal(qlpiv.read_image(path), ref)
+
+
+@pytest.mark.parametrize("kw", [{"optimize": True}, {"restart_marker_blocks": 3},
+                                {"restart_marker_rows": 1}, {"qtables": [[1] * 64]},
+                                {"qtables": [[255] * 64]}])
+def test_jpeg_tables_and_restart_intervals(tmp_path, kw):
+    """Optimised Huffman tables, restart intervals, and extreme quantisation
+    (all 1: the widest coefficients; all 255: the largest overshoot)."""
+    f = particles(7, 75, 90)
+    f[20:40, 20:60] = 255
+    f[50:60] = 0
+    path = tmp_path / "f.jpg"
+    Image.fromarray(f).save(path, **kw)
+    ref = np.asarray(Image.open(path))
+    assert np.array_equal(qlpiv.read_image(path), ref)
+
+
 def test_unsupported_layouts_are_refused(tmp_path):
     rgb = np.zeros((8, 8, 3), np.uint8)
     for name in ("c.png", "c.tif"):
@@ -66,8 +104,11 @@ def test_unsupported_layouts_are_refused(tmp_path):
     Image.fromarray(np.zeros((8, 8), np.uint16)).save(tmp_path / "d16.png")
     with pytest.raises(qlpiv.QlpError):
         qlpiv.read_image(tmp_path / "d16.png")
-    Image.fromarray(np.zeros((8, 8), np.uint8)).save(tmp_path / "j.jpg")
-    with pytest.raises(qlpiv.QlpError):
-        qlpiv.read_image(tmp_path / "j.jpg")
+    for name, kw in (("p.jpg", {"progressive": True}), ("rgb.jpg", {})):
+        img = Image.fromarray(rgb if name == "rgb.jpg" else np.zeros((16, 16), np.uint8))
+        img.save(tmp_path / name, **kw)
+        with pytest.raises(qlpiv.QlpError) as e:
+            qlpiv.read_image(tmp_path / name)
+        assert e.value.code == qlpiv._lib.QLP_ERR_UNSUPPORTED
     with pytest.raises(qlpiv.QlpError):
         qlpiv.read_image(tmp_path / "missing.tif")
