@@ -406,15 +406,20 @@ typedef struct qlp_piv_config {
     int xcorr_form;           /* QLP_XCORR_* arithmetic of the 16, 32 and 64 px levels
                                  (128 px levels always use QLP_XCORR_EXACT) */
     int warp_order;           /* QLP_WARP_ORDER_* of the Micro, Mini and Full warps */
-    int frame_type;           /* QLP_FRAME_U8 (qlp_piv_run_dev) or QLP_FRAME_F32
-                                 (qlp_piv_run_f32_dev) */
+    int frame_type;           /* QLP_FRAME_U8 (qlp_piv_run_dev), QLP_FRAME_F32
+                                 (qlp_piv_run_f32_dev) or QLP_FRAME_U16
+                                 (qlp_piv_run_u16_dev) */
+    int frame_bits;           /* QLP_FRAME_U16: bit depth 10, 12 or 16 */
 } qlp_piv_config;
 
-/* Pixel type of a plan's frames: 8-bit (images/Image.java) or float
+/* Pixel type of a plan's frames: 8-bit (images/Image.java), float
  * (images/ImageFloat.java, what GaussianFilter2D.applyFilter(IImage, IImage)
- * :116-126 writes when frames are filtered before warping). */
+ * :116-126 writes when frames are filtered before warping), or 10/12/16-bit
+ * (images/ImageInt16.java, pixel depths Image10Bit / Image12Bit / Image16Bit of
+ * ImageFactoryEnum.create :20-31). */
 #define QLP_FRAME_U8 0
 #define QLP_FRAME_F32 1
+#define QLP_FRAME_U16 2
 
 typedef struct qlp_piv_plan qlp_piv_plan;
 
@@ -437,6 +442,12 @@ int qlp_piv_run_dev(qlp_piv_plan *plan, const uint8_t *imgsA, const uint8_t *img
  * maximum 65535) and correlates them in the exact form.  Warp modes
  * QLP_WARP_NONE and QLP_WARP_MICRO (every order). */
 int qlp_piv_run_f32_dev(qlp_piv_plan *plan, const float *imgsA, const float *imgsB);
+/* The loop for 10/12/16-bit frames (frame_type QLP_FRAME_U16; imgsA, imgsB
+ * device, pairs x H x W uint16): the frames are converted to float in plan
+ * buffers and run as float frames with the ImageInt16 rules: clipped tiles take
+ * the MatrixInt16 maximum (>= 16, MatrixInt16.java:16-41, :371-405), Micro-warped
+ * tiles the bit-depth maximum 2^bits - 1 (ImageInt16.getMaximumValue :557-559). */
+int qlp_piv_run_u16_dev(qlp_piv_plan *plan, const uint16_t *imgsA, const uint16_t *imgsB);
 /* Device pointers of a level's results (valid after the run has completed):
  * u, v (pairs x n displacements, vertical then horizontal), peaks (pairs x n x
  * n_peaks x 4: i', j', value', status), planes (pairs x n x D x D when stored
@@ -539,6 +550,12 @@ int qlp_batch_destroy(qlp_batch *b);
  * reference converts them through Java2D). */
 /* Decode 'path' into out (H x W bytes, row-major); out NULL only reports H, W. */
 int qlp_image_read(const char *path, uint8_t *out, size_t out_bytes, int *H, int *W);
+/* 10/12/16-bit frames (images/ImageInt16.readImageToBuffer :183-184, the
+ * TYPE_USHORT_GRAY raster copied as it is): 16-bit grayscale TIFF strips (either
+ * byte order; uncompressed, PackBits, LZW or Deflate; predictor 1 or 2) and
+ * 16-bit grayscale non-interlaced PNG, into out (H x W uint16).  8-bit files,
+ * which the reference widens through Java2D, return QLP_ERR_UNSUPPORTED. */
+int qlp_image_read16(const char *path, uint16_t *out, size_t out_elems, int *H, int *W);
 
 /* ---- result export -----------------------------------------------------------
  *

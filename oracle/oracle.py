@@ -62,6 +62,12 @@ def lib():
         L.or_warp_micro_side_f32.argtypes = [_f32p, c_int, c_int, c_int, c_int, c_int, c_float,
                                              c_float, c_int, _f32p]
         L.or_warp_micro_side_f32.restype = c_float
+        _u16p = np.ctypeslib.ndpointer(np.uint16, flags="C_CONTIGUOUS")
+        L.or_clip_tile_u16.argtypes = [_u16p, c_int, c_int, c_int, c_int, c_int, _f32p]
+        L.or_clip_tile_u16.restype = c_float
+        L.or_warp_micro_side_u16.argtypes = [_u16p, c_int, c_int, c_int, c_int, c_int, c_int,
+                                             c_float, c_float, c_int, _f32p]
+        L.or_warp_micro_side_u16.restype = c_float
         L.or_warp_micro.argtypes = [_u8p, c_int, c_int, c_int, c_int, c_int, c_float, c_float, _u8p]
         L.or_warp_micro_side.argtypes = [_u8p, c_int, c_int, c_int, c_int, c_int, c_float, c_float,
                                          c_int, _u8p]
@@ -341,6 +347,23 @@ def warp_micro_f32(img, top, left, N, du, dv, side=SIDE_B_SECOND):
     tile = np.zeros((N, N), np.float32)
     mx = lib().or_warp_micro_side_f32(img, img.shape[0], img.shape[1], int(top), int(left), N,
                                       float(du), float(dv), int(side), tile)
+    return tile, mx
+
+
+def clip_tile_u16(img, top, left, N):
+    """N x N window of a 10/12/16-bit frame as floats; (tile, max >= 16)."""
+    img = np.ascontiguousarray(img, np.uint16)
+    tile = np.zeros((N, N), np.float32)
+    mx = lib().or_clip_tile_u16(img, img.shape[0], img.shape[1], int(top), int(left), N, tile)
+    return tile, mx
+
+
+def warp_micro_u16(img, bits, top, left, N, du, dv, side=SIDE_B_SECOND):
+    """Micro warp of a 10/12/16-bit frame's tile: (float tile, 2^bits - 1)."""
+    img = np.ascontiguousarray(img, np.uint16)
+    tile = np.zeros((N, N), np.float32)
+    mx = lib().or_warp_micro_side_u16(img, img.shape[0], img.shape[1], int(bits), int(top), int(left),
+                                      N, float(du), float(dv), int(side), tile)
     return tile, mx
 
 

@@ -1104,9 +1104,13 @@ static int fm_round(float x) { return (int)floorf(x + 0.5f); }
 typedef struct {
     const uint8_t *u8;
     const float *f32;
+    const uint16_t *u16;
 } or_frame;
 
-static float frame_px(const or_frame *fr, size_t k) { return fr->f32 ? fr->f32[k] : (float)fr->u8[k]; }
+static float frame_px(const or_frame *fr, size_t k)
+{
+    return fr->f32 ? fr->f32[k] : (fr->u16 ? (float)fr->u16[k] : (float)fr->u8[k]);
+}
 
 static float warp_pixel_frame(const or_frame *img, int H, int W, float u, float v, short i, short j,
                               int first);
@@ -1114,7 +1118,7 @@ static float warp_pixel_frame(const or_frame *img, int H, int W, float u, float 
 static float warp_pixel_order(const uint8_t *img, int H, int W, float u, float v, short i, short j,
                               int first)
 {
-    const or_frame fr = {img, NULL};
+    const or_frame fr = {img, NULL, NULL};
     return warp_pixel_frame(&fr, H, W, u, v, i, j, first);
 }
 
@@ -1251,7 +1255,7 @@ float or_clip_tile_f32(const float *img, int H, int W, int top, int left, int N,
 float or_warp_micro_side_f32(const float *img, int H, int W, int top, int left, int N, float dispU,
                              float dispV, int side, float *tile)
 {
-    const or_frame fr = {NULL, img};
+    const or_frame fr = {NULL, img, NULL};
     const or_side g = side_geometry(side, top, left, dispU, dispV);
     const float fu = dispU - (float)floor(dispU);
     const float fv = dispV - (float)floor(dispV);
@@ -1266,6 +1270,50 @@ float or_warp_micro_side_f32(const float *img, int H, int W, int top, int left, 
                                                g.first);
         }
     return 65535.0f;
+}
+
+/* ImageInt16.clipImageMatrix (images/ImageInt16.java, MatrixInt16 tiles):
+ * the MatrixByte rules with 16-bit samples, the maximum over the in-frame
+ * pixels clamped to >= 16 (MatrixInt16.java:16-41, copySubMatrixFromArray
+ * :371-405).  Tile values are returned as floats. */
+float or_clip_tile_u16(const uint16_t *img, int H, int W, int top, int left, int N, float *tile)
+{
+    int maxv = 0;
+    for (int i = 0; i < N; i++)
+        for (int j = 0; j < N; j++) {
+            int y = top + i, x = left + j;
+            int v = 0;
+            if (y >= 0 && y < H && x >= 0 && x < W) {
+                v = img[(size_t)y * W + x];
+                if (v > maxv) maxv = v;
+            }
+            tile[i * N + j] = (float)v;
+        }
+    if (maxv < 16) maxv = 16;
+    return (float)maxv;
+}
+
+/* Micro warp of one tile of a 10/12/16-bit frame: the rounded samples; the
+ * tile maximum is the frame's bit-depth maximum (ImageInt16.getMaximumValue
+ * :557-559, ImageBitDepthEnum: 1023, 4095, 65535), returned here. */
+float or_warp_micro_side_u16(const uint16_t *img, int H, int W, int bits, int top, int left, int N,
+                             float dispU, float dispV, int side, float *tile)
+{
+    const or_frame fr = {NULL, NULL, img};
+    const or_side g = side_geometry(side, top, left, dispU, dispV);
+    const float fu = dispU - (float)floor(dispU);
+    const float fv = dispV - (float)floor(dispV);
+    for (int i = 0; i < N; i++)
+        for (int j = 0; j < N; j++) {
+            float u = fu, v = fv;
+            if (g.half) {
+                u = u / 2.0f + g.offU;
+                v = v / 2.0f + g.offV;
+            }
+            tile[i * N + j] = warp_pixel_frame(&fr, H, W, u, v, (short)(i + g.top), (short)(j + g.left),
+                                               g.first);
+        }
+    return (float)((1 << bits) - 1);
 }
 
 /* SecondImage order: the B tile shifted by floor(u), floor(v), sampled at the

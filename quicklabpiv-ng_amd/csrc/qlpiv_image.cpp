@@ -1,10 +1,12 @@
 // Frame ingest (host code): 8-bit grayscale TIFF, PNG and JPEG files to uint8
-// frames.
+// frames, and 16-bit grayscale TIFF and PNG files to uint16 frames.
 //
 // The reference reads frames with javax.imageio (jobs/ImageReaderJob.compute
 // :80-110) and, for 8-bit images, copies a TYPE_BYTE_GRAY raster as it is
 // (images/Image.readImageToBuffer :146-156).  This reader decodes exactly that
-// case: one 8-bit sample per pixel, black is zero.  Other layouts (colour,
+// case: one 8-bit sample per pixel, black is zero; and for 10/12/16-bit
+// frames (images/ImageInt16.readImageToBuffer :183-184) a TYPE_USHORT_GRAY
+// raster copied as it is: one 16-bit sample per pixel.  Other layouts (colour,
 // palette, alpha, 1/2/4/16-bit, tiles, progressive JPEG) go through Java2D
 // conversions or other decoders in the reference and are rejected with
 // QLP_ERR_UNSUPPORTED.
@@ -51,7 +53,9 @@ bool read_file(const char *path, std::vector<uint8_t> &buf)
 
 struct Image {
     int H = 0, W = 0;
-    std::vector<uint8_t> px;
+    bool wide = false;           // 16-bit samples requested (ImageInt16 frames)
+    std::vector<uint8_t> px;     // 8-bit frames
+    std::vector<uint16_t> px16;  // 16-bit frames
 };
 
 // ---- zlib -----------------------------------------------------------------------
@@ -208,19 +212,23 @@ int read_tiff(const std::vector<uint8_t> &b, Image &img)
     }
     if (!r.ok || W == 0 || H == 0 || offs.empty() || offs.size() != counts.size())
         return fail(QLP_ERR_ARG, "image: malformed TIFF directory");
-    if (bps != 8 || spp != 1 || photo != 1 || planar != 1 || tiled || (pred != 1 && pred != 2) ||
+    if (bps != (img.wide ? 16u : 8u) || spp != 1 || photo != 1 || planar != 1 || tiled ||
+        (pred != 1 && pred != 2) ||
         (comp != 1 && comp != 5 && comp != 8 && comp != 32946 && comp != 32773))
         return fail(QLP_ERR_UNSUPPORTED,
-                    "image: only 8-bit grayscale (black is zero) TIFF strips, uncompressed, "
-                    "PackBits, LZW or Deflate, are read");
+                    img.wide ? "image: only 16-bit grayscale (black is zero) TIFF strips, uncompressed, "
+                               "PackBits, LZW or Deflate, are read as 16-bit frames"
+                             : "image: only 8-bit grayscale (black is zero) TIFF strips, uncompressed, "
+                               "PackBits, LZW or Deflate, are read");
     if (rps > H) rps = H;
+    const size_t bpp = img.wide ? 2 : 1;
     img.W = (int)W;
     img.H = (int)H;
     img.px.clear();
-    img.px.reserve((size_t)W * H);
-    for (size_t s = 0; s < offs.size() && img.px.size() < (size_t)W * H; s++) {
-        const size_t rows = std::min<size_t>(rps, H - img.px.size() / W);
-        const size_t expect = rows * W;
+    img.px.reserve((size_t)W * H * bpp);
+    for (size_t s = 0; s < offs.size() && img.px.size() < (size_t)W * H * bpp; s++) {
+        const size_t rows = std::min<size_t>(rps, H - img.px.size() / (W * bpp));
+        const size_t expect = rows * W * bpp;
         if ((size_t)offs[s] + counts[s] > b.size()) return fail(QLP_ERR_ARG, "image: TIFF strip out of file");
         const uint8_t *src = b.data() + offs[s];
         std::vector<uint8_t> strip;
@@ -237,13 +245,25 @@ int read_tiff(const std::vector<uint8_t> &b, Image &img)
         }
         if (!good) return fail(QLP_ERR_ARG, "image: corrupt TIFF strip");
         strip.resize(expect);
-        if (pred == 2)
+        if (pred == 2 && !img.wide)
             for (size_t y = 0; y < rows; y++)
                 for (size_t x = 1; x < W; x++)
                     strip[y * W + x] = (uint8_t)(strip[y * W + x] + strip[y * W + x - 1]);
         img.px.insert(img.px.end(), strip.begin(), strip.end());
     }
-    if (img.px.size() != (size_t)W * H) return fail(QLP_ERR_ARG, "image: TIFF strips too short");
+    if (img.px.size() != (size_t)W * H * bpp) return fail(QLP_ERR_ARG, "image: TIFF strips too short");
+    if (img.wide) {
+        // samples in the file's byte order; horizontal differencing on whole samples
+        img.px16.resize((size_t)W * H);
+        for (size_t k = 0; k < img.px16.size(); k++)
+            img.px16[k] = r.le ? (uint16_t)(img.px[2 * k] | img.px[2 * k + 1] << 8)
+                               : (uint16_t)(img.px[2 * k] << 8 | img.px[2 * k + 1]);
+        if (pred == 2)
+            for (size_t y = 0; y < H; y++)
+                for (size_t x = 1; x < W; x++)
+                    img.px16[y * W + x] = (uint16_t)(img.px16[y * W + x] + img.px16[y * W + x - 1]);
+        img.px.clear();
+    }
     return QLP_OK;
 }
 
@@ -266,9 +286,11 @@ int read_png(const std::vector<uint8_t> &b, Image &img)
             if (len < 13) return fail(QLP_ERR_ARG, "image: bad PNG header");
             W = be32(d);
             H = be32(d + 4);
-            if (d[8] != 8 || d[9] != 0 || d[10] != 0 || d[11] != 0 || d[12] != 0)
+            if (d[8] != (img.wide ? 16 : 8) || d[9] != 0 || d[10] != 0 || d[11] != 0 || d[12] != 0)
                 return fail(QLP_ERR_UNSUPPORTED,
-                            "image: only 8-bit grayscale, non-interlaced PNG files are read");
+                            img.wide ? "image: only 16-bit grayscale, non-interlaced PNG files are "
+                                       "read as 16-bit frames"
+                                     : "image: only 8-bit grayscale, non-interlaced PNG files are read");
             header = true;
         } else if (!memcmp(type, "IDAT", 4)) {
             z.insert(z.end(), d, d + len);
@@ -278,19 +300,22 @@ int read_png(const std::vector<uint8_t> &b, Image &img)
         o += 12 + len;
     }
     if (!header || W == 0 || H == 0) return fail(QLP_ERR_ARG, "image: PNG without header");
+    // rows of RB bytes; the filters work on bytes, a neighbour is one sample (bpp bytes) back
+    const size_t bpp = img.wide ? 2 : 1, RB = (size_t)W * bpp;
     std::vector<uint8_t> raw;
-    if (!inflate_all(z.data(), z.size(), raw, (size_t)H * (W + 1)))
+    if (!inflate_all(z.data(), z.size(), raw, (size_t)H * (RB + 1)))
         return fail(QLP_ERR_ARG, "image: corrupt PNG data");
     img.W = (int)W;
     img.H = (int)H;
-    img.px.assign((size_t)W * H, 0);
+    img.px.assign((size_t)H * RB, 0);
     for (size_t y = 0; y < H; y++) {
-        const uint8_t f = raw[y * (W + 1)];
-        const uint8_t *in = &raw[y * (W + 1) + 1];
-        uint8_t *row = &img.px[y * W];
-        const uint8_t *up = y ? &img.px[(y - 1) * W] : nullptr;
-        for (size_t x = 0; x < W; x++) {
-            const int a = x ? row[x - 1] : 0, bb = up ? up[x] : 0, c = (x && up) ? up[x - 1] : 0;
+        const uint8_t f = raw[y * (RB + 1)];
+        const uint8_t *in = &raw[y * (RB + 1) + 1];
+        uint8_t *row = &img.px[y * RB];
+        const uint8_t *up = y ? &img.px[(y - 1) * RB] : nullptr;
+        for (size_t x = 0; x < RB; x++) {
+            const int a = x >= bpp ? row[x - bpp] : 0, bb = up ? up[x] : 0,
+                      c = (x >= bpp && up) ? up[x - bpp] : 0;
             int p;
             switch (f) {
             case 0: p = 0; break;
@@ -306,6 +331,12 @@ int read_png(const std::vector<uint8_t> &b, Image &img)
             }
             row[x] = (uint8_t)(in[x] + p);
         }
+    }
+    if (img.wide) {  // PNG samples are big-endian
+        img.px16.resize((size_t)W * H);
+        for (size_t k = 0; k < img.px16.size(); k++)
+            img.px16[k] = (uint16_t)(img.px[2 * k] << 8 | img.px[2 * k + 1]);
+        img.px.clear();
     }
     return QLP_OK;
 }
@@ -604,13 +635,29 @@ int decode(const char *path, Image &img)
     if (b.size() >= 8 && !memcmp(b.data(), png_sig, 8)) return read_png(b, img);
     if (b.size() >= 8 && ((b[0] == 'I' && b[1] == 'I') || (b[0] == 'M' && b[1] == 'M')))
         return read_tiff(b, img);
-    if (b.size() >= 4 && b[0] == 0xff && b[1] == 0xd8 && b[2] == 0xff) return read_jpeg(b, img);
+    if (b.size() >= 4 && b[0] == 0xff && b[1] == 0xd8 && b[2] == 0xff) {
+        if (img.wide) return fail(QLP_ERR_UNSUPPORTED, "image: JPEG files hold 8-bit samples");
+        return read_jpeg(b, img);
+    }
     return fail(QLP_ERR_UNSUPPORTED, std::string("image: unsupported file format: ") + path);
 }
 
 }  // namespace
 
 extern "C" {
+
+int qlp_image_read16(const char *path, uint16_t *out, size_t out_elems, int *H, int *W)
+{
+    Image img;
+    img.wide = true;
+    if (int rc = decode(path, img)) return rc;
+    if (H) *H = img.H;
+    if (W) *W = img.W;
+    if (!out) return QLP_OK;
+    if (out_elems < img.px16.size()) return fail(QLP_ERR_ARG, "image: output buffer too small");
+    memcpy(out, img.px16.data(), img.px16.size() * sizeof(uint16_t));
+    return QLP_OK;
+}
 
 int qlp_image_read(const char *path, uint8_t *out, size_t out_bytes, int *H, int *W)
 {

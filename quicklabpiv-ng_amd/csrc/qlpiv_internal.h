@@ -151,11 +151,15 @@ int launch_warp_micro(Context *ctx, const uint8_t *img, int H, int W, int n, int
 // Tiles of float frames (ImageFloat): a clip (at (top, left), or shifted by the
 // rounded (u, v) when u and v are given) with the MatrixFloat maximum, and the
 // Micro warp of one side with the maximum 65535.
+// clamp16: the MatrixInt16 maximum (>= 16) instead of the MatrixFloat one (0 -> 1);
+// wmax: the maximum of a warped tile (65535 for ImageFloat, 2^bits - 1 for ImageInt16).
 int launch_clip_f32(Context *ctx, const float *img, int H, int W, int n, int N, const int *top,
-                    const int *left, const float *du, const float *dv, float *tiles, float *maxv);
+                    const int *left, const float *du, const float *dv, float *tiles, float *maxv,
+                    int clamp16 = 0);
 int launch_warp_micro_f32(Context *ctx, const float *img, int H, int W, int n, int N, const int *top,
                           const int *left, const float *u, const float *v, float *tiles, float *maxv,
-                          int side = 0);
+                          int side = 0, float wmax = 65535.0f);
+int launch_u16_to_f32(Context *ctx, const uint16_t *in, float *out, size_t n);
 // Spline-field warps (qlpiv_splinewarp.hip).
 int launch_warp_mini(Context *ctx, int pairs, const uint8_t *imgs, int H, int W, int nI, int nJ,
                      int N, const int *top, const int *left, const float *u, const float *v,

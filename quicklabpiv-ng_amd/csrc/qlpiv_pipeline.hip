@@ -152,8 +152,9 @@ struct qlp_piv_plan {
     float *mA = nullptr, *mB = nullptr;
     void *fwork = nullptr;
     size_t tiles_bytes = 0, fwork_bytes = 0;
-    // float frames: the A and B float tiles of every level (sized for the largest)
-    float *fA = nullptr, *fB = nullptr;
+    // float and 10/12/16-bit frames: the A and B float tiles of every level
+    // (sized for the largest); 10/12/16-bit frames converted to float
+    float *fA = nullptr, *fB = nullptr, *fframes = nullptr;
     size_t ftiles_bytes = 0;
     // IAs of the largest IA-128 pass: sizes the context's tile-slot workspace
     int big_ias = 0;
@@ -189,10 +190,10 @@ static void plan_free(qlp_piv_plan *p)
             if (b) (void)hipFree(b);
     }
     for (void *b : {(void *)p->tA, (void *)p->tB, (void *)p->wimg, (void *)p->mA, (void *)p->mB,
-                    p->fwork, (void *)p->fA, (void *)p->fB})
+                    p->fwork, (void *)p->fA, (void *)p->fB, (void *)p->fframes})
         if (b) (void)hipFree(b);
     p->tA = p->tB = p->wimg = nullptr;
-    p->mA = p->mB = p->fA = p->fB = nullptr;
+    p->mA = p->mB = p->fA = p->fB = p->fframes = nullptr;
     p->fwork = nullptr;
     for (hipEvent_t e : p->ev) (void)hipEventDestroy(e);
     p->ev.clear();
@@ -295,13 +296,17 @@ static int plan_layout(int H, int W, int pairs, const qlp_piv_config *cfg, qlp_p
         return fail(QLP_ERR_ARG, "piv_plan_create: unknown velocity inheritance strategy");
     if (cfg->xcorr_form < QLP_XCORR_EXACT || cfg->xcorr_form > QLP_XCORR_MFMA_BF16_SINGLE)
         return fail(QLP_ERR_ARG, "piv_plan_create: unknown correlation form");
-    if (cfg->frame_type != QLP_FRAME_U8 && cfg->frame_type != QLP_FRAME_F32)
+    if (cfg->frame_type != QLP_FRAME_U8 && cfg->frame_type != QLP_FRAME_F32 &&
+        cfg->frame_type != QLP_FRAME_U16)
         return fail(QLP_ERR_ARG, "piv_plan_create: unknown frame type");
-    const bool f32 = cfg->frame_type == QLP_FRAME_F32;
+    if (cfg->frame_type == QLP_FRAME_U16 && cfg->frame_bits != 10 && cfg->frame_bits != 12 &&
+        cfg->frame_bits != 16)
+        return fail(QLP_ERR_ARG, "piv_plan_create: 10/12/16-bit frames need a bit depth of 10, 12 or 16");
+    const bool f32 = cfg->frame_type != QLP_FRAME_U8;
     if (f32 && cfg->warp_mode != QLP_WARP_NONE && cfg->warp_mode != QLP_WARP_MICRO)
-        return fail(QLP_ERR_UNSUPPORTED, "piv_plan_create: float frames take no warp or the Micro warp");
+        return fail(QLP_ERR_UNSUPPORTED, "piv_plan_create: float and 10/12/16-bit frames take no warp or the Micro warp");
     if (f32 && cfg->xcorr_form != QLP_XCORR_EXACT)
-        return fail(QLP_ERR_UNSUPPORTED, "piv_plan_create: float frames take the exact correlation form");
+        return fail(QLP_ERR_UNSUPPORTED, "piv_plan_create: float and 10/12/16-bit frames take the exact correlation form");
     const bool valid = cfg->valid.n_stages > 0;
     p->H = H;
     p->W = W;
@@ -339,6 +344,7 @@ static int plan_layout(int H, int W, int pairs, const qlp_piv_config *cfg, qlp_p
     }
     if (cfg->warp_mode == QLP_WARP_FULL && p->fwork_bytes)
         need += (size_t)full_frames(*cfg) * pairs * H * W + p->fwork_bytes;
+    if (cfg->frame_type == QLP_FRAME_U16) need += 2 * sizeof(float) * (size_t)pairs * H * W;
     return QLP_OK;
 }
 
@@ -410,7 +416,7 @@ int qlp_piv_plan_create(qlp_ctx *h, int H, int W, int pairs, const qlp_piv_confi
         if (!rc && l > 0 && cfg->inherit_mode != QLP_INHERIT_DIRECT)
             rc = prepare_inherit(cfg->inherit_mode, p->levels[l - 1], L, need);
     }
-    const bool f32 = cfg->frame_type == QLP_FRAME_F32;
+    const bool f32 = cfg->frame_type != QLP_FRAME_U8;
     if (!rc && ((tiles_route(*cfg) && p->tiles_bytes) || f32)) {
         size_t mx = 0;
         for (const Level &L : p->levels) mx = std::max(mx, (size_t)pairs * L.n);
@@ -425,6 +431,8 @@ int qlp_piv_plan_create(qlp_ctx *h, int H, int W, int pairs, const qlp_piv_confi
         if (!rc) rc = check_hip(hipMalloc(&p->mA, mx * 4), "hipMalloc plan");
         if (!rc) rc = check_hip(hipMalloc(&p->mB, mx * 4), "hipMalloc plan");
     }
+    if (!rc && cfg->frame_type == QLP_FRAME_U16)
+        rc = check_hip(hipMalloc(&p->fframes, 2 * sizeof(float) * (size_t)pairs * H * W), "hipMalloc plan");
     if (!rc && cfg->warp_mode == QLP_WARP_FULL && p->fwork_bytes) {
         rc = check_hip(hipMalloc(&p->wimg, (size_t)full_frames(*cfg) * pairs * H * W), "hipMalloc plan");
         if (!rc) rc = check_hip(hipMalloc(&p->fwork, p->fwork_bytes), "hipMalloc plan");
@@ -484,6 +492,10 @@ static int float_tiles(qlp_piv_plan *p, const Level &L, size_t l, const float *i
     const int order = c.warp_order;
     const bool micro = l > 0 && c.warp_mode == QLP_WARP_MICRO;
     const size_t tN = (size_t)L.N * L.N, frame = (size_t)p->H * p->W;
+    // ImageInt16 rules for 10/12/16-bit frames, ImageFloat ones otherwise
+    const bool i16 = c.frame_type == QLP_FRAME_U16;
+    const int clamp16 = i16 ? 1 : 0;
+    const float wmax = i16 ? (float)((1 << c.frame_bits) - 1) : 65535.0f;
     int rc = QLP_OK;
     for (int k = 0; k < p->pairs && !rc; k++) {
         const float *a = imgsA + k * frame, *b = imgsB + k * frame;
@@ -492,17 +504,19 @@ static int float_tiles(qlp_piv_plan *p, const Level &L, size_t l, const float *i
         const float *u = L.u + o, *v = L.v + o;
         if (micro && order != QLP_WARP_ORDER_SECOND)
             rc = launch_warp_micro_f32(ctx, a, p->H, p->W, L.n, L.N, L.top, L.left, u, v, ta, ma,
-                                       order == QLP_WARP_ORDER_FIRST ? SIDE_A_FIRST : SIDE_A_BOTH);
+                                       order == QLP_WARP_ORDER_FIRST ? SIDE_A_FIRST : SIDE_A_BOTH, wmax);
         else
-            rc = launch_clip_f32(ctx, a, p->H, p->W, L.n, L.N, L.top, L.left, nullptr, nullptr, ta, ma);
+            rc = launch_clip_f32(ctx, a, p->H, p->W, L.n, L.N, L.top, L.left, nullptr, nullptr, ta, ma,
+                                 clamp16);
         if (rc) break;
         if (micro && order != QLP_WARP_ORDER_FIRST)
             rc = launch_warp_micro_f32(ctx, b, p->H, p->W, L.n, L.N, L.top, L.left, u, v, tb, mb,
-                                       order == QLP_WARP_ORDER_SECOND ? SIDE_B_SECOND : SIDE_B_BOTH);
+                                       order == QLP_WARP_ORDER_SECOND ? SIDE_B_SECOND : SIDE_B_BOTH, wmax);
         else if (l > 0 && c.warp_mode == QLP_WARP_NONE)
-            rc = launch_clip_f32(ctx, b, p->H, p->W, L.n, L.N, L.top, L.left, u, v, tb, mb);
+            rc = launch_clip_f32(ctx, b, p->H, p->W, L.n, L.N, L.top, L.left, u, v, tb, mb, clamp16);
         else
-            rc = launch_clip_f32(ctx, b, p->H, p->W, L.n, L.N, L.top, L.left, nullptr, nullptr, tb, mb);
+            rc = launch_clip_f32(ctx, b, p->H, p->W, L.n, L.N, L.top, L.left, nullptr, nullptr, tb, mb,
+                                 clamp16);
     }
     return rc;
 }
@@ -514,7 +528,7 @@ int qlp_piv_run_dev(qlp_piv_plan *p, const uint8_t *imgsA, const uint8_t *imgsB)
 {
     if (!p || !imgsA || !imgsB) return fail(QLP_ERR_ARG, "piv_run: null argument");
     if (p->cfg.frame_type != QLP_FRAME_U8)
-        return fail(QLP_ERR_ARG, "piv_run: the plan was made for float frames (qlp_piv_run_f32_dev)");
+        return fail(QLP_ERR_ARG, "piv_run: the plan was not made for 8-bit frames");
     return run_levels(p, imgsA, imgsB, nullptr, nullptr);
 }
 
@@ -522,8 +536,20 @@ int qlp_piv_run_f32_dev(qlp_piv_plan *p, const float *imgsA, const float *imgsB)
 {
     if (!p || !imgsA || !imgsB) return fail(QLP_ERR_ARG, "piv_run: null argument");
     if (p->cfg.frame_type != QLP_FRAME_F32)
-        return fail(QLP_ERR_ARG, "piv_run: the plan was made for 8-bit frames (qlp_piv_run_dev)");
+        return fail(QLP_ERR_ARG, "piv_run: the plan was not made for float frames");
     return run_levels(p, nullptr, nullptr, imgsA, imgsB);
+}
+
+int qlp_piv_run_u16_dev(qlp_piv_plan *p, const uint16_t *imgsA, const uint16_t *imgsB)
+{
+    if (!p || !imgsA || !imgsB) return fail(QLP_ERR_ARG, "piv_run: null argument");
+    if (p->cfg.frame_type != QLP_FRAME_U16)
+        return fail(QLP_ERR_ARG, "piv_run: the plan was not made for 10/12/16-bit frames");
+    const size_t n = (size_t)p->pairs * p->H * p->W;
+    int rc = launch_u16_to_f32(p->ctx, imgsA, p->fframes, n);
+    if (!rc) rc = launch_u16_to_f32(p->ctx, imgsB, p->fframes + n, n);
+    if (rc) return rc;
+    return run_levels(p, nullptr, nullptr, p->fframes, p->fframes + n);
 }
 
 static int run_levels(qlp_piv_plan *p, const uint8_t *imgsA, const uint8_t *imgsB,

@@ -14,6 +14,8 @@
 //   maximum 255 (doBiLinearWarp :298).
 // Rows of the image are read by consecutive lanes (coalesced); one work-group
 // per tile.
+#include <algorithm>
+
 #include "qlpiv_device.h"
 #include "qlpiv_internal.h"
 
@@ -87,7 +89,7 @@ __global__ __launch_bounds__(256) void clip_f32_kernel(const float *__restrict__
                                                       const int *__restrict__ left,
                                                       const float *__restrict__ du,
                                                       const float *__restrict__ dv,
-                                                      float *__restrict__ tiles,
+                                                      int clamp16, float *__restrict__ tiles,
                                                       float *__restrict__ maxv)
 {
     __shared__ float s_max[4];
@@ -120,21 +122,41 @@ __global__ __launch_bounds__(256) void clip_f32_kernel(const float *__restrict__
         float m = 0.0f;
         for (int w = 0; w < (int)(blockDim.x >> 6); w++)
             if (s_max[w] > m) m = s_max[w];
-        maxv[t] = m == 0.0f ? 1.0f : m;
+        maxv[t] = clamp16 ? (m < 16.0f ? 16.0f : m) : (m == 0.0f ? 1.0f : m);
     }
+}
+
+// 10/12/16-bit frames (ImageInt16) as float frames: every sample converted
+// exactly.
+__global__ __launch_bounds__(256) void u16_to_f32_kernel(const uint16_t *__restrict__ in,
+                                                        float *__restrict__ out, size_t n)
+{
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+         k += (size_t)gridDim.x * blockDim.x)
+        out[k] = (float)in[k];
+}
+
+int launch_u16_to_f32(Context *ctx, const uint16_t *in, float *out, size_t n)
+{
+    if (n == 0) return QLP_OK;
+    const size_t blocks = std::min<size_t>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(u16_to_f32_kernel, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, in, out, n);
+    return check_hip(hipGetLastError(), "frame conversion launch");
 }
 
 // WARP_MICRO_F32: the Micro warp of one side of a float frame's tile, sampled
 // as for 8-bit frames (readPixel of ImageFloat :531-533); the rounded values
-// can exceed 255, and the tile carries the ImageFloat maximum 65535
-// (getMaximumValue :541-543, doBiLinearWarp :298).
+// can exceed 255, and the tile carries the frame's maximum wmax
+// (doBiLinearWarp :298): 65535 for ImageFloat (getMaximumValue :541-543), the
+// bit-depth maximum for ImageInt16 (:557-559).  With clamp16 the clip takes
+// the MatrixInt16 maximum instead (>= 16, MatrixInt16.java:16-41, :371-405).
 __global__ __launch_bounds__(256) void warp_micro_f32_kernel(const float *__restrict__ img, int H,
                                                             int W, int n, int N,
                                                             const int *__restrict__ top,
                                                             const int *__restrict__ left,
                                                             const float *__restrict__ du,
                                                             const float *__restrict__ dv, int side,
-                                                            float *__restrict__ tiles,
+                                                            float wmax, float *__restrict__ tiles,
                                                             float *__restrict__ maxv)
 {
     const int t = blockIdx.x;
@@ -146,26 +168,27 @@ __global__ __launch_bounds__(256) void warp_micro_f32_kernel(const float *__rest
         const int i = idx / N, j = idx - (idx / N) * N;
         tiles[(size_t)t * N * N + idx] = side_sample(img, H, W, g, ru, rv, i, j);
     }
-    if (threadIdx.x == 0 && maxv) maxv[t] = 65535.0f;
+    if (threadIdx.x == 0 && maxv) maxv[t] = wmax;
 }
 
 int launch_clip_f32(Context *ctx, const float *img, int H, int W, int n, int N, const int *top,
-                    const int *left, const float *du, const float *dv, float *tiles, float *maxv)
+                    const int *left, const float *du, const float *dv, float *tiles, float *maxv,
+                    int clamp16)
 {
     if (n == 0) return QLP_OK;
     if ((du == nullptr) != (dv == nullptr)) return fail(QLP_ERR_ARG, "clip: u and v go together");
     hipLaunchKernelGGL(clip_f32_kernel, dim3(n), dim3(256), 0, ctx->stream, img, H, W, n, N, top, left,
-                       du, dv, tiles, maxv);
+                       du, dv, clamp16, tiles, maxv);
     return check_hip(hipGetLastError(), "float clip kernel launch");
 }
 
 int launch_warp_micro_f32(Context *ctx, const float *img, int H, int W, int n, int N, const int *top,
                           const int *left, const float *u, const float *v, float *tiles, float *maxv,
-                          int side)
+                          int side, float wmax)
 {
     if (n == 0) return QLP_OK;
     hipLaunchKernelGGL(warp_micro_f32_kernel, dim3(n), dim3(256), 0, ctx->stream, img, H, W, n, N,
-                       top, left, u, v, side, tiles, maxv);
+                       top, left, u, v, side, wmax, tiles, maxv);
     return check_hip(hipGetLastError(), "float warp kernel launch");
 }
 

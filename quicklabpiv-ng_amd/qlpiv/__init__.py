@@ -26,7 +26,7 @@ from ._lib import (QLP_NORM_A_MAX, QLP_NORM_PER_TILE, QLP_PEAK_FROM_CENTER, QLP_
 
 __all__ = [
     "Batch", "Context", "CrossCorrelationJob", "JobResultEnum", "PivPlan", "QlpError", "Server",
-    "VelocityExporter", "read_image",
+    "VelocityExporter", "read_image", "read_image16",
     "piv_config", "plan_estimate",
     "VectorValidatorJob", "build",
     "QLP_NORM_PER_TILE", "QLP_NORM_A_MAX", "QLP_PEAK_FROM_CENTER", "QLP_PEAK_MULTI",
@@ -561,6 +561,7 @@ def piv_config(**config):
     cfg.xcorr_form = config.get("xcorr_form", QLP_XCORR_EXACT)
     cfg.warp_order = config.get("warp_order", _lib.QLP_WARP_ORDER_SECOND)
     cfg.frame_type = config.get("frame_type", _lib.QLP_FRAME_U8)
+    cfg.frame_bits = config.get("frame_bits", 16 if cfg.frame_type == _lib.QLP_FRAME_U16 else 0)
     return cfg
 
 
@@ -574,7 +575,7 @@ def plan_estimate(H, W, pairs=1, **config):
 
 
 def read_image(path):
-    """An 8-bit grayscale TIFF or PNG frame as an H x W uint8 array
+    """An 8-bit grayscale TIFF, PNG or JPEG frame as an H x W uint8 array
     (qlp_image_read: ImageReaderJob for 8-bit images)."""
     lib = _lib.load()
     H, W = ctypes.c_int(), ctypes.c_int()
@@ -582,6 +583,18 @@ def read_image(path):
     check(lib.qlp_image_read(p, None, 0, ctypes.byref(H), ctypes.byref(W)))
     out = np.empty((H.value, W.value), np.uint8)
     check(lib.qlp_image_read(p, _ptr(out), out.nbytes, ctypes.byref(H), ctypes.byref(W)))
+    return out
+
+
+def read_image16(path):
+    """A 16-bit grayscale TIFF or PNG frame as an H x W uint16 array
+    (qlp_image_read16: ImageReaderJob for 10/12/16-bit images)."""
+    lib = _lib.load()
+    H, W = ctypes.c_int(), ctypes.c_int()
+    p = str(path).encode()
+    check(lib.qlp_image_read16(p, None, 0, ctypes.byref(H), ctypes.byref(W)))
+    out = np.empty((H.value, W.value), np.uint16)
+    check(lib.qlp_image_read16(p, _ptr(out), out.size, ctypes.byref(H), ctypes.byref(W)))
     return out
 
 
@@ -832,10 +845,13 @@ class PivPlan:
             pass
 
     def run(self, imgsA, imgsB):
-        """Enqueue all levels (device tensors pairs x H x W: uint8, or float32
-        for a plan made with frame_type QLP_FRAME_F32)."""
+        """Enqueue all levels (device tensors pairs x H x W: uint8, float32 for a
+        plan made with frame_type QLP_FRAME_F32, 16-bit samples (torch.uint16 or
+        int16 holding them) for QLP_FRAME_U16)."""
         if self.cfg.frame_type == _lib.QLP_FRAME_F32:
             check(self.ctx.lib.qlp_piv_run_f32_dev(self.h, _ptr(imgsA), _ptr(imgsB)))
+        elif self.cfg.frame_type == _lib.QLP_FRAME_U16:
+            check(self.ctx.lib.qlp_piv_run_u16_dev(self.h, _ptr(imgsA), _ptr(imgsB)))
         else:
             check(self.ctx.lib.qlp_piv_run_dev(self.h, _ptr(imgsA), _ptr(imgsB)))
 

@@ -71,6 +71,7 @@ SIGNATURES = {
                                 ctypes.POINTER(_i), _vp, _vp]),
     "qlp_piv_run_dev": (_i, [_vp, _vp, _vp]),
     "qlp_piv_run_f32_dev": (_i, [_vp, _vp, _vp]),
+    "qlp_piv_run_u16_dev": (_i, [_vp, _vp, _vp]),
     "qlp_piv_result_dev": (_i, [_vp, _i, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
                                 ctypes.POINTER(_vp), ctypes.POINTER(_vp)]),
     "qlp_piv_validation_dev": (_i, [_vp, _i, ctypes.POINTER(_vp), ctypes.POINTER(_vp)]),
@@ -99,6 +100,7 @@ SIGNATURES = {
     "qlp_warp_images_dev": (_i, [_vp, _i, _vp, _vp, _i, _i, _i, _i, _i, _i, _vp, _vp, _vp, _vp,
                                  _vp, _vp]),
     "qlp_image_read": (_i, [ctypes.c_char_p, _vp, _sz, ctypes.POINTER(_i), ctypes.POINTER(_i)]),
+    "qlp_image_read16": (_i, [ctypes.c_char_p, _vp, _sz, ctypes.POINTER(_i), ctypes.POINTER(_i)]),
     "qlp_export_open": (_i, [ctypes.c_char_p, _vp, ctypes.c_char_p, ctypes.POINTER(_vp)]),
     "qlp_export_frame": (_i, [_vp, _i, _i, _i, _i, _vp, _i, _i, _vp, _vp, _vp]),
     "qlp_export_close": (_i, [_vp]),
@@ -128,7 +130,7 @@ QLP_WARP_ORDER_SECOND = 0
 QLP_WARP_ORDER_FIRST = 1
 QLP_WARP_ORDER_BOTH = 2
 QLP_SIDE_B_SECOND, QLP_SIDE_A_FIRST, QLP_SIDE_A_BOTH, QLP_SIDE_B_BOTH = 0, 1, 2, 3
-QLP_FRAME_U8, QLP_FRAME_F32 = 0, 1
+QLP_FRAME_U8, QLP_FRAME_F32, QLP_FRAME_U16 = 0, 1, 2
 
 
 QLP_INHERIT_DIRECT = 0
@@ -195,7 +197,7 @@ class PivConfig(ctypes.Structure):
                 ("n_peaks", ctypes.c_int), ("peak_kernel_width", ctypes.c_int),
                 ("valid", ValidConfig), ("inherit_mode", ctypes.c_int),
                 ("xcorr_form", ctypes.c_int), ("warp_order", ctypes.c_int),
-                ("frame_type", ctypes.c_int)]
+                ("frame_type", ctypes.c_int), ("frame_bits", ctypes.c_int)]
 
 
 class ServerConfig(ctypes.Structure):

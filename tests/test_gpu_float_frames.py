@@ -1,4 +1,5 @@
-"""Float tiles and float frames (frames filtered before warping) on the GPU.
+"""Float tiles, float frames (frames filtered before warping) and 10/12/16-bit
+frames on the GPU.
 
 GaussianFilter2D.applyFilter(IImage, IImage) (images/filters/GaussianFilter2D
 .java:116-126, JavaImageFilterJob.compute :50-53) turns an 8-bit frame into an
@@ -186,6 +187,76 @@ def test_float_frame_plan_against_oracle(ctx, fpair, warp, order):
     plan.close()
 
 
+@pytest.mark.parametrize("bits,warp,order", [(12, 1, 0), (16, 1, 2), (10, 0, 0)])
+def test_u16_frame_plan_against_oracle(ctx, bits, warp, order):
+    """10/12/16-bit frames (ImageInt16): clipped tiles take the MatrixInt16
+    maximum (>= 16), Micro-warped tiles the bit-depth maximum; every level
+    equals the oracle from the same inherited displacements."""
+    import torch
+
+    import qlpiv
+    from qlpiv.synth import particle_pair
+
+    a8, b8, _ = particle_pair(192, 192, seed=50 + bits, shift=(2.3, -1.2), core_radius=50.0,
+                              u_max=3.0)
+    scale = (1 << bits) - 1
+    rng = np.random.default_rng(bits)
+    ha = (a8.astype(np.float64) / 255.0 * scale + rng.uniform(0, 3, a8.shape)).clip(0, scale)
+    hb = (b8.astype(np.float64) / 255.0 * scale + rng.uniform(0, 3, b8.shape)).clip(0, scale)
+    ha, hb = ha.astype(np.uint16), hb.astype(np.uint16)
+    ha[:40, :40] = 3  # a dim corner: maxima below 16 are clamped
+    hb[:40, :40] = 5
+    da = torch.from_numpy(ha.view(np.int16)).cuda().unsqueeze(0).contiguous()
+    db = torch.from_numpy(hb.view(np.int16)).cuda().unsqueeze(0).contiguous()
+    plan = qlpiv.PivPlan(ctx, 192, 192, 1, ia_start=64, ia_end=16, overlap_factor=0.5,
+                         warp_mode=warp, warp_order=order, subpix_kind=qlpiv.QLP_SUBPIX_GUO,
+                         subpix_pixels=3, subpix_iterations=20, keep_planes=2,
+                         frame_type=qlpiv._lib.QLP_FRAME_U16, frame_bits=bits)
+    say(f"u16-frame plan bits={bits} warp={warp} order={order}")
+    plan.run(da, db)
+    sides_a = {1: O.SIDE_A_FIRST, 2: O.SIDE_A_BOTH}
+    sides_b = {0: O.SIDE_B_SECOND, 2: O.SIDE_B_BOTH}
+    prev_u = prev_v = None
+    for lv, L in enumerate(plan.levels):
+        u, v, pk = (x[0] for x in plan.result(lv))
+        planes = plane_copy(plan, lv)
+        N, nI, nJ = L["ia"], L["nI"], L["nJ"]
+        if lv == 0:
+            uin = np.zeros((nI, nJ), np.float32)
+            vin = np.zeros((nI, nJ), np.float32)
+        else:
+            uin = direct_inherit(prev_u, nI, nJ, round_it=(warp == 0))
+            vin = direct_inherit(prev_v, nI, nJ, round_it=(warp == 0))
+        c = np.float32((2 * N - 1) // 2)
+        for i in range(nI):
+            for j in range(nJ):
+                t, l0 = int(L["top"][i, j]), int(L["left"][i, j])
+                if lv > 0 and warp and order in sides_a:
+                    tA, mA = O.warp_micro_u16(ha, bits, t, l0, N, uin[i, j], vin[i, j], sides_a[order])
+                else:
+                    tA, mA = O.clip_tile_u16(ha, t, l0, N)
+                if lv > 0 and warp and order in sides_b:
+                    tB, mB = O.warp_micro_u16(hb, bits, t, l0, N, uin[i, j], vin[i, j], sides_b[order])
+                elif lv > 0 and not warp:
+                    tb = int(np.floor(np.float32(t) + uin[i, j] + np.float32(0.5)))
+                    lb = int(np.floor(np.float32(l0) + vin[i, j] + np.float32(0.5)))
+                    tB, mB = O.clip_tile_u16(hb, tb, lb, N)
+                else:
+                    tB, mB = O.clip_tile_u16(hb, t, l0, N)
+                plane = O.xcorr_javagpu(tA, tB, mA, mB)
+                if (i * nJ + j) % 4 == 0:
+                    assert np.array_equal(planes[i, j], plane), (lv, i, j)
+                rp = O.find_max_from_center(plane)
+                rs = O.guo(plane, 3, 20, rp[3], rp[:3])
+                eu = np.float32(uin[i, j] + np.float32(rs[0] - c))
+                ev = np.float32(vin[i, j] + np.float32(rs[1] - c))
+                assert int(pk[i, j, 3]) == int(rs[3]), (lv, i, j)
+                assert abs(u[i, j] - eu) <= 1e-4 and abs(v[i, j] - ev) <= 1e-4, (lv, i, j)
+        say(f"level {lv}: N={N} {nI}x{nJ} equal")
+        prev_u, prev_v = u, v
+    plan.close()
+
+
 def plane_copy(plan, lv):
     import ctypes
 
@@ -212,6 +283,9 @@ def test_float_frame_plan_errors(ctx):
     with pytest.raises(qlpiv.QlpError):
         qlpiv.PivPlan(ctx, 128, 128, 1, ia_start=32, ia_end=16, xcorr_form=1,
                       frame_type=qlpiv._lib.QLP_FRAME_F32)
+    with pytest.raises(qlpiv.QlpError):
+        qlpiv.PivPlan(ctx, 128, 128, 1, ia_start=32, ia_end=16,
+                      frame_type=qlpiv._lib.QLP_FRAME_U16, frame_bits=14)
     plan = qlpiv.PivPlan(ctx, 128, 128, 1, ia_start=32, ia_end=16,
                          frame_type=qlpiv._lib.QLP_FRAME_F32)
     img8 = torch.zeros((1, 128, 128), dtype=torch.uint8, device="cuda")

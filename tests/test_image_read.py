@@ -94,6 +94,78 @@ def test_jpeg_tables_and_restart_intervals(tmp_path, kw):
     assert np.array_equal(qlpiv.read_image(path), ref)
 
 
+def frame16(seed, H=61, W=77):
+    rng = np.random.default_rng(seed)
+    f = rng.integers(0, 65536, (H, W)).astype(np.uint16)
+    f[H // 10:H // 3, W // 8:W // 2] = 4095  # runs for the run-length and dictionary coders
+    f[H // 2] = np.arange(W) * 811 % 65536
+    return f
+
+
+@pytest.mark.parametrize("compression", [None, "packbits", "tiff_lzw", "tiff_adobe_deflate"])
+@pytest.mark.parametrize("order", ["<", ">"])
+def test_tiff16_round_trip(tmp_path, compression, order):
+    """16-bit grayscale TIFF strips in both byte orders and every accepted
+    compression read back as the written samples."""
+    f = frame16(3)
+    path = tmp_path / "f16.tif"
+    mode = "I;16" if order == "<" else "I;16B"
+    Image.frombytes(mode, (f.shape[1], f.shape[0]), f.astype(order + "u2").tobytes()).save(
+        path, compression=compression, tiffinfo={278: 16})
+    assert np.array_equal(qlpiv.read_image16(path), f)
+
+
+def test_tiff16_horizontal_predictor(tmp_path):
+    """Predictor 2 on 16-bit samples: strips written by hand (uncompressed,
+    differences of whole samples modulo 2^16, little endian)."""
+    import struct
+
+    f = frame16(4, 9, 11)
+    d = f.astype(np.int64)
+    diff = np.concatenate([d[:, :1], np.diff(d, axis=1)], axis=1) % 65536
+    data = diff.astype("<u2").tobytes()
+    H, W = f.shape
+    tags = [(256, 3, 1, W), (257, 3, 1, H), (258, 3, 1, 16), (259, 3, 1, 1), (262, 3, 1, 1),
+            (273, 4, 1, 0), (277, 3, 1, 1), (278, 3, 1, H), (279, 4, 1, len(data)), (317, 3, 1, 2)]
+    ifd_off = 8
+    data_off = ifd_off + 2 + 12 * len(tags) + 4
+    out = bytearray(b"II*\x00" + struct.pack("<I", ifd_off) + struct.pack("<H", len(tags)))
+    for tag, typ, cnt, val in tags:
+        val = data_off if tag == 273 else val
+        out += struct.pack("<HHI", tag, typ, cnt) + (struct.pack("<HH", val, 0) if typ == 3
+                                                      else struct.pack("<I", val))
+    out += struct.pack("<I", 0) + data
+    path = tmp_path / "p16.tif"
+    path.write_bytes(bytes(out))
+    assert np.array_equal(qlpiv.read_image16(path), f)
+
+
+def test_png16_round_trip(tmp_path):
+    """16-bit grayscale PNG (big-endian samples, every row filter chosen by
+    the encoder) reads back exactly."""
+    f = frame16(5)
+    path = tmp_path / "f16.png"
+    Image.frombytes("I;16", (f.shape[1], f.shape[0]), f.astype("<u2").tobytes()).save(path)
+    with Image.open(path) as im:
+        assert im.mode.startswith("I")
+    assert np.array_equal(qlpiv.read_image16(path), f)
+
+
+def test_16bit_reads_refuse_8bit_files(tmp_path):
+    """8-bit files are widened through Java2D by the reference's 16-bit
+    reader: refused; 16-bit files are refused by the 8-bit reader."""
+    f8 = frame(9)
+    for name in ("g8.png", "g8.tif", "g8.jpg"):
+        Image.fromarray(f8).save(tmp_path / name)
+        with pytest.raises(qlpiv.QlpError) as e:
+            qlpiv.read_image16(tmp_path / name)
+        assert e.value.code == qlpiv._lib.QLP_ERR_UNSUPPORTED
+    f = frame16(6)
+    Image.frombytes("I;16", (f.shape[1], f.shape[0]), f.astype("<u2").tobytes()).save(tmp_path / "h.tif")
+    with pytest.raises(qlpiv.QlpError):
+        qlpiv.read_image(tmp_path / "h.tif")
+
+
 def test_unsupported_layouts_are_refused(tmp_path):
     rgb = np.zeros((8, 8, 3), np.uint8)
     for name in ("c.png", "c.tif"):

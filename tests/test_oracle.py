@@ -409,6 +409,32 @@ def test_float_warp_micro_reads_float_frames():
     assert np.all(b == 38.0)
 
 
+def test_u16_clip_and_warp_follow_the_8bit_rules():
+    """10/12/16-bit frames: 8-bit values widened give the 8-bit clip and warp
+    (every side); the clip maximum keeps the >= 16 clamp with 16-bit values;
+    integer displacements shift the frame; the warp maximum is 2^bits - 1."""
+    rng = np.random.default_rng(6)
+    img8 = rng.integers(0, 256, (70, 80)).astype(np.uint8)
+    w = img8.astype(np.uint16)
+    for side in range(4):
+        for top, left, du, dv in [(10, 12, 1.3, -2.7), (-4, 60, -0.4, 3.9)]:
+            a = O.warp_micro(img8, top, left, 16, du, dv, side)
+            for bits in (10, 12, 16):
+                b, m = O.warp_micro_u16(w, bits, top, left, 16, du, dv, side)
+                assert np.array_equal(a.astype(np.float32), b) and m == (1 << bits) - 1
+    for top, left in [(0, 0), (-5, 70), (60, -7)]:
+        a, ma = O.clip_tile(img8, top, left, 16)
+        b, mb = O.clip_tile_u16(w, top, left, 16)
+        assert np.array_equal(a.astype(np.float32), b) and ma == mb
+    big = rng.integers(0, 65536, (70, 80)).astype(np.uint16)
+    t, m = O.clip_tile_u16(big, 3, 4, 16)
+    assert np.array_equal(t, big[3:19, 4:20].astype(np.float32)) and m == float(big[3:19, 4:20].max())
+    t, m = O.clip_tile_u16(np.full((70, 80), 9, np.uint16), 3, 4, 16)
+    assert m == 16.0
+    t, _ = O.warp_micro_u16(big, 16, 20, 30, 16, 3.0, -2.0)
+    assert np.array_equal(t, big[23:39, 28:44].astype(np.float32))
+
+
 def test_warp_micro_invariants():
     rng = np.random.default_rng(2)
     img = rng.integers(0, 256, (96, 96)).astype(np.uint8)
