@@ -26,7 +26,7 @@ from ._lib import (QLP_NORM_A_MAX, QLP_NORM_PER_TILE, QLP_PEAK_FROM_CENTER, QLP_
 
 __all__ = [
     "Batch", "Context", "CrossCorrelationJob", "JobResultEnum", "PivPlan", "QlpError", "Server",
-    "VelocityExporter", "read_image", "read_image16",
+    "VelocityExporter", "composite_last_level_start_step", "read_image", "read_image16",
     "piv_config", "plan_estimate",
     "VectorValidatorJob", "build",
     "QLP_NORM_PER_TILE", "QLP_NORM_A_MAX", "QLP_PEAK_FROM_CENTER", "QLP_PEAK_MULTI",
@@ -534,6 +534,22 @@ class CrossCorrelationJob:
     def dispose(self):
         self.inputs.clear()
         self.results.clear()
+
+
+def composite_last_level_start_step(n_levels, interpolator_start_step, apply_first_at_last):
+    """First level a plan interpolates for CompositeLastLevelInterpolator
+    (interpolators/CompositeLastLevelInterpolator.java:37-54) with a
+    cross-correlation first-levels interpolator and no last-level one.
+
+    The reference computes isLastStep = currentStep != maxAdaptiveSteps - 1
+    (:38), true on every level except the last, so the first-levels
+    interpolator runs on the last level only, or on every level when
+    applyMainInterpolatorAtLastStep is set; the job applies the composite from
+    interpolatorStartStep on (jobs/LocalPIVOpenCLGpuJob.java:373).  The
+    last-level interpolators it is meant for are the dense optical-flow ones
+    (LucasKanade, LiuShenWithLucasKanade), which this library does not provide."""
+    first = 0 if apply_first_at_last else n_levels - 1
+    return max(int(interpolator_start_step), first)
 
 
 def piv_config(**config):

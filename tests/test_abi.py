@@ -83,3 +83,21 @@ def test_synthetic_pair_is_seeded_and_dense():
     a3, _, _ = particle_pair(128, 128, seed=6)
     assert np.array_equal(a1, a2) and np.array_equal(b1, b2) and not np.array_equal(a1, a3)
     assert a1.dtype == np.uint8 and 10 < a1.mean() < 60
+
+
+def test_composite_last_level_interpolator_start_step():
+    """CompositeLastLevelInterpolator (first-levels interpolator only): the
+    reference's inverted isLastStep (:38) applies it on the last level alone,
+    or on every level with applyMainInterpolatorAtLastStep, and never before
+    the job's interpolatorStartStep."""
+    from qlpiv import composite_last_level_start_step as f
+
+    assert f(3, 0, False) == 2 and f(3, 0, True) == 0
+    assert f(3, 1, True) == 1 and f(3, 2, False) == 2 and f(1, 0, False) == 0
+    # the levels the composite interpolates, by the reference's own rule
+    for n in (1, 2, 3, 4):
+        for start in range(n + 1):
+            for apply in (False, True):
+                levels = [s for s in range(n)
+                          if s >= start and ((s == n - 1) or apply)]
+                assert levels == [s for s in range(n) if s >= f(n, start, apply)]
