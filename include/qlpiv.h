@@ -439,8 +439,9 @@ int qlp_piv_run_dev(qlp_piv_plan *plan, const uint8_t *imgsA, const uint8_t *img
 /* The same loop for float frames (plan created with frame_type QLP_FRAME_F32;
  * imgsA, imgsB device, pairs x H x W floats): every level cuts float tiles
  * (ImageFloat.clipImageMatrix :278-384; the Micro warp of the order's sides,
- * maximum 65535) and correlates them in the exact form.  Warp modes
- * QLP_WARP_NONE and QLP_WARP_MICRO (every order). */
+ * maximum 65535; the Mini warp likewise) and correlates them in the exact form.
+ * Warp modes QLP_WARP_NONE, QLP_WARP_MICRO and QLP_WARP_MINI (every order); the
+ * Full warp returns QLP_ERR_UNSUPPORTED. */
 int qlp_piv_run_f32_dev(qlp_piv_plan *plan, const float *imgsA, const float *imgsB);
 /* The loop for 10/12/16-bit frames (frame_type QLP_FRAME_U16; imgsA, imgsB
  * device, pairs x H x W uint16): the frames are converted to float in plan

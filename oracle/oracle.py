@@ -101,6 +101,11 @@ def lib():
                                    _f32p, c_int, c_int, _u8p]
         L.or_warp_mini_side.argtypes = [_u8p, c_int, c_int, c_int, c_int, _i32p, _i32p, c_int,
                                         _f32p, _f32p, c_int, c_int, c_int, _u8p]
+        L.or_warp_mini_side_f32.argtypes = [_f32p, c_int, c_int, c_int, c_int, _i32p, _i32p, c_int,
+                                            _f32p, _f32p, c_int, c_int, c_int, _f32p]
+        L.or_warp_mini_side_u16.argtypes = [np.ctypeslib.ndpointer(np.uint16, flags="C_CONTIGUOUS"),
+                                            c_int, c_int, c_int, c_int, _i32p, _i32p, c_int, _f32p,
+                                            _f32p, c_int, c_int, c_int, _f32p]
         L.or_full_field.argtypes = [c_int, c_int, _i32p, _i32p, c_int, c_int, _f32p, _f32p, c_int,
                                     c_int, _f32p, _f32p]
         L.or_warp_image.argtypes = [_u8p, c_int, c_int, _f32p, _f32p, _u8p]
@@ -413,6 +418,25 @@ def warp_mini(img, top, left, N, u, v, ti, tj, side=SIDE_B_SECOND):
     tile = np.zeros((N, N), np.uint8)
     lib().or_warp_mini_side(img, img.shape[0], img.shape[1], top.shape[0], top.shape[1], top, left,
                             N, u, v, int(ti), int(tj), int(side), tile)
+    return tile
+
+
+def warp_mini_frame(img, top, left, N, u, v, ti, tj, side=SIDE_B_SECOND):
+    """Mini-warped tile of a float (ImageFloat) or uint16 (ImageInt16) frame,
+    as float samples."""
+    top = np.ascontiguousarray(top, np.int32)
+    left = np.ascontiguousarray(left, np.int32)
+    u = np.ascontiguousarray(u, np.float32)
+    v = np.ascontiguousarray(v, np.float32)
+    tile = np.zeros((N, N), np.float32)
+    if img.dtype == np.uint16:
+        img = np.ascontiguousarray(img)
+        fn = lib().or_warp_mini_side_u16
+    else:
+        img = np.ascontiguousarray(img, np.float32)
+        fn = lib().or_warp_mini_side_f32
+    fn(img, img.shape[0], img.shape[1], top.shape[0], top.shape[1], top, left, N, u, v, int(ti),
+       int(tj), int(side), tile)
     return tile
 
 

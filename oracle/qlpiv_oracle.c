@@ -2618,9 +2618,11 @@ static float field_value(const or_field *F, int c, float y, float x)
  * this tile's, is evaluated from the B reference window (top + floor(u),
  * left + floor(v)) in every order, and applied at the side's window.  Grid of
  * nI x nJ tiles of N px, row-major tops/lefts. */
-void or_warp_mini_side(const uint8_t *img, int H, int W, int nI, int nJ, const int *top,
-                       const int *left, int N, const float *u, const float *v, int ti, int tj,
-                       int side, uint8_t *tile)
+/* The Mini warp of tile (ti, tj) for a side, from a frame of any pixel type,
+ * into float samples. */
+static void warp_mini_frame(const or_frame *fr, int H, int W, int nI, int nJ, const int *top,
+                            const int *left, int N, const float *u, const float *v, int ti, int tj,
+                            int side, float *tile)
 {
     or_field F;
     const int t = ti * nJ + tj;
@@ -2642,10 +2644,45 @@ void or_warp_mini_side(const uint8_t *img, int H, int W, int nI, int nJ, const i
     for (int i = 0; i < N; i++)
         for (int j = 0; j < N; j++) {
             const float y = (float)(refTop + i), x = (float)(refLeft + j);
-            const float du = field_value(&F, 0, y, x), dv = field_value(&F, 1, y, x);
-            tile[i * N + j] = side_pixel(img, H, W, &g, du, dv, i, j);
+            float du = field_value(&F, 0, y, x), dv = field_value(&F, 1, y, x);
+            if (g.half) {
+                du = du / 2.0f + g.offU;
+                dv = dv / 2.0f + g.offV;
+            }
+            tile[i * N + j] = warp_pixel_frame(fr, H, W, du, dv, (short)(i + g.top), (short)(j + g.left),
+                                               g.first);
         }
     field_free(&F);
+}
+
+void or_warp_mini_side(const uint8_t *img, int H, int W, int nI, int nJ, const int *top,
+                       const int *left, int N, const float *u, const float *v, int ti, int tj,
+                       int side, uint8_t *tile)
+{
+    const or_frame fr = {img, NULL, NULL};
+    float *t = (float *)malloc(sizeof(float) * N * N);
+    warp_mini_frame(&fr, H, W, nI, nJ, top, left, N, u, v, ti, tj, side, t);
+    for (int k = 0; k < N * N; k++) tile[k] = (uint8_t)t[k];
+    free(t);
+}
+
+/* The Mini warp of a float frame (ImageFloat) or a 10/12/16-bit frame
+ * (ImageInt16): the rounded samples as floats (doBiLinearWarp :292-334 writes
+ * them into a MatrixFloat whose maximum is image.getMaximumValue(), :298). */
+void or_warp_mini_side_f32(const float *img, int H, int W, int nI, int nJ, const int *top,
+                           const int *left, int N, const float *u, const float *v, int ti, int tj,
+                           int side, float *tile)
+{
+    const or_frame fr = {NULL, img, NULL};
+    warp_mini_frame(&fr, H, W, nI, nJ, top, left, N, u, v, ti, tj, side, tile);
+}
+
+void or_warp_mini_side_u16(const uint16_t *img, int H, int W, int nI, int nJ, const int *top,
+                           const int *left, int N, const float *u, const float *v, int ti, int tj,
+                           int side, float *tile)
+{
+    const or_frame fr = {NULL, NULL, img};
+    warp_mini_frame(&fr, H, W, nI, nJ, top, left, N, u, v, ti, tj, side, tile);
 }
 
 void or_warp_mini(const uint8_t *img, int H, int W, int nI, int nJ, const int *top,

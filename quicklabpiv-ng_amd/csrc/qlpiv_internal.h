@@ -160,6 +160,11 @@ int launch_warp_micro_f32(Context *ctx, const float *img, int H, int W, int n, i
                           const int *left, const float *u, const float *v, float *tiles, float *maxv,
                           int side = 0, float wmax = 65535.0f);
 int launch_u16_to_f32(Context *ctx, const uint16_t *in, float *out, size_t n);
+// The Mini warp of one side of float frames (pairs x H x W) into float tiles
+// with the maximum wmax (qlpiv_splinewarp.hip).
+int launch_warp_mini_f32(Context *ctx, int pairs, const float *imgs, int H, int W, int nI, int nJ,
+                         int N, const int *top, const int *left, const float *u, const float *v,
+                         float *tiles, float *maxv, int side, float wmax);
 // Spline-field warps (qlpiv_splinewarp.hip).
 int launch_warp_mini(Context *ctx, int pairs, const uint8_t *imgs, int H, int W, int nI, int nJ,
                      int N, const int *top, const int *left, const float *u, const float *v,

@@ -303,8 +303,12 @@ static int plan_layout(int H, int W, int pairs, const qlp_piv_config *cfg, qlp_p
         cfg->frame_bits != 16)
         return fail(QLP_ERR_ARG, "piv_plan_create: 10/12/16-bit frames need a bit depth of 10, 12 or 16");
     const bool f32 = cfg->frame_type != QLP_FRAME_U8;
-    if (f32 && cfg->warp_mode != QLP_WARP_NONE && cfg->warp_mode != QLP_WARP_MICRO)
-        return fail(QLP_ERR_UNSUPPORTED, "piv_plan_create: float and 10/12/16-bit frames take no warp or the Micro warp");
+    // the Full warp of an ImageFloat tiles with the source frame's maximum, and of an
+    // ImageInt16 stores each sample as (byte) into its short matrix
+    // (MatrixInt16.setElement(float) :95-101): not provided
+    if (f32 && cfg->warp_mode == QLP_WARP_FULL)
+        return fail(QLP_ERR_UNSUPPORTED,
+                    "piv_plan_create: float and 10/12/16-bit frames take no warp, Micro or Mini");
     if (f32 && cfg->xcorr_form != QLP_XCORR_EXACT)
         return fail(QLP_ERR_UNSUPPORTED, "piv_plan_create: float and 10/12/16-bit frames take the exact correlation form");
     const bool valid = cfg->valid.n_stages > 0;
@@ -481,40 +485,55 @@ int qlp_piv_plan_level(qlp_piv_plan *p, int level, int *ia, int *nI, int *nJ, in
     return QLP_OK;
 }
 
-// Float tiles of one level of a float-frame plan (ImageFloat frames): A and B
-// clipped (B shifted by the rounded displacement with no warp), or the Micro
-// warp of the order's sides, for every pair.
+// Float tiles of one level of a float-frame plan (ImageFloat frames, or
+// ImageInt16 frames converted to float): A and B clipped (B shifted by the
+// rounded displacement with no warp), or the Micro or Mini warp of the order's
+// sides, for every pair.
 static int float_tiles(qlp_piv_plan *p, const Level &L, size_t l, const float *imgsA,
                        const float *imgsB)
 {
     Context *ctx = p->ctx;
     const qlp_piv_config &c = p->cfg;
     const int order = c.warp_order;
-    const bool micro = l > 0 && c.warp_mode == QLP_WARP_MICRO;
-    const size_t tN = (size_t)L.N * L.N, frame = (size_t)p->H * p->W;
     // ImageInt16 rules for 10/12/16-bit frames, ImageFloat ones otherwise
     const bool i16 = c.frame_type == QLP_FRAME_U16;
     const int clamp16 = i16 ? 1 : 0;
     const float wmax = i16 ? (float)((1 << c.frame_bits) - 1) : 65535.0f;
+    const bool warp = l > 0 && (c.warp_mode == QLP_WARP_MICRO || c.warp_mode == QLP_WARP_MINI);
+    const bool warpA = warp && order != QLP_WARP_ORDER_SECOND;
+    const bool warpB = warp && order != QLP_WARP_ORDER_FIRST;
+    const int sa = order == QLP_WARP_ORDER_FIRST ? SIDE_A_FIRST : SIDE_A_BOTH;
+    const int sb = order == QLP_WARP_ORDER_SECOND ? SIDE_B_SECOND : SIDE_B_BOTH;
     int rc = QLP_OK;
+    if (l > 0 && c.warp_mode == QLP_WARP_MINI) {  // one launch covers every pair
+        if (warpA)
+            rc = launch_warp_mini_f32(ctx, p->pairs, imgsA, p->H, p->W, L.nI, L.nJ, L.N, L.top,
+                                      L.left, L.u, L.v, p->fA, p->mA, sa, wmax);
+        if (!rc && warpB)
+            rc = launch_warp_mini_f32(ctx, p->pairs, imgsB, p->H, p->W, L.nI, L.nJ, L.N, L.top,
+                                      L.left, L.u, L.v, p->fB, p->mB, sb, wmax);
+        if (rc) return rc;
+    }
+    const bool micro = l > 0 && c.warp_mode == QLP_WARP_MICRO;
+    const size_t tN = (size_t)L.N * L.N, frame = (size_t)p->H * p->W;
     for (int k = 0; k < p->pairs && !rc; k++) {
         const float *a = imgsA + k * frame, *b = imgsB + k * frame;
         const size_t o = (size_t)k * L.n;
         float *ta = p->fA + o * tN, *tb = p->fB + o * tN, *ma = p->mA + o, *mb = p->mB + o;
         const float *u = L.u + o, *v = L.v + o;
-        if (micro && order != QLP_WARP_ORDER_SECOND)
-            rc = launch_warp_micro_f32(ctx, a, p->H, p->W, L.n, L.N, L.top, L.left, u, v, ta, ma,
-                                       order == QLP_WARP_ORDER_FIRST ? SIDE_A_FIRST : SIDE_A_BOTH, wmax);
-        else
+        if (micro && warpA)
+            rc = launch_warp_micro_f32(ctx, a, p->H, p->W, L.n, L.N, L.top, L.left, u, v, ta, ma, sa,
+                                       wmax);
+        else if (!warpA)
             rc = launch_clip_f32(ctx, a, p->H, p->W, L.n, L.N, L.top, L.left, nullptr, nullptr, ta, ma,
                                  clamp16);
         if (rc) break;
-        if (micro && order != QLP_WARP_ORDER_FIRST)
-            rc = launch_warp_micro_f32(ctx, b, p->H, p->W, L.n, L.N, L.top, L.left, u, v, tb, mb,
-                                       order == QLP_WARP_ORDER_SECOND ? SIDE_B_SECOND : SIDE_B_BOTH, wmax);
-        else if (l > 0 && c.warp_mode == QLP_WARP_NONE)
+        if (micro && warpB)
+            rc = launch_warp_micro_f32(ctx, b, p->H, p->W, L.n, L.N, L.top, L.left, u, v, tb, mb, sb,
+                                       wmax);
+        else if (!warpB && l > 0 && c.warp_mode == QLP_WARP_NONE)
             rc = launch_clip_f32(ctx, b, p->H, p->W, L.n, L.N, L.top, L.left, u, v, tb, mb, clamp16);
-        else
+        else if (!warpB)
             rc = launch_clip_f32(ctx, b, p->H, p->W, L.n, L.N, L.top, L.left, nullptr, nullptr, tb, mb,
                                  clamp16);
     }

@@ -147,13 +147,17 @@ __device__ double field_at(const double *ys, int ny, const double *xs, int nx, c
 
 constexpr int MINI_R = 7;  // the tile neighbourhood (createMiniDisplacements :207)
 
+// T: pixel type of the frames (uint8_t, or float for ImageFloat and converted
+// ImageInt16 frames); O: tile type (uint8_t, or float); wmax: the maximum a
+// warped tile carries (image.getMaximumValue(), doBiLinearWarp :298).
+template <typename T, typename O>
 __global__ __launch_bounds__(64) void warp_mini_kernel(int n, int nI, int nJ, int N, int H, int W,
-                                                       const uint8_t *__restrict__ imgs,
+                                                       const T *__restrict__ imgs,
                                                        const int *__restrict__ top,
                                                        const int *__restrict__ left,
                                                        const float *__restrict__ u,
                                                        const float *__restrict__ v, int side,
-                                                       uint8_t *__restrict__ tiles,
+                                                       float wmax, O *__restrict__ tiles,
                                                        float *__restrict__ maxv)
 {
     __shared__ double s_y[MINI_R], s_x[MINI_R];
@@ -162,7 +166,7 @@ __global__ __launch_bounds__(64) void warp_mini_kernel(int n, int nI, int nJ, in
     const int ti = t / nJ, tj = t - ti * nJ;
     const float *__restrict__ pu = u + (size_t)pair * n;
     const float *__restrict__ pv = v + (size_t)pair * n;
-    const uint8_t *__restrict__ img = imgs + (size_t)pair * H * W;
+    const T *__restrict__ img = imgs + (size_t)pair * H * W;
     const float u0 = pu[t], v0 = pv[t];
     const double fu0 = floor((double)u0), fv0 = floor((double)v0);
     // the field is evaluated from the B reference window in every order
@@ -195,7 +199,7 @@ __global__ __launch_bounds__(64) void warp_mini_kernel(int n, int nI, int nJ, in
     __syncthreads();
     const float ymin = (float)s_y[0], ymax = (float)s_y[ny - 1];
     const float xmin = (float)s_x[0], xmax = (float)s_x[nx - 1];
-    uint8_t *__restrict__ out = tiles + ((size_t)pair * n + t) * N * N;
+    O *__restrict__ out = tiles + ((size_t)pair * n + t) * N * N;
     for (int p = k; p < N * N; p += 64) {
         const int i = p / N, j = p - (p / N) * N;
         float y = (float)(tTop + i), x = (float)(tLeft + j);
@@ -203,9 +207,9 @@ __global__ __launch_bounds__(64) void warp_mini_kernel(int n, int nI, int nJ, in
         y = y < ymin ? ymin : (y > ymax ? ymax : y);
         const float du = (float)field_at(s_y, ny, s_x, nx, s_f[0], s_fy[0], s_fx[0], spline, y, x);
         const float dv = (float)field_at(s_y, ny, s_x, nx, s_f[1], s_fy[1], s_fx[1], spline, y, x);
-        out[p] = (uint8_t)side_sample(img, H, W, g, du, dv, i, j);
+        out[p] = (O)side_sample(img, H, W, g, du, dv, i, j);
     }
-    if (k == 0 && maxv) maxv[(size_t)pair * n + t] = 255.0f;
+    if (k == 0 && maxv) maxv[(size_t)pair * n + t] = wmax;
 }
 
 int launch_warp_mini(Context *ctx, int pairs, const uint8_t *imgs, int H, int W, int nI, int nJ,
@@ -218,8 +222,22 @@ int launch_warp_mini(Context *ctx, int pairs, const uint8_t *imgs, int H, int W,
     // reference's BiLinearInterpolatingFunction reads past its arrays)
     if (nI < 2 || nJ < 2)
         return fail(QLP_ERR_UNSUPPORTED, "mini warp: the grid needs 2 tiles in each direction");
-    hipLaunchKernelGGL(warp_mini_kernel, dim3(pairs * n), dim3(64), 0, ctx->stream, n, nI, nJ, N,
-                       H, W, imgs, top, left, u, v, side, tiles, maxv);
+    hipLaunchKernelGGL((warp_mini_kernel<uint8_t, uint8_t>), dim3(pairs * n), dim3(64), 0,
+                       ctx->stream, n, nI, nJ, N, H, W, imgs, top, left, u, v, side, 255.0f, tiles,
+                       maxv);
+    return check_hip(hipGetLastError(), "mini warp launch");
+}
+
+int launch_warp_mini_f32(Context *ctx, int pairs, const float *imgs, int H, int W, int nI, int nJ,
+                         int N, const int *top, const int *left, const float *u, const float *v,
+                         float *tiles, float *maxv, int side, float wmax)
+{
+    const int n = nI * nJ;
+    if (pairs <= 0 || n <= 0) return QLP_OK;
+    if (nI < 2 || nJ < 2)
+        return fail(QLP_ERR_UNSUPPORTED, "mini warp: the grid needs 2 tiles in each direction");
+    hipLaunchKernelGGL((warp_mini_kernel<float, float>), dim3(pairs * n), dim3(64), 0, ctx->stream,
+                       n, nI, nJ, N, H, W, imgs, top, left, u, v, side, wmax, tiles, maxv);
     return check_hip(hipGetLastError(), "mini warp launch");
 }
 

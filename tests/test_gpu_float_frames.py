@@ -128,7 +128,7 @@ def direct_inherit(pu, nI, nJ, round_it=False):
     return out
 
 
-@pytest.mark.parametrize("warp,order", [(1, 0), (1, 2), (1, 1), (0, 0)])
+@pytest.mark.parametrize("warp,order", [(1, 0), (1, 2), (1, 1), (0, 0), (2, 0), (2, 2), (2, 1)])
 def test_float_frame_plan_against_oracle(ctx, fpair, warp, order):
     """Every level of a float-frame plan (64 -> 16 px, Direct inheritance,
     FromCenter, Guo 3 px) equals the oracle restatement from the same inherited
@@ -161,11 +161,17 @@ def test_float_frame_plan_against_oracle(ctx, fpair, warp, order):
         for i in range(nI):
             for j in range(nJ):
                 t, l0 = int(L["top"][i, j]), int(L["left"][i, j])
-                if lv > 0 and warp and order in sides_a:
+                if lv > 0 and warp == 2 and order in sides_a:
+                    tA, mA = O.warp_mini_frame(ha, L["top"], L["left"], N, uin, vin, i, j,
+                                               sides_a[order]), 65535.0
+                elif lv > 0 and warp and order in sides_a:
                     tA, mA = O.warp_micro_f32(ha, t, l0, N, uin[i, j], vin[i, j], sides_a[order])
                 else:
                     tA, mA = O.clip_tile_f32(ha, t, l0, N)
-                if lv > 0 and warp and order in sides_b:
+                if lv > 0 and warp == 2 and order in sides_b:
+                    tB, mB = O.warp_mini_frame(hb, L["top"], L["left"], N, uin, vin, i, j,
+                                               sides_b[order]), 65535.0
+                elif lv > 0 and warp and order in sides_b:
                     tB, mB = O.warp_micro_f32(hb, t, l0, N, uin[i, j], vin[i, j], sides_b[order])
                 elif lv > 0 and not warp:
                     tb = int(np.floor(np.float32(t) + uin[i, j] + np.float32(0.5)))
@@ -187,7 +193,7 @@ def test_float_frame_plan_against_oracle(ctx, fpair, warp, order):
     plan.close()
 
 
-@pytest.mark.parametrize("bits,warp,order", [(12, 1, 0), (16, 1, 2), (10, 0, 0)])
+@pytest.mark.parametrize("bits,warp,order", [(12, 1, 0), (16, 1, 2), (10, 0, 0), (12, 2, 2)])
 def test_u16_frame_plan_against_oracle(ctx, bits, warp, order):
     """10/12/16-bit frames (ImageInt16): clipped tiles take the MatrixInt16
     maximum (>= 16), Micro-warped tiles the bit-depth maximum; every level
@@ -231,11 +237,17 @@ def test_u16_frame_plan_against_oracle(ctx, bits, warp, order):
         for i in range(nI):
             for j in range(nJ):
                 t, l0 = int(L["top"][i, j]), int(L["left"][i, j])
-                if lv > 0 and warp and order in sides_a:
+                if lv > 0 and warp == 2 and order in sides_a:
+                    tA, mA = O.warp_mini_frame(ha, L["top"], L["left"], N, uin, vin, i, j,
+                                               sides_a[order]), float((1 << bits) - 1)
+                elif lv > 0 and warp and order in sides_a:
                     tA, mA = O.warp_micro_u16(ha, bits, t, l0, N, uin[i, j], vin[i, j], sides_a[order])
                 else:
                     tA, mA = O.clip_tile_u16(ha, t, l0, N)
-                if lv > 0 and warp and order in sides_b:
+                if lv > 0 and warp == 2 and order in sides_b:
+                    tB, mB = O.warp_mini_frame(hb, L["top"], L["left"], N, uin, vin, i, j,
+                                               sides_b[order]), float((1 << bits) - 1)
+                elif lv > 0 and warp and order in sides_b:
                     tB, mB = O.warp_micro_u16(hb, bits, t, l0, N, uin[i, j], vin[i, j], sides_b[order])
                 elif lv > 0 and not warp:
                     tb = int(np.floor(np.float32(t) + uin[i, j] + np.float32(0.5)))
@@ -278,7 +290,7 @@ def test_float_frame_plan_errors(ctx):
     import qlpiv
 
     with pytest.raises(qlpiv.QlpError):
-        qlpiv.PivPlan(ctx, 128, 128, 1, ia_start=32, ia_end=16, warp_mode=qlpiv._lib.QLP_WARP_MINI,
+        qlpiv.PivPlan(ctx, 128, 128, 1, ia_start=32, ia_end=16, warp_mode=qlpiv._lib.QLP_WARP_FULL,
                       frame_type=qlpiv._lib.QLP_FRAME_F32)
     with pytest.raises(qlpiv.QlpError):
         qlpiv.PivPlan(ctx, 128, 128, 1, ia_start=32, ia_end=16, xcorr_form=1,
