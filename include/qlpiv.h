@@ -586,6 +586,16 @@ int qlp_export_open(const char *filename, const qlp_export_params *params, const
 int qlp_export_frame(qlp_exporter *exporter, int frame, int ia_height, int ia_width,
                      int adaptive_step, const int margins[4], int nI, int nJ, const float *u,
                      const float *v, const uint8_t *invalid);
+/* exportDataToFile(PIVResults) :185-238 for one velocity map of whole results
+ * (PIVMap): a struct "velocities<H>x<W>" with iaHeight, iaWidth,
+ * multiVolumeExport (and, for a multi-volume file, multiVolumeStartOffset =
+ * start_offset, the first map's absolute frame offset), margins and u, v of
+ * nI x nJ x n_maps singles; frame k of u, v (HOST, n_frames x nI x nJ
+ * row-major) goes to map index frame_index[k] (Velocities.getFrameNumber).
+ * Values are written as they are (no NaN marking, no swap). */
+int qlp_export_results(qlp_exporter *exporter, int ia_height, int ia_width, const int margins[4],
+                       int nI, int nJ, int n_maps, int n_frames, const int *frame_index,
+                       const float *u, const float *v, int start_offset);
 /* Write the file and release the exporter. */
 int qlp_export_close(qlp_exporter *exporter);
 

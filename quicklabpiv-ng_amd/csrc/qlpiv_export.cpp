@@ -5,7 +5,10 @@
 // closeFile :115-129): the 128-byte header, the char variable "exportDate",
 // the 1 x 1 struct "parameters", and per IA size a 1 x 1 struct
 // "velocities<H>x<W>" with iaHeight, iaWidth, adaptiveStep, margins (struct:
-// top, left, bottom, right) and u, v (single, nI x nJ x frames).  The element
+// top, left, bottom, right) and u, v (single, nI x nJ x frames); or, from whole
+// results (exportDataToFile(PIVResults) :185-238), per velocity map a struct
+// with iaHeight, iaWidth, multiVolumeExport (and multiVolumeStartOffset),
+// margins and u, v (nI x nJ x maps, the values as they are).  The element
 // layout is the reference's (exporter/MatlabLevel5Matrix.java :417-481,
 // MatlabLevel5Struct.java :50-66, MatlabLevel5Header.java :62-108):
 // uncompressed, little endian, every sub-element padded to 8 bytes, scalars as
@@ -162,6 +165,7 @@ struct qlp_exporter {
     std::string filename, date;
     int total_maps = 1;
     int mark_invalid_nan = 0;
+    int multi_volume = 0;
     std::unique_ptr<Element> parameters;
     // velocity structs by IA height, in creation order
     std::vector<std::pair<std::string, std::unique_ptr<Element>>> maps;
@@ -186,6 +190,7 @@ int qlp_export_open(const char *filename, const qlp_export_params *p, const char
     e->date = date_text ? date_text : java_date_now();
     e->total_maps = p->total_maps;
     e->mark_invalid_nan = p->mark_invalid_nan;
+    e->multi_volume = p->multi_volume ? 1 : 0;
     e->parameters = structure();
     Element &s = *e->parameters;
     s.add("imageWidth", int32_scalar(p->image_width));
@@ -260,6 +265,51 @@ int qlp_export_frame(qlp_exporter *e, int frame, int ia_height, int ia_width, in
             memcpy(&eu->data[idx], &fu, 4);
             memcpy(&ev->data[idx], &fv, 4);
         }
+    return QLP_OK;
+}
+
+int qlp_export_results(qlp_exporter *e, int ia_height, int ia_width, const int margins[4], int nI,
+                       int nJ, int n_maps, int n_frames, const int *frame_index, const float *u,
+                       const float *v, int start_offset)
+{
+    if (!e || !margins || (n_frames > 0 && (!frame_index || !u || !v)))
+        return fail(QLP_ERR_ARG, "export_results: null argument");
+    if (nI < 1 || nJ < 1 || ia_height < 1 || ia_width < 1 || n_maps < 1 || n_frames < 0)
+        return fail(QLP_ERR_ARG, "export_results: bad sizes");
+    auto s = structure();
+    s->add("iaHeight", int32_scalar(ia_height));
+    s->add("iaWidth", int32_scalar(ia_width));
+    s->add("multiVolumeExport", int32_scalar(e->multi_volume));
+    if (e->multi_volume) s->add("multiVolumeStartOffset", int32_scalar(start_offset));
+    auto m = structure();
+    m->add("top", int32_scalar(margins[0]));
+    m->add("left", int32_scalar(margins[1]));
+    m->add("bottom", int32_scalar(margins[2]));
+    m->add("right", int32_scalar(margins[3]));
+    s->add("margins", std::move(m));
+    for (const char *c : {"u", "v"}) {
+        auto a = std::make_unique<Element>();
+        a->cls = mxSINGLE;
+        a->data_type = miSINGLE;
+        a->dims = {nI, nJ, n_maps};
+        a->data.assign((size_t)nI * nJ * n_maps * 4, 0);
+        s->add(c, std::move(a));
+    }
+    Element *eu = s->field("u"), *ev = s->field("v");
+    for (int k = 0; k < n_frames; k++) {
+        const int fr = frame_index[k];
+        if (fr < 0 || fr >= n_maps) return fail(QLP_ERR_ARG, "Invalid frame number was specified");
+        // column-major per frame: i fastest (the reference's j-outer, i-inner loop)
+        size_t idx = (size_t)fr * nI * nJ * 4;
+        const float *pu = u + (size_t)k * nI * nJ, *pv = v + (size_t)k * nI * nJ;
+        for (int j = 0; j < nJ; j++)
+            for (int i = 0; i < nI; i++, idx += 4) {
+                memcpy(&eu->data[idx], &pu[(size_t)i * nJ + j], 4);
+                memcpy(&ev->data[idx], &pv[(size_t)i * nJ + j], 4);
+            }
+    }
+    e->maps.emplace_back("velocities" + std::to_string(ia_height) + "x" + std::to_string(ia_width),
+                         std::move(s));
     return QLP_OK;
 }
 

@@ -652,6 +652,18 @@ class VelocityExporter:
                                         int(adaptive_step), m, nI, nJ, _ptr(u), _ptr(v),
                                         _ptr(inv)))
 
+    def export_results(self, frames, u, v, ia, n_maps, margins=(0, 0, 0, 0), start_offset=0):
+        """One velocity map of whole results (exportDataToFile(PIVResults)):
+        u, v (len(frames) x nI x nJ host arrays) go to the map indices
+        'frames'; ia = (height, width); n_maps = PIVMap.getNumberOfMaps()."""
+        u = np.ascontiguousarray(u, np.float32)
+        v = np.ascontiguousarray(v, np.float32)
+        fr = np.ascontiguousarray(frames, np.int32)
+        n, nI, nJ = u.shape
+        m = (ctypes.c_int * 4)(*[int(x) for x in margins])
+        check(self.lib.qlp_export_results(self.h, int(ia[0]), int(ia[1]), m, nI, nJ, int(n_maps), n,
+                                          _ptr(fr), _ptr(u), _ptr(v), int(start_offset)))
+
     def close(self):
         if self.h:
             h, self.h = self.h, None

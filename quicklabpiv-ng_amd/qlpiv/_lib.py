@@ -103,6 +103,7 @@ SIGNATURES = {
     "qlp_image_read16": (_i, [ctypes.c_char_p, _vp, _sz, ctypes.POINTER(_i), ctypes.POINTER(_i)]),
     "qlp_export_open": (_i, [ctypes.c_char_p, _vp, ctypes.c_char_p, ctypes.POINTER(_vp)]),
     "qlp_export_frame": (_i, [_vp, _i, _i, _i, _i, _vp, _i, _i, _vp, _vp, _vp]),
+    "qlp_export_results": (_i, [_vp, _i, _i, _vp, _i, _i, _i, _i, _vp, _vp, _vp, _i]),
     "qlp_export_close": (_i, [_vp]),
     "qlp_batch_create": (_i, [_vp, _i, _vp, _vp, ctypes.POINTER(_vp)]),
     "qlp_batch_fix": (_i, [_vp, _i, _i, _i]),

@@ -91,3 +91,42 @@ def test_export_errors(tmp_path):
     with pytest.raises(qlpiv.QlpError):
         ex.export(0, np.zeros((2, 4), np.float32), np.zeros((2, 4), np.float32), (16, 16), 0)
     ex.close()
+
+
+@pytest.mark.parametrize("multi", [False, True])
+def test_whole_results_export(tmp_path, multi):
+    """exportDataToFile(PIVResults) :185-238: one struct per velocity map with
+    iaHeight, iaWidth, multiVolumeExport (and multiVolumeStartOffset for a
+    multi-volume file), margins and u, v of nI x nJ x maps, frames at their map
+    indices, values as they are (NaN kept, no swap); maps never given are 0."""
+    rng = np.random.default_rng(7)
+    path = tmp_path / "results.mat"
+    kw = dict(multi_volume=True, next_filename="next.mat") if multi else {}
+    u32, v32 = rng.normal(0, 2, (2, 11, 13)).astype(np.float32), rng.normal(0, 2, (2, 11, 13)).astype(np.float32)
+    u32[1, 3, 4] = np.nan
+    u16, v16 = rng.normal(0, 2, (3, 23, 27)).astype(np.float32), rng.normal(0, 2, (3, 23, 27)).astype(np.float32)
+    with qlpiv.VelocityExporter(path, (384, 448), (32, 32), (16, 16), 0.5, total_maps=4,
+                                date_text=DATE, **kw) as ex:
+        ex.export_results([0, 2], u32, v32, (32, 32), 4, margins=(5, 6, 7, 8), start_offset=40)
+        ex.export_results([0, 1, 3], u16, v16, (16, 16), 4, margins=(1, 2, 3, 4), start_offset=40)
+    m = scipy.io.loadmat(path)
+    names = [k for k in m if k.startswith("velocities")]
+    assert names == ["velocities32x32", "velocities16x16"]
+    for name, (u, v, frames, marg) in {"velocities32x32": (u32, v32, [0, 2], (5, 6, 7, 8)),
+                                        "velocities16x16": (u16, v16, [0, 1, 3], (1, 2, 3, 4))}.items():
+        s = m[name][0, 0]
+        fields = list(s.dtype.names)
+        want = ["iaHeight", "iaWidth", "multiVolumeExport"] + (["multiVolumeStartOffset"] if multi else []) + \
+               ["margins", "u", "v"]
+        assert fields == want
+        assert s["multiVolumeExport"][0, 0] == (1 if multi else 0)
+        if multi:
+            assert s["multiVolumeStartOffset"][0, 0] == 40
+        mg = s["margins"][0, 0]
+        assert [int(mg[k][0, 0]) for k in ("top", "left", "bottom", "right")] == list(marg)
+        assert s["u"].shape == u.shape[1:] + (4,) and s["u"].dtype == np.float32
+        for k, fr in enumerate(frames):
+            assert np.array_equal(s["u"][:, :, fr], u[k], equal_nan=True)
+            assert np.array_equal(s["v"][:, :, fr], v[k])
+        for fr in set(range(4)) - set(frames):
+            assert not s["u"][:, :, fr].any()
