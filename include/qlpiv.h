@@ -439,15 +439,18 @@ int qlp_piv_run_dev(qlp_piv_plan *plan, const uint8_t *imgsA, const uint8_t *img
 /* The same loop for float frames (plan created with frame_type QLP_FRAME_F32;
  * imgsA, imgsB device, pairs x H x W floats): every level cuts float tiles
  * (ImageFloat.clipImageMatrix :278-384; the Micro warp of the order's sides,
- * maximum 65535; the Mini warp likewise) and correlates them in the exact form.
- * Warp modes QLP_WARP_NONE, QLP_WARP_MICRO and QLP_WARP_MINI (every order); the
- * Full warp returns QLP_ERR_UNSUPPORTED. */
+ * maximum 65535; the Mini warp likewise; the Full warp's tiles cut from the
+ * warped frames with the source frame's maximum, MatrixFloat.copyMatrixRegion
+ * :437-455) and correlates them in the exact form.  Every warp mode and order. */
 int qlp_piv_run_f32_dev(qlp_piv_plan *plan, const float *imgsA, const float *imgsB);
 /* The loop for 10/12/16-bit frames (frame_type QLP_FRAME_U16; imgsA, imgsB
  * device, pairs x H x W uint16): the frames are converted to float in plan
  * buffers and run as float frames with the ImageInt16 rules: clipped tiles take
  * the MatrixInt16 maximum (>= 16, MatrixInt16.java:16-41, :371-405), Micro-warped
- * tiles the bit-depth maximum 2^bits - 1 (ImageInt16.getMaximumValue :557-559). */
+ * tiles the bit-depth maximum 2^bits - 1 (ImageInt16.getMaximumValue :557-559).
+ * No warp, Micro or Mini; the reference's Full warp of these frames truncates
+ * each sample to a byte (MatrixInt16.setElement(float) :95-101) and returns
+ * QLP_ERR_UNSUPPORTED here. */
 int qlp_piv_run_u16_dev(qlp_piv_plan *plan, const uint16_t *imgsA, const uint16_t *imgsB);
 /* Device pointers of a level's results (valid after the run has completed):
  * u, v (pairs x n displacements, vertical then horizontal), peaks (pairs x n x

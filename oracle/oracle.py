@@ -110,6 +110,7 @@ def lib():
                                     c_int, _f32p, _f32p]
         L.or_warp_image.argtypes = [_u8p, c_int, c_int, _f32p, _f32p, _u8p]
         L.or_warp_image_side.argtypes = [_u8p, c_int, c_int, _f32p, _f32p, c_int, _u8p]
+        L.or_warp_image_side_f32.argtypes = [_f32p, c_int, c_int, _f32p, _f32p, c_int, _f32p]
         _lib = L
     return _lib
 
@@ -456,6 +457,15 @@ def warp_image(img, fu, fv, side=SIDE_B_SECOND):
     out = np.zeros_like(img)
     lib().or_warp_image_side(img, img.shape[0], img.shape[1], np.ascontiguousarray(fu, np.float32),
                              np.ascontiguousarray(fv, np.float32), int(side), out)
+    return out
+
+
+def warp_image_f32(img, fu, fv, side=SIDE_B_SECOND):
+    """Full warp of a float frame (ImageFloat): rounded samples as floats."""
+    img = np.ascontiguousarray(img, np.float32)
+    out = np.zeros_like(img)
+    lib().or_warp_image_side_f32(img, img.shape[0], img.shape[1], np.ascontiguousarray(fu, np.float32),
+                                 np.ascontiguousarray(fv, np.float32), int(side), out)
     return out
 
 

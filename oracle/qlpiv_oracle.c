@@ -2735,6 +2735,24 @@ void or_warp_image_side(const uint8_t *img, int H, int W, const float *fu, const
         }
 }
 
+/* The same for a float frame (ImageFloat): the rounded samples as floats. */
+void or_warp_image_side_f32(const float *img, int H, int W, const float *fu, const float *fv,
+                            int side, float *out)
+{
+    const or_frame fr = {NULL, img, NULL};
+    const int half = side == OR_SIDE_A_BOTH || side == OR_SIDE_B_BOTH;
+    const int first = side == OR_SIDE_A_FIRST || side == OR_SIDE_A_BOTH;
+    for (int i = 0; i < H; i++)
+        for (int j = 0; j < W; j++) {
+            float u = fu[(size_t)i * W + j], v = fv[(size_t)i * W + j];
+            if (half) {
+                u /= 2.0f;
+                v /= 2.0f;
+            }
+            out[(size_t)i * W + j] = warp_pixel_frame(&fr, H, W, u, v, (short)i, (short)j, first);
+        }
+}
+
 void or_warp_image(const uint8_t *img, int H, int W, const float *fu, const float *fv,
                    uint8_t *out)
 {

@@ -165,6 +165,14 @@ int launch_u16_to_f32(Context *ctx, const uint16_t *in, float *out, size_t n);
 int launch_warp_mini_f32(Context *ctx, int pairs, const float *imgs, int H, int W, int nI, int nJ,
                          int N, const int *top, const int *left, const float *u, const float *v,
                          float *tiles, float *maxv, int side, float wmax);
+// The Full warp of float frames (as launch_full_warp), and the maximum of each
+// pair's frame (0 -> 1) into per_pair and spread to the pair's n tiles in dst.
+int launch_full_warp_f32(Context *ctx, int pairs, const float *const *imgs, int H, int W, int nI,
+                         int nJ, int tH, int tW, const int *top_h, const int *left_h, const float *u,
+                         const float *v, void *work, float *const *outs, const int *sides,
+                         int n_sides);
+int launch_frame_max_spread(Context *ctx, const float *imgs, int pairs, int H, int W, int n,
+                            float *per_pair, float *dst);
 // Spline-field warps (qlpiv_splinewarp.hip).
 int launch_warp_mini(Context *ctx, int pairs, const uint8_t *imgs, int H, int W, int nI, int nJ,
                      int N, const int *top, const int *left, const float *u, const float *v,

@@ -155,6 +155,7 @@ struct qlp_piv_plan {
     // float and 10/12/16-bit frames: the A and B float tiles of every level
     // (sized for the largest); 10/12/16-bit frames converted to float
     float *fA = nullptr, *fB = nullptr, *fframes = nullptr;
+    float *fmaxp = nullptr;  // Full warp of float frames: each pair's frame maximum
     size_t ftiles_bytes = 0;
     // IAs of the largest IA-128 pass: sizes the context's tile-slot workspace
     int big_ias = 0;
@@ -190,10 +191,10 @@ static void plan_free(qlp_piv_plan *p)
             if (b) (void)hipFree(b);
     }
     for (void *b : {(void *)p->tA, (void *)p->tB, (void *)p->wimg, (void *)p->mA, (void *)p->mB,
-                    p->fwork, (void *)p->fA, (void *)p->fB, (void *)p->fframes})
+                    p->fwork, (void *)p->fA, (void *)p->fB, (void *)p->fframes, (void *)p->fmaxp})
         if (b) (void)hipFree(b);
     p->tA = p->tB = p->wimg = nullptr;
-    p->mA = p->mB = p->fA = p->fB = p->fframes = nullptr;
+    p->mA = p->mB = p->fA = p->fB = p->fframes = p->fmaxp = nullptr;
     p->fwork = nullptr;
     for (hipEvent_t e : p->ev) (void)hipEventDestroy(e);
     p->ev.clear();
@@ -303,12 +304,11 @@ static int plan_layout(int H, int W, int pairs, const qlp_piv_config *cfg, qlp_p
         cfg->frame_bits != 16)
         return fail(QLP_ERR_ARG, "piv_plan_create: 10/12/16-bit frames need a bit depth of 10, 12 or 16");
     const bool f32 = cfg->frame_type != QLP_FRAME_U8;
-    // the Full warp of an ImageFloat tiles with the source frame's maximum, and of an
-    // ImageInt16 stores each sample as (byte) into its short matrix
-    // (MatrixInt16.setElement(float) :95-101): not provided
-    if (f32 && cfg->warp_mode == QLP_WARP_FULL)
+    // the Full warp of an ImageInt16 stores each sample as (byte) into its short
+    // matrix (MatrixInt16.setElement(float) :95-101): not provided
+    if (cfg->frame_type == QLP_FRAME_U16 && cfg->warp_mode == QLP_WARP_FULL)
         return fail(QLP_ERR_UNSUPPORTED,
-                    "piv_plan_create: float and 10/12/16-bit frames take no warp, Micro or Mini");
+                    "piv_plan_create: 10/12/16-bit frames take no warp, Micro or Mini");
     if (f32 && cfg->xcorr_form != QLP_XCORR_EXACT)
         return fail(QLP_ERR_UNSUPPORTED, "piv_plan_create: float and 10/12/16-bit frames take the exact correlation form");
     const bool valid = cfg->valid.n_stages > 0;
@@ -347,7 +347,8 @@ static int plan_layout(int H, int W, int pairs, const qlp_piv_config *cfg, qlp_p
         need += 2 * p->tiles_bytes + 2 * p->ftiles_bytes + 2 * mx * 4;
     }
     if (cfg->warp_mode == QLP_WARP_FULL && p->fwork_bytes)
-        need += (size_t)full_frames(*cfg) * pairs * H * W + p->fwork_bytes;
+        need += (size_t)full_frames(*cfg) * pairs * H * W * (f32 ? sizeof(float) : 1) +
+                p->fwork_bytes + (f32 ? sizeof(float) * pairs : 0);
     if (cfg->frame_type == QLP_FRAME_U16) need += 2 * sizeof(float) * (size_t)pairs * H * W;
     return QLP_OK;
 }
@@ -438,8 +439,11 @@ int qlp_piv_plan_create(qlp_ctx *h, int H, int W, int pairs, const qlp_piv_confi
     if (!rc && cfg->frame_type == QLP_FRAME_U16)
         rc = check_hip(hipMalloc(&p->fframes, 2 * sizeof(float) * (size_t)pairs * H * W), "hipMalloc plan");
     if (!rc && cfg->warp_mode == QLP_WARP_FULL && p->fwork_bytes) {
-        rc = check_hip(hipMalloc(&p->wimg, (size_t)full_frames(*cfg) * pairs * H * W), "hipMalloc plan");
+        rc = check_hip(hipMalloc(&p->wimg, (size_t)full_frames(*cfg) * pairs * H * W *
+                                               (f32 ? sizeof(float) : 1)),
+                       "hipMalloc plan");
         if (!rc) rc = check_hip(hipMalloc(&p->fwork, p->fwork_bytes), "hipMalloc plan");
+        if (!rc && f32) rc = check_hip(hipMalloc(&p->fmaxp, sizeof(float) * pairs), "hipMalloc plan");
     }
     // the plan's IA-128 passes read the images and find peaks
     if (!rc && p->big_ias) rc = prepare_big(ctx, p->big_ias, cfg->norm_mode, !f32, true, SHAPE_AUTO);
@@ -505,6 +509,44 @@ static int float_tiles(qlp_piv_plan *p, const Level &L, size_t l, const float *i
     const int sa = order == QLP_WARP_ORDER_FIRST ? SIDE_A_FIRST : SIDE_A_BOTH;
     const int sb = order == QLP_WARP_ORDER_SECOND ? SIDE_B_SECOND : SIDE_B_BOTH;
     int rc = QLP_OK;
+    if (l > 0 && c.warp_mode == QLP_WARP_FULL) {
+        // warp whole frames, cut the tiles from them with the source frame's maximum
+        // (clipImageMatrix(0, 0, H, W) then copyMatrixRegion, BiLinearImageWarpingStrategy
+        // .warpAndClipImage :50-131, MatrixFloat.copyMatrixRegion :437-455)
+        const size_t frames = (size_t)p->pairs * p->H * p->W;
+        float *wA = (float *)p->wimg, *wB = wA + (order == QLP_WARP_ORDER_BOTH ? frames : 0);
+        const float *src[2];
+        float *dst[2];
+        int sides[2], ns = 0;
+        if (order != QLP_WARP_ORDER_SECOND) {
+            src[ns] = imgsA;
+            dst[ns] = wA;
+            sides[ns++] = sa;
+        }
+        if (order != QLP_WARP_ORDER_FIRST) {
+            src[ns] = imgsB;
+            dst[ns] = wB;
+            sides[ns++] = sb;
+        }
+        rc = launch_full_warp_f32(ctx, p->pairs, src, p->H, p->W, L.nI, L.nJ, L.N, L.N,
+                                  L.top_h.data(), L.left_h.data(), L.u, L.v, p->fwork, dst, sides, ns);
+        const size_t tN = (size_t)L.N * L.N, frame = (size_t)p->H * p->W;
+        for (int k = 0; k < p->pairs && !rc; k++) {
+            const size_t o = (size_t)k * L.n;
+            const float *a = (order != QLP_WARP_ORDER_SECOND ? wA : imgsA) + k * frame;
+            const float *b = (order != QLP_WARP_ORDER_FIRST ? wB : imgsB) + k * frame;
+            rc = launch_clip_f32(ctx, a, p->H, p->W, L.n, L.N, L.top, L.left, nullptr, nullptr,
+                                 p->fA + o * tN, p->mA + o, clamp16);
+            if (!rc)
+                rc = launch_clip_f32(ctx, b, p->H, p->W, L.n, L.N, L.top, L.left, nullptr, nullptr,
+                                     p->fB + o * tN, p->mB + o, clamp16);
+        }
+        if (!rc && order != QLP_WARP_ORDER_SECOND)
+            rc = launch_frame_max_spread(ctx, imgsA, p->pairs, p->H, p->W, L.n, p->fmaxp, p->mA);
+        if (!rc && order != QLP_WARP_ORDER_FIRST)
+            rc = launch_frame_max_spread(ctx, imgsB, p->pairs, p->H, p->W, L.n, p->fmaxp, p->mB);
+        return rc;
+    }
     if (l > 0 && c.warp_mode == QLP_WARP_MINI) {  // one launch covers every pair
         if (warpA)
             rc = launch_warp_mini_f32(ctx, p->pairs, imgsA, p->H, p->W, L.nI, L.nJ, L.N, L.top,

@@ -276,11 +276,12 @@ __global__ void full_slopes_kernel(int pairs, int nI, int nJ, const double *__re
 
 // One thread per pixel: the field there, then the sample of frame imgs for
 // side 'side' (doBiLinearWarp :143-183: BothImages halves the field).
+template <typename T, typename O>
 __global__ void full_warp_kernel(int pairs, int H, int W, int nI, int nJ,
                                  const double *__restrict__ ys, const double *__restrict__ xs,
                                  const double *__restrict__ f, const double *__restrict__ fy,
-                                 const double *__restrict__ fx, const uint8_t *__restrict__ imgs,
-                                 int side, uint8_t *__restrict__ out)
+                                 const double *__restrict__ fx, const T *__restrict__ imgs,
+                                 int side, O *__restrict__ out)
 {
     const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t plane = (size_t)H * W;
@@ -305,7 +306,51 @@ __global__ void full_warp_kernel(int pairs, int H, int W, int nI, int nJ,
         du = -du;
         dv = -dv;
     }
-    out[g] = (uint8_t)warp_pixel(imgs + pair * plane, H, W, du, dv, (short)i, (short)j);
+    out[g] = (O)warp_pixel(imgs + pair * plane, H, W, du, dv, (short)i, (short)j);
+}
+
+// Maximum over each pair's frame (strict >, from 0; 0 -> 1): the MatrixFloat
+// maximum of clipImageMatrix(0, 0, H, W) (MatrixFloat.java:45-60), which the
+// Full warp's tiles inherit (copyMatrixRegion :437-455).  One work-group per pair.
+__global__ __launch_bounds__(1024) void frame_max_kernel(const float *__restrict__ imgs, size_t plane,
+                                                         float *__restrict__ out)
+{
+    __shared__ float s_m[16];
+    const float *img = imgs + (size_t)blockIdx.x * plane;
+    float m = 0.0f;
+    for (size_t k = threadIdx.x; k < plane; k += blockDim.x)
+        if (img[k] > m) m = img[k];
+    for (int o = 32; o >= 1; o >>= 1) {
+        const float x = __shfl_xor(m, o, 64);
+        if (x > m) m = x;
+    }
+    if ((threadIdx.x & 63) == 0) s_m[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float r = 0.0f;
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++)
+            if (s_m[w] > r) r = s_m[w];
+        out[blockIdx.x] = r == 0.0f ? 1.0f : r;
+    }
+}
+
+// dst[pair * n + t] = per_pair[pair]
+__global__ void spread_kernel(const float *__restrict__ per_pair, int n, int pairs,
+                              float *__restrict__ dst)
+{
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < n * pairs) dst[g] = per_pair[g / n];
+}
+
+int launch_frame_max_spread(Context *ctx, const float *imgs, int pairs, int H, int W, int n,
+                            float *per_pair, float *dst)
+{
+    if (pairs <= 0) return QLP_OK;
+    hipLaunchKernelGGL(frame_max_kernel, dim3(pairs), dim3(1024), 0, ctx->stream, imgs,
+                       (size_t)H * W, per_pair);
+    hipLaunchKernelGGL(spread_kernel, dim3((n * pairs + 255) / 256), dim3(256), 0, ctx->stream,
+                       per_pair, n, pairs, dst);
+    return check_hip(hipGetLastError(), "frame maximum launch");
 }
 
 size_t full_warp_work_bytes(int pairs, int nI, int nJ)
@@ -315,10 +360,11 @@ size_t full_warp_work_bytes(int pairs, int nI, int nJ)
            (size_t)pairs * 2 * (nI + nJ) * 4 * L * 8;
 }
 
-int launch_full_warp(Context *ctx, int pairs, const uint8_t *const *imgs, int H, int W, int nI,
-                     int nJ, int tH, int tW, const int *top_h, const int *left_h, const float *u,
-                     const float *v, void *work, uint8_t *const *outs, const int *sides,
-                     int n_sides)
+template <typename T>
+static int launch_full_warp_t(Context *ctx, int pairs, const T *const *imgs, int H, int W, int nI,
+                              int nJ, int tH, int tW, const int *top_h, const int *left_h,
+                              const float *u, const float *v, void *work, T *const *outs,
+                              const int *sides, int n_sides)
 {
     if (pairs <= 0) return QLP_OK;
     if (nI < 2 || nJ < 2)
@@ -347,10 +393,28 @@ int launch_full_warp(Context *ctx, int pairs, const uint8_t *const *imgs, int H,
     }
     const size_t total = (size_t)pairs * H * W;
     for (int k = 0; k < n_sides; k++)
-        hipLaunchKernelGGL(full_warp_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
-                           ctx->stream, pairs, H, W, nI, nJ, ys, xs, f, fy, fx, imgs[k], sides[k],
+        hipLaunchKernelGGL((full_warp_kernel<T, T>), dim3((unsigned)((total + 255) / 256)), dim3(256),
+                           0, ctx->stream, pairs, H, W, nI, nJ, ys, xs, f, fy, fx, imgs[k], sides[k],
                            outs[k]);
     return check_hip(hipGetLastError(), "full warp launch");
+}
+
+int launch_full_warp(Context *ctx, int pairs, const uint8_t *const *imgs, int H, int W, int nI,
+                     int nJ, int tH, int tW, const int *top_h, const int *left_h, const float *u,
+                     const float *v, void *work, uint8_t *const *outs, const int *sides,
+                     int n_sides)
+{
+    return launch_full_warp_t<uint8_t>(ctx, pairs, imgs, H, W, nI, nJ, tH, tW, top_h, left_h, u, v,
+                                       work, outs, sides, n_sides);
+}
+
+int launch_full_warp_f32(Context *ctx, int pairs, const float *const *imgs, int H, int W, int nI,
+                         int nJ, int tH, int tW, const int *top_h, const int *left_h, const float *u,
+                         const float *v, void *work, float *const *outs, const int *sides,
+                         int n_sides)
+{
+    return launch_full_warp_t<float>(ctx, pairs, imgs, H, W, nI, nJ, tH, tW, top_h, left_h, u, v,
+                                     work, outs, sides, n_sides);
 }
 
 int warp_clip_tiles(Context *ctx, int mode, int order, const uint8_t *imgsA, const uint8_t *imgsB,

@@ -128,7 +128,8 @@ def direct_inherit(pu, nI, nJ, round_it=False):
     return out
 
 
-@pytest.mark.parametrize("warp,order", [(1, 0), (1, 2), (1, 1), (0, 0), (2, 0), (2, 2), (2, 1)])
+@pytest.mark.parametrize("warp,order", [(1, 0), (1, 2), (1, 1), (0, 0), (2, 0), (2, 2), (2, 1),
+                                        (3, 0), (3, 1), (3, 2)])
 def test_float_frame_plan_against_oracle(ctx, fpair, warp, order):
     """Every level of a float-frame plan (64 -> 16 px, Direct inheritance,
     FromCenter, Guo 3 px) equals the oracle restatement from the same inherited
@@ -158,17 +159,27 @@ def test_float_frame_plan_against_oracle(ctx, fpair, warp, order):
             vin = direct_inherit(prev_v, nI, nJ, round_it=(warp == 0))
         D = 2 * N - 1
         c = np.float32(D // 2)
+        if lv > 0 and warp == 3:
+            # Full warp: whole frames warped, tiles cut from them with the source
+            # frame's maximum
+            fu, fv = O.full_field(L["top"], L["left"], N, uin, vin, 256, 256)
+            wa = O.warp_image_f32(ha, fu, fv, sides_a[order]) if order in sides_a else None
+            wb = O.warp_image_f32(hb, fu, fv, sides_b[order]) if order in sides_b else None
         for i in range(nI):
             for j in range(nJ):
                 t, l0 = int(L["top"][i, j]), int(L["left"][i, j])
-                if lv > 0 and warp == 2 and order in sides_a:
+                if lv > 0 and warp == 3 and wa is not None:
+                    tA, mA = wa[t:t + N, l0:l0 + N].copy(), max(float(ha.max()), 1.0)
+                elif lv > 0 and warp == 2 and order in sides_a:
                     tA, mA = O.warp_mini_frame(ha, L["top"], L["left"], N, uin, vin, i, j,
                                                sides_a[order]), 65535.0
                 elif lv > 0 and warp and order in sides_a:
                     tA, mA = O.warp_micro_f32(ha, t, l0, N, uin[i, j], vin[i, j], sides_a[order])
                 else:
                     tA, mA = O.clip_tile_f32(ha, t, l0, N)
-                if lv > 0 and warp == 2 and order in sides_b:
+                if lv > 0 and warp == 3 and wb is not None:
+                    tB, mB = wb[t:t + N, l0:l0 + N].copy(), max(float(hb.max()), 1.0)
+                elif lv > 0 and warp == 2 and order in sides_b:
                     tB, mB = O.warp_mini_frame(hb, L["top"], L["left"], N, uin, vin, i, j,
                                                sides_b[order]), 65535.0
                 elif lv > 0 and warp and order in sides_b:
@@ -291,7 +302,7 @@ def test_float_frame_plan_errors(ctx):
 
     with pytest.raises(qlpiv.QlpError):
         qlpiv.PivPlan(ctx, 128, 128, 1, ia_start=32, ia_end=16, warp_mode=qlpiv._lib.QLP_WARP_FULL,
-                      frame_type=qlpiv._lib.QLP_FRAME_F32)
+                      frame_type=qlpiv._lib.QLP_FRAME_U16, frame_bits=12)
     with pytest.raises(qlpiv.QlpError):
         qlpiv.PivPlan(ctx, 128, 128, 1, ia_start=32, ia_end=16, xcorr_form=1,
                       frame_type=qlpiv._lib.QLP_FRAME_F32)

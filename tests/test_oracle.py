@@ -435,6 +435,29 @@ def test_u16_clip_and_warp_follow_the_8bit_rules():
     assert np.array_equal(t, big[23:39, 28:44].astype(np.float32))
 
 
+def test_float_full_and_mini_warps_follow_the_8bit_ones():
+    """Full and Mini warps over float and 16-bit frames: integer-valued frames
+    give the 8-bit warps' samples on every side."""
+    rng = np.random.default_rng(8)
+    img8 = rng.integers(0, 256, (64, 80)).astype(np.uint8)
+    fu = rng.uniform(-3, 3, (64, 80)).astype(np.float32)
+    fv = rng.uniform(-3, 3, (64, 80)).astype(np.float32)
+    nI, nJ, N = 3, 4, 16
+    top = (np.arange(nI)[:, None] * 16 + np.zeros((1, nJ), int)).astype(np.int32)
+    left = (np.zeros((nI, 1), int) + np.arange(nJ)[None, :] * 16).astype(np.int32)
+    u = rng.uniform(-2, 2, (nI, nJ)).astype(np.float32)
+    v = rng.uniform(-2, 2, (nI, nJ)).astype(np.float32)
+    for side in range(4):
+        a = O.warp_image(img8, fu, fv, side).astype(np.float32)
+        assert np.array_equal(a, O.warp_image_f32(img8.astype(np.float32), fu, fv, side))
+        for ti, tj in [(0, 0), (1, 2), (2, 3)]:
+            m8 = O.warp_mini(img8, top, left, N, u, v, ti, tj, side).astype(np.float32)
+            assert np.array_equal(m8, O.warp_mini_frame(img8.astype(np.float32), top, left, N, u, v,
+                                                         ti, tj, side))
+            assert np.array_equal(m8, O.warp_mini_frame(img8.astype(np.uint16), top, left, N, u, v,
+                                                         ti, tj, side))
+
+
 def test_warp_micro_invariants():
     rng = np.random.default_rng(2)
     img = rng.integers(0, 256, (96, 96)).astype(np.uint8)
