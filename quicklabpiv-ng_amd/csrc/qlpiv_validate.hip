@@ -497,24 +497,38 @@ __device__ void visit(const VParams &p, size_t base, int t, int &corrected, int 
     p.flags[gt] = inv ? F_INVALID : 0;
 }
 
+// A tile is ready once every neighbour visited before it in the spiral (and not
+// locked) has been visited in this sweep, in an earlier round.  All neighbour
+// loads are issued before any decision, so a check costs one memory round trip.
 __device__ bool ready(const VParams &p, size_t base, int t, int sweep_start, int round)
 {
     const int i = t / p.nJ, j = t - (t / p.nJ) * p.nJ;
     const int rt = p.rank[t];
-    for (int a = 0; a < 3; a++)
-        for (int b = 0; b < 3; b++) {
-            if (a == 1 && b == 1) continue;
-            const int k = nb_index(p, i, j, a, b);
-            if (k < 0 || p.rank[k] > rt) continue;
-            if (ldc(p.flags + base + k) & F_LOCKED) continue;
-            const int d = ldc(p.done + base + k);
-            if (d < sweep_start || d >= round) return false;
-        }
-    return true;
+    int k[8], rk[8], fl[8], dn[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        const int a = (q + (q >= 4)) / 3, b = (q + (q >= 4)) % 3;  // the 8 neighbours
+        k[q] = nb_index(p, i, j, a, b);
+    }
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        const int kk = k[q] < 0 ? t : k[q];
+        rk[q] = p.rank[kk];
+        fl[q] = ldc(p.flags + base + kk);
+        dn[q] = ldc(p.done + base + kk);
+    }
+    bool ok = true;
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        if (k[q] < 0 || rk[q] > rt || (fl[q] & F_LOCKED)) continue;
+        if (dn[q] < sweep_start || dn[q] >= round) ok = false;
+    }
+    return ok;
 }
 
 // 256 lanes: the active tiles are a few percent of the grid, and a smaller
-// group leaves the visit its registers (no spills)
+// group leaves the visit its registers (256 VGPRs, 80 bytes of scratch per
+// lane on gfx950; a larger group would spill far more)
 constexpr int SWEEP_THREADS = 256;
 
 __global__ __launch_bounds__(SWEEP_THREADS) void valid_sweep_kernel(VParams p)
