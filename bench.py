@@ -43,11 +43,13 @@ def pass_bytes(n_ia, N):
     return n_ia * (2 * N * N + 4 * (2 * N - 1) ** 2 + 8)
 
 
-def cpu_baseline(a, b, top, left, u, v, N, nI, nJ, stages):
+def cpu_baseline(a, b, levels, stages):
     """Reference CPU path per IA (clip A, micro-warp B, CrossCorrelationSoftRealFFTJob
-    arithmetic, FindMaximumMultiPeaks, Guo 3 px of every peak) and the level's
-    vector validation, restated in C by the oracle; one thread, over one
-    level's IAs of the same pair."""
+    arithmetic, FindMaximumMultiPeaks, Guo 3 px of every peak) and each level's
+    vector validation, restated in C by the oracle; one thread, over every level
+    of the same pair.  levels: (top, left, u, v, N, nI, nJ) per level, u and v
+    the displacements the level inherits (zero at the first level, where the B
+    tile's micro warp by zero is a plain clip)."""
     import ctypes
 
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -58,25 +60,32 @@ def cpu_baseline(a, b, top, left, u, v, N, nI, nJ, stages):
     L.or_cpu_level_batch_multi.argtypes = [O._u8p, O._u8p, ctypes.c_int, ctypes.c_int,
                                            ctypes.c_int, ctypes.c_int, i32p, i32p, O._f32p,
                                            O._f32p, ctypes.c_int, ctypes.c_int, O._f32p, i32p]
-    n = top.size
     P = 3
-    out = np.zeros((n, P, 4), np.float32)
-    cnt = np.zeros(n, np.int32)
-    L.or_cpu_level_batch_multi(a, b, H, W, 64, N, top[:64], left[:64], u[:64], v[:64], P, 3,
-                               out[:64], cnt[:64])
-    t0 = time.perf_counter()
-    L.or_cpu_level_batch_multi(a, b, H, W, n, N, top, left, u, v, P, 3, out, cnt)
-    c = np.float32(N - 1)
-    inc = (out[..., :2] - c).astype(np.float32)
-    up = (u + inc[:, 0, 0]).astype(np.float32).reshape(nI, nJ)
-    vp = (v + inc[:, 0, 1]).astype(np.float32).reshape(nI, nJ)
-    O.validate(top.reshape(nI, nJ), left.reshape(nI, nJ), up, vp, stages,
-               inc.reshape(nI, nJ, P, 2), cnt.reshape(nI, nJ), 0, False)
-    dt = time.perf_counter() - t0
-    return {"value": round(n / dt, 1), "unit": "IAs/s", "cores": 1, "kind": "port",
-            "sample": f"the {n} IAs of {N}x{N} of one 2048x2048 pair: clip, micro-warp, Java CPU "
-                      f"xcorr, MultiPeaks 3x3/3, Guo 3px per peak, multi-peak median "
-                      f"validation (C restatement, -O2 -ffp-contract=off, 1 thread), {dt:.2f} s"}
+    top, left, u, v, N = levels[0][:5]
+    out = np.zeros((64, P, 4), np.float32)
+    cnt = np.zeros(64, np.int32)
+    L.or_cpu_level_batch_multi(a, b, H, W, 64, N, top[:64], left[:64], u[:64], v[:64], P, 3, out, cnt)
+    total, dt = 0, 0.0
+    for top, left, u, v, N, nI, nJ in levels:
+        n = top.size
+        out = np.zeros((n, P, 4), np.float32)
+        cnt = np.zeros(n, np.int32)
+        t0 = time.perf_counter()
+        L.or_cpu_level_batch_multi(a, b, H, W, n, N, top, left, u, v, P, 3, out, cnt)
+        c = np.float32(N - 1)
+        inc = (out[..., :2] - c).astype(np.float32)
+        up = (u + inc[:, 0, 0]).astype(np.float32).reshape(nI, nJ)
+        vp = (v + inc[:, 0, 1]).astype(np.float32).reshape(nI, nJ)
+        O.validate(top.reshape(nI, nJ), left.reshape(nI, nJ), up, vp, stages,
+                   inc.reshape(nI, nJ, P, 2), cnt.reshape(nI, nJ), 0, False)
+        dt += time.perf_counter() - t0
+        total += n
+    sizes = "/".join(str(lv[4]) for lv in levels)
+    return {"value": round(total / dt, 1), "unit": "IAs/s", "cores": 1, "kind": "port",
+            "sample": f"all {total} IAs of one 2048x2048 pair ({sizes} px levels, each from the "
+                      f"device's inherited field): clip, micro-warp, Java CPU xcorr, MultiPeaks "
+                      f"3x3/3, Guo 3px per peak, multi-peak median validation (C restatement, "
+                      f"-O2 -ffp-contract=off, 1 thread), {dt:.2f} s"}
 
 
 def time_plan(ctx, plan, da, db, steps, warmup):
@@ -485,7 +494,7 @@ def main():
 
     mfma = None
     if not args.no_mfma:
-        print("[bench] MFMA form (fp16 hi+lo, 16/32 px levels)", file=sys.stderr, flush=True)
+        print("[bench] MFMA form (fp16 hi+lo, all three levels)", file=sys.stderr, flush=True)
         plan_m = qlpiv.PivPlan(ctx, H, W, 1, xcorr_form=qlpiv.QLP_XCORR_MFMA_F16, **c2)
         dt_m, step_m, per_m = time_plan(ctx, plan_m, da, db, args.steps, args.warmup)
         um, vm, _ = plan_m.result(len(plan_m.levels) - 1)
@@ -513,19 +522,22 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         print("[bench] timing the CPU baseline sample", file=sys.stderr, flush=True)
-        u, v, _ = plan.result(lv32 - 1)  # the displacements the 32 px level inherits
-        L = plan.levels[lv32]
-        P = plan.levels[lv32 - 1]
-        ui = np.zeros((L["nI"], L["nJ"]), np.float32)
-        vi = np.zeros_like(ui)
-        for i in range(L["nI"]):
-            for j in range(L["nJ"]):
-                if i // 2 < P["nI"] and j // 2 < P["nJ"]:
-                    ui[i, j] = u[0, i // 2, j // 2]
-                    vi[i, j] = v[0, i // 2, j // 2]
-        cpu = cpu_baseline(a, b, np.ascontiguousarray(L["top"].ravel(), np.int32),
+        levels = []
+        for lv, L in enumerate(plan.levels):
+            ui = np.zeros((L["nI"], L["nJ"]), np.float32)
+            vi = np.zeros_like(ui)
+            if lv > 0:  # Direct inheritance from the device's previous level
+                u, v, _ = plan.result(lv - 1)
+                P = plan.levels[lv - 1]
+                for i in range(L["nI"]):
+                    for j in range(L["nJ"]):
+                        if i // 2 < P["nI"] and j // 2 < P["nJ"]:
+                            ui[i, j] = u[0, i // 2, j // 2]
+                            vi[i, j] = v[0, i // 2, j // 2]
+            levels.append((np.ascontiguousarray(L["top"].ravel(), np.int32),
                            np.ascontiguousarray(L["left"].ravel(), np.int32), ui.ravel(), vi.ravel(),
-                           32, L["nI"], L["nJ"], stages)
+                           L["ia"], L["nI"], L["nJ"]))
+        cpu = cpu_baseline(a, b, levels, stages)
     plan.close()
     ctx.close()
     value = n_step * args.steps / dt
