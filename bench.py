@@ -565,7 +565,7 @@ def main():
                    "level_ms": level_ms(plan, per_level),
                    "ias_per_s_at_32x32": round(n_levels[lv32] / (float(per_level[lv32].sum()) * 1e-3), 1)},
         "mfma_form": mfma,
-        "roofline": {"bound": "hbm", "kernel": "xcorr_pass_kernel<6,false,true,false> (32x32 level)",
+        "roofline": {"bound": "hbm", "kernel": "xcorr_pass_kernel<6,false,true,false,512> (32x32 level, 512 lanes)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "avg_launch_ms": round(pass32_ms, 4),

@@ -1,6 +1,7 @@
 // C ABI of libqlpiv: contexts, errors, host-buffer wrappers.
 #include <math.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <string>
@@ -114,6 +115,7 @@ int qlp_ctx_create(int hip_device, size_t hbm_budget_bytes, qlp_ctx **out)
     ctx->hbm_budget = hbm_budget_bytes;
     const char *form = getenv("QLPIV_XCORR_FORM");
     ctx->xcorr_form = (form && strcmp(form, "lds") == 0) ? FORM_LDS : FORM_LANE;
+    if (const char *wide = getenv("QLPIV_PASS_WIDE")) ctx->pass_wide = atoi(wide) != 0;
     rc = check_hip(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "hipStreamCreate");
     if (rc) {
         delete h;

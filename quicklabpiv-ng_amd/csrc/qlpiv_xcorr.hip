@@ -761,6 +761,17 @@ static void launch_pass_mc(Context *ctx, const PassParams &p, bool from_image, b
                                0, ctx->stream, p, twf, twi);
         return;
     }
+    if (from_image && ctx->pass_wide && LOGM >= 6) {
+        // twice the default work-group: every line of a 1D pass in one round
+        constexpr int T2 = X::THREADS * 2;
+        if (peak)
+            hipLaunchKernelGGL((xcorr_pass_kernel<LOGM, MODE_CPU, true, true, T2>), grid, dim3(T2), 0,
+                               ctx->stream, p, twf, twi);
+        else
+            hipLaunchKernelGGL((xcorr_pass_kernel<LOGM, MODE_CPU, true, false, T2>), grid, dim3(T2), 0,
+                               ctx->stream, p, twf, twi);
+        return;
+    }
     if (from_image) {
         if (peak)
             hipLaunchKernelGGL((xcorr_pass_kernel<LOGM, MODE_CPU, true, true>), grid, block, 0,
