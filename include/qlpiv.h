@@ -495,6 +495,7 @@ int qlp_piv_plan_estimate(int H, int W, int pairs, const qlp_piv_config *cfg, si
 #define QLP_SHAPE_LANE 1      /* one lane per transform line (16 px IAs) */
 #define QLP_SHAPE_LDS 2       /* one work-group per IA, default size */
 #define QLP_SHAPE_LDS_HALF 3  /* one work-group per IA, half size */
+#define QLP_SHAPE_LDS_WIDE 4  /* one work-group per IA, twice the size (16, 32, 64 px) */
 #define QLP_SCHEDULE_TUNED (-1)
 #define QLP_SCHEDULE_SERIAL 0
 #define QLP_SCHEDULE_CONCURRENT 1

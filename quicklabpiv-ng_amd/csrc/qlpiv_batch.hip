@@ -116,11 +116,14 @@ static int shape_candidates(int N, int arith, int *out)
         out[0] = SHAPE_LANE;
         out[1] = SHAPE_LDS;
         out[2] = SHAPE_LDS_HALF;
-        return 3;
+        out[3] = SHAPE_LDS_WIDE;
+        return 4;
     }
     out[0] = SHAPE_LDS;
     out[1] = SHAPE_LDS_HALF;
-    return 2;
+    if (N == 128) return 2;
+    out[2] = SHAPE_LDS_WIDE;
+    return 3;
 }
 
 // Key of the context's shape cache: IA size, IA count rounded up to a power of
@@ -538,7 +541,7 @@ int qlp_batch_fix(qlp_batch *b, int ia, int shape, int schedule)
 {
     if (!b) return fail(QLP_ERR_ARG, "null batch");
     if (b->built) return fail(QLP_ERR_ARG, "batch_fix: the batch has already run");
-    if (shape < QLP_SHAPE_AUTO || shape > QLP_SHAPE_LDS_HALF)
+    if (shape < QLP_SHAPE_AUTO || shape > QLP_SHAPE_LDS_WIDE)
         return fail(QLP_ERR_ARG, "batch_fix: unknown shape");
     if (schedule < QLP_SCHEDULE_TUNED || schedule > QLP_SCHEDULE_CONCURRENT)
         return fail(QLP_ERR_ARG, "batch_fix: unknown schedule");

@@ -92,6 +92,7 @@ enum PassShape {
                          // (N = 128: 512 threads, 32-line batches)
     SHAPE_LDS_HALF = 3,  // half the work-group size, tiles input only
                          // (N = 128: 256 threads, 16-line batches)
+    SHAPE_LDS_WIDE = 4,  // twice the work-group size, tiles input only (N <= 64)
 };
 
 // One correlation pass over n IA pairs.  Tiles come either from u8 tile arrays

@@ -751,6 +751,16 @@ static void launch_pass_mc(Context *ctx, const PassParams &p, bool from_image, b
     const float2 *twf = ctx->tables.tw[li][TW_FWD];
     const float2 *twi = ctx->tables.tw[li][MODE_CPU ? TW_INV_CPU : TW_INV_GPU];
     dim3 grid(p.n), block(X::THREADS);
+    if (p.shape == SHAPE_LDS_WIDE && !from_image) {
+        constexpr int T2 = X::THREADS * 2;
+        if (peak)
+            hipLaunchKernelGGL((xcorr_pass_kernel<LOGM, MODE_CPU, false, true, T2>), grid, dim3(T2),
+                               0, ctx->stream, p, twf, twi);
+        else
+            hipLaunchKernelGGL((xcorr_pass_kernel<LOGM, MODE_CPU, false, false, T2>), grid, dim3(T2),
+                               0, ctx->stream, p, twf, twi);
+        return;
+    }
     if (p.shape == SHAPE_LDS_HALF && !from_image) {
         constexpr int H = X::THREADS / 2;
         if (peak)

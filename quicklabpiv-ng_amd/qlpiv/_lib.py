@@ -119,6 +119,7 @@ QLP_SHAPE_AUTO = 0
 QLP_SHAPE_LANE = 1
 QLP_SHAPE_LDS = 2
 QLP_SHAPE_LDS_HALF = 3
+QLP_SHAPE_LDS_WIDE = 4
 QLP_SCHEDULE_TUNED = -1
 QLP_SCHEDULE_SERIAL = 0
 QLP_SCHEDULE_CONCURRENT = 1

@@ -67,7 +67,7 @@ def check_equal(batch, ref):
 
 
 CASES = [("tuned", None, -1), ("lds serial", 2, 0), ("half concurrent", 3, 1),
-         ("lane concurrent", 1, 1)]
+         ("lane concurrent", 1, 1), ("wide serial", 4, 0)]
 
 
 @pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
