@@ -751,7 +751,10 @@ static void launch_pass_mc(Context *ctx, const PassParams &p, bool from_image, b
     const float2 *twf = ctx->tables.tw[li][TW_FWD];
     const float2 *twi = ctx->tables.tw[li][MODE_CPU ? TW_INV_CPU : TW_INV_GPU];
     dim3 grid(p.n), block(X::THREADS);
-    if (p.shape == SHAPE_LDS_WIDE && !from_image) {
+    // from tiles, the default shape is the wide one too (the C3 batch tuner measured
+    // it faster at 32 and 64 px: 2.17 vs 2.31 ms and 3.21 vs 3.95 ms per class)
+    if (!from_image && (p.shape == SHAPE_LDS_WIDE ||
+                        (p.shape == SHAPE_AUTO && ctx->pass_wide && LOGM >= 6))) {
         constexpr int T2 = X::THREADS * 2;
         if (peak)
             hipLaunchKernelGGL((xcorr_pass_kernel<LOGM, MODE_CPU, false, true, T2>), grid, dim3(T2),
