@@ -18,7 +18,9 @@ import struct
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "_build", "libqlpiv_oracle.so")
+# QLPIV_ORACLE_LIB selects another build of the same source (the sanitizer
+# build of tests/sanitize/).
+_LIB_PATH = os.environ.get("QLPIV_ORACLE_LIB") or os.path.join(_HERE, "_build", "libqlpiv_oracle.so")
 _lib = None
 
 _f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")

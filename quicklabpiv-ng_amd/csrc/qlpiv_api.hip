@@ -4,6 +4,8 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <new>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -19,6 +21,22 @@ int fail(int code, const std::string &msg)
 {
     set_error(msg);
     return code;
+}
+
+// The status of an exception that reached a C-ABI entry point (every entry
+// point's body is a function-try-block ending here): no C++ exception may
+// cross into a JNI or ctypes caller.
+int fail_exception()
+{
+    try {
+        throw;
+    } catch (const std::bad_alloc &) {
+        return fail(QLP_ERR_COMPUTE, "host memory allocation failed");
+    } catch (const std::exception &e) {
+        return fail(QLP_ERR_COMPUTE, std::string("internal error: ") + e.what());
+    } catch (...) {
+        return fail(QLP_ERR_COMPUTE, "internal error: unknown exception");
+    }
 }
 
 int check_hip(hipError_t e, const char *what)
@@ -99,7 +117,7 @@ const char *qlp_last_error(void) { return g_last_error.c_str(); }
 const char *qlp_version(void) { return "qlpiv-mi355x 0.1 (gfx950)"; }
 
 int qlp_ctx_create(int hip_device, size_t hbm_budget_bytes, qlp_ctx **out)
-{
+try {
     if (!out) return fail(QLP_ERR_ARG, "qlp_ctx_create: null output pointer");
     *out = nullptr;
     int ndev = 0;
@@ -142,10 +160,12 @@ int qlp_ctx_create(int hip_device, size_t hbm_budget_bytes, qlp_ctx **out)
     }
     *out = h;
     return QLP_OK;
+} catch (...) {
+    return qlpiv::fail_exception();
 }
 
 int qlp_ctx_destroy(qlp_ctx *h)
-{
+try {
     if (!h) return QLP_OK;
     Context *ctx = &h->c;
     (void)hipSetDevice(ctx->device);
@@ -163,12 +183,16 @@ int qlp_ctx_destroy(qlp_ctx *h)
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete h;
     return QLP_OK;
+} catch (...) {
+    return qlpiv::fail_exception();
 }
 
 int qlp_ctx_synchronize(qlp_ctx *h)
-{
+try {
     if (!h) return fail(QLP_ERR_ARG, "null context");
     return sync_ctx(&h->c);
+} catch (...) {
+    return qlpiv::fail_exception();
 }
 
 void *qlp_ctx_stream(qlp_ctx *h) { return h ? (void *)h->c.stream : nullptr; }
@@ -185,18 +209,20 @@ static int xcorr_args(int n, int N, int mode)
 
 int qlp_xcorr_batch_dev(qlp_ctx *h, int n, int N, const uint8_t *tA, const uint8_t *tB,
                         const float *maxA, const float *maxB, int mode, float *planes)
-{
+try {
     if (!h) return fail(QLP_ERR_ARG, "null context");
     int rc = xcorr_args(n, N, mode);
     if (rc) return rc;
     if (n > 0 && (!tA || !tB || !maxA || !planes || (mode == QLP_NORM_PER_TILE && !maxB)))
         return fail(QLP_ERR_ARG, "xcorr: null buffer");
     return launch_xcorr(&h->c, n, N, tA, tB, maxA, maxB, mode, planes);
+} catch (...) {
+    return qlpiv::fail_exception();
 }
 
 int qlp_xcorr_f32_dev(qlp_ctx *h, int n, int N, const float *tA, const float *tB, const float *maxA,
                       const float *maxB, int mode, float *planes)
-{
+try {
     if (!h) return fail(QLP_ERR_ARG, "null context");
     int rc = xcorr_args(n, N, mode);
     if (rc || n == 0) return rc;
@@ -212,11 +238,13 @@ int qlp_xcorr_f32_dev(qlp_ctx *h, int n, int N, const float *tA, const float *tB
     p.maxB = maxB ? maxB : maxA;
     p.planes = planes;
     return launch_pass(&h->c, p);
+} catch (...) {
+    return qlpiv::fail_exception();
 }
 
 int qlp_xcorr_f32(qlp_ctx *h, int n, int N, const float *tA, const float *tB, const float *maxA,
                   const float *maxB, int mode, float *planes)
-{
+try {
     if (!h) return fail(QLP_ERR_ARG, "null context");
     int rc = xcorr_args(n, N, mode);
     if (rc || n == 0) return rc;
@@ -246,11 +274,13 @@ int qlp_xcorr_f32(qlp_ctx *h, int n, int N, const float *tA, const float *tB, co
                    "copy planes");
     if (rc) return rc;
     return sync_ctx(ctx);
+} catch (...) {
+    return qlpiv::fail_exception();
 }
 
 int qlp_xcorr_form_dev(qlp_ctx *h, int form, int n, int N, const uint8_t *tA, const uint8_t *tB,
                        const float *maxA, const float *maxB, int mode, float *planes)
-{
+try {
     if (form == QLP_XCORR_EXACT) return qlp_xcorr_batch_dev(h, n, N, tA, tB, maxA, maxB, mode, planes);
     if (!h) return fail(QLP_ERR_ARG, "null context");
     int rc = xcorr_args(n, N, mode);
@@ -271,11 +301,13 @@ int qlp_xcorr_form_dev(qlp_ctx *h, int form, int n, int N, const uint8_t *tA, co
     p.maxB = mode == QLP_NORM_PER_TILE ? maxB : maxA;
     p.planes = planes;
     return launch_pass_mfma(&h->c, p, form, false, false);
+} catch (...) {
+    return qlpiv::fail_exception();
 }
 
 int qlp_xcorr_batch(qlp_ctx *h, int n, int N, const uint8_t *tA, const uint8_t *tB,
                     const float *maxA, const float *maxB, int mode, float *planes)
-{
+try {
     if (!h) return fail(QLP_ERR_ARG, "null context");
     int rc = xcorr_args(n, N, mode);
     if (rc || n == 0) return rc;
@@ -304,21 +336,25 @@ int qlp_xcorr_batch(qlp_ctx *h, int n, int N, const uint8_t *tA, const uint8_t *
                    "copy planes");
     if (rc) return rc;
     return sync_ctx(ctx);
+} catch (...) {
+    return qlpiv::fail_exception();
 }
 
 int qlp_find_peaks_dev(qlp_ctx *h, int kind, int n, int D, const float *planes, int n_peaks,
                        int kernel_width, float *peaks, float *min_floor)
-{
+try {
     if (!h) return fail(QLP_ERR_ARG, "null context");
     if (n < 0) return fail(QLP_ERR_ARG, "find_peaks: negative count");
     if (kind == QLP_PEAK_FROM_CENTER && n_peaks != 1)
         return fail(QLP_ERR_ARG, "find_peaks: FromCenter finds exactly one peak");
     return launch_find_peaks(&h->c, kind, n, D, planes, n_peaks, kernel_width, peaks, min_floor);
+} catch (...) {
+    return qlpiv::fail_exception();
 }
 
 int qlp_find_peaks(qlp_ctx *h, int kind, int n, int D, const float *planes, int n_peaks,
                    int kernel_width, float *peaks, float *min_floor)
-{
+try {
     if (!h) return fail(QLP_ERR_ARG, "null context");
     if (n < 0 || n_peaks < 1) return fail(QLP_ERR_ARG, "find_peaks: bad counts");
     if (n == 0) return QLP_OK;
@@ -338,23 +374,27 @@ int qlp_find_peaks(qlp_ctx *h, int kind, int n, int D, const float *planes, int 
         (rc = check_hip(hipMemcpyAsync(min_floor, dF, sizeof(float) * n, hipMemcpyDeviceToHost, s), "copy floor")))
         return rc;
     return sync_ctx(ctx);
+} catch (...) {
+    return qlpiv::fail_exception();
 }
 
 int qlp_subpixel_dev(qlp_ctx *h, int kind, int pixels, int iterations, int n, int D, int n_peaks,
                      const float *planes, const float *min_floor, const float *peaks_in,
                      float *peaks_out)
-{
+try {
     if (!h) return fail(QLP_ERR_ARG, "null context");
     if (n < 0 || n_peaks < 1) return fail(QLP_ERR_ARG, "subpixel: bad counts");
     if (kind < 0) return fail(QLP_ERR_ARG, "subpixel: unknown interpolator kind");
     return launch_subpixel(&h->c, kind, pixels, iterations, n, D, n_peaks, planes, min_floor,
                            peaks_in, peaks_out);
+} catch (...) {
+    return qlpiv::fail_exception();
 }
 
 int qlp_subpixel(qlp_ctx *h, int kind, int pixels, int iterations, int n, int D, int n_peaks,
                  const float *planes, const float *min_floor, const float *peaks_in,
                  float *peaks_out)
-{
+try {
     if (!h) return fail(QLP_ERR_ARG, "null context");
     if (n < 0 || n_peaks < 1) return fail(QLP_ERR_ARG, "subpixel: bad counts");
     if (kind < 0) return fail(QLP_ERR_ARG, "subpixel: unknown interpolator kind");
@@ -378,51 +418,61 @@ int qlp_subpixel(qlp_ctx *h, int kind, int pixels, int iterations, int n, int D,
     rc = check_hip(hipMemcpyAsync(peaks_out, dO, sizeof(float) * 4 * (size_t)n * n_peaks, hipMemcpyDeviceToHost, s), "copy out");
     if (rc) return rc;
     return sync_ctx(ctx);
+} catch (...) {
+    return qlpiv::fail_exception();
 }
 
 int qlp_clip_tiles_dev(qlp_ctx *h, const uint8_t *img, int H, int W, int n, int N, const int *top,
                        const int *left, uint8_t *tiles, float *max_out)
-{
+try {
     if (!h) return fail(QLP_ERR_ARG, "null context");
     if (n < 0 || N < 1 || H < 1 || W < 1) return fail(QLP_ERR_ARG, "clip: bad sizes");
     return launch_clip(&h->c, img, H, W, n, N, top, left, tiles, max_out);
+} catch (...) {
+    return qlpiv::fail_exception();
 }
 
 int qlp_warp_micro_tiles_dev(qlp_ctx *h, const uint8_t *img, int H, int W, int n, int N,
                              const int *top, const int *left, const float *u, const float *v,
                              uint8_t *tiles, float *max_out)
-{
+try {
     if (!h) return fail(QLP_ERR_ARG, "null context");
     if (n < 0 || N < 1 || H < 1 || W < 1) return fail(QLP_ERR_ARG, "warp: bad sizes");
     return launch_warp_micro(&h->c, img, H, W, n, N, top, left, u, v, tiles, max_out);
+} catch (...) {
+    return qlpiv::fail_exception();
 }
 
 int qlp_clip_tiles_f32_dev(qlp_ctx *h, const float *img, int H, int W, int n, int N, const int *top,
                            const int *left, const float *u, const float *v, float *tiles,
                            float *max_out)
-{
+try {
     if (!h) return fail(QLP_ERR_ARG, "null context");
     if (n < 0 || N < 1 || H < 1 || W < 1) return fail(QLP_ERR_ARG, "clip: bad sizes");
     if (n > 0 && (!img || !top || !left || !tiles || !max_out)) return fail(QLP_ERR_ARG, "clip: null buffer");
     return launch_clip_f32(&h->c, img, H, W, n, N, top, left, u, v, tiles, max_out);
+} catch (...) {
+    return qlpiv::fail_exception();
 }
 
 int qlp_warp_micro_tiles_f32_dev(qlp_ctx *h, const float *img, int H, int W, int n, int N,
                                  const int *top, const int *left, const float *u, const float *v,
                                  int side, float *tiles, float *max_out)
-{
+try {
     if (!h) return fail(QLP_ERR_ARG, "null context");
     if (n < 0 || N < 1 || H < 1 || W < 1) return fail(QLP_ERR_ARG, "warp: bad sizes");
     if (side < QLP_SIDE_B_SECOND || side > QLP_SIDE_B_BOTH) return fail(QLP_ERR_ARG, "warp: unknown side");
     if (n > 0 && (!img || !top || !left || !u || !v || !tiles)) return fail(QLP_ERR_ARG, "warp: null buffer");
     return launch_warp_micro_f32(&h->c, img, H, W, n, N, top, left, u, v, tiles, max_out, side);
+} catch (...) {
+    return qlpiv::fail_exception();
 }
 
 int qlp_xcorr_images_form_dev(qlp_ctx *h, int form, int n, int N, int mode, const uint8_t *imgA,
                               const uint8_t *imgB, int H, int W, const int *top, const int *left,
                               const float *du, const float *dv, int warp_mode, float *planes,
                               float *peaks, float *min_floor, float *nbr, int nbr_half)
-{
+try {
     if (!h) return fail(QLP_ERR_ARG, "null context");
     if (form < QLP_XCORR_EXACT || form > QLP_XCORR_MFMA_BF16_SINGLE)
         return fail(QLP_ERR_ARG, "xcorr: unknown correlation form");
@@ -455,15 +505,19 @@ int qlp_xcorr_images_form_dev(qlp_ctx *h, int form, int n, int N, int mode, cons
     p.nbr_half = nbr_half;
     p.arith = form;
     return launch_pass(&h->c, p);
+} catch (...) {
+    return qlpiv::fail_exception();
 }
 
 int qlp_xcorr_images_dev(qlp_ctx *h, int n, int N, int mode, const uint8_t *imgA,
                          const uint8_t *imgB, int H, int W, const int *top, const int *left,
                          const float *du, const float *dv, int warp_mode, float *planes,
                          float *peaks, float *min_floor, float *nbr, int nbr_half)
-{
+try {
     return qlp_xcorr_images_form_dev(h, QLP_XCORR_EXACT, n, N, mode, imgA, imgB, H, W, top, left,
                                      du, dv, warp_mode, planes, peaks, min_floor, nbr, nbr_half);
+} catch (...) {
+    return qlpiv::fail_exception();
 }
 
 }  // extern "C"

@@ -415,7 +415,7 @@ extern "C" {
 
 int qlp_batch_create(qlp_ctx *h, int n_pairs, const qlp_batch_pair *pairs,
                      const qlp_piv_config *cfg, qlp_batch **out)
-{
+try {
     if (!h || !cfg || !out || (n_pairs > 0 && !pairs)) return fail(QLP_ERR_ARG, "batch_create: null argument");
     *out = nullptr;
     if (n_pairs < 1) return fail(QLP_ERR_ARG, "batch_create: no pairs");
@@ -535,10 +535,12 @@ int qlp_batch_create(qlp_ctx *h, int n_pairs, const qlp_batch_pair *pairs,
     }
     *out = b;
     return QLP_OK;
+} catch (...) {
+    return qlpiv::fail_exception();
 }
 
 int qlp_batch_fix(qlp_batch *b, int ia, int shape, int schedule)
-{
+try {
     if (!b) return fail(QLP_ERR_ARG, "null batch");
     if (b->built) return fail(QLP_ERR_ARG, "batch_fix: the batch has already run");
     if (shape < QLP_SHAPE_AUTO || shape > QLP_SHAPE_LDS_WIDE)
@@ -559,10 +561,12 @@ int qlp_batch_fix(qlp_batch *b, int ia, int shape, int schedule)
         b->schedule_fixed = true;
     }
     return QLP_OK;
+} catch (...) {
+    return qlpiv::fail_exception();
 }
 
 int qlp_batch_run_dev(qlp_batch *b, const uint8_t *const *imgsA, const uint8_t *const *imgsB)
-{
+try {
     if (!b || !imgsA || !imgsB) return fail(QLP_ERR_ARG, "batch_run: null argument");
     for (int p = 0; p < b->npairs; p++)
         if (!imgsA[p] || !imgsB[p]) return fail(QLP_ERR_ARG, "batch_run: null frame");
@@ -582,21 +586,25 @@ int qlp_batch_run_dev(qlp_batch *b, const uint8_t *const *imgsA, const uint8_t *
         if (rc) return rc;
     }
     return check_hip(hipGraphLaunch(b->exec, ctx->stream), "batch graph launch");
+} catch (...) {
+    return qlpiv::fail_exception();
 }
 
 int qlp_batch_grid(qlp_batch *b, int pair, int *nI, int *nJ, size_t *total_ias)
-{
+try {
     if (!b) return fail(QLP_ERR_ARG, "null batch");
     if (pair < 0 || pair >= b->npairs) return fail(QLP_ERR_ARG, "batch: pair out of range");
     if (nI) *nI = b->slot[pair].nI;
     if (nJ) *nJ = b->slot[pair].nJ;
     if (total_ias) *total_ias = b->total;
     return QLP_OK;
+} catch (...) {
+    return qlpiv::fail_exception();
 }
 
 int qlp_batch_result_dev(qlp_batch *b, int pair, const float **u, const float **v,
                          const float **sub)
-{
+try {
     if (!b) return fail(QLP_ERR_ARG, "null batch");
     if (pair < 0 || pair >= b->npairs) return fail(QLP_ERR_ARG, "batch: pair out of range");
     const PairSlot &s = b->slot[pair];
@@ -605,10 +613,12 @@ int qlp_batch_result_dev(qlp_batch *b, int pair, const float **u, const float **
     if (v) *v = c.v + s.first;
     if (sub) *sub = c.sub + (size_t)s.first * 4;
     return QLP_OK;
+} catch (...) {
+    return qlpiv::fail_exception();
 }
 
 int qlp_batch_report_get(qlp_batch *b, qlp_batch_report *r)
-{
+try {
     if (!b || !r) return fail(QLP_ERR_ARG, "batch_report: null argument");
     memset(r, 0, sizeof(*r));
     for (const BatchClass &c : b->cls) {
@@ -627,15 +637,19 @@ int qlp_batch_report_get(qlp_batch *b, qlp_batch_report *r)
     r->serial_ms = b->serial_ms;
     r->concurrent_ms = b->concurrent_ms;
     return QLP_OK;
+} catch (...) {
+    return qlpiv::fail_exception();
 }
 
 int qlp_batch_destroy(qlp_batch *b)
-{
+try {
     if (!b) return QLP_OK;
     (void)hipStreamSynchronize(b->ctx->stream);
     batch_free(b);
     delete b;
     return QLP_OK;
+} catch (...) {
+    return qlpiv::fail_exception();
 }
 
 }  // extern "C"

@@ -29,6 +29,7 @@
 
 namespace qlpiv {
 int fail(int code, const std::string &msg);
+int fail_exception();
 }
 
 namespace {
@@ -174,11 +175,19 @@ struct qlp_exporter {
 
 using qlpiv::fail;
 
+// A MAT-file element records its byte size in 32 bits (and the reference's
+// float[] arrays hold at most 2^31 - 1 values): an nI x nJ x maps single array
+// must stay below 2^31 bytes.
+static bool array_fits(int nI, int nJ, int maps)
+{
+    return (uint64_t)nI * (uint64_t)nJ * (uint64_t)maps <= 0x7fffffffu / 4;
+}
+
 extern "C" {
 
 int qlp_export_open(const char *filename, const qlp_export_params *p, const char *date_text,
                     qlp_exporter **out)
-{
+try {
     if (!filename || !p || !out) return fail(QLP_ERR_ARG, "export_open: null argument");
     *out = nullptr;
     if (p->total_maps < 1) return fail(QLP_ERR_ARG, "export_open: total_maps must be >= 1");
@@ -212,17 +221,21 @@ int qlp_export_open(const char *filename, const qlp_export_params *p, const char
     }
     *out = e.release();
     return QLP_OK;
+} catch (...) {
+    return qlpiv::fail_exception();
 }
 
 int qlp_export_frame(qlp_exporter *e, int frame, int ia_height, int ia_width, int adaptive_step,
                      const int margins[4], int nI, int nJ, const float *u, const float *v,
                      const uint8_t *invalid)
-{
+try {
     if (!e || !margins || !u || !v) return fail(QLP_ERR_ARG, "export_frame: null argument");
     if (nI < 1 || nJ < 1 || ia_height < 1 || ia_width < 1)
         return fail(QLP_ERR_ARG, "export_frame: bad sizes");
     if (frame < 0 || frame >= e->total_maps)
         return fail(QLP_ERR_ARG, "Invalid frame number was specified");
+    if (!array_fits(nI, nJ, e->total_maps))
+        return fail(QLP_ERR_ARG, "export_frame: velocity array exceeds a MAT-file element");
     auto it = e->by_height.find(ia_height);
     Element *vel;
     if (it == e->by_height.end()) {
@@ -266,16 +279,20 @@ int qlp_export_frame(qlp_exporter *e, int frame, int ia_height, int ia_width, in
             memcpy(&ev->data[idx], &fv, 4);
         }
     return QLP_OK;
+} catch (...) {
+    return qlpiv::fail_exception();
 }
 
 int qlp_export_results(qlp_exporter *e, int ia_height, int ia_width, const int margins[4], int nI,
                        int nJ, int n_maps, int n_frames, const int *frame_index, const float *u,
                        const float *v, int start_offset)
-{
+try {
     if (!e || !margins || (n_frames > 0 && (!frame_index || !u || !v)))
         return fail(QLP_ERR_ARG, "export_results: null argument");
     if (nI < 1 || nJ < 1 || ia_height < 1 || ia_width < 1 || n_maps < 1 || n_frames < 0)
         return fail(QLP_ERR_ARG, "export_results: bad sizes");
+    if (!array_fits(nI, nJ, n_maps))
+        return fail(QLP_ERR_ARG, "export_results: velocity array exceeds a MAT-file element");
     auto s = structure();
     s->add("iaHeight", int32_scalar(ia_height));
     s->add("iaWidth", int32_scalar(ia_width));
@@ -311,10 +328,12 @@ int qlp_export_results(qlp_exporter *e, int ia_height, int ia_width, const int m
     e->maps.emplace_back("velocities" + std::to_string(ia_height) + "x" + std::to_string(ia_width),
                          std::move(s));
     return QLP_OK;
+} catch (...) {
+    return qlpiv::fail_exception();
 }
 
 int qlp_export_close(qlp_exporter *e)
-{
+try {
     if (!e) return fail(QLP_ERR_ARG, "export_close: null exporter");
     std::unique_ptr<qlp_exporter> own(e);
     std::vector<uint8_t> o;
@@ -338,6 +357,8 @@ int qlp_export_close(qlp_exporter *e)
     const int c = fclose(f);
     if (w != o.size() || c != 0) return fail(QLP_ERR_COMPUTE, "Failed to export data to file");
     return QLP_OK;
+} catch (...) {
+    return qlpiv::fail_exception();
 }
 
 }  // extern "C"

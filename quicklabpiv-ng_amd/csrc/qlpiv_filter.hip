@@ -91,7 +91,7 @@ extern "C" {
 
 int qlp_gaussian_filter_dev(qlp_ctx *h, int n_img, int H, int W, const void *in, int in_is_float,
                             float sigma, int kernel_px, float *tmp, float *out)
-{
+try {
     if (!h || !in || !out) return fail(QLP_ERR_ARG, "gaussian_filter: null argument");
     if (n_img < 0 || H < 1 || W < 1) return fail(QLP_ERR_ARG, "gaussian_filter: bad image size");
     if (kernel_px < 1 || kernel_px > GF_MAX_K)
@@ -123,6 +123,8 @@ int qlp_gaussian_filter_dev(qlp_ctx *h, int n_img, int H, int W, const void *in,
     hipLaunchKernelGGL(gf_cols_kernel, dim3(blocks), dim3(256), 0, ctx->stream, n_img, H, W, tmp, gk,
                        out);
     return check_hip(hipGetLastError(), "gaussian filter launch");
+} catch (...) {
+    return qlpiv::fail_exception();
 }
 
 }  // extern "C"

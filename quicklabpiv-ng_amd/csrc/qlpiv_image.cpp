@@ -33,6 +33,7 @@
 
 namespace qlpiv {
 int fail(int code, const std::string &msg);
+int fail_exception();
 }
 
 using qlpiv::fail;
@@ -51,6 +52,16 @@ bool read_file(const char *path, std::vector<uint8_t> &buf)
     return ok;
 }
 
+// A frame is one Java array in the reference (images/Image.java: byte[] or
+// short[] of height * width elements), so it holds at most 2^31 - 1 bytes;
+// larger, empty or overflowing header dimensions are rejected before anything
+// is allocated from them.
+bool dims_ok(uint64_t W, uint64_t H, uint64_t bpp)
+{
+    return W >= 1 && H >= 1 && W <= 0x7fffffffu && H <= 0x7fffffffu &&
+           W * H <= 0x7fffffffu / bpp;
+}
+
 struct Image {
     int H = 0, W = 0;
     bool wide = false;           // 16-bit samples requested (ImageInt16 frames)
@@ -62,18 +73,33 @@ struct Image {
 
 bool inflate_all(const uint8_t *src, size_t n, std::vector<uint8_t> &out, size_t expect)
 {
-    out.assign(expect, 0);
+    // The output grows with what the stream actually inflates to (doubling from
+    // 1 MB), so a small file that declares a huge frame allocates little.
+    out.clear();
     z_stream z;
     memset(&z, 0, sizeof z);
     if (inflateInit(&z) != Z_OK) return false;
     z.next_in = (Bytef *)src;
-    z.avail_in = (uInt)n;
-    z.next_out = out.data();
-    z.avail_out = (uInt)expect;
-    const int rc = inflate(&z, Z_FINISH);
-    const size_t got = expect - z.avail_out;
+    z.avail_in = (uInt)std::min<size_t>(n, 0xffffffffu);
+    bool ok = true;
+    while (out.size() < expect) {
+        const size_t old = out.size();
+        const size_t grow = std::min(expect - old, std::max<size_t>(old, 1u << 20));
+        out.resize(old + grow);
+        z.next_out = out.data() + old;
+        z.avail_out = (uInt)grow;
+        const int rc = inflate(&z, Z_NO_FLUSH);
+        const size_t got = grow - z.avail_out;
+        out.resize(old + got);
+        if (rc == Z_STREAM_END) break;
+        if (rc != Z_OK && rc != Z_BUF_ERROR) {
+            ok = false;
+            break;
+        }
+        if (got == 0) break;  // the stream ends short of the frame
+    }
     inflateEnd(&z);
-    return (rc == Z_STREAM_END || rc == Z_OK || rc == Z_BUF_ERROR) && got == expect;
+    return ok && out.size() == expect;
 }
 
 // ---- TIFF ---------------------------------------------------------------------
@@ -157,12 +183,12 @@ bool unlzw(const uint8_t *s, size_t n, std::vector<uint8_t> &out, size_t expect)
         std::vector<uint8_t> entry;
         if (code < (int)table.size()) {
             entry = table[code];
-            if (prev >= 0) {
+            if (prev >= 0 && table.size() < 4096) {  // a full table waits for Clear
                 std::vector<uint8_t> add = table[prev];
                 add.push_back(entry[0]);
                 table.push_back(add);
             }
-        } else if (code == (int)table.size() && prev >= 0) {
+        } else if (code == (int)table.size() && prev >= 0 && table.size() < 4096) {
             entry = table[prev];
             entry.push_back(table[prev][0]);
             table.push_back(entry);
@@ -220,12 +246,17 @@ int read_tiff(const std::vector<uint8_t> &b, Image &img)
                                "PackBits, LZW or Deflate, are read as 16-bit frames"
                              : "image: only 8-bit grayscale (black is zero) TIFF strips, uncompressed, "
                                "PackBits, LZW or Deflate, are read");
-    if (rps > H) rps = H;
     const size_t bpp = img.wide ? 2 : 1;
+    if (!dims_ok(W, H, bpp)) return fail(QLP_ERR_ARG, "image: TIFF dimensions out of range");
+    if (comp == 1) {  // uncompressed strips hold the samples as they are
+        uint64_t have = 0;
+        for (uint32_t c : counts) have += c;
+        if (have < (uint64_t)W * H * bpp) return fail(QLP_ERR_ARG, "image: TIFF strips too short");
+    }
+    if (rps > H) rps = H;
     img.W = (int)W;
     img.H = (int)H;
     img.px.clear();
-    img.px.reserve((size_t)W * H * bpp);
     for (size_t s = 0; s < offs.size() && img.px.size() < (size_t)W * H * bpp; s++) {
         const size_t rows = std::min<size_t>(rps, H - img.px.size() / (W * bpp));
         const size_t expect = rows * W * bpp;
@@ -299,9 +330,11 @@ int read_png(const std::vector<uint8_t> &b, Image &img)
         }
         o += 12 + len;
     }
-    if (!header || W == 0 || H == 0) return fail(QLP_ERR_ARG, "image: PNG without header");
+    if (!header) return fail(QLP_ERR_ARG, "image: PNG without header");
     // rows of RB bytes; the filters work on bytes, a neighbour is one sample (bpp bytes) back
     const size_t bpp = img.wide ? 2 : 1, RB = (size_t)W * bpp;
+    if (!dims_ok(W, H, bpp) || (uint64_t)H * (RB + 1) > 0xffffffffu)
+        return fail(QLP_ERR_ARG, "image: PNG dimensions out of range");
     std::vector<uint8_t> raw;
     if (!inflate_all(z.data(), z.size(), raw, (size_t)H * (RB + 1)))
         return fail(QLP_ERR_ARG, "image: corrupt PNG data");
@@ -375,9 +408,13 @@ struct BitReader {
     uint32_t acc = 0;
     int n = 0;
     bool marker = false;  // a marker was met: the segment's data is exhausted
+    bool past = false;    // bits were read beyond the segment's data
     int fill_byte()
     {
-        if (marker || p >= end) return 0;
+        if (marker || p >= end) {
+            past = true;
+            return 0;
+        }
         int b = *p;
         if (b == 0xff) {
             if (p + 1 < end && p[1] == 0x00) {
@@ -428,8 +465,12 @@ const int kZigzag[64] = {0,  1,  8,  16, 9,  2,  3,  10, 17, 24, 32, 25, 18, 11,
                          35, 42, 49, 56, 57, 50, 43, 36, 29, 22, 15, 23, 30, 37, 44, 51,
                          58, 59, 52, 45, 38, 31, 39, 46, 53, 60, 61, 54, 47, 55, 62, 63};
 
-// Inverse DCT of one dequantised 8 x 8 block: the Loeffler-Ligtenberg-Moschytz
-// factorisation in 32-bit fixed point with 13-bit constants (c = round(x *
+// Inverse DCT of one dequantised 8 x 8 block.  The step sequence (the rotation
+// constants, CONST_BITS 13, PASS1_BITS 2, the DC-only column shortcut and the
+// final descale) is the Independent JPEG Group's "islow" integer IDCT
+// (jidctint.c, IJG libjpeg), which javax.imageio's decoder uses: matching its
+// samples bit for bit needs the same fixed-point steps.  It is the
+// Loeffler-Ligtenberg-Moschytz factorisation in fixed point with 13-bit constants (c = round(x *
 // 2^13) of the rotation factors below), columns first keeping 2 extra bits,
 // then rows, rounding each pass by adding half the discarded weight; the
 // result is offset by 128 and clamped to 0..255.
@@ -576,7 +617,11 @@ int read_jpeg(const std::vector<uint8_t> &b, Image &img)
             if (td > 3 || ta > 3 || !dc[td].present || !ac[ta].present || !qok[qsel])
                 return fail(QLP_ERR_ARG, "jpeg: scan refers to a missing table");
             const int bw = (W + 7) / 8, bh = (H + 7) / 8;
-            std::vector<uint8_t> full((size_t)bw * 8 * bh * 8);
+            if (!dims_ok((uint64_t)bw * 8, (uint64_t)bh * 8, 1))
+                return fail(QLP_ERR_ARG, "jpeg: image dimensions out of range");
+            // decoded block rows; grown one row at a time, so the buffer follows
+            // the scan data actually present
+            std::vector<uint8_t> full;
             BitReader br{segend, end};
             int pred = 0, coef[64];
             const int nblocks = bw * bh;
@@ -595,6 +640,10 @@ int read_jpeg(const std::vector<uint8_t> &b, Image &img)
                 const int s = huff_decode(br, dc[td]);
                 if (s < 0 || s > 11) return fail(QLP_ERR_ARG, "jpeg: corrupt DC code");
                 pred += extend(br.bits(s), s);
+                // a valid 8-bit image's DC coefficient (pred times its quantiser)
+                // lies within +-1024, so |pred| <= 32767 accepts every valid file
+                // and keeps pred * q inside an int
+                if (pred < -32767 || pred > 32767) return fail(QLP_ERR_ARG, "jpeg: DC coefficient out of range");
                 coef[0] = pred * q[qsel][0];
                 for (int k = 1; k < 64;) {
                     const int rs = huff_decode(br, ac[ta]);
@@ -611,7 +660,11 @@ int read_jpeg(const std::vector<uint8_t> &b, Image &img)
                     coef[z] = extend(br.bits(sz), sz) * q[qsel][z];
                     k++;
                 }
+                // a valid scan never reads past its data (Huffman codes are read bit
+                // by bit); a truncated one is refused rather than zero-filled
+                if (br.past) return fail(QLP_ERR_ARG, "jpeg: scan data ends before the last block");
                 const int by = blk / bw, bx = blk - by * bw;
+                if (bx == 0) full.resize((size_t)(by + 1) * 8 * bw * 8);
                 idct8x8(coef, full.data() + (size_t)by * 8 * bw * 8 + bx * 8, bw * 8);
             }
             img.H = H;
@@ -647,7 +700,7 @@ int decode(const char *path, Image &img)
 extern "C" {
 
 int qlp_image_read16(const char *path, uint16_t *out, size_t out_elems, int *H, int *W)
-{
+try {
     Image img;
     img.wide = true;
     if (int rc = decode(path, img)) return rc;
@@ -657,10 +710,12 @@ int qlp_image_read16(const char *path, uint16_t *out, size_t out_elems, int *H, 
     if (out_elems < img.px16.size()) return fail(QLP_ERR_ARG, "image: output buffer too small");
     memcpy(out, img.px16.data(), img.px16.size() * sizeof(uint16_t));
     return QLP_OK;
+} catch (...) {
+    return qlpiv::fail_exception();
 }
 
 int qlp_image_read(const char *path, uint8_t *out, size_t out_bytes, int *H, int *W)
-{
+try {
     Image img;
     if (int rc = decode(path, img)) return rc;
     if (H) *H = img.H;
@@ -669,6 +724,8 @@ int qlp_image_read(const char *path, uint8_t *out, size_t out_bytes, int *H, int
     if (out_bytes < img.px.size()) return fail(QLP_ERR_ARG, "image: output buffer too small");
     memcpy(out, img.px.data(), img.px.size());
     return QLP_OK;
+} catch (...) {
+    return qlpiv::fail_exception();
 }
 
 }  // extern "C"

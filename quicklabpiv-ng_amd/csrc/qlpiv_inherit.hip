@@ -391,7 +391,7 @@ int qlp_inherit_velocity_dev(qlp_ctx *h, int mode, int pairs, int pnI, int pnJ, 
                              const int *ptop, const int *pleft, int cnI, int cnJ, int cH, int cW,
                              const int *ctop, const int *cleft, int round_it, const float *pu,
                              const float *pv, float *u, float *v)
-{
+try {
     if (!h || !ptop || !pleft || !ctop || !cleft || !pu || !pv || !u || !v)
         return fail(QLP_ERR_ARG, "inherit: null argument");
     if (pairs < 0 || pnI < 1 || pnJ < 1 || cnI < 1 || cnJ < 1 || pH < 1 || pW < 1 || cH < 1 || cW < 1)
@@ -440,6 +440,8 @@ int qlp_inherit_velocity_dev(qlp_ctx *h, int mode, int pairs, int pnI, int pnJ, 
                                       (const float *)(b + b1 + b2 + b3), round_it, pu, pv, u, v);
     }
     return fail(QLP_ERR_ARG, "inherit: unknown strategy");
+} catch (...) {
+    return qlpiv::fail_exception();
 }
 
 }  // extern "C"

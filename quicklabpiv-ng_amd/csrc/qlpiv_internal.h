@@ -74,6 +74,8 @@ struct Context {
 
 void set_error(const std::string &msg);
 int fail(int code, const std::string &msg);
+// Status of the exception being handled (call inside a catch block).
+int fail_exception();
 int check_hip(hipError_t e, const char *what);
 Context *ctx_of(qlp_ctx *h);
 

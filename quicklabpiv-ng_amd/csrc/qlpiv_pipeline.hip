@@ -354,7 +354,7 @@ static int plan_layout(int H, int W, int pairs, const qlp_piv_config *cfg, qlp_p
 }
 
 int qlp_piv_plan_estimate(int H, int W, int pairs, const qlp_piv_config *cfg, size_t *bytes)
-{
+try {
     if (!cfg || !bytes) return fail(QLP_ERR_ARG, "piv_plan_estimate: null argument");
     qlp_piv_plan p;
     size_t need = 0;
@@ -366,11 +366,13 @@ int qlp_piv_plan_estimate(int H, int W, int pairs, const qlp_piv_config *cfg, si
     if (p.big_ias) need += big_work_estimate(p.big_ias);
     *bytes = need;
     return QLP_OK;
+} catch (...) {
+    return qlpiv::fail_exception();
 }
 
 int qlp_piv_plan_create(qlp_ctx *h, int H, int W, int pairs, const qlp_piv_config *cfg,
                         qlp_piv_plan **out)
-{
+try {
     if (!h || !cfg || !out) return fail(QLP_ERR_ARG, "piv_plan_create: null argument");
     *out = nullptr;
     Context *ctx = ctx_of(h);
@@ -456,26 +458,32 @@ int qlp_piv_plan_create(qlp_ctx *h, int H, int W, int pairs, const qlp_piv_confi
     }
     *out = p;
     return QLP_OK;
+} catch (...) {
+    return qlpiv::fail_exception();
 }
 
 int qlp_piv_plan_destroy(qlp_piv_plan *p)
-{
+try {
     if (!p) return QLP_OK;
     if (p->ctx) (void)hipStreamSynchronize(p->ctx->stream);
     plan_free(p);
     delete p;
     return QLP_OK;
+} catch (...) {
+    return qlpiv::fail_exception();
 }
 
 int qlp_piv_plan_levels(qlp_piv_plan *p, int *n_levels)
-{
+try {
     if (!p || !n_levels) return fail(QLP_ERR_ARG, "null argument");
     *n_levels = (int)p->levels.size();
     return QLP_OK;
+} catch (...) {
+    return qlpiv::fail_exception();
 }
 
 int qlp_piv_plan_level(qlp_piv_plan *p, int level, int *ia, int *nI, int *nJ, int *top, int *left)
-{
+try {
     if (!p) return fail(QLP_ERR_ARG, "null plan");
     if (level < 0 || level >= (int)p->levels.size()) return fail(QLP_ERR_ARG, "level out of range");
     const Level &L = p->levels[level];
@@ -487,6 +495,8 @@ int qlp_piv_plan_level(qlp_piv_plan *p, int level, int *ia, int *nI, int *nJ, in
     if (left)
         for (int k = 0; k < L.n; k++) left[k] = L.left_h[k];
     return QLP_OK;
+} catch (...) {
+    return qlpiv::fail_exception();
 }
 
 // Float tiles of one level of a float-frame plan (ImageFloat frames, or
@@ -586,23 +596,27 @@ static int run_levels(qlp_piv_plan *p, const uint8_t *imgsA, const uint8_t *imgs
                       const float *fimgsA, const float *fimgsB);
 
 int qlp_piv_run_dev(qlp_piv_plan *p, const uint8_t *imgsA, const uint8_t *imgsB)
-{
+try {
     if (!p || !imgsA || !imgsB) return fail(QLP_ERR_ARG, "piv_run: null argument");
     if (p->cfg.frame_type != QLP_FRAME_U8)
         return fail(QLP_ERR_ARG, "piv_run: the plan was not made for 8-bit frames");
     return run_levels(p, imgsA, imgsB, nullptr, nullptr);
+} catch (...) {
+    return qlpiv::fail_exception();
 }
 
 int qlp_piv_run_f32_dev(qlp_piv_plan *p, const float *imgsA, const float *imgsB)
-{
+try {
     if (!p || !imgsA || !imgsB) return fail(QLP_ERR_ARG, "piv_run: null argument");
     if (p->cfg.frame_type != QLP_FRAME_F32)
         return fail(QLP_ERR_ARG, "piv_run: the plan was not made for float frames");
     return run_levels(p, nullptr, nullptr, imgsA, imgsB);
+} catch (...) {
+    return qlpiv::fail_exception();
 }
 
 int qlp_piv_run_u16_dev(qlp_piv_plan *p, const uint16_t *imgsA, const uint16_t *imgsB)
-{
+try {
     if (!p || !imgsA || !imgsB) return fail(QLP_ERR_ARG, "piv_run: null argument");
     if (p->cfg.frame_type != QLP_FRAME_U16)
         return fail(QLP_ERR_ARG, "piv_run: the plan was not made for 10/12/16-bit frames");
@@ -611,6 +625,8 @@ int qlp_piv_run_u16_dev(qlp_piv_plan *p, const uint16_t *imgsA, const uint16_t *
     if (!rc) rc = launch_u16_to_f32(p->ctx, imgsB, p->fframes + n, n);
     if (rc) return rc;
     return run_levels(p, nullptr, nullptr, p->fframes, p->fframes + n);
+} catch (...) {
+    return qlpiv::fail_exception();
 }
 
 static int run_levels(qlp_piv_plan *p, const uint8_t *imgsA, const uint8_t *imgsB,
@@ -785,7 +801,7 @@ static int run_levels(qlp_piv_plan *p, const uint8_t *imgsA, const uint8_t *imgs
 }
 
 int qlp_piv_plan_set_timing(qlp_piv_plan *p, int enable)
-{
+try {
     if (!p) return fail(QLP_ERR_ARG, "null plan");
     if (enable && p->ev.empty()) {
         p->ev.resize(4 * p->levels.size());
@@ -796,11 +812,13 @@ int qlp_piv_plan_set_timing(qlp_piv_plan *p, int enable)
     }
     p->timing = enable != 0;
     return QLP_OK;
+} catch (...) {
+    return qlpiv::fail_exception();
 }
 
 int qlp_piv_plan_timing(qlp_piv_plan *p, int level, float *state_and_pass_ms, float *fit_ms,
                         float *valid_ms)
-{
+try {
     if (!p) return fail(QLP_ERR_ARG, "null plan");
     if (!p->timing) return fail(QLP_ERR_ARG, "timing is not enabled on this plan");
     if (level < 0 || level >= (int)p->levels.size()) return fail(QLP_ERR_ARG, "level out of range");
@@ -816,11 +834,13 @@ int qlp_piv_plan_timing(qlp_piv_plan *p, int level, float *state_and_pass_ms, fl
     if (fit_ms) *fit_ms = b;
     if (valid_ms) *valid_ms = c;
     return QLP_OK;
+} catch (...) {
+    return qlpiv::fail_exception();
 }
 
 int qlp_piv_result_dev(qlp_piv_plan *p, int level, const float **u, const float **v,
                        const float **peaks, const float **planes)
-{
+try {
     if (!p) return fail(QLP_ERR_ARG, "null plan");
     if (level < 0 || level >= (int)p->levels.size()) return fail(QLP_ERR_ARG, "level out of range");
     const Level &L = p->levels[level];
@@ -829,16 +849,20 @@ int qlp_piv_result_dev(qlp_piv_plan *p, int level, const float **u, const float 
     if (peaks) *peaks = L.sub;
     if (planes) *planes = L.planes;
     return QLP_OK;
+} catch (...) {
+    return qlpiv::fail_exception();
 }
 
 int qlp_piv_validation_dev(qlp_piv_plan *p, int level, const uint8_t **invalid, const int **stats)
-{
+try {
     if (!p) return fail(QLP_ERR_ARG, "null plan");
     if (level < 0 || level >= (int)p->levels.size()) return fail(QLP_ERR_ARG, "level out of range");
     const Level &L = p->levels[level];
     if (invalid) *invalid = L.invalid;
     if (stats) *stats = L.stats;
     return QLP_OK;
+} catch (...) {
+    return qlpiv::fail_exception();
 }
 
 }  // extern "C"

@@ -225,7 +225,14 @@ void worker_loop(qlp_server *s)
         // retire completed requests
         for (Slot &sl : s->slots) {
             if (sl.cur && hipEventQuery(sl.done) == hipSuccess) {
-                finish(s, sl);
+                try {
+                    finish(s, sl);
+                } catch (...) {  // no exception may end the worker thread
+                    const auto r = sl.cur;
+                    sl.cur.reset();
+                    sl.cur_plan = nullptr;
+                    fail_request(s, r, fail_exception(), "server: result");
+                }
                 progress = true;
             }
         }
@@ -264,7 +271,12 @@ void worker_loop(qlp_server *s)
             const size_t need = plan_need + (frame_need ? frame_need - 2 * free_slot->frame_bytes : 0);
             if (!make_room(s, need, free_slot, &key)) break;  // wait for memory to free up
             s->pending.pop_front();
-            int rc = launch(s, *free_slot, r, key, frame_need, plan_need);
+            int rc;
+            try {
+                rc = launch(s, *free_slot, r, key, frame_need, plan_need);
+            } catch (...) {
+                rc = fail_exception();
+            }
             if (rc) {
                 free_slot->cur.reset();
                 free_slot->cur_plan = nullptr;
@@ -300,7 +312,7 @@ double ms_since(const Clock::time_point &t0, const Clock::time_point &t)
 extern "C" {
 
 int qlp_server_create(int hip_device, const qlp_server_config *cfg, qlp_server **out)
-{
+try {
     if (!cfg || !out) return fail(QLP_ERR_ARG, "server_create: null argument");
     *out = nullptr;
     if (cfg->slots < 1 || cfg->slots > QLP_SERVER_MAX_SLOTS)
@@ -326,10 +338,12 @@ int qlp_server_create(int hip_device, const qlp_server_config *cfg, qlp_server *
     s->worker = std::thread(worker_loop, s);
     *out = s;
     return QLP_OK;
+} catch (...) {
+    return qlpiv::fail_exception();
 }
 
 int qlp_server_destroy(qlp_server *s)
-{
+try {
     if (!s) return QLP_OK;
     {
         std::lock_guard<std::mutex> lk(s->mu);
@@ -348,10 +362,12 @@ int qlp_server_destroy(qlp_server *s)
     }
     delete s;
     return QLP_OK;
+} catch (...) {
+    return qlpiv::fail_exception();
 }
 
 int qlp_server_submit(qlp_server *s, const qlp_request *req, uint64_t *ticket)
-{
+try {
     if (!s || !req || !ticket) return fail(QLP_ERR_ARG, "server_submit: null argument");
     if (!req->imgA || !req->imgB || req->H < 1 || req->W < 1)
         return fail(QLP_ERR_ARG, "server_submit: bad frames");
@@ -372,10 +388,12 @@ int qlp_server_submit(qlp_server *s, const qlp_request *req, uint64_t *ticket)
     s->cv_work.notify_one();
     *ticket = r->ticket;
     return QLP_OK;
+} catch (...) {
+    return qlpiv::fail_exception();
 }
 
 int qlp_server_wait(qlp_server *s, uint64_t ticket, int timeout_ms, int *done)
-{
+try {
     if (!s || !done) return fail(QLP_ERR_ARG, "server_wait: null argument");
     std::unique_lock<std::mutex> lk(s->mu);
     auto it = s->all.find(ticket);
@@ -389,11 +407,13 @@ int qlp_server_wait(qlp_server *s, uint64_t ticket, int timeout_ms, int *done)
     *done = finished() ? 1 : 0;
     if (r->status == 3) return fail(r->rc, r->err);
     return QLP_OK;
+} catch (...) {
+    return qlpiv::fail_exception();
 }
 
 int qlp_server_result(qlp_server *s, uint64_t ticket, const float **u, const float **v, int *nI,
                       int *nJ, int *ia, qlp_request_times *times)
-{
+try {
     if (!s) return fail(QLP_ERR_ARG, "server_result: null server");
     std::lock_guard<std::mutex> lk(s->mu);
     auto it = s->all.find(ticket);
@@ -413,10 +433,12 @@ int qlp_server_result(qlp_server *s, uint64_t ticket, const float **u, const flo
         times->ias = (double)r.ias;
     }
     return QLP_OK;
+} catch (...) {
+    return qlpiv::fail_exception();
 }
 
 int qlp_server_release(qlp_server *s, uint64_t ticket)
-{
+try {
     if (!s) return fail(QLP_ERR_ARG, "server_release: null server");
     std::lock_guard<std::mutex> lk(s->mu);
     auto it = s->all.find(ticket);
@@ -424,10 +446,12 @@ int qlp_server_release(qlp_server *s, uint64_t ticket)
     if (it->second->status < 2) return fail(QLP_ERR_ARG, "server_release: request not finished");
     s->all.erase(it);
     return QLP_OK;
+} catch (...) {
+    return qlpiv::fail_exception();
 }
 
 int qlp_server_get_stats(qlp_server *s, qlp_server_stats *st)
-{
+try {
     if (!s || !st) return fail(QLP_ERR_ARG, "server_stats: null argument");
     std::lock_guard<std::mutex> lk(s->mu);
     st->completed = s->completed;
@@ -441,6 +465,8 @@ int qlp_server_get_stats(qlp_server *s, qlp_server_stats *st)
     for (const Slot &sl : s->slots) running += sl.cur ? 1 : 0;
     st->running = running;
     return QLP_OK;
+} catch (...) {
+    return qlpiv::fail_exception();
 }
 
 }  // extern "C"

@@ -457,7 +457,7 @@ int qlp_warp_clip_tiles_dev(qlp_ctx *h, int warp_mode, int order, const uint8_t 
                             const uint8_t *imgsB, int H, int W, int pairs, int nI, int nJ, int N,
                             const int *top, const int *left, const float *u, const float *v,
                             uint8_t *tilesA, float *maxA, uint8_t *tilesB, float *maxB)
-{
+try {
     if (!h || !imgsA || !imgsB || !top || !left || !u || !v || !tilesA || !maxA || !tilesB || !maxB)
         return fail(QLP_ERR_ARG, "warp_clip_tiles: null argument");
     if (warp_mode != QLP_WARP_MICRO && warp_mode != QLP_WARP_MINI)
@@ -468,12 +468,14 @@ int qlp_warp_clip_tiles_dev(qlp_ctx *h, int warp_mode, int order, const uint8_t 
         return fail(QLP_ERR_ARG, "warp_clip_tiles: bad sizes");
     return warp_clip_tiles(ctx_of(h), warp_mode, order, imgsA, imgsB, H, W, pairs, nI, nJ, N, top,
                            left, u, v, tilesA, maxA, tilesB, maxB);
+} catch (...) {
+    return qlpiv::fail_exception();
 }
 
 int qlp_warp_images_dev(qlp_ctx *h, int order, const uint8_t *imgsA, const uint8_t *imgsB, int H,
                         int W, int pairs, int nI, int nJ, int N, const int *top, const int *left,
                         const float *u, const float *v, uint8_t *outA, uint8_t *outB)
-{
+try {
     if (!h || !top || !left || !u || !v) return fail(QLP_ERR_ARG, "warp_images: null argument");
     if (order < QLP_WARP_ORDER_SECOND || order > QLP_WARP_ORDER_BOTH)
         return fail(QLP_ERR_ARG, "warp_images: unknown warp order");
@@ -498,23 +500,27 @@ int qlp_warp_images_dev(qlp_ctx *h, int order, const uint8_t *imgsA, const uint8
     if (int rc = ensure_iwork(ctx, full_warp_work_bytes(pairs, nI, nJ))) return rc;
     return launch_full_warp(ctx, pairs, src, H, W, nI, nJ, N, N, top, left, u, v, ctx->iwork, dst,
                             sides, ns);
+} catch (...) {
+    return qlpiv::fail_exception();
 }
 
 int qlp_warp_mini_tiles_dev(qlp_ctx *h, const uint8_t *img, int H, int W, int pairs, int nI,
                             int nJ, int N, const int *top, const int *left, const float *u,
                             const float *v, uint8_t *tiles, float *maxv)
-{
+try {
     if (!h || !img || !top || !left || !u || !v || !tiles)
         return fail(QLP_ERR_ARG, "warp_mini: null argument");
     if (H < 1 || W < 1 || pairs < 0 || nI < 1 || nJ < 1 || N < 1)
         return fail(QLP_ERR_ARG, "warp_mini: bad sizes");
     return launch_warp_mini(ctx_of(h), pairs, img, H, W, nI, nJ, N, top, left, u, v, tiles, maxv);
+} catch (...) {
+    return qlpiv::fail_exception();
 }
 
 int qlp_warp_image_dev(qlp_ctx *h, const uint8_t *img, int H, int W, int pairs, int nI, int nJ,
                        int N, const int *top, const int *left, const float *u, const float *v,
                        uint8_t *out)
-{
+try {
     if (!h || !img || !top || !left || !u || !v || !out)
         return fail(QLP_ERR_ARG, "warp_image: null argument");
     if (H < 1 || W < 1 || pairs < 0 || nI < 1 || nJ < 1 || N < 1)
@@ -524,6 +530,8 @@ int qlp_warp_image_dev(qlp_ctx *h, const uint8_t *img, int H, int W, int pairs, 
     const int side = SIDE_B_SECOND;
     return launch_full_warp(ctx, pairs, &img, H, W, nI, nJ, N, N, top, left, u, v, ctx->iwork, &out,
                             &side, 1);
+} catch (...) {
+    return qlpiv::fail_exception();
 }
 
 }  // extern "C"

@@ -763,7 +763,7 @@ int qlp_validate_vectors_dev(qlp_ctx *h, int pairs, int nI, int nJ, const int *t
                              const int *left, float *u, float *v, int n_peaks,
                              const float *peak_uv, const int *peak_count,
                              const qlp_valid_config *cfg, uint8_t *invalid, int *stats)
-{
+try {
     if (!h || !cfg || !top || !left || !u || !v) return fail(QLP_ERR_ARG, "validate: null argument");
     if (pairs < 0 || nI < 1 || nJ < 1) return fail(QLP_ERR_ARG, "validate: bad grid");
     if (n_peaks < 1 || n_peaks > 16) return fail(QLP_ERR_ARG, "validate: number of peaks must be 1..16");
@@ -821,13 +821,15 @@ int qlp_validate_vectors_dev(qlp_ctx *h, int pairs, int nI, int nJ, const int *t
     vl.invalid = invalid;
     vl.stats = stats;
     return launch_validate(ctx, *cfg, vl);
+} catch (...) {
+    return qlpiv::fail_exception();
 }
 
 int qlp_validate_vectors(qlp_ctx *h, int pairs, int nI, int nJ, const int *top, const int *left,
                          float *u, float *v, int n_peaks, const float *peak_uv,
                          const int *peak_count, const qlp_valid_config *cfg, uint8_t *invalid,
                          int *stats)
-{
+try {
     if (!h || !cfg || !top || !left || !u || !v) return fail(QLP_ERR_ARG, "validate: null argument");
     if (pairs < 0 || nI < 1 || nJ < 1) return fail(QLP_ERR_ARG, "validate: bad grid");
     if (n_peaks < 1 || n_peaks > 16) return fail(QLP_ERR_ARG, "validate: number of peaks must be 1..16");
@@ -858,6 +860,8 @@ int qlp_validate_vectors(qlp_ctx *h, int pairs, int nI, int nJ, const int *top, 
     if (!rc && stats) rc = check_hip(hipMemcpy(stats, ds, (size_t)pairs * 16, hipMemcpyDeviceToHost), "copy stats");
     (void)hipFree(d);
     return rc;
+} catch (...) {
+    return qlpiv::fail_exception();
 }
 
 }  // extern "C"
