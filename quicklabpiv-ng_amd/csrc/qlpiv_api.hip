@@ -133,7 +133,6 @@ try {
     ctx->hbm_budget = hbm_budget_bytes;
     const char *form = getenv("QLPIV_XCORR_FORM");
     ctx->xcorr_form = (form && strcmp(form, "lds") == 0) ? FORM_LDS : FORM_LANE;
-    if (const char *wide = getenv("QLPIV_PASS_WIDE")) ctx->pass_wide = atoi(wide) != 0;
     rc = check_hip(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "hipStreamCreate");
     if (rc) {
         delete h;

@@ -551,9 +551,15 @@ try {
         return fail(QLP_ERR_ARG, "batch_fix: IA size must be 0, 16, 32, 64 or 128");
     if (shape == QLP_SHAPE_LANE && ia != 16 && ia != 0)
         return fail(QLP_ERR_ARG, "batch_fix: the lane shape takes 16 px IAs");
+    if (shape == QLP_SHAPE_LDS_WIDE && ia == 128)
+        return fail(QLP_ERR_ARG, "batch_fix: the wide shape takes 16, 32 and 64 px IAs");
     for (BatchClass &c : b->cls) {
         if (ia != 0 && c.N != ia) continue;
-        c.shape = (shape == QLP_SHAPE_LANE && c.N != 16) ? SHAPE_AUTO : shape;
+        // shapes a class cannot take (lane above 16 px, wide at 128 px) fall back
+        // to the class's default when every class is fixed at once
+        const bool fits = !(shape == QLP_SHAPE_LANE && c.N != 16) &&
+                          !(shape == QLP_SHAPE_LDS_WIDE && c.N == 128);
+        c.shape = fits ? shape : SHAPE_AUTO;
         c.fixed = true;
     }
     if (schedule != QLP_SCHEDULE_TUNED) {

@@ -44,12 +44,6 @@ enum XcorrForm { FORM_LANE = 0, FORM_LDS = 1 };
 struct Context {
     int device = 0;
     int xcorr_form = FORM_LANE;
-    // The 32 / 64 px passes (from images, and from tiles with the automatic
-    // shape) run twice the default work-group (512 /
-    // 1024 lanes: every line of a 1D pass in one round; 8 / 4 waves per SIMD
-    // instead of 4 / 2).  Measured on MI355X, C2 step: 32 px pass 0.298 ->
-    // 0.267 ms, 64 px 0.475 -> 0.400 ms.  QLPIV_PASS_WIDE=0 restores the default.
-    int pass_wide = 1;
     size_t hbm_budget = 0;
     size_t hbm_in_use = 0;
     hipStream_t stream = nullptr;
