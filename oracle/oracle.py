@@ -57,6 +57,11 @@ def lib():
         L.or_gauss1d.argtypes = [_f32p, c_int, c_int, c_int, c_float, _f32p, _f32p]
         L.or_guo.argtypes = [_f32p, c_int, c_int, c_int, c_int, c_float, _f32p, _f32p]
         L.or_poly.argtypes = [_f32p, c_int, c_int, _f32p, _f32p]
+        L.or_guo_coefs.argtypes = [c_int, c_int, _f32p, _f32p, _f64p]
+        L.or_pinv_solve3.argtypes = [_f64p, _f64p, _f64p]
+        L.or_lu_singular3.argtypes = [_f64p]
+        L.or_lu_singular3.restype = c_int
+        L.or_gauss_lsq.argtypes = [c_int, _f64p, _f64p, _f64p]
         L.or_clip_tile.argtypes = [_u8p, c_int, c_int, c_int, c_int, c_int, _u8p]
         L.or_clip_tile.restype = c_float
         L.or_clip_tile_f32.argtypes = [_f32p, c_int, c_int, c_int, c_int, c_int, _f32p]
@@ -309,6 +314,36 @@ def guo(plane, pixels, iterations, min_floor, peak):
     lib().or_guo(plane, plane.shape[0], plane.shape[1], pixels, iterations, float(min_floor),
                  np.ascontiguousarray(peak, np.float32), res)
     return res
+
+
+def guo_coefs(xs, ys, iterations):
+    """Coefficients (a, b, c) of ln y = a + b x + c x^2 after 'iterations' solves
+    of Guo's weighted least squares (float xs, ys as the reference holds them)."""
+    xs = np.ascontiguousarray(xs, np.float32)
+    ys = np.ascontiguousarray(ys, np.float32)
+    out = np.zeros(3, np.float64)
+    lib().or_guo_coefs(int(iterations), xs.size, xs, ys, out)
+    return out
+
+
+def pinv_solve3(A, b):
+    out = np.zeros(3, np.float64)
+    lib().or_pinv_solve3(np.ascontiguousarray(A, np.float64).ravel(),
+                         np.ascontiguousarray(b, np.float64), out)
+    return out
+
+
+def lu_singular3(A):
+    return bool(lib().or_lu_singular3(np.ascontiguousarray(A, np.float64).ravel()))
+
+
+def gauss_lsq(x, y):
+    """(a, b, c) of the least-squares Gaussian a exp(-(x-b)^2 / (2c^2))."""
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.ascontiguousarray(y, np.float64)
+    out = np.zeros(3, np.float64)
+    lib().or_gauss_lsq(x.size, x, y, out)
+    return out
 
 
 def poly(plane, peak):

@@ -21,10 +21,16 @@
  *   or_find_max_from_center  - brute-force argmax with the documented tie-break
  *   or_multipeaks / or_list_*- FindMaximumMultiPeakTests KAT and the 14
  *                              SimpleFixedLengthFloatLinkedListTests sequences
- *   or_gauss1d               - 3 px: closed form; 5/7 px: least-squares optimum;
- *                              GaussianTestHelper cases (567, tol 8e-3)
- *   or_guo                   - exact on noise-free Gaussians; HongWeiGuo cases.
- *                              The singular-matrix SVD branch is parity unpinned.
+ *   or_gauss1d               - 3 px: closed form; 5/7 px: the least-squares optimum
+ *                              equals scipy.optimize.least_squares on noisy samples
+ *                              (<= 1e-6 px); GaussianTestHelper cases (567, tol 8e-3)
+ *   or_guo                   - exact on noise-free Gaussians; HongWeiGuo cases; one
+ *                              iteration = data-weighted least squares (numpy lstsq);
+ *                              20 iterations = the re-weighting fixed point on noisy
+ *                              5-15 px data (tests/test_oracle_fits.py)
+ *   pinv_solve3 (SVD branch) - minimum-norm least squares (numpy pinv with
+ *                              commons-math's threshold) on full-rank, small-valued
+ *                              and exactly rank-1/rank-2 matrices
  *   or_poly                  - closed form
  *   or_warp_micro            - identity at zero displacement, pure shift at
  *                              integer displacement, separable bilinear otherwise
@@ -879,7 +885,13 @@ static int lu_solve3(const double A[3][3], const double b[3], double x[3])
 
 /* Pseudo-inverse solve of the symmetric normal matrix through a cyclic Jacobi
  * eigen-decomposition: the SingularValueDecomposition fallback (:117-122).
- * Parity unpinned (commons-math's SVD iteration is not restated). */
+ * commons-math3 3.6.1 SingularValueDecomposition keeps the singular values
+ * above tol = max(m * s_max * 2^-52, sqrt(Double.MIN_NORMAL)) with m = 3 rows
+ * and its solver applies the pseudo-inverse; for a symmetric matrix the
+ * singular values are the absolute eigenvalues.  Pinned by the minimum-norm
+ * least-squares solution (numpy pinv / lstsq) on well-conditioned,
+ * small-valued and exactly rank-1 / rank-2 matrices; commons-math's own SVD
+ * iteration is not restated, so last-ulp agreement is not claimed. */
 static void pinv_solve3(const double A[3][3], const double b[3], double x[3])
 {
     double a[3][3], v[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
@@ -912,7 +924,7 @@ static void pinv_solve3(const double A[3][3], const double b[3], double x[3])
     }
     double smax = 0;
     for (int k = 0; k < 3; k++) smax = fmax(smax, fabs(a[k][k]));
-    double tol = 3 * smax * 2.220446049250313e-16;
+    double tol = fmax(3 * smax * 2.220446049250313e-16, 1.4916681462400413e-154);
     for (int r = 0; r < 3; r++) x[r] = 0;
     for (int k = 0; k < 3; k++) {
         double s = a[k][k];
@@ -926,10 +938,10 @@ static void pinv_solve3(const double A[3][3], const double b[3], double x[3])
 /* Gaussian1DHongweiGuoInterpolator.initMatrices :47-70 / updateMatrices :72-102
  * / estimateGaussianParameters :104-130.  xs, ys are float arrays (Java float[]);
  * the y^2 of the data term is a float product, everything else double. */
-static void guo_estimate(int iterations, int n, const float *xs, const float *ys, float *mean,
-                         float *amp)
+static void guo_coefs(int iterations, int n, const float *xs, const float *ys, double coefs[3])
 {
-    double A[3][3] = {{0}}, b[3] = {0}, coefs[3] = {0, 0, 0};
+    double A[3][3] = {{0}}, b[3] = {0};
+    coefs[0] = coefs[1] = coefs[2] = 0.0;
     for (int i = 0; i < n; i++) {
         double ySquare = (double)(ys[i] * ys[i]);
         double xSquare = (double)(xs[i] * xs[i]);
@@ -974,8 +986,46 @@ static void guo_estimate(int iterations, int n, const float *xs, const float *ys
             b[2] += xSquare * ySquare * logy;
         }
     }
+}
+
+static void guo_estimate(int iterations, int n, const float *xs, const float *ys, float *mean,
+                         float *amp)
+{
+    double coefs[3];
+    guo_coefs(iterations, n, xs, ys, coefs);
     *mean = (float)(-coefs[1] / (2 * coefs[2]));
     *amp = (float)(exp(coefs[0] - coefs[1] * coefs[1] / (4 * coefs[2])));
+}
+
+/* Test hooks: the Guo coefficients (a, b, c of ln y = a + b x + c x^2) after
+ * 'iterations' solves, and the pseudo-inverse solve on its own (A row-major). */
+void or_guo_coefs(int iterations, int n, const float *xs, const float *ys, double *coefs)
+{
+    guo_coefs(iterations, n, xs, ys, coefs);
+}
+
+void or_pinv_solve3(const double *A9, const double *b, double *x)
+{
+    double A[3][3];
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) A[r][c] = A9[r * 3 + c];
+    pinv_solve3(A, b, x);
+}
+
+/* 1 when commons-math's LUDecomposition finds the matrix singular (pivot
+ * below 1e-11), i.e. when Guo takes the SVD fallback. */
+int or_lu_singular3(const double *A9)
+{
+    double A[3][3], b[3] = {0, 0, 0}, x[3];
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) A[r][c] = A9[r * 3 + c];
+    return !lu_solve3(A, b, x);
+}
+
+/* The least-squares Gaussian of the 5/7-point Gaussian1D fit: (a, b, c). */
+void or_gauss_lsq(int n, const double *x, const double *y, double *par)
+{
+    gauss_lsq(n, x, y, par);
 }
 
 /* Gaussian1DHongweiGuoInterpolator.interpolate :133-248 for one peak. */
