@@ -55,6 +55,7 @@ extern "C" {
 /* Peak finders (maximum/MaximumFinderFactoryEnum.java:17-41). */
 #define QLP_PEAK_FROM_CENTER 0 /* FindMaximumFromCenter */
 #define QLP_PEAK_MULTI 1       /* FindMaximumMultiPeaks(numberOfPeaks, kernelWidth) */
+#define QLP_PEAK_SIMPLE 2      /* FindMaximumSimple (maximum/FindMaximumSimple.java:19-48) */
 
 /* Sub-pixel interpolators (interpolators/CrossCorrelationInterpolatorFactoryEnum.java). */
 #define QLP_SUBPIX_GAUSS1D 0 /* Gaussian1DInterpolator(pixels) */
@@ -141,8 +142,11 @@ int qlp_xcorr_form_dev(qlp_ctx *ctx, int form, int n, int N, const uint8_t *tile
 
 /* IMaximumFinder.findMaximum(Matrix) (maximum/IMaximumFinder.java:16-17) over n
  * planes of D x D.  kind QLP_PEAK_FROM_CENTER (FindMaximumFromCenter.findMaximum
- * :47-183; n_peaks must be 1) or QLP_PEAK_MULTI (FindMaximumMultiPeaks.findMaximum
- * :42-162 with numberOfPeaks = n_peaks <= 16 and kernelWidth).
+ * :47-183; n_peaks must be 1), QLP_PEAK_MULTI (FindMaximumMultiPeaks.findMaximum
+ * :42-162 with numberOfPeaks = n_peaks <= 16 and kernelWidth) or QLP_PEAK_SIMPLE
+ * (FindMaximumSimple.findMaximum :19-48: the first strictly largest positive
+ * value in row-major order, (D/2, D/2) with value 0 when none is positive;
+ * n_peaks must be 1).
  *   peaks     : n x n_peaks x 3 floats (i, j, value), peak 0 the main peak;
  *               missing peaks are zero
  *   min_floor : n floats (MaxCrossResult.getMinFloor) */
@@ -397,8 +401,9 @@ typedef struct qlp_piv_config {
     int interpolator_start_step; /* first level that is interpolated */
     int keep_planes;          /* 0: no planes; 1: store the last level's planes; 2: store
                                  every level's planes (what the Java job returns) */
-    int peak_kind;            /* QLP_PEAK_FROM_CENTER (fused into the pass) or
-                                 QLP_PEAK_MULTI (planes are then stored at every level) */
+    int peak_kind;            /* QLP_PEAK_FROM_CENTER (fused into the pass), QLP_PEAK_MULTI
+                                 or QLP_PEAK_SIMPLE (planes are then stored at every
+                                 level) */
     int n_peaks;              /* MultiPeaks numberOfPeaks (1 for FromCenter) */
     int peak_kernel_width;    /* MultiPeaks kernelWidth */
     qlp_valid_config valid;   /* vector validation after every level (n_stages 0: off) */

@@ -50,6 +50,10 @@ def lib():
         L.or_xcorr_javagpu.argtypes = [_f32p, _f32p, c_float, c_float, c_int, _f32p]
         L.or_xcorr_batch_u8.argtypes = [c_int, c_int, _u8p, _u8p, _f32p, _f32p, c_int, _f32p]
         L.or_find_max_from_center.argtypes = [_f32p, c_int, c_int, _f32p]
+        L.or_find_max_simple.argtypes = [_f32p, c_int, c_int, _f32p]
+        L.or_level_batch.argtypes = [_u8p, _u8p, c_int, c_int, c_int, c_int, _i32p, _i32p, _f32p,
+                                     _f32p, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int,
+                                     ctypes.c_void_p, _f32p, _i32p]
         L.or_list_script.argtypes = [c_int, c_int, c_int, _f32p, _i32p, _f32p]
         L.or_list_script.restype = c_int
         L.or_find_max_multipeaks.argtypes = [_f32p, c_int, c_int, c_int, c_int, _f32p, _f32p]
@@ -279,6 +283,36 @@ def find_max_from_center(plane):
     plane = np.ascontiguousarray(plane, np.float32)
     out = np.zeros(4, np.float32)
     lib().or_find_max_from_center(plane, plane.shape[0], plane.shape[1], out)
+    return out  # i, j, value, minFloor
+
+
+def level_batch(a, b, top, left, u, v, N, warp, mode=0, peak_kind=0, P=1, k=3, fit_kind=1,
+                pixels=3, iterations=20, planes=False):
+    """One level of the per-frame loop for the IAs at (top, left) of the pair
+    (a, b) from the inherited displacements (u, v) (or_level_batch).  Returns
+    (out: n x P x 4 fits, count: n, planes: n x D x D or None)."""
+    a = np.ascontiguousarray(a, np.uint8)
+    b = np.ascontiguousarray(b, np.uint8)
+    top = np.ascontiguousarray(np.ravel(top), np.int32)
+    left = np.ascontiguousarray(np.ravel(left), np.int32)
+    n = top.size
+    u = np.ascontiguousarray(np.ravel(u), np.float32) if u is not None else np.zeros(n, np.float32)
+    v = np.ascontiguousarray(np.ravel(v), np.float32) if v is not None else np.zeros(n, np.float32)
+    np_ = P if peak_kind == 1 else 1
+    out = np.zeros((n, np_, 4), np.float32)
+    cnt = np.zeros(n, np.int32)
+    D = 2 * N - 1
+    pl = np.empty((n, D, D), np.float32) if planes else None
+    lib().or_level_batch(a, b, a.shape[0], a.shape[1], n, N, top, left, u, v, int(warp), int(mode),
+                         int(peak_kind), int(P), int(k), int(fit_kind), int(pixels),
+                         int(iterations), None if pl is None else pl.ctypes.data, out, cnt)
+    return out, cnt, pl
+
+
+def find_max_simple(plane):
+    plane = np.ascontiguousarray(plane, np.float32)
+    out = np.zeros(4, np.float32)
+    lib().or_find_max_simple(plane, plane.shape[0], plane.shape[1], out)
     return out  # i, j, value, minFloor
 
 

@@ -389,6 +389,33 @@ static float cand_dist(const or_cand *c, short ci, short cj)
 
 /* FindMaximumFromCenter.findMaximum :47-183 with updateMaxResults :33-44.
  * out[0..3] = peak i, peak j, peak value, minFloor. */
+/* FindMaximumSimple.findMaximum :19-48: row-major scan, strictly larger than the
+ * running maximum (initially 0); no positive value -> (height/2, width/2) with
+ * value 0; minFloor = the minimum (initially Float.MAX_VALUE). */
+void or_find_max_simple(const float *m, int dimI, int dimJ, float *out)
+{
+    float mx = 0.0f, mn = 3.40282347e+38f;
+    int pi = 0, pj = 0;
+    for (int i = 0; i < dimI; i++)
+        for (int j = 0; j < dimJ; j++) {
+            float value = m[i * dimJ + j];
+            if (value > mx) {
+                mx = value;
+                pi = i;
+                pj = j;
+            }
+            if (value < mn) mn = value;
+        }
+    if (mx == 0.0f) {
+        pi = dimI / 2;
+        pj = dimJ / 2;
+    }
+    out[0] = (float)pi;
+    out[1] = (float)pj;
+    out[2] = mx;
+    out[3] = mn;
+}
+
 void or_find_max_from_center(const float *m, int dimI, int dimJ, float *out)
 {
 #define EL(a, b) m[(a) * dimJ + (b)]
@@ -1467,6 +1494,97 @@ void or_cpu_level_batch_multi(const uint8_t *imgA, const uint8_t *imgB, int H, i
     free(A);
     free(B);
     free(plane);
+}
+
+/* One level of the per-frame loop (LocalPIVOpenCLGpuJob.call :188-409) for n
+ * IAs of one frame pair, in any configuration the device plan runs:
+ *   warp       0: B clipped at (top, left) (the first level); 1: B clipped at
+ *              (round(top + u), round(left + v)) (NoImageWarpingStrategy
+ *              :79-81); 2: B micro-warped by (u, v), maximum 255
+ *   mode       0: CrossCorrelationRealFFTParStdJob arithmetic (each tile by its
+ *              own maximum); 1: CrossCorrelationSoftRealFFTJob (both by A's)
+ *   peak_kind  0: FindMaximumFromCenter; 1: FindMaximumMultiPeaks(P, k);
+ *              2: FindMaximumSimple
+ *   fit_kind   -1: none (the integer peak, status 4); 0: Gaussian1D(pixels);
+ *              1: Guo(pixels, iterations); 2: Polynomial -- on every peak found
+ * planes (n x D x D) may be NULL; out: n x P x 4 (i', j', value', status; peaks
+ * not found are zero); count: n peaks found (at least 1), may be NULL. */
+void or_level_batch(const uint8_t *imgA, const uint8_t *imgB, int H, int W, int n, int N,
+                    const int *top, const int *left, const float *u, const float *v, int warp,
+                    int mode, int peak_kind, int P, int k, int fit_kind, int pixels,
+                    int iterations, float *planes, float *out, int *count)
+{
+    const int D = 2 * N - 1;
+    uint8_t *ta = (uint8_t *)malloc((size_t)N * N), *tb = (uint8_t *)malloc((size_t)N * N);
+    float *A = (float *)malloc(sizeof(float) * N * N);
+    float *B = (float *)malloc(sizeof(float) * N * N);
+    float *own = (float *)malloc(sizeof(float) * D * D);
+    float pk[3 * OR_LIST_MAXP + 1];
+    for (int t = 0; t < n; t++) {
+        float *plane = planes ? planes + (size_t)t * D * D : own;
+        const float mA = or_clip_tile(imgA, H, W, top[t], left[t], N, ta);
+        float mB;
+        if (warp == 2) {
+            or_warp_micro(imgB, H, W, top[t], left[t], N, u[t], v[t], tb);
+            mB = 255.0f;
+        } else if (warp == 1) {
+            const int tt = (int)floorf((float)top[t] + u[t] + 0.5f);
+            const int ll = (int)floorf((float)left[t] + v[t] + 0.5f);
+            mB = or_clip_tile(imgB, H, W, tt, ll, N, tb);
+        } else {
+            mB = or_clip_tile(imgB, H, W, top[t], left[t], N, tb);
+        }
+        for (int q = 0; q < N * N; q++) {
+            A[q] = (float)ta[q];
+            B[q] = (float)tb[q];
+        }
+        if (mode == 0)
+            or_xcorr_javagpu(A, B, mA, mB, N, plane);
+        else
+            or_xcorr_javacpu(A, B, mA, N, plane);
+        int found = 1;
+        float mf;
+        if (peak_kind == 1) {
+            found = or_find_max_multipeaks(plane, D, D, P, k, pk, &mf);
+        } else {
+            float r[4];
+            if (peak_kind == 2)
+                or_find_max_simple(plane, D, D, r);
+            else
+                or_find_max_from_center(plane, D, D, r);
+            pk[0] = r[0];
+            pk[1] = r[1];
+            pk[2] = r[2];
+            mf = r[3];
+        }
+        const int np = peak_kind == 1 ? P : 1;
+        for (int q = 0; q < np; q++) {
+            float *o = out + 4 * ((size_t)t * np + q);
+            if (q >= found) {
+                o[0] = o[1] = o[2] = o[3] = 0.0f;
+                continue;
+            }
+            const float *pq = pk + 3 * q;
+            if (fit_kind == 0)
+                or_gauss1d(plane, D, D, pixels, mf, pq, o);
+            else if (fit_kind == 1)
+                or_guo(plane, D, D, pixels, iterations, mf, pq, o);
+            else if (fit_kind == 2)
+                or_poly(plane, D, D, pq, o);
+            else {
+                o[0] = pq[0];
+                o[1] = pq[1];
+                o[2] = pq[2];
+                o[3] = 4.0f;
+            }
+        }
+        if (count) count[t] = found > 1 ? found : 1;
+    }
+    free(ta);
+    free(tb);
+    free(A);
+    free(B);
+    free(own);
 }
 
 /* ------------------------------------------------------------------------ */

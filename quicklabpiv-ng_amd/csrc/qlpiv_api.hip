@@ -344,8 +344,8 @@ int qlp_find_peaks_dev(qlp_ctx *h, int kind, int n, int D, const float *planes, 
 try {
     if (!h) return fail(QLP_ERR_ARG, "null context");
     if (n < 0) return fail(QLP_ERR_ARG, "find_peaks: negative count");
-    if (kind == QLP_PEAK_FROM_CENTER && n_peaks != 1)
-        return fail(QLP_ERR_ARG, "find_peaks: FromCenter finds exactly one peak");
+    if ((kind == QLP_PEAK_FROM_CENTER || kind == QLP_PEAK_SIMPLE) && n_peaks != 1)
+        return fail(QLP_ERR_ARG, "find_peaks: FromCenter and Simple find exactly one peak");
     return launch_find_peaks(&h->c, kind, n, D, planes, n_peaks, kernel_width, peaks, min_floor);
 } catch (...) {
     return qlpiv::fail_exception();

@@ -10,6 +10,8 @@
 // over cells with value >= 0, or (0,0) with value 0 when no such cell exists.
 // One wavefront per plane folds that 64-bit key with cross-lane shuffles.
 //
+// SIMPLE (maximum/FindMaximumSimple.java:19-48): see peak_simple_kernel.
+//
 // MULTI (maximum/FindMaximumMultiPeaks.java:42-162): multipeak_segment
 // (qlpiv_multipeak.h) with 8 to 64 lanes per plane.
 #include <float.h>
@@ -65,6 +67,53 @@ __global__ __launch_bounds__(256) void peak_from_center_kernel(int n, int D,
             pi = (float)(bestCell / D);
             pj = (float)(bestCell % D);
             pv = m[bestCell];
+        }
+        peaks[(size_t)ia * 3 + 0] = pi;
+        peaks[(size_t)ia * 3 + 1] = pj;
+        peaks[(size_t)ia * 3 + 2] = pv;
+        min_floor[ia] = mn;
+    }
+}
+
+// SIMPLE (maximum/FindMaximumSimple.java:19-48): a row-major scan of every cell
+// that keeps a strictly larger value, starting from 0; with no positive value
+// the peak is (D/2, D/2) with value 0.  The first maximal cell in row-major
+// order maximises the key (value bits, ~index) over cells with value > 0
+// (positive floats order as their bit patterns; NaN never compares greater).
+// minFloor is the minimum over every cell (NaN never compares smaller).
+__global__ __launch_bounds__(256) void peak_simple_kernel(int n, int D,
+                                                          const float *__restrict__ planes,
+                                                          float *__restrict__ peaks,
+                                                          float *__restrict__ min_floor)
+{
+    const int lane = threadIdx.x & 63;
+    const int ia = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (ia >= n) return;
+    const float *m = planes + (size_t)ia * D * D;
+    uint64_t best = 0;
+    float mn = FLT_MAX;
+    for (int idx = lane; idx < D * D; idx += 64) {
+        const float v = m[idx];
+        if (v < mn) mn = v;
+        if (v > 0.0f) {
+            const uint64_t key = (uint64_t)__float_as_uint(v) << 32 | (uint32_t)~(uint32_t)idx;
+            if (key > best) best = key;
+        }
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        const uint64_t ob = shfl_xor_u64(best, o);
+        const float om = __shfl_xor(mn, o, 64);
+        if (ob > best) best = ob;
+        if (om < mn) mn = om;
+    }
+    if (lane == 0) {
+        float pi = (float)(D / 2), pj = (float)(D / 2), pv = 0.0f;
+        if (best != 0) {
+            const int cell = (int)~(uint32_t)best;
+            pi = (float)(cell / D);
+            pj = (float)(cell % D);
+            pv = __uint_as_float((uint32_t)(best >> 32));
         }
         peaks[(size_t)ia * 3 + 0] = pi;
         peaks[(size_t)ia * 3 + 1] = pj;
@@ -143,6 +192,9 @@ int launch_find_peaks(Context *ctx, int kind, int n, int D, const float *planes,
     if (kind == QLP_PEAK_FROM_CENTER) {
         hipLaunchKernelGGL(peak_from_center_kernel, dim3((n + 3) / 4), dim3(256), 0, ctx->stream, n,
                            D, planes, peaks, min_floor);
+    } else if (kind == QLP_PEAK_SIMPLE) {
+        hipLaunchKernelGGL(peak_simple_kernel, dim3((n + 3) / 4), dim3(256), 0, ctx->stream, n, D,
+                           planes, peaks, min_floor);
     } else if (kind == QLP_PEAK_MULTI) {
         if (n_peaks < 1 || n_peaks > MP_MAX_PEAKS)
             return fail(QLP_ERR_ARG, "find_peaks: number of peaks must be 1..16");

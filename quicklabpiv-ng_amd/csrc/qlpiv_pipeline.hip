@@ -10,8 +10,8 @@
 //      Area, Distance or BiCubicSpline (qlpiv_inherit.hip); NoImageWarping
 //      rounds the inherited value;
 //   2. one fused pass: clip A, clip/shift/micro-warp B, correlate, FromCenter,
-//      export the peak neighbourhood (with MultiPeaks: the pass stores the
-//      planes and the peak search runs over them);
+//      export the peak neighbourhood (with MultiPeaks or the Simple finder: the
+//      pass stores the planes and the peak search runs over them);
 //   3. the sub-pixel fit of every peak and the displacement update
 //      u += i' - D/2 from the main peak;
 //   4. optionally the vector validation with secondary-peak and bilinear
@@ -276,10 +276,12 @@ static int plan_layout(int H, int W, int pairs, const qlp_piv_config *cfg, qlp_p
             return fail(QLP_ERR_ARG, "piv_plan_create: sub-pixel window must be odd and <= 15");
     }
     const bool multi = cfg->peak_kind == QLP_PEAK_MULTI;
-    if (cfg->peak_kind != QLP_PEAK_FROM_CENTER && !multi)
+    // the Simple and MultiPeaks finders run on stored planes
+    const bool separate = cfg->peak_kind != QLP_PEAK_FROM_CENTER;
+    if (cfg->peak_kind != QLP_PEAK_FROM_CENTER && !multi && cfg->peak_kind != QLP_PEAK_SIMPLE)
         return fail(QLP_ERR_ARG, "piv_plan_create: unknown peak finder kind");
     if (!multi && cfg->n_peaks > 1)
-        return fail(QLP_ERR_ARG, "find_peaks: FromCenter finds exactly one peak");
+        return fail(QLP_ERR_ARG, "find_peaks: FromCenter and Simple find exactly one peak");
     const int P = multi ? cfg->n_peaks : 1;
     if (multi) {
         if (P < 1 || P > 16) return fail(QLP_ERR_ARG, "find_peaks: number of peaks must be 1..16");
@@ -325,7 +327,7 @@ static int plan_layout(int H, int W, int pairs, const qlp_piv_config *cfg, qlp_p
         if (L.nI < 1 || L.nJ < 1) return fail(QLP_ERR_ARG, "piv_plan_create: no IA fits the frame");
         const size_t n = L.n, pn = (size_t)pairs * n, D = 2 * tile - 1;
         need += n * 8 + pn * (8 + 12 * P + 4 + 4 * 2 * NBR_L + 16 * P);
-        if (multi || cfg->keep_planes >= 2 || (cfg->keep_planes == 1 && l == nlev - 1))
+        if (separate || cfg->keep_planes >= 2 || (cfg->keep_planes == 1 && l == nlev - 1))
             need += pn * D * D * 4;
         if (multi) need += pn * 4;
         if (valid) need += n * 4 + valid_work_bytes(pairs, (int)n) + pn + (size_t)pairs * 16;
@@ -385,6 +387,7 @@ try {
         return rc;
     }
     const bool multi = cfg->peak_kind == QLP_PEAK_MULTI;
+    const bool separate = cfg->peak_kind != QLP_PEAK_FROM_CENTER;
     const bool valid = cfg->valid.n_stages > 0;
     const int P = p->P;
     if (ctx->hbm_budget && ctx->hbm_in_use + need > ctx->hbm_budget) {
@@ -402,7 +405,7 @@ try {
         if (!rc) rc = check_hip(hipMalloc(&L.mf, pn * 4), "hipMalloc plan");
         if (!rc) rc = check_hip(hipMalloc(&L.nbr, pn * 4 * 2 * NBR_L), "hipMalloc plan");
         if (!rc) rc = check_hip(hipMalloc(&L.sub, pn * 16 * P), "hipMalloc plan");
-        if (!rc && (multi || cfg->keep_planes >= 2 || (cfg->keep_planes == 1 && l + 1 == p->levels.size())))
+        if (!rc && (separate || cfg->keep_planes >= 2 || (cfg->keep_planes == 1 && l + 1 == p->levels.size())))
             rc = check_hip(hipMalloc(&L.planes, pn * D * D * 4), "hipMalloc plan");
         if (!rc && multi) rc = check_hip(hipMalloc(&L.count, pn * 4), "hipMalloc plan");
         if (!rc && valid) {
@@ -635,6 +638,7 @@ static int run_levels(qlp_piv_plan *p, const uint8_t *imgsA, const uint8_t *imgs
     Context *ctx = p->ctx;
     const qlp_piv_config &c = p->cfg;
     const bool multi = c.peak_kind == QLP_PEAK_MULTI;
+    const bool separate = c.peak_kind != QLP_PEAK_FROM_CENTER;
     for (size_t l = 0; l < p->levels.size(); l++) {
         Level &L = p->levels[l];
         const int pn = p->pairs * L.n;
@@ -751,7 +755,7 @@ static int run_levels(qlp_piv_plan *p, const uint8_t *imgsA, const uint8_t *imgs
         }
         q.planes = L.planes;
         q.arith = c.xcorr_form;
-        if (!multi) {  // FromCenter and the fit neighbourhood fused into the pass
+        if (!separate) {  // FromCenter and the fit neighbourhood fused into the pass
             q.peaks = L.peaks;
             q.min_floor = L.mf;
             q.nbr = L.nbr;
@@ -761,9 +765,9 @@ static int run_levels(qlp_piv_plan *p, const uint8_t *imgsA, const uint8_t *imgs
         if (rc) return rc;
         if (p->timing) (void)hipEventRecord(p->ev[4 * l + 1], ctx->stream);
         const int kind = ((int)l >= c.interpolator_start_step) ? c.subpix_kind : -1;
-        if (multi) {
-            rc = launch_find_peaks(ctx, QLP_PEAK_MULTI, pn, D, L.planes, p->P, c.peak_kernel_width,
-                                   L.peaks, L.mf, L.count);
+        if (separate) {
+            rc = launch_find_peaks(ctx, c.peak_kind, pn, D, L.planes, p->P, c.peak_kernel_width,
+                                   L.peaks, L.mf, multi ? L.count : nullptr);
             if (!rc)
                 rc = launch_subpixel(ctx, kind, c.subpix_pixels, c.subpix_iterations, pn, D, p->P,
                                      L.planes, L.mf, L.peaks, L.sub);

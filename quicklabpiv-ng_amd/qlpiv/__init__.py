@@ -20,6 +20,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import (QLP_NORM_A_MAX, QLP_NORM_PER_TILE, QLP_PEAK_FROM_CENTER, QLP_PEAK_MULTI,
+                   QLP_PEAK_SIMPLE,
                    QLP_SUBPIX_GAUSS1D, QLP_SUBPIX_GUO, QLP_SUBPIX_POLY, QLP_XCORR_EXACT,
                    QLP_XCORR_MFMA_BF16, QLP_XCORR_MFMA_BF16_SINGLE, QLP_XCORR_MFMA_F16,
                    QLP_XCORR_MFMA_F16_SINGLE, QlpError, build, check)
@@ -30,6 +31,7 @@ __all__ = [
     "piv_config", "plan_estimate",
     "VectorValidatorJob", "build",
     "QLP_NORM_PER_TILE", "QLP_NORM_A_MAX", "QLP_PEAK_FROM_CENTER", "QLP_PEAK_MULTI",
+    "QLP_PEAK_SIMPLE",
     "QLP_SUBPIX_GAUSS1D", "QLP_SUBPIX_GUO", "QLP_SUBPIX_POLY", "QLP_XCORR_EXACT",
     "QLP_XCORR_MFMA_F16", "QLP_XCORR_MFMA_BF16", "QLP_XCORR_MFMA_F16_SINGLE",
     "QLP_XCORR_MFMA_BF16_SINGLE",
@@ -911,6 +913,21 @@ class PivPlan:
         pshape = shp + ((4,) if self.P == 1 else (self.P, 4))
         return (_fetch(u, n, np.float32).reshape(shp), _fetch(v, n, np.float32).reshape(shp),
                 _fetch(pk, 4 * n * self.P, np.float32).reshape(pshape))
+
+    def planes(self, level=-1):
+        """Host copy of a level's stored correlation planes (pairs x nI x nJ x D x
+        D), or None when the level keeps no planes."""
+        lv = level % len(self.levels)
+        L = self.levels[lv]
+        u, v, pk, pl = (ctypes.c_void_p() for _ in range(4))
+        self.ctx.synchronize()
+        check(self.ctx.lib.qlp_piv_result_dev(self.h, lv, ctypes.byref(u), ctypes.byref(v),
+                                              ctypes.byref(pk), ctypes.byref(pl)))
+        if not pl.value:
+            return None
+        D = 2 * L["ia"] - 1
+        n = L["nI"] * L["nJ"] * self.pairs
+        return _fetch(pl, n * D * D, np.float32).reshape(self.pairs, L["nI"], L["nJ"], D, D)
 
     def validation(self, level=-1):
         """Host copies of a level's validation outcome: invalid flags

@@ -25,6 +25,7 @@ QLP_NORM_PER_TILE = 0
 QLP_NORM_A_MAX = 1
 QLP_PEAK_FROM_CENTER = 0
 QLP_PEAK_MULTI = 1
+QLP_PEAK_SIMPLE = 2
 QLP_SUBPIX_GAUSS1D = 0
 QLP_SUBPIX_GUO = 1
 QLP_SUBPIX_POLY = 2

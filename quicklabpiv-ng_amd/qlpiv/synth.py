@@ -29,19 +29,22 @@ def rankine_displacement(x, y, W, H, core_radius=128.0, u_max=6.0):
 
 
 def _render(H, W, xs, ys, i0, diameter, rng_noise, noise_sigma):
-    img = np.zeros(H * W, np.float64)
     rad = int(np.ceil(diameter)) + 1
     ix = np.floor(xs).astype(np.int64)
     iy = np.floor(ys).astype(np.int64)
     k = 8.0 / (diameter * diameter)
+    # every (particle, offset) contribution in one bincount: per pixel the sum
+    # runs over the offsets in order, particles in order within an offset
+    idx, val = [], []
     for oy in range(-rad, rad + 2):
         for ox in range(-rad, rad + 2):
             px, py = ix + ox, iy + oy
             ok = (px >= 0) & (px < W) & (py >= 0) & (py < H)
-            d2 = (px - xs) ** 2 + (py - ys) ** 2
-            val = i0 * np.exp(-k * d2)
-            img += np.bincount((py[ok] * W + px[ok]), weights=val[ok], minlength=H * W)
-    img = img.reshape(H, W)
+            d2 = (px[ok] - xs[ok]) ** 2 + (py[ok] - ys[ok]) ** 2
+            idx.append(py[ok] * W + px[ok])
+            val.append(i0[ok] * np.exp(-k * d2))
+    img = np.bincount(np.concatenate(idx), weights=np.concatenate(val),
+                      minlength=H * W).reshape(H, W)
     if noise_sigma > 0:
         img += rng_noise.normal(0.0, noise_sigma, img.shape)
     return np.clip(np.rint(img), 0, 255).astype(np.uint8)

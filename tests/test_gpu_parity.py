@@ -330,3 +330,22 @@ def test_job_interface_and_errors(ctx):
         ctx.xcorr(np.zeros((2, 24, 24), np.uint8), np.zeros((2, 24, 24), np.uint8),
                   np.ones(2, np.float32), np.ones(2, np.float32))
     assert e.value.code == qlpiv._lib.QLP_ERR_ARG
+
+
+@pytest.mark.parametrize("D", [31, 63, 127])
+def test_simple_bit_exact(ctx, D):
+    """FindMaximumSimple: row 0 and column 0 included, first maximum in row-major
+    order, (D/2, D/2, 0) without a positive value, NaN never taken."""
+    import qlpiv
+
+    rng = np.random.default_rng(D + 1)
+    planes = planes_for_peaks(rng, 64, D)
+    planes[8] = np.nan
+    planes[9, 1, 1] = np.nan
+    planes[10, 0, 0] = 1e5
+    say(f"simple D={D}")
+    peaks, mf = ctx.find_peaks(planes, qlpiv.QLP_PEAK_SIMPLE, 1)
+    for k in range(len(planes)):
+        ref = O.find_max_simple(planes[k])
+        assert tuple(peaks[k, 0]) == tuple(ref[:3]), k
+        assert mf[k] == ref[3], k

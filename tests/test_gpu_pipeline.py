@@ -189,3 +189,43 @@ def test_plan_errors(ctx):
         qlpiv.PivPlan(small, 2048, 2048, 4)
     assert e.value.code == qlpiv._lib.QLP_ERR_BUDGET
     small.close()
+
+
+PLAN_CASES = [
+    # name, plan options, oracle options
+    ("simple-gauss5-micro",
+     dict(warp_mode=1, peak_kind=2, subpix_kind=0, subpix_pixels=5),
+     dict(warp_mode=1, norm_mode=0, peak_kind=2, fit_kind=0, pixels=5)),
+    ("fromcenter-guo-start1-micro",
+     dict(warp_mode=1, subpix_kind=1, subpix_pixels=3, subpix_iterations=20,
+          interpolator_start_step=1),
+     dict(warp_mode=1, norm_mode=0, peak_kind=0, fit_kind=1, start_step=1)),
+    ("multi-poly-start2-shift-amax",
+     dict(warp_mode=0, norm_mode=1, peak_kind=1, n_peaks=3, peak_kernel_width=3, subpix_kind=2,
+          interpolator_start_step=2),
+     dict(warp_mode=0, norm_mode=1, peak_kind=1, n_peaks=3, fit_kind=2, start_step=2)),
+    ("simple-guo7-start1-shift",
+     dict(warp_mode=0, peak_kind=2, subpix_kind=1, subpix_pixels=7, subpix_iterations=20,
+          interpolator_start_step=1, keep_planes=2),
+     dict(warp_mode=0, norm_mode=0, peak_kind=2, fit_kind=1, pixels=7, start_step=1)),
+]
+
+
+@pytest.mark.parametrize("case", PLAN_CASES, ids=[c[0] for c in PLAN_CASES])
+def test_plan_finders_fits_and_start_step(ctx, pair256, case):
+    """The device loop with each peak finder (Simple, FromCenter, MultiPeaks),
+    each fit and an interpolator start step above 0 (the levels before it keep
+    the integer peak, CompositeLastLevelInterpolator / LocalPIVOpenCLGpuJob
+    :373), level by level against the oracle."""
+    import torch
+
+    import qlpiv
+    from plan_oracle import check_plan_levels
+
+    name, popt, oopt = case
+    a, b, _ = pair256
+    plan = qlpiv.PivPlan(ctx, 256, 256, 1, ia_start=64, ia_end=16, overlap_factor=0.5, **popt)
+    say(f"plan {name}")
+    plan.run(torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda())
+    check_plan_levels(plan, a, b, log=say, **oopt)
+    plan.close()

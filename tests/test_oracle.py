@@ -482,3 +482,102 @@ def test_warp_micro_invariants():
             ref[i, j] = np.floor(v + 0.5)
     assert np.max(np.abs(got - ref)) <= 1.0  # float vs double rounding at x.5 only
     assert np.mean(got == ref) > 0.98
+
+
+# --------------------------------------------------------------------------
+# FindMaximumSimple and the per-level batch
+# --------------------------------------------------------------------------
+
+def simple_brute(plane):
+    """FindMaximumSimple.findMaximum :19-48 stated on the array: the first cell in
+    row-major order holding the largest value above 0 ((D/2, D/2, 0) without
+    one); minFloor the smallest non-NaN value (Float.MAX_VALUE without one)."""
+    flat = plane.ravel()
+    pos = np.where(flat > 0, flat, -np.inf)
+    if not np.any(flat > 0):
+        pi, pj, pv = plane.shape[0] // 2, plane.shape[1] // 2, 0.0
+    else:
+        idx = int(np.argmax(pos))  # first occurrence of the maximum
+        pi, pj, pv = idx // plane.shape[1], idx % plane.shape[1], flat[idx]
+    finite = flat[~np.isnan(flat)]
+    mn = finite.min() if finite.size else np.float32(3.4028235e38)
+    return pi, pj, np.float32(pv), np.float32(mn)
+
+
+def test_find_max_simple_brute_force():
+    rng = np.random.default_rng(5)
+    planes = []
+    for D in (3, 10, 31, 63):
+        p = rng.random((12, D, D)).astype(np.float32) * 100
+        p[0] = np.round(p[0] / 30)            # ties: first in row-major order wins
+        p[1] = -1.0                           # nothing positive
+        p[2] = 0.0
+        p[3, 0, 0] = 1e4                      # row 0 / column 0 are visited
+        p[4, D - 1, D - 1] = 1e4
+        p[5, :, :] = np.nan
+        p[6, 1, 1] = np.nan
+        p[7, D // 2, 0] = p[7, D // 2, D - 1] = 500
+        p[8] = -np.abs(p[8])
+        p[8, D - 1, 0] = 1e-30               # a tiny positive maximum
+        p[9, 2 % D, 1] = -0.0
+        planes += list(p)
+    for p in planes:
+        r = O.find_max_simple(p)
+        pi, pj, pv, mn = simple_brute(p)
+        assert (r[0], r[1], r[2]) == (pi, pj, pv)
+        assert r[3] == mn
+
+
+def test_level_batch_composes_the_restated_steps():
+    """or_level_batch equals the per-IA oracle functions called one by one, for
+    every warp, mode, peak finder and fit (the full-size GPU tests use it)."""
+    from qlpiv.geometry import level_grid
+    from qlpiv.synth import particle_pair
+
+    a, b, _ = particle_pair(128, 128, seed=3, shift=(1.2, -2.1))
+    N = 16
+    tops, lefts = level_grid(128, 128, N, 0.5)
+    top, left = tops.ravel()[::5].astype(np.int32), lefts.ravel()[::5].astype(np.int32)
+    n = top.size
+    rng = np.random.default_rng(0)
+    u = rng.uniform(-3, 3, n).astype(np.float32)
+    v = rng.uniform(-3, 3, n).astype(np.float32)
+    D = 2 * N - 1
+    for warp in (0, 1, 2):
+        for mode in (0, 1):
+            for peak_kind, P in ((0, 1), (1, 3), (2, 1)):
+                for fit in (-1, 0, 1, 2):
+                    out, cnt, pl = O.level_batch(a, b, top, left, u, v, N, warp, mode, peak_kind,
+                                                 P, 3, fit, 3, 20, planes=True)
+                    for t in range(n):
+                        tA, mA = O.clip_tile(a, top[t], left[t], N)
+                        if warp == 2:
+                            tB, mB = O.warp_micro(b, top[t], left[t], N, u[t], v[t]), 255.0
+                        elif warp == 1:
+                            tt = int(np.floor(np.float32(top[t]) + u[t] + np.float32(0.5)))
+                            ll = int(np.floor(np.float32(left[t]) + v[t] + np.float32(0.5)))
+                            tB, mB = O.clip_tile(b, tt, ll, N)
+                        else:
+                            tB, mB = O.clip_tile(b, top[t], left[t], N)
+                        A, B = tA.astype(np.float32), tB.astype(np.float32)
+                        ref = O.xcorr_javagpu(A, B, mA, mB) if mode == 0 else O.xcorr_javacpu(A, B, mA)
+                        assert np.array_equal(pl[t], ref)
+                        if peak_kind == 1:
+                            pk, found, mf = O.find_max_multipeaks(ref, P, 3)
+                        else:
+                            r = O.find_max_simple(ref) if peak_kind == 2 else O.find_max_from_center(ref)
+                            pk, found, mf = r[None, :3], 1, r[3]
+                        assert cnt[t] == max(found, 1)
+                        for q in range(out.shape[1]):
+                            if q >= found:
+                                assert not out[t, q].any()
+                                continue
+                            if fit == 0:
+                                e = O.gauss1d(ref, 3, mf, pk[q])
+                            elif fit == 1:
+                                e = O.guo(ref, 3, 20, mf, pk[q])
+                            elif fit == 2:
+                                e = O.poly(ref, pk[q])
+                            else:
+                                e = np.array([*pk[q], 4.0], np.float32)
+                            assert np.array_equal(out[t, q], e), (warp, mode, peak_kind, fit, t, q)
