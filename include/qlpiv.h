@@ -106,6 +106,18 @@ int qlp_xcorr_batch(qlp_ctx *ctx, int n, int N, const uint8_t *tilesA, const uin
 int qlp_xcorr_batch_dev(qlp_ctx *ctx, int n, int N, const uint8_t *tilesA, const uint8_t *tilesB,
                         const float *maxA, const float *maxB, int mode, float *planes_out);
 
+/* qlp_xcorr_batch_dev without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments,
+ * layouts and values as qlp_xcorr_batch_dev.  The 128 px correlation keeps its
+ * 512 KB tiles in a library-owned static device array of 64 slots (32 MiB per
+ * GPU, held until the process ends) and runs on at most 64 work-groups; 16, 32
+ * and 64 px tiles stay in LDS.  Calls that overlap on one GPU share that array
+ * and race: serialise them (one stream, or one call at a time). */
+int qlp_xcorr_batch_noalloc_dev(qlp_ctx *ctx, int n, int N, const uint8_t *tilesA,
+                                const uint8_t *tilesB, const float *maxA, const float *maxB,
+                                int mode, float *planes_out);
+
 /* The same job on float tiles: the MatrixFloat tiles of frames filtered before
  * warping (images/ImageFloat.java clipImageMatrix :278-384, fed by
  * GaussianFilter2D.applyFilter(IImage, IImage) :116-126 through

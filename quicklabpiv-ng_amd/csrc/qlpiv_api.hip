@@ -157,6 +157,12 @@ try {
             ctx->hbm_in_use += sizeof(float2) * (M / 2);
         }
     }
+    // the allocation-free entry point's scratch address on this device (current)
+    rc = init_noalloc_scratch(hip_device);
+    if (rc) {
+        qlp_ctx_destroy(h);
+        return rc;
+    }
     *out = h;
     return QLP_OK;
 } catch (...) {
@@ -215,6 +221,19 @@ try {
     if (n > 0 && (!tA || !tB || !maxA || !planes || (mode == QLP_NORM_PER_TILE && !maxB)))
         return fail(QLP_ERR_ARG, "xcorr: null buffer");
     return launch_xcorr(&h->c, n, N, tA, tB, maxA, maxB, mode, planes);
+} catch (...) {
+    return qlpiv::fail_exception();
+}
+
+int qlp_xcorr_batch_noalloc_dev(qlp_ctx *h, int n, int N, const uint8_t *tA, const uint8_t *tB,
+                                const float *maxA, const float *maxB, int mode, float *planes)
+try {
+    if (!h) return fail(QLP_ERR_ARG, "null context");
+    int rc = xcorr_args(n, N, mode);
+    if (rc) return rc;
+    if (n > 0 && (!tA || !tB || !maxA || !planes || (mode == QLP_NORM_PER_TILE && !maxB)))
+        return fail(QLP_ERR_ARG, "xcorr: null buffer");
+    return launch_xcorr_noalloc(&h->c, n, N, tA, tB, maxA, maxB, mode, planes);
 } catch (...) {
     return qlpiv::fail_exception();
 }

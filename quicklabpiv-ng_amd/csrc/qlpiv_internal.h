@@ -124,6 +124,12 @@ struct PassParams {
 int launch_xcorr(Context *ctx, int n, int N, const uint8_t *tA, const uint8_t *tB,
                  const float *maxA, const float *maxB, int mode, float *planes);
 int launch_pass(Context *ctx, const PassParams &p);
+// The allocation-free correlation: the IA-128 slots in a static device array
+// whose per-device address init_noalloc_scratch (with the device current)
+// looks up once; launch_xcorr_noalloc allocates nothing.
+int init_noalloc_scratch(int device);
+int launch_xcorr_noalloc(Context *ctx, int n, int N, const uint8_t *tA, const uint8_t *tB,
+                         const float *maxA, const float *maxB, int mode, float *planes);
 int launch_pass_lane(Context *ctx, const PassParams &p, bool from_image, bool peak);
 // IA 128: the tile lives in an HBM workspace (ctx->bigwork) of one 512 KB slot
 // per resident work-group.  big_work_need is the workspace of a pass of n IAs

@@ -24,6 +24,8 @@
 //   false : CrossCorrelationRealFFTParStdJob (each tile by its own max; product
 //           divided by M^2 once; inverse through the re/im swap of the forward
 //           transform)
+#include <mutex>
+
 #include "qlpiv_device.h"
 #include "qlpiv_epilogue.h"
 #include "qlpiv_internal.h"
@@ -846,6 +848,77 @@ int launch_pass(Context *ctx, const PassParams &p)
     default: return fail(QLP_ERR_UNSUPPORTED, "xcorr: IA size N must be 16, 32, 64 or 128");
     }
     return check_hip(hipGetLastError(), "xcorr pass launch");
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free correlation (qlp_xcorr_batch_noalloc_dev).  The IA-128 tile
+// slots live in this file-scope device array instead of the context's
+// hipMalloc'd workspace: NOALLOC_SLOTS slots of 512 KB (32 MiB per GPU, held
+// until the process ends), so the 128 px pass runs on at most NOALLOC_SLOTS
+// work-groups, each looping over its IAs.  16 / 32 / 64 px passes keep their
+// tile in LDS and need no slot.  Every call on a GPU shares that GPU's array:
+// calls that overlap on one GPU race.  The slots need no clearing: a pass
+// writes every slot row it later reads.
+constexpr int NOALLOC_SLOTS = 64;
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)NOALLOC_SLOTS * big::SLOT_BYTES;
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static __device__ __align__(256) uint8_t scratch[NOALLOC_SCRATCH_BYTES];
+// the array's address on each device (hipGetSymbolAddress with that device
+// current); host code uses these, never the array's name
+static void *g_scratch_ptr[NOALLOC_MAX_DEVICES];
+static std::mutex g_scratch_mu;
+
+int init_noalloc_scratch(int device)
+{
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES)
+        return fail(QLP_ERR_DEVICE, "no-alloc scratch: device index beyond 63");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return QLP_OK;
+    void *ptr = nullptr;
+    int rc = check_hip(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    if (rc) return rc;
+    g_scratch_ptr[device] = ptr;
+    return QLP_OK;
+}
+
+template <bool MODE_CPU>
+static int launch_big_noalloc(Context *ctx, const PassParams &p, float2 *work)
+{
+    auto k = xcorr_big_kernel<MODE_CPU, false, false, 512, 32>;
+    static const int groups = resident_groups(k, 512);
+    int slots = p.n < groups ? p.n : groups;
+    if (slots > NOALLOC_SLOTS) slots = NOALLOC_SLOTS;
+    const float2 *twf = ctx->tables.tw[big::LOGM - 5][TW_FWD];
+    const float2 *twi = ctx->tables.tw[big::LOGM - 5][MODE_CPU ? TW_INV_CPU : TW_INV_GPU];
+    hipLaunchKernelGGL(k, dim3(slots), dim3(512), 0, ctx->stream, p, twf, twi, work);
+    return check_hip(hipGetLastError(), "IA-128 pass launch");
+}
+
+int launch_xcorr_noalloc(Context *ctx, int n, int N, const uint8_t *tA, const uint8_t *tB,
+                         const float *maxA, const float *maxB, int mode, float *planes)
+{
+    if (N != 16 && N != 32 && N != 64 && N != 128)
+        return fail(QLP_ERR_ARG, "xcorr_noalloc: IA size N must be 16, 32, 64 or 128 (a 128 px "
+                                 "tile fills one 512 KB scratch slot)");
+    PassParams p;
+    p.n = n;
+    p.N = N;
+    p.mode = mode;
+    p.tA = tA;
+    p.tB = tB;
+    p.maxA = maxA;
+    p.maxB = maxB;
+    p.planes = planes;
+    if (n == 0) return QLP_OK;
+    if (N < 128) return launch_pass(ctx, p);  // the tile stays in LDS: no scratch
+    void *ptr = nullptr;
+    if (ctx->device >= 0 && ctx->device < NOALLOC_MAX_DEVICES) {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        ptr = g_scratch_ptr[ctx->device];
+    }
+    if (!ptr) return fail(QLP_ERR_DEVICE, "xcorr_noalloc: scratch address not set up for this device");
+    return mode == QLP_NORM_A_MAX ? launch_big_noalloc<true>(ctx, p, (float2 *)ptr)
+                                  : launch_big_noalloc<false>(ctx, p, (float2 *)ptr);
 }
 
 int launch_xcorr(Context *ctx, int n, int N, const uint8_t *tA, const uint8_t *tB,

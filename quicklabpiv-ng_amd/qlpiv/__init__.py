@@ -125,6 +125,20 @@ class Context:
                                            _ptr(maxB), int(mode), _ptr(out)))
         return out
 
+    def xcorr_noalloc(self, tilesA, tilesB, maxA, maxB=None, mode=QLP_NORM_PER_TILE, out=None):
+        """xcorr on device tensors through qlp_xcorr_batch_noalloc_dev: the
+        library allocates nothing (out must be given or is allocated here by
+        torch, outside the library)."""
+        import torch
+
+        n, N = int(tilesA.shape[0]), int(tilesA.shape[-1])
+        D = 2 * N - 1
+        if out is None:
+            out = torch.empty((n, D, D), dtype=torch.float32, device=tilesA.device)
+        check(self.lib.qlp_xcorr_batch_noalloc_dev(self.h, n, N, _ptr(tilesA), _ptr(tilesB),
+                                                   _ptr(maxA), _ptr(maxB), int(mode), _ptr(out)))
+        return out
+
     def xcorr_f32(self, tilesA, tilesB, maxA, maxB=None, mode=QLP_NORM_PER_TILE, out=None):
         """Correlate n pairs of float tiles (MatrixFloat, e.g. cut from filtered
         frames) in the exact form; numpy inputs use the host entry point, torch

@@ -46,6 +46,7 @@ SIGNATURES = {
     "qlp_version": (ctypes.c_char_p, []),
     "qlp_xcorr_batch": (_i, [_vp, _i, _i, _vp, _vp, _vp, _vp, _i, _vp]),
     "qlp_xcorr_batch_dev": (_i, [_vp, _i, _i, _vp, _vp, _vp, _vp, _i, _vp]),
+    "qlp_xcorr_batch_noalloc_dev": (_i, [_vp, _i, _i, _vp, _vp, _vp, _vp, _i, _vp]),
     "qlp_xcorr_f32": (_i, [_vp, _i, _i, _vp, _vp, _vp, _vp, _i, _vp]),
     "qlp_xcorr_f32_dev": (_i, [_vp, _i, _i, _vp, _vp, _vp, _vp, _i, _vp]),
     "qlp_clip_tiles_f32_dev": (_i, [_vp, _vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp]),
