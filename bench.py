@@ -333,18 +333,7 @@ def serve_report(n_req, rate, slots):
           flush=True)
 
 
-BATCH_SIZES = [(512, 512), (1024, 1024), (1024, 2048), (2048, 2048)]  # H x W
-
-
-def _batch_frame(spec):
-    k, h, w, _ia, seed, R, U = spec
-    from qlpiv.synth import particle_pair
-
-    a, b, _ = particle_pair(h, w, seed=seed, shift=(0.0, 0.0), core_radius=R, u_max=U)
-    return k, a, b
-
-
-def batch_report(steps, warmup):
+def batch_report(steps, warmup, gen_workers=8):
     """The batching configuration: 64 frame pairs, 16 each at 512x512,
     1024x1024, 1024x2048 and 2048x2048, the IA size of each pair drawn from
     {16, 32, 64, 128} at 50 % overlap, per-pair Rankine vortex R ~ U(32, 256),
@@ -352,28 +341,21 @@ def batch_report(steps, warmup):
     normalisation, FromCenter, Guo 3 px / 20 iterations) for all 64 pairs in one
     graph launch (qlp_batch), against the same work as one one-level plan per
     pair run one after the other."""
-    from multiprocessing import Pool
-
     import torch
 
     import qlpiv
+    from qlpiv.synth import batch_frames, batch_specs
 
-    rng = np.random.default_rng(1700)
-    specs = []
-    for (h, w) in BATCH_SIZES:
-        for _ in range(16):
-            specs.append((len(specs), h, w, int(rng.choice([16, 32, 64, 128])), 1700 + len(specs),
-                          float(rng.uniform(32, 256)), float(rng.uniform(2, 10))))
+    specs = batch_specs(1700)
     print(f"[batch] generating {len(specs)} frame pairs", file=sys.stderr, flush=True)
-    with Pool(8) as pool:
-        got = sorted(pool.map(_batch_frame, specs), key=lambda x: x[0])
-    dA = [torch.from_numpy(a).cuda() for _, a, _ in got]
-    dB = [torch.from_numpy(b).cuda() for _, _, b in got]
+    got = batch_frames(specs, gen_workers)
+    dA = [torch.from_numpy(a).cuda() for a, _ in got]
+    dB = [torch.from_numpy(b).cuda() for _, b in got]
     del got
     cfg = dict(overlap_factor=0.5, subpix_kind=qlpiv.QLP_SUBPIX_GUO, subpix_pixels=3,
                subpix_iterations=20)
     ctx = qlpiv.Context(0)
-    pairs = [(h, w, ia) for (_, h, w, ia, _, _, _) in specs]
+    pairs = [(h, w, ia) for (h, w, ia, _, _, _) in specs]
     batch = qlpiv.Batch(ctx, pairs, **cfg)
     print(f"[batch] {batch.total_ias} IAs; tuning and building the graph", file=sys.stderr, flush=True)
     batch.run(dA, dB)
@@ -448,6 +430,9 @@ def main():
     ap.add_argument("--serve-slots", type=int, default=8)
     ap.add_argument("--batch", action="store_true",
                     help="print the batching report (64 mixed pairs in one graph launch)")
+    ap.add_argument("--gen-workers", type=int, default=8,
+                    help="processes generating the batch frames (0: in this process, for "
+                         "profilers that do not follow forks)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (one MI355X per box); "
@@ -460,7 +445,7 @@ def main():
         serve_report(args.serve_requests, args.serve_rate, args.serve_slots)
         return
     if args.batch:
-        batch_report(args.steps, args.warmup)
+        batch_report(args.steps, args.warmup, args.gen_workers)
         return
 
     import torch

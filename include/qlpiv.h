@@ -492,15 +492,21 @@ int qlp_piv_plan_timing(qlp_piv_plan *plan, int level, float *state_and_pass_ms,
  * the budget check of qlp_piv_plan_create and of the server's admission). */
 int qlp_piv_plan_estimate(int H, int W, int pairs, const qlp_piv_config *cfg, size_t *bytes);
 
-/* ---- batches of frame pairs of mixed sizes in one graph launch --------------
+/* ---- batches of frame pairs of mixed sizes, one graph launch per level -------
  *
- * No counterpart in the reference, which runs its single-level frame job
- * (jobs/LocalPIVOpenCLGpuJob.call :188-409: clip, correlation, FromCenter peak,
- * sub-pixel fit) one frame pair at a time.  A batch holds many pairs, each with
- * its own frame size and IA size (16, 32, 64 or 128), and runs that job for all
- * of them as ONE launch of a HIP graph built once and replayed for every new set
- * of frames.  Per IA the results equal those of a one-level plan
- * (ia_start = ia_end = the pair's IA size) with the same configuration.
+ * No counterpart in the reference, which runs its per-frame job
+ * (jobs/LocalPIVOpenCLGpuJob.call :188-409: clip or micro warp, correlation,
+ * FromCenter peak, sub-pixel fit, Direct inheritance between the adaptive
+ * levels) one frame pair at a time.  A batch holds many pairs, each with its
+ * own frame size, last IA size (16, 32, 64 or 128) and number of adaptive
+ * levels, and runs level k of every pair as ONE launch of a HIP graph of
+ * persistent kernels, captured once and replayed for every level and every new
+ * set of frames (a step table in device memory tells the kernels what each
+ * launch holds).  Per IA the results equal those of a multi-pass plan of that
+ * pair alone (ia_start = ia << (levels - 1), ia_end = ia) with the same
+ * configuration.  Configurations: FromCenter peak, Direct inheritance, no warp
+ * or the SecondImage micro warp, no validation, 8-bit frames, any fit and
+ * arithmetic form (QLP_ERR_UNSUPPORTED otherwise).
  *
  * Autotuner: on the first run every IA-size class is timed with each kernel
  * shape that computes its correlation (QLP_SHAPE_*; the exact shapes give the
@@ -519,8 +525,9 @@ int qlp_piv_plan_estimate(int H, int W, int pairs, const qlp_piv_config *cfg, si
 
 typedef struct qlp_batch qlp_batch;
 typedef struct {
-    int H, W; /* frame size */
-    int ia;   /* IA size: 16, 32, 64 or 128 */
+    int H, W;   /* frame size */
+    int ia;     /* IA size of the last level: 16, 32, 64 or 128 */
+    int levels; /* adaptive levels, the first at ia << (levels - 1) (0 or 1: one) */
 } qlp_batch_pair;
 typedef struct {
     int ia, n_ias;     /* IA size and IAs of the class */
@@ -536,24 +543,25 @@ typedef struct {
     float serial_ms, concurrent_ms;  /* best of 3 graph launches (0 if not timed) */
 } qlp_batch_report;
 
-/* Lay out the IAs of n_pairs pairs (first-level grid of each frame with the
- * config's overlap and margins) and allocate their device state.  Used fields
- * of cfg: overlap_factor, margins, norm_mode, subpix_kind / _pixels /
- * _iterations, interpolator_start_step, xcorr_form (16, 32 and 64 px classes). */
+/* Lay out the IAs of every level of n_pairs pairs (the plan's level grids with
+ * the config's overlap and margins) and allocate their device state.  Used
+ * fields of cfg: overlap_factor, margins, norm_mode, warp_mode (NONE or MICRO),
+ * subpix_kind / _pixels / _iterations, interpolator_start_step, xcorr_form
+ * (16, 32 and 64 px classes); the others must keep their defaults. */
 int qlp_batch_create(qlp_ctx *ctx, int n_pairs, const qlp_batch_pair *pairs,
                      const qlp_piv_config *cfg, qlp_batch **out);
 /* Fix the kernel shape of the class of IA size 'ia' (ia 0: every class) and/or
  * the schedule (QLP_SCHEDULE_TUNED keeps it measured), before the first run. */
 int qlp_batch_fix(qlp_batch *b, int ia, int shape, int schedule);
-/* Correlate every pair: imgsA[k], imgsB[k] are device pointers to pair k's
- * H x W uint8 frames (any address each run).  Asynchronous on the context's
- * stream; the first run tunes and builds the graph (synchronising). */
+/* Run every level of every pair: imgsA[k], imgsB[k] are device pointers to
+ * pair k's H x W uint8 frames (any address each run).  Asynchronous on the
+ * context's stream; the first run tunes and builds the graph (synchronising). */
 int qlp_batch_run_dev(qlp_batch *b, const uint8_t *const *imgsA, const uint8_t *const *imgsB);
-/* Grid of pair k and the total IA count of the batch. */
+/* Last-level grid of pair k and the IA count of the batch (every level). */
 int qlp_batch_grid(qlp_batch *b, int pair, int *nI, int *nJ, size_t *total_ias);
-/* Device pointers of pair k's results (valid once the run has completed):
- * u, v (nI*nJ displacements, vertical then horizontal) and sub (nI*nJ x 4:
- * i', j', value', status of the sub-pixel fit). */
+/* Device pointers of pair k's last-level results (valid once the run has
+ * completed): u, v (nI*nJ displacements, vertical then horizontal) and sub
+ * (nI*nJ x 4: i', j', value', status of the sub-pixel fit). */
 int qlp_batch_result_dev(qlp_batch *b, int pair, const float **u, const float **v,
                          const float **sub);
 int qlp_batch_report_get(qlp_batch *b, qlp_batch_report *report);

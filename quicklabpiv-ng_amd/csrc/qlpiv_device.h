@@ -7,6 +7,13 @@
 namespace qlpiv {
 
 // commons-math3 FastMath.round(float) = (int)floor(x + 0.5f)
+// IA count of a pass: from device memory for a persistent launch.
+template <typename PP>
+__device__ __forceinline__ int pass_count(const PP &p)
+{
+    return p.n_dev ? *p.n_dev : p.n;
+}
+
 __device__ __forceinline__ int fm_round(float x) { return (int)floorf(x + 0.5f); }
 
 // BiLinearImageMicroAndMiniWarpingStrategy.warpPixelOnImage :336-407, SecondImage

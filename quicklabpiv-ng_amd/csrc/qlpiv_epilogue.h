@@ -30,8 +30,10 @@ __device__ __forceinline__ void from_center_visit(float v, int i, int j, int h, 
 template <int NT, int D>
 __device__ __forceinline__ void peak_epilogue(const PassParams &p, int ia, uint64_t best,
                                               int bestCell, float mn, const float *src, int rs,
-                                              int cs, uint64_t *s_key, int *s_cell, float *s_min)
+                                              int cs, uint64_t *s_key, int *s_cell, float *s_min,
+                                              int tid = -1)
 {
+    const int tx = tid < 0 ? (int)threadIdx.x : tid;
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) {
         const uint64_t ob = shfl_xor_u64(best, o);
@@ -43,13 +45,13 @@ __device__ __forceinline__ void peak_epilogue(const PassParams &p, int ia, uint6
         }
         if (om < mn) mn = om;
     }
-    if ((threadIdx.x & 63) == 0) {
-        s_key[threadIdx.x >> 6] = best;
-        s_cell[threadIdx.x >> 6] = bestCell;
-        s_min[threadIdx.x >> 6] = mn;
+    if ((tx & 63) == 0) {
+        s_key[tx >> 6] = best;
+        s_cell[tx >> 6] = bestCell;
+        s_min[tx >> 6] = mn;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (tx == 0) {
         for (int w = 1; w < NT / 64; w++) {
             if (s_key[w] > best) {
                 best = s_key[w];

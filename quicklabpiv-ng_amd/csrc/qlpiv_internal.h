@@ -118,7 +118,33 @@ struct PassParams {
     int nbr_half = 0;
     int arith = QLP_XCORR_EXACT;  // QLP_XCORR_*: arithmetic form of the correlation
     int shape = 0;                // PassShape of an exact pass
+    // Persistent launch: the IA count lives in device memory (written before the
+    // launch, e.g. by a step of a captured graph) and n is only the capacity of
+    // the buffers.  The grid is then sized to the work-groups the device holds
+    // at once and each work-group loops over IAs.
+    const int *n_dev = nullptr;
 };
+
+// Work-groups of one kernel that the device holds at once.
+template <typename K>
+static inline int resident_groups(K kernel, int threads, size_t lds = 0)
+{
+    int per_cu = 0, dev = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, lds) != hipSuccess ||
+        hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return 256;
+    return (per_cu < 1 ? 1 : per_cu) * cus;
+}
+
+// Grid of a pass over n IAs in blocks of 'per_block' IAs: one block per group
+// of IAs, or for a persistent launch at most the resident work-groups.
+static inline int pass_grid(const PassParams &p, int per_block, int resident)
+{
+    const int blocks = (p.n + per_block - 1) / per_block;
+    if (!p.n_dev) return blocks;
+    return blocks < resident ? (blocks < 1 ? 1 : blocks) : resident;
+}
 
 // Launchers (qlpiv_xcorr.hip, qlpiv_peaks.hip, qlpiv_subpixel.hip, qlpiv_warp.hip).
 int launch_xcorr(Context *ctx, int n, int N, const uint8_t *tA, const uint8_t *tB,
@@ -147,9 +173,12 @@ int launch_find_peaks(Context *ctx, int kind, int n, int D, const float *planes,
 int launch_subpixel(Context *ctx, int kind, int pixels, int iterations, int n, int D, int n_peaks,
                     const float *planes, const float *min_floor, const float *peaks_in,
                     float *peaks_out);
+// n_dev: persistent launch (n is the capacity, the count is in device memory);
+// fit_on: per-IA flag, 0 keeps the integer peak (status SKIPPED).
 int launch_subpixel_nbr(Context *ctx, int kind, int pixels, int iterations, int n, int D,
                         int nbr_half, const float *nbr, const float *min_floor,
-                        const float *peaks_in, float *peaks_out, float *du, float *dv);
+                        const float *peaks_in, float *peaks_out, float *du, float *dv,
+                        const int *n_dev = nullptr, const uint8_t *fit_on = nullptr);
 int launch_clip(Context *ctx, const uint8_t *img, int H, int W, int n, int N, const int *top,
                 const int *left, uint8_t *tiles, float *maxv);
 // Tile warps of one side (WarpSide, qlpiv_device.h; SIDE_B_SECOND is the
@@ -244,10 +273,54 @@ int launch_inherit_direct(Context *ctx, int pairs, int nI, int nJ, int pI, int p
                           const float *pu, const float *pv, float *u, float *v);
 int ensure_iwork(Context *ctx, size_t bytes);
 
+// ---- the step engine (qlpiv_engine.hip): one adaptive level of many frame
+// pairs per step, in one persistent, graph-captured launch -----------------
+constexpr int ENG_CLASSES = 4;     // IA sizes 16, 32, 64, 128
+constexpr int ENG_MAX_SEGS = 4096; // pair levels in one step, all classes
+enum { STEP_CLIP = 0, STEP_SHIFT = 1, STEP_MICRO = 2 };
+// One pair level of a step.
+struct StepSeg {
+    const uint8_t *a, *b;   // the pair's frames (device, H x W)
+    const int *top, *left;  // the level's IA origins (device, nI * nJ)
+    const float *pu, *pv;   // the parent level's field (device, pnI * pnJ), null at the first level
+    float *u, *v;           // this level's field (device, nI * nJ), written by the step
+    float *sub;             // fit records (device, nI * nJ x 4) or null
+    int H, W, nI, nJ, pnI, pnJ;
+    int first;              // first IA of the segment in its class (set by engine_step)
+    int warp;               // B tile: STEP_CLIP, STEP_SHIFT (no warp, rounded inheritance)
+                            // or STEP_MICRO (micro warp); the first level always clips
+    int fit;                // 1: sub-pixel fit; 0: the integer peak (before the start step)
+    int pad;
+};
+struct StepHeader {
+    int n[ENG_CLASSES];     // IAs of each class in the step
+    int nseg[ENG_CLASSES];  // segments of each class
+    int base[ENG_CLASSES];  // first segment of each class
+    int pad;
+};
+struct EngineConfig {
+    int norm_mode, subpix_kind, subpix_pixels, subpix_iterations, xcorr_form;
+};
+struct Engine;
+int eng_class_of(int ia);
+size_t engine_bytes(const EngineConfig &cfg, const size_t cap[ENG_CLASSES]);
+// cap: the most IAs of each class one step may hold (0: the class is unused).
+int engine_create(Context *ctx, const EngineConfig &cfg, const size_t cap[ENG_CLASSES], Engine **out);
+int engine_fix(Engine *e, int ia, int shape, int schedule);
+// Enqueue one step on the context's stream (the first step tunes and builds
+// the graph, synchronising).  segs[c]: the pair levels of class c.
+int engine_step(Engine *e, const std::vector<StepSeg> segs[ENG_CLASSES]);
+int engine_report(const Engine *e, qlp_batch_report *r);
+void engine_destroy(Engine *e);
+
 // First-level IA origins of a frame (AdaptiveInterAreaStrategySuperPosition,
 // qlpiv_pipeline.hip); margins = top, bottom, left, right.
 void grid_origins(int H, int W, int tile, float overlap_factor, const int margins[4], int &nI,
                   int &nJ, std::vector<int> &top, std::vector<int> &left);
+// IA origins of any adaptive level (first: the first level's placement, else
+// the later levels' float accumulation), as a plan lays them out.
+void level_origins(int H, int W, int tile, float overlap_factor, const int margins[4], bool first,
+                   int &nI, int &nJ, std::vector<int> &top, std::vector<int> &left);
 __global__ void fill_kernel(float *__restrict__ x, float value, size_t n);
 
 }  // namespace qlpiv

@@ -93,6 +93,17 @@ static void level_grid(int H, int W, int tile, float of, const int m[4], bool le
     }
 }
 
+void level_origins(int H, int W, int tile, float overlap_factor, const int margins[4], bool first,
+                   int &nI, int &nJ, std::vector<int> &top, std::vector<int> &left)
+{
+    Level L;
+    level_grid(H, W, tile, overlap_factor, margins, first, L);
+    nI = L.nI;
+    nJ = L.nJ;
+    top = std::move(L.top_h);
+    left = std::move(L.left_h);
+}
+
 void grid_origins(int H, int W, int tile, float overlap_factor, const int margins[4], int &nI,
                   int &nJ, std::vector<int> &top, std::vector<int> &left)
 {

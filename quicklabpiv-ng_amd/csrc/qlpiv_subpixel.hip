@@ -427,14 +427,14 @@ __device__ void guo_axis(int iterations, int half, int c, const float *s, float 
 // = n, followed by the displacement update u += i' - D/2, v += j' - D/2 of
 // the multi-pass loop, Tile.accumulateDisplacement :103-136).
 template <bool FROM_NBR, int HM>
-__global__ __launch_bounds__(256) void guo_kernel(int pixels, int iterations, int items, int D,
-                                                  int P, const float *__restrict__ src,
-                                                  int nbr_half, const float *__restrict__ min_floor,
-                                                  const float *__restrict__ peaks_in,
-                                                  float *__restrict__ peaks_out,
-                                                  float *__restrict__ du, float *__restrict__ dv)
+__device__ __forceinline__ void guo_item(int gid, int pixels, int iterations, int items, int D,
+                                         int P, const float *__restrict__ src, int nbr_half,
+                                         const float *__restrict__ min_floor,
+                                         const float *__restrict__ peaks_in,
+                                         float *__restrict__ peaks_out, float *__restrict__ du,
+                                         float *__restrict__ dv,
+                                         const uint8_t *__restrict__ fit_on)
 {
-    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
     const int item = gid >> 1, axis = gid & 1;
     const bool in = item < items;
     const int ia = FROM_NBR ? item : item / (P > 0 ? P : 1);
@@ -448,8 +448,8 @@ __global__ __launch_bounds__(256) void guo_kernel(int pixels, int iterations, in
     const int maxI = (int)pI, maxJ = (int)pJ;
     const int half = pixels / 2;
     const bool skip = (maxI == 0 && maxJ == 0) || maxI - half < 0 || maxI + half >= D ||
-                      maxJ - half < 0 || maxJ + half >= D;
-    const bool low = mf < -5.15f;
+                      maxJ - half < 0 || maxJ + half >= D || (fit_on && in && !fit_on[item]);
+    const bool low = !skip && mf < -5.15f;
     float mean = 0.0f, amp = 0.0f;
     if (in && !skip && !low) {
         const float add = (mf < 0.0f) ? (-mf + 1e-8f) : 0.0f;
@@ -512,6 +512,36 @@ __global__ __launch_bounds__(256) void guo_kernel(int pixels, int iterations, in
     }
 }
 
+//   n_dev  : persistent launch: the item count in device memory (items is then
+//            the capacity); the grid strides over the items
+//   fit_on : per-item flag (or null): 0 keeps the integer peak, as a level
+//            before the interpolator start step
+template <bool FROM_NBR, int HM, bool PERSIST = false>
+__global__ __launch_bounds__(256) void guo_kernel(int pixels, int iterations, int items, int D,
+                                                  int P, const float *__restrict__ src,
+                                                  int nbr_half, const float *__restrict__ min_floor,
+                                                  const float *__restrict__ peaks_in,
+                                                  float *__restrict__ peaks_out,
+                                                  float *__restrict__ du, float *__restrict__ dv,
+                                                  const int *__restrict__ n_dev,
+                                                  const uint8_t *__restrict__ fit_on)
+{
+    if (!PERSIST) {
+        guo_item<FROM_NBR, HM>((int)(blockIdx.x * blockDim.x + threadIdx.x), pixels, iterations,
+                               items, D, P, src, nbr_half, min_floor, peaks_in, peaks_out, du, dv,
+                               fit_on);
+        return;
+    }
+    items = *n_dev;
+    const int stride = (int)(gridDim.x * blockDim.x);  // even: the lane pairs stay aligned
+    for (int base = (int)(blockIdx.x * blockDim.x); base < 2 * items; base += stride) {
+        int tx = (int)threadIdx.x;
+        asm volatile("" : "+v"(tx));  // nothing built on it is hoisted out of the loop
+        guo_item<FROM_NBR, HM>(base + tx, pixels, iterations, items, D, P, src, nbr_half,
+                               min_floor, peaks_in, peaks_out, du, dv, fit_on);
+    }
+}
+
 // Fits for every (IA, peak) of planes stored in HBM.
 __global__ __launch_bounds__(256) void subpixel_kernel(int kind, int pixels, int iterations, int n,
                                                       int D, int P, const float *__restrict__ planes,
@@ -550,17 +580,16 @@ __global__ __launch_bounds__(256) void subpixel_kernel(int kind, int pixels, int
 // displacement update of the multi-pass loop: u += i' - D/2, v += j' - D/2
 // (MaxCrossResult.getNthDisplacementU :141-149, Tile.accumulateDisplacement
 // :103-136 with out-of-bound clipping allowed).
-__global__ __launch_bounds__(256) void subpixel_nbr_kernel(int kind, int pixels, int iterations,
+__device__ __forceinline__ void subpixel_nbr_item(int ia, int kind, int pixels, int iterations,
                                                           int n, int D, int nbr_half,
                                                           const float *__restrict__ nbr,
                                                           const float *__restrict__ min_floor,
                                                           const float *__restrict__ peaks_in,
                                                           float *__restrict__ peaks_out,
                                                           float *__restrict__ du,
-                                                          float *__restrict__ dv)
+                                                          float *__restrict__ dv,
+                                                          const uint8_t *__restrict__ fit_on)
 {
-    const int ia = blockIdx.x * blockDim.x + threadIdx.x;
-    if (ia >= n) return;
     const int L = 2 * nbr_half + 1;
     const float *nb = nbr + (size_t)ia * 2 * L;
     float col[2 * NBR_MAX_HALF + 1], row[2 * NBR_MAX_HALF + 1];
@@ -570,7 +599,7 @@ __global__ __launch_bounds__(256) void subpixel_nbr_kernel(int kind, int pixels,
     }
     float res[4];
     const float pI = peaks_in[(size_t)ia * 3 + 0], pJ = peaks_in[(size_t)ia * 3 + 1];
-    if (kind < 0) {  // no interpolator: the integer peak
+    if (kind < 0 || (fit_on && !fit_on[ia])) {  // no interpolator: the integer peak
         res[0] = pI;
         res[1] = pJ;
         res[2] = peaks_in[(size_t)ia * 3 + 2];
@@ -590,6 +619,34 @@ __global__ __launch_bounds__(256) void subpixel_nbr_kernel(int kind, int pixels,
         const int c = D / 2;
         du[ia] = du[ia] + (res[0] - (float)c);
         dv[ia] = dv[ia] + (res[1] - (float)c);
+    }}
+
+template <bool PERSIST>
+__global__ __launch_bounds__(256) void subpixel_nbr_kernel(int kind, int pixels, int iterations,
+                                                          int n, int D, int nbr_half,
+                                                          const float *__restrict__ nbr,
+                                                          const float *__restrict__ min_floor,
+                                                          const float *__restrict__ peaks_in,
+                                                          float *__restrict__ peaks_out,
+                                                          float *__restrict__ du,
+                                                          float *__restrict__ dv,
+                                                          const int *__restrict__ n_dev,
+                                                          const uint8_t *__restrict__ fit_on)
+{
+    if (!PERSIST) {
+        const int ia = blockIdx.x * blockDim.x + threadIdx.x;
+        if (ia < n)
+            subpixel_nbr_item(ia, kind, pixels, iterations, n, D, nbr_half, nbr, min_floor,
+                              peaks_in, peaks_out, du, dv, fit_on);
+        return;
+    }
+    n = *n_dev;  // persistent launch: the grid strides over the IAs
+    for (int base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
+        int tx = (int)threadIdx.x;
+        asm volatile("" : "+v"(tx));  // nothing built on it is hoisted out of the loop
+        if (base + tx < n)
+            subpixel_nbr_item(base + tx, kind, pixels, iterations, n, D, nbr_half, nbr, min_floor,
+                              peaks_in, peaks_out, du, dv, fit_on);
     }
 }
 
@@ -619,7 +676,7 @@ int launch_subpixel(Context *ctx, int kind, int pixels, int iterations, int n, i
                : pixels == 7 ? guo_kernel<false, 3> : guo_kernel<false, NBR_MAX_HALF>;
         hipLaunchKernelGGL(k, dim3((2 * total + 255) / 256), dim3(256), 0, ctx->stream, pixels,
                            iterations, total, D, n_peaks, planes, 0, min_floor, peaks_in,
-                           peaks_out, nullptr, nullptr);
+                           peaks_out, nullptr, nullptr, nullptr, nullptr);
     } else
         hipLaunchKernelGGL(subpixel_kernel, dim3((total + 255) / 256), dim3(256), 0, ctx->stream,
                            kind, pixels, iterations, n, D, n_peaks, planes, min_floor, peaks_in,
@@ -629,9 +686,12 @@ int launch_subpixel(Context *ctx, int kind, int pixels, int iterations, int n, i
 
 int launch_subpixel_nbr(Context *ctx, int kind, int pixels, int iterations, int n, int D,
                         int nbr_half, const float *nbr, const float *min_floor,
-                        const float *peaks_in, float *peaks_out, float *du, float *dv)
+                        const float *peaks_in, float *peaks_out, float *du, float *dv,
+                        const int *n_dev, const uint8_t *fit_on)
 {
     if (n == 0) return QLP_OK;
+    // a persistent launch (n = the capacity) takes at most 8,192 work-groups
+    if (n_dev && n > 8192 * 128) n = 8192 * 128;
     if (kind >= 0) {
         int rc = check_subpixel_args(kind, pixels, iterations);
         if (rc) return rc;
@@ -639,15 +699,19 @@ int launch_subpixel_nbr(Context *ctx, int kind, int pixels, int iterations, int 
         if (half > nbr_half) return fail(QLP_ERR_ARG, "subpixel: neighbourhood too small");
     }
     if (kind == QLP_SUBPIX_GUO) {
-        auto k = pixels == 3 ? guo_kernel<true, 1> : pixels == 5 ? guo_kernel<true, 2>
-               : pixels == 7 ? guo_kernel<true, 3> : guo_kernel<true, NBR_MAX_HALF>;
+        auto k = n_dev ? (pixels == 3 ? guo_kernel<true, 1, true> : pixels == 5 ? guo_kernel<true, 2, true>
+                          : pixels == 7 ? guo_kernel<true, 3, true> : guo_kernel<true, NBR_MAX_HALF, true>)
+                       : (pixels == 3 ? guo_kernel<true, 1> : pixels == 5 ? guo_kernel<true, 2>
+                          : pixels == 7 ? guo_kernel<true, 3> : guo_kernel<true, NBR_MAX_HALF>);
         hipLaunchKernelGGL(k, dim3((2 * n + 255) / 256), dim3(256), 0, ctx->stream, pixels,
                            iterations, n, D, 1, nbr, nbr_half, min_floor, peaks_in, peaks_out, du,
-                           dv);
-    } else
-        hipLaunchKernelGGL(subpixel_nbr_kernel, dim3((n + 255) / 256), dim3(256), 0, ctx->stream,
-                           kind, pixels, iterations, n, D, nbr_half, nbr, min_floor, peaks_in,
-                           peaks_out, du, dv);
+                           dv, n_dev, fit_on);
+    } else {
+        auto k = n_dev ? subpixel_nbr_kernel<true> : subpixel_nbr_kernel<false>;
+        hipLaunchKernelGGL(k, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, kind, pixels,
+                           iterations, n, D, nbr_half, nbr, min_floor, peaks_in, peaks_out, du, dv,
+                           n_dev, fit_on);
+    }
     return check_hip(hipGetLastError(), "subpixel kernel launch");
 }
 

@@ -48,6 +48,17 @@ struct XcShape {
 
 __device__ __forceinline__ int bitrev_n(int x, int bits) { return (int)(__brev((unsigned)x) >> (32 - bits)); }
 
+// The thread index, hidden from the optimiser: address arithmetic built on it
+// is then recomputed in each iteration of a loop over IAs (or line batches)
+// instead of being hoisted out of the loop into registers for every unrolled
+// load.  The helpers below take it as 'tid' (-1: threadIdx.x).
+__device__ __forceinline__ int opaque_tid()
+{
+    int t = threadIdx.x;
+    asm volatile("" : "+v"(t));
+    return t;
+}
+
 // The reference butterfly: lo = a + w*b, hi = a - w*b with w = c + i*s, both
 // results computed from the inputs before either is stored.
 __device__ __forceinline__ void bfly(float &ar, float &ai, float &br, float &bi, float2 w)
@@ -73,8 +84,10 @@ __device__ __forceinline__ void bfly(float &ar, float &ai, float &br, float &bi,
 template <int LOGM, bool ZERO_HALF, bool PRESCALE, int NT = XcShape<LOGM>::THREADS,
           int MAXLINES = XcShape<LOGM>::M>
 __device__ __forceinline__ void fft_lines(float *__restrict__ re, float *__restrict__ im, int LS,
-                                          int PS, int nlines, const float2 *__restrict__ tw)
+                                          int PS, int nlines, const float2 *__restrict__ tw,
+                                          int tid = -1)
 {
+    const int tx = tid < 0 ? (int)threadIdx.x : tid;
     using X = XcShape<LOGM>;
     constexpr int M = X::M, E = X::E, LOGE = X::LOGE, T = X::T, B = LOGM - LOGE;
     constexpr int ROUNDS = (MAXLINES * T + NT - 1) / NT;
@@ -83,13 +96,13 @@ __device__ __forceinline__ void fft_lines(float *__restrict__ re, float *__restr
 
     // pass 1 gather: thread (l, t) takes positions p = t*E + q, which after the
     // perfect shuffle hold the original indices bitrev(p)
-#pragma unroll
+    #pragma unroll
     for (int rd = 0; rd < ROUNDS; rd++) {
-        const int task = threadIdx.x + rd * NT;
+        const int task = tx + rd * NT;
         if (task < ntasks) {
             const int l = task % nlines, t = task / nlines;
             const int base = l * LS;
-#pragma unroll
+            #pragma unroll
             for (int q = 0; q < E; q += (ZERO_HALF ? 2 : 1)) {
                 const int o = bitrev_n(t * E + q, LOGM);
                 rr[rd][q] = re[base + o * PS];
@@ -98,16 +111,16 @@ __device__ __forceinline__ void fft_lines(float *__restrict__ re, float *__restr
         }
     }
     __syncthreads();
-#pragma unroll
+    #pragma unroll
     for (int rd = 0; rd < ROUNDS; rd++) {
-        const int task = threadIdx.x + rd * NT;
+        const int task = tx + rd * NT;
         if (task < ntasks) {
             const int l = task % nlines, t = task / nlines;
             const int base = l * LS;
             float *a = rr[rd], *b = ri[rd];
             if (PRESCALE) {
                 const float fM = (float)M;
-#pragma unroll
+                #pragma unroll
                 for (int q = 0; q < E; q++) {
                     a[q] = a[q] / fM;
                     b[q] = b[q] / fM;
@@ -116,18 +129,18 @@ __device__ __forceinline__ void fft_lines(float *__restrict__ re, float *__restr
             int s0 = 0;
             if (ZERO_HALF) {
                 // stage 0 pairs (2u, 2u+1); the odd partner is zero padding
-#pragma unroll
+                #pragma unroll
                 for (int q = 0; q < E; q += 2) {
                     a[q + 1] = a[q];
                     b[q + 1] = b[q];
                 }
                 s0 = 1;
             }
-#pragma unroll
+            #pragma unroll
             for (int s = 0; s < LOGE; s++) {
                 if (s < s0) continue;
                 const int L = 1 << s;
-#pragma unroll
+                #pragma unroll
                 for (int q = 0; q < E; q++) {
                     if (q & L) continue;
                     const int k = (M >> (s + 1)) * (q & (L - 1));
@@ -144,7 +157,7 @@ __device__ __forceinline__ void fft_lines(float *__restrict__ re, float *__restr
                     }
                 }
             }
-#pragma unroll
+            #pragma unroll
             for (int q = 0; q < E; q++) {
                 const int p = t * E + q;
                 re[base + p * PS] = a[q];
@@ -154,27 +167,27 @@ __device__ __forceinline__ void fft_lines(float *__restrict__ re, float *__restr
     }
     __syncthreads();
     // pass 2: positions p = t + (q << B), stages LOGE..LOGM-1, in place
-    for (int task = threadIdx.x; task < ntasks; task += NT) {
+    for (int task = tx; task < ntasks; task += NT) {
         const int l = task % nlines, t = task / nlines;
         const int base = l * LS;
         float a[E], b[E];
-#pragma unroll
+        #pragma unroll
         for (int q = 0; q < E; q++) {
             const int p = t + (q << B);
             a[q] = re[base + p * PS];
             b[q] = im[base + p * PS];
         }
-#pragma unroll
+        #pragma unroll
         for (int s = LOGE; s < LOGM; s++) {
             const int Lq = 1 << (s - B);
-#pragma unroll
+            #pragma unroll
             for (int q = 0; q < E; q++) {
                 if (q & Lq) continue;
                 const int k = (M >> (s + 1)) * (t + ((q & (Lq - 1)) << B));
                 bfly(a[q], b[q], a[q + Lq], b[q + Lq], tw[k]);
             }
         }
-#pragma unroll
+        #pragma unroll
         for (int q = 0; q < E; q++) {
             const int p = t + (q << B);
             re[base + p * PS] = a[q];
@@ -191,10 +204,11 @@ __device__ __forceinline__ void fft_lines(float *__restrict__ re, float *__restr
 // P[-k] = conj(P[k]) holds bit for bit whichever element owns the pair.
 // Row stride S; NT threads.
 template <int LOGM, bool MODE_CPU, int NT = XcShape<LOGM>::THREADS, int S = XcShape<LOGM>::S>
-__device__ __forceinline__ void unpack_product(float *__restrict__ re, float *__restrict__ im)
+__device__ __forceinline__ void unpack_product(float *__restrict__ re, float *__restrict__ im,
+                                               int tid = -1)
 {
     constexpr int M = XcShape<LOGM>::M;
-    for (int idx = threadIdx.x; idx < M * M; idx += NT) {
+    for (int idx = tid < 0 ? (int)threadIdx.x : tid; idx < M * M; idx += NT) {
         const int r = idx >> LOGM, c = idx & (M - 1);
         const int nr = (M - r) & (M - 1), nc = (M - c) & (M - 1);
         if (idx > nr * M + nc) continue;
@@ -220,14 +234,15 @@ __device__ __forceinline__ void unpack_product(float *__restrict__ re, float *__
 
 // Block-wide maximum of an int (tile maxima).
 template <int THREADS>
-__device__ __forceinline__ int block_max_int(int v, int *s_red)
+__device__ __forceinline__ int block_max_int(int v, int *s_red, int tid = -1)
 {
-#pragma unroll
+    const int tx = tid < 0 ? (int)threadIdx.x : tid;
+    #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
-    if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = v;
+    if ((tx & 63) == 0) s_red[tx >> 6] = v;
     __syncthreads();
     int m = s_red[0];
-#pragma unroll
+    #pragma unroll
     for (int w = 1; w < THREADS / 64; w++) m = max(m, s_red[w]);
     __syncthreads();
     return m;
@@ -240,7 +255,10 @@ __device__ __forceinline__ int block_max_int(int v, int *s_red)
 //                maxima are computed here (>= 16 clamp; 255 for a warped B)
 //   PEAK       : FromCenter on the plane while it is still in LDS
 //   NT_        : work-group size (0: the default of XcShape)
-template <int LOGM, bool MODE_CPU, bool FROM_IMAGE, bool PEAK, int NT_ = 0>
+//   PERSIST    : persistent launch: the IA count in device memory (p.n_dev),
+//                every work-group loops over IAs; otherwise one IA per
+//                work-group in straight-line code
+template <int LOGM, bool MODE_CPU, bool FROM_IMAGE, bool PEAK, int NT_ = 0, bool PERSIST = false>
 __global__ __launch_bounds__((XcShape<LOGM, NT_>::THREADS)) void xcorr_pass_kernel(
     PassParams p, const float2 *__restrict__ tw_fwd, const float2 *__restrict__ tw_inv)
 {
@@ -255,120 +273,160 @@ __global__ __launch_bounds__((XcShape<LOGM, NT_>::THREADS)) void xcorr_pass_kern
     __shared__ int s_cell[NT / 64];
     __shared__ float s_min[NT / 64];
 
-    const int ia = blockIdx.x;
-    if (ia >= p.n) return;
+    const int n = PERSIST ? *p.n_dev : p.n;
+    if ((int)blockIdx.x >= n) return;
     for (int k = threadIdx.x; k < M / 2; k += NT) {
         s_twf[k] = tw_fwd[k];
         s_twi[k] = tw_inv[k];
     }
-
-    // ---- tile formation + normalisation: v / max * 16 + 1, F mirrored -------
-    // (Matrix.copyMirroredMatrixToArrayNormalizeAndOffset :282-288,
-    //  copyMatrixToArrayAndNormalizeAndOffset :175-181; only the N x N data
-    //  block is written, the transforms never read the zero padding)
-    float va[PIX], vb[PIX];
-    float mA, mB;
-    if (FROM_IMAGE) {
-        const int per_pair = p.per_pair > 0 ? p.per_pair : p.n;
-        const int pair = ia / per_pair, t = ia - pair * per_pair;
-        const size_t img_off = (size_t)pair * p.H * p.W;
-        const uint8_t *__restrict__ imA = p.imgA + img_off;
-        const uint8_t *__restrict__ imB = p.imgB + img_off;
-        const int y0 = p.top[t], x0 = p.left[t];
-        int by0 = y0, bx0 = x0;
-        float fu = 0.0f, fv = 0.0f;
-        const bool warp = (p.bmode == B_WARP_MICRO) && p.du != nullptr;
-        if (p.du != nullptr) {
-            const float u = p.du[ia], v = p.dv[ia];  // per IA of the batch
-            if (p.bmode == B_CLIP_SHIFT) {
-                by0 = fm_round((float)y0 + u);
-                bx0 = fm_round((float)x0 + v);
-            } else if (warp) {
-                const float flu = floorf(u), flv = floorf(v);
-                by0 = (short)(y0 + (int)flu);
-                bx0 = (short)(x0 + (int)flv);
-                fu = u - flu;  // createMicroDisplacements :195-196
-                fv = v - flv;
-            }
-        }
-        int amax = 0, bmax = 0;
-#pragma unroll
+    // persistent launch: this work-group's first tiles, loaded ahead
+    float pa[PERSIST ? PIX : 1], pb[PERSIST ? PIX : 1], pmA = 0.0f, pmB = 0.0f;
+    if constexpr (PERSIST && !FROM_IMAGE) {
+        const int ia0 = blockIdx.x;
+        pmA = p.maxA[ia0];
+        pmB = MODE_CPU ? pmA : p.maxB[ia0];
+        const size_t at = (size_t)ia0 * N * N;
+        #pragma unroll
         for (int k = 0; k < PIX; k++) {
-            const int idx = threadIdx.x + k * NT;
-            const int r = idx / N, c = idx - (idx / N) * N;
-            int a = 0, b = 0;
-            const int ya = y0 + r, xa = x0 + c;
-            if (ya >= 0 && ya < p.H && xa >= 0 && xa < p.W) a = imA[(size_t)ya * p.W + xa];
-            if (warp) {
-                b = (int)warp_pixel(imB, p.H, p.W, fu, fv, by0 + r, bx0 + c);
-            } else {
-                const int yb = by0 + r, xb = bx0 + c;
-                if (yb >= 0 && yb < p.H && xb >= 0 && xb < p.W) {
-                    b = imB[(size_t)yb * p.W + xb];
-                    bmax = max(bmax, b);
+            pa[k] = (float)p.tA[at + threadIdx.x + k * NT];
+            pb[k] = (float)p.tB[at + threadIdx.x + k * NT];
+        }
+    }
+    // one IA per work-group, or (persistent launch) every gridDim.x-th IA
+    for (int ia = blockIdx.x; ia < n; ia += gridDim.x) {
+        const int tid = PERSIST ? opaque_tid() : (int)threadIdx.x;
+        // ---- tile formation + normalisation: v / max * 16 + 1, F mirrored -------
+        // (Matrix.copyMirroredMatrixToArrayNormalizeAndOffset :282-288,
+        //  copyMatrixToArrayAndNormalizeAndOffset :175-181; only the N x N data
+        //  block is written, the transforms never read the zero padding)
+        float va[PIX], vb[PIX];
+        float mA, mB;
+        if (FROM_IMAGE) {
+            const int per_pair = p.per_pair > 0 ? p.per_pair : n;
+            const int pair = ia / per_pair, t = ia - pair * per_pair;
+            const size_t img_off = (size_t)pair * p.H * p.W;
+            const uint8_t *__restrict__ imA = p.imgA + img_off;
+            const uint8_t *__restrict__ imB = p.imgB + img_off;
+            const int y0 = p.top[t], x0 = p.left[t];
+            int by0 = y0, bx0 = x0;
+            float fu = 0.0f, fv = 0.0f;
+            const bool warp = (p.bmode == B_WARP_MICRO) && p.du != nullptr;
+            if (p.du != nullptr) {
+                const float u = p.du[ia], v = p.dv[ia];  // per IA of the batch
+                if (p.bmode == B_CLIP_SHIFT) {
+                    by0 = fm_round((float)y0 + u);
+                    bx0 = fm_round((float)x0 + v);
+                } else if (warp) {
+                    const float flu = floorf(u), flv = floorf(v);
+                    by0 = (short)(y0 + (int)flu);
+                    bx0 = (short)(x0 + (int)flv);
+                    fu = u - flu;  // createMicroDisplacements :195-196
+                    fv = v - flv;
                 }
             }
-            amax = max(amax, a);
-            va[k] = (float)a;
-            vb[k] = (float)b;
-        }
-        amax = block_max_int<NT>(amax, s_red);
-        mA = (float)(amax < 16 ? 16 : amax);
-        if (warp || p.bmax255) {
-            mB = 255.0f;  // warped tiles carry the image maximum (doBiLinearWarp :298)
+            int amax = 0, bmax = 0;
+            #pragma unroll
+            for (int k = 0; k < PIX; k++) {
+                const int idx = tid + k * NT;
+                const int r = idx / N, c = idx - (idx / N) * N;
+                int a = 0, b = 0;
+                const int ya = y0 + r, xa = x0 + c;
+                if (ya >= 0 && ya < p.H && xa >= 0 && xa < p.W) a = imA[(size_t)ya * p.W + xa];
+                if (warp) {
+                    b = (int)warp_pixel(imB, p.H, p.W, fu, fv, by0 + r, bx0 + c);
+                } else {
+                    const int yb = by0 + r, xb = bx0 + c;
+                    if (yb >= 0 && yb < p.H && xb >= 0 && xb < p.W) {
+                        b = imB[(size_t)yb * p.W + xb];
+                        bmax = max(bmax, b);
+                    }
+                }
+                amax = max(amax, a);
+                va[k] = (float)a;
+                vb[k] = (float)b;
+            }
+            amax = block_max_int<NT>(amax, s_red, tid);
+            mA = (float)(amax < 16 ? 16 : amax);
+            if (warp || p.bmax255) {
+                mB = 255.0f;  // warped tiles carry the image maximum (doBiLinearWarp :298)
+            } else {
+                bmax = block_max_int<NT>(bmax, s_red, tid);
+                mB = (float)(bmax < 16 ? 16 : bmax);
+            }
+            if (MODE_CPU) mB = mA;
+        } else if (PERSIST) {
+            // the tiles were loaded one IA ahead; load the next IA's now, so that
+            // its loads are in flight during this IA's transforms
+            mA = pmA;
+            mB = MODE_CPU ? mA : pmB;
+            #pragma unroll
+            for (int k = 0; k < PIX; k++) {
+                va[k] = pa[k];
+                vb[k] = pb[k];
+            }
+            const int nx = ia + (int)gridDim.x;
+            if (nx < n) {
+                pmA = p.maxA[nx];
+                pmB = MODE_CPU ? pmA : p.maxB[nx];
+                const size_t at = (size_t)nx * N * N;
+                #pragma unroll
+                for (int k = 0; k < PIX; k++) {
+                    const int idx = tid + k * NT;
+                    pa[k] = (float)p.tA[at + idx];
+                    pb[k] = (float)p.tB[at + idx];
+                }
+            }
         } else {
-            bmax = block_max_int<NT>(bmax, s_red);
-            mB = (float)(bmax < 16 ? 16 : bmax);
+            mA = p.maxA[ia];
+            mB = MODE_CPU ? mA : p.maxB[ia];
+            // 8-bit tiles (MatrixByte) or float tiles (MatrixFloat)
+            const size_t at = (size_t)ia * N * N;
+            #pragma unroll
+            for (int k = 0; k < PIX; k++) {
+                const int idx = tid + k * NT;
+                va[k] = p.tAf ? p.tAf[at + idx] : (float)p.tA[at + idx];
+                vb[k] = p.tBf ? p.tBf[at + idx] : (float)p.tB[at + idx];
+            }
         }
-        if (MODE_CPU) mB = mA;
-    } else {
-        mA = p.maxA[ia];
-        mB = MODE_CPU ? mA : p.maxB[ia];
-        // 8-bit tiles (MatrixByte) or float tiles (MatrixFloat)
-        const size_t at = (size_t)ia * N * N;
-#pragma unroll
+        #pragma unroll
         for (int k = 0; k < PIX; k++) {
-            const int idx = threadIdx.x + k * NT;
-            va[k] = p.tAf ? p.tAf[at + idx] : (float)p.tA[at + idx];
-            vb[k] = p.tBf ? p.tBf[at + idx] : (float)p.tB[at + idx];
+            const int idx = tid + k * NT;
+            const int mi = N * N - 1 - idx;  // F'[r][c] = A[N-1-r][N-1-c]
+            s_re[(mi / N) * S + (mi % N)] = va[k] / mA * 16.0f + 1.0f;
+            s_im[(idx / N) * S + (idx % N)] = vb[k] / mB * 16.0f + 1.0f;
         }
-    }
-#pragma unroll
-    for (int k = 0; k < PIX; k++) {
-        const int idx = threadIdx.x + k * NT;
-        const int mi = N * N - 1 - idx;  // F'[r][c] = A[N-1-r][N-1-c]
-        s_re[(mi / N) * S + (mi % N)] = va[k] / mA * 16.0f + 1.0f;
-        s_im[(idx / N) * S + (idx % N)] = vb[k] / mB * 16.0f + 1.0f;
-    }
-    __syncthreads();
+        __syncthreads();
 
-    // ---- transforms ----------------------------------------------------------
-    // forward: rows (only the N data rows are non-zero), then all columns
-    fft_lines<LOGM, true, false, NT>(s_re, s_im, S, 1, N, s_twf);
-    fft_lines<LOGM, true, false, NT>(s_re, s_im, 1, S, M, s_twf);
-    unpack_product<LOGM, MODE_CPU, NT>(s_re, s_im);
-    // inverse: all rows, then the D columns that reach the output
-    fft_lines<LOGM, false, MODE_CPU, NT>(s_re, s_im, S, 1, M, s_twi);
-    fft_lines<LOGM, false, MODE_CPU, NT>(s_re, s_im, 1, S, D, s_twi);
+        // ---- transforms ----------------------------------------------------------
+        // forward: rows (only the N data rows are non-zero), then all columns
+        fft_lines<LOGM, true, false, NT>(s_re, s_im, S, 1, N, s_twf, tid);
+        fft_lines<LOGM, true, false, NT>(s_re, s_im, 1, S, M, s_twf, tid);
+        unpack_product<LOGM, MODE_CPU, NT>(s_re, s_im, tid);
+        // inverse: all rows, then the D columns that reach the output
+        fft_lines<LOGM, false, MODE_CPU, NT>(s_re, s_im, S, 1, M, s_twi, tid);
+        fft_lines<LOGM, false, MODE_CPU, NT>(s_re, s_im, 1, S, D, s_twi, tid);
 
-    // ---- epilogues -------------------------------------------------------------
-    if (p.planes) {
-        float *out = p.planes + (size_t)ia * D * D;
-        for (int idx = threadIdx.x; idx < D * D; idx += NT) {
-            const int i = idx / D, j = idx - (idx / D) * D;
-            out[idx] = s_re[i * S + j];
+        // ---- epilogues -------------------------------------------------------------
+        if (p.planes) {
+            float *out = p.planes + (size_t)ia * D * D;
+            for (int idx = tid; idx < D * D; idx += NT) {
+                const int i = idx / D, j = idx - (idx / D) * D;
+                out[idx] = s_re[i * S + j];
+            }
         }
-    }
-    if (PEAK) {
-        constexpr int h = D / 2;
-        uint64_t best = 0;
-        int bestCell = 0;
-        float mn = 3.40282347e+38f;
-        for (int idx = threadIdx.x; idx < D * D; idx += NT) {
-            const int i = idx / D, j = idx - (idx / D) * D;
-            from_center_visit(s_re[i * S + j], i, j, h, idx, best, bestCell, mn);
+        if (PEAK) {
+            constexpr int h = D / 2;
+            uint64_t best = 0;
+            int bestCell = 0;
+            float mn = 3.40282347e+38f;
+            for (int idx = tid; idx < D * D; idx += NT) {
+                const int i = idx / D, j = idx - (idx / D) * D;
+                from_center_visit(s_re[i * S + j], i, j, h, idx, best, bestCell, mn);
+            }
+            peak_epilogue<NT, D>(p, ia, best, bestCell, mn, s_re, S, 1, s_key, s_cell, s_min, tid);
         }
-        peak_epilogue<NT, D>(p, ia, best, bestCell, mn, s_re, S, 1, s_key, s_cell, s_min);
+        if (!PERSIST) break;
+        __syncthreads();  // the next IA reuses the LDS tile
     }
 }
 
@@ -415,16 +473,6 @@ __device__ __forceinline__ int paired_line(int k, int l)
     return l < H ? l + H : l - H;
 }
 
-// The thread index, hidden from the optimiser: address arithmetic built on it
-// is then recomputed in each batch instead of being hoisted out of the IA loop
-// into registers for every unrolled load.
-__device__ __forceinline__ int opaque_tid()
-{
-    int t = threadIdx.x;
-    asm volatile("" : "+v"(t));
-    return t;
-}
-
 // NT threads, batches of BL lines (two shapes: 512 x 32 and 256 x 16).
 template <bool MODE_CPU, bool FROM_IMAGE, bool PEAK, int NT, int BL>
 __global__ __launch_bounds__(NT) void xcorr_big_kernel(PassParams p,
@@ -453,7 +501,8 @@ __global__ __launch_bounds__(NT) void xcorr_big_kernel(PassParams p,
     }
     float2 *__restrict__ ws = work + (size_t)blockIdx.x * (SLOT_BYTES / sizeof(float2));
 
-    for (int ia = blockIdx.x; ia < p.n; ia += gridDim.x) {
+    const int n = pass_count(p);
+    for (int ia = blockIdx.x; ia < n; ia += gridDim.x) {
         // ---- pixel sources (as in the in-LDS kernel) and tile maxima ----------
         const uint8_t *a8 = nullptr, *b8 = nullptr;
         int y0 = 0, x0 = 0, by0 = 0, bx0 = 0;
@@ -461,7 +510,7 @@ __global__ __launch_bounds__(NT) void xcorr_big_kernel(PassParams p,
         bool warp = false;
         float mA, mB;
         if (FROM_IMAGE) {
-            const int per_pair = p.per_pair > 0 ? p.per_pair : p.n;
+            const int per_pair = p.per_pair > 0 ? p.per_pair : n;
             const int pair = ia / per_pair, t = ia - pair * per_pair;
             const size_t img_off = (size_t)pair * p.H * p.W;
             a8 = p.imgA + img_off;
@@ -530,14 +579,14 @@ __global__ __launch_bounds__(NT) void xcorr_big_kernel(PassParams p,
         for (int rb = 0; rb < N; rb += BL) {
             const int tid = opaque_tid();
             float va[IT_HALF], vb[IT_HALF];
-#pragma unroll
+            #pragma unroll
             for (int q = 0; q < IT_HALF; q++) {
                 const int idx = tid + q * NT;
                 const int l = idx / N, c = idx - (idx / N) * N, r = rb + l;
                 va[q] = pixA(N - 1 - r, N - 1 - c);
                 vb[q] = pixB(r, c);
             }
-#pragma unroll
+            #pragma unroll
             for (int q = 0; q < IT_HALF; q++) {
                 const int idx = tid + q * NT;
                 const int l = idx / N, c = idx - (idx / N) * N;
@@ -556,13 +605,13 @@ __global__ __launch_bounds__(NT) void xcorr_big_kernel(PassParams p,
         for (int k = 0; k < NB; k++) {
             const int tid = opaque_tid();
             float2 x[IT_HALF];
-#pragma unroll
+            #pragma unroll
             for (int q = 0; q < IT_HALF; q++) {
                 const int idx = tid + q * NT;
                 const int r = idx / BL, l = idx - (idx / BL) * BL;
                 x[q] = ws[(size_t)r * M + paired_index<BL>(k, l)];
             }
-#pragma unroll
+            #pragma unroll
             for (int q = 0; q < IT_HALF; q++) {
                 const int idx = tid + q * NT;
                 const int r = idx / BL, l = idx - (idx / BL) * BL;
@@ -605,7 +654,7 @@ __global__ __launch_bounds__(NT) void xcorr_big_kernel(PassParams p,
         for (int k = 0; k < NB; k++) {
             const int tid = opaque_tid();
             float2 x[IT_FULL];
-#pragma unroll
+            #pragma unroll
             for (int q = 0; q < IT_FULL; q++) {
                 const int idx = tid + q * NT;
                 const int l = idx / M, c = idx - (idx / M) * M;
@@ -614,7 +663,7 @@ __global__ __launch_bounds__(NT) void xcorr_big_kernel(PassParams p,
                 x[q] = ws[mirror ? (size_t)(M - r) * M + ((M - c) & (M - 1)) : (size_t)r * M + c];
                 if (mirror) x[q].y = -x[q].y;
             }
-#pragma unroll
+            #pragma unroll
             for (int q = 0; q < IT_FULL; q++) {
                 const int idx = tid + q * NT;
                 const int l = idx / M, c = idx - (idx / M) * M;
@@ -638,13 +687,13 @@ __global__ __launch_bounds__(NT) void xcorr_big_kernel(PassParams p,
             const int nl = min(BL, D - cb);
             const int tid = opaque_tid();
             float2 x[IT_FULL];
-#pragma unroll
+            #pragma unroll
             for (int q = 0; q < IT_FULL; q++) {
                 const int idx = tid + q * NT;
                 const int r = idx / BL, l = idx - (idx / BL) * BL;
                 if (l < nl) x[q] = ws[(size_t)r * M + cb + l];
             }
-#pragma unroll
+            #pragma unroll
             for (int q = 0; q < IT_FULL; q++) {
                 const int idx = tid + q * NT;
                 const int r = idx / BL, l = idx - (idx / BL) * BL;
@@ -678,18 +727,6 @@ __global__ __launch_bounds__(NT) void xcorr_big_kernel(PassParams p,
         }
         __syncthreads();  // the slot is rewritten by the next IA
     }
-}
-
-// Work-groups of one kernel that the device holds at once (its workspace slots).
-template <typename K>
-static int resident_groups(K kernel, int threads)
-{
-    int per_cu = 0, dev = 0, cus = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0) != hipSuccess ||
-        hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
-        return 256;
-    return (per_cu < 1 ? 1 : per_cu) * cus;
 }
 
 // need != null: only report the workspace bytes of this pass.
@@ -745,6 +782,33 @@ int prepare_big(Context *ctx, int n, int mode, bool from_image, bool peak, int s
     return ensure_bigwork(ctx, big_work_need(n, mode, from_image, peak, shape));
 }
 
+// One launch of xcorr_pass_kernel: a work-group per IA, or for a persistent
+// launch the resident work-groups.
+template <int LOGM, bool MODE_CPU, bool FROM_IMAGE, bool PEAK, int NT>
+static void launch_pass_k(Context *ctx, const PassParams &p, const float2 *twf, const float2 *twi)
+{
+    if constexpr (!FROM_IMAGE && PEAK) {
+        if (p.n_dev) {  // persistent: the step engine's passes (tiles, fused peak)
+            auto k = xcorr_pass_kernel<LOGM, MODE_CPU, false, true, NT, true>;
+            hipLaunchKernelGGL(k, dim3(pass_grid(p, 1, resident_groups(k, NT))), dim3(NT), 0,
+                               ctx->stream, p, twf, twi);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((xcorr_pass_kernel<LOGM, MODE_CPU, FROM_IMAGE, PEAK, NT>), dim3(p.n),
+                       dim3(NT), 0, ctx->stream, p, twf, twi);
+}
+
+template <int LOGM, bool MODE_CPU, bool FROM_IMAGE, int NT>
+static void launch_pass_fp(Context *ctx, const PassParams &p, bool peak, const float2 *twf,
+                           const float2 *twi)
+{
+    if (peak)
+        launch_pass_k<LOGM, MODE_CPU, FROM_IMAGE, true, NT>(ctx, p, twf, twi);
+    else
+        launch_pass_k<LOGM, MODE_CPU, FROM_IMAGE, false, NT>(ctx, p, twf, twi);
+}
+
 template <int LOGM, bool MODE_CPU>
 static void launch_pass_mc(Context *ctx, const PassParams &p, bool from_image, bool peak)
 {
@@ -752,28 +816,14 @@ static void launch_pass_mc(Context *ctx, const PassParams &p, bool from_image, b
     const int li = LOGM - 5;
     const float2 *twf = ctx->tables.tw[li][TW_FWD];
     const float2 *twi = ctx->tables.tw[li][MODE_CPU ? TW_INV_CPU : TW_INV_GPU];
-    dim3 grid(p.n), block(X::THREADS);
     // from tiles, the default shape is the wide one too (the C3 batch tuner measured
     // it faster at 32 and 64 px: 2.17 vs 2.31 ms and 3.21 vs 3.95 ms per class)
-    if (!from_image && (p.shape == SHAPE_LDS_WIDE ||
-                        (p.shape == SHAPE_AUTO && LOGM >= 6))) {
-        constexpr int T2 = X::THREADS * 2;
-        if (peak)
-            hipLaunchKernelGGL((xcorr_pass_kernel<LOGM, MODE_CPU, false, true, T2>), grid, dim3(T2),
-                               0, ctx->stream, p, twf, twi);
-        else
-            hipLaunchKernelGGL((xcorr_pass_kernel<LOGM, MODE_CPU, false, false, T2>), grid, dim3(T2),
-                               0, ctx->stream, p, twf, twi);
+    if (!from_image && (p.shape == SHAPE_LDS_WIDE || (p.shape == SHAPE_AUTO && LOGM >= 6))) {
+        launch_pass_fp<LOGM, MODE_CPU, false, X::THREADS * 2>(ctx, p, peak, twf, twi);
         return;
     }
     if (p.shape == SHAPE_LDS_HALF && !from_image) {
-        constexpr int H = X::THREADS / 2;
-        if (peak)
-            hipLaunchKernelGGL((xcorr_pass_kernel<LOGM, MODE_CPU, false, true, H>), grid, dim3(H),
-                               0, ctx->stream, p, twf, twi);
-        else
-            hipLaunchKernelGGL((xcorr_pass_kernel<LOGM, MODE_CPU, false, false, H>), grid, dim3(H),
-                               0, ctx->stream, p, twf, twi);
+        launch_pass_fp<LOGM, MODE_CPU, false, X::THREADS / 2>(ctx, p, peak, twf, twi);
         return;
     }
     if constexpr (LOGM >= 6) {
@@ -781,30 +831,15 @@ static void launch_pass_mc(Context *ctx, const PassParams &p, bool from_image, b
             // twice the default work-group: every line of a 1D pass in one round
             // (measured on MI355X, C2 step: 32 px pass 0.298 -> 0.267 ms, 64 px
             // 0.475 -> 0.400 ms)
-            constexpr int T2 = X::THREADS * 2;
-            if (peak)
-                hipLaunchKernelGGL((xcorr_pass_kernel<LOGM, MODE_CPU, true, true, T2>), grid, dim3(T2),
-                                   0, ctx->stream, p, twf, twi);
-            else
-                hipLaunchKernelGGL((xcorr_pass_kernel<LOGM, MODE_CPU, true, false, T2>), grid,
-                                   dim3(T2), 0, ctx->stream, p, twf, twi);
+            launch_pass_fp<LOGM, MODE_CPU, true, X::THREADS * 2>(ctx, p, peak, twf, twi);
             return;
         }
-    }
-    if (from_image) {
-        if (peak)
-            hipLaunchKernelGGL((xcorr_pass_kernel<LOGM, MODE_CPU, true, true>), grid, block, 0,
-                               ctx->stream, p, twf, twi);
-        else
-            hipLaunchKernelGGL((xcorr_pass_kernel<LOGM, MODE_CPU, true, false>), grid, block, 0,
-                               ctx->stream, p, twf, twi);
+        launch_pass_fp<LOGM, MODE_CPU, false, X::THREADS>(ctx, p, peak, twf, twi);
     } else {
-        if (peak)
-            hipLaunchKernelGGL((xcorr_pass_kernel<LOGM, MODE_CPU, false, true>), grid, block, 0,
-                               ctx->stream, p, twf, twi);
+        if (from_image)
+            launch_pass_fp<LOGM, MODE_CPU, true, X::THREADS>(ctx, p, peak, twf, twi);
         else
-            hipLaunchKernelGGL((xcorr_pass_kernel<LOGM, MODE_CPU, false, false>), grid, block, 0,
-                               ctx->stream, p, twf, twi);
+            launch_pass_fp<LOGM, MODE_CPU, false, X::THREADS>(ctx, p, peak, twf, twi);
     }
 }
 
@@ -822,6 +857,8 @@ int launch_pass(Context *ctx, const PassParams &p)
     if (p.n == 0) return QLP_OK;
     const bool from_image = p.imgA != nullptr;
     const bool peak = p.peaks != nullptr || p.min_floor != nullptr || p.nbr != nullptr;
+    if (p.n_dev && (from_image || !peak || p.tAf || p.tBf))
+        return fail(QLP_ERR_ARG, "pass: persistent passes take 8-bit tiles and the fused peak");
     if (p.nbr && (p.nbr_half < 0 || p.nbr_half > 7))
         return fail(QLP_ERR_ARG, "pass: neighbourhood half-width must be 0..7");
     // measured on MI355X (C2 multipass): the lane-per-line form is faster for

@@ -235,6 +235,9 @@ __global__ __launch_bounds__(64 * MF_WAVES, 3) void xcorr_mfma_kernel(MfmaParams
     const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
     unsigned short *__restrict__ xh = s_xh[wave];
     unsigned short *__restrict__ xl = s_xl[wave];
+    // a persistent launch reads the IA count from device memory
+    const int n = pass_count(p);
+    if (p.n_dev) n_iter = ((n + MF_WAVES - 1) / MF_WAVES + (int)gridDim.x - 1) / (int)gridDim.x;
 
     for (int it = 0; it < n_iter; it++) {
         // The lane coordinates are made opaque once per iteration: everything
@@ -247,7 +250,7 @@ __global__ __launch_bounds__(64 * MF_WAVES, 3) void xcorr_mfma_kernel(MfmaParams
         const int j = lane % MT, g = lane / MT;
         auto frag = [&](int f, int lo) -> u4 { return s_frag[(f * 2 + lo) * 64 + lane]; };
         const int ia = (it * (int)gridDim.x + (int)blockIdx.x) * MF_WAVES + wave;
-        const bool valid = ia < p.n;
+        const bool valid = ia < n;
 
         // ---- tiles in the B layout of Hf a: element (kb, s) = row kphys, column j
         float ta[X::KN][8], tg[X::KN][8];
@@ -256,7 +259,7 @@ __global__ __launch_bounds__(64 * MF_WAVES, 3) void xcorr_mfma_kernel(MfmaParams
             int amax = 0, bmax = 0;
             bool warp = false;
             if (valid) {
-                const int per_pair = p.per_pair > 0 ? p.per_pair : p.n;
+                const int per_pair = p.per_pair > 0 ? p.per_pair : n;
                 const int pair = ia / per_pair, tt = ia - pair * per_pair;
                 const size_t img_off = (size_t)pair * p.H * p.W;
                 const uint8_t *__restrict__ imA = p.imgA + img_off;
@@ -651,7 +654,8 @@ __global__ __launch_bounds__(m64::NT, 2) void xcorr_mfma64_kernel(PassParams p,
     const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
     const int sr = wave >> 1, sc = wave & 1;
 
-    for (int ia = blockIdx.x; ia < p.n; ia += gridDim.x) {
+    const int n = pass_count(p);
+    for (int ia = blockIdx.x; ia < n; ia += gridDim.x) {
         int lane = threadIdx.x & 63;
         asm volatile("" : "+v"(lane));  // see xcorr_mfma_kernel
         const int j = lane % MT, g = lane / MT;
@@ -663,7 +667,7 @@ __global__ __launch_bounds__(m64::NT, 2) void xcorr_mfma64_kernel(PassParams p,
         float fu = 0.0f, fv = 0.0f;
         bool warp = false;
         if (FROM_IMAGE) {
-            const int per_pair = p.per_pair > 0 ? p.per_pair : p.n;
+            const int per_pair = p.per_pair > 0 ? p.per_pair : n;
             const int pair = ia / per_pair, tt = ia - pair * per_pair;
             const size_t img_off = (size_t)pair * p.H * p.W;
             a8 = p.imgA + img_off;

@@ -693,18 +693,20 @@ class VelocityExporter:
 
 
 class Batch:
-    """Frame pairs of mixed frame and IA sizes run as one graph launch
-    (qlp_batch): the single-level frame job (clip, correlation, FromCenter peak,
-    sub-pixel fit) of every pair, with the kernel shapes and the schedule picked
-    by the library's autotuner on the first run.
+    """Frame pairs of mixed frame and IA sizes run by the step engine
+    (qlp_batch): the per-frame job (clip or micro warp, correlation, FromCenter
+    peak, sub-pixel fit, Direct inheritance between levels) of every pair, one
+    graph launch per adaptive level, with the kernel shapes and the schedule
+    picked by the library's autotuner on the first run.
 
-    pairs: list of (H, W, ia).  config: the PivPlan keys (overlap_factor,
-    margins, norm_mode, subpix_*, interpolator_start_step, xcorr_form)."""
+    pairs: list of (H, W, ia) or (H, W, ia, levels) (ia: last-level IA size).
+    config: the PivPlan keys (overlap_factor, margins, norm_mode, warp_mode,
+    subpix_*, interpolator_start_step, xcorr_form)."""
 
     def __init__(self, ctx, pairs, **config):
         self.ctx = ctx
         self.lib = ctx.lib
-        self.pairs = [tuple(int(x) for x in p) for p in pairs]
+        self.pairs = [tuple(int(x) for x in p) + ((1,) if len(p) == 3 else ()) for p in pairs]
         arr = (_lib.BatchPair * len(self.pairs))(*[_lib.BatchPair(*p) for p in self.pairs])
         cfg = piv_config(**config)
         h = ctypes.c_void_p()

@@ -242,7 +242,8 @@ class ExportParams(ctypes.Structure):
 
 class BatchPair(ctypes.Structure):
     """qlp_batch_pair (include/qlpiv.h)."""
-    _fields_ = [("H", ctypes.c_int), ("W", ctypes.c_int), ("ia", ctypes.c_int)]
+    _fields_ = [("H", ctypes.c_int), ("W", ctypes.c_int), ("ia", ctypes.c_int),
+                ("levels", ctypes.c_int)]
 
 
 class BatchClassReport(ctypes.Structure):

@@ -75,3 +75,37 @@ def particle_pair(H, W, seed, density=0.05, diameter=2.5, noise_sigma=2.0, shift
     a = _render(H, W, xs, ys, i0, diameter, rng_noise, noise_sigma)
     b = _render(H, W, xs + dx, ys + dy, i0, diameter, rng_noise, noise_sigma)
     return a, b, field
+
+
+# The mixed batch workload: 64 pairs, 16 each at 512x512, 1024x1024, 1024x2048
+# and 2048x2048 (H x W), the IA size of each pair drawn from {16, 32, 64, 128},
+# per-pair Rankine vortex R ~ U(32, 256), U ~ U(2, 10).
+BATCH_SIZES = [(512, 512), (1024, 1024), (1024, 2048), (2048, 2048)]
+
+
+def batch_specs(seed=1700):
+    """(H, W, ia, frame seed, core radius, max speed) of the 64 pairs."""
+    rng = np.random.default_rng(seed)
+    specs = []
+    for (h, w) in BATCH_SIZES:
+        for _ in range(16):
+            specs.append((h, w, int(rng.choice([16, 32, 64, 128])), seed + len(specs),
+                          float(rng.uniform(32, 256)), float(rng.uniform(2, 10))))
+    return specs
+
+
+def _spec_frames(spec):
+    h, w, _ia, s, R, U = spec
+    a, b, _ = particle_pair(h, w, seed=s, shift=(0.0, 0.0), core_radius=R, u_max=U)
+    return a, b
+
+
+def batch_frames(specs, workers=8):
+    """The frame pairs of batch_specs, generated in a process pool (workers = 0:
+    in this process, e.g. under a profiler that does not follow forks)."""
+    if workers <= 0:
+        return [_spec_frames(s) for s in specs]
+    from multiprocessing import Pool
+
+    with Pool(workers) as pool:
+        return pool.map(_spec_frames, specs)
