@@ -632,18 +632,25 @@ int qlp_export_close(qlp_exporter *exporter);
  *
  * Replaces the reference's pool of per-thread PIV workers over a list of frame
  * pairs (InputFiles.java:172-224 with OpenClGpuManager.java:124-142: one job
- * and device queue per worker thread) by one server per GPU.  Requests are
- * admitted in arrival order as soon as a slot (a context with its own stream)
- * is free and the HBM they need fits the server's budget (cached plans of idle
- * slots are evicted, least recently used first); requests on different slots
- * overlap on the device.  Frames, twiddles and every level's displacement state
- * stay on the device for the request's whole multi-pass run; the last level's
- * field is returned in host memory. */
+ * and device queue per worker thread) by one server per GPU with continuous
+ * batching.  Requests are admitted in arrival order as soon as the HBM they
+ * need fits the server's budget.  Batched path (FromCenter, Direct inheritance,
+ * no warp or the SecondImage micro warp, no validation, 8-bit frames, any fit
+ * and arithmetic form): each step of the server runs the next adaptive level
+ * of every admitted request, of every frame and IA size, as one launch of a
+ * captured graph of persistent kernels (the batch engine of qlp_batch);
+ * requests that arrive meanwhile join the following steps.  Other
+ * configurations run on plan slots (a context with its own stream each), which
+ * overlap on the device.  Frames, twiddles, the level grids and every level's
+ * displacement state stay on the device for the request's whole multi-pass
+ * run; the last level's field is returned in host memory.  Per request the
+ * result equals a multi-pass plan of that request alone. */
 #define QLP_SERVER_MAX_SLOTS 16
 
 typedef struct qlp_server_config {
-    size_t hbm_budget; /* bytes of device memory the server may hold */
-    int slots;         /* requests in flight at once (1..QLP_SERVER_MAX_SLOTS) */
+    size_t hbm_budget;  /* bytes of device memory the server may hold */
+    int slots;          /* plan-path requests in flight at once (1..QLP_SERVER_MAX_SLOTS) */
+    int batch_requests; /* batched-path requests admitted at once (0: 512) */
 } qlp_server_config;
 
 typedef struct qlp_request {

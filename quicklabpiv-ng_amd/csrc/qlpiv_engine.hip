@@ -701,6 +701,8 @@ int engine_step(Engine *e, const std::vector<StepSeg> segs[ENG_CLASSES])
     return check_hip(hipGraphLaunch(e->exec, ctx->stream), "engine graph launch");
 }
 
+size_t engine_cap(const Engine *e, int cls) { return e->cls[cls].cap; }
+
 int engine_report(const Engine *e, qlp_batch_report *r)
 {
     memset(r, 0, sizeof(*r));

@@ -311,6 +311,7 @@ int engine_fix(Engine *e, int ia, int shape, int schedule);
 // the graph, synchronising).  segs[c]: the pair levels of class c.
 int engine_step(Engine *e, const std::vector<StepSeg> segs[ENG_CLASSES]);
 int engine_report(const Engine *e, qlp_batch_report *r);
+size_t engine_cap(const Engine *e, int cls);
 void engine_destroy(Engine *e);
 
 // First-level IA origins of a frame (AdaptiveInterAreaStrategySuperPosition,

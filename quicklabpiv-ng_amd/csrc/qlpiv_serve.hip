@@ -3,23 +3,31 @@
 // The reference processes frame pairs with a pool of worker threads, one
 // correlation job (and device queue) per thread (InputFiles.java:172-224,
 // OpenClGpuManager.java:124-142).  Here one process owns the GPU and a server
-// object runs the requests:
-//   - slots: each slot is a library context (its own HIP stream, twiddle tables
-//     and workspaces) plus device frame buffers and a cache of multi-pass plans
-//     keyed by the request's frame size and configuration; requests on
-//     different slots overlap on the device;
-//   - admission: requests wait in arrival order and take a free slot as soon as
-//     the HBM they need (frames, plus the plan unless the slot holds it
-//     already) fits the server's budget; idle slots' cached plans are evicted,
-//     least recently used first, to make room;
-//   - a worker thread admits, launches (frames to the device, the multi-pass
-//     run, the last level's field back to pinned host memory, one event) and
-//     retires completed requests; callers submit, wait and read results.
-// Frames, twiddles and the displacement state of every level stay on the device
-// for the whole multi-pass run of a request.
+// object runs the requests with continuous batching:
+//   - batched path (the configurations the step engine runs: FromCenter,
+//     Direct inheritance, no warp or the micro warp, no validation, 8-bit
+//     frames): every admitted request holds its frames, its level geometry
+//     (shared by the requests of one frame size and configuration) and the
+//     displacement field of each level on the device; each step of the worker
+//     advances EVERY admitted request by one adaptive level, all of them, of
+//     every frame and IA size, in one launch of the step engine's captured
+//     graph of persistent kernels (qlpiv_engine.hip).  Requests that arrive
+//     while others are on the GPU are admitted between steps as soon as their
+//     memory fits the budget, so their levels join the running steps; a request
+//     whose last level has run has its field copied to pinned host memory and
+//     retires;
+//   - plan path (every other configuration): slots, each a library context
+//     (its own HIP stream) with device frame buffers and a cache of multi-pass
+//     plans keyed by frame size and configuration; requests on different slots
+//     overlap on the device;
+//   - admission in arrival order against the server's HBM budget; cached plans
+//     of idle slots and unused level geometries are evicted, least recently
+//     used first, to make room.
 #include <hip/hip_runtime.h>
 #include <string.h>
 
+#include <algorithm>
+#include <array>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -35,6 +43,19 @@ namespace qlpiv {
 
 using Clock = std::chrono::steady_clock;
 
+// The IA grids of every level of one frame size and configuration (device).
+struct GeoLevel {
+    int N = 0, nI = 0, nJ = 0;
+    int *top = nullptr, *left = nullptr;
+    std::vector<int> top_h, left_h;  // kept until the server ends (upload sources)
+};
+struct Geo {
+    std::vector<GeoLevel> lv;
+    size_t bytes = 0;
+    int users = 0;
+    uint64_t last_use = 0;
+};
+
 struct ServeRequest {
     uint64_t ticket = 0;
     qlp_request req{};
@@ -46,6 +67,19 @@ struct ServeRequest {
     int nI = 0, nJ = 0, ia = 0;
     std::vector<float> u, v;
     size_t ias = 0;
+    // batched path state
+    bool batched = false;
+    int engine = -1;                // index of the server's engine for this configuration
+    Geo *geo = nullptr;
+    int next_level = 0;             // the level the next step runs
+    const uint8_t *fa = nullptr, *fb = nullptr;  // device frames (own copies or the caller's)
+    uint8_t *own = nullptr;         // server-owned frame copies (2 H W), or null
+    std::vector<float *> lu, lv;    // each level's field (device)
+    float *hu = nullptr, *hv = nullptr;  // pinned copies of the last level's field
+    size_t hcap = 0;
+    hipEvent_t done_ev = nullptr;
+    bool done_enqueued = false;
+    size_t bytes = 0;               // device bytes held (frames and fields)
 };
 
 struct PlanEntry {
@@ -74,6 +108,23 @@ struct qlp_server {
     int device = 0;
     size_t budget = 0, in_use = 0, peak = 0;
     std::vector<Slot> slots;
+    // batched path: one context (stream) for every step, engines by
+    // configuration, level geometries by frame size and configuration
+    qlp_ctx *ectx = nullptr;
+    struct EngineEntry {
+        EngineConfig cfg;
+        Engine *eng = nullptr;
+        size_t bytes = 0;
+    };
+    std::vector<EngineEntry> engines;
+    std::map<std::string, std::unique_ptr<Geo>> geos;
+    std::deque<std::shared_ptr<ServeRequest>> active;  // admitted, in admission order
+    int max_active = 512;
+    std::multimap<size_t, float *> pinned_free;  // pinned result buffers by capacity (floats)
+    std::vector<hipEvent_t> events_free;
+    std::deque<hipEvent_t> step_events;          // steps in flight
+    std::vector<hipEvent_t> step_events_free;
+    uint64_t steps = 0;
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::deque<std::shared_ptr<ServeRequest>> pending;
@@ -216,6 +267,374 @@ int launch(qlp_server *s, Slot &sl, const std::shared_ptr<ServeRequest> &r, cons
     return QLP_OK;
 }
 
+// ---- batched path -----------------------------------------------------------
+
+// The configurations the step engine runs (every other one takes a plan slot).
+bool engine_ok(const qlp_piv_config &c)
+{
+    return c.peak_kind == QLP_PEAK_FROM_CENTER && c.n_peaks <= 1 && c.valid.n_stages == 0 &&
+           c.inherit_mode == QLP_INHERIT_DIRECT && c.frame_type == QLP_FRAME_U8 &&
+           (c.warp_mode == QLP_WARP_NONE ||
+            (c.warp_mode == QLP_WARP_MICRO && c.warp_order == QLP_WARP_ORDER_SECOND)) &&
+           c.keep_planes == 0 && eng_class_of(c.ia_end) >= 0 && eng_class_of(c.ia_start) >= 0;
+}
+
+std::string geo_key(const qlp_request &r)
+{
+    const qlp_piv_config &c = r.cfg;
+    const int v[9] = {r.H, r.W, c.ia_start, c.ia_end, c.margin_top, c.margin_bottom,
+                      c.margin_left, c.margin_right, 0};
+    std::string k(reinterpret_cast<const char *>(v), sizeof v);
+    k.append(reinterpret_cast<const char *>(&c.overlap_factor), sizeof(float));
+    return k;
+}
+
+// Level grids of a request (host side), and the device bytes they need.
+int build_geo(const qlp_request &r, Geo &g)
+{
+    const qlp_piv_config &c = r.cfg;
+    const int m[4] = {c.margin_top, c.margin_bottom, c.margin_left, c.margin_right};
+    g.lv.clear();
+    g.bytes = 0;
+    for (int l = 0, tile = c.ia_start; tile >= c.ia_end; l++, tile /= 2) {
+        GeoLevel L;
+        L.N = tile;
+        level_origins(r.H, r.W, tile, c.overlap_factor, m, l == 0, L.nI, L.nJ, L.top_h, L.left_h);
+        if (L.nI < 1 || L.nJ < 1) return fail(QLP_ERR_ARG, "server: no IA fits the frame");
+        g.bytes += (size_t)L.nI * L.nJ * 8;
+        g.lv.push_back(std::move(L));
+    }
+    return QLP_OK;
+}
+
+// Step capacity of each IA class (IAs), scaled so that the engine takes at most
+// a quarter of the budget.
+void engine_caps(const EngineConfig &ec, size_t budget, size_t cap[ENG_CLASSES])
+{
+    const size_t base[ENG_CLASSES] = {1u << 20, 1u << 18, 1u << 16, 1u << 14};
+    for (int shift = 0;; shift++) {
+        for (int c = 0; c < ENG_CLASSES; c++) cap[c] = std::max<size_t>(base[c] >> shift, 64);
+        if (engine_bytes(ec, cap) <= budget / 4 || shift >= 10) return;
+    }
+}
+
+int engine_for(qlp_server *s, const qlp_piv_config &c, int &index)
+{
+    const EngineConfig ec{c.norm_mode, c.subpix_kind, c.subpix_pixels, c.subpix_iterations,
+                          c.xcorr_form};
+    for (size_t k = 0; k < s->engines.size(); k++) {
+        const EngineConfig &o = s->engines[k].cfg;
+        if (o.norm_mode == ec.norm_mode && o.subpix_kind == ec.subpix_kind &&
+            o.subpix_pixels == ec.subpix_pixels && o.subpix_iterations == ec.subpix_iterations &&
+            o.xcorr_form == ec.xcorr_form) {
+            index = (int)k;
+            return QLP_OK;
+        }
+    }
+    size_t cap[ENG_CLASSES];
+    engine_caps(ec, s->budget, cap);
+    qlp_server::EngineEntry e;
+    e.cfg = ec;
+    e.bytes = engine_bytes(ec, cap);
+    if (s->in_use + e.bytes > s->budget) return fail(QLP_ERR_BUDGET, "server: no room for a batch engine");
+    if (int rc = engine_create(ctx_of(s->ectx), ec, cap, &e.eng)) return rc;
+    s->in_use += e.bytes;
+    s->engines.push_back(e);
+    index = (int)s->engines.size() - 1;
+    return QLP_OK;
+}
+
+hipStream_t estream(qlp_server *s) { return (hipStream_t)qlp_ctx_stream(s->ectx); }
+
+// Frees unused level geometries (least recently used first) until `need` more
+// bytes fit the budget.  Called with the lock held.
+bool free_geos(qlp_server *s, size_t need)
+{
+    while (s->in_use + need > s->budget) {
+        auto best = s->geos.end();
+        for (auto it = s->geos.begin(); it != s->geos.end(); ++it)
+            if (it->second->users == 0 && (best == s->geos.end() || it->second->last_use < best->second->last_use))
+                best = it;
+        if (best == s->geos.end()) return false;
+        for (GeoLevel &L : best->second->lv) {
+            (void)hipFreeAsync(L.top, estream(s));
+            (void)hipFreeAsync(L.left, estream(s));
+        }
+        s->in_use -= best->second->bytes;
+        s->geos.erase(best);
+        s->evictions++;
+    }
+    return true;
+}
+
+// Admits request r on the batched path if its memory fits the budget now.
+// Returns 1 when admitted, 0 when it must wait, -1 when it failed (rc set).
+int admit_batched(qlp_server *s, const std::shared_ptr<ServeRequest> &r, int &rc)
+{
+    rc = QLP_OK;
+    if ((int)s->active.size() >= s->max_active) return 0;
+    const qlp_request &q = r->req;
+    int ei = -1;
+    if ((rc = engine_for(s, q.cfg, ei))) return -1;
+    const std::string gk = geo_key(q);
+    auto git = s->geos.find(gk);
+    Geo fresh;
+    Geo *g = nullptr;
+    if (git != s->geos.end()) {
+        g = git->second.get();
+    } else {
+        if ((rc = build_geo(q, fresh))) return -1;
+        g = &fresh;
+    }
+    size_t state = 0;
+    for (const GeoLevel &L : g->lv) {
+        const size_t n = (size_t)L.nI * L.nJ;
+        state += n * 8;
+        if (n > engine_cap(s->engines[ei].eng, eng_class_of(L.N))) {
+            rc = fail(QLP_ERR_BUDGET, "server: a level holds more IAs than a batch step (raise the budget)");
+            return -1;
+        }
+    }
+    const size_t frames = q.frames_on_device ? 0 : 2 * (size_t)q.H * q.W;
+    const size_t need = state + frames + (git == s->geos.end() ? g->bytes : 0);
+    if (s->in_use + need > s->budget) {
+        if (!free_geos(s, need) && !make_room(s, need, nullptr, nullptr)) return 0;
+        if (s->in_use + need > s->budget) return 0;
+    }
+    hipStream_t st = estream(s);
+    if (git == s->geos.end()) {
+        for (GeoLevel &L : fresh.lv) {
+            const size_t n = (size_t)L.nI * L.nJ;
+            if ((rc = check_hip(hipMallocAsync((void **)&L.top, n * 4, st), "server geometry")) ||
+                (rc = check_hip(hipMallocAsync((void **)&L.left, n * 4, st), "server geometry")) ||
+                (rc = check_hip(hipMemcpyAsync(L.top, L.top_h.data(), n * 4, hipMemcpyHostToDevice, st),
+                                "server geometry")) ||
+                (rc = check_hip(hipMemcpyAsync(L.left, L.left_h.data(), n * 4, hipMemcpyHostToDevice, st),
+                                "server geometry")))
+                return -1;
+        }
+        auto up = std::make_unique<Geo>(std::move(fresh));
+        g = up.get();
+        s->in_use += g->bytes;
+        s->geos.emplace(gk, std::move(up));
+    }
+    g->users++;
+    g->last_use = ++s->use_clock;
+    r->batched = true;
+    r->engine = ei;
+    r->geo = g;
+    r->next_level = 0;
+    r->bytes = state + frames;
+    s->in_use += r->bytes;
+    if (frames) {
+        if ((rc = check_hip(hipMallocAsync((void **)&r->own, frames, st), "server frames"))) return -1;
+        const size_t fb = (size_t)q.H * q.W;
+        if ((rc = check_hip(hipMemcpyAsync(r->own, q.imgA, fb, hipMemcpyHostToDevice, st), "server frames")) ||
+            (rc = check_hip(hipMemcpyAsync(r->own + fb, q.imgB, fb, hipMemcpyHostToDevice, st), "server frames")))
+            return -1;
+        r->fa = r->own;
+        r->fb = r->own + fb;
+    } else {
+        r->fa = q.imgA;
+        r->fb = q.imgB;
+    }
+    r->ias = 0;
+    for (const GeoLevel &L : g->lv) {
+        const size_t n = (size_t)L.nI * L.nJ;
+        float *u = nullptr, *v = nullptr;
+        if ((rc = check_hip(hipMallocAsync((void **)&u, n * 4, st), "server field")) ||
+            (rc = check_hip(hipMallocAsync((void **)&v, n * 4, st), "server field")))
+            return -1;
+        r->lu.push_back(u);
+        r->lv.push_back(v);
+        r->ias += n;
+    }
+    const GeoLevel &last = g->lv.back();
+    r->ia = last.N;
+    r->nI = last.nI;
+    r->nJ = last.nJ;
+    if (s->in_use > s->peak) s->peak = s->in_use;
+    r->status = 1;
+    r->t_admit = Clock::now();
+    s->active.push_back(r);
+    return 1;
+}
+
+// Returns the device state of a batched request (stream-ordered frees).
+void release_batched(qlp_server *s, ServeRequest &r)
+{
+    hipStream_t st = estream(s);
+    for (float *p : r.lu) (void)hipFreeAsync(p, st);
+    for (float *p : r.lv) (void)hipFreeAsync(p, st);
+    r.lu.clear();
+    r.lv.clear();
+    if (r.own) (void)hipFreeAsync(r.own, st);
+    r.own = nullptr;
+    if (r.hu) s->pinned_free.emplace(r.hcap, r.hu);
+    if (r.hv) s->pinned_free.emplace(r.hcap, r.hv);
+    r.hu = r.hv = nullptr;
+    if (r.done_ev) s->events_free.push_back(r.done_ev);
+    r.done_ev = nullptr;
+    if (r.geo) r.geo->users--;
+    r.geo = nullptr;
+    s->in_use -= r.bytes;
+    r.bytes = 0;
+}
+
+int pinned_get(qlp_server *s, size_t n, float **out, size_t &cap)
+{
+    auto it = s->pinned_free.lower_bound(n);
+    if (it != s->pinned_free.end()) {
+        *out = it->second;
+        cap = it->first;
+        s->pinned_free.erase(it);
+        return QLP_OK;
+    }
+    size_t c = 1024;
+    while (c < n) c <<= 1;
+    cap = c;
+    return check_hip(hipHostMalloc((void **)out, c * 4, hipHostMallocDefault), "server pinned buffer");
+}
+
+// Retires the batched requests whose last level and field copy have completed.
+bool retire_batched(qlp_server *s)
+{
+    bool progress = false;
+    for (auto it = s->active.begin(); it != s->active.end();) {
+        const std::shared_ptr<ServeRequest> r = *it;
+        if (!r->done_enqueued || hipEventQuery(r->done_ev) != hipSuccess) {
+            ++it;
+            continue;
+        }
+        try {
+            const size_t n = (size_t)r->nI * r->nJ;
+            r->u.assign(r->hu, r->hu + n);
+            r->v.assign(r->hv, r->hv + n);
+            r->t_done = Clock::now();
+            r->status = 2;
+            s->completed++;
+            s->ias_done += r->ias;
+        } catch (...) {
+            fail_request(s, r, fail_exception(), "server: result");
+        }
+        release_batched(s, *r);
+        it = s->active.erase(it);
+        progress = true;
+    }
+    return progress;
+}
+
+// One step: the next level of every admitted request that has one, as far as
+// the engines' class capacities allow, then the field copy of every request
+// whose last level ran.  At most two steps are in flight.
+bool step_batched(qlp_server *s)
+{
+    while (!s->step_events.empty() && hipEventQuery(s->step_events.front()) == hipSuccess) {
+        s->step_events_free.push_back(s->step_events.front());
+        s->step_events.pop_front();
+    }
+    if (s->step_events.size() >= 2) return false;
+    const size_t ne = s->engines.size();
+    std::vector<std::array<std::vector<StepSeg>, ENG_CLASSES>> segs(ne);
+    std::vector<std::array<size_t, ENG_CLASSES>> count(ne);
+    for (auto &c : count) c.fill(0);
+    std::vector<std::shared_ptr<ServeRequest>> stepped;
+    int nseg = 0;
+    for (const auto &r : s->active) {
+        if (r->next_level >= (int)r->lu.size() || nseg >= ENG_MAX_SEGS) continue;
+        const int l = r->next_level;
+        const GeoLevel &L = r->geo->lv[l];
+        const int c = eng_class_of(L.N);
+        const size_t n = (size_t)L.nI * L.nJ;
+        if (count[r->engine][c] + n > engine_cap(s->engines[r->engine].eng, c)) continue;
+        StepSeg g{};
+        g.a = r->fa;
+        g.b = r->fb;
+        g.top = L.top;
+        g.left = L.left;
+        if (l > 0) {
+            const GeoLevel &P = r->geo->lv[l - 1];
+            g.pu = r->lu[l - 1];
+            g.pv = r->lv[l - 1];
+            g.pnI = P.nI;
+            g.pnJ = P.nJ;
+        }
+        g.u = r->lu[l];
+        g.v = r->lv[l];
+        g.H = r->req.H;
+        g.W = r->req.W;
+        g.nI = L.nI;
+        g.nJ = L.nJ;
+        g.warp = r->req.cfg.warp_mode == QLP_WARP_MICRO ? STEP_MICRO : STEP_SHIFT;
+        g.fit = l >= r->req.cfg.interpolator_start_step ? 1 : 0;
+        segs[r->engine][c].push_back(g);
+        count[r->engine][c] += n;
+        nseg++;
+        stepped.push_back(r);
+    }
+    if (stepped.empty()) return false;
+    hipStream_t st = estream(s);
+    for (size_t e = 0; e < ne; e++) {
+        bool any = false;
+        for (int c = 0; c < ENG_CLASSES; c++) any = any || !segs[e][c].empty();
+        if (!any) continue;
+        if (int rc = engine_step(s->engines[e].eng, segs[e].data())) {
+            // the requests of a failed step fail; their state is released at once
+            for (auto &r : stepped)
+                if (r->engine == (int)e) r->next_level = -1;
+            for (auto it = s->active.begin(); it != s->active.end();) {
+                if ((*it)->next_level == -1) {
+                    fail_request(s, *it, rc, "server: step");
+                    (void)hipStreamSynchronize(st);
+                    release_batched(s, **it);
+                    it = s->active.erase(it);
+                } else {
+                    ++it;
+                }
+            }
+        }
+    }
+    for (auto &r : stepped) {
+        if (r->next_level < 0) continue;
+        r->next_level++;
+        if (r->next_level < (int)r->lu.size()) continue;
+        // last level: the field to pinned host memory, then the request's event
+        const size_t n = (size_t)r->nI * r->nJ;
+        int rc = pinned_get(s, n, &r->hu, r->hcap);
+        size_t cap2 = 0;
+        if (!rc) rc = pinned_get(s, n, &r->hv, cap2);
+        if (!rc && !r->done_ev) {
+            if (!s->events_free.empty()) {
+                r->done_ev = s->events_free.back();
+                s->events_free.pop_back();
+            } else {
+                rc = check_hip(hipEventCreateWithFlags(&r->done_ev, hipEventDisableTiming), "server event");
+            }
+        }
+        if (!rc) rc = check_hip(hipMemcpyAsync(r->hu, r->lu.back(), n * 4, hipMemcpyDeviceToHost, st), "server result");
+        if (!rc) rc = check_hip(hipMemcpyAsync(r->hv, r->lv.back(), n * 4, hipMemcpyDeviceToHost, st), "server result");
+        if (!rc) rc = check_hip(hipEventRecord(r->done_ev, st), "server result");
+        if (rc) {
+            fail_request(s, r, rc, "server: result copy");
+            (void)hipStreamSynchronize(st);
+            release_batched(s, *r);
+            s->active.erase(std::find(s->active.begin(), s->active.end(), r));
+            continue;
+        }
+        r->done_enqueued = true;
+    }
+    hipEvent_t ev = nullptr;
+    if (!s->step_events_free.empty()) {
+        ev = s->step_events_free.back();
+        s->step_events_free.pop_back();
+    } else if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+        ev = nullptr;
+    }
+    if (ev && hipEventRecord(ev, st) == hipSuccess) s->step_events.push_back(ev);
+    s->steps++;
+    return true;
+}
+
 void worker_loop(qlp_server *s)
 {
     (void)hipSetDevice(s->device);
@@ -236,11 +655,32 @@ void worker_loop(qlp_server *s)
                 progress = true;
             }
         }
+        if (retire_batched(s)) progress = true;
         if (progress) s->cv_done.notify_all();
-        // admit in arrival order while a slot is free and the head request fits
+        // admit in arrival order while the head request fits: the batched path
+        // (memory and the in-flight limit) or a free plan slot
         while (!s->pending.empty()) {
-            Slot *free_slot = nullptr;
             const std::shared_ptr<ServeRequest> r = s->pending.front();
+            if (engine_ok(r->req.cfg)) {
+                int rc = QLP_OK, got;
+                try {
+                    got = admit_batched(s, r, rc);
+                } catch (...) {
+                    rc = fail_exception();
+                    got = -1;
+                }
+                if (got == 0) break;  // wait for memory or a batch place to free up
+                s->pending.pop_front();
+                if (got < 0) {
+                    (void)hipStreamSynchronize(estream(s));
+                    release_batched(s, *r);
+                    fail_request(s, r, rc, "server: admission");
+                    s->cv_done.notify_all();
+                }
+                progress = true;
+                continue;
+            }
+            Slot *free_slot = nullptr;
             const std::string key = plan_key(r->req);
             // prefer a free slot that already holds this plan
             for (Slot &sl : s->slots)
@@ -285,13 +725,19 @@ void worker_loop(qlp_server *s)
             }
             progress = true;
         }
+        // the next level of every admitted batched request
+        try {
+            if (step_batched(s)) progress = true;
+        } catch (...) {
+            (void)fail_exception();
+        }
         if (s->stop && s->pending.empty()) {
-            bool busy = false;
+            bool busy = !s->active.empty();
             for (Slot &sl : s->slots) busy |= (bool)sl.cur;
             if (!busy) break;
         }
         if (!progress) {
-            bool busy = false;
+            bool busy = !s->active.empty();
             for (Slot &sl : s->slots) busy |= (bool)sl.cur;
             // poll the device while work is in flight, else sleep until a submit
             if (busy)
@@ -322,6 +768,11 @@ try {
     s->device = hip_device;
     s->budget = cfg->hbm_budget;
     s->t0 = Clock::now();
+    s->max_active = cfg->batch_requests > 0 ? cfg->batch_requests : 512;
+    if (int rc = qlp_ctx_create(hip_device, 0, &s->ectx)) {
+        delete s;
+        return rc;
+    }
     s->slots.resize(cfg->slots);
     for (Slot &sl : s->slots) {
         int rc = qlp_ctx_create(hip_device, 0, &sl.ctx);
@@ -331,6 +782,7 @@ try {
                 if (o.done) (void)hipEventDestroy(o.done);
                 if (o.ctx) qlp_ctx_destroy(o.ctx);
             }
+            qlp_ctx_destroy(s->ectx);
             delete s;
             return rc;
         }
@@ -360,6 +812,18 @@ try {
         if (sl.done) (void)hipEventDestroy(sl.done);
         if (sl.ctx) qlp_ctx_destroy(sl.ctx);
     }
+    (void)hipStreamSynchronize(estream(s));
+    for (auto &e : s->engines) engine_destroy(e.eng);
+    for (auto &g : s->geos)
+        for (GeoLevel &L : g.second->lv) {
+            (void)hipFree(L.top);
+            (void)hipFree(L.left);
+        }
+    for (auto &b : s->pinned_free) (void)hipHostFree(b.second);
+    for (hipEvent_t e : s->events_free) (void)hipEventDestroy(e);
+    for (hipEvent_t e : s->step_events) (void)hipEventDestroy(e);
+    for (hipEvent_t e : s->step_events_free) (void)hipEventDestroy(e);
+    qlp_ctx_destroy(s->ectx);
     delete s;
     return QLP_OK;
 } catch (...) {
@@ -461,7 +925,7 @@ try {
     st->hbm_in_use = s->in_use;
     st->hbm_peak = s->peak;
     st->ias_done = (double)s->ias_done;
-    int running = 0;
+    int running = (int)s->active.size();
     for (const Slot &sl : s->slots) running += sl.cur ? 1 : 0;
     st->running = running;
     return QLP_OK;

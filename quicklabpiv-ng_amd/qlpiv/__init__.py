@@ -771,7 +771,9 @@ class Batch:
 
 class Server:
     """The serving form (qlp_server): a request loop over frame pairs of mixed
-    sizes with HBM-budget admission and requests overlapping on the device.
+    sizes with HBM-budget admission and continuous batching (every step runs
+    the next level of every admitted request in one graph launch; other
+    configurations run on plan slots that overlap on the device).
     Replaces the reference's worker-thread pool over frame pairs
     (InputFiles.java:172-224, OpenClGpuManager.java:124-142).
 
@@ -779,9 +781,9 @@ class Server:
     CUDA tensors) and the PivPlan configuration keys; the frames must stay alive
     until the request has finished (wait())."""
 
-    def __init__(self, device=0, hbm_budget=64 << 30, slots=4):
+    def __init__(self, device=0, hbm_budget=64 << 30, slots=4, batch_requests=0):
         self.lib = _lib.load()
-        cfg = _lib.ServerConfig(int(hbm_budget), int(slots))
+        cfg = _lib.ServerConfig(int(hbm_budget), int(slots), int(batch_requests))
         h = ctypes.c_void_p()
         check(self.lib.qlp_server_create(int(device), ctypes.byref(cfg), ctypes.byref(h)))
         self.h = h

@@ -206,7 +206,8 @@ class PivConfig(ctypes.Structure):
 
 class ServerConfig(ctypes.Structure):
     """qlp_server_config (include/qlpiv.h)."""
-    _fields_ = [("hbm_budget", ctypes.c_size_t), ("slots", ctypes.c_int)]
+    _fields_ = [("hbm_budget", ctypes.c_size_t), ("slots", ctypes.c_int),
+                ("batch_requests", ctypes.c_int)]
 
 
 class Request(ctypes.Structure):

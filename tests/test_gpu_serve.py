@@ -131,3 +131,73 @@ def test_device_frames_and_errors():
     assert e.value.code == qlpiv._lib.QLP_ERR_BUDGET
     small.close()
     srv.close()
+
+
+def test_c5_full_size_mix_equals_alone():
+    """The serving configuration at its sizes: frames 512^2, 1024^2, 1024x2048
+    and 2048^2, final IA 16 / 32 / 64 with 1-3 levels, micro warp, FromCenter,
+    Guo 3 px, a 64 GB budget; 36 requests submitted in bursts while earlier
+    ones run (continuous batching), in the exact and the MFMA form.  Every
+    result equals the same request run alone through a plan, bit for bit; two
+    of the plans are held level by level to the oracle."""
+    import time
+
+    import torch
+
+    import qlpiv
+    from plan_oracle import check_plan_levels
+    from qlpiv.synth import particle_pair
+
+    sizes = [(512, 512), (1024, 1024), (1024, 2048), (2048, 2048)]
+    frames = {}
+    for k, (H, W) in enumerate(sizes):
+        a, b, _ = particle_pair(H, W, seed=1500 + k, shift=(3.3, 3.3), core_radius=40.0 + 50 * k,
+                                u_max=2.0 + 2 * k)
+        frames[(H, W)] = (a, b)
+    rng = np.random.default_rng(1600)
+    jobs = []
+    for k in range(36):
+        H, W = sizes[int(rng.integers(4))]
+        ia_end = int(rng.choice([16, 32, 64]))
+        levels = int(rng.integers(1, {16: 3, 32: 2, 64: 1}[ia_end] + 1))
+        form = qlpiv.QLP_XCORR_MFMA_F16 if k % 3 == 2 else qlpiv.QLP_XCORR_EXACT
+        jobs.append(((H, W), dict(ia_start=ia_end << (levels - 1), ia_end=ia_end, warp_mode=1,
+                                  subpix_kind=qlpiv.QLP_SUBPIX_GUO, subpix_pixels=3,
+                                  subpix_iterations=20, xcorr_form=form)))
+    ctx = qlpiv.Context(0)
+    alone = {}
+    for shape, cfg in jobs:
+        key = (shape, tuple(sorted(cfg.items())))
+        if key not in alone:
+            alone[key] = _alone(ctx, *frames[shape], cfg)
+    say(f"C5: {len(alone)} distinct requests run alone")
+    # two of the plans, level by level against the oracle
+    for shape, cfg in (((2048, 2048), dict(ia_start=64, ia_end=16)), ((1024, 2048), dict(ia_start=32, ia_end=16))):
+        a, b = frames[shape]
+        plan = qlpiv.PivPlan(ctx, shape[0], shape[1], 1, warp_mode=1, subpix_kind=1, subpix_pixels=3,
+                             subpix_iterations=20, **cfg)
+        plan.run(torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda())
+        check_plan_levels(plan, a, b, warp_mode=1, norm_mode=0, peak_kind=0, fit_kind=1, log=say)
+        plan.close()
+    ctx.close()
+    pinned = {s: tuple(torch.from_numpy(x).pin_memory().numpy() for x in frames[s]) for s in sizes}
+    srv = qlpiv.Server(0, hbm_budget=64 << 30, slots=2)
+    tickets = []
+    for burst in range(4):
+        for shape, cfg in jobs[burst * 9:(burst + 1) * 9]:
+            tickets.append(srv.submit(*pinned[shape], **cfg))
+        time.sleep(0.002)  # the next burst arrives while these run
+    lat = []
+    for k, t in enumerate(tickets):
+        assert srv.wait(t, 120000), k
+        u, v, ia, times = srv.result(t)
+        shape, cfg = jobs[k]
+        ua, va = alone[(shape, tuple(sorted(cfg.items())))]
+        assert ia == cfg["ia_end"]
+        assert np.array_equal(u, ua) and np.array_equal(v, va), (k, shape, cfg)
+        lat.append(times["done_ms"] - times["submit_ms"])
+    st = srv.stats()
+    say(f"C5: {len(jobs)} requests equal alone; latency p50 {np.percentile(lat, 50):.2f} ms, "
+        f"p99 {np.percentile(lat, 99):.2f} ms; stats {st}")
+    assert st["completed"] == len(jobs) and st["failed"] == 0 and st["hbm_peak"] <= 64 << 30
+    srv.close()
