@@ -14,11 +14,18 @@ vector validation of the reference UI's multi-peak normalised median model
 peak 2, median + bilinear; one sweep).  The frames live in HBM before the
 timed region.
 
-Prints one JSON line (the driver's contract).  The roofline entry is the
-correlation pass of the 32 x 32 level, timed with HIP events on the library's
-stream; the CPU baseline is the oracle's C restatement of the reference's Java
-CPU path (correlation, MultiPeaks, Guo fits, validation), on this host, over the
-16,129 IAs of the 32 x 32 level.
+Prints one JSON line (the driver's contract).  value is the metric's figure:
+the 16,129 IAs of the 32 x 32 level over that level's device time in the step
+(correlation pass, peaks and fits, validation; the plan's HIP events over the
+timed steps); the all-level rate and per-level times sit in config.  The
+roofline entry is the 32 x 32 level's correlation pass.  The CPU baseline is
+the oracle's C restatement of the reference's Java CPU path on this host over
+the same 32 x 32 level (one thread, and IA-parallel over the host threads),
+plus the single-pass 1024 x 1024 CPU-path configuration.  Unless --no-legs,
+the line also carries bounded runs of the batching configuration (64 mixed
+pairs in one graph launch) and of the serving configuration (a Poisson stream
+at 400 req/s through the request loop, against the plain FP32 plan one
+request at a time).
 """
 import argparse
 import json
@@ -43,80 +50,161 @@ def pass_bytes(n_ia, N):
     return n_ia * (2 * N * N + 4 * (2 * N - 1) ** 2 + 8)
 
 
-def cpu_baseline(a, b, levels, stages):
-    """Reference CPU path per IA (clip A, micro-warp B, CrossCorrelationSoftRealFFTJob
-    arithmetic, FindMaximumMultiPeaks, Guo 3 px of every peak) and each level's
-    vector validation, restated in C by the oracle; one thread, over every level
-    of the same pair.  levels: (top, left, u, v, N, nI, nJ) per level, u and v
-    the displacements the level inherits (zero at the first level, where the B
-    tile's micro warp by zero is a plain clip)."""
-    import ctypes
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
+
+def cpu_threads():
+    """Host threads for the IA-parallel variant: the cores this process may run
+    on, capped by OMP_NUM_THREADS (the box's CPU share for one GPU)."""
+    n = len(os.sched_getaffinity(0))
+    cap = os.environ.get("OMP_NUM_THREADS")
+    return max(1, min(n, int(cap))) if cap and cap.isdigit() else n
+
+
+def _median_time(fn, runs):
+    fn()  # warm-up (caches, page faults, lazy library load)
+    ts = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts)), ts
+
+
+def _level_parallel(O, threads, a, b, top, left, u, v, N, **kw):
+    """One level's IAs split into `threads` contiguous chunks, each chunk one
+    oracle call on its own thread (ctypes releases the GIL during the call)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    n = top.size
+    cuts = np.linspace(0, n, threads + 1).astype(int)
+    with ThreadPoolExecutor(threads) as ex:
+        parts = list(ex.map(lambda k: O.level_batch(a, b, top[cuts[k]:cuts[k + 1]],
+                                                    left[cuts[k]:cuts[k + 1]],
+                                                    u[cuts[k]:cuts[k + 1]], v[cuts[k]:cuts[k + 1]],
+                                                    N, **kw),
+                            range(threads)))
+    return np.concatenate([p[0] for p in parts]), np.concatenate([p[1] for p in parts])
+
+
+def cpu_baseline(a, b, lv32, stages, runs=3):
+    """The reference's CPU path restated in C by the oracle (built -O2
+    -ffp-contract=off; not JVM timing), on this host's cores:
+      * value: the 32 x 32 level of the same C2 pair (16,129 IAs from the
+        device's inherited field): clip A, micro-warp B (SecondImage), the Java
+        CPU correlation job's arithmetic (CrossCorrelationSoftRealFFTJob),
+        MultiPeaks 3 x 3 / 3, Guo 3 px / 20 iterations on every peak, the
+        multi-peak normalised median validation; one thread;
+      * all_cores: the same split by IA over the host threads (restatement,
+        IA-parallel -- not a reference capability: the Java CPU job runs one
+        IA after another);
+      * c1: the reference CPU-path configuration (one 1024 x 1024 pair, 63 x 63
+        IAs of 32 x 32, FromCenter, Gaussian 3-point, single pass, no warp),
+        one thread and all threads.
+    Each figure is the median of `runs` timed runs after one warm-up run."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
 
-    L = O.lib()
-    i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
-    L.or_cpu_level_batch_multi.argtypes = [O._u8p, O._u8p, ctypes.c_int, ctypes.c_int,
-                                           ctypes.c_int, ctypes.c_int, i32p, i32p, O._f32p,
-                                           O._f32p, ctypes.c_int, ctypes.c_int, O._f32p, i32p]
+    from qlpiv.geometry import level_grid
+    from qlpiv.synth import particle_pair
+
+    threads = cpu_threads()
+    top, left, u, v, N, nI, nJ = lv32
+    n = top.size
     P = 3
-    top, left, u, v, N = levels[0][:5]
-    out = np.zeros((64, P, 4), np.float32)
-    cnt = np.zeros(64, np.int32)
-    L.or_cpu_level_batch_multi(a, b, H, W, 64, N, top[:64], left[:64], u[:64], v[:64], P, 3, out, cnt)
-    total, dt = 0, 0.0
-    for top, left, u, v, N, nI, nJ in levels:
-        n = top.size
-        out = np.zeros((n, P, 4), np.float32)
-        cnt = np.zeros(n, np.int32)
-        t0 = time.perf_counter()
-        L.or_cpu_level_batch_multi(a, b, H, W, n, N, top, left, u, v, P, 3, out, cnt)
-        c = np.float32(N - 1)
+    c = np.float32(N - 1)
+    lvkw = dict(warp=2, mode=1, peak_kind=1, P=P, k=3, fit_kind=1, pixels=3, iterations=20)
+
+    def validate(out, cnt):
         inc = (out[..., :2] - c).astype(np.float32)
         up = (u + inc[:, 0, 0]).astype(np.float32).reshape(nI, nJ)
         vp = (v + inc[:, 0, 1]).astype(np.float32).reshape(nI, nJ)
         O.validate(top.reshape(nI, nJ), left.reshape(nI, nJ), up, vp, stages,
                    inc.reshape(nI, nJ, P, 2), cnt.reshape(nI, nJ), 0, False)
-        dt += time.perf_counter() - t0
-        total += n
-    sizes = "/".join(str(lv[4]) for lv in levels)
-    return {"value": round(total / dt, 1), "unit": "IAs/s", "cores": 1, "kind": "port",
-            "sample": f"all {total} IAs of one 2048x2048 pair ({sizes} px levels, each from the "
-                      f"device's inherited field): clip, micro-warp, Java CPU xcorr, MultiPeaks "
-                      f"3x3/3, Guo 3px per peak, multi-peak median validation (C restatement, "
-                      f"-O2 -ffp-contract=off, 1 thread), {dt:.2f} s"}
+
+    def one():
+        out, cnt, _ = O.level_batch(a, b, top, left, u, v, N, **lvkw)
+        validate(out, cnt)
+
+    def allc():
+        out, cnt = _level_parallel(O, threads, a, b, top, left, u, v, N, **lvkw)
+        validate(out, cnt)
+
+    print(f"[bench] CPU baseline: 32x32 level, 1 and {threads} threads", file=sys.stderr, flush=True)
+    t1, ts1 = _median_time(one, runs)
+    tm, tsm = _median_time(allc, runs)
+
+    a1, b1, _ = particle_pair(1024, 1024, seed=1000, shift=(0.0, 0.0), core_radius=128.0, u_max=6.0)
+    ct, cl = level_grid(1024, 1024, 32, 0.5)
+    ct = np.ascontiguousarray(ct.ravel(), np.int32)
+    cl = np.ascontiguousarray(cl.ravel(), np.int32)
+    z = np.zeros(ct.size, np.float32)
+    c1kw = dict(warp=0, mode=1, peak_kind=0, P=1, k=3, fit_kind=0, pixels=3, iterations=20)
+    print("[bench] CPU baseline: C1 pair", file=sys.stderr, flush=True)
+    c1_1, _ = _median_time(lambda: O.level_batch(a1, b1, ct, cl, z, z, 32, **c1kw), max(runs, 5))
+    c1_m, _ = _median_time(lambda: _level_parallel(O, threads, a1, b1, ct, cl, z, z, 32, **c1kw),
+                           max(runs, 5))
+    return {"value": round(n / t1, 1), "unit": "IAs/s", "cores": 1, "kind": "port",
+            "cpu_model": cpu_model(), "host_threads_available": len(os.sched_getaffinity(0)),
+            "sample": f"the {n} IAs of the 32x32 level of the same 2048x2048 pair (inherited "
+                      f"field from the device's 64x64 level): clip, micro-warp, Java CPU xcorr, "
+                      f"MultiPeaks 3x3/3, Guo 3px/20it per peak, multi-peak median validation; "
+                      f"C restatement of the reference CPU path (-O2 -ffp-contract=off, not JVM "
+                      f"timing), 1 thread, median of {runs} runs after a warm-up "
+                      f"({', '.join(f'{x:.2f}' for x in ts1)} s)",
+            "level_latency_ms": round(t1 * 1e3, 2),
+            "all_cores": {"label": "restatement, IA-parallel -- not a reference capability",
+                          "value": round(n / tm, 1), "unit": "IAs/s", "cores": threads,
+                          "level_latency_ms": round(tm * 1e3, 2),
+                          "runs_s": [round(x, 3) for x in tsm]},
+            "c1": {"workload": "C1: one 1024x1024 pair, 63x63 IAs of 32x32 at 50% overlap, Java "
+                               "CPU xcorr (A-max normalisation), FromCenter, Gaussian 3-point, "
+                               "single pass, no warp",
+                   "ias": int(ct.size),
+                   "one_thread": {"ias_per_s": round(ct.size / c1_1, 1),
+                                  "pair_latency_ms": round(c1_1 * 1e3, 2)},
+                   "all_cores": {"ias_per_s": round(ct.size / c1_m, 1),
+                                 "pair_latency_ms": round(c1_m * 1e3, 2), "cores": threads,
+                                 "label": "restatement, IA-parallel -- not a reference capability"},
+                   "runs": max(runs, 5)}}
 
 
 def time_plan(ctx, plan, da, db, steps, warmup):
-    """Run warmup untimed steps, then time exactly `steps` steps (host clock
-    around the whole run, HIP events per step on the library's stream), then
-    per-level kernel times over five more runs (the plan's own events)."""
+    """Run warmup untimed steps, then time exactly `steps` steps: the host clock
+    around the whole run, HIP events per step on the library's stream and the
+    plan's own per-level events read after each step.  Returns (wall seconds,
+    per-step ms, per-level mean ms as levels x (pass, peaks+fit, validation))."""
     import torch
 
     s = ctx.torch_stream()
+    plan.set_timing(True)
     for _ in range(warmup):
         plan.run(da, db)
     ctx.synchronize()
     evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
            for _ in range(steps)]
+    per_level = []
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for e0, e1 in evs:
         e0.record(s)
         plan.run(da, db)
         e1.record(s)
+        per_level.append(plan.timing())  # waits for this step
     ctx.synchronize()
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    step_ms = np.array([e0.elapsed_time(e1) for e0, e1 in evs])
-    plan.set_timing(True)
-    per_level = []
-    for _ in range(5):
-        plan.run(da, db)
-        per_level.append(plan.timing())
     plan.set_timing(False)
-    return dt, step_ms, np.mean(np.array(per_level), axis=0)  # levels x (pass, fit, validation)
+    step_ms = np.array([e0.elapsed_time(e1) for e0, e1 in evs])
+    return dt, step_ms, np.mean(np.array(per_level), axis=0)
 
 
 def precision_report(total32, total64):
@@ -238,99 +326,145 @@ def serve_schedule(n, rate, seed):
     return out
 
 
-def serve_run(qlpiv, frames, schedule, form, budget, slots, saturate):
-    """Submit the schedule (at its arrival times, or all at once when saturate)
-    to a fresh server; returns per-request latencies, IAs and the wall time."""
-    srv = qlpiv.Server(0, hbm_budget=budget, slots=slots)
-    common = dict(overlap_factor=0.5, warp_mode=qlpiv._lib.QLP_WARP_MICRO,
-                  subpix_kind=qlpiv.QLP_SUBPIX_GUO, subpix_pixels=3, subpix_iterations=20,
-                  xcorr_form=form)
-    # warm-up: every (size, IA) combination once, so that each slot's plans exist
-    warm = [srv.submit(*frames[s][0], ia_start=a, ia_end=e, **common)
-            for s in range(len(SERVE_SIZES)) for (a, e) in ((64, 16), (32, 16), (16, 16),
-                                                          (64, 32), (32, 32), (64, 64))
-            for _ in range(slots)]
+def serve_frames(qlpiv, seed=1500):
+    """One synthetic pair per serving frame size, device-resident (requests are
+    served from frames already in HBM, for the server and the plain form alike)."""
+    import torch
+
+    from qlpiv.synth import particle_pair
+
+    frames = []
+    for k, (h, w) in enumerate(SERVE_SIZES):
+        rng = np.random.default_rng(k + 10)
+        a, b, _ = particle_pair(h, w, seed=seed + 10 * k, shift=(3.3, 3.3),
+                                core_radius=float(rng.uniform(32, 256)),
+                                u_max=float(rng.uniform(2, 10)))
+        frames.append((torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda()))
+    torch.cuda.synchronize()
+    return frames
+
+
+def serve_common(qlpiv, form):
+    return dict(overlap_factor=0.5, warp_mode=qlpiv._lib.QLP_WARP_MICRO,
+                subpix_kind=qlpiv.QLP_SUBPIX_GUO, subpix_pixels=3, subpix_iterations=20,
+                xcorr_form=form)
+
+
+def latency_stats(arrive, done, ias, wall):
+    lat = (np.array(done) - np.array(arrive)) * 1e3
+    return {"requests": len(lat), "ias": int(ias), "wall_s": round(wall, 4),
+            "ias_per_s": round(ias / wall, 1), "req_per_s": round(len(lat) / wall, 2),
+            "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
+                           "p99": round(float(np.percentile(lat, 99)), 3),
+                           "max": round(float(lat.max()), 3)}}
+
+
+def serve_server(qlpiv, frames, schedule, form, saturate):
+    """The request loop (continuous batching) over the schedule: submitted at
+    their arrival times, or all at once when saturate."""
+    srv = qlpiv.Server(0, hbm_budget=64 << 30, slots=2)
+    common = serve_common(qlpiv, form)
+    # warm-up: every (size, levels) combination once (engine graphs, level grids)
+    warm = [srv.submit(*frames[s], ia_start=a, ia_end=e, **common)
+            for s in range(len(SERVE_SIZES))
+            for (a, e) in ((64, 16), (32, 16), (16, 16), (64, 32), (32, 32), (64, 64))]
     for t in warm:
         srv.wait(t)
         srv.result(t)
     tickets = []
     t0 = time.perf_counter()
-    for k, (ta, s, a, e) in enumerate(schedule):
+    for (ta, s, a, e) in schedule:
         if not saturate:
             while time.perf_counter() - t0 < ta:
                 pass
-        tickets.append(srv.submit(*frames[s][k % len(frames[s])], ia_start=a, ia_end=e, **common))
-    lat, ias, first, last = [], 0, None, None
+        tickets.append(srv.submit(*frames[s], ia_start=a, ia_end=e, **common))
+    arrive, done, ias = [], [], 0
+    first = last = None
     for t in tickets:
         srv.wait(t)
         _, _, _, tm = srv.result(t)
-        lat.append(tm["done_ms"] - tm["submit_ms"])
+        arrive.append(tm["submit_ms"] * 1e-3)
+        done.append(tm["done_ms"] * 1e-3)
         ias += tm["ias"]
         first = tm["submit_ms"] if first is None else min(first, tm["submit_ms"])
         last = tm["done_ms"] if last is None else max(last, tm["done_ms"])
     st = srv.stats()
     srv.close()
-    wall = (last - first) * 1e-3
-    lat = np.array(lat)
-    return {"requests": len(tickets), "ias": ias, "wall_s": round(wall, 4),
-            "ias_per_s": round(ias / wall, 1), "req_per_s": round(len(tickets) / wall, 2),
-            "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
-                           "p99": round(float(np.percentile(lat, 99)), 3),
-                           "max": round(float(lat.max()), 3)},
-            "hbm_peak_bytes": st["hbm_peak"], "failed": st["failed"]}
+    out = latency_stats(arrive, done, ias, (last - first) * 1e-3)
+    out["hbm_peak_bytes"] = st["hbm_peak"]
+    out["failed"] = st["failed"]
+    return out
 
 
-def serve_report(n_req, rate, slots):
+def serve_plain(qlpiv, frames, schedule, saturate):
+    """The plain FP32 GPU form on the same stream: the drop-in multi-pass plan,
+    one request at a time in arrival order (each starts when it has arrived and
+    the previous one has finished), the last level's field copied to the host."""
+    ctx = qlpiv.Context(0)
+    plans = {}
+    common = serve_common(qlpiv, qlpiv.QLP_XCORR_EXACT)
+    for (_, s, a, e) in schedule:
+        if (s, a, e) not in plans:
+            H, W = SERVE_SIZES[s]
+            plans[(s, a, e)] = qlpiv.PivPlan(ctx, H, W, 1, ia_start=a, ia_end=e, **common)
+            plans[(s, a, e)].run(*frames[s])
+            plans[(s, a, e)].result(-1)
+    arrive, done, ias = [], [], 0
+    t0 = time.perf_counter()
+    for (ta, s, a, e) in schedule:
+        if not saturate:
+            while time.perf_counter() - t0 < ta:
+                pass
+        arr = ta if not saturate else 0.0
+        p = plans[(s, a, e)]
+        p.run(*frames[s])
+        p.result(-1)  # synchronises and copies the last level's field to the host
+        arrive.append(arr)
+        done.append(time.perf_counter() - t0)
+        ias += sum(L["nI"] * L["nJ"] for L in p.levels)
+    wall = time.perf_counter() - t0
+    for p in plans.values():
+        p.close()
+    ctx.close()
+    return latency_stats(arrive, done, ias, wall)
+
+
+def serve_report(n_req, rate):
     """The serving configuration: a Poisson stream of frame-pair requests of mixed
-    sizes through the request loop (qlp_server) with a 64 GB HBM budget, for the
-    plain FP32 form and the MFMA fp16 hi+lo form, at the offered rate and
-    saturated (all requests queued at once)."""
-    import torch
-
+    sizes (512^2, 1024^2, 1024x2048, 2048^2; final IA 16/32/64 with 1-3 levels;
+    micro warp, FromCenter, Guo 3 px) through the request loop with continuous
+    batching and a 64 GB HBM budget, in the MFMA fp16 hi+lo and the exact FP32
+    form, against the plain FP32 form (the drop-in plan, one request at a time),
+    at the offered rate and saturated (all requests queued at once).  Frames are
+    device-resident for every form."""
     import qlpiv
-    from qlpiv.synth import particle_pair
 
-    frames = []
-    for k, (h, w) in enumerate(SERVE_SIZES):
-        pairs = []
-        for r in range(2):
-            a, b, _ = particle_pair(h, w, seed=1500 + 10 * k + r, shift=(3.3, 3.3),
-                                    core_radius=float(np.random.default_rng(k + 10 * r).uniform(32, 256)),
-                                    u_max=float(np.random.default_rng(k + 10 * r + 1).uniform(2, 10)))
-            pa = torch.from_numpy(a).pin_memory().numpy()
-            pb = torch.from_numpy(b).pin_memory().numpy()
-            pairs.append((pa, pb))
-        frames.append(pairs)
-        print(f"[serve] frames {h}x{w} ready", file=sys.stderr, flush=True)
+    frames = serve_frames(qlpiv)
     schedule = serve_schedule(n_req, rate, seed=1600)
-    budget = 64 << 30
     out = {}
-    for name, form in (("exact_fp32", qlpiv.QLP_XCORR_EXACT), ("mfma_f16_split", qlpiv.QLP_XCORR_MFMA_F16)):
+    for name, form in (("serving_mfma_f16_split", qlpiv.QLP_XCORR_MFMA_F16),
+                       ("serving_exact_fp32", qlpiv.QLP_XCORR_EXACT)):
         for mode in ("offered", "saturated"):
             print(f"[serve] {name} {mode}: {n_req} requests", file=sys.stderr, flush=True)
-            out.setdefault(name, {})[mode] = serve_run(qlpiv, frames, schedule, form, budget,
-                                                       slots, mode == "saturated")
-    # the plain FP32 form on the same stream: one request at a time (one slot)
-    print(f"[serve] plain FP32 form, one request at a time: {n_req} requests", file=sys.stderr,
-          flush=True)
-    out["plain_fp32_one_at_a_time"] = {"saturated": serve_run(qlpiv, frames, schedule,
-                                                              qlpiv.QLP_XCORR_EXACT, budget, 1,
-                                                              True)}
-    print(json.dumps({"metric": "serving: throughput and latency of a Poisson stream of mixed "
-                                "frame-pair requests",
-                      "config": {"arrival_rate_req_per_s": rate, "requests": n_req,
-                                 "sizes_HxW": SERVE_SIZES, "final_ia": [16, 32, 64],
-                                 "levels": "1-3", "slots": slots, "hbm_budget_bytes": budget,
-                                 "per_request": "micro warp, FromCenter, Guo 3px/20it, direct "
-                                                "inheritance; last level returned to the host"},
-                      "forms": out,
-                      "speedup_mfma_vs_exact_saturated": round(
-                          out["mfma_f16_split"]["saturated"]["ias_per_s"]
-                          / out["exact_fp32"]["saturated"]["ias_per_s"], 3),
-                      "speedup_serving_vs_plain_fp32": round(
-                          out["mfma_f16_split"]["saturated"]["ias_per_s"]
-                          / out["plain_fp32_one_at_a_time"]["saturated"]["ias_per_s"], 3)}),
-          flush=True)
+            out.setdefault(name, {})[mode] = serve_server(qlpiv, frames, schedule, form,
+                                                          mode == "saturated")
+    for mode in ("offered", "saturated"):
+        print(f"[serve] plain FP32 form {mode}: {n_req} requests", file=sys.stderr, flush=True)
+        out.setdefault("plain_fp32_one_at_a_time", {})[mode] = serve_plain(
+            qlpiv, frames, schedule, mode == "saturated")
+    plain = out["plain_fp32_one_at_a_time"]["saturated"]["ias_per_s"]
+    return {"metric": "serving: throughput and latency of a Poisson stream of mixed frame-pair "
+                      "requests",
+            "config": {"arrival_rate_req_per_s": rate, "requests": n_req,
+                       "sizes_HxW": SERVE_SIZES, "final_ia": [16, 32, 64], "levels": "1-3",
+                       "hbm_budget_bytes": 64 << 30,
+                       "per_request": "micro warp, FromCenter, Guo 3px/20it, direct inheritance; "
+                                      "last level returned to the host; frames device-resident"},
+            "forms": out,
+            "speedup_serving_mfma_vs_plain_saturated": round(
+                out["serving_mfma_f16_split"]["saturated"]["ias_per_s"] / plain, 3),
+            "speedup_serving_exact_vs_plain_saturated": round(
+                out["serving_exact_fp32"]["saturated"]["ias_per_s"] / plain, 3)}
 
 
 def batch_report(steps, warmup, gen_workers=8):
@@ -389,7 +523,7 @@ def batch_report(steps, warmup, gen_workers=8):
     batch.close()
     ctx.close()
     mix = {str(n): sum(1 for p in pairs if p[2] == n) for n in (16, 32, 64, 128)}
-    print(json.dumps({"metric": "batching: correlated IAs/sec of 64 mixed frame pairs in one graph launch",
+    return {"metric": "batching: correlated IAs/sec of 64 mixed frame pairs in one graph launch",
                       "value": round(batch.total_ias / ms * 1e3, 1), "unit": "IAs/s",
                       "ms_per_batch": round(ms, 4), "steps": steps, "warmup": warmup,
                       "ias": batch.total_ias, "dtype": "f32", "data": "synthetic",
@@ -402,7 +536,7 @@ def batch_report(steps, warmup, gen_workers=8):
                       "sequential_plans": {"ms_per_batch": round(ms_seq, 4),
                                            "ias_per_s": round(batch.total_ias / ms_seq * 1e3, 1)},
                       "speedup_vs_sequential_plans": round(ms_seq / ms, 3),
-                      "results_equal_to_plans": bool(same)}), flush=True)
+                      "results_equal_to_plans": bool(same)}
 
 
 def level_ms(plan, per_level):
@@ -412,22 +546,34 @@ def level_ms(plan, per_level):
             for k in range(len(plan.levels))}
 
 
+def direct_inherit(pu, nI, nJ):
+    """Direct inheritance: IA (i, j) takes the parent level's (i // 2, j // 2)
+    where that exists, else 0."""
+    out = np.zeros((nI, nJ), np.float32)
+    ci, cj = min(nI, 2 * pu.shape[0]), min(nJ, 2 * pu.shape[1])
+    out[:ci, :cj] = pu[np.arange(ci)[:, None] // 2, np.arange(cj)[None, :] // 2]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-mfma", action="store_true", help="skip the matrix-core form's line")
+    ap.add_argument("--no-mfma", action="store_true", help="skip the matrix-core form")
+    ap.add_argument("--no-legs", action="store_true",
+                    help="skip the bounded batching and serving legs of the default line")
+    ap.add_argument("--leg-requests", type=int, default=800,
+                    help="requests per serving run in the default line's serving leg")
     ap.add_argument("--precision", action="store_true",
                     help="print the precision report of the correlation forms instead")
     ap.add_argument("--precision-ias", type=int, nargs=2, default=(200000, 50000),
                     metavar=("N32", "N64"), help="IAs of 32x32 and of 64x64 in the report")
     ap.add_argument("--serve", action="store_true",
-                    help="print the serving report (Poisson stream through the request loop)")
+                    help="print the full serving report (Poisson stream through the request loop)")
     ap.add_argument("--serve-requests", type=int, default=2000)
     ap.add_argument("--serve-rate", type=float, default=400.0)
-    ap.add_argument("--serve-slots", type=int, default=8)
     ap.add_argument("--batch", action="store_true",
                     help="print the batching report (64 mixed pairs in one graph launch)")
     ap.add_argument("--gen-workers", type=int, default=8,
@@ -442,10 +588,10 @@ def main():
         precision_report(*args.precision_ias)
         return
     if args.serve:
-        serve_report(args.serve_requests, args.serve_rate, args.serve_slots)
+        print(json.dumps(serve_report(args.serve_requests, args.serve_rate)), flush=True)
         return
     if args.batch:
-        batch_report(args.steps, args.warmup, args.gen_workers)
+        print(json.dumps(batch_report(args.steps, args.warmup, args.gen_workers)), flush=True)
         return
 
     import torch
@@ -469,12 +615,14 @@ def main():
     plan = qlpiv.PivPlan(ctx, H, W, 1, **c2)
     n_levels = [L["nI"] * L["nJ"] for L in plan.levels]
     n_step = sum(n_levels)
+    lv32 = [L["ia"] for L in plan.levels].index(32)
+    n32 = n_levels[lv32]
     print(f"[bench] exact FP32 form: {args.warmup} warmup + {args.steps} timed steps of "
           f"{n_step} IAs", file=sys.stderr, flush=True)
     dt, step_ms, per_level = time_plan(ctx, plan, da, db, args.steps, args.warmup)
-    lv32 = [L["ia"] for L in plan.levels].index(32)
+    lvl32_ms = float(per_level[lv32].sum())
     pass32_ms = float(per_level[lv32][0])
-    achieved = pass_bytes(n_levels[lv32], 32) / (pass32_ms * 1e-3) / 1e9
+    achieved = pass_bytes(n32, 32) / (pass32_ms * 1e-3) / 1e9
     ue, ve, _ = plan.result(len(plan.levels) - 1)
 
     mfma = None
@@ -485,14 +633,17 @@ def main():
         um, vm, _ = plan_m.result(len(plan_m.levels) - 1)
         d = np.hypot(um - ue, vm - ve).ravel()
         plan_m.close()
+        m32_ms = float(per_m[lv32].sum())
         mfma = {"form": "MFMA fp16 hi+lo operands, fp32 accumulation (all three levels)",
-                "value": round(n_step * args.steps / dt_m, 1), "unit": "IAs/s",
+                "ias_per_s_at_32x32": round(n32 / (m32_ms * 1e-3), 1), "unit": "IAs/s",
+                "speedup_vs_exact": round(lvl32_ms / m32_ms, 3),
+                "pass_32x32_ms": round(float(per_m[lv32][0]), 4),
                 "ms_per_step": round(dt_m / args.steps * 1e3, 3),
+                "ias_per_s_all_levels": round(n_step * args.steps / dt_m, 1),
+                "speedup_vs_exact_step": round(dt / dt_m, 3),
                 "pair_latency_ms": {"p50": round(float(np.percentile(step_m, 50)), 3),
                                     "p99": round(float(np.percentile(step_m, 99)), 3)},
                 "level_ms": level_ms(plan, per_m),
-                "ias_per_s_at_32x32": round(n_levels[lv32] / (float(per_m[lv32].sum()) * 1e-3), 1),
-                "speedup_vs_exact": round(dt / dt_m, 3),
                 "final_field_vs_exact_px": {"median": round(float(np.median(d)), 6),
                                             "p99": round(float(np.percentile(d, 99)), 6),
                                             "frac_within_0.01": round(float((d <= 0.01).mean()), 5)}}
@@ -501,34 +652,32 @@ def main():
     tpath = os.path.join(ROOT, "profiles", "xcorr_pass_c2_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
-            t = json.load(f)
-        traffic = t.get("hbm_bytes_per_launch", {}).get("32")
+            traffic = json.load(f).get("hbm_bytes_per_launch", {}).get("32")
 
     cpu = None
     if not args.no_cpu_baseline:
-        print("[bench] timing the CPU baseline sample", file=sys.stderr, flush=True)
-        levels = []
-        for lv, L in enumerate(plan.levels):
-            ui = np.zeros((L["nI"], L["nJ"]), np.float32)
-            vi = np.zeros_like(ui)
-            if lv > 0:  # Direct inheritance from the device's previous level
-                u, v, _ = plan.result(lv - 1)
-                P = plan.levels[lv - 1]
-                for i in range(L["nI"]):
-                    for j in range(L["nJ"]):
-                        if i // 2 < P["nI"] and j // 2 < P["nJ"]:
-                            ui[i, j] = u[0, i // 2, j // 2]
-                            vi[i, j] = v[0, i // 2, j // 2]
-            levels.append((np.ascontiguousarray(L["top"].ravel(), np.int32),
-                           np.ascontiguousarray(L["left"].ravel(), np.int32), ui.ravel(), vi.ravel(),
-                           L["ia"], L["nI"], L["nJ"]))
-        cpu = cpu_baseline(a, b, levels, stages)
+        L32 = plan.levels[lv32]
+        u64, v64, _ = plan.result(lv32 - 1)
+        ui = direct_inherit(u64[0], L32["nI"], L32["nJ"])
+        vi = direct_inherit(v64[0], L32["nI"], L32["nJ"])
+        cpu = cpu_baseline(a, b, (np.ascontiguousarray(L32["top"].ravel(), np.int32),
+                                  np.ascontiguousarray(L32["left"].ravel(), np.int32),
+                                  ui.ravel(), vi.ravel(), 32, L32["nI"], L32["nJ"]), stages)
+    lvms = level_ms(plan, per_level)
     plan.close()
     ctx.close()
-    value = n_step * args.steps / dt
+    del da, db
+
+    legs = {}
+    if not args.no_legs:
+        print("[bench] batching leg", file=sys.stderr, flush=True)
+        legs["batch"] = batch_report(10, 2, min(args.gen_workers, cpu_threads()))
+        print("[bench] serving leg", file=sys.stderr, flush=True)
+        legs["serving"] = serve_report(args.leg_requests, 400.0)
+
     line = {
-        "metric": "correlated IAs/sec (C2 multipass 64->32->16, all levels)",
-        "value": round(value, 1),
+        "metric": "correlated IAs/sec at 32x32; p50/p99 pair latency; MFMA vs FP32 form",
+        "value": round(n32 / (lvl32_ms * 1e-3), 1),
         "unit": "IAs/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -544,19 +693,27 @@ def main():
                                "storing every plane, MultiPeaks 3 peaks/3x3, Guo 3px/20it on "
                                "every peak, multi-peak normalised median validation (2.0, 0.1) "
                                "with secondary-peak and bilinear replacement, 1 sweep",
+                   "value_is": "the 16,129 IAs of the 32x32 level / that level's device time "
+                               "(correlation pass + peaks and fits + validation, the plan's own "
+                               "HIP events, mean over the timed steps)",
                    "ias_per_step": n_step, "ias_per_level": n_levels,
+                   "ias_per_s_all_levels": round(n_step * args.steps / dt, 1),
                    "pair_latency_ms": {"p50": round(float(np.percentile(step_ms, 50)), 3),
                                        "p99": round(float(np.percentile(step_ms, 99)), 3)},
-                   "level_ms": level_ms(plan, per_level),
-                   "ias_per_s_at_32x32": round(n_levels[lv32] / (float(per_level[lv32].sum()) * 1e-3), 1)},
+                   "level_ms": lvms},
         "mfma_form": mfma,
-        "roofline": {"bound": "hbm", "kernel": "xcorr_pass_kernel<6,false,true,false,512> (32x32 level, 512 lanes)",
+        "roofline": {"bound": "hbm",
+                     "kernel": "xcorr_pass_kernel<6,false,true,false,512> (32x32 level, 512 lanes)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "traffic_source": "profiles/xcorr_pass_c2_traffic.json: rocprofv3 --pmc "
+                                       "FETCH_SIZE and WRITE_SIZE of this kernel on this workload, "
+                                       "scaled by a copy kernel's reading (not measured in this run)",
                      "avg_launch_ms": round(pass32_ms, 4),
-                     "algorithmic_bytes_per_launch": pass_bytes(n_levels[lv32], 32)},
+                     "algorithmic_bytes_per_launch": pass_bytes(n32, 32)},
         "cpu_baseline": cpu,
     }
+    line.update(legs)
     print(json.dumps(line), flush=True)
 
 
