@@ -398,31 +398,46 @@ def serve_server(qlpiv, frames, schedule, form, saturate):
 
 def serve_plain(qlpiv, frames, schedule, saturate):
     """The plain FP32 GPU form on the same stream: the drop-in multi-pass plan,
-    one request at a time in arrival order (each starts when it has arrived and
-    the previous one has finished), the last level's field copied to the host."""
+    one request at a time in arrival order, each enqueued when it has arrived
+    with its last level's field copied to pinned host memory behind it on the
+    stream (no host wait between requests: the copies overlap the next
+    request's kernels as the server's do).  A request is done when the stream
+    passes the event recorded after its copy."""
+    import torch
+
     ctx = qlpiv.Context(0)
+    stream = ctx.torch_stream()
     plans = {}
     common = serve_common(qlpiv, qlpiv.QLP_XCORR_EXACT)
+    hosts = []
     for (_, s, a, e) in schedule:
         if (s, a, e) not in plans:
             H, W = SERVE_SIZES[s]
             plans[(s, a, e)] = qlpiv.PivPlan(ctx, H, W, 1, ia_start=a, ia_end=e, **common)
             plans[(s, a, e)].run(*frames[s])
             plans[(s, a, e)].result(-1)
-    arrive, done, ias = [], [], 0
+        L = plans[(s, a, e)].levels[-1]
+        hosts.append(tuple(torch.empty((1, L["nI"], L["nJ"]), dtype=torch.float32,
+                                       pin_memory=True).numpy() for _ in range(2)))
+    evs = [torch.cuda.Event(enable_timing=True) for _ in schedule]
+    start = torch.cuda.Event(enable_timing=True)
+    ctx.synchronize()
+    ias = 0
     t0 = time.perf_counter()
-    for (ta, s, a, e) in schedule:
+    start.record(stream)
+    for k, (ta, s, a, e) in enumerate(schedule):
         if not saturate:
             while time.perf_counter() - t0 < ta:
                 pass
-        arr = ta if not saturate else 0.0
         p = plans[(s, a, e)]
         p.run(*frames[s])
-        p.result(-1)  # synchronises and copies the last level's field to the host
-        arrive.append(arr)
-        done.append(time.perf_counter() - t0)
+        p.result_async(hosts[k][0], hosts[k][1], -1)
+        evs[k].record(stream)
         ias += sum(L["nI"] * L["nJ"] for L in p.levels)
+    evs[-1].synchronize()
     wall = time.perf_counter() - t0
+    done = [start.elapsed_time(ev) * 1e-3 for ev in evs]
+    arrive = [0.0 if saturate else ta for (ta, _, _, _) in schedule]
     for p in plans.values():
         p.close()
     ctx.close()

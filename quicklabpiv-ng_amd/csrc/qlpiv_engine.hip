@@ -22,14 +22,16 @@
 //
 // Every kernel is persistent: its grid is the work-groups the device holds at
 // once (or fewer), and it reads the step's IA count of its class from device
-// memory, so the whole chain of all four classes is captured ONCE into a HIP
-// graph and replayed for every step whatever the step holds.  Per step the
-// host writes a step table (per class: IA count and the segments, one per
-// pair level: frames, grid, parent and output arrays, first IA) into pinned
-// memory, copies it to the device and launches the graph.  The classes run on
-// forked streams inside the graph or in sequence, whichever the first step
-// measures faster; each class's kernel shape is picked by timing the
-// candidates on the first step that holds the class (cached in the context).
+// memory, so the chain of the classes a step holds is captured once into a HIP
+// graph and replayed for every later step that holds the same classes,
+// whatever their IA counts (one graph per set of classes, at most 15).  Per
+// step the host writes a step table (per class: IA count and the segments, one
+// per pair level: frames, grid, parent and output arrays, first IA) into
+// pinned memory, copies it to the device and launches the graph of the step's
+// classes.  The classes run on forked streams inside the graph or in sequence,
+// whichever the first step with that set measures faster; each class's kernel
+// shape is picked by timing the candidates on the first step that holds the
+// class (cached in the context).
 #include <stddef.h>
 #include <string.h>
 
@@ -214,10 +216,17 @@ struct Engine {
     size_t table_bytes = 0;
     int schedule = QLP_SCHEDULE_TUNED;
     bool schedule_fixed = false;
-    float serial_ms = 0.0f, concurrent_ms = 0.0f;
-    bool built = false;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
+    bool prepared = false, built = false;
+    // one captured graph per set of classes (bit c: class c holds IAs)
+    struct Graph {
+        int mask = 0;
+        int schedule = 0;
+        float serial_ms = 0.0f, concurrent_ms = 0.0f;
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+    };
+    std::vector<Graph> graphs;
+    int last_graph = -1;
     std::vector<hipStream_t> branch;
     std::vector<hipEvent_t> ev;
     uint64_t steps = 0;
@@ -267,8 +276,10 @@ void engine_destroy(Engine *e)
 {
     if (!e) return;
     if (e->ctx) (void)hipStreamSynchronize(e->ctx->stream);
-    if (e->exec) (void)hipGraphExecDestroy(e->exec);
-    if (e->graph) (void)hipGraphDestroy(e->graph);
+    for (Engine::Graph &g : e->graphs) {
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        if (g.graph) (void)hipGraphDestroy(g.graph);
+    }
     for (hipStream_t s : e->branch) (void)hipStreamDestroy(s);
     for (hipEvent_t x : e->ev) (void)hipEventDestroy(x);
     for (int k = 0; k < 2; k++) {
@@ -522,14 +533,14 @@ static int time_class(Engine *e, int cls, int shape, float &ms)
     return rc;
 }
 
-static int capture(Engine *e, bool concurrent, hipGraph_t &graph, hipGraphExec_t &exec)
+static int capture(Engine *e, int mask, bool concurrent, hipGraph_t &graph, hipGraphExec_t &exec)
 {
     hipStream_t s = e->ctx->stream;
     graph = nullptr;
     exec = nullptr;
     std::vector<int> order;
     for (int k = 0; k < ENG_CLASSES; k++)
-        if (e->cls[k].cap) order.push_back(k);
+        if (e->cls[k].cap && (mask >> k & 1)) order.push_back(k);
     // longest class first, so that the short ones fill in behind it
     std::sort(order.begin(), order.end(),
               [&](int x, int y) { return e->cls[x].cost_ms > e->cls[y].cost_ms; });
@@ -582,35 +593,31 @@ static int time_graph(Engine *e, hipGraphExec_t exec, float &ms)
     return rc;
 }
 
-// First step: tune the shapes of the classes this step holds, then capture the
-// graph in both schedules and keep the faster.  The step table is already on
-// the device; every tuning run recomputes the same values (a step only reads
-// its frames and the parent levels).
-static int build(Engine *e, const int counts[ENG_CLASSES])
+// The kernel shape of each class this step holds for the first time: the
+// context's cached pick for its size, or the fastest candidate on this step.
+// The step table is already on the device; every tuning run recomputes the
+// same values (a step only reads its frames and the parent levels).
+static int tune_classes(Engine *e, const int counts[ENG_CLASSES])
 {
     Context *ctx = e->ctx;
-    int rc = prepare_tables(e);
-    if (rc) return rc;
     for (int c = 0; c < ENG_CLASSES; c++) {
         EngineClass &C = e->cls[c];
-        if (!C.cap) continue;
+        if (!C.cap || !counts[c] || C.tuned) continue;
         int cand[4];
         const int nc = shape_candidates(C.N, e->cfg.xcorr_form, cand);
-        const uint64_t key = tune_key(C.N, counts[c] ? counts[c] : (int)C.cap, e->cfg.xcorr_form,
-                                      e->cfg.norm_mode);
+        const uint64_t key = tune_key(C.N, counts[c], e->cfg.xcorr_form, e->cfg.norm_mode);
         C.n_cand = 0;
         if (!C.fixed) {
             auto hit = ctx->tuned_shape.find(key);
             if (hit != ctx->tuned_shape.end()) {
                 C.shape = hit->second;
-            } else if (nc == 1 || counts[c] == 0) {
+            } else if (nc == 1) {
                 C.shape = cand[0];
             } else {
                 float best = 1e30f;
                 for (int k = 0; k < nc; k++) {
                     float ms = 0.0f;
-                    rc = time_class(e, c, cand[k], ms);
-                    if (rc) return rc;
+                    if (int rc = time_class(e, c, cand[k], ms)) return rc;
                     C.cand_shape[C.n_cand] = cand[k];
                     C.cand_ms[C.n_cand++] = ms;
                     if (ms < best) {
@@ -621,36 +628,57 @@ static int build(Engine *e, const int counts[ENG_CLASSES])
                 ctx->tuned_shape[key] = C.shape;
             }
         }
-        if (counts[c]) {
-            float ms = 0.0f;
-            rc = time_class(e, c, C.shape, ms);
-            if (rc) return rc;
-            C.cost_ms = ms;
-        }
+        float ms = 0.0f;
+        if (int rc = time_class(e, c, C.shape, ms)) return rc;
+        C.cost_ms = ms;
+        C.tuned = true;
     }
+    return QLP_OK;
+}
+
+// The graph of the classes in `mask`: captured on the first step that holds
+// exactly these classes.  The first step with more than one class captures
+// both schedules, times them and keeps the faster; that choice then holds for
+// every later set (a new set costs one capture, no timing runs).
+static int graph_for(Engine *e, int mask, int &index)
+{
+    for (size_t k = 0; k < e->graphs.size(); k++)
+        if (e->graphs[k].mask == mask) {
+            index = (int)k;
+            return QLP_OK;
+        }
+    int classes = 0;
+    for (int c = 0; c < ENG_CLASSES; c++) classes += mask >> c & 1;
+    const bool decided = e->schedule_fixed || e->schedule != QLP_SCHEDULE_TUNED;
+    const bool both = !decided && classes > 1;
+    const int only = decided ? e->schedule : 0;
     hipGraph_t g[2] = {nullptr, nullptr};
     hipGraphExec_t x[2] = {nullptr, nullptr};
     float t[2] = {0.0f, 0.0f};
+    int rc = QLP_OK;
     for (int conc = 0; conc < 2 && !rc; conc++) {
-        if (e->schedule_fixed && e->schedule != conc) continue;
-        rc = capture(e, conc == 1, g[conc], x[conc]);
-        if (!rc && !e->schedule_fixed) rc = time_graph(e, x[conc], t[conc]);
+        if (!both && conc != only) continue;
+        rc = capture(e, mask, conc == 1, g[conc], x[conc]);
+        if (!rc && both) rc = time_graph(e, x[conc], t[conc]);
     }
-    const int pick = e->schedule_fixed ? e->schedule : (t[1] < t[0] ? 1 : 0);
-    if (!rc) {
-        e->serial_ms = e->schedule_fixed ? 0.0f : t[0];
-        e->concurrent_ms = e->schedule_fixed ? 0.0f : t[1];
-        e->schedule = pick;
-        e->graph = g[pick];
-        e->exec = x[pick];
-        e->built = true;
-    }
+    const int pick = both ? (t[1] < t[0] ? 1 : 0) : only;
     for (int k = 0; k < 2; k++) {
         if (!rc && k == pick) continue;
         if (x[k]) (void)hipGraphExecDestroy(x[k]);
         if (g[k]) (void)hipGraphDestroy(g[k]);
     }
-    return rc;
+    if (rc) return rc;
+    if (both) e->schedule = pick;
+    Engine::Graph G;
+    G.mask = mask;
+    G.schedule = pick;
+    G.serial_ms = both ? t[0] : 0.0f;
+    G.concurrent_ms = both ? t[1] : 0.0f;
+    G.graph = g[pick];
+    G.exec = x[pick];
+    e->graphs.push_back(G);
+    index = (int)e->graphs.size() - 1;
+    return QLP_OK;
 }
 
 int engine_step(Engine *e, const std::vector<StepSeg> segs[ENG_CLASSES])
@@ -692,13 +720,29 @@ int engine_step(Engine *e, const std::vector<StepSeg> segs[ENG_CLASSES])
                    "engine step table");
     if (!rc) rc = check_hip(hipEventRecord(e->staged[k], ctx->stream), "engine staging");
     if (rc) return rc;
-    if (!e->built) {
+    int mask = 0;
+    bool fresh = false;
+    for (int c = 0; c < ENG_CLASSES; c++)
+        if (counts[c]) {
+            mask |= 1 << c;
+            fresh = fresh || !e->cls[c].tuned;
+        }
+    int gi = -1;
+    for (size_t g = 0; g < e->graphs.size(); g++)
+        if (e->graphs[g].mask == mask) gi = (int)g;
+    if (gi < 0 || fresh) {
+        // a new set of classes: tables, shapes and the graph, on this step's table
         rc = check_hip(hipStreamSynchronize(ctx->stream), "engine build");
-        if (!rc) rc = build(e, counts);
+        if (!rc && !e->prepared) rc = prepare_tables(e);
+        e->prepared = e->prepared || !rc;
+        if (!rc) rc = tune_classes(e, counts);
+        if (!rc) rc = graph_for(e, mask, gi);
         if (rc) return rc;
+        e->built = true;
     }
+    e->last_graph = gi;
     e->steps++;
-    return check_hip(hipGraphLaunch(e->exec, ctx->stream), "engine graph launch");
+    return check_hip(hipGraphLaunch(e->graphs[gi].exec, ctx->stream), "engine graph launch");
 }
 
 size_t engine_cap(const Engine *e, int cls) { return e->cls[cls].cap; }
@@ -719,9 +763,14 @@ int engine_report(const Engine *e, qlp_batch_report *r)
             o.cand_ms[k] = C.cand_ms[k];
         }
     }
-    r->schedule = e->schedule;
-    r->serial_ms = e->serial_ms;
-    r->concurrent_ms = e->concurrent_ms;
+    if (e->last_graph >= 0) {  // the graph of the last step
+        const Engine::Graph &G = e->graphs[e->last_graph];
+        r->schedule = G.schedule;
+        r->serial_ms = G.serial_ms;
+        r->concurrent_ms = G.concurrent_ms;
+    } else {
+        r->schedule = e->schedule;
+    }
     return QLP_OK;
 }
 

@@ -20,6 +20,8 @@
 //   log-parabola with the parabolic fallback for negative samples.
 #include <float.h>
 
+#include <algorithm>
+
 #include "qlpiv_internal.h"
 
 namespace qlpiv {
@@ -690,8 +692,6 @@ int launch_subpixel_nbr(Context *ctx, int kind, int pixels, int iterations, int 
                         const int *n_dev, const uint8_t *fit_on)
 {
     if (n == 0) return QLP_OK;
-    // a persistent launch (n = the capacity) takes at most 8,192 work-groups
-    if (n_dev && n > 8192 * 128) n = 8192 * 128;
     if (kind >= 0) {
         int rc = check_subpixel_args(kind, pixels, iterations);
         if (rc) return rc;
@@ -703,14 +703,19 @@ int launch_subpixel_nbr(Context *ctx, int kind, int pixels, int iterations, int 
                           : pixels == 7 ? guo_kernel<true, 3, true> : guo_kernel<true, NBR_MAX_HALF, true>)
                        : (pixels == 3 ? guo_kernel<true, 1> : pixels == 5 ? guo_kernel<true, 2>
                           : pixels == 7 ? guo_kernel<true, 3> : guo_kernel<true, NBR_MAX_HALF>);
-        hipLaunchKernelGGL(k, dim3((2 * n + 255) / 256), dim3(256), 0, ctx->stream, pixels,
-                           iterations, n, D, 1, nbr, nbr_half, min_floor, peaks_in, peaks_out, du,
-                           dv, n_dev, fit_on);
+        // a persistent launch (n = the capacity) takes at most the work-groups
+        // the device holds at once: a step with few IAs then pays for few
+        // empty work-groups
+        int grid = (2 * n + 255) / 256;
+        if (n_dev) grid = std::min(grid, resident_groups(k, 256));
+        hipLaunchKernelGGL(k, dim3(grid), dim3(256), 0, ctx->stream, pixels, iterations, n, D, 1,
+                           nbr, nbr_half, min_floor, peaks_in, peaks_out, du, dv, n_dev, fit_on);
     } else {
         auto k = n_dev ? subpixel_nbr_kernel<true> : subpixel_nbr_kernel<false>;
-        hipLaunchKernelGGL(k, dim3((n + 255) / 256), dim3(256), 0, ctx->stream, kind, pixels,
-                           iterations, n, D, nbr_half, nbr, min_floor, peaks_in, peaks_out, du, dv,
-                           n_dev, fit_on);
+        int grid = (n + 255) / 256;
+        if (n_dev) grid = std::min(grid, resident_groups(k, 256));
+        hipLaunchKernelGGL(k, dim3(grid), dim3(256), 0, ctx->stream, kind, pixels, iterations, n,
+                           D, nbr_half, nbr, min_floor, peaks_in, peaks_out, du, dv, n_dev, fit_on);
     }
     return check_hip(hipGetLastError(), "subpixel kernel launch");
 }

@@ -47,6 +47,9 @@ typedef unsigned int u4 __attribute__((ext_vector_type(4)));  // one operand fra
 typedef unsigned int u2 __attribute__((ext_vector_type(2)));
 
 constexpr int MF_WAVES = 4;  // wavefronts (IAs in flight) per work-group
+#ifndef QLPIV_MF_MINW
+#define QLPIV_MF_MINW 3  // work-groups per CU the register budget is sized for (16/32 px)
+#endif
 
 template <int N>
 struct Mf {
@@ -217,7 +220,7 @@ struct MfmaParams {
 };
 
 template <int N, bool BF, bool SPLIT, bool FROM_IMAGE, bool PEAK>
-__global__ __launch_bounds__(64 * MF_WAVES, 3) void xcorr_mfma_kernel(MfmaParams mp, int n_iter)
+__global__ __launch_bounds__(64 * MF_WAVES, QLPIV_MF_MINW) void xcorr_mfma_kernel(MfmaParams mp, int n_iter)
 {
     using X = Mf<N>;
     static_assert(X::RT == 2, "one wavefront per IA holds 2 x 2 tiles");

@@ -932,6 +932,25 @@ class PivPlan:
         return (_fetch(u, n, np.float32).reshape(shp), _fetch(v, n, np.float32).reshape(shp),
                 _fetch(pk, 4 * n * self.P, np.float32).reshape(pshape))
 
+    def result_async(self, u_host, v_host, level=-1):
+        """Enqueue copies of a level's u, v (pairs x nI x nJ float32) into the
+        given host arrays on the context's stream, after the work enqueued so
+        far; they are complete once the stream has passed them (an event
+        recorded after this call).  Pinned host memory makes them asynchronous
+        to the host."""
+        lv = level % len(self.levels)
+        L = self.levels[lv]
+        n = L["nI"] * L["nJ"] * self.pairs
+        for h in (u_host, v_host):
+            if h.dtype != np.float32 or h.size != n or not h.flags["C_CONTIGUOUS"]:
+                raise QlpError(_lib.QLP_ERR_ARG, "result_async: host arrays of pairs x nI x nJ "
+                                                 "contiguous float32")
+        u, v, pk, pl = (ctypes.c_void_p() for _ in range(4))
+        check(self.ctx.lib.qlp_piv_result_dev(self.h, lv, ctypes.byref(u), ctypes.byref(v),
+                                              ctypes.byref(pk), ctypes.byref(pl)))
+        _hip_copy_to_host_async(u_host, u.value, self.ctx.stream_handle)
+        _hip_copy_to_host_async(v_host, v.value, self.ctx.stream_handle)
+
     def planes(self, level=-1):
         """Host copy of a level's stored correlation planes (pairs x nI x nJ x D x
         D), or None when the level keeps no planes."""
@@ -972,8 +991,8 @@ def _fetch(ptr, count, dtype):
 _hip_rt = None
 
 
-def _hip_copy_to_host(out: np.ndarray, src: int):
-    """Device-to-host copy with the process's HIP runtime (the one torch loaded)."""
+def _hip():
+    """The process's HIP runtime (the one torch loaded)."""
     global _hip_rt
     if _hip_rt is None:
         import torch  # noqa: F401  (loads libamdhip64.so.7 first)
@@ -981,7 +1000,23 @@ def _hip_copy_to_host(out: np.ndarray, src: int):
         rt = ctypes.CDLL("libamdhip64.so.7")
         rt.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
         rt.hipMemcpy.restype = ctypes.c_int
+        rt.hipMemcpyAsync.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
+                                      ctypes.c_int, ctypes.c_void_p]
+        rt.hipMemcpyAsync.restype = ctypes.c_int
         _hip_rt = rt
-    rc = _hip_rt.hipMemcpy(out.ctypes.data, ctypes.c_void_p(src), out.nbytes, 2)  # D2H
+    return _hip_rt
+
+
+def _hip_copy_to_host(out: np.ndarray, src: int):
+    """Device-to-host copy (synchronous)."""
+    rc = _hip().hipMemcpy(out.ctypes.data, ctypes.c_void_p(src), out.nbytes, 2)  # D2H
     if rc != 0:
         raise QlpError(_lib.QLP_ERR_DEVICE, f"hipMemcpy failed ({rc})")
+
+
+def _hip_copy_to_host_async(out: np.ndarray, src: int, stream: int):
+    """Device-to-host copy enqueued on `stream`."""
+    rc = _hip().hipMemcpyAsync(out.ctypes.data, ctypes.c_void_p(src), out.nbytes, 2,
+                               ctypes.c_void_p(stream))
+    if rc != 0:
+        raise QlpError(_lib.QLP_ERR_DEVICE, f"hipMemcpyAsync failed ({rc})")
