@@ -151,4 +151,25 @@ __device__ __forceinline__ uint64_t from_center_key(float v, int i, int j, int h
            (uint64_t)(0xffff - visit_order(i, j, h));
 }
 
+// The same argmax in two phases, for kernels that hold the plane: (1) the
+// largest selectable value vmax over the visited cells (fmaxf from -1 ignores
+// NaN; vmax < 0 means no selectable cell), (2) among the visited cells whose
+// value equals vmax (+0 and -0 alike), the largest tie key below: nearest to
+// (h+1, h+1), then the earliest visited.  The winner is the cell with the
+// largest from_center_key, but the distance and visit order are computed only
+// for the cells that reach the maximum.  The tie key is never 0.
+__device__ __forceinline__ uint32_t from_center_tie(int i, int j, int h)
+{
+    const int c = h + 1;
+    const int d2 = (i - c) * (i - c) + (j - c) * (j - c);
+    return ((uint32_t)(0x7fff - d2) << 16) | (uint32_t)(0xffff - visit_order(i, j, h));
+}
+
+__device__ __forceinline__ float wave_max_float(float v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
 }  // namespace qlpiv

@@ -248,6 +248,20 @@ __device__ __forceinline__ int block_max_int(int v, int *s_red, int tid = -1)
     return m;
 }
 
+// Block-wide maximum of a float (NaN ignored), returned to every thread.
+template <int THREADS>
+__device__ __forceinline__ float block_max_float(float v, float *s_red, int tid)
+{
+    v = wave_max_float(v);
+    if ((tid & 63) == 0) s_red[tid >> 6] = v;
+    __syncthreads();
+    float m = s_red[0];
+#pragma unroll
+    for (int w = 1; w < THREADS / 64; w++) m = fmaxf(m, s_red[w]);
+    __syncthreads();
+    return m;
+}
+
 // One correlation pass: tile formation, exact FFT correlation, and the
 // optional epilogues (plane store, FromCenter peak with its neighbourhood).
 //   FROM_IMAGE : tiles are cut from the images (A: clip at (top, left);
@@ -415,13 +429,31 @@ __global__ __launch_bounds__((XcShape<LOGM, NT_>::THREADS)) void xcorr_pass_kern
             }
         }
         if (PEAK) {
+            // FromCenter in two phases (from_center_tie): the largest value of
+            // the visited cells (rows and columns 1..D-1), then the tie keys of
+            // the cells that equal it
             constexpr int h = D / 2;
-            uint64_t best = 0;
-            int bestCell = 0;
-            float mn = 3.40282347e+38f;
+            float vmax = -1.0f, mn = 3.40282347e+38f;
             for (int idx = tid; idx < D * D; idx += NT) {
                 const int i = idx / D, j = idx - (idx / D) * D;
-                from_center_visit(s_re[i * S + j], i, j, h, idx, best, bestCell, mn);
+                if (i == 0 || j == 0) continue;
+                const float v = s_re[i * S + j];
+                vmax = fmaxf(vmax, v);
+                if (v < mn) mn = v;
+            }
+            vmax = block_max_float<NT>(vmax, s_min, tid);
+            uint64_t best = 0;
+            int bestCell = 0;
+            if (vmax >= 0.0f) {
+                for (int idx = tid; idx < D * D; idx += NT) {
+                    const int i = idx / D, j = idx - (idx / D) * D;
+                    if (i == 0 || j == 0 || !(s_re[i * S + j] == vmax)) continue;
+                    const uint32_t tie = from_center_tie(i, j, h);
+                    if (tie > best) {
+                        best = tie;
+                        bestCell = idx;
+                    }
+                }
             }
             peak_epilogue<NT, D>(p, ia, best, bestCell, mn, s_re, S, 1, s_key, s_cell, s_min, tid);
         }

@@ -131,6 +131,7 @@ __global__ __launch_bounds__(Lx<LOGM>::THREADS) void xcorr_lane_kernel(
     __shared__ uint64_t s_key[X::IAS][NW > 1 ? NW : 1];
     __shared__ int s_cell[X::IAS][NW > 1 ? NW : 1];
     __shared__ float s_min[X::IAS][NW > 1 ? NW : 1];
+    __shared__ float s_vmax[X::IAS][NW > 1 ? NW : 1];
 
     // One block of IAs per work-group: contiguous runs of work-groups per XCD
     // (work-groups are dealt round-robin to the 8 XCDs, so block b runs on XCD
@@ -367,20 +368,39 @@ __global__ __launch_bounds__(Lx<LOGM>::THREADS) void xcorr_lane_kernel(
             }
         }
         if (PEAK) {
+            // FromCenter in two phases (from_center_tie): the largest value of
+            // the IA's visited cells, then the tie keys of the cells equal to it
             constexpr int h = D / 2;
-            uint64_t best = 0;
-            int bestCell = 0;
-            float mn = 3.40282347e+38f;
+            float vmax = -1.0f, mn = 3.40282347e+38f;
             if (col_ok && c >= 1) {
                 #pragma unroll
                 for (int e = 0; e < E; e++) {
                     const int row = half * E + e;
                     if (row < 1 || row >= D) continue;
                     const float v = re[e];
+                    vmax = fmaxf(vmax, v);
                     if (v < mn) mn = v;
-                    const uint64_t key = from_center_key(v, row, c, h);
-                    if (key > best) {
-                        best = key;
+                }
+            }
+            #pragma unroll
+            for (int o = (TPI < 64 ? TPI : 64) / 2; o >= 1; o >>= 1)
+                vmax = fmaxf(vmax, __shfl_xor(vmax, o, 64));
+            if (NW > 1) {
+                if ((t & 63) == 0) s_vmax[sub][t >> 6] = vmax;
+                __syncthreads();
+                #pragma unroll
+                for (int w = 0; w < NW; w++) vmax = fmaxf(vmax, s_vmax[sub][w]);
+            }
+            uint64_t best = 0;
+            int bestCell = 0;
+            if (col_ok && c >= 1 && vmax >= 0.0f) {
+                #pragma unroll
+                for (int e = 0; e < E; e++) {
+                    const int row = half * E + e;
+                    if (row < 1 || row >= D || !(re[e] == vmax)) continue;
+                    const uint32_t tie = from_center_tie(row, c, h);
+                    if (tie > best) {
+                        best = tie;
                         bestCell = row * D + c;
                     }
                 }

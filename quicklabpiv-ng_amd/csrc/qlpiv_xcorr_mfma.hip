@@ -535,10 +535,10 @@ __global__ __launch_bounds__(64 * MF_WAVES, QLPIV_MF_MINW) void xcorr_mfma_kerne
             }
         }
         if (PEAK) {
+            // FromCenter in two phases (from_center_tie): the largest value,
+            // then the tie keys of the cells that equal it
             constexpr int h = D / 2;
-            uint64_t best = 0;
-            int bestCell = 0;
-            float mn = 3.40282347e+38f;
+            float vmax = -1.0f, mn = 3.40282347e+38f;
 #pragma unroll
             for (int rt = 0; rt < 2; rt++)
 #pragma unroll
@@ -548,23 +548,43 @@ __global__ __launch_bounds__(64 * MF_WAVES, QLPIV_MF_MINW) void xcorr_mfma_kerne
                         const int a = MT * rt + mrow<MT>(v, g), b = MT * ct + j;
                         if (a == 0 || b == 0 || a >= D || b >= D) continue;
                         const float val = R[rt][ct][v] * unscale;
+                        vmax = fmaxf(vmax, val);
                         if (val < mn) mn = val;
-                        const uint64_t key = from_center_key(val, a, b, h);
-                        if (key > best) {
-                            best = key;
-                            bestCell = a * D + b;
-                        }
                     }
 #pragma unroll
             for (int o = 32; o >= 1; o >>= 1) {
-                const uint64_t ob = shfl_xor_u64(best, o);
-                const int oc = __shfl_xor(bestCell, o, 64);
+                vmax = fmaxf(vmax, __shfl_xor(vmax, o, 64));
                 const float om = __shfl_xor(mn, o, 64);
+                if (om < mn) mn = om;
+            }
+            uint32_t best = 0;
+            int bestCell = 0;
+            if (vmax >= 0.0f) {
+#pragma unroll
+                for (int rt = 0; rt < 2; rt++)
+#pragma unroll
+                    for (int ct = 0; ct < 2; ct++)
+#pragma unroll
+                        for (int v = 0; v < NS; v++) {
+                            const int a = MT * rt + mrow<MT>(v, g), b = MT * ct + j;
+                            if (a == 0 || b == 0 || a >= D || b >= D) continue;
+                            if (R[rt][ct][v] * unscale == vmax) {
+                                const uint32_t tie = from_center_tie(a, b, h);
+                                if (tie > best) {
+                                    best = tie;
+                                    bestCell = a * D + b;
+                                }
+                            }
+                        }
+            }
+#pragma unroll
+            for (int o = 32; o >= 1; o >>= 1) {
+                const uint32_t ob = (uint32_t)__shfl_xor((int)best, o, 64);
+                const int oc = __shfl_xor(bestCell, o, 64);
                 if (ob > best) {
                     best = ob;
                     bestCell = oc;
                 }
-                if (om < mn) mn = om;
             }
             int pi = 0, pj = 0;
             if (best != 0) {
@@ -650,6 +670,7 @@ __global__ __launch_bounds__(m64::NT, 2) void xcorr_mfma64_kernel(PassParams p,
     __shared__ uint64_t s_key[4];
     __shared__ int s_cell[4];
     __shared__ float s_min[4];
+    __shared__ float s_vmax[4];
     unsigned short *__restrict__ bh = s_buf[0];
     unsigned short *__restrict__ bl = s_buf[1];
     float *__restrict__ s_plane = (float *)&s_buf[0][0];
@@ -892,9 +913,7 @@ __global__ __launch_bounds__(m64::NT, 2) void xcorr_mfma64_kernel(PassParams p,
         // ---- epilogues: plane store, FromCenter -------------------------------------
         constexpr float unscale = (float)(1 << (X::E1 + X::E2));
         float *__restrict__ out = p.planes ? p.planes + (size_t)ia * D * D : nullptr;
-        uint64_t best = 0;
-        int bestCell = 0;
-        float mn = 3.40282347e+38f;
+        float vmax = -1.0f, mn = 3.40282347e+38f;
 #pragma unroll
         for (int bt = 0; bt < 4; bt++)
 #pragma unroll
@@ -905,11 +924,35 @@ __global__ __launch_bounds__(m64::NT, 2) void xcorr_mfma64_kernel(PassParams p,
                 if (out) out[(unsigned)(a * D + b)] = val;
                 if (PEAK) {
                     s_plane[a * M + b] = val;
-                    from_center_visit(val, a, b, D / 2, a * D + b, best, bestCell, mn);
+                    if (a != 0 && b != 0) {  // FromCenter phase 1 (from_center_tie)
+                        vmax = fmaxf(vmax, val);
+                        if (val < mn) mn = val;
+                    }
                 }
             }
         if (PEAK) {
-            __syncthreads();
+            vmax = wave_max_float(vmax);
+            if ((threadIdx.x & 63) == 0) s_vmax[wave] = vmax;
+            __syncthreads();  // the plane and the per-wave maxima
+            vmax = fmaxf(fmaxf(s_vmax[0], s_vmax[1]), fmaxf(s_vmax[2], s_vmax[3]));
+            uint64_t best = 0;
+            int bestCell = 0;
+            if (vmax >= 0.0f) {
+#pragma unroll
+                for (int bt = 0; bt < 4; bt++)
+#pragma unroll
+                    for (int v = 0; v < NS; v++) {
+                        const int a = MT * wave + mrow<MT>(v, g), b = MT * bt + j;
+                        if (a == 0 || b == 0 || a >= D || b >= D) continue;
+                        if (R[bt][v] * unscale == vmax) {
+                            const uint32_t tie = from_center_tie(a, b, D / 2);
+                            if (tie > best) {
+                                best = tie;
+                                bestCell = a * D + b;
+                            }
+                        }
+                    }
+            }
             peak_epilogue<NT, D>(p, ia, best, bestCell, mn, s_plane, M, 1, s_key, s_cell, s_min);
         }
         __syncthreads();  // the buffer is rewritten by the next IA
