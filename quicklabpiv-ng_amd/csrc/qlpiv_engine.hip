@@ -210,8 +210,10 @@ struct Engine {
     size_t bigwork_bytes = 0;
     // the step table: device copy and two pinned staging buffers
     uint8_t *table_d = nullptr;
-    uint8_t *stage_h[2] = {nullptr, nullptr};
-    hipEvent_t staged[2] = {nullptr, nullptr};
+    // the step table: pinned staging buffers in turn, each rewritten once its
+    // previous copy to the device has run (ENG_STAGES steps may be queued)
+    uint8_t *stage_h[ENG_STAGES] = {};
+    hipEvent_t staged[ENG_STAGES] = {};
     int next_stage = 0;
     size_t table_bytes = 0;
     int schedule = QLP_SCHEDULE_TUNED;
@@ -282,7 +284,7 @@ void engine_destroy(Engine *e)
     }
     for (hipStream_t s : e->branch) (void)hipStreamDestroy(s);
     for (hipEvent_t x : e->ev) (void)hipEventDestroy(x);
-    for (int k = 0; k < 2; k++) {
+    for (int k = 0; k < ENG_STAGES; k++) {
         if (e->staged[k]) (void)hipEventDestroy(e->staged[k]);
         if (e->stage_h[k]) (void)hipHostFree(e->stage_h[k]);
     }
@@ -312,7 +314,7 @@ int engine_create(Context *ctx, const EngineConfig &cfg, const size_t cap[ENG_CL
     e->table_bytes = sizeof(StepHeader) + (size_t)ENG_MAX_SEGS * sizeof(StepSeg);
     int rc = check_hip(hipSetDevice(ctx->device), "hipSetDevice");
     if (!rc) rc = ealloc(e, e->table_d, e->table_bytes);
-    for (int k = 0; k < 2 && !rc; k++) {
+    for (int k = 0; k < ENG_STAGES && !rc; k++) {
         rc = check_hip(hipHostMalloc((void **)&e->stage_h[k], e->table_bytes, 0), "hipHostMalloc engine");
         if (!rc) rc = check_hip(hipEventCreateWithFlags(&e->staged[k], hipEventDisableTiming), "engine event");
     }
@@ -695,7 +697,7 @@ int engine_step(Engine *e, const std::vector<StepSeg> segs[ENG_CLASSES])
     if (total_segs == 0) return QLP_OK;
     // the staging buffer is rewritten once its previous copy has run
     const int k = e->next_stage;
-    e->next_stage ^= 1;
+    e->next_stage = (k + 1) % ENG_STAGES;
     int rc = check_hip(hipEventSynchronize(e->staged[k]), "engine staging");
     if (rc) return rc;
     StepHeader *h = (StepHeader *)e->stage_h[k];

@@ -277,6 +277,7 @@ int ensure_iwork(Context *ctx, size_t bytes);
 // pairs per step, in one persistent, graph-captured launch -----------------
 constexpr int ENG_CLASSES = 4;     // IA sizes 16, 32, 64, 128
 constexpr int ENG_MAX_SEGS = 4096; // pair levels in one step, all classes
+constexpr int ENG_STAGES = 4;      // steps whose tables may be queued at once
 enum { STEP_CLIP = 0, STEP_SHIFT = 1, STEP_MICRO = 2 };
 // One pair level of a step.
 struct StepSeg {

@@ -42,6 +42,7 @@
 namespace qlpiv {
 
 using Clock = std::chrono::steady_clock;
+constexpr int ADMIT_PER_ROUND = 32;  // admissions per round of the worker loop
 
 // The IA grids of every level of one frame size and configuration (device).
 struct GeoLevel {
@@ -74,11 +75,10 @@ struct ServeRequest {
     int next_level = 0;             // the level the next step runs
     const uint8_t *fa = nullptr, *fb = nullptr;  // device frames (own copies or the caller's)
     uint8_t *own = nullptr;         // server-owned frame copies (2 H W), or null
-    std::vector<float *> lu, lv;    // each level's field (device)
+    std::vector<float *> lu, lv;    // each level's field (device), parts of `fields`
+    float *fields = nullptr;
     float *hu = nullptr, *hv = nullptr;  // pinned copies of the last level's field
-    size_t hcap = 0;
-    hipEvent_t done_ev = nullptr;
-    bool done_enqueued = false;
+    size_t hucap = 0, hvcap = 0;         // their capacities (floats)
     size_t bytes = 0;               // device bytes held (frames and fields)
 };
 
@@ -121,8 +121,17 @@ struct qlp_server {
     std::deque<std::shared_ptr<ServeRequest>> active;  // admitted, in admission order
     int max_active = 512;
     std::multimap<size_t, float *> pinned_free;  // pinned result buffers by capacity (floats)
-    std::vector<hipEvent_t> events_free;
-    std::deque<hipEvent_t> step_events;          // steps in flight
+    std::vector<void *> pinned_slabs;            // their pinned allocations
+    uint8_t *slab_at = nullptr;                  // unused part of the newest slab
+    size_t slab_left = 0;
+    // steps in flight, oldest first: the event recorded after each step (and
+    // the field copies of the requests whose last level it ran) and those
+    // requests, which retire when it completes
+    struct StepRec {
+        hipEvent_t ev = nullptr;
+        std::vector<std::shared_ptr<ServeRequest>> done;
+    };
+    std::deque<StepRec> inflight;
     std::vector<hipEvent_t> step_events_free;
     uint64_t steps = 0;
     std::mutex mu;
@@ -389,7 +398,7 @@ int admit_batched(qlp_server *s, const std::shared_ptr<ServeRequest> &r, int &rc
     size_t state = 0;
     for (const GeoLevel &L : g->lv) {
         const size_t n = (size_t)L.nI * L.nJ;
-        state += n * 8;
+        state += ((n + 63) & ~(size_t)63) * 8;  // u and v, as allocated below
         if (n > engine_cap(s->engines[ei].eng, eng_class_of(L.N))) {
             rc = fail(QLP_ERR_BUDGET, "server: a level holds more IAs than a batch step (raise the budget)");
             return -1;
@@ -438,15 +447,17 @@ int admit_batched(qlp_server *s, const std::shared_ptr<ServeRequest> &r, int &rc
         r->fa = q.imgA;
         r->fb = q.imgB;
     }
+    // every level's u and v in one stream-ordered allocation
     r->ias = 0;
+    size_t total = 0;
+    for (const GeoLevel &L : g->lv) total += 2 * (((size_t)L.nI * L.nJ + 63) & ~(size_t)63);
+    if ((rc = check_hip(hipMallocAsync((void **)&r->fields, total * 4, st), "server field"))) return -1;
+    float *at = r->fields;
     for (const GeoLevel &L : g->lv) {
-        const size_t n = (size_t)L.nI * L.nJ;
-        float *u = nullptr, *v = nullptr;
-        if ((rc = check_hip(hipMallocAsync((void **)&u, n * 4, st), "server field")) ||
-            (rc = check_hip(hipMallocAsync((void **)&v, n * 4, st), "server field")))
-            return -1;
-        r->lu.push_back(u);
-        r->lv.push_back(v);
+        const size_t n = (size_t)L.nI * L.nJ, padded = (n + 63) & ~(size_t)63;
+        r->lu.push_back(at);
+        r->lv.push_back(at + padded);
+        at += 2 * padded;
         r->ias += n;
     }
     const GeoLevel &last = g->lv.back();
@@ -464,23 +475,25 @@ int admit_batched(qlp_server *s, const std::shared_ptr<ServeRequest> &r, int &rc
 void release_batched(qlp_server *s, ServeRequest &r)
 {
     hipStream_t st = estream(s);
-    for (float *p : r.lu) (void)hipFreeAsync(p, st);
-    for (float *p : r.lv) (void)hipFreeAsync(p, st);
+    if (r.fields) (void)hipFreeAsync(r.fields, st);
+    r.fields = nullptr;
     r.lu.clear();
     r.lv.clear();
     if (r.own) (void)hipFreeAsync(r.own, st);
     r.own = nullptr;
-    if (r.hu) s->pinned_free.emplace(r.hcap, r.hu);
-    if (r.hv) s->pinned_free.emplace(r.hcap, r.hv);
+    if (r.hu) s->pinned_free.emplace(r.hucap, r.hu);
+    if (r.hv) s->pinned_free.emplace(r.hvcap, r.hv);
     r.hu = r.hv = nullptr;
-    if (r.done_ev) s->events_free.push_back(r.done_ev);
-    r.done_ev = nullptr;
+    r.hucap = r.hvcap = 0;
     if (r.geo) r.geo->users--;
     r.geo = nullptr;
     s->in_use -= r.bytes;
     r.bytes = 0;
 }
 
+// A pinned result buffer of at least n floats: a free one, else carved from a
+// pinned slab (8 MiB or the buffer's size), so that host allocations, which
+// wait for the device, stay rare.  Capacities are powers of two.
 int pinned_get(qlp_server *s, size_t n, float **out, size_t &cap)
 {
     auto it = s->pinned_free.lower_bound(n);
@@ -492,48 +505,65 @@ int pinned_get(qlp_server *s, size_t n, float **out, size_t &cap)
     }
     size_t c = 1024;
     while (c < n) c <<= 1;
+    const size_t bytes = c * 4;
+    if (s->slab_left < bytes) {
+        const size_t slab = std::max<size_t>(bytes, (size_t)8 << 20);
+        void *p = nullptr;
+        if (int rc = check_hip(hipHostMalloc(&p, slab, hipHostMallocDefault), "server pinned buffer")) return rc;
+        s->pinned_slabs.push_back(p);
+        s->slab_at = (uint8_t *)p;
+        s->slab_left = slab;
+    }
+    *out = (float *)s->slab_at;
     cap = c;
-    return check_hip(hipHostMalloc((void **)out, c * 4, hipHostMallocDefault), "server pinned buffer");
+    s->slab_at += bytes;
+    s->slab_left -= bytes;
+    return QLP_OK;
 }
 
-// Retires the batched requests whose last level and field copy have completed.
+// Retires the requests of the completed steps (oldest first): their last
+// level and field copy have run.
 bool retire_batched(qlp_server *s)
 {
     bool progress = false;
-    for (auto it = s->active.begin(); it != s->active.end();) {
-        const std::shared_ptr<ServeRequest> r = *it;
-        if (!r->done_enqueued || hipEventQuery(r->done_ev) != hipSuccess) {
-            ++it;
-            continue;
+    while (!s->inflight.empty()) {
+        qlp_server::StepRec &rec = s->inflight.front();
+        if (rec.ev && hipEventQuery(rec.ev) != hipSuccess) break;
+        for (const std::shared_ptr<ServeRequest> &r : rec.done) {
+            try {
+                const size_t n = (size_t)r->nI * r->nJ;
+                r->u.assign(r->hu, r->hu + n);
+                r->v.assign(r->hv, r->hv + n);
+                r->t_done = Clock::now();
+                r->status = 2;
+                s->completed++;
+                s->ias_done += r->ias;
+            } catch (...) {
+                fail_request(s, r, fail_exception(), "server: result");
+            }
+            release_batched(s, *r);
         }
-        try {
-            const size_t n = (size_t)r->nI * r->nJ;
-            r->u.assign(r->hu, r->hu + n);
-            r->v.assign(r->hv, r->hv + n);
-            r->t_done = Clock::now();
-            r->status = 2;
-            s->completed++;
-            s->ias_done += r->ias;
-        } catch (...) {
-            fail_request(s, r, fail_exception(), "server: result");
+        if (!rec.done.empty()) {
+            s->active.erase(std::remove_if(s->active.begin(), s->active.end(),
+                                           [](const std::shared_ptr<ServeRequest> &r) {
+                                               return r->status != 1;
+                                           }),
+                            s->active.end());
+            progress = true;
         }
-        release_batched(s, *r);
-        it = s->active.erase(it);
-        progress = true;
+        if (rec.ev) s->step_events_free.push_back(rec.ev);
+        s->inflight.pop_front();
     }
     return progress;
 }
 
 // One step: the next level of every admitted request that has one, as far as
 // the engines' class capacities allow, then the field copy of every request
-// whose last level ran.  At most two steps are in flight.
+// whose last level ran.  At most ENG_STAGES - 1 steps are in flight, so the
+// host prepares the next steps while the device runs the queued ones.
 bool step_batched(qlp_server *s)
 {
-    while (!s->step_events.empty() && hipEventQuery(s->step_events.front()) == hipSuccess) {
-        s->step_events_free.push_back(s->step_events.front());
-        s->step_events.pop_front();
-    }
-    if (s->step_events.size() >= 2) return false;
+    if ((int)s->inflight.size() >= ENG_STAGES - 1) return false;
     const size_t ne = s->engines.size();
     std::vector<std::array<std::vector<StepSeg>, ENG_CLASSES>> segs(ne);
     std::vector<std::array<size_t, ENG_CLASSES>> count(ne);
@@ -594,26 +624,18 @@ bool step_batched(qlp_server *s)
             }
         }
     }
+    qlp_server::StepRec rec;
     for (auto &r : stepped) {
         if (r->next_level < 0) continue;
         r->next_level++;
         if (r->next_level < (int)r->lu.size()) continue;
-        // last level: the field to pinned host memory, then the request's event
+        // last level: the field to pinned host memory; the request retires with
+        // this step's event
         const size_t n = (size_t)r->nI * r->nJ;
-        int rc = pinned_get(s, n, &r->hu, r->hcap);
-        size_t cap2 = 0;
-        if (!rc) rc = pinned_get(s, n, &r->hv, cap2);
-        if (!rc && !r->done_ev) {
-            if (!s->events_free.empty()) {
-                r->done_ev = s->events_free.back();
-                s->events_free.pop_back();
-            } else {
-                rc = check_hip(hipEventCreateWithFlags(&r->done_ev, hipEventDisableTiming), "server event");
-            }
-        }
+        int rc = pinned_get(s, n, &r->hu, r->hucap);
+        if (!rc) rc = pinned_get(s, n, &r->hv, r->hvcap);
         if (!rc) rc = check_hip(hipMemcpyAsync(r->hu, r->lu.back(), n * 4, hipMemcpyDeviceToHost, st), "server result");
         if (!rc) rc = check_hip(hipMemcpyAsync(r->hv, r->lv.back(), n * 4, hipMemcpyDeviceToHost, st), "server result");
-        if (!rc) rc = check_hip(hipEventRecord(r->done_ev, st), "server result");
         if (rc) {
             fail_request(s, r, rc, "server: result copy");
             (void)hipStreamSynchronize(st);
@@ -621,16 +643,21 @@ bool step_batched(qlp_server *s)
             s->active.erase(std::find(s->active.begin(), s->active.end(), r));
             continue;
         }
-        r->done_enqueued = true;
+        rec.done.push_back(r);
     }
-    hipEvent_t ev = nullptr;
     if (!s->step_events_free.empty()) {
-        ev = s->step_events_free.back();
+        rec.ev = s->step_events_free.back();
         s->step_events_free.pop_back();
-    } else if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
-        ev = nullptr;
+    } else if (hipEventCreateWithFlags(&rec.ev, hipEventDisableTiming) != hipSuccess) {
+        rec.ev = nullptr;
     }
-    if (ev && hipEventRecord(ev, st) == hipSuccess) s->step_events.push_back(ev);
+    if (!rec.ev || hipEventRecord(rec.ev, st) != hipSuccess) {
+        // no event: wait here, the step then counts as complete
+        (void)hipStreamSynchronize(st);
+        if (rec.ev) s->step_events_free.push_back(rec.ev);
+        rec.ev = nullptr;
+    }
+    s->inflight.push_back(std::move(rec));
     s->steps++;
     return true;
 }
@@ -658,8 +685,12 @@ void worker_loop(qlp_server *s)
         if (retire_batched(s)) progress = true;
         if (progress) s->cv_done.notify_all();
         // admit in arrival order while the head request fits: the batched path
-        // (memory and the in-flight limit) or a free plan slot
-        while (!s->pending.empty()) {
+        // (memory and the in-flight limit) or a free plan slot.  At most
+        // ADMIT_PER_ROUND requests per round, so that a burst of arrivals does
+        // not hold back the next step
+        int admitted = 0;
+        while (!s->pending.empty() && admitted < ADMIT_PER_ROUND) {
+            admitted++;
             const std::shared_ptr<ServeRequest> r = s->pending.front();
             if (engine_ok(r->req.cfg)) {
                 int rc = QLP_OK, got;
@@ -819,9 +850,9 @@ try {
             (void)hipFree(L.top);
             (void)hipFree(L.left);
         }
-    for (auto &b : s->pinned_free) (void)hipHostFree(b.second);
-    for (hipEvent_t e : s->events_free) (void)hipEventDestroy(e);
-    for (hipEvent_t e : s->step_events) (void)hipEventDestroy(e);
+    for (void *p : s->pinned_slabs) (void)hipHostFree(p);
+    for (auto &rec : s->inflight)
+        if (rec.ev) (void)hipEventDestroy(rec.ev);
     for (hipEvent_t e : s->step_events_free) (void)hipEventDestroy(e);
     qlp_ctx_destroy(s->ectx);
     delete s;

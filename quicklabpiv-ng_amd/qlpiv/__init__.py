@@ -789,13 +789,16 @@ class Server:
         self.h = h
         self._keep = {}
 
-    def submit(self, imgA, imgB, **config):
+    def submit(self, imgA, imgB, cfg=None, **config):
+        """Queue one frame pair; cfg: a prebuilt piv_config(...) (then no keys),
+        which saves rebuilding it for every request."""
         H, W = int(imgA.shape[0]), int(imgA.shape[1])
         on_dev = not isinstance(imgA, np.ndarray)
         if not on_dev:
             imgA = np.ascontiguousarray(imgA, np.uint8)
             imgB = np.ascontiguousarray(imgB, np.uint8)
-        req = _lib.Request(H, W, _ptr(imgA), _ptr(imgB), int(on_dev), piv_config(**config))
+        req = _lib.Request(H, W, _ptr(imgA), _ptr(imgB), int(on_dev),
+                           cfg if cfg is not None else piv_config(**config))
         t = ctypes.c_uint64()
         check(self.lib.qlp_server_submit(self.h, ctypes.byref(req), ctypes.byref(t)))
         self._keep[t.value] = (imgA, imgB)
