@@ -535,9 +535,21 @@ __global__ __launch_bounds__(64 * MF_WAVES, QLPIV_MF_MINW) void xcorr_mfma_kerne
             }
         }
         if (PEAK) {
-            // FromCenter in two phases (from_center_tie): the largest value,
-            // then the tie keys of the cells that equal it
-            constexpr int h = D / 2;
+            // FromCenter in two phases (from_center_tie) on the accumulator
+            // scale (unscale is a power of two: order and equality are kept).
+            // Visited cells are rows and columns 1 .. D-1: column b = MT ct + j
+            // leaves out j = 0 of tile 0 and j = MT-1 of tile 1 (b = D); row
+            // a = MT rt + mrow(v, g) is 0 only in slot (0, 0) of group 0 and D
+            // only in slot (1, NS-1) of the last group.
+            constexpr int h = D / 2, G = 64 / MT;
+            const bool colok[2] = {j != 0, j != MT - 1};
+            const bool row0ok = g != 0, rowDok = g != G - 1;
+            auto visited = [&](int rt, int ct, int v) -> bool {
+                bool ok = colok[ct];
+                if (rt == 0 && v == 0) ok = ok && row0ok;
+                if (rt == 1 && v == NS - 1) ok = ok && rowDok;
+                return ok;
+            };
             float vmax = -1.0f, mn = 3.40282347e+38f;
 #pragma unroll
             for (int rt = 0; rt < 2; rt++)
@@ -545,37 +557,55 @@ __global__ __launch_bounds__(64 * MF_WAVES, QLPIV_MF_MINW) void xcorr_mfma_kerne
                 for (int ct = 0; ct < 2; ct++)
 #pragma unroll
                     for (int v = 0; v < NS; v++) {
-                        const int a = MT * rt + mrow<MT>(v, g), b = MT * ct + j;
-                        if (a == 0 || b == 0 || a >= D || b >= D) continue;
-                        const float val = R[rt][ct][v] * unscale;
-                        vmax = fmaxf(vmax, val);
-                        if (val < mn) mn = val;
+                        const bool ok = visited(rt, ct, v);
+                        const float x = R[rt][ct][v];
+                        vmax = fmaxf(vmax, ok ? x : -1.0f);
+                        mn = fminf(mn, ok ? x : 3.40282347e+38f);
                     }
 #pragma unroll
             for (int o = 32; o >= 1; o >>= 1) {
                 vmax = fmaxf(vmax, __shfl_xor(vmax, o, 64));
-                const float om = __shfl_xor(mn, o, 64);
-                if (om < mn) mn = om;
+                mn = fminf(mn, __shfl_xor(mn, o, 64));
             }
+            // phase 2: each lane notes its cells equal to the maximum (almost
+            // always none or one) and computes the tie key of that one
             uint32_t best = 0;
             int bestCell = 0;
             if (vmax >= 0.0f) {
+                int cnt = 0, code = 0;
 #pragma unroll
                 for (int rt = 0; rt < 2; rt++)
 #pragma unroll
                     for (int ct = 0; ct < 2; ct++)
 #pragma unroll
                         for (int v = 0; v < NS; v++) {
-                            const int a = MT * rt + mrow<MT>(v, g), b = MT * ct + j;
-                            if (a == 0 || b == 0 || a >= D || b >= D) continue;
-                            if (R[rt][ct][v] * unscale == vmax) {
+                            const bool m = visited(rt, ct, v) && R[rt][ct][v] == vmax;
+                            cnt += m ? 1 : 0;
+                            code = m ? ((rt * 2 + ct) * NS + v) : code;
+                        }
+                if (__ballot(cnt > 1) == 0) {
+                    if (cnt == 1) {
+                        const int v = code % NS, ct = (code / NS) & 1, rt = code / (2 * NS);
+                        const int a = MT * rt + mrow<MT>(v, g), b = MT * ct + j;
+                        best = from_center_tie(a, b, h);
+                        bestCell = a * D + b;
+                    }
+                } else {  // a lane holds several equal maxima: every cell's key
+#pragma unroll
+                    for (int rt = 0; rt < 2; rt++)
+#pragma unroll
+                        for (int ct = 0; ct < 2; ct++)
+#pragma unroll
+                            for (int v = 0; v < NS; v++) {
+                                if (!(visited(rt, ct, v) && R[rt][ct][v] == vmax)) continue;
+                                const int a = MT * rt + mrow<MT>(v, g), b = MT * ct + j;
                                 const uint32_t tie = from_center_tie(a, b, h);
                                 if (tie > best) {
                                     best = tie;
                                     bestCell = a * D + b;
                                 }
                             }
-                        }
+                }
             }
 #pragma unroll
             for (int o = 32; o >= 1; o >>= 1) {
@@ -586,45 +616,58 @@ __global__ __launch_bounds__(64 * MF_WAVES, QLPIV_MF_MINW) void xcorr_mfma_kerne
                     bestCell = oc;
                 }
             }
-            int pi = 0, pj = 0;
-            if (best != 0) {
-                pi = bestCell / D;
-                pj = bestCell % D;
-            }
-            float pv = 0.0f;
-            const int L = 2 * p.nbr_half + 1;
+            // the winner holds vmax; the neighbourhood: the row through the peak
+            // sits in one slot (uniform) of one lane group, the column through
+            // it in the G lanes of one column
+            const int pc = __builtin_amdgcn_readfirstlane(best != 0 ? bestCell : 0);
+            const int pi = pc / D, pj = pc % D;
+            const float pv = best != 0 ? vmax * unscale : 0.0f;
+            if (valid && p.nbr) {
+                const int nh = p.nbr_half, L = 2 * nh + 1;
+                float *nb = p.nbr + (size_t)ia * 2 * L;
+                const int rts = pi / MT, ms = pi % MT;
+                const int gs = (MT == 32) ? ((ms >> 2) & 1) : (ms >> 2);
+                const int vs = (MT == 32) ? (((ms >> 3) << 2) | (ms & 3)) : (ms & 3);
+                float rv[2] = {0.0f, 0.0f};
 #pragma unroll
-            for (int rt = 0; rt < 2; rt++)
+                for (int rt = 0; rt < 2; rt++)
 #pragma unroll
-                for (int ct = 0; ct < 2; ct++)
-#pragma unroll
-                    for (int v = 0; v < NS; v++) {
-                        const int a = MT * rt + mrow<MT>(v, g), b = MT * ct + j;
-                        if (a >= D || b >= D) continue;
-                        const float val = R[rt][ct][v] * unscale;
-                        if (best != 0 && a == pi && b == pj) pv = val;
-                        if (valid && p.nbr) {
-                            float *nb = p.nbr + (size_t)ia * 2 * L;
-                            const int di = a - pi, dj = b - pj;
-                            if (b == pj && di >= -p.nbr_half && di <= p.nbr_half)
-                                nb[di + p.nbr_half] = val;
-                            if (a == pi && dj >= -p.nbr_half && dj <= p.nbr_half)
-                                nb[L + dj + p.nbr_half] = val;
+                    for (int v = 0; v < NS; v++)
+                        if (rt == rts && v == vs) {
+                            rv[0] = R[rt][0][v];
+                            rv[1] = R[rt][1][v];
                         }
-                    }
-            // the peak value: one lane holds it, the others hold 0
 #pragma unroll
-            for (int o = 32; o >= 1; o >>= 1) pv += __shfl_xor(pv, o, 64);
+                for (int ct = 0; ct < 2; ct++) {
+                    const int b = MT * ct + j, dj = b - pj;
+                    if (g == gs && b < D && dj >= -nh && dj <= nh) nb[L + dj + nh] = rv[ct] * unscale;
+                }
+                const int cts = pj / MT;
+                if (j == pj % MT) {
+#pragma unroll
+                    for (int rt = 0; rt < 2; rt++)
+#pragma unroll
+                        for (int ct = 0; ct < 2; ct++) {
+                            if (ct != cts) continue;
+#pragma unroll
+                            for (int v = 0; v < NS; v++) {
+                                const int a = MT * rt + mrow<MT>(v, g), di = a - pi;
+                                if (a < D && di >= -nh && di <= nh) nb[di + nh] = R[rt][ct][v] * unscale;
+                            }
+                        }
+                }
+            }
             if (valid && lane == 0) {
                 if (p.peaks) {
                     p.peaks[(size_t)ia * 3 + 0] = (float)pi;
                     p.peaks[(size_t)ia * 3 + 1] = (float)pj;
                     p.peaks[(size_t)ia * 3 + 2] = pv;
                 }
-                if (p.min_floor) p.min_floor[ia] = mn;
+                if (p.min_floor) p.min_floor[ia] = mn * unscale;
             }
             // neighbourhood cells outside the plane are 0 (as in the exact pass)
-            if (valid && p.nbr && lane < L) {
+            if (valid && p.nbr && lane < 2 * p.nbr_half + 1) {
+                const int L = 2 * p.nbr_half + 1;
                 float *nb = p.nbr + (size_t)ia * 2 * L;
                 const int d = lane - p.nbr_half;
                 if (pi + d < 0 || pi + d >= D) nb[lane] = 0.0f;
