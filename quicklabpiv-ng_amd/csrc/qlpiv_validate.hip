@@ -466,6 +466,8 @@ __global__ __launch_bounds__(256) void valid_init_kernel(VParams p, int pairs)
     if (p.st[0].validator == QLP_VALIDATOR_MEDIAN)
         inv = median_invalid(p, base, t / p.nJ, t % p.nJ, tu, tv, p.st[0].threshold, p.st[0].eps);
     p.flags[gid] = inv ? F_INVALID : F_LOCKED;
+    // a locked tile ends valid; the sweep writes the flags of the active ones
+    if (p.invalid) p.invalid[gid] = 0;
     if (inv) {
         const int slot = atomicAdd(p.act_count + pair, 1);
         p.act[base + slot] = t;
@@ -586,18 +588,17 @@ __global__ __launch_bounds__(SWEEP_THREADS) void valid_sweep_kernel(VParams p)
     int inv = 0;
     for (int a = threadIdx.x; a < cnt; a += SWEEP_THREADS) {
         const size_t gt = base + act[a];
-        if (p.flags[gt] & F_INVALID) {
+        const bool bad = (p.flags[gt] & F_INVALID) != 0;
+        if (bad) {
             p.u[gt] = 0.0f;
             p.v[gt] = 0.0f;
             inv++;
         }
+        if (p.invalid) p.invalid[gt] = bad ? 1 : 0;  // the other tiles: 0 from the initializer
     }
     if (inv) atomicAdd(&s_inv, inv);
     if (err) atomicOr(&s_err, 2);
     __syncthreads();
-    if (p.invalid)
-        for (int t = threadIdx.x; t < p.n; t += SWEEP_THREADS)
-            p.invalid[base + t] = (p.flags[base + t] & F_INVALID) ? 1 : 0;
     if (p.stats && threadIdx.x == 0) {
         int *st = p.stats + (size_t)pair * 4;
         st[0] = sweeps;
