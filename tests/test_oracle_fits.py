@@ -162,6 +162,30 @@ def test_guo_converges_to_reweighting_fixed_point(n):
         assert residual(O.guo_coefs(xs, ys, 1)) > 1e-8
 
 
+def test_guo_three_samples_iterations_reproduce_the_first():
+    """With 3 samples the weighted normal equations are square: the first solve
+    is the interpolant, and the 20 iterations of the reference reproduce it up
+    to FP64 rounding (the device stops after the first successful solve for
+    3-pixel fits).  The fitted centre agrees to 1e-5 px, the amplitude to a
+    relative 1e-6; a 5-sample fit does move between its iterations (the
+    shortcut is specific to 3 samples)."""
+    rng = np.random.default_rng(33)
+    for _ in range(300):
+        xs, ys = noisy_gaussian(rng, 3, 0.05)
+        c1, c20 = O.guo_coefs(xs, ys, 1), O.guo_coefs(xs, ys, 20)
+        m1, m20 = -c1[1] / (2 * c1[2]), -c20[1] / (2 * c20[2])
+        assert abs(np.float32(m1) - np.float32(m20)) <= 1e-5
+        a1 = np.exp(c1[0] - c1[1] ** 2 / (4 * c1[2]))
+        a20 = np.exp(c20[0] - c20[1] ** 2 / (4 * c20[2]))
+        assert abs(a1 - a20) <= 1e-6 * abs(a20)
+    moved = 0
+    for _ in range(100):
+        xs, ys = noisy_gaussian(rng, 5, 0.08)
+        c1, c20 = O.guo_coefs(xs, ys, 1), O.guo_coefs(xs, ys, 20)
+        moved += abs(c1[1] / (2 * c1[2]) - c20[1] / (2 * c20[2])) > 1e-6
+    assert moved > 50
+
+
 def test_guo_interpolator_uses_the_iterated_coefficients():
     """or_guo's mean and amplitude come from the coefficients above."""
     rng = np.random.default_rng(7)
