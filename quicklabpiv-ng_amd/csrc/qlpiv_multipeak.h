@@ -98,22 +98,25 @@ __device__ __forceinline__ void block_max(const float *__restrict__ m, int rs, i
                 const int r = top + oi, cc = left + oj;
                 val[oi * KW + oj] = (r < D && cc < D) ? m[r * rs + cc * cs] : 0.0f;
             }
+        // the same comparisons in the same order, tracking one cell index; a
+        // cell past the edge reads 0, which never exceeds bv (>= 0)
+        int first = -1;
 #pragma unroll
         for (int oi = 0; oi < KW; oi++)
 #pragma unroll
             for (int oj = 0; oj < KW; oj++) {
-                const int r = top + oi, cc = left + oj;
-                if (r < D && cc < D) {
-                    const float v = val[oi * KW + oj];
-                    if (v > bv) {
-                        bv = v;
-                        bi = r;
-                        bj = cc;
-                        upd = 1;
-                    }
-                    if (v < mn) mn = v;
+                const float v = val[oi * KW + oj];
+                if (v > bv) {
+                    bv = v;
+                    first = oi * KW + oj;
                 }
+                if (top + oi < D && left + oj < D && v < mn) mn = v;
             }
+        if (first >= 0) {
+            bi = top + first / KW;
+            bj = left + first % KW;
+            upd = 1;
+        }
     } else {
         for (int oi = 0; oi < k; oi++) {
             const int r = top + oi;
