@@ -19,14 +19,19 @@
 // hi*hi + hi*lo + lo*hi.  Power-of-two scalings keep fp16 operands in range;
 // they are exact and undone on the fp32 result.
 //
-// One wavefront owns one IA.  The chain is laid out so that an MFMA result
-// feeds the next product as its B operand straight from registers: the K index
-// of every product follows the row order of the accumulator layout (kphys),
-// and the constant A operands (Hf, Hout^T) are stored in that order.  Only the
-// two transposes of the 2D transforms (after Hf a, after Hout^T C^T) go
-// through LDS.  The constant fragments are staged in LDS once per work-group.
-// Operand fragments are kept as four packed 32-bit words (two 16-bit values
-// each), converted in pairs.
+// One wavefront owns one IA (N = 16, 32) and the whole chain stays in its
+// registers.  An MFMA contracts over the rows of an accumulator whether that
+// accumulator enters as the A or the B operand (the K index of every product
+// follows the row order of the accumulator layout, kphys, and the constant
+// fragments are stored in that order); as the B operand its lane index stays
+// the output column, as the A operand it becomes the output row.  The chain
+//     P = x^T Hf^T  (pixels as A)     Q = Hf P = E^T  (P as B)
+//     Y = C Hout    (spectrum as A)   R = Hout^T Y    (Y as B)
+// therefore needs no transpose.  An A fragment and a B fragment of one matrix
+// hold the same values per lane, so one table of Hf and Hout^T fragments
+// serves both roles; it is staged in LDS once per work-group.  Operand
+// fragments are kept as four packed 32-bit words (two 16-bit values each),
+// converted in pairs.
 #include "qlpiv_device.h"
 #include "qlpiv_epilogue.h"
 #include "qlpiv_internal.h"
@@ -60,10 +65,6 @@ struct Mf {
     static constexpr int RT = M / MT;                // tiles per matrix side (2)
     static constexpr int KN = (N + KB - 1) / KB;     // K blocks over the tile size
     static constexpr int KM = M / KB;                // K blocks over the transform size
-    static constexpr int SV = 36;                    // LDS row stride of a V tile (halves)
-    static constexpr int SY = (MT == 32) ? 20 : 36;  // LDS row stride of a Y block
-    static constexpr int YCT = (MT == 32) ? 1 : 2;   // Y column tiles per K block
-    static constexpr int XS = (MT * SV > M * SY) ? MT * SV : M * SY;  // per-wave LDS (halves)
     static constexpr int NF_HF = RT * KN;            // Hf fragments
     static constexpr int NF_HO = RT * KM;            // Hout^T fragments
     static constexpr int NF = NF_HF + NF_HO;
@@ -73,6 +74,8 @@ struct Mf {
     // instead pushes its small coefficients into fp16 subnormals (measured:
     // 2e-6 instead of 7e-7 of the plane maximum).
     static constexpr int E1 = (N == 16) ? 4 : (N == 32 ? 6 : 8);
+    // 2^-ES on each image's E gives their products 4 / M^2 / 2^E1 (N = 16, 32)
+    static constexpr int ES = (N == 16) ? 6 : 8;
     static constexpr int E2 = (M == 32) ? 5 : (M == 64 ? 6 : 7);
 };
 
@@ -193,17 +196,6 @@ __device__ __forceinline__ u4 bfrag_from_lds(const unsigned short *__restrict__ 
     return (u4){x0[0], x0[1], x1[0], x1[1]};
 }
 
-// Orders this wavefront's LDS writes before its later LDS reads by other lanes
-// of the same wavefront.  A wavefront's LDS instructions execute in issue
-// order, so only the compiler has to be kept from moving accesses across (the
-// fences); the per-wave LDS regions need no work-group barrier.
-__device__ __forceinline__ void wave_lds_sync()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 // f(integral_constant<I>) for I = B .. E-1
 template <int B, int E, typename F>
 __device__ __forceinline__ void static_for(F &&f)
@@ -224,20 +216,17 @@ __global__ __launch_bounds__(64 * MF_WAVES, QLPIV_MF_MINW) void xcorr_mfma_kerne
 {
     using X = Mf<N>;
     static_assert(X::RT == 2, "one wavefront per IA holds 2 x 2 tiles");
-    constexpr int M = X::M, D = X::D, MT = X::MT, NS = X::NS;
+    constexpr int D = X::D, MT = X::MT, NS = X::NS;
     const PassParams &p = mp.p;
-    __shared__ u4 s_frag[X::NF * 2 * 64];
-    __shared__ __attribute__((aligned(16))) unsigned short s_xh[MF_WAVES][X::XS];
-    __shared__ __attribute__((aligned(16))) unsigned short s_xl[MF_WAVES][X::XS];
+    // the constant fragments, then one entry of row sums of Hf per lane
+    __shared__ u4 s_frag[(X::NF * 2 + 1) * 64];
 
-    for (int k = threadIdx.x; k < X::NF * 2 * 64; k += 64 * MF_WAVES) s_frag[k] = mp.frags[k];
+    for (int k = threadIdx.x; k < (X::NF * 2 + 1) * 64; k += 64 * MF_WAVES) s_frag[k] = mp.frags[k];
     __syncthreads();
 
     // wave-uniform (scalar) wave index: the IA index and its global base
     // pointers are then scalar, and loads/stores take 32-bit lane offsets
     const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
-    unsigned short *__restrict__ xh = s_xh[wave];
-    unsigned short *__restrict__ xl = s_xl[wave];
     // a persistent launch reads the IA count from device memory
     const int n = pass_count(p);
     if (p.n_dev) n_iter = ((n + MF_WAVES - 1) / MF_WAVES + (int)gridDim.x - 1) / (int)gridDim.x;
@@ -255,7 +244,7 @@ __global__ __launch_bounds__(64 * MF_WAVES, QLPIV_MF_MINW) void xcorr_mfma_kerne
         const int ia = (it * (int)gridDim.x + (int)blockIdx.x) * MF_WAVES + wave;
         const bool valid = ia < n;
 
-        // ---- tiles in the B layout of Hf a: element (kb, s) = row kphys, column j
+        // ---- tiles in the A layout of x^T: element (kb, s) = row kphys, column j
         float ta[X::KN][8], tg[X::KN][8];
         float mA = 16.0f, mB = 16.0f;
         if (FROM_IMAGE) {
@@ -339,111 +328,109 @@ __global__ __launch_bounds__(64 * MF_WAVES, QLPIV_MF_MINW) void xcorr_mfma_kerne
                 }
         }
         if (p.mode == QLP_NORM_A_MAX) mB = mA;
+        // the pixels as packed 16-bit operands (small integers: exact)
+        u4 xt[2][X::KN];
+#pragma unroll
+        for (int kb = 0; kb < X::KN; kb++)
+#pragma unroll
+            for (int s = 0; s < 8; s += 2) {
+                unsigned h, l;
+                split2<BF>(ta[kb][s], ta[kb][s + 1], h, l);
+                xt[0][kb][s >> 1] = h;
+                split2<BF>(tg[kb][s], tg[kb][s + 1], h, l);
+                xt[1][kb][s >> 1] = h;
+            }
 
-        // ---- forward: W = Hf (Hf x)^T = E^T for x = a, g ------------------------
+        // ---- forward: E = Hf x' Hf^T for x' = x * (16 / max) + 1, x = a, g ------
+        // P = x'^T Hf^T with the pixels as the A operand (lane = tile column n,
+        // K = tile row m).  The pixels are small integers, exact in fp16 and
+        // bf16, so they need no lo part; the normalisation is affine and is
+        // applied to P:  x'^T Hf^T = (16 / max) (x^T Hf^T + (max / 16) 1 h^T)
+        // with h[rho] = sum_m Hf[rho][m], the second term the accumulator's
+        // initial value.  Q = Hf P then takes P from the accumulator as its B
+        // operand (K = n): Q[sigma][rho] = E[rho][sigma], lane = rho.
+        // E leaves scaled by 2^-ES (folded into the scaling of P), so that the
+        // spectrum's products carry the inverse's 4 / M^2 and the range
+        // scaling 2^-E1 (4 / M^2 / 2^E1 = 2^-2ES).
         Acc<MT> W[2][2][2];  // [image][rt][ct]
+        const float4 hrow = __builtin_bit_cast(float4, s_frag[X::NF * 2 * 64 + lane]);
 #pragma unroll
         for (int im = 0; im < 2; im++) {
-            u4 bh[X::KN], bl[X::KN];
+            const float mx = im == 0 ? mA : mB;
             // x / max * 16 + 1 (Matrix.copyMatrixToArrayAndNormalizeAndOffset
-            // :175-181) as x * (16 / max) + 1: one division per tile; within an
-            // ulp of the reference's value, far below the form's error
-            const float sx = 16.0f / (im == 0 ? mA : mB);
-#pragma unroll
-            for (int kb = 0; kb < X::KN; kb++)
-#pragma unroll
-                for (int s = 0; s < 8; s += 2) {
-                    float y[2];
-#pragma unroll
-                    for (int e = 0; e < 2; e++) {
-                        const int r = kphys<MT>(kb, g, s + e);
-                        const float x = im == 0 ? ta[kb][s + e] : tg[kb][s + e];
-                        y[e] = r < N ? x * sx + 1.0f : 0.0f;
-                    }
-                    unsigned h, l;
-                    split2<BF>(y[0], y[1], h, l);
-                    bh[kb][s >> 1] = h;
-                    bl[kb][s >> 1] = l;
-                }
-            // V = Hf x (M x N) one row tile ct at a time, to LDS as [rho - MT ct][m];
-            // W = Hf V^T (M x M), column tile ct, reads it transposed
+            // :175-181) as (x + max / 16) * (16 / max): one division per tile,
+            // within an ulp or two of the reference's value, far below the
+            // form's error
+            const float sx = 16.0f / mx * (1.0f / (float)(1 << X::ES)), inv = mx * 0.0625f;
 #pragma unroll
             for (int ct = 0; ct < 2; ct++) {
-                Acc<MT> acc = {};
+                const float h0 = (ct == 0 ? hrow.x : hrow.y) * inv;
+                Acc<MT> acc;
 #pragma unroll
-                for (int kb = 0; kb < X::KN; kb++)
-                    acc = mma3<BF, SPLIT, MT>(frag(ct * X::KN + kb, 0), frag(ct * X::KN + kb, 1),
-                                              bh[kb], bl[kb], acc);
+                for (int v = 0; v < NS; v++) acc[v] = h0;
+#pragma unroll
+                for (int kb = 0; kb < X::KN; kb++) {
+                    if constexpr (SPLIT) acc = mma<BF, MT>(xt[im][kb], frag(ct * X::KN + kb, 1), acc);
+                    acc = mma<BF, MT>(xt[im][kb], frag(ct * X::KN + kb, 0), acc);
+                }
+                u4 ph[X::KN], pl[X::KN];  // P's column tile ct as B fragments
+#pragma unroll
+                for (int kb = 0; kb < X::KN; kb++) ph[kb] = pl[kb] = (u4){0u, 0u, 0u, 0u};
 #pragma unroll
                 for (int v = 0; v < NS; v += 2) {
                     unsigned h, l;
-                    split2<BF>(acc[v], acc[v + 1], h, l);
-                    const int r = mrow<MT>(v, g);  // rows r and r + 1
-                    xh[r * X::SV + j] = (unsigned short)h;
-                    xl[r * X::SV + j] = (unsigned short)l;
-                    xh[(r + 1) * X::SV + j] = (unsigned short)(h >> 16);
-                    xl[(r + 1) * X::SV + j] = (unsigned short)(l >> 16);
+                    split2<BF>(acc[v] * sx, acc[v + 1] * sx, h, l);
+                    ph[acc_kb<MT>(0, v)][acc_s<MT>(0, v) >> 1] = h;
+                    pl[acc_kb<MT>(0, v)][acc_s<MT>(0, v) >> 1] = l;
                 }
-                wave_lds_sync();
 #pragma unroll
                 for (int rt = 0; rt < 2; rt++) {
                     Acc<MT> w = {};
-                    static_for<0, X::KN>([&](auto kc) {
-                        constexpr int kb = decltype(kc)::value;
-                        w = mma3<BF, SPLIT, MT>(
-                            frag(rt * X::KN + kb, 0), frag(rt * X::KN + kb, 1),
-                            bfrag_from_lds<MT, X::SV, kb, 0, N>(xh, j, g),
-                            bfrag_from_lds<MT, X::SV, kb, 0, N>(xl, j, g), w);
-                    });
+#pragma unroll
+                    for (int kb = 0; kb < X::KN; kb++)
+                        w = mma3<BF, SPLIT, MT>(frag(rt * X::KN + kb, 0), frag(rt * X::KN + kb, 1),
+                                                ph[kb], pl[kb], w);
                     W[im][rt][ct] = w;
                 }
-                wave_lds_sync();
             }
         }
 
         // ---- spectrum: real-basis coefficients of conj(A^) G^ / M^2 ------------
         // W[.][rt][ct] at (lane, slot) = E[rho][sigma] with rho = MT ct + j
         // (row frequency k = j) and sigma = MT rt + row (column frequency l):
-        // CC = [0][0], SC = [0][1], CS = [1][0], SS = [1][1].  k = 0 or l = 0
-        // hold the two edge frequencies (0 and M/2) that have no sine partner.
-        constexpr float sg = 4.0f / (float)(M * M) / (float)(1 << X::E1);
-        constexpr float se = 2.0f / (float)(M * M) / (float)(1 << X::E1);
-        constexpr float sc = 1.0f / (float)(M * M) / (float)(1 << X::E1);
-        u4 ch[2][X::KM], cl[2][X::KM];  // C^T as B fragments [ct][kb]
+        // CC = [0][0], SC = [0][1], CS = [1][0], SS = [1][1].  Lane j = 0 holds
+        // the two edge row frequencies k = 0 (ct 0) and M/2 (ct 1), which have
+        // no sine partner, and slot 0 of lane group 0 the edge column
+        // frequencies l = 0 (rt 0) and M/2 (rt 1).  There each output is the
+        // general formula with A's terms substituted: the edge line's own pair
+        // in the (CC, CS) or (CC, SC) roles and zeros for the missing partners.
+        // The edge weights (1/2 per edge axis of 4 / M^2) are halvings of
+        // Hout's rows 0 and M/2.
+        u4 ch[2][X::KM], cl[2][X::KM];  // C as A fragments [ct][kb]
+        const bool kk = (j == 0);
 #pragma unroll
         for (int v0 = 0; v0 < NS; v0 += 2) {
             float o[2][2][2];  // [slot v0 + e][rt][ct]
 #pragma unroll
             for (int e = 0; e < 2; e++) {
                 const int v = v0 + e;
-                const bool kk = (j == 0), ll = (mrow<MT>(v, g) == 0);
+                const bool ll = (v == 0) && (g == 0);  // mrow(v, g) == 0
+                const bool edge = kk || ll;
                 const float CC = W[0][0][0][v], SC = W[0][0][1][v], CS = W[0][1][0][v],
                             SS = W[0][1][1][v];
                 const float cc = W[1][0][0][v], scg = W[1][0][1][v], cs = W[1][1][0][v],
                             ss = W[1][1][1][v];
-                if (!kk && !ll) {
-                    o[e][0][0] = sg * (CC * cc + SS * ss + SC * scg + CS * cs);
-                    o[e][1][1] = sg * (CC * ss + SS * cc - SC * cs - CS * scg);
-                    o[e][0][1] = -sg * (SC * cc - CC * scg + SS * cs - CS * ss);
-                    o[e][1][0] = sg * (CC * cs - CS * cc + SC * ss - SS * scg);
-                } else if (kk && !ll) {
-                    // frequencies k = 0 (ct 0) and k = M/2 (ct 1): CC = [0][ct], CS = [1][ct]
-                    o[e][0][0] = se * (CC * cc + CS * cs);
-                    o[e][1][0] = se * (CC * cs - CS * cc);
-                    o[e][0][1] = se * (SC * scg + SS * ss);
-                    o[e][1][1] = se * (SC * ss - SS * scg);
-                } else if (!kk && ll) {
-                    // frequencies l = 0 (rt 0) and l = M/2 (rt 1): CC = [rt][0], SC = [rt][1]
-                    o[e][0][0] = se * (CC * cc + SC * scg);
-                    o[e][0][1] = -se * (SC * cc - CC * scg);
-                    o[e][1][0] = se * (CS * cs + SS * ss);
-                    o[e][1][1] = -se * (SS * cs - CS * ss);
-                } else {
-                    o[e][0][0] = sc * CC * cc;
-                    o[e][0][1] = sc * SC * scg;
-                    o[e][1][0] = sc * CS * cs;
-                    o[e][1][1] = sc * SS * ss;
-                }
+                // A's (CC, CS, SC, SS) roles for the outputs 00, 10, 01, 11
+                const float ssm = edge ? 0.0f : SS;
+                const float a00[4] = {CC, ll ? 0.0f : CS, kk ? 0.0f : SC, ssm};
+                const float a10[4] = {ll ? CS : CC, ll ? 0.0f : CS, kk ? 0.0f : (ll ? SS : SC), ssm};
+                const float a01[4] = {kk ? SC : CC, ll ? 0.0f : (kk ? SS : CS), kk ? 0.0f : SC, ssm};
+                const float a11[4] = {kk ? (ll ? SS : SC) : (ll ? CS : CC), ll ? 0.0f : (kk ? SS : CS),
+                                      kk ? 0.0f : (ll ? SS : SC), ssm};
+                o[e][0][0] = a00[0] * cc + a00[3] * ss + a00[2] * scg + a00[1] * cs;
+                o[e][1][1] = a11[0] * ss + a11[3] * cc - a11[2] * cs - a11[1] * scg;
+                o[e][0][1] = a01[0] * scg - a01[2] * cc + a01[1] * ss - a01[3] * cs;
+                o[e][1][0] = a10[0] * cs - a10[1] * cc + a10[2] * ss - a10[3] * scg;
             }
 #pragma unroll
             for (int rt = 0; rt < 2; rt++)
@@ -456,63 +443,47 @@ __global__ __launch_bounds__(64 * MF_WAVES, QLPIV_MF_MINW) void xcorr_mfma_kerne
                 }
         }
 
-        // ---- inverse: Y = Hout^T C^T, then R = Hout^T Y^T ------------------------
-        // R accumulates over the K blocks of rho.  For each K block the Y columns
-        // it needs (16 rho of one Y column tile for MT 32, all 32 for MT 16) go
-        // to LDS as [p][rho - KB kb], where the second product reads them
-        // transposed; a Y column tile is computed once, before its first K block.
-        Acc<MT> R[2][2] = {};
-        Acc<MT> Yt[2][X::YCT];  // Y tiles [rt][column tile of the current block]
-        static_for<0, X::KM>([&](auto kc) {
-            constexpr int kb = decltype(kc)::value;
-            constexpr int ct0 = (MT == 32) ? (kb >> 1) : 0, half = (MT == 32) ? (kb & 1) : 0;
-            if constexpr (half == 0) {
+        // ---- inverse: Y = C Hout, then R = Hout^T Y -------------------------------
+        // C enters as the A operand (lane = rho, K = sigma), Y leaves with
+        // lane = q and rho in the accumulator rows, which the second product
+        // takes as its K index: no transpose.
+        u4 yh[2][X::KM], yl[2][X::KM];  // Y as B fragments [qt][kb]
 #pragma unroll
-                for (int rt = 0; rt < 2; rt++)
+        for (int qt = 0; qt < 2; qt++)
 #pragma unroll
-                    for (int c2 = 0; c2 < X::YCT; c2++) {
-                        Acc<MT> acc = {};
+            for (int kb = 0; kb < X::KM; kb++) yh[qt][kb] = yl[qt][kb] = (u4){0u, 0u, 0u, 0u};
 #pragma unroll
-                        for (int k = 0; k < X::KM; k++) {
-                            const int f = X::NF_HF + rt * X::KM + k;
-                            acc = mma3<BF, SPLIT, MT>(frag(f, 0), frag(f, 1), ch[ct0 + c2][k],
-                                                      cl[ct0 + c2][k], acc);
-                        }
-                        Yt[rt][c2] = acc * (1.0f / (float)(1 << X::E2));
-                    }
-            }
-            // the lanes holding this block's rho write them (all lanes for MT 16)
-            if (MT == 16 || (j >> 4) == half) {
+        for (int ct = 0; ct < 2; ct++)
 #pragma unroll
-                for (int rt = 0; rt < 2; rt++)
+            for (int qt = 0; qt < 2; qt++) {
+                Acc<MT> acc = {};
 #pragma unroll
-                    for (int c2 = 0; c2 < X::YCT; c2++)
-#pragma unroll
-                        for (int v = 0; v < NS; v += 2) {
-                            unsigned h, l;
-                            split2<BF>(Yt[rt][c2][v], Yt[rt][c2][v + 1], h, l);
-                            const int r = MT * rt + mrow<MT>(v, g);  // rows r and r + 1
-                            const int col = MT * c2 + j - 16 * half;
-                            xh[r * X::SY + col] = (unsigned short)h;
-                            xl[r * X::SY + col] = (unsigned short)l;
-                            xh[(r + 1) * X::SY + col] = (unsigned short)(h >> 16);
-                            xl[(r + 1) * X::SY + col] = (unsigned short)(l >> 16);
-                        }
-            }
-            wave_lds_sync();
-#pragma unroll
-            for (int rt = 0; rt < 2; rt++)
-#pragma unroll
-                for (int ct = 0; ct < 2; ct++) {
-                    const int f = X::NF_HF + rt * X::KM + kb;
-                    R[rt][ct] = mma3<BF, SPLIT, MT>(
-                        frag(f, 0), frag(f, 1),
-                        bfrag_from_lds<MT, X::SY, kb, X::KB * kb, X::KB>(xh, MT * ct + j, g),
-                        bfrag_from_lds<MT, X::SY, kb, X::KB * kb, X::KB>(xl, MT * ct + j, g),
-                        R[rt][ct]);
+                for (int kb = 0; kb < X::KM; kb++) {
+                    const int f = X::NF_HF + qt * X::KM + kb;
+                    acc = mma3<BF, SPLIT, MT>(ch[ct][kb], cl[ct][kb], frag(f, 0), frag(f, 1), acc);
                 }
-            wave_lds_sync();
-        });
+#pragma unroll
+                for (int v = 0; v < NS; v += 2) {
+                    unsigned h, l;
+                    split2<BF>(acc[v] * (1.0f / (float)(1 << X::E2)),
+                               acc[v + 1] * (1.0f / (float)(1 << X::E2)), h, l);
+                    yh[qt][acc_kb<MT>(ct, v)][acc_s<MT>(ct, v) >> 1] = h;
+                    yl[qt][acc_kb<MT>(ct, v)][acc_s<MT>(ct, v) >> 1] = l;
+                }
+            }
+        Acc<MT> R[2][2];
+#pragma unroll
+        for (int rt = 0; rt < 2; rt++)
+#pragma unroll
+            for (int qt = 0; qt < 2; qt++) {
+                Acc<MT> acc = {};
+#pragma unroll
+                for (int kb = 0; kb < X::KM; kb++) {
+                    const int f = X::NF_HF + rt * X::KM + kb;
+                    acc = mma3<BF, SPLIT, MT>(frag(f, 0), frag(f, 1), yh[qt][kb], yl[qt][kb], acc);
+                }
+                R[rt][qt] = acc;
+            }
 
         // ---- epilogues: plane store, FromCenter -----------------------------------
         constexpr float unscale = (float)(1 << (X::E1 + X::E2));
@@ -1068,9 +1039,24 @@ static void build_frags(bool bf, std::vector<unsigned short> &out)
     };
     // Hf[rho][n] = h_rho(n)
     put(0, [&](int r, int nn) { return mf_basis(M, r, nn); }, M, N, X::KN, 0);
-    // Hout^T[p][sigma] = h_sigma(p - (N - 1))
-    put(1, [&](int pp, int sg) { return mf_basis(M, sg, (long)pp - (N - 1)); }, M, M, X::KM,
-        X::NF_HF);
+    // Hout^T[p][sigma] = h_sigma(p - (N - 1)); for N = 16 and 32 the edge
+    // frequencies sigma = 0 and M/2 carry their inverse weight 1/2 here
+    put(1,
+        [&](int pp, int sg) {
+            const double w = (N != 64 && (sg == 0 || sg == M / 2)) ? 0.5 : 1.0;
+            return w * mf_basis(M, sg, (long)pp - (N - 1));
+        },
+        M, M, X::KM, X::NF_HF);
+    // row sums h[rho] = sum_n Hf[rho][n], lane (i + MT g): {h[i], h[MT + i], 0, 0} as floats
+    out.resize(out.size() + 64 * 8, 0);
+    for (int lane = 0; lane < 64; lane++)
+        for (int c = 0; c < 2; c++) {
+            const int r = MT * c + lane % MT;
+            double sum = 0.0;
+            for (int nn = 0; nn < N; nn++) sum += mf_basis(M, r, nn);
+            const float f = (float)sum;
+            memcpy(&out[((size_t)X::NF * 2 * 64 + lane) * 8 + 2 * c], &f, 4);
+        }
 }
 
 static int mfma_tables(Context *ctx, int N, bool bf, const u4 **out)
