@@ -21,7 +21,9 @@ __device__ __forceinline__ int fm_round(float x) { return (int)floorf(x + 0.5f);
 // the image (which zeroes the fractional offset there), neighbour on the side
 // of the offset, bilinear weights evaluated left to right in float, value
 // rounded half up.
-template <typename T>
+// I is the type of the offsets into the frame: size_t, or unsigned where the
+// caller has bounded H * W below 2^32.
+template <typename T, typename I = size_t>
 __device__ __forceinline__ float warp_pixel(const T *__restrict__ img, int H, int W, float u,
                                             float v, int i, int j)
 {
@@ -46,10 +48,18 @@ __device__ __forceinline__ float warp_pixel(const T *__restrict__ img, int H, in
     if (oI + pI < 0 || oI + pI >= H) oI = 0;
     if (oJ + pJ < 0 || oJ + pJ >= W) oJ = 0;
     const float wU = fabsf(dU), wV = fabsf(dV);
-    const float tl = img[(size_t)pI * W + pJ];
-    const float tr = img[(size_t)pI * W + pJ + oJ];
-    const float bl = img[(size_t)(pI + oI) * W + pJ];
-    const float br = img[(size_t)(pI + oI) * W + pJ + oJ];
+    float tl, tr, bl, br;
+    if constexpr (sizeof(I) == sizeof(size_t)) {
+        tl = img[(size_t)pI * W + pJ];
+        tr = img[(size_t)pI * W + pJ + oJ];
+        bl = img[(size_t)(pI + oI) * W + pJ];
+        br = img[(size_t)(pI + oI) * W + pJ + oJ];
+    } else {
+        tl = img[(I)pI * (I)W + (I)pJ];
+        tr = img[(I)pI * (I)W + (I)(pJ + oJ)];
+        bl = img[(I)(pI + oI) * (I)W + (I)pJ];
+        br = img[(I)(pI + oI) * (I)W + (I)(pJ + oJ)];
+    }
     const float f = (1.0f - wU) * (1.0f - wV) * tl + (1.0f - wU) * wV * tr +
                     wU * (1.0f - wV) * bl + wU * wV * br;
     return (float)fm_round(f);

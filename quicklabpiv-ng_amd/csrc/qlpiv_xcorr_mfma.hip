@@ -150,8 +150,8 @@ typedef __bf16 mb2 __attribute__((ext_vector_type(2)));
 typedef float mf2 __attribute__((ext_vector_type(2)));
 
 // two values into packed hi and lo words (x0 in the low half): packed
-// conversions, three instructions per value
-template <bool BF>
+// conversions, three instructions per value (two with MIX in fp16)
+template <bool BF, bool MIX = true>
 __device__ __forceinline__ void split2(float x0, float x1, unsigned &h, unsigned &l)
 {
     const mf2 x = {x0, x1};
@@ -160,11 +160,23 @@ __device__ __forceinline__ void split2(float x0, float x1, unsigned &h, unsigned
         const mb2 lo = __builtin_convertvector(x - __builtin_convertvector(hi, mf2), mb2);
         h = __builtin_bit_cast(unsigned, hi);
         l = __builtin_bit_cast(unsigned, lo);
-    } else {
+    } else if constexpr (!MIX) {
         const mh2 hi = __builtin_convertvector(x, mh2);
         const mh2 lo = __builtin_convertvector(x - __builtin_convertvector(hi, mf2), mh2);
         h = __builtin_bit_cast(unsigned, hi);
         l = __builtin_bit_cast(unsigned, lo);
+    } else {
+        // x - hi as one mixed-precision fma per value (fp16 operand, exact
+        // product by 1, one rounding: the same value as widening and subtracting)
+        const mh2 hi = __builtin_convertvector(x, mh2);
+        h = __builtin_bit_cast(unsigned, hi);
+        float d0, d1;
+        asm("v_fma_mix_f32 %0, -%1, 1.0, %2 op_sel_hi:[1,0,0]" : "=v"(d0) : "v"(h), "v"(x0));
+        asm("v_fma_mix_f32 %0, -%1, 1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]"
+            : "=v"(d1)
+            : "v"(h), "v"(x1));
+        const mf2 d = {d0, d1};
+        l = __builtin_bit_cast(unsigned, __builtin_convertvector(d, mh2));
     }
 }
 
@@ -237,72 +249,85 @@ __global__ __launch_bounds__(64 * MF_WAVES, QLPIV_MF_MINW) void xcorr_mfma_kerne
         // plane cells and their FromCenter order) is then recomputed from one
         // register plus constants instead of being hoisted out of the loop as
         // dozens of loop-invariant registers.
+        // (The bound below keeps the lane's range known: the tile rows a lane
+        // holds are then compile-time padding or not.)
         int lane = threadIdx.x & 63;
         asm volatile("" : "+v"(lane));
+        __builtin_assume(lane >= 0 && lane < 64);
         const int j = lane % MT, g = lane / MT;
         auto frag = [&](int f, int lo) -> u4 { return s_frag[(f * 2 + lo) * 64 + lane]; };
         const int ia = (it * (int)gridDim.x + (int)blockIdx.x) * MF_WAVES + wave;
         const bool valid = ia < n;
+        const int iac = valid ? ia : 0;  // the IA whose buffers an idle wave reads
 
         // ---- tiles in the A layout of x^T: element (kb, s) = row kphys, column j
+        // Every load is unconditional, from an address clamped into its buffer,
+        // and the value is masked afterwards: no branch separates the loads,
+        // so all of them are in flight together.  An idle wave (no IA) works
+        // on the buffers of IA 0 and stores nothing.
         float ta[X::KN][8], tg[X::KN][8];
         float mA = 16.0f, mB = 16.0f;
         if (FROM_IMAGE) {
             int amax = 0, bmax = 0;
-            bool warp = false;
-            if (valid) {
-                const int per_pair = p.per_pair > 0 ? p.per_pair : n;
-                const int pair = ia / per_pair, tt = ia - pair * per_pair;
-                const size_t img_off = (size_t)pair * p.H * p.W;
-                const uint8_t *__restrict__ imA = p.imgA + img_off;
-                const uint8_t *__restrict__ imB = p.imgB + img_off;
-                const int y0 = p.top[tt], x0 = p.left[tt];
-                int by0 = y0, bx0 = x0;
-                float fu = 0.0f, fv = 0.0f;
-                warp = (p.bmode == B_WARP_MICRO) && p.du != nullptr;
-                if (p.du != nullptr) {
-                    const float u = p.du[ia], v = p.dv[ia];
-                    if (p.bmode == B_CLIP_SHIFT) {
-                        by0 = fm_round((float)y0 + u);
-                        bx0 = fm_round((float)x0 + v);
-                    } else if (warp) {
-                        const float flu = floorf(u), flv = floorf(v);
-                        by0 = (short)(y0 + (int)flu);
-                        bx0 = (short)(x0 + (int)flv);
-                        fu = u - flu;  // createMicroDisplacements :195-196
-                        fv = v - flv;
-                    }
+            const int per_pair = p.per_pair > 0 ? p.per_pair : n;
+            const int pair = iac / per_pair, tt = iac - pair * per_pair;
+            const size_t img_off = (size_t)pair * p.H * p.W;
+            const uint8_t *__restrict__ imA = p.imgA + img_off;
+            const uint8_t *__restrict__ imB = p.imgB + img_off;
+            const int y0 = p.top[tt], x0 = p.left[tt];
+            int by0 = y0, bx0 = x0;
+            float fu = 0.0f, fv = 0.0f;
+            const bool warp = (p.bmode == B_WARP_MICRO) && p.du != nullptr;
+            if (p.du != nullptr) {
+                const float u = p.du[iac], v = p.dv[iac];
+                if (p.bmode == B_CLIP_SHIFT) {
+                    by0 = fm_round((float)y0 + u);
+                    bx0 = fm_round((float)x0 + v);
+                } else if (warp) {
+                    const float flu = floorf(u), flv = floorf(v);
+                    by0 = (short)(y0 + (int)flu);
+                    bx0 = (short)(x0 + (int)flv);
+                    fu = u - flu;  // createMicroDisplacements :195-196
+                    fv = v - flv;
                 }
+            }
+            // 32-bit offsets: the launcher bounds H * W below 2^31
+            const unsigned H = (unsigned)p.H, Wd = (unsigned)p.W;
+            // one loop per B form (the form is uniform; a branch inside the
+            // loop would separate the loads)
+            auto load = [&](auto warp_c) {
+                constexpr bool WARP = decltype(warp_c)::value;
 #pragma unroll
                 for (int kb = 0; kb < X::KN; kb++)
 #pragma unroll
                     for (int s = 0; s < 8; s++) {
                         const int r = kphys<MT>(kb, g, s);
-                        int a = 0, b = 0;
-                        if (r < N) {
-                            const int ya = y0 + r, xa = x0 + j;
-                            if (ya >= 0 && ya < p.H && xa >= 0 && xa < p.W)
-                                a = imA[(size_t)ya * p.W + xa];
-                            if (warp) {
-                                b = (int)warp_pixel(imB, p.H, p.W, fu, fv, by0 + r, bx0 + j);
-                            } else {
-                                const int yb = by0 + r, xb = bx0 + j;
-                                if (yb >= 0 && yb < p.H && xb >= 0 && xb < p.W) {
-                                    b = imB[(size_t)yb * p.W + xb];
-                                    bmax = max(bmax, b);
-                                }
-                            }
-                            amax = max(amax, a);
+                        const bool row = r < N;
+                        const unsigned ya = (unsigned)(y0 + r), xa = (unsigned)(x0 + j);
+                        const bool ina = row && ya < H && xa < Wd;
+                        int a = imA[ina ? ya * Wd + xa : 0u];
+                        a = ina ? a : 0;
+                        int b;
+                        if constexpr (WARP) {
+                            b = (int)warp_pixel<uint8_t, unsigned>(imB, p.H, p.W, fu, fv, by0 + r,
+                                                                   bx0 + j);
+                            b = row ? b : 0;
+                        } else {
+                            const unsigned yb = (unsigned)(by0 + r), xb = (unsigned)(bx0 + j);
+                            const bool inb = row && yb < H && xb < Wd;
+                            b = imB[inb ? yb * Wd + xb : 0u];
+                            b = inb ? b : 0;
+                            bmax = max(bmax, b);
                         }
+                        amax = max(amax, a);
                         ta[kb][s] = (float)a;
                         tg[kb][s] = (float)b;
                     }
-            } else {
-#pragma unroll
-                for (int kb = 0; kb < X::KN; kb++)
-#pragma unroll
-                    for (int s = 0; s < 8; s++) ta[kb][s] = tg[kb][s] = 0.0f;
-            }
+            };
+            if (warp)
+                load(std::true_type{});
+            else
+                load(std::false_type{});
 #pragma unroll
             for (int o = 32; o >= 1; o >>= 1) {
                 amax = max(amax, __shfl_xor(amax, o, 64));
@@ -315,16 +340,18 @@ __global__ __launch_bounds__(64 * MF_WAVES, QLPIV_MF_MINW) void xcorr_mfma_kerne
                 mA = p.maxA[ia];
                 mB = p.maxB[ia];
             }
-            const uint8_t *a8 = p.tA + (size_t)(valid ? ia : 0) * N * N;
-            const uint8_t *b8 = p.tB + (size_t)(valid ? ia : 0) * N * N;
+            const uint8_t *a8 = p.tA + (size_t)iac * N * N;
+            const uint8_t *b8 = p.tB + (size_t)iac * N * N;
 #pragma unroll
             for (int kb = 0; kb < X::KN; kb++)
 #pragma unroll
                 for (int s = 0; s < 8; s++) {
                     const int r = kphys<MT>(kb, g, s);
-                    const bool in = valid && r < N;
-                    ta[kb][s] = in ? (float)a8[(unsigned)(r * N + j)] : 0.0f;
-                    tg[kb][s] = in ? (float)b8[(unsigned)(r * N + j)] : 0.0f;
+                    const bool in = r < N;
+                    const unsigned off = in ? (unsigned)(r * N + j) : 0u;
+                    const int a = a8[off], b = b8[off];
+                    ta[kb][s] = in ? (float)a : 0.0f;
+                    tg[kb][s] = in ? (float)b : 0.0f;
                 }
         }
         if (p.mode == QLP_NORM_A_MAX) mB = mA;
@@ -781,7 +808,7 @@ __global__ __launch_bounds__(m64::NT, 2) void xcorr_mfma64_kernel(PassParams p,
 #pragma unroll
                     for (int s = 0; s < 8; s += 2) {
                         unsigned h, l;
-                        split2<BF>(t[c][kb][s] * sx + 1.0f, t[c][kb][s + 1] * sx + 1.0f, h, l);
+                        split2<BF, false>(t[c][kb][s] * sx + 1.0f, t[c][kb][s + 1] * sx + 1.0f, h, l);
                         xh[c][kb][s >> 1] = h;
                         xl[c][kb][s >> 1] = l;
                     }
@@ -796,7 +823,7 @@ __global__ __launch_bounds__(m64::NT, 2) void xcorr_mfma64_kernel(PassParams p,
 #pragma unroll
                 for (int v = 0; v < NS; v += 2) {
                     unsigned h, l;
-                    split2<BF>(acc[v], acc[v + 1], h, l);
+                    split2<BF, false>(acc[v], acc[v + 1], h, l);
                     const int r = MT * wave + mrow<MT>(v, g), m = MT * c + j;
                     bh[r * SV + m] = (unsigned short)h;
                     bl[r * SV + m] = (unsigned short)l;
@@ -866,7 +893,7 @@ __global__ __launch_bounds__(m64::NT, 2) void xcorr_mfma64_kernel(PassParams p,
 #pragma unroll
                 for (int ci = 0; ci < 2; ci++) {
                     unsigned h, l;
-                    split2<BF>(o[0][ri][ci], o[1][ri][ci], h, l);
+                    split2<BF, false>(o[0][ri][ci], o[1][ri][ci], h, l);
                     const int rho = MT * (sc + 2 * ci) + j;
                     const int sigma = MT * (sr + 2 * ri) + mrow<MT>(v0, g);
                     *(unsigned *)&bh[rho * SC + sigma] = h;
@@ -901,7 +928,7 @@ __global__ __launch_bounds__(m64::NT, 2) void xcorr_mfma64_kernel(PassParams p,
 #pragma unroll
             for (int v = 0; v < NS; v += 2) {
                 unsigned h, l;
-                split2<BF>(Y[ct][v], Y[ct][v + 1], h, l);
+                split2<BF, false>(Y[ct][v], Y[ct][v + 1], h, l);
                 const int r = MT * wave + mrow<MT>(v, g), rho = MT * ct + j;
                 bh[r * SC + rho] = (unsigned short)h;
                 bl[r * SC + rho] = (unsigned short)l;
@@ -1139,6 +1166,10 @@ int launch_pass_mfma(Context *ctx, const PassParams &p, int form, bool from_imag
     if (p.n <= 0) return QLP_OK;
     if (p.N != 16 && p.N != 32 && p.N != 64)
         return fail(QLP_ERR_UNSUPPORTED, "mfma correlation: IA size must be 16, 32 or 64");
+    // frames are indexed with 32-bit offsets (the reference's frames are Java
+    // arrays, below 2^31 elements too)
+    if (from_image && (long long)p.H * p.W > 0x7fffffffLL)
+        return fail(QLP_ERR_UNSUPPORTED, "mfma correlation: frames of 2^31 pixels or more");
     const bool bf = (form == QLP_XCORR_MFMA_BF16 || form == QLP_XCORR_MFMA_BF16_SINGLE);
     const bool split = (form == QLP_XCORR_MFMA_F16 || form == QLP_XCORR_MFMA_BF16);
     if (form < QLP_XCORR_MFMA_F16 || form > QLP_XCORR_MFMA_BF16_SINGLE)
