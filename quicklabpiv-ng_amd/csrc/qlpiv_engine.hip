@@ -36,6 +36,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 #include "qlpiv_device.h"
@@ -126,26 +127,40 @@ __global__ __launch_bounds__(256) void step_tiles_kernel(
             H = S.H;
             W = S.W;
             uint8_t *oa = tA + (size_t)ia * N * N, *ob = tB + (size_t)ia * N * N;
-            for (int r = 0; r < N; r++) {
+            // Loads are unconditional, from addresses clamped into the frame
+            // (32-bit offsets: frames below 2^31 pixels), and masked afterwards,
+            // one loop per B form, so that the loads of several rows are in
+            // flight together instead of one branch and wait per pixel.
+            const unsigned Hu = (unsigned)H, Wu = (unsigned)W;
+            auto rows = [&](auto warp_c) {
+#pragma unroll 8
+                for (int r = 0; r < N; r++) {
 #pragma unroll
-                for (int c = c0; c < N; c += LPI) {
-                    int a = 0, b = 0;
-                    const int ya = y0 + r, xa = x0 + c;
-                    if (ya >= 0 && ya < H && xa >= 0 && xa < W) a = fa[(size_t)ya * W + xa];
-                    if (warp) {
-                        b = (int)warp_pixel(fb, H, W, fu, fv, by0 + r, bx0 + c);
-                    } else {
-                        const int yb = by0 + r, xb = bx0 + c;
-                        if (yb >= 0 && yb < H && xb >= 0 && xb < W) {
-                            b = fb[(size_t)yb * W + xb];
+                    for (int c = c0; c < N; c += LPI) {
+                        const unsigned ya = (unsigned)(y0 + r), xa = (unsigned)(x0 + c);
+                        const bool ina = ya < Hu && xa < Wu;
+                        int a = fa[ina ? ya * Wu + xa : 0u];
+                        a = ina ? a : 0;
+                        int b;
+                        if constexpr (decltype(warp_c)::value) {
+                            b = (int)warp_pixel<uint8_t, unsigned>(fb, H, W, fu, fv, by0 + r, bx0 + c);
+                        } else {
+                            const unsigned yb = (unsigned)(by0 + r), xb = (unsigned)(bx0 + c);
+                            const bool inb = yb < Hu && xb < Wu;
+                            b = fb[inb ? yb * Wu + xb : 0u];
+                            b = inb ? b : 0;
                             mb = max(mb, b);
                         }
+                        ma = max(ma, a);
+                        oa[r * N + c] = (uint8_t)a;
+                        ob[r * N + c] = (uint8_t)b;
                     }
-                    ma = max(ma, a);
-                    oa[r * N + c] = (uint8_t)a;
-                    ob[r * N + c] = (uint8_t)b;
                 }
-            }
+            };
+            if (warp)
+                rows(std::true_type{});
+            else
+                rows(std::false_type{});
         }
         // maxima over the LPI lanes of each tile
 #pragma unroll
@@ -688,7 +703,12 @@ int engine_step(Engine *e, const std::vector<StepSeg> segs[ENG_CLASSES])
     Context *ctx = e->ctx;
     int counts[ENG_CLASSES] = {0, 0, 0, 0}, total_segs = 0;
     for (int c = 0; c < ENG_CLASSES; c++) {
-        for (const StepSeg &s : segs[c]) counts[c] += s.nI * s.nJ;
+        for (const StepSeg &s : segs[c]) {
+            // the tile kernel indexes frames with 32-bit offsets
+            if ((long long)s.H * s.W > 0x7fffffffLL)
+                return fail(QLP_ERR_UNSUPPORTED, "engine step: frames of 2^31 pixels or more");
+            counts[c] += s.nI * s.nJ;
+        }
         total_segs += (int)segs[c].size();
         if ((size_t)counts[c] > e->cls[c].cap)
             return fail(QLP_ERR_ARG, "engine step: a class exceeds its IA capacity");
