@@ -24,6 +24,7 @@
 //   false : CrossCorrelationRealFFTParStdJob (each tile by its own max; product
 //           divided by M^2 once; inverse through the re/im swap of the forward
 //           transform)
+#include <type_traits>
 #include <mutex>
 
 #include "qlpiv_device.h"
@@ -339,26 +340,38 @@ __global__ __launch_bounds__((XcShape<LOGM, NT_>::THREADS)) void xcorr_pass_kern
                 }
             }
             int amax = 0, bmax = 0;
-            #pragma unroll
-            for (int k = 0; k < PIX; k++) {
-                const int idx = tid + k * NT;
-                const int r = idx / N, c = idx - (idx / N) * N;
-                int a = 0, b = 0;
-                const int ya = y0 + r, xa = x0 + c;
-                if (ya >= 0 && ya < p.H && xa >= 0 && xa < p.W) a = imA[(size_t)ya * p.W + xa];
-                if (warp) {
-                    b = (int)warp_pixel(imB, p.H, p.W, fu, fv, by0 + r, bx0 + c);
-                } else {
-                    const int yb = by0 + r, xb = bx0 + c;
-                    if (yb >= 0 && yb < p.H && xb >= 0 && xb < p.W) {
-                        b = imB[(size_t)yb * p.W + xb];
+            // unconditional loads from addresses clamped into the frame (32-bit
+            // offsets: the launcher bounds H * W below 2^31), masked afterwards,
+            // one loop per B form: the loads of all pixels are in flight together
+            const unsigned Hu = (unsigned)p.H, Wu = (unsigned)p.W;
+            auto load = [&](auto warp_c) {
+                #pragma unroll
+                for (int k = 0; k < PIX; k++) {
+                    const int idx = tid + k * NT;
+                    const int r = idx / N, c = idx - (idx / N) * N;
+                    const unsigned ya = (unsigned)(y0 + r), xa = (unsigned)(x0 + c);
+                    const bool ina = ya < Hu && xa < Wu;
+                    int a = imA[ina ? ya * Wu + xa : 0u];
+                    a = ina ? a : 0;
+                    int b;
+                    if constexpr (decltype(warp_c)::value) {
+                        b = (int)warp_pixel<uint8_t, unsigned>(imB, p.H, p.W, fu, fv, by0 + r, bx0 + c);
+                    } else {
+                        const unsigned yb = (unsigned)(by0 + r), xb = (unsigned)(bx0 + c);
+                        const bool inb = yb < Hu && xb < Wu;
+                        b = imB[inb ? yb * Wu + xb : 0u];
+                        b = inb ? b : 0;
                         bmax = max(bmax, b);
                     }
+                    amax = max(amax, a);
+                    va[k] = (float)a;
+                    vb[k] = (float)b;
                 }
-                amax = max(amax, a);
-                va[k] = (float)a;
-                vb[k] = (float)b;
-            }
+            };
+            if (warp)
+                load(std::true_type{});
+            else
+                load(std::false_type{});
             amax = block_max_int<NT>(amax, s_red, tid);
             mA = (float)(amax < 16 ? 16 : amax);
             if (warp || p.bmax255) {
@@ -893,6 +906,10 @@ int launch_pass(Context *ctx, const PassParams &p)
         return fail(QLP_ERR_ARG, "pass: persistent passes take 8-bit tiles and the fused peak");
     if (p.nbr && (p.nbr_half < 0 || p.nbr_half > 7))
         return fail(QLP_ERR_ARG, "pass: neighbourhood half-width must be 0..7");
+    // the passes from frames index them with 32-bit offsets (the reference's
+    // frames are Java arrays, below 2^31 elements too)
+    if (from_image && (long long)p.H * p.W > 0x7fffffffLL)
+        return fail(QLP_ERR_UNSUPPORTED, "pass: frames of 2^31 pixels or more");
     // measured on MI355X (C2 multipass): the lane-per-line form is faster for
     // N = 16 (0.293 vs 0.323 ms for 65,025 IAs), the LDS-exchange form for
     // N = 32 (0.381 vs 0.436 ms for 16,129 IAs) and N = 64, where more

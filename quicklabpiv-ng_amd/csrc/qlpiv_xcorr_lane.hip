@@ -20,6 +20,7 @@
 // M = 32 runs two IAs per 64-lane work-group, M = 64 one IA per wavefront,
 // M = 128 one IA per 256-lane work-group.  Work-groups are assigned to IAs in
 // contiguous runs per XCD, so overlapping IAs share an L2.
+#include <type_traits>
 #include "qlpiv_device.h"
 #include "qlpiv_internal.h"
 
@@ -198,26 +199,39 @@ __global__ __launch_bounds__(Lx<LOGM>::THREADS) void xcorr_lane_kernel(
                         fv = v - flv;
                     }
                 }
-                #pragma unroll
-                for (int k = 0; k < X::PIX; k++) {
-                    const int idx = t + k * TPI;
-                    const int r = idx / N, c = idx % N;
-                    int a = 0, b = 0;
-                    const int ya = y0 + r, xa = x0 + c;
-                    if (ya >= 0 && ya < p.H && xa >= 0 && xa < p.W) a = imA[(size_t)ya * p.W + xa];
-                    if (warp) {
-                        b = (int)warp_pixel(imB, p.H, p.W, fu, fv, by0 + r, bx0 + c);
-                    } else {
-                        const int yb = by0 + r, xb = bx0 + c;
-                        if (yb >= 0 && yb < p.H && xb >= 0 && xb < p.W) {
-                            b = imB[(size_t)yb * p.W + xb];
+                // unconditional loads from addresses clamped into the frame
+                // (32-bit offsets: the launcher bounds H * W below 2^31), masked
+                // afterwards, one loop per B form: all loads in flight together
+                const unsigned Hu = (unsigned)p.H, Wu = (unsigned)p.W;
+                auto load = [&](auto warp_c) {
+                    #pragma unroll
+                    for (int k = 0; k < X::PIX; k++) {
+                        const int idx = t + k * TPI;
+                        const int r = idx / N, c = idx % N;
+                        const unsigned ya = (unsigned)(y0 + r), xa = (unsigned)(x0 + c);
+                        const bool ina = ya < Hu && xa < Wu;
+                        int a = imA[ina ? ya * Wu + xa : 0u];
+                        a = ina ? a : 0;
+                        int b;
+                        if constexpr (decltype(warp_c)::value) {
+                            b = (int)warp_pixel<uint8_t, unsigned>(imB, p.H, p.W, fu, fv, by0 + r,
+                                                                   bx0 + c);
+                        } else {
+                            const unsigned yb = (unsigned)(by0 + r), xb = (unsigned)(bx0 + c);
+                            const bool inb = yb < Hu && xb < Wu;
+                            b = imB[inb ? yb * Wu + xb : 0u];
+                            b = inb ? b : 0;
                             bmax = max(bmax, b);
                         }
+                        amax = max(amax, a);
+                        va[k] = (float)a;
+                        vb[k] = (float)b;
                     }
-                    amax = max(amax, a);
-                    va[k] = (float)a;
-                    vb[k] = (float)b;
-                }
+                };
+                if (warp)
+                    load(std::true_type{});
+                else
+                    load(std::false_type{});
             } else {
                 #pragma unroll
                 for (int k = 0; k < X::PIX; k++) va[k] = vb[k] = 0.0f;
