@@ -74,8 +74,8 @@ struct Mf {
     // instead pushes its small coefficients into fp16 subnormals (measured:
     // 2e-6 instead of 7e-7 of the plane maximum).
     static constexpr int E1 = (N == 16) ? 4 : (N == 32 ? 6 : 8);
-    // 2^-ES on each image's E gives their products 4 / M^2 / 2^E1 (N = 16, 32)
-    static constexpr int ES = (N == 16) ? 6 : 8;
+    // 2^-ES on each image's E gives their products 4 / M^2 / 2^E1
+    static constexpr int ES = (N == 16) ? 6 : (N == 32 ? 8 : 10);
     static constexpr int E2 = (M == 32) ? 5 : (M == 64 ? 6 : 7);
 };
 
@@ -189,23 +189,6 @@ __device__ __forceinline__ Acc<MT> mma3(u4 ah, u4 al, u4 bh, u4 bl, Acc<MT> c)
         c = mma<BF, MT>(ah, bl, c);
     }
     return mma<BF, MT>(ah, bh, c);
-}
-
-// B fragment from an LDS matrix stored [j][k] (row j = the fragment column,
-// stride S halves): element s holds column kphys(kb, g, s) - koff; a group of
-// four columns at or beyond kmax is 0.  The four columns of a group, for every
-// lane group g, stay inside one aligned block of 16 (MT 16) or 8 (MT 32)
-// columns and kmax is a multiple of 16, so whether a group is in range does
-// not depend on g: the test is made on compile-time values.
-template <int MT, int S, int KB_, int KOFF, int KMAX>
-__device__ __forceinline__ u4 bfrag_from_lds(const unsigned short *__restrict__ base, int row,
-                                             int g)
-{
-    constexpr int b0 = kphys<MT>(KB_, 0, 0) - KOFF, b1 = kphys<MT>(KB_, 0, 4) - KOFF;
-    u2 x0 = {0u, 0u}, x1 = {0u, 0u};
-    if constexpr (b0 < KMAX) x0 = *(const u2 *)(base + row * S + b0 + 4 * g);
-    if constexpr (b1 < KMAX) x1 = *(const u2 *)(base + row * S + b1 + 4 * g);
-    return (u4){x0[0], x0[1], x1[0], x1[1]};
 }
 
 // f(integral_constant<I>) for I = B .. E-1
@@ -679,25 +662,27 @@ __global__ __launch_bounds__(64 * MF_WAVES, QLPIV_MF_MINW) void xcorr_mfma_kerne
 //
 // At M = 128 every matrix of the chain is 4 x 4 tiles of 32 x 32, too many for
 // one wavefront's registers.  The four wavefronts of a work-group share one IA:
-// wave (sr, sc) owns the forward and spectrum tiles rt in {sr, sr + 2} (column
-// frequency) x ct in {sc, sc + 2} (row frequency).  Basis tile t + 2 holds the
+// wave (sr, sc) owns the spectrum tiles sigma in {sr, sr + 2} (column
+// frequency) x rho in {sc, sc + 2} (row frequency).  Basis tile t + 2 holds the
 // sines of tile t's cosines (tile 2 starts with cos(pi x), the partner of
 // k = 0), so each wave holds whole (cos/sin x cos/sin) quads and forms its part
-// of the spectrum alone.  The transposes between the products go through one
-// LDS buffer, in the order of the chain:
-//   V  = Hf x          wave w: row tile w                 -> LDS [rho][m]
-//   W  = Hf V^T        wave (sr, sc): its 4 tiles, x = a then g
-//   C  = spectrum      wave-local                         -> LDS [rho][sigma]
-//   Y  = Hout^T C^T    wave w: row tile w (4 column tiles) -> LDS [p][rho]
-//   R  = Hout^T Y^T    wave w: row tile w                 -> plane, FromCenter
-// The constant fragments (Hf: 4 x 4, Hout^T: 4 x 8) are read from global memory
-// (96 KB, L2-resident).  Work-groups loop over the IAs.
+// of the spectrum alone.  The chain is the one-wave kernel's (no transposes):
+//   tiles : wave w forms two of the eight x^T A fragments of each image
+//           (n tile sr, K blocks 2 sc, 2 sc + 1) -> LDS; every wave reads all
+//   P = x^T Hf^T  for its rho tiles (both n tiles; the wave pairs with the same
+//                 sc compute the same P)          Q = Hf P = E^T  (its quad)
+//   C             its quad -> LDS as A fragments (rho tile, sigma K block)
+//   Y = C Hout    rho tiles {sc, sc + 2} x q tiles {sr, sr + 2}, K = all sigma
+//                 (its own and its partner's C)  -> LDS as B fragments
+//   R = Hout^T Y  p tiles {sc, sc + 2} x q tiles {sr, sr + 2}, K = all rho
+// The exchanges move whole 16-byte fragments (one LDS instruction per
+// fragment and lane).  The constant fragments (Hf: 4 x 4, Hout^T: 4 x 8) are
+// read from global memory (L2-resident).  Work-groups loop over the IAs.
 namespace m64 {
 constexpr int N = 64, M = 128, D = 127, MT = 32, NS = 16, KN = 4, KM = 8;
-constexpr int SV = 72;   // LDS row stride of V (halves)
-constexpr int SC = 136;  // LDS row stride of C and Y (halves)
 constexpr int NT = 256;
-constexpr int NF_HF = 16;  // Hf fragments first, then Hout^T (32)
+constexpr int NF_HF = 16;       // Hf fragments first, then Hout^T (32)
+constexpr int XCH = 64 * 64;    // u4 entries of the exchange buffer (64 KB)
 }  // namespace m64
 
 template <bool BF, bool SPLIT, bool FROM_IMAGE, bool PEAK>
@@ -706,15 +691,14 @@ __global__ __launch_bounds__(m64::NT, 2) void xcorr_mfma64_kernel(PassParams p,
 {
     using namespace m64;
     using X = Mf<64>;
-    // hi and lo halves of V, C or Y; the plane (fp32, stride M) at the end
-    __shared__ __attribute__((aligned(16))) unsigned short s_buf[2][M * SC];
+    // tile fragments, then C, then Y fragments, then (PEAK) the plane, stride M
+    __shared__ u4 s_x[XCH];
     __shared__ uint64_t s_key[4];
     __shared__ int s_cell[4];
     __shared__ float s_min[4];
     __shared__ float s_vmax[4];
-    unsigned short *__restrict__ bh = s_buf[0];
-    unsigned short *__restrict__ bl = s_buf[1];
-    float *__restrict__ s_plane = (float *)&s_buf[0][0];
+    __shared__ int s_max[2][4];
+    float *__restrict__ s_plane = (float *)&s_x[0];
 
     const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
     const int sr = wave >> 1, sc = wave & 1;
@@ -723,8 +707,10 @@ __global__ __launch_bounds__(m64::NT, 2) void xcorr_mfma64_kernel(PassParams p,
     for (int ia = blockIdx.x; ia < n; ia += gridDim.x) {
         int lane = threadIdx.x & 63;
         asm volatile("" : "+v"(lane));  // see xcorr_mfma_kernel
+        __builtin_assume(lane >= 0 && lane < 64);
         const int j = lane % MT, g = lane / MT;
         auto frag = [&](int f, int lo) -> u4 { return frags[(f * 2 + lo) * 64 + lane]; };
+        auto xs = [&](int e) -> u4 & { return s_x[e * 64 + lane]; };
 
         // ---- pixel sources (as in the other forms) ------------------------------
         const uint8_t *a8 = nullptr, *b8 = nullptr;
@@ -751,7 +737,7 @@ __global__ __launch_bounds__(m64::NT, 2) void xcorr_mfma64_kernel(PassParams p,
                     const float flu = floorf(u), flv = floorf(v);
                     by0 = (short)(y0 + (int)flu);
                     bx0 = (short)(x0 + (int)flv);
-                    fu = u - flu;
+                    fu = u - flu;  // createMicroDisplacements :195-196
                     fv = v - flv;
                 }
             }
@@ -759,195 +745,339 @@ __global__ __launch_bounds__(m64::NT, 2) void xcorr_mfma64_kernel(PassParams p,
             a8 = p.tA + (size_t)ia * N * N;
             b8 = p.tB + (size_t)ia * N * N;
         }
-        // pixel (r, m) of tile A or B (in-frame test; B: shift or micro warp)
-        auto pix = [&](int im, int r, int m, bool &in) -> int {
-            in = true;
-            if (!FROM_IMAGE) return (im == 0 ? a8 : b8)[r * N + m];
-            if (im == 1 && warp) return (int)warp_pixel(b8, p.H, p.W, fu, fv, by0 + r, bx0 + m);
-            const int y = (im == 0 ? y0 : by0) + r, x = (im == 0 ? x0 : bx0) + m;
-            in = y >= 0 && y < p.H && x >= 0 && x < p.W;
-            return in ? (im == 0 ? a8 : b8)[(size_t)y * p.W + x] : 0;
-        };
 
-        // ---- forward, one image at a time: V = Hf x, then W = Hf V^T ------------
-        Acc<MT> W[2][2][2];  // [image][rt sr / sr+2][ct sc / sc+2]
-        float mA = 16.0f;
+        // ---- tiles: this wave's x^T A fragments (n tile sr, K blocks 2 sc + kq):
+        // lane (i, g) element s = x[kphys(kb, g, s)][32 sr + i].  Loads are
+        // unconditional from clamped addresses (32-bit offsets: the launcher
+        // bounds H * W below 2^31), one loop per B form.
+        {
+            const unsigned Hu = (unsigned)p.H, Wu = (unsigned)p.W;
+            int mx[2] = {0, 0};
+            float tv[2][2][8];  // [image][kq][s]
+            // offsets first, then every load, then the masks: the loads issue
+            // back to back
+            auto load = [&](auto warp_c) {
+                const uint8_t *__restrict__ src[2] = {a8, b8};
+                unsigned off[2][2][8];
+                bool in[2][2][8];
+                int raw[2][2][8];
 #pragma unroll
-        for (int im = 0; im < 2; im++) {
-            // the tile in the B layout of Hf x: (kb, s) = row kphys, column 32 c + j
-            float t[2][KN][8];
-            int mx = 0;
-#pragma unroll
-            for (int c = 0; c < 2; c++)
-#pragma unroll
-                for (int kb = 0; kb < KN; kb++)
+                for (int kq = 0; kq < 2; kq++)
 #pragma unroll
                     for (int s = 0; s < 8; s++) {
-                        bool in;
-                        const int v = pix(im, kphys<MT>(kb, g, s), MT * c + j, in);
-                        if (in) mx = max(mx, v);
-                        t[c][kb][s] = (float)v;
+                        const int r = kphys<MT>(2 * sc + kq, g, s), c = MT * sr + j;
+#pragma unroll
+                        for (int im = 0; im < 2; im++) {
+                            if (!FROM_IMAGE) {
+                                off[im][kq][s] = (unsigned)(r * N + c);
+                                in[im][kq][s] = true;
+                            } else {
+                                const unsigned y = (unsigned)((im == 0 ? y0 : by0) + r);
+                                const unsigned x = (unsigned)((im == 0 ? x0 : bx0) + c);
+                                in[im][kq][s] = y < Hu && x < Wu;
+                                off[im][kq][s] = in[im][kq][s] ? y * Wu + x : 0u;
+                            }
+                        }
                     }
-            float mt;
-            if (FROM_IMAGE) {
 #pragma unroll
-                for (int o = 32; o >= 1; o >>= 1) mx = max(mx, __shfl_xor(mx, o, 64));
-                mt = (im == 1 && (warp || p.bmax255)) ? 255.0f : (float)(mx < 16 ? 16 : mx);
-            } else {
-                mt = im == 0 ? p.maxA[ia] : p.maxB[ia];
-            }
-            if (im == 0) mA = mt;
-            if (im == 1 && p.mode == QLP_NORM_A_MAX) mt = mA;
-            // x / max * 16 + 1 as x * (16 / max) + 1, as in the other sizes
-            const float sx = 16.0f / mt;
-            u4 xh[2][KN], xl[2][KN];
+                for (int kq = 0; kq < 2; kq++)
 #pragma unroll
-            for (int c = 0; c < 2; c++)
+                    for (int s = 0; s < 8; s++)
 #pragma unroll
-                for (int kb = 0; kb < KN; kb++)
+                        for (int im = 0; im < 2; im++) {
+                            if (FROM_IMAGE && decltype(warp_c)::value && im == 1) {
+                                const int r = kphys<MT>(2 * sc + kq, g, s), c = MT * sr + j;
+                                raw[im][kq][s] = (int)warp_pixel<uint8_t, unsigned>(
+                                    b8, p.H, p.W, fu, fv, by0 + r, bx0 + c);
+                                in[im][kq][s] = true;
+                            } else {
+                                raw[im][kq][s] = src[im][off[im][kq][s]];
+                            }
+                        }
+#pragma unroll
+                for (int kq = 0; kq < 2; kq++)
+#pragma unroll
+                    for (int s = 0; s < 8; s++)
+#pragma unroll
+                        for (int im = 0; im < 2; im++) {
+                            const int v = in[im][kq][s] ? raw[im][kq][s] : 0;
+                            mx[im] = max(mx[im], v);
+                            tv[im][kq][s] = (float)v;
+                        }
+            };
+            if (warp)
+                load(std::true_type{});
+            else
+                load(std::false_type{});
+#pragma unroll
+            for (int im = 0; im < 2; im++) {
+#pragma unroll
+                for (int kq = 0; kq < 2; kq++) {
+                    u4 f;
 #pragma unroll
                     for (int s = 0; s < 8; s += 2) {
                         unsigned h, l;
-                        split2<BF, false>(t[c][kb][s] * sx + 1.0f, t[c][kb][s + 1] * sx + 1.0f, h, l);
-                        xh[c][kb][s >> 1] = h;
-                        xl[c][kb][s >> 1] = l;
+                        split2<BF>(tv[im][kq][s], tv[im][kq][s + 1], h, l);  // exact: l == 0
+                        f[s >> 1] = h;
                     }
-            // V row tile 'wave' (32 basis rows x 64 columns) to LDS [rho][m]
+                    xs(im * 8 + sr * 4 + 2 * sc + kq) = f;
+                }
+                if (FROM_IMAGE) {
 #pragma unroll
-            for (int c = 0; c < 2; c++) {
-                Acc<MT> acc = {};
+                    for (int o = 32; o >= 1; o >>= 1) mx[im] = max(mx[im], __shfl_xor(mx[im], o, 64));
+                    if (lane == 0) s_max[im][wave] = mx[im];
+                }
+            }
+        }
+        __syncthreads();  // the tile fragments and the partial maxima
+        float mA, mB;
+        if (FROM_IMAGE) {
+            const int am = max(max(s_max[0][0], s_max[0][1]), max(s_max[0][2], s_max[0][3]));
+            const int bm = max(max(s_max[1][0], s_max[1][1]), max(s_max[1][2], s_max[1][3]));
+            mA = (float)(am < 16 ? 16 : am);
+            mB = (warp || p.bmax255) ? 255.0f : (float)(bm < 16 ? 16 : bm);
+        } else {
+            mA = p.maxA[ia];
+            mB = p.maxB[ia];
+        }
+        if (p.mode == QLP_NORM_A_MAX) mB = mA;
+
+        // ---- forward: P = x'^T Hf^T (rho tiles sc, sc + 2), Q = Hf P (quad) ----
+        // the normalisation folded as in the one-wave kernel; E leaves scaled
+        // by 2^-ES
+        Acc<MT> W[2][2][2];  // [image][ri: sigma tile sr + 2 ri][ci: rho tile sc + 2 ci]
+        const float4 hrow = __builtin_bit_cast(float4, frags[X::NF * 2 * 64 + lane]);
+        const float hv[4] = {hrow.x, hrow.y, hrow.z, hrow.w};
 #pragma unroll
-                for (int kb = 0; kb < KN; kb++)
-                    acc = mma3<BF, SPLIT, MT>(frag(wave * KN + kb, 0), frag(wave * KN + kb, 1),
-                                              xh[c][kb], xl[c][kb], acc);
+        for (int im = 0; im < 2; im++) {
+            const float mx = im == 0 ? mA : mB;
+            const float sx = 16.0f / mx * (1.0f / (float)(1 << X::ES)), inv = mx * 0.0625f;
+#pragma unroll
+            for (int ci = 0; ci < 2; ci++) {
+                const int ct = sc + 2 * ci;
+                const float h0 = hv[ct] * inv;
+                u4 ph[KN], pl[KN];  // P's rho tile ct as B fragments (K = n)
+                {
+                    // K block outermost (both n tiles per constant fragment);
+                    // block kb + 1's fragment is loaded while block kb's run
+                    Acc<MT> acc[2];
+#pragma unroll
+                    for (int nt = 0; nt < 2; nt++)
+#pragma unroll
+                        for (int v = 0; v < NS; v++) acc[nt][v] = h0;
+                    u4 fh = frag(ct * KN, 0), fl = frag(ct * KN, 1);
+#pragma unroll
+                    for (int kb = 0; kb < KN; kb++) {
+                        u4 nh = fh, nl = fl;
+                        if (kb + 1 < KN) {
+                            nh = frag(ct * KN + kb + 1, 0);
+                            nl = frag(ct * KN + kb + 1, 1);
+                        }
+#pragma unroll
+                        for (int nt = 0; nt < 2; nt++) {
+                            const u4 xa = xs(im * 8 + nt * 4 + kb);  // tile fragment (n tile, K block)
+                            if constexpr (SPLIT) acc[nt] = mma<BF, MT>(xa, fl, acc[nt]);
+                            acc[nt] = mma<BF, MT>(xa, fh, acc[nt]);
+                        }
+                        fh = nh;
+                        fl = nl;
+                    }
+#pragma unroll
+                    for (int nt = 0; nt < 2; nt++)
+#pragma unroll
+                        for (int v = 0; v < NS; v += 2) {
+                            unsigned h, l;
+                            split2<BF, false>(acc[nt][v] * sx, acc[nt][v + 1] * sx, h, l);
+                            ph[acc_kb<MT>(nt, v)][acc_s<MT>(nt, v) >> 1] = h;
+                            pl[acc_kb<MT>(nt, v)][acc_s<MT>(nt, v) >> 1] = l;
+                        }
+                }
+                {
+                    Acc<MT> w[2] = {};
+                    u4 fr[2][2];  // [ri][hi/lo] of block kb
+#pragma unroll
+                    for (int ri = 0; ri < 2; ri++)
+#pragma unroll
+                        for (int hl = 0; hl < 2; hl++) fr[ri][hl] = frag((sr + 2 * ri) * KN, hl);
+#pragma unroll
+                    for (int kb = 0; kb < KN; kb++) {
+                        u4 nx[2][2];
+#pragma unroll
+                        for (int ri = 0; ri < 2; ri++)
+#pragma unroll
+                            for (int hl = 0; hl < 2; hl++)
+                                nx[ri][hl] = kb + 1 < KN ? frag((sr + 2 * ri) * KN + kb + 1, hl)
+                                                         : fr[ri][hl];
+#pragma unroll
+                        for (int ri = 0; ri < 2; ri++)
+                            w[ri] = mma3<BF, SPLIT, MT>(fr[ri][0], fr[ri][1], ph[kb], pl[kb], w[ri]);
+#pragma unroll
+                        for (int ri = 0; ri < 2; ri++)
+#pragma unroll
+                            for (int hl = 0; hl < 2; hl++) fr[ri][hl] = nx[ri][hl];
+                    }
+#pragma unroll
+                    for (int ri = 0; ri < 2; ri++) W[im][ri][ci] = w[ri];
+                }
+                // keep the scheduler from hoisting the next rho tile's constant
+                // loads over this one (their registers spill)
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        }
+        __syncthreads();  // every wave has read the tile fragments
+
+        // ---- spectrum of the quad (the one-wave kernel's formulas) -> LDS -------
+        // C A fragment (rho tile ct, sigma K block kb8, hi/lo) at entry
+        // (ct * KM + kb8) * 2 + hl; this wave's K blocks: 2 rt + (v >> 3)
+        {
+            const bool kk = (sc == 0) && (j == 0);
+            u4 ch[2][4], cl[2][4];  // [ci][kq = 2 ri + (v >> 3)]
+#pragma unroll
+            for (int v0 = 0; v0 < NS; v0 += 2) {
+                float o[2][2][2];  // [slot v0 + e][ri][ci]
+#pragma unroll
+                for (int e = 0; e < 2; e++) {
+                    const int v = v0 + e;
+                    const bool ll = (sr == 0) && (v == 0) && (g == 0);  // mrow(v, g) == 0
+                    const bool edge = kk || ll;
+                    const float CC = W[0][0][0][v], SC = W[0][0][1][v], CS = W[0][1][0][v],
+                                SS = W[0][1][1][v];
+                    const float cc = W[1][0][0][v], scg = W[1][0][1][v], cs = W[1][1][0][v],
+                                ss = W[1][1][1][v];
+                    const float ssm = edge ? 0.0f : SS;
+                    const float a00[4] = {CC, ll ? 0.0f : CS, kk ? 0.0f : SC, ssm};
+                    const float a10[4] = {ll ? CS : CC, ll ? 0.0f : CS, kk ? 0.0f : (ll ? SS : SC), ssm};
+                    const float a01[4] = {kk ? SC : CC, ll ? 0.0f : (kk ? SS : CS), kk ? 0.0f : SC, ssm};
+                    const float a11[4] = {kk ? (ll ? SS : SC) : (ll ? CS : CC),
+                                          ll ? 0.0f : (kk ? SS : CS), kk ? 0.0f : (ll ? SS : SC), ssm};
+                    o[e][0][0] = a00[0] * cc + a00[3] * ss + a00[2] * scg + a00[1] * cs;
+                    o[e][1][1] = a11[0] * ss + a11[3] * cc - a11[2] * cs - a11[1] * scg;
+                    o[e][0][1] = a01[0] * scg - a01[2] * cc + a01[1] * ss - a01[3] * cs;
+                    o[e][1][0] = a10[0] * cs - a10[1] * cc + a10[2] * ss - a10[3] * scg;
+                }
+#pragma unroll
+                for (int ri = 0; ri < 2; ri++)
+#pragma unroll
+                    for (int ci = 0; ci < 2; ci++) {
+                        unsigned h, l;
+                        split2<BF, false>(o[0][ri][ci], o[1][ri][ci], h, l);
+                        ch[ci][2 * ri + (v0 >> 3)][(v0 & 7) >> 1] = h;
+                        cl[ci][2 * ri + (v0 >> 3)][(v0 & 7) >> 1] = l;
+                    }
+            }
+#pragma unroll
+            for (int ci = 0; ci < 2; ci++)
+#pragma unroll
+                for (int kq = 0; kq < 4; kq++) {
+                    const int kb8 = 2 * (sr + 2 * (kq >> 1)) + (kq & 1);
+                    const int e = ((sc + 2 * ci) * KM + kb8) * 2;
+                    xs(e) = ch[ci][kq];
+                    xs(e + 1) = cl[ci][kq];
+                }
+        }
+        __syncthreads();
+
+        // ---- Y = C Hout: rho tiles {sc, sc + 2} x q tiles {sr, sr + 2} ----------
+        Acc<MT> Y[2][2];  // [ci][qi]
+#pragma unroll
+        for (int ci = 0; ci < 2; ci++)
+#pragma unroll
+            for (int qi = 0; qi < 2; qi++) Y[ci][qi] = Acc<MT>{};
+        // K block outermost: the constant fragments of block kb + 1 are loaded
+        // (from L2) while block kb's products run
+        {
+            u4 hq[2][2];  // [qi][hi/lo] of block kb
+#pragma unroll
+            for (int qi = 0; qi < 2; qi++)
+#pragma unroll
+                for (int hl = 0; hl < 2; hl++) hq[qi][hl] = frag(NF_HF + (sr + 2 * qi) * KM, hl);
+#pragma unroll
+            for (int kb = 0; kb < KM; kb++) {
+                u4 hn[2][2];
+                if (kb + 1 < KM) {
+#pragma unroll
+                    for (int qi = 0; qi < 2; qi++)
+#pragma unroll
+                        for (int hl = 0; hl < 2; hl++)
+                            hn[qi][hl] = frag(NF_HF + (sr + 2 * qi) * KM + kb + 1, hl);
+                }
+#pragma unroll
+                for (int ci = 0; ci < 2; ci++) {
+                    const int ct = sc + 2 * ci;
+                    const u4 ah = xs((ct * KM + kb) * 2), al = xs((ct * KM + kb) * 2 + 1);
+#pragma unroll
+                    for (int qi = 0; qi < 2; qi++)
+                        Y[ci][qi] = mma3<BF, SPLIT, MT>(ah, al, hq[qi][0], hq[qi][1], Y[ci][qi]);
+                }
+                if (kb + 1 < KM) {
+#pragma unroll
+                    for (int qi = 0; qi < 2; qi++)
+#pragma unroll
+                        for (int hl = 0; hl < 2; hl++) hq[qi][hl] = hn[qi][hl];
+                }
+            }
+        }
+        __syncthreads();  // every wave has read C
+        // Y B fragment (q tile qt, rho K block kb8, hi/lo) at entry (qt * KM + kb8) * 2 + hl
+#pragma unroll
+        for (int ci = 0; ci < 2; ci++)
+#pragma unroll
+            for (int qi = 0; qi < 2; qi++) {
+                u4 yh[2], yl[2];
 #pragma unroll
                 for (int v = 0; v < NS; v += 2) {
                     unsigned h, l;
-                    split2<BF, false>(acc[v], acc[v + 1], h, l);
-                    const int r = MT * wave + mrow<MT>(v, g), m = MT * c + j;
-                    bh[r * SV + m] = (unsigned short)h;
-                    bl[r * SV + m] = (unsigned short)l;
-                    bh[(r + 1) * SV + m] = (unsigned short)(h >> 16);
-                    bl[(r + 1) * SV + m] = (unsigned short)(l >> 16);
+                    split2<BF, false>(Y[ci][qi][v] * (1.0f / (float)(1 << X::E2)),
+                                      Y[ci][qi][v + 1] * (1.0f / (float)(1 << X::E2)), h, l);
+                    yh[v >> 3][(v & 7) >> 1] = h;
+                    yl[v >> 3][(v & 7) >> 1] = l;
+                }
+#pragma unroll
+                for (int hb = 0; hb < 2; hb++) {
+                    const int e = ((sr + 2 * qi) * KM + 2 * (sc + 2 * ci) + hb) * 2;
+                    xs(e) = yh[hb];
+                    xs(e + 1) = yl[hb];
                 }
             }
-            __syncthreads();
-#pragma unroll
-            for (int ri = 0; ri < 2; ri++)
-#pragma unroll
-                for (int ci = 0; ci < 2; ci++) {
-                    const int rt = sr + 2 * ri, ct = sc + 2 * ci;
-                    Acc<MT> w = {};
-                    static_for<0, KN>([&](auto kc) {
-                        constexpr int kb = decltype(kc)::value;
-                        w = mma3<BF, SPLIT, MT>(frag(rt * KN + kb, 0), frag(rt * KN + kb, 1),
-                                                bfrag_from_lds<MT, SV, kb, 0, N>(bh, MT * ct + j, g),
-                                                bfrag_from_lds<MT, SV, kb, 0, N>(bl, MT * ct + j, g),
-                                                w);
-                    });
-                    W[im][ri][ci] = w;
-                }
-            __syncthreads();  // V is rewritten (next image) or replaced by C
-        }
-
-        // ---- spectrum (the quad formulas of the one-wave kernel) -> LDS [rho][sigma]
-        constexpr float sg = 4.0f / (float)(M * M) / (float)(1 << X::E1);
-        constexpr float se = 2.0f / (float)(M * M) / (float)(1 << X::E1);
-        constexpr float s1 = 1.0f / (float)(M * M) / (float)(1 << X::E1);
-#pragma unroll
-        for (int v0 = 0; v0 < NS; v0 += 2) {
-            float o[2][2][2];  // [slot v0 + e][ri][ci]
-#pragma unroll
-            for (int e = 0; e < 2; e++) {
-                const int v = v0 + e;
-                const bool kk = sc == 0 && j == 0, ll = sr == 0 && mrow<MT>(v, g) == 0;
-                const float CC = W[0][0][0][v], SC_ = W[0][0][1][v], CS = W[0][1][0][v],
-                            SS = W[0][1][1][v];
-                const float cc = W[1][0][0][v], scg = W[1][0][1][v], cs = W[1][1][0][v],
-                            ss = W[1][1][1][v];
-                if (!kk && !ll) {
-                    o[e][0][0] = sg * (CC * cc + SS * ss + SC_ * scg + CS * cs);
-                    o[e][1][1] = sg * (CC * ss + SS * cc - SC_ * cs - CS * scg);
-                    o[e][0][1] = -sg * (SC_ * cc - CC * scg + SS * cs - CS * ss);
-                    o[e][1][0] = sg * (CC * cs - CS * cc + SC_ * ss - SS * scg);
-                } else if (kk && !ll) {
-                    o[e][0][0] = se * (CC * cc + CS * cs);
-                    o[e][1][0] = se * (CC * cs - CS * cc);
-                    o[e][0][1] = se * (SC_ * scg + SS * ss);
-                    o[e][1][1] = se * (SC_ * ss - SS * scg);
-                } else if (!kk && ll) {
-                    o[e][0][0] = se * (CC * cc + SC_ * scg);
-                    o[e][0][1] = -se * (SC_ * cc - CC * scg);
-                    o[e][1][0] = se * (CS * cs + SS * ss);
-                    o[e][1][1] = -se * (SS * cs - CS * ss);
-                } else {
-                    o[e][0][0] = s1 * CC * cc;
-                    o[e][0][1] = s1 * SC_ * scg;
-                    o[e][1][0] = s1 * CS * cs;
-                    o[e][1][1] = s1 * SS * ss;
-                }
-            }
-            // slots v0, v0 + 1 are sigma, sigma + 1 of the same rho: one 32-bit store
-#pragma unroll
-            for (int ri = 0; ri < 2; ri++)
-#pragma unroll
-                for (int ci = 0; ci < 2; ci++) {
-                    unsigned h, l;
-                    split2<BF, false>(o[0][ri][ci], o[1][ri][ci], h, l);
-                    const int rho = MT * (sc + 2 * ci) + j;
-                    const int sigma = MT * (sr + 2 * ri) + mrow<MT>(v0, g);
-                    *(unsigned *)&bh[rho * SC + sigma] = h;
-                    *(unsigned *)&bl[rho * SC + sigma] = l;
-                }
-        }
         __syncthreads();
 
-        // ---- inverse: Y = Hout^T C^T (row tile 'wave'), then R = Hout^T Y^T ------
-        u4 hoh[KM], hol[KM];
+        // ---- R = Hout^T Y: p tiles {sc, sc + 2} x q tiles {sr, sr + 2} ---------
+        Acc<MT> R[2][2];  // [pi][qi]
 #pragma unroll
-        for (int kb = 0; kb < KM; kb++) {
-            hoh[kb] = frag(NF_HF + wave * KM + kb, 0);
-            hol[kb] = frag(NF_HF + wave * KM + kb, 1);
-        }
-        Acc<MT> Y[4];
+        for (int pi = 0; pi < 2; pi++)
 #pragma unroll
-        for (int ct = 0; ct < 4; ct++) {
-            Acc<MT> acc = {};
-            static_for<0, KM>([&](auto kc) {
-                constexpr int kb = decltype(kc)::value;
-                acc = mma3<BF, SPLIT, MT>(hoh[kb], hol[kb],
-                                          bfrag_from_lds<MT, SC, kb, 0, M>(bh, MT * ct + j, g),
-                                          bfrag_from_lds<MT, SC, kb, 0, M>(bl, MT * ct + j, g),
-                                          acc);
-            });
-            Y[ct] = acc * (1.0f / (float)(1 << X::E2));
-        }
-        __syncthreads();  // every wave has read C
+            for (int qi = 0; qi < 2; qi++) R[pi][qi] = Acc<MT>{};
+        {
+            u4 hp[2][2];  // [pi][hi/lo] of block kb, the next block's loaded meanwhile
 #pragma unroll
-        for (int ct = 0; ct < 4; ct++)
+            for (int pi = 0; pi < 2; pi++)
 #pragma unroll
-            for (int v = 0; v < NS; v += 2) {
-                unsigned h, l;
-                split2<BF, false>(Y[ct][v], Y[ct][v + 1], h, l);
-                const int r = MT * wave + mrow<MT>(v, g), rho = MT * ct + j;
-                bh[r * SC + rho] = (unsigned short)h;
-                bl[r * SC + rho] = (unsigned short)l;
-                bh[(r + 1) * SC + rho] = (unsigned short)(h >> 16);
-                bl[(r + 1) * SC + rho] = (unsigned short)(l >> 16);
+                for (int hl = 0; hl < 2; hl++) hp[pi][hl] = frag(NF_HF + (sc + 2 * pi) * KM, hl);
+#pragma unroll
+            for (int kb = 0; kb < KM; kb++) {
+                u4 hn[2][2];
+                if (kb + 1 < KM) {
+#pragma unroll
+                    for (int pi = 0; pi < 2; pi++)
+#pragma unroll
+                        for (int hl = 0; hl < 2; hl++)
+                            hn[pi][hl] = frag(NF_HF + (sc + 2 * pi) * KM + kb + 1, hl);
+                }
+#pragma unroll
+                for (int qi = 0; qi < 2; qi++) {
+                    const int qt = sr + 2 * qi;
+                    const u4 bh = xs((qt * KM + kb) * 2), bl = xs((qt * KM + kb) * 2 + 1);
+#pragma unroll
+                    for (int pi = 0; pi < 2; pi++)
+                        R[pi][qi] = mma3<BF, SPLIT, MT>(hp[pi][0], hp[pi][1], bh, bl, R[pi][qi]);
+                }
+                if (kb + 1 < KM) {
+#pragma unroll
+                    for (int pi = 0; pi < 2; pi++)
+#pragma unroll
+                        for (int hl = 0; hl < 2; hl++) hp[pi][hl] = hn[pi][hl];
+                }
             }
-        __syncthreads();
-        Acc<MT> R[4];
-#pragma unroll
-        for (int bt = 0; bt < 4; bt++) {
-            Acc<MT> acc = {};
-            static_for<0, KM>([&](auto kc) {
-                constexpr int kb = decltype(kc)::value;
-                acc = mma3<BF, SPLIT, MT>(hoh[kb], hol[kb],
-                                          bfrag_from_lds<MT, SC, kb, 0, M>(bh, MT * bt + j, g),
-                                          bfrag_from_lds<MT, SC, kb, 0, M>(bl, MT * bt + j, g),
-                                          acc);
-            });
-            R[bt] = acc;
         }
         __syncthreads();  // every wave has read Y: the buffer takes the plane
 
@@ -956,21 +1086,23 @@ __global__ __launch_bounds__(m64::NT, 2) void xcorr_mfma64_kernel(PassParams p,
         float *__restrict__ out = p.planes ? p.planes + (size_t)ia * D * D : nullptr;
         float vmax = -1.0f, mn = 3.40282347e+38f;
 #pragma unroll
-        for (int bt = 0; bt < 4; bt++)
+        for (int pi = 0; pi < 2; pi++)
 #pragma unroll
-            for (int v = 0; v < NS; v++) {
-                const int a = MT * wave + mrow<MT>(v, g), b = MT * bt + j;
-                if (a >= D || b >= D) continue;
-                const float val = R[bt][v] * unscale;
-                if (out) out[(unsigned)(a * D + b)] = val;
-                if (PEAK) {
-                    s_plane[a * M + b] = val;
-                    if (a != 0 && b != 0) {  // FromCenter phase 1 (from_center_tie)
-                        vmax = fmaxf(vmax, val);
-                        if (val < mn) mn = val;
+            for (int qi = 0; qi < 2; qi++)
+#pragma unroll
+                for (int v = 0; v < NS; v++) {
+                    const int a = MT * (sc + 2 * pi) + mrow<MT>(v, g), b = MT * (sr + 2 * qi) + j;
+                    if (a >= D || b >= D) continue;
+                    const float val = R[pi][qi][v] * unscale;
+                    if (out) out[(unsigned)(a * D + b)] = val;
+                    if (PEAK) {
+                        s_plane[a * M + b] = val;
+                        if (a != 0 && b != 0) {  // FromCenter phase 1 (from_center_tie)
+                            vmax = fmaxf(vmax, val);
+                            if (val < mn) mn = val;
+                        }
                     }
                 }
-            }
         if (PEAK) {
             vmax = wave_max_float(vmax);
             if ((threadIdx.x & 63) == 0) s_vmax[wave] = vmax;
@@ -980,19 +1112,22 @@ __global__ __launch_bounds__(m64::NT, 2) void xcorr_mfma64_kernel(PassParams p,
             int bestCell = 0;
             if (vmax >= 0.0f) {
 #pragma unroll
-                for (int bt = 0; bt < 4; bt++)
+                for (int pi = 0; pi < 2; pi++)
 #pragma unroll
-                    for (int v = 0; v < NS; v++) {
-                        const int a = MT * wave + mrow<MT>(v, g), b = MT * bt + j;
-                        if (a == 0 || b == 0 || a >= D || b >= D) continue;
-                        if (R[bt][v] * unscale == vmax) {
-                            const uint32_t tie = from_center_tie(a, b, D / 2);
-                            if (tie > best) {
-                                best = tie;
-                                bestCell = a * D + b;
+                    for (int qi = 0; qi < 2; qi++)
+#pragma unroll
+                        for (int v = 0; v < NS; v++) {
+                            const int a = MT * (sc + 2 * pi) + mrow<MT>(v, g),
+                                      b = MT * (sr + 2 * qi) + j;
+                            if (a == 0 || b == 0 || a >= D || b >= D) continue;
+                            if (R[pi][qi][v] * unscale == vmax) {
+                                const uint32_t tie = from_center_tie(a, b, D / 2);
+                                if (tie > best) {
+                                    best = tie;
+                                    bestCell = a * D + b;
+                                }
                             }
                         }
-                    }
             }
             peak_epilogue<NT, D>(p, ia, best, bestCell, mn, s_plane, M, 1, s_key, s_cell, s_min);
         }
@@ -1066,18 +1201,19 @@ static void build_frags(bool bf, std::vector<unsigned short> &out)
     };
     // Hf[rho][n] = h_rho(n)
     put(0, [&](int r, int nn) { return mf_basis(M, r, nn); }, M, N, X::KN, 0);
-    // Hout^T[p][sigma] = h_sigma(p - (N - 1)); for N = 16 and 32 the edge
-    // frequencies sigma = 0 and M/2 carry their inverse weight 1/2 here
+    // Hout^T[p][sigma] = h_sigma(p - (N - 1)); the edge frequencies sigma = 0
+    // and M/2 carry their inverse weight 1/2 here
     put(1,
         [&](int pp, int sg) {
-            const double w = (N != 64 && (sg == 0 || sg == M / 2)) ? 0.5 : 1.0;
+            const double w = (sg == 0 || sg == M / 2) ? 0.5 : 1.0;
             return w * mf_basis(M, sg, (long)pp - (N - 1));
         },
         M, M, X::KM, X::NF_HF);
-    // row sums h[rho] = sum_n Hf[rho][n], lane (i + MT g): {h[i], h[MT + i], 0, 0} as floats
+    // row sums h[rho] = sum_n Hf[rho][n], lane (i + MT g): h[MT c + i] for the
+    // M / MT tiles c (2 or 4), as floats
     out.resize(out.size() + 64 * 8, 0);
     for (int lane = 0; lane < 64; lane++)
-        for (int c = 0; c < 2; c++) {
+        for (int c = 0; c < M / MT; c++) {
             const int r = MT * c + lane % MT;
             double sum = 0.0;
             for (int nn = 0; nn < N; nn++) sum += mf_basis(M, r, nn);
