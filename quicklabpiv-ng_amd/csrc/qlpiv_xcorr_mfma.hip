@@ -679,7 +679,7 @@ __global__ __launch_bounds__(64 * MF_WAVES, QLPIV_MF_MINW) void xcorr_mfma_kerne
 // fragment and lane).  The constant fragments (Hf: 4 x 4, Hout^T: 4 x 8) are
 // read from global memory (L2-resident).  Work-groups loop over the IAs.
 namespace m64 {
-constexpr int N = 64, M = 128, D = 127, MT = 32, NS = 16, KN = 4, KM = 8;
+constexpr int N = 64, D = 127, MT = 32, NS = 16, KN = 4, KM = 8;
 constexpr int NT = 256;
 constexpr int NF_HF = 16;       // Hf fragments first, then Hout^T (32)
 constexpr int XCH = 64 * 64;    // u4 entries of the exchange buffer (64 KB)
@@ -691,14 +691,13 @@ __global__ __launch_bounds__(m64::NT, 2) void xcorr_mfma64_kernel(PassParams p,
 {
     using namespace m64;
     using X = Mf<64>;
-    // tile fragments, then C, then Y fragments, then (PEAK) the plane, stride M
+    // tile fragments, then C, then Y fragments, then (PEAK) the fit neighbourhood
     __shared__ u4 s_x[XCH];
-    __shared__ uint64_t s_key[4];
+    __shared__ uint32_t s_key[4];
     __shared__ int s_cell[4];
     __shared__ float s_min[4];
     __shared__ float s_vmax[4];
     __shared__ int s_max[2][4];
-    float *__restrict__ s_plane = (float *)&s_x[0];
 
     const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
     const int sr = wave >> 1, sc = wave & 1;
@@ -1083,34 +1082,54 @@ __global__ __launch_bounds__(m64::NT, 2) void xcorr_mfma64_kernel(PassParams p,
 
         // ---- epilogues: plane store, FromCenter -------------------------------------
         constexpr float unscale = (float)(1 << (X::E1 + X::E2));
-        float *__restrict__ out = p.planes ? p.planes + (size_t)ia * D * D : nullptr;
-        float vmax = -1.0f, mn = 3.40282347e+38f;
+        if (p.planes) {
+            float *__restrict__ out = p.planes + (size_t)ia * D * D;
 #pragma unroll
-        for (int pi = 0; pi < 2; pi++)
+            for (int pi = 0; pi < 2; pi++)
 #pragma unroll
-            for (int qi = 0; qi < 2; qi++)
+                for (int qi = 0; qi < 2; qi++)
 #pragma unroll
-                for (int v = 0; v < NS; v++) {
-                    const int a = MT * (sc + 2 * pi) + mrow<MT>(v, g), b = MT * (sr + 2 * qi) + j;
-                    if (a >= D || b >= D) continue;
-                    const float val = R[pi][qi][v] * unscale;
-                    if (out) out[(unsigned)(a * D + b)] = val;
-                    if (PEAK) {
-                        s_plane[a * M + b] = val;
-                        if (a != 0 && b != 0) {  // FromCenter phase 1 (from_center_tie)
-                            vmax = fmaxf(vmax, val);
-                            if (val < mn) mn = val;
-                        }
+                    for (int v = 0; v < NS; v++) {
+                        const int a = MT * (sc + 2 * pi) + mrow<MT>(v, g), b = MT * (sr + 2 * qi) + j;
+                        if (a < D && b < D) out[(unsigned)(a * D + b)] = R[pi][qi][v] * unscale;
                     }
-                }
+        }
         if (PEAK) {
-            vmax = wave_max_float(vmax);
-            if ((threadIdx.x & 63) == 0) s_vmax[wave] = vmax;
-            __syncthreads();  // the plane and the per-wave maxima
+            // FromCenter in two phases (from_center_tie) on the accumulator
+            // scale (unscale is a power of two: order and equality are kept),
+            // each wave over its four tiles, the waves joined through LDS
+            auto visited = [&](int a, int b) { return a != 0 && b != 0 && a < D && b < D; };
+            float vmax = -1.0f, mn = 3.40282347e+38f;
+#pragma unroll
+            for (int pi = 0; pi < 2; pi++)
+#pragma unroll
+                for (int qi = 0; qi < 2; qi++)
+#pragma unroll
+                    for (int v = 0; v < NS; v++) {
+                        const int a = MT * (sc + 2 * pi) + mrow<MT>(v, g), b = MT * (sr + 2 * qi) + j;
+                        const bool ok = visited(a, b);
+                        const float x = R[pi][qi][v];
+                        vmax = fmaxf(vmax, ok ? x : -1.0f);
+                        mn = fminf(mn, ok ? x : 3.40282347e+38f);
+                    }
+#pragma unroll
+            for (int o = 32; o >= 1; o >>= 1) {
+                vmax = fmaxf(vmax, __shfl_xor(vmax, o, 64));
+                mn = fminf(mn, __shfl_xor(mn, o, 64));
+            }
+            if (lane == 0) {
+                s_vmax[wave] = vmax;
+                s_min[wave] = mn;
+            }
+            __syncthreads();
             vmax = fmaxf(fmaxf(s_vmax[0], s_vmax[1]), fmaxf(s_vmax[2], s_vmax[3]));
-            uint64_t best = 0;
+            mn = fminf(fminf(s_min[0], s_min[1]), fminf(s_min[2], s_min[3]));
+            // phase 2: the tie key of the cells equal to the maximum (a lane
+            // holds none or one almost always)
+            uint32_t best = 0;
             int bestCell = 0;
             if (vmax >= 0.0f) {
+                int cnt = 0, code = 0;
 #pragma unroll
                 for (int pi = 0; pi < 2; pi++)
 #pragma unroll
@@ -1119,17 +1138,114 @@ __global__ __launch_bounds__(m64::NT, 2) void xcorr_mfma64_kernel(PassParams p,
                         for (int v = 0; v < NS; v++) {
                             const int a = MT * (sc + 2 * pi) + mrow<MT>(v, g),
                                       b = MT * (sr + 2 * qi) + j;
-                            if (a == 0 || b == 0 || a >= D || b >= D) continue;
-                            if (R[pi][qi][v] * unscale == vmax) {
+                            const bool m = visited(a, b) && R[pi][qi][v] == vmax;
+                            cnt += m ? 1 : 0;
+                            code = m ? a * D + b : code;
+                        }
+                if (__ballot(cnt > 1) == 0) {
+                    if (cnt == 1) {
+                        best = from_center_tie(code / D, code % D, D / 2);
+                        bestCell = code;
+                    }
+                } else {  // a lane holds several equal maxima: every cell's key
+#pragma unroll
+                    for (int pi = 0; pi < 2; pi++)
+#pragma unroll
+                        for (int qi = 0; qi < 2; qi++)
+#pragma unroll
+                            for (int v = 0; v < NS; v++) {
+                                const int a = MT * (sc + 2 * pi) + mrow<MT>(v, g),
+                                          b = MT * (sr + 2 * qi) + j;
+                                if (!(visited(a, b) && R[pi][qi][v] == vmax)) continue;
                                 const uint32_t tie = from_center_tie(a, b, D / 2);
                                 if (tie > best) {
                                     best = tie;
                                     bestCell = a * D + b;
                                 }
                             }
-                        }
+                }
             }
-            peak_epilogue<NT, D>(p, ia, best, bestCell, mn, s_plane, M, 1, s_key, s_cell, s_min);
+#pragma unroll
+            for (int o = 32; o >= 1; o >>= 1) {
+                const uint32_t ob = (uint32_t)__shfl_xor((int)best, o, 64);
+                const int oc = __shfl_xor(bestCell, o, 64);
+                if (ob > best) {
+                    best = ob;
+                    bestCell = oc;
+                }
+            }
+            if (lane == 0) {
+                s_key[wave] = best;
+                s_cell[wave] = bestCell;
+            }
+            __syncthreads();
+            best = 0;
+            bestCell = 0;
+#pragma unroll
+            for (int w = 0; w < 4; w++)
+                if ((uint32_t)s_key[w] > best) {  // keys are unique per cell
+                    best = (uint32_t)s_key[w];
+                    bestCell = s_cell[w];
+                }
+            // key 0: no visited value >= 0, the reference keeps (0, 0, 0)
+            const int pc = best != 0 ? bestCell : 0;
+            const int pI = pc / D, pJ = pc % D;
+            // the neighbourhood (column through the peak, then the row) through
+            // LDS: every cell inside the plane has one holder lane
+            float *__restrict__ s_nb = (float *)&s_x[0];
+            const int nh = p.nbr_half, L = 2 * nh + 1;
+            if (p.nbr) {
+                if (sc == ((pI >> 5) & 1)) {  // row pI: slot vs of lane group gs
+                    const int ps = pI >> 6, ms = pI & 31;
+                    const int gs = (ms >> 2) & 1, vs = ((ms >> 3) << 2) | (ms & 3);
+                    float rv[2] = {0.0f, 0.0f};
+#pragma unroll
+                    for (int pi = 0; pi < 2; pi++)
+#pragma unroll
+                        for (int v = 0; v < NS; v++)
+                            if (pi == ps && v == vs) {
+                                rv[0] = R[pi][0][v];
+                                rv[1] = R[pi][1][v];
+                            }
+#pragma unroll
+                    for (int qi = 0; qi < 2; qi++) {
+                        const int b = MT * (sr + 2 * qi) + j, d = b - pJ;
+                        if (g == gs && b < D && d >= -nh && d <= nh) s_nb[L + d + nh] = rv[qi] * unscale;
+                    }
+                }
+                if (sr == ((pJ >> 5) & 1) && j == (pJ & 31)) {  // column pJ
+                    const int qs = pJ >> 6;
+#pragma unroll
+                    for (int pi = 0; pi < 2; pi++)
+#pragma unroll
+                        for (int v = 0; v < NS; v++) {
+                            const int a = MT * (sc + 2 * pi) + mrow<MT>(v, g), d = a - pI;
+                            if (a < D && d >= -nh && d <= nh)
+                                s_nb[d + nh] = (qs == 0 ? R[pi][0][v] : R[pi][1][v]) * unscale;
+                        }
+                }
+            }
+            // the peak value is the maximum (the winning cell equals it)
+            float pv = 0.0f;
+            if (best != 0) pv = vmax * unscale;
+            __syncthreads();
+            if (wave == 0) {
+                if (lane == 0) {
+                    if (p.peaks) {
+                        p.peaks[(size_t)ia * 3 + 0] = (float)pI;
+                        p.peaks[(size_t)ia * 3 + 1] = (float)pJ;
+                        p.peaks[(size_t)ia * 3 + 2] = pv;
+                    }
+                    if (p.min_floor) p.min_floor[ia] = mn * unscale;
+                }
+                // cells outside the plane are 0 (as in the exact pass)
+                if (p.nbr && lane < L) {
+                    float *nb = p.nbr + (size_t)ia * 2 * L;
+                    const int d = lane - nh;
+                    nb[lane] = (pI + d >= 0 && pI + d < D) ? s_nb[lane] : 0.0f;
+                    nb[L + lane] = (pJ + d >= 0 && pJ + d < D) ? s_nb[L + lane] : 0.0f;
+                }
+            }
         }
         __syncthreads();  // the buffer is rewritten by the next IA
     }
