@@ -201,6 +201,26 @@ __device__ __forceinline__ void static_for(F &&f)
     }
 }
 
+// The FromCenter tie key (from_center_tie) and cell of the best of a lane's
+// cells that equal the maximum: bit idx of mask marks cell idx, which cell()
+// places at plane row a, column b.  The loop runs once per marked cell.
+template <typename F>
+__device__ __forceinline__ void best_tie(uint64_t mask, int h, int D, F cell, uint32_t &best,
+                                         int &bestCell)
+{
+    while (mask) {
+        const int idx = __builtin_ctzll(mask);
+        mask &= mask - 1;
+        int a, b;
+        cell(idx, a, b);
+        const uint32_t tie = from_center_tie(a, b, h);
+        if (tie > best) {
+            best = tie;
+            bestCell = a * D + b;
+        }
+    }
+}
+
 struct MfmaParams {
     PassParams p;
     const u4 *frags;  // NF fragments x {hi, lo} x 64 lanes
@@ -548,12 +568,12 @@ __global__ __launch_bounds__(64 * MF_WAVES, QLPIV_MF_MINW) void xcorr_mfma_kerne
                 vmax = fmaxf(vmax, __shfl_xor(vmax, o, 64));
                 mn = fminf(mn, __shfl_xor(mn, o, 64));
             }
-            // phase 2: each lane notes its cells equal to the maximum (almost
-            // always none or one) and computes the tie key of that one
+            // phase 2: each lane marks its cells equal to the maximum (almost
+            // always none or one) and computes the tie keys of those only
             uint32_t best = 0;
             int bestCell = 0;
             if (vmax >= 0.0f) {
-                int cnt = 0, code = 0;
+                uint64_t mask = 0;
 #pragma unroll
                 for (int rt = 0; rt < 2; rt++)
 #pragma unroll
@@ -561,32 +581,15 @@ __global__ __launch_bounds__(64 * MF_WAVES, QLPIV_MF_MINW) void xcorr_mfma_kerne
 #pragma unroll
                         for (int v = 0; v < NS; v++) {
                             const bool m = visited(rt, ct, v) && R[rt][ct][v] == vmax;
-                            cnt += m ? 1 : 0;
-                            code = m ? ((rt * 2 + ct) * NS + v) : code;
+                            mask |= m ? (uint64_t)1 << ((rt * 2 + ct) * NS + v) : 0;
                         }
-                if (__ballot(cnt > 1) == 0) {
-                    if (cnt == 1) {
-                        const int v = code % NS, ct = (code / NS) & 1, rt = code / (2 * NS);
-                        const int a = MT * rt + mrow<MT>(v, g), b = MT * ct + j;
-                        best = from_center_tie(a, b, h);
-                        bestCell = a * D + b;
-                    }
-                } else {  // a lane holds several equal maxima: every cell's key
-#pragma unroll
-                    for (int rt = 0; rt < 2; rt++)
-#pragma unroll
-                        for (int ct = 0; ct < 2; ct++)
-#pragma unroll
-                            for (int v = 0; v < NS; v++) {
-                                if (!(visited(rt, ct, v) && R[rt][ct][v] == vmax)) continue;
-                                const int a = MT * rt + mrow<MT>(v, g), b = MT * ct + j;
-                                const uint32_t tie = from_center_tie(a, b, h);
-                                if (tie > best) {
-                                    best = tie;
-                                    bestCell = a * D + b;
-                                }
-                            }
-                }
+                best_tie(mask, h, D,
+                         [&](int idx, int &a, int &b) {
+                             const int v = idx % NS, ct = (idx / NS) & 1, rt = idx / (2 * NS);
+                             a = MT * rt + mrow<MT>(v, g);
+                             b = MT * ct + j;
+                         },
+                         best, bestCell);
             }
 #pragma unroll
             for (int o = 32; o >= 1; o >>= 1) {
@@ -1124,12 +1127,12 @@ __global__ __launch_bounds__(m64::NT, 2) void xcorr_mfma64_kernel(PassParams p,
             __syncthreads();
             vmax = fmaxf(fmaxf(s_vmax[0], s_vmax[1]), fmaxf(s_vmax[2], s_vmax[3]));
             mn = fminf(fminf(s_min[0], s_min[1]), fminf(s_min[2], s_min[3]));
-            // phase 2: the tie key of the cells equal to the maximum (a lane
+            // phase 2: the tie keys of the cells equal to the maximum (a lane
             // holds none or one almost always)
             uint32_t best = 0;
             int bestCell = 0;
             if (vmax >= 0.0f) {
-                int cnt = 0, code = 0;
+                uint64_t mask = 0;
 #pragma unroll
                 for (int pi = 0; pi < 2; pi++)
 #pragma unroll
@@ -1139,31 +1142,15 @@ __global__ __launch_bounds__(m64::NT, 2) void xcorr_mfma64_kernel(PassParams p,
                             const int a = MT * (sc + 2 * pi) + mrow<MT>(v, g),
                                       b = MT * (sr + 2 * qi) + j;
                             const bool m = visited(a, b) && R[pi][qi][v] == vmax;
-                            cnt += m ? 1 : 0;
-                            code = m ? a * D + b : code;
+                            mask |= m ? (uint64_t)1 << ((pi * 2 + qi) * NS + v) : 0;
                         }
-                if (__ballot(cnt > 1) == 0) {
-                    if (cnt == 1) {
-                        best = from_center_tie(code / D, code % D, D / 2);
-                        bestCell = code;
-                    }
-                } else {  // a lane holds several equal maxima: every cell's key
-#pragma unroll
-                    for (int pi = 0; pi < 2; pi++)
-#pragma unroll
-                        for (int qi = 0; qi < 2; qi++)
-#pragma unroll
-                            for (int v = 0; v < NS; v++) {
-                                const int a = MT * (sc + 2 * pi) + mrow<MT>(v, g),
-                                          b = MT * (sr + 2 * qi) + j;
-                                if (!(visited(a, b) && R[pi][qi][v] == vmax)) continue;
-                                const uint32_t tie = from_center_tie(a, b, D / 2);
-                                if (tie > best) {
-                                    best = tie;
-                                    bestCell = a * D + b;
-                                }
-                            }
-                }
+                best_tie(mask, D / 2, D,
+                         [&](int idx, int &a, int &b) {
+                             const int v = idx % NS, qi = (idx / NS) & 1, pi = idx / (2 * NS);
+                             a = MT * (sc + 2 * pi) + mrow<MT>(v, g);
+                             b = MT * (sr + 2 * qi) + j;
+                         },
+                         best, bestCell);
             }
 #pragma unroll
             for (int o = 32; o >= 1; o >>= 1) {
