@@ -15,8 +15,8 @@
 //            fit neighbourhood (any kernel shape or arithmetic form);
 //   fit    : the sub-pixel fit and the displacement update u += i' - D/2
 //            (Tile.accumulateDisplacement), or the integer peak for a level
-//            before the interpolator start step;
-//   store  : each IA's field value and fit record into its pair's level arrays.
+//            before the interpolator start step, written with the fit record
+//            straight into the IA's pair level arrays.
 // The per-level values are those of a one-pair multi-pass plan
 // (qlp_piv_run_dev) with the same configuration.
 //
@@ -179,27 +179,6 @@ __global__ __launch_bounds__(256) void step_tiles_kernel(
     }
 }
 
-// Each IA's updated displacement and fit record into its pair's level arrays.
-__global__ __launch_bounds__(256) void step_store_kernel(const StepHeader *__restrict__ h,
-                                                         const StepSeg *__restrict__ segs, int cls,
-                                                         const int *__restrict__ seg_of,
-                                                         const float *__restrict__ cu,
-                                                         const float *__restrict__ cv,
-                                                         const float *__restrict__ sub)
-{
-    const int n = h->n[cls];
-    for (int ia = blockIdx.x * blockDim.x + threadIdx.x; ia < n; ia += gridDim.x * blockDim.x) {
-        const StepSeg &S = segs[seg_of[ia]];
-        const int t = ia - S.first;
-        S.u[t] = cu[ia];
-        S.v[t] = cv[ia];
-        if (S.sub) {
-            const float4 r = reinterpret_cast<const float4 *>(sub)[ia];
-            reinterpret_cast<float4 *>(S.sub)[t] = r;
-        }
-    }
-}
-
 struct EngineClass {
     int N = 0;
     size_t cap = 0;
@@ -211,7 +190,7 @@ struct EngineClass {
     int last_n = 0;
     uint8_t *tA = nullptr, *tB = nullptr, *fit_on = nullptr;
     float *mA = nullptr, *mB = nullptr, *cu = nullptr, *cv = nullptr;
-    float *peaks = nullptr, *mf = nullptr, *nbr = nullptr, *sub = nullptr;
+    float *peaks = nullptr, *mf = nullptr, *nbr = nullptr;
     int *seg_of = nullptr;
 };
 
@@ -270,7 +249,7 @@ size_t engine_bytes(const EngineConfig &cfg, const size_t cap[ENG_CLASSES])
     for (int c = 0; c < ENG_CLASSES; c++) {
         const size_t n = cap[c], N = kEngSizes[c];
         if (!n) continue;
-        b += n * (2 * N * N + 1 + 4 * 6 + 12 + 8 * nbr_len() + 16 + 4) + 16 * 256;
+        b += n * (2 * N * N + 1 + 4 * 6 + 12 + 8 * nbr_len() + 4) + 16 * 256;
     }
     if (cap[3]) b += big_work_bytes(cfg, cap[3]);
     return b;
@@ -350,7 +329,6 @@ int engine_create(Context *ctx, const EngineConfig &cfg, const size_t cap[ENG_CL
         if (!rc) rc = ealloc(e, C.peaks, n * 3);
         if (!rc) rc = ealloc(e, C.mf, n);
         if (!rc) rc = ealloc(e, C.nbr, n * 2 * L);
-        if (!rc) rc = ealloc(e, C.sub, n * 4);
         if (!rc) rc = ealloc(e, C.seg_of, n);
     }
     if (!rc && cap[3]) {
@@ -485,15 +463,14 @@ static int enqueue_class(Engine *e, int cls, int shape, hipStream_t s)
     q.shape = shape;
     rc = launch_pass(&sub, q);
     if (rc) return rc;
+    // the fit writes each IA's field value and fit record into its pair level
     const int D = 2 * C.N - 1;
+    FitScatter sc;
+    sc.segs = segs_of(e);
+    sc.seg_of = C.seg_of;
     rc = launch_subpixel_nbr(&sub, e->cfg.subpix_kind, e->cfg.subpix_pixels, e->cfg.subpix_iterations,
-                             (int)C.cap, D, ENG_NBR_HALF, C.nbr, C.mf, C.peaks, C.sub, C.cu, C.cv,
-                             n_dev, C.fit_on);
-    if (rc) return rc;
-    const int grid = (int)std::min<size_t>((C.cap + 255) / 256, 1024);
-    hipLaunchKernelGGL(step_store_kernel, dim3(grid), dim3(256), 0, s, hdr_of(e), segs_of(e), cls,
-                       C.seg_of, C.cu, C.cv, C.sub);
-    rc = check_hip(hipGetLastError(), "engine store launch");
+                             (int)C.cap, D, ENG_NBR_HALF, C.nbr, C.mf, C.peaks, nullptr, C.cu, C.cv,
+                             n_dev, C.fit_on, &sc);
     if (rc) return rc;
     // every workspace was sized before: nothing may have grown during the enqueue
     bool grew = sub.bigwork != e->bigwork;

@@ -175,10 +175,14 @@ int launch_subpixel(Context *ctx, int kind, int pixels, int iterations, int n, i
                     float *peaks_out);
 // n_dev: persistent launch (n is the capacity, the count is in device memory);
 // fit_on: per-IA flag, 0 keeps the integer peak (status SKIPPED).
+struct FitScatter;
+// scatter (optional, with n_dev): each IA's updated displacement and fit record
+// go straight to its pair level's arrays (the step engine's store)
 int launch_subpixel_nbr(Context *ctx, int kind, int pixels, int iterations, int n, int D,
                         int nbr_half, const float *nbr, const float *min_floor,
                         const float *peaks_in, float *peaks_out, float *du, float *dv,
-                        const int *n_dev = nullptr, const uint8_t *fit_on = nullptr);
+                        const int *n_dev = nullptr, const uint8_t *fit_on = nullptr,
+                        const FitScatter *scatter = nullptr);
 int launch_clip(Context *ctx, const uint8_t *img, int H, int W, int n, int N, const int *top,
                 const int *left, uint8_t *tiles, float *maxv);
 // Tile warps of one side (WarpSide, qlpiv_device.h; SIDE_B_SECOND is the
@@ -292,6 +296,12 @@ struct StepSeg {
                             // or STEP_MICRO (micro warp); the first level always clips
     int fit;                // 1: sub-pixel fit; 0: the integer peak (before the start step)
     int pad;
+};
+// Where a step's fits land: IA ia belongs to segment segs[seg_of[ia]] at
+// index ia - first of its level arrays (u, v and, when set, sub).
+struct FitScatter {
+    const StepSeg *segs = nullptr;
+    const int *seg_of = nullptr;
 };
 struct StepHeader {
     int n[ENG_CLASSES];     // IAs of each class in the step

@@ -482,7 +482,7 @@ void release_batched(qlp_server *s, ServeRequest &r)
     if (r.own) (void)hipFreeAsync(r.own, st);
     r.own = nullptr;
     if (r.hu) s->pinned_free.emplace(r.hucap, r.hu);
-    if (r.hv) s->pinned_free.emplace(r.hvcap, r.hv);
+    if (r.hv && r.hvcap) s->pinned_free.emplace(r.hvcap, r.hv);
     r.hu = r.hv = nullptr;
     r.hucap = r.hvcap = 0;
     if (r.geo) r.geo->users--;
@@ -631,11 +631,16 @@ bool step_batched(qlp_server *s)
         if (r->next_level < (int)r->lu.size()) continue;
         // last level: the field to pinned host memory; the request retires with
         // this step's event
-        const size_t n = (size_t)r->nI * r->nJ;
-        int rc = pinned_get(s, n, &r->hu, r->hucap);
-        if (!rc) rc = pinned_get(s, n, &r->hv, r->hvcap);
-        if (!rc) rc = check_hip(hipMemcpyAsync(r->hu, r->lu.back(), n * 4, hipMemcpyDeviceToHost, st), "server result");
-        if (!rc) rc = check_hip(hipMemcpyAsync(r->hv, r->lv.back(), n * 4, hipMemcpyDeviceToHost, st), "server result");
+        // u and v of a level are adjacent in the request's field allocation
+        // (v at u + padded): one copy into one pinned buffer
+        const size_t n = (size_t)r->nI * r->nJ, padded = (n + 63) & ~(size_t)63;
+        int rc = pinned_get(s, 2 * padded, &r->hu, r->hucap);
+        if (!rc) {
+            r->hv = r->hu + padded;
+            r->hvcap = 0;  // part of hu's buffer
+            rc = check_hip(hipMemcpyAsync(r->hu, r->lu.back(), 2 * padded * 4, hipMemcpyDeviceToHost, st),
+                           "server result");
+        }
         if (rc) {
             fail_request(s, r, rc, "server: result copy");
             (void)hipStreamSynchronize(st);

@@ -435,6 +435,36 @@ __device__ void guo_axis(int iterations, int half, int c, const float *s, float 
 // neighbourhoods exported by the fused correlation pass (FROM_NBR true; items
 // = n, followed by the displacement update u += i' - D/2, v += j' - D/2 of
 // the multi-pass loop, Tile.accumulateDisplacement :103-136).
+// The fit record of IA ia and the displacement update of the multi-pass loop,
+// u += i' - D/2, v += j' - D/2 (MaxCrossResult.getNthDisplacementU :141-149,
+// Tile.accumulateDisplacement :103-136 with out-of-bound clipping allowed):
+// into peaks_out / du / dv, or with a scatter straight into the IA's pair level.
+__device__ __forceinline__ void fit_result(int ia, int D, const float res[4],
+                                           float *__restrict__ peaks_out, float *__restrict__ du,
+                                           float *__restrict__ dv, const FitScatter &sc)
+{
+    const int c = D / 2;
+    if (sc.segs) {
+        const StepSeg &S = sc.segs[sc.seg_of[ia]];
+        const int t = ia - S.first;
+        S.u[t] = du[ia] + (res[0] - (float)c);
+        S.v[t] = dv[ia] + (res[1] - (float)c);
+        if (S.sub) reinterpret_cast<float4 *>(S.sub)[t] = make_float4(res[0], res[1], res[2], res[3]);
+        return;
+    }
+    if (peaks_out) {
+        float *o = peaks_out + (size_t)ia * 4;
+        o[0] = res[0];
+        o[1] = res[1];
+        o[2] = res[2];
+        o[3] = res[3];
+    }
+    if (du) {
+        du[ia] = du[ia] + (res[0] - (float)c);
+        dv[ia] = dv[ia] + (res[1] - (float)c);
+    }
+}
+
 template <bool FROM_NBR, int HM>
 __device__ __forceinline__ void guo_item(int gid, int pixels, int iterations, int items, int D,
                                          int P, const float *__restrict__ src, int nbr_half,
@@ -442,7 +472,8 @@ __device__ __forceinline__ void guo_item(int gid, int pixels, int iterations, in
                                          const float *__restrict__ peaks_in,
                                          float *__restrict__ peaks_out, float *__restrict__ du,
                                          float *__restrict__ dv,
-                                         const uint8_t *__restrict__ fit_on)
+                                         const uint8_t *__restrict__ fit_on,
+                                         const FitScatter &sc)
 {
     const int item = gid >> 1, axis = gid & 1;
     const bool in = item < items;
@@ -507,18 +538,7 @@ __device__ __forceinline__ void guo_item(int gid, int pixels, int iterations, in
         res[2] = mx;
         res[3] = (float)((yValid ? ST_I : 0) | (xValid ? ST_J : 0));
     }
-    if (peaks_out) {
-        float *o = peaks_out + (size_t)item * 4;
-        o[0] = res[0];
-        o[1] = res[1];
-        o[2] = res[2];
-        o[3] = res[3];
-    }
-    if (FROM_NBR && du) {
-        const int c = D / 2;
-        du[item] = du[item] + (res[0] - (float)c);
-        dv[item] = dv[item] + (res[1] - (float)c);
-    }
+    fit_result(item, D, res, peaks_out, FROM_NBR ? du : nullptr, dv, sc);
 }
 
 //   n_dev  : persistent launch: the item count in device memory (items is then
@@ -533,12 +553,13 @@ __global__ __launch_bounds__(256) void guo_kernel(int pixels, int iterations, in
                                                   float *__restrict__ peaks_out,
                                                   float *__restrict__ du, float *__restrict__ dv,
                                                   const int *__restrict__ n_dev,
-                                                  const uint8_t *__restrict__ fit_on)
+                                                  const uint8_t *__restrict__ fit_on,
+                                                  FitScatter sc)
 {
     if (!PERSIST) {
         guo_item<FROM_NBR, HM>((int)(blockIdx.x * blockDim.x + threadIdx.x), pixels, iterations,
                                items, D, P, src, nbr_half, min_floor, peaks_in, peaks_out, du, dv,
-                               fit_on);
+                               fit_on, sc);
         return;
     }
     items = *n_dev;
@@ -547,7 +568,7 @@ __global__ __launch_bounds__(256) void guo_kernel(int pixels, int iterations, in
         int tx = (int)threadIdx.x;
         asm volatile("" : "+v"(tx));  // nothing built on it is hoisted out of the loop
         guo_item<FROM_NBR, HM>(base + tx, pixels, iterations, items, D, P, src, nbr_half,
-                               min_floor, peaks_in, peaks_out, du, dv, fit_on);
+                               min_floor, peaks_in, peaks_out, du, dv, fit_on, sc);
     }
 }
 
@@ -597,7 +618,8 @@ __device__ __forceinline__ void subpixel_nbr_item(int ia, int kind, int pixels, 
                                                           float *__restrict__ peaks_out,
                                                           float *__restrict__ du,
                                                           float *__restrict__ dv,
-                                                          const uint8_t *__restrict__ fit_on)
+                                                          const uint8_t *__restrict__ fit_on,
+                                                          const FitScatter &sc)
 {
     const int L = 2 * nbr_half + 1;
     const float *nb = nbr + (size_t)ia * 2 * L;
@@ -617,18 +639,8 @@ __device__ __forceinline__ void subpixel_nbr_item(int ia, int kind, int pixels, 
         fit_peak(kind, pixels, iterations, D, pI, pJ, peaks_in[(size_t)ia * 3 + 2], min_floor[ia],
                  col, row, res);
     }
-    if (peaks_out) {
-        float *o = peaks_out + (size_t)ia * 4;
-        o[0] = res[0];
-        o[1] = res[1];
-        o[2] = res[2];
-        o[3] = res[3];
-    }
-    if (du) {
-        const int c = D / 2;
-        du[ia] = du[ia] + (res[0] - (float)c);
-        dv[ia] = dv[ia] + (res[1] - (float)c);
-    }}
+    fit_result(ia, D, res, peaks_out, du, dv, sc);
+}
 
 template <bool PERSIST>
 __global__ __launch_bounds__(256) void subpixel_nbr_kernel(int kind, int pixels, int iterations,
@@ -640,13 +652,14 @@ __global__ __launch_bounds__(256) void subpixel_nbr_kernel(int kind, int pixels,
                                                           float *__restrict__ du,
                                                           float *__restrict__ dv,
                                                           const int *__restrict__ n_dev,
-                                                          const uint8_t *__restrict__ fit_on)
+                                                          const uint8_t *__restrict__ fit_on,
+                                                          FitScatter sc)
 {
     if (!PERSIST) {
         const int ia = blockIdx.x * blockDim.x + threadIdx.x;
         if (ia < n)
             subpixel_nbr_item(ia, kind, pixels, iterations, n, D, nbr_half, nbr, min_floor,
-                              peaks_in, peaks_out, du, dv, fit_on);
+                              peaks_in, peaks_out, du, dv, fit_on, sc);
         return;
     }
     n = *n_dev;  // persistent launch: the grid strides over the IAs
@@ -655,7 +668,7 @@ __global__ __launch_bounds__(256) void subpixel_nbr_kernel(int kind, int pixels,
         asm volatile("" : "+v"(tx));  // nothing built on it is hoisted out of the loop
         if (base + tx < n)
             subpixel_nbr_item(base + tx, kind, pixels, iterations, n, D, nbr_half, nbr, min_floor,
-                              peaks_in, peaks_out, du, dv, fit_on);
+                              peaks_in, peaks_out, du, dv, fit_on, sc);
     }
 }
 
@@ -685,7 +698,7 @@ int launch_subpixel(Context *ctx, int kind, int pixels, int iterations, int n, i
                : pixels == 7 ? guo_kernel<false, 3> : guo_kernel<false, NBR_MAX_HALF>;
         hipLaunchKernelGGL(k, dim3((2 * total + 255) / 256), dim3(256), 0, ctx->stream, pixels,
                            iterations, total, D, n_peaks, planes, 0, min_floor, peaks_in,
-                           peaks_out, nullptr, nullptr, nullptr, nullptr);
+                           peaks_out, nullptr, nullptr, nullptr, nullptr, FitScatter{});
     } else
         hipLaunchKernelGGL(subpixel_kernel, dim3((total + 255) / 256), dim3(256), 0, ctx->stream,
                            kind, pixels, iterations, n, D, n_peaks, planes, min_floor, peaks_in,
@@ -696,9 +709,12 @@ int launch_subpixel(Context *ctx, int kind, int pixels, int iterations, int n, i
 int launch_subpixel_nbr(Context *ctx, int kind, int pixels, int iterations, int n, int D,
                         int nbr_half, const float *nbr, const float *min_floor,
                         const float *peaks_in, float *peaks_out, float *du, float *dv,
-                        const int *n_dev, const uint8_t *fit_on)
+                        const int *n_dev, const uint8_t *fit_on, const FitScatter *scatter)
 {
     if (n == 0) return QLP_OK;
+    if (scatter && (!n_dev || !du || !dv))
+        return fail(QLP_ERR_ARG, "subpixel: a scatter takes a persistent launch and the displacements");
+    const FitScatter sc = scatter ? *scatter : FitScatter{};
     if (kind >= 0) {
         int rc = check_subpixel_args(kind, pixels, iterations);
         if (rc) return rc;
@@ -716,13 +732,13 @@ int launch_subpixel_nbr(Context *ctx, int kind, int pixels, int iterations, int 
         int grid = (2 * n + 255) / 256;
         if (n_dev) grid = std::min(grid, resident_groups(k, 256));
         hipLaunchKernelGGL(k, dim3(grid), dim3(256), 0, ctx->stream, pixels, iterations, n, D, 1,
-                           nbr, nbr_half, min_floor, peaks_in, peaks_out, du, dv, n_dev, fit_on);
+                           nbr, nbr_half, min_floor, peaks_in, peaks_out, du, dv, n_dev, fit_on, sc);
     } else {
         auto k = n_dev ? subpixel_nbr_kernel<true> : subpixel_nbr_kernel<false>;
         int grid = (n + 255) / 256;
         if (n_dev) grid = std::min(grid, resident_groups(k, 256));
         hipLaunchKernelGGL(k, dim3(grid), dim3(256), 0, ctx->stream, kind, pixels, iterations, n,
-                           D, nbr_half, nbr, min_floor, peaks_in, peaks_out, du, dv, n_dev, fit_on);
+                           D, nbr_half, nbr, min_floor, peaks_in, peaks_out, du, dv, n_dev, fit_on, sc);
     }
     return check_hip(hipGetLastError(), "subpixel kernel launch");
 }
