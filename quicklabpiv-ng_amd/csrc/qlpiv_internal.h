@@ -249,6 +249,9 @@ struct ValidLaunch {
     int *stats = nullptr;         // pairs*4 or null
     float *bu = nullptr, *bv = nullptr;
     int *flags = nullptr, *done = nullptr, *act = nullptr, *act_count = nullptr;
+    // act_count is 0 on entry (zeroed when its workspace was allocated; every
+    // sweep returns it to 0): the launch then skips clearing it
+    bool counts_zero = false;
 };
 int check_valid_config(const qlp_valid_config &cfg);
 void spiral_rank(int nI, int nJ, int *rank);

@@ -422,6 +422,12 @@ try {
         if (!rc && valid) {
             rc = check_hip(hipMalloc(&L.rank, n * 4), "hipMalloc plan");
             if (!rc) rc = check_hip(hipMalloc(&L.vwork, valid_work_bytes(pairs, (int)n)), "hipMalloc plan");
+            if (!rc) {  // the active counts start at 0 and every sweep leaves them at 0
+                ValidLaunch z;
+                valid_work_carve(L.vwork, pairs, (int)n, z);
+                rc = check_hip(hipMemset(z.act_count, 0, (size_t)pairs * 4), "plan validation counts");
+                if (!rc) rc = check_hip(hipDeviceSynchronize(), "plan validation counts");
+            }
             if (!rc) rc = check_hip(hipMalloc(&L.invalid, pn), "hipMalloc plan");
             if (!rc) rc = check_hip(hipMalloc(&L.stats, (size_t)pairs * 16), "hipMalloc plan");
             if (!rc) {
@@ -807,6 +813,7 @@ static int run_levels(qlp_piv_plan *p, const uint8_t *imgsA, const uint8_t *imgs
             vl.invalid = L.invalid;
             vl.stats = L.stats;
             valid_work_carve(L.vwork, p->pairs, L.n, vl);
+            vl.counts_zero = true;
             rc = launch_validate(ctx, c.valid, vl);
             if (rc) return rc;
         }

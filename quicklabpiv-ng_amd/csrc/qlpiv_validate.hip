@@ -145,12 +145,64 @@ __device__ __forceinline__ bool flt_before(float x, float y)
     return x < y;
 }
 
+// The 3 x 3 window around tile (i, j) without the tile: slot s is window
+// position (a, b) in reading order, (1, 1) skipped.  Its displacements (and
+// validity flags) cannot change during a visit of (i, j) (no neighbour is
+// visited in the same round), so a visit loads them once.  The loads are
+// unconditional (a slot outside the grid reads the tile itself and is then
+// ignored), so that they are in flight together.
+struct Window {
+    float nu[8], nv[8];
+    int fl[8];  // flags (loaded by load_window_flags)
+    bool present[8];
+    int k[8];   // grid index, -1 outside the grid
+};
+
+__device__ __forceinline__ void window_slot(int s, int &a, int &b)
+{
+    a = s < 4 ? s / 3 : (s + 1) / 3;
+    b = s < 4 ? s % 3 : (s + 1) % 3;
+}
+
+__device__ __forceinline__ Window load_window(const VParams &p, size_t base, int i, int j)
+{
+    Window w;
+    size_t o[8];
+#pragma unroll
+    for (int s = 0; s < 8; s++) {
+        int a, b;
+        window_slot(s, a, b);
+        const int ii = i + a - 1, jj = j + b - 1;
+        w.present[s] = ii >= 0 && ii < p.nI && jj >= 0 && jj < p.nJ;
+        w.k[s] = w.present[s] ? ii * p.nJ + jj : -1;
+        o[s] = base + (size_t)(w.present[s] ? ii * p.nJ + jj : i * p.nJ + j);
+    }
+#pragma unroll
+    for (int s = 0; s < 8; s++) {
+        w.nu[s] = ldc(p.u + o[s]);
+        w.nv[s] = ldc(p.v + o[s]);
+        w.fl[s] = 0;
+    }
+#pragma unroll
+    for (int s = 0; s < 8; s++) {
+        if (!w.present[s]) w.nu[s] = w.nv[s] = 0.0f;
+    }
+    return w;
+}
+
+__device__ __forceinline__ void load_window_flags(const VParams &p, size_t base, int i, int j,
+                                                  Window &w)
+{
+#pragma unroll
+    for (int s = 0; s < 8; s++) w.fl[s] = ldc(p.flags + base + (w.k[s] >= 0 ? w.k[s] : i * p.nJ + j));
+}
+
 // NormalizedMedianValidator.validateVector :63-136: the new invalid state.
 // The eight window slots stay in registers; sorted positions are found by
 // ranks (ties by window order, which is the insertion order of the
 // reference's list, so the stable sort's order).
-__device__ bool median_invalid(const VParams &p, size_t base, int i, int j, float tu, float tv,
-                               float thr, float eps)
+__device__ bool median_invalid(const VParams &p, const Window &w, size_t base, int i, int j,
+                               float tu, float tv, float thr, float eps)
 {
     float nu[8], nv[8], key[8];
     bool present[8];
@@ -158,16 +210,10 @@ __device__ bool median_invalid(const VParams &p, size_t base, int i, int j, floa
     bool anynan = false;
 #pragma unroll
     for (int s = 0; s < 8; s++) {
-        const int a = s < 4 ? s / 3 : (s + 1) / 3, b = s < 4 ? s % 3 : (s + 1) % 3;
-        const int ii = i + a - 1, jj = j + b - 1;
-        present[s] = ii >= 0 && ii < p.nI && jj >= 0 && jj < p.nJ;
-        nu[s] = nv[s] = 0.0f;
-        if (present[s]) {
-            const size_t o = base + (size_t)ii * p.nJ + jj;
-            nu[s] = ldc(p.u + o);
-            nv[s] = ldc(p.v + o);
-            cnt++;
-        }
+        present[s] = w.present[s];
+        nu[s] = w.nu[s];
+        nv[s] = w.nv[s];
+        cnt += present[s] ? 1 : 0;
         key[s] = nu[s] * nu[s] + nv[s] * nv[s];
         anynan |= present[s] && isnan(key[s]);
     }
@@ -236,31 +282,30 @@ __device__ bool median_invalid(const VParams &p, size_t base, int i, int j, floa
 
 // DifferenceValidator.validateVector :34-63: the tile's own state is kept when
 // none of the three neighbours it consults (upper-left, upper, left) is valid.
-__device__ bool difference_invalid(const VParams &p, size_t base, int i, int j, float tu, float tv,
-                                   float thr, bool cur)
+// (window slots 0, 1, 3: upper-left, upper, left; the window's flags loaded)
+__device__ bool difference_invalid(const Window &w, float tu, float tv, float thr, bool cur)
 {
     const float thr2 = thr * thr;
     bool inv = cur;
-    for (int a = 0; a < 2; a++)
-        for (int b = 0; b < 2; b++) {
-            if (a == 1 && b == 1) continue;
-            const int ii = i + a - 1, jj = j + b - 1;
-            if (ii < 0 || ii >= p.nI || jj < 0 || jj >= p.nJ) continue;
-            const size_t o = base + (size_t)ii * p.nJ + jj;
-            if (ldc(p.flags + o) & F_INVALID) continue;
-            const float du = ldc(p.u + o) - tu, dv = ldc(p.v + o) - tv;
-            const float distance = du * du + dv * dv;
-            inv = !(distance < thr2);
-        }
+    const int slots[3] = {0, 1, 3};
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        const int s = slots[q];
+        if (!w.present[s] || (w.fl[s] & F_INVALID)) continue;
+        const float du = w.nu[s] - tu, dv = w.nv[s] - tv;
+        const float distance = du * du + dv * dv;
+        inv = !(distance < thr2);
+    }
     return inv;
 }
 
-__device__ __forceinline__ bool stage_invalid(const VParams &p, int s, size_t base, int i, int j,
-                                              float tu, float tv, bool cur)
+__device__ __forceinline__ bool stage_invalid(const VParams &p, const Window &w, int s, size_t base,
+                                              int i, int j, float tu, float tv, bool cur)
 {
     const VStage &st = p.st[s];
-    if (st.validator == QLP_VALIDATOR_MEDIAN) return median_invalid(p, base, i, j, tu, tv, st.threshold, st.eps);
-    return difference_invalid(p, base, i, j, tu, tv, st.threshold, cur);
+    if (st.validator == QLP_VALIDATOR_MEDIAN)
+        return median_invalid(p, w, base, i, j, tu, tv, st.threshold, st.eps);
+    return difference_invalid(w, tu, tv, st.threshold, cur);
 }
 
 // Neighbour of the 3x3 window (a, b in 0..2) or -1 outside the grid.
@@ -276,17 +321,35 @@ struct NbTile {
     float u, v, top, left;
 };
 
-__device__ __forceinline__ NbTile nb_tile(const VParams &p, size_t base, int i, int j, int a, int b)
+constexpr int PK_PRE = 4;  // peaks of a tile loaded with its visit's other inputs
+
+// Everything one sweep step reads for an active tile, loaded in one batch:
+// its own state and peaks, the window (displacements and flags), the
+// neighbours' spiral ranks, done rounds and origins.  When the tile turns out
+// ready, no neighbour the loads saw can change before its visit ends: the
+// earlier ones were visited in earlier rounds, the later ones wait for it.
+struct VisitIn {
+    Window w;
+    int rk[8], dn[8];
+    float ntop[8], nleft[8];
+    int rank, done, flag, pcount;
+    float tu, tv, bu, bv, ttop, tleft;
+    float pk[PK_PRE][2];
+};
+
+// Window neighbour (a, b) from a visit's loaded inputs: validity, displacement
+// and tile origin.
+__device__ __forceinline__ NbTile nb_tile(const VisitIn &d, int a, int b)
 {
     NbTile t{false, 0.0f, 0.0f, 0.0f, 0.0f};
-    const int k = nb_index(p, i, j, a, b);
-    if (k < 0) return t;
-    t.top = (float)p.top[k];
-    t.left = (float)p.left[k];
-    if (ldc(p.flags + base + k) & F_INVALID) return t;
+    const int s = a * 3 + b - (a * 3 + b > 4 ? 1 : 0);
+    if (d.w.k[s] < 0) return t;
+    t.top = d.ntop[s];
+    t.left = d.nleft[s];
+    if (d.w.fl[s] & F_INVALID) return t;
     t.ok = true;
-    t.u = ldc(p.u + base + k);
-    t.v = ldc(p.v + base + k);
+    t.u = d.w.nu[s];
+    t.v = d.w.nv[s];
     return t;
 }
 
@@ -330,13 +393,13 @@ __device__ __forceinline__ float weight(float num, float den, int &err)
 }
 
 // BilinearReplacement.replaceVector :145-427 on an invalid tile.
-__device__ void replace_bilinear(const VParams &p, size_t base, int i, int j, float ttop,
+__device__ void replace_bilinear(const VParams &p, const VisitIn &d, int i, int j, float ttop,
                                  float tleft, float &tu, float &tv, bool &inv, int &err)
 {
-    const NbTile left = nb_tile(p, base, i, j, 1, 0), right = nb_tile(p, base, i, j, 1, 2);
-    const NbTile top = nb_tile(p, base, i, j, 0, 1), bottom = nb_tile(p, base, i, j, 2, 1);
-    const NbTile tl = nb_tile(p, base, i, j, 0, 0), br = nb_tile(p, base, i, j, 2, 2);
-    const NbTile bl = nb_tile(p, base, i, j, 2, 0), tr = nb_tile(p, base, i, j, 0, 2);
+    const NbTile left = nb_tile(d, 1, 0), right = nb_tile(d, 1, 2);
+    const NbTile top = nb_tile(d, 0, 1), bottom = nb_tile(d, 2, 1);
+    const NbTile tl = nb_tile(d, 0, 0), br = nb_tile(d, 2, 2);
+    const NbTile bl = nb_tile(d, 2, 0), tr = nb_tile(d, 0, 2);
     const bool tlx = nb_index(p, i, j, 0, 0) >= 0, brx = nb_index(p, i, j, 2, 2) >= 0;
     const bool blx = nb_index(p, i, j, 2, 0) >= 0, trx = nb_index(p, i, j, 0, 2) >= 0;
     // tile alignment checks :174-189
@@ -400,7 +463,7 @@ __device__ void replace_bilinear(const VParams &p, size_t base, int i, int j, fl
         for (int a = 0; a < 3; a++)
             for (int b = 0; b < 3; b++) {
                 if (a == 1 && b == 1) continue;
-                const NbTile t = nb_tile(p, base, i, j, a, b);
+                const NbTile t = nb_tile(d, a, b);
                 if (t.ok) {
                     ru = t.u;
                     rv = t.v;
@@ -432,15 +495,26 @@ __device__ void replace_bilinear(const VParams &p, size_t base, int i, int j, fl
 
 // SecondaryPeakReplacement.replaceVector :41-75: undo the main peak's
 // increment, add the chosen peak's.
-__device__ __forceinline__ void replace_secondary(const VParams &p, size_t gt, int k, float &tu,
+__device__ __forceinline__ void replace_secondary(const VParams &p, size_t gt, int k, int total,
+                                                  const float (&pk)[PK_PRE][2], float &tu,
                                                   float &tv, bool &inv)
 {
-    const int total = p.pcount ? p.pcount[gt] : p.P;
     if (total <= k || k >= p.P) return;
     const float *m = p.pk + gt * p.P * p.pk_stride;
-    const float uOld = m[0] - p.pk_off, vOld = m[1] - p.pk_off;
-    const float uNew = m[(size_t)k * p.pk_stride] - p.pk_off;
-    const float vNew = m[(size_t)k * p.pk_stride + 1] - p.pk_off;
+    float ku = 0.0f, kv = 0.0f;
+#pragma unroll
+    for (int q = 1; q < PK_PRE; q++)
+        if (q == k) {
+            ku = pk[q][0];
+            kv = pk[q][1];
+        }
+    if (k >= PK_PRE) {
+        ku = m[(size_t)k * p.pk_stride];
+        kv = m[(size_t)k * p.pk_stride + 1];
+    }
+    const float uOld = pk[0][0] - p.pk_off, vOld = pk[0][1] - p.pk_off;
+    const float uNew = ku - p.pk_off;
+    const float vNew = kv - p.pk_off;
     float u = tu, v = tv;
     u -= uOld;
     v -= vOld;
@@ -463,8 +537,11 @@ __global__ __launch_bounds__(256) void valid_init_kernel(VParams p, int pairs)
     p.bv[gid] = tv;
     p.done[gid] = -1;
     bool inv = true;
-    if (p.st[0].validator == QLP_VALIDATOR_MEDIAN)
-        inv = median_invalid(p, base, t / p.nJ, t % p.nJ, tu, tv, p.st[0].threshold, p.st[0].eps);
+    if (p.st[0].validator == QLP_VALIDATOR_MEDIAN) {
+        const int i = t / p.nJ, j = t % p.nJ;
+        inv = median_invalid(p, load_window(p, base, i, j), base, i, j, tu, tv, p.st[0].threshold,
+                             p.st[0].eps);
+    }
     p.flags[gid] = inv ? F_INVALID : F_LOCKED;
     // a locked tile ends valid; the sweep writes the flags of the active ones
     if (p.invalid) p.invalid[gid] = 0;
@@ -474,58 +551,77 @@ __global__ __launch_bounds__(256) void valid_init_kernel(VParams p, int pairs)
     }
 }
 
-// Visit of one active tile by the validator loop (:283-353).
-__device__ void visit(const VParams &p, size_t base, int t, int &corrected, int &err)
+__device__ __forceinline__ void load_visit(const VParams &p, size_t base, int t, VisitIn &d)
 {
     const int i = t / p.nJ, j = t - (t / p.nJ) * p.nJ;
     const size_t gt = base + t;
-    float tu = p.u[gt], tv = p.v[gt];
-    bool inv = (p.flags[gt] & F_INVALID) != 0;
-    inv = stage_invalid(p, 0, base, i, j, tu, tv, inv);
-    const float ttop = (float)p.top[t], tleft = (float)p.left[t];
+    d.w = load_window(p, base, i, j);
+    load_window_flags(p, base, i, j, d.w);
+#pragma unroll
+    for (int s = 0; s < 8; s++) {
+        const int kk = d.w.k[s] < 0 ? t : d.w.k[s];
+        d.rk[s] = p.rank[kk];
+        d.dn[s] = ldc(p.done + base + kk);
+        d.ntop[s] = (float)p.top[kk];
+        d.nleft[s] = (float)p.left[kk];
+    }
+    d.rank = p.rank[t];
+    d.done = p.done[gt];
+    d.flag = p.flags[gt];
+    d.tu = p.u[gt];
+    d.tv = p.v[gt];
+    d.bu = p.bu[gt];
+    d.bv = p.bv[gt];
+    d.ttop = (float)p.top[t];
+    d.tleft = (float)p.left[t];
+    d.pcount = p.pcount ? p.pcount[gt] : p.P;
+    const float *m = p.pk + gt * p.P * p.pk_stride;
+#pragma unroll
+    for (int q = 0; q < PK_PRE; q++) {
+        const int qq = q < p.P ? q : 0;
+        d.pk[q][0] = m[(size_t)qq * p.pk_stride];
+        d.pk[q][1] = m[(size_t)qq * p.pk_stride + 1];
+    }
+}
+
+// A tile is ready once every neighbour visited before it in the spiral (and not
+// locked) has been visited in this sweep, in an earlier round.
+__device__ __forceinline__ bool ready(const VisitIn &d, int sweep_start, int round)
+{
+    bool ok = true;
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        if (d.w.k[q] < 0 || d.rk[q] > d.rank || (d.w.fl[q] & F_LOCKED)) continue;
+        if (d.dn[q] < sweep_start || d.dn[q] >= round) ok = false;
+    }
+    return ok;
+}
+
+// Visit of one active tile by the validator loop (:283-353).
+__device__ void visit(const VParams &p, size_t base, int t, const VisitIn &d, int &corrected,
+                      int &err)
+{
+    const int i = t / p.nJ, j = t - (t / p.nJ) * p.nJ;
+    const size_t gt = base + t;
+    float tu = d.tu, tv = d.tv;
+    bool inv = (d.flag & F_INVALID) != 0;
+    inv = stage_invalid(p, d.w, 0, base, i, j, tu, tv, inv);
     for (int s = 0; s < p.n_stages && inv; s++) {
         const int rep = p.st[s].replacement;
         if (rep != QLP_REPLACE_NONE) {
-            tu = p.bu[gt];  // Tile.restoreDisplacement
-            tv = p.bv[gt];
-            if (rep == QLP_REPLACE_SECONDARY_PEAK) replace_secondary(p, gt, p.st[s].peak_index, tu, tv, inv);
-            else replace_bilinear(p, base, i, j, ttop, tleft, tu, tv, inv, err);
+            tu = d.bu;  // Tile.restoreDisplacement
+            tv = d.bv;
+            if (rep == QLP_REPLACE_SECONDARY_PEAK)
+                replace_secondary(p, gt, p.st[s].peak_index, d.pcount, d.pk, tu, tv, inv);
+            else
+                replace_bilinear(p, d, i, j, d.ttop, d.tleft, tu, tv, inv, err);
         }
-        inv = stage_invalid(p, s, base, i, j, tu, tv, inv);
+        inv = stage_invalid(p, d.w, s, base, i, j, tu, tv, inv);
         if (!inv) corrected++;
     }
     p.u[gt] = tu;
     p.v[gt] = tv;
     p.flags[gt] = inv ? F_INVALID : 0;
-}
-
-// A tile is ready once every neighbour visited before it in the spiral (and not
-// locked) has been visited in this sweep, in an earlier round.  All neighbour
-// loads are issued before any decision, so a check costs one memory round trip.
-__device__ bool ready(const VParams &p, size_t base, int t, int sweep_start, int round)
-{
-    const int i = t / p.nJ, j = t - (t / p.nJ) * p.nJ;
-    const int rt = p.rank[t];
-    int k[8], rk[8], fl[8], dn[8];
-#pragma unroll
-    for (int q = 0; q < 8; q++) {
-        const int a = (q + (q >= 4)) / 3, b = (q + (q >= 4)) % 3;  // the 8 neighbours
-        k[q] = nb_index(p, i, j, a, b);
-    }
-#pragma unroll
-    for (int q = 0; q < 8; q++) {
-        const int kk = k[q] < 0 ? t : k[q];
-        rk[q] = p.rank[kk];
-        fl[q] = ldc(p.flags + base + kk);
-        dn[q] = ldc(p.done + base + kk);
-    }
-    bool ok = true;
-#pragma unroll
-    for (int q = 0; q < 8; q++) {
-        if (k[q] < 0 || rk[q] > rt || (fl[q] & F_LOCKED)) continue;
-        if (dn[q] < sweep_start || dn[q] >= round) ok = false;
-    }
-    return ok;
 }
 
 // 256 lanes: the active tiles are a few percent of the grid, and a smaller
@@ -552,14 +648,18 @@ __global__ __launch_bounds__(SWEEP_THREADS) void valid_sweep_kernel(VParams p)
             int mine = 0;
             for (int a = threadIdx.x; a < cnt; a += SWEEP_THREADS) {
                 const int t = act[a];
-                if (p.done[base + t] >= sweep_start) continue;
-                if (!ready(p, base, t, sweep_start, round)) continue;
-                visit(p, base, t, corrected, err);
+                VisitIn d;
+                load_visit(p, base, t, d);
+                if (d.done >= sweep_start) continue;
+                if (!ready(d, sweep_start, round)) continue;
+                visit(p, base, t, d, corrected, err);
                 p.done[base + t] = round;
                 mine++;
             }
             if (mine) atomicAdd(&s_prog, mine);
-            __threadfence();
+            // the work-group barrier orders this round's stores before the next
+            // round's loads (every tile of the pair is handled by this one
+            // work-group; the device-coherent loads read past the L1)
             __syncthreads();
             processed += s_prog;
             round++;
@@ -599,6 +699,9 @@ __global__ __launch_bounds__(SWEEP_THREADS) void valid_sweep_kernel(VParams p)
     if (inv) atomicAdd(&s_inv, inv);
     if (err) atomicOr(&s_err, 2);
     __syncthreads();
+    // the active count returns to 0 for the next call (every thread read it at
+    // the start, before the barriers above)
+    if (threadIdx.x == 0) p.act_count[pair] = 0;
     if (p.stats && threadIdx.x == 0) {
         int *st = p.stats + (size_t)pair * 4;
         st[0] = sweeps;
@@ -737,8 +840,9 @@ int launch_validate(Context *ctx, const qlp_valid_config &cfg, const ValidLaunch
     p.act_count = vl.act_count;
     p.invalid = vl.invalid;
     p.stats = vl.stats;
-    hipLaunchKernelGGL(zero_int_kernel, dim3((vl.pairs + 255) / 256), dim3(256), 0, ctx->stream,
-                       vl.act_count, vl.pairs);
+    if (!vl.counts_zero)
+        hipLaunchKernelGGL(zero_int_kernel, dim3((vl.pairs + 255) / 256), dim3(256), 0, ctx->stream,
+                           vl.act_count, vl.pairs);
     const size_t total = (size_t)vl.pairs * p.n;
     hipLaunchKernelGGL(valid_init_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
                        ctx->stream, p, vl.pairs);
