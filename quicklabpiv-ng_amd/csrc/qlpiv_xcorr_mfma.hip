@@ -234,9 +234,12 @@ __global__ __launch_bounds__(64 * MF_WAVES, QLPIV_MF_MINW) void xcorr_mfma_kerne
     constexpr int D = X::D, MT = X::MT, NS = X::NS;
     const PassParams &p = mp.p;
     // the constant fragments, then one entry of row sums of Hf per lane
-    __shared__ u4 s_frag[(X::NF * 2 + 1) * 64];
+    // (then the Hout^T fragments again, scaled by 2^(E1 + E2): the inverse's
+    // second product then returns the plane at its true scale)
+    __shared__ u4 s_frag[(X::NF * 2 + 1 + X::NF_HO * 2) * 64];
 
-    for (int k = threadIdx.x; k < (X::NF * 2 + 1) * 64; k += 64 * MF_WAVES) s_frag[k] = mp.frags[k];
+    for (int k = threadIdx.x; k < (X::NF * 2 + 1 + X::NF_HO * 2) * 64; k += 64 * MF_WAVES)
+        s_frag[k] = mp.frags[k];
     __syncthreads();
 
     // wave-uniform (scalar) wave index: the IA index and its global base
@@ -258,7 +261,10 @@ __global__ __launch_bounds__(64 * MF_WAVES, QLPIV_MF_MINW) void xcorr_mfma_kerne
         asm volatile("" : "+v"(lane));
         __builtin_assume(lane >= 0 && lane < 64);
         const int j = lane % MT, g = lane / MT;
-        auto frag = [&](int f, int lo) -> u4 { return s_frag[(f * 2 + lo) * 64 + lane]; };
+        // fragment f (hi / lo); f >= NF: the scaled Hout^T copy of f - NF, after the h row
+        auto frag = [&](int f, int lo) -> u4 {
+            return s_frag[(f * 2 + lo + (f >= X::NF ? 1 - X::NF_HF * 2 : 0)) * 64 + lane];
+        };
         const int ia = (it * (int)gridDim.x + (int)blockIdx.x) * MF_WAVES + wave;
         const bool valid = ia < n;
         const int iac = valid ? ia : 0;  // the IA whose buffers an idle wave reads
@@ -376,8 +382,8 @@ __global__ __launch_bounds__(64 * MF_WAVES, QLPIV_MF_MINW) void xcorr_mfma_kerne
         // K = tile row m).  The pixels are small integers, exact in fp16 and
         // bf16, so they need no lo part; the normalisation is affine and is
         // applied to P:  x'^T Hf^T = (16 / max) (x^T Hf^T + (max / 16) 1 h^T)
-        // with h[rho] = sum_m Hf[rho][m], the second term the accumulator's
-        // initial value.  Q = Hf P then takes P from the accumulator as its B
+        // with h[rho] = sum_m Hf[rho][m], the second term added when P is
+        // scaled.  Q = Hf P then takes P from the accumulator as its B
         // operand (K = n): Q[sigma][rho] = E[rho][sigma], lane = rho.
         // E leaves scaled by 2^-ES (folded into the scaling of P), so that the
         // spectrum's products carry the inverse's 4 / M^2 and the range
@@ -394,10 +400,9 @@ __global__ __launch_bounds__(64 * MF_WAVES, QLPIV_MF_MINW) void xcorr_mfma_kerne
             const float sx = 16.0f / mx * (1.0f / (float)(1 << X::ES)), inv = mx * 0.0625f;
 #pragma unroll
             for (int ct = 0; ct < 2; ct++) {
-                const float h0 = (ct == 0 ? hrow.x : hrow.y) * inv;
-                Acc<MT> acc;
-#pragma unroll
-                for (int v = 0; v < NS; v++) acc[v] = h0;
+                // (x^T Hf^T + (max / 16) h) (16 / max) as one fma per value
+                const float h0 = (ct == 0 ? hrow.x : hrow.y) * inv * sx;
+                Acc<MT> acc = {};
 #pragma unroll
                 for (int kb = 0; kb < X::KN; kb++) {
                     if constexpr (SPLIT) acc = mma<BF, MT>(xt[im][kb], frag(ct * X::KN + kb, 1), acc);
@@ -409,7 +414,7 @@ __global__ __launch_bounds__(64 * MF_WAVES, QLPIV_MF_MINW) void xcorr_mfma_kerne
 #pragma unroll
                 for (int v = 0; v < NS; v += 2) {
                     unsigned h, l;
-                    split2<BF>(acc[v] * sx, acc[v + 1] * sx, h, l);
+                    split2<BF>(__builtin_fmaf(acc[v], sx, h0), __builtin_fmaf(acc[v + 1], sx, h0), h, l);
                     ph[acc_kb<MT>(0, v)][acc_s<MT>(0, v) >> 1] = h;
                     pl[acc_kb<MT>(0, v)][acc_s<MT>(0, v) >> 1] = l;
                 }
@@ -509,14 +514,14 @@ __global__ __launch_bounds__(64 * MF_WAVES, QLPIV_MF_MINW) void xcorr_mfma_kerne
                 Acc<MT> acc = {};
 #pragma unroll
                 for (int kb = 0; kb < X::KM; kb++) {
-                    const int f = X::NF_HF + rt * X::KM + kb;
+                    const int f = X::NF + X::NF_HF + rt * X::KM + kb;  // the scaled copy
                     acc = mma3<BF, SPLIT, MT>(frag(f, 0), frag(f, 1), yh[qt][kb], yl[qt][kb], acc);
                 }
                 R[rt][qt] = acc;
             }
 
         // ---- epilogues: plane store, FromCenter -----------------------------------
-        constexpr float unscale = (float)(1 << (X::E1 + X::E2));
+        constexpr float unscale = 1.0f;  // carried by the scaled Hout^T copy
         if (valid && p.planes) {
             // row a = D (the last row of row tile 1) is only reached by the last
             // slot, column b = D only in column tile 1
@@ -712,6 +717,10 @@ __global__ __launch_bounds__(m64::NT, 2) void xcorr_mfma64_kernel(PassParams p,
         __builtin_assume(lane >= 0 && lane < 64);
         const int j = lane % MT, g = lane / MT;
         auto frag = [&](int f, int lo) -> u4 { return frags[(f * 2 + lo) * 64 + lane]; };
+        // Hout^T fragment f scaled by 2^(E1 + E2) (the table's copy after the h row)
+        auto fscaled = [&](int f, int lo) -> u4 {
+            return frags[(X::NF * 2 + 1 + (f - NF_HF) * 2 + lo) * 64 + lane];
+        };
         auto xs = [&](int e) -> u4 & { return s_x[e * 64 + lane]; };
 
         // ---- pixel sources (as in the other forms) ------------------------------
@@ -857,16 +866,12 @@ __global__ __launch_bounds__(m64::NT, 2) void xcorr_mfma64_kernel(PassParams p,
 #pragma unroll
             for (int ci = 0; ci < 2; ci++) {
                 const int ct = sc + 2 * ci;
-                const float h0 = hv[ct] * inv;
+                const float h0 = hv[ct] * inv * sx;
                 u4 ph[KN], pl[KN];  // P's rho tile ct as B fragments (K = n)
                 {
                     // K block outermost (both n tiles per constant fragment);
                     // block kb + 1's fragment is loaded while block kb's run
-                    Acc<MT> acc[2];
-#pragma unroll
-                    for (int nt = 0; nt < 2; nt++)
-#pragma unroll
-                        for (int v = 0; v < NS; v++) acc[nt][v] = h0;
+                    Acc<MT> acc[2] = {};
                     u4 fh = frag(ct * KN, 0), fl = frag(ct * KN, 1);
 #pragma unroll
                     for (int kb = 0; kb < KN; kb++) {
@@ -889,7 +894,8 @@ __global__ __launch_bounds__(m64::NT, 2) void xcorr_mfma64_kernel(PassParams p,
 #pragma unroll
                         for (int v = 0; v < NS; v += 2) {
                             unsigned h, l;
-                            split2<BF, false>(acc[nt][v] * sx, acc[nt][v + 1] * sx, h, l);
+                            split2<BF, false>(__builtin_fmaf(acc[nt][v], sx, h0),
+                                              __builtin_fmaf(acc[nt][v + 1], sx, h0), h, l);
                             ph[acc_kb<MT>(nt, v)][acc_s<MT>(nt, v) >> 1] = h;
                             pl[acc_kb<MT>(nt, v)][acc_s<MT>(nt, v) >> 1] = l;
                         }
@@ -1054,7 +1060,7 @@ __global__ __launch_bounds__(m64::NT, 2) void xcorr_mfma64_kernel(PassParams p,
 #pragma unroll
             for (int pi = 0; pi < 2; pi++)
 #pragma unroll
-                for (int hl = 0; hl < 2; hl++) hp[pi][hl] = frag(NF_HF + (sc + 2 * pi) * KM, hl);
+                for (int hl = 0; hl < 2; hl++) hp[pi][hl] = fscaled(NF_HF + (sc + 2 * pi) * KM, hl);
 #pragma unroll
             for (int kb = 0; kb < KM; kb++) {
                 u4 hn[2][2];
@@ -1063,7 +1069,7 @@ __global__ __launch_bounds__(m64::NT, 2) void xcorr_mfma64_kernel(PassParams p,
                     for (int pi = 0; pi < 2; pi++)
 #pragma unroll
                         for (int hl = 0; hl < 2; hl++)
-                            hn[pi][hl] = frag(NF_HF + (sc + 2 * pi) * KM + kb + 1, hl);
+                            hn[pi][hl] = fscaled(NF_HF + (sc + 2 * pi) * KM + kb + 1, hl);
                 }
 #pragma unroll
                 for (int qi = 0; qi < 2; qi++) {
@@ -1084,7 +1090,7 @@ __global__ __launch_bounds__(m64::NT, 2) void xcorr_mfma64_kernel(PassParams p,
         __syncthreads();  // every wave has read Y: the buffer takes the plane
 
         // ---- epilogues: plane store, FromCenter -------------------------------------
-        constexpr float unscale = (float)(1 << (X::E1 + X::E2));
+        constexpr float unscale = 1.0f;  // carried by the scaled Hout^T copy
         if (p.planes) {
             float *__restrict__ out = p.planes + (size_t)ia * D * D;
 #pragma unroll
@@ -1280,13 +1286,17 @@ static void host_split(double x, bool bf, unsigned short &h, unsigned short &l)
 
 // Fragments of the constant A operands in kphys order: for fragment (rt, kb) of
 // an R x K matrix, lane (i + MT g) element s holds A[MT rt + i][kphys(kb, g, s)].
+// Table: the NF fragments (hi, lo), the row sums of Hf, then the Hout^T
+// fragments again scaled by 2^(E1 + E2) (the final product's copy).
 template <int N>
 static void build_frags(bool bf, std::vector<unsigned short> &out)
 {
     using X = Mf<N>;
     constexpr int M = X::M, MT = X::MT;
-    out.assign((size_t)X::NF * 2 * 64 * 8, 0);
-    auto put = [&](int f, auto A, int rows, int cols, int kblocks, int fbase) {
+    const size_t scaled_at = (size_t)X::NF * 2 + 1;  // entry of the scaled copy
+    out.assign((scaled_at + (size_t)X::NF_HO * 2) * 64 * 8, 0);
+    // entry(ff, hl): table entry of fragment ff's hi (0) or lo (1) part
+    auto put = [&](auto A, int rows, int cols, int kblocks, int fbase, auto entry) {
         for (int rt = 0; rt < X::RT; rt++)
             for (int kb = 0; kb < kblocks; kb++) {
                 const int ff = fbase + rt * kblocks + kb;
@@ -1296,25 +1306,27 @@ static void build_frags(bool bf, std::vector<unsigned short> &out)
                         const double x = (i < rows && k < cols) ? A(i, k) : 0.0;
                         unsigned short h, l;
                         host_split(x, bf, h, l);
-                        out[((size_t)(ff * 2 + 0) * 64 + lane) * 8 + s] = h;
-                        out[((size_t)(ff * 2 + 1) * 64 + lane) * 8 + s] = l;
+                        out[((size_t)entry(ff, 0) * 64 + lane) * 8 + s] = h;
+                        out[((size_t)entry(ff, 1) * 64 + lane) * 8 + s] = l;
                     }
             }
-        (void)f;
     };
+    auto plain = [](int ff, int hl) { return (size_t)ff * 2 + hl; };
     // Hf[rho][n] = h_rho(n)
-    put(0, [&](int r, int nn) { return mf_basis(M, r, nn); }, M, N, X::KN, 0);
+    put([&](int r, int nn) { return mf_basis(M, r, nn); }, M, N, X::KN, 0, plain);
     // Hout^T[p][sigma] = h_sigma(p - (N - 1)); the edge frequencies sigma = 0
     // and M/2 carry their inverse weight 1/2 here
-    put(1,
-        [&](int pp, int sg) {
+    auto hout = [&](double scale) {
+        return [&, scale](int pp, int sg) {
             const double w = (sg == 0 || sg == M / 2) ? 0.5 : 1.0;
-            return w * mf_basis(M, sg, (long)pp - (N - 1));
-        },
-        M, M, X::KM, X::NF_HF);
+            return scale * w * mf_basis(M, sg, (long)pp - (N - 1));
+        };
+    };
+    put(hout(1.0), M, M, X::KM, X::NF_HF, plain);
+    put(hout((double)(1 << (X::E1 + X::E2))), M, M, X::KM, X::NF_HF,
+        [&](int ff, int hl) { return scaled_at + (size_t)(ff - X::NF_HF) * 2 + hl; });
     // row sums h[rho] = sum_n Hf[rho][n], lane (i + MT g): h[MT c + i] for the
     // M / MT tiles c (2 or 4), as floats
-    out.resize(out.size() + 64 * 8, 0);
     for (int lane = 0; lane < 64; lane++)
         for (int c = 0; c < M / MT; c++) {
             const int r = MT * c + lane % MT;
