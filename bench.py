@@ -364,10 +364,16 @@ def serve_server(qlpiv, frames, schedule, form, saturate):
     their arrival times, or all at once when saturate."""
     srv = qlpiv.Server(0, hbm_budget=64 << 30, slots=2)
     common = serve_common(qlpiv, form)
-    # warm-up: every (size, levels) combination once (engine graphs, level grids)
+    # warm-up: every (size, levels) combination (engine graphs, level grids), then
+    # a burst of the stream's first requests (steps large enough for the engine
+    # to time its schedule)
     warm = [srv.submit(*frames[s], ia_start=a, ia_end=e, **common)
             for s in range(len(SERVE_SIZES))
             for (a, e) in ((64, 16), (32, 16), (16, 16), (64, 32), (32, 32), (64, 64))]
+    for t in warm:
+        srv.wait(t)
+        srv.result(t)
+    warm = [srv.submit(*frames[s], ia_start=a, ia_end=e, **common) for (_, s, a, e) in schedule[:200]]
     for t in warm:
         srv.wait(t)
         srv.result(t)
@@ -448,33 +454,51 @@ def serve_plain(qlpiv, frames, schedule, saturate):
     return latency_stats(arrive, done, ias, wall)
 
 
-def serve_report(n_req, rate):
+def serve_report(n_offered, n_saturated, rate, reps):
     """The serving configuration: a Poisson stream of frame-pair requests of mixed
     sizes (512^2, 1024^2, 1024x2048, 2048^2; final IA 16/32/64 with 1-3 levels;
     micro warp, FromCenter, Guo 3 px) through the request loop with continuous
     batching and a 64 GB HBM budget, in the MFMA fp16 hi+lo and the exact FP32
-    form, against the plain FP32 form (the drop-in plan, one request at a time),
-    at the offered rate and saturated (all requests queued at once).  Frames are
-    device-resident for every form."""
+    form, against the plain FP32 form (the drop-in plan, one request at a time).
+    'offered': the first n_offered requests of the stream at their arrival
+    times (latency under the 400 req/s load); 'saturated': n_saturated requests
+    queued at once (the throughput limit), `reps` runs per form on fresh servers,
+    the median run reported beside every run's rate.  Frames are device-resident
+    for every form."""
     import qlpiv
 
     frames = serve_frames(qlpiv)
-    schedule = serve_schedule(n_req, rate, seed=1600)
+    schedule = serve_schedule(max(n_offered, n_saturated), rate, seed=1600)
     out = {}
-    for name, form in (("serving_mfma_f16_split", qlpiv.QLP_XCORR_MFMA_F16),
-                       ("serving_exact_fp32", qlpiv.QLP_XCORR_EXACT)):
-        for mode in ("offered", "saturated"):
-            print(f"[serve] {name} {mode}: {n_req} requests", file=sys.stderr, flush=True)
-            out.setdefault(name, {})[mode] = serve_server(qlpiv, frames, schedule, form,
-                                                          mode == "saturated")
-    for mode in ("offered", "saturated"):
-        print(f"[serve] plain FP32 form {mode}: {n_req} requests", file=sys.stderr, flush=True)
-        out.setdefault("plain_fp32_one_at_a_time", {})[mode] = serve_plain(
-            qlpiv, frames, schedule, mode == "saturated")
+
+    def median_run(runs):
+        runs = sorted(runs, key=lambda r: r["ias_per_s"])
+        med = dict(runs[len(runs) // 2])
+        med["runs_ias_per_s"] = [r["ias_per_s"] for r in runs]
+        return med
+
+    forms = (("serving_mfma_f16_split", qlpiv.QLP_XCORR_MFMA_F16),
+             ("serving_exact_fp32", qlpiv.QLP_XCORR_EXACT))
+    for name, form in forms:
+        print(f"[serve] {name} offered: {n_offered} requests", file=sys.stderr, flush=True)
+        out.setdefault(name, {})["offered"] = serve_server(qlpiv, frames, schedule[:n_offered],
+                                                           form, False)
+        print(f"[serve] {name} saturated: {reps} x {n_saturated} requests", file=sys.stderr,
+              flush=True)
+        out[name]["saturated"] = median_run(
+            [serve_server(qlpiv, frames, schedule[:n_saturated], form, True) for _ in range(reps)])
+    print(f"[serve] plain FP32 form offered: {n_offered} requests", file=sys.stderr, flush=True)
+    out["plain_fp32_one_at_a_time"] = {"offered": serve_plain(qlpiv, frames, schedule[:n_offered],
+                                                              False)}
+    print(f"[serve] plain FP32 form saturated: {reps} x {n_saturated} requests", file=sys.stderr,
+          flush=True)
+    out["plain_fp32_one_at_a_time"]["saturated"] = median_run(
+        [serve_plain(qlpiv, frames, schedule[:n_saturated], True) for _ in range(reps)])
     plain = out["plain_fp32_one_at_a_time"]["saturated"]["ias_per_s"]
     return {"metric": "serving: throughput and latency of a Poisson stream of mixed frame-pair "
                       "requests",
-            "config": {"arrival_rate_req_per_s": rate, "requests": n_req,
+            "config": {"arrival_rate_req_per_s": rate, "requests_offered": n_offered,
+                       "requests_saturated": n_saturated, "saturated_runs": reps,
                        "sizes_HxW": SERVE_SIZES, "final_ia": [16, 32, 64], "levels": "1-3",
                        "hbm_budget_bytes": 64 << 30,
                        "per_request": "micro warp, FromCenter, Guo 3px/20it, direct inheritance; "
@@ -583,8 +607,10 @@ def main():
     ap.add_argument("--no-mfma", action="store_true", help="skip the matrix-core form")
     ap.add_argument("--no-legs", action="store_true",
                     help="skip the bounded batching and serving legs of the default line")
-    ap.add_argument("--leg-requests", type=int, default=800,
-                    help="requests per serving run in the default line's serving leg")
+    ap.add_argument("--leg-requests", type=int, nargs=2, default=(400, 2000),
+                    metavar=("OFFERED", "SATURATED"),
+                    help="requests of the offered-rate run and of each saturated run in the "
+                         "default line's serving leg")
     ap.add_argument("--precision", action="store_true",
                     help="print the precision report of the correlation forms instead")
     ap.add_argument("--precision-ias", type=int, nargs=2, default=(200000, 50000),
@@ -592,6 +618,8 @@ def main():
     ap.add_argument("--serve", action="store_true",
                     help="print the full serving report (Poisson stream through the request loop)")
     ap.add_argument("--serve-requests", type=int, default=2000)
+    ap.add_argument("--serve-runs", type=int, default=3,
+                    help="saturated runs per form (the median is reported)")
     ap.add_argument("--serve-rate", type=float, default=400.0)
     ap.add_argument("--batch", action="store_true",
                     help="print the batching report (64 mixed pairs in one graph launch)")
@@ -607,7 +635,8 @@ def main():
         precision_report(*args.precision_ias)
         return
     if args.serve:
-        print(json.dumps(serve_report(args.serve_requests, args.serve_rate)), flush=True)
+        print(json.dumps(serve_report(args.serve_requests, args.serve_requests, args.serve_rate,
+                                      args.serve_runs)), flush=True)
         return
     if args.batch:
         print(json.dumps(batch_report(args.steps, args.warmup, args.gen_workers)), flush=True)
@@ -692,7 +721,7 @@ def main():
         print("[bench] batching leg", file=sys.stderr, flush=True)
         legs["batch"] = batch_report(10, 2, min(args.gen_workers, cpu_threads()))
         print("[bench] serving leg", file=sys.stderr, flush=True)
-        legs["serving"] = serve_report(args.leg_requests, 400.0)
+        legs["serving"] = serve_report(args.leg_requests[0], args.leg_requests[1], 400.0, 3)
 
     line = {
         "metric": "correlated IAs/sec at 32x32; p50/p99 pair latency; MFMA vs FP32 form",

@@ -630,44 +630,70 @@ static int tune_classes(Engine *e, const int counts[ENG_CLASSES])
     return QLP_OK;
 }
 
-// The graph of the classes in `mask`: captured on the first step that holds
-// exactly these classes.  The first step with more than one class captures
-// both schedules, times them and keeps the faster; that choice then holds for
-// every later set (a new set costs one capture, no timing runs).
-static int graph_for(Engine *e, int mask, int &index)
+// The schedule is timed on the first step that holds more than one class and
+// at least kScheduleTuneIAs IAs; graphs captured before are serial.  On small
+// steps running the classes side by side looks faster (their short kernels
+// overlap), which larger steps do not repay: on a saturated stream of mixed
+// requests (steps of a few hundred thousand IAs) serial steps ran 1.3x the
+// concurrent ones, and a choice timed on a small first step was the wrong one.
+constexpr int kScheduleTuneIAs = 1 << 18;
+
+static void drop_graph(Engine::Graph &G)
 {
-    for (size_t k = 0; k < e->graphs.size(); k++)
-        if (e->graphs[k].mask == mask) {
-            index = (int)k;
-            return QLP_OK;
-        }
-    int classes = 0;
-    for (int c = 0; c < ENG_CLASSES; c++) classes += mask >> c & 1;
+    if (G.exec) (void)hipGraphExecDestroy(G.exec);
+    if (G.graph) (void)hipGraphDestroy(G.graph);
+    G.exec = nullptr;
+    G.graph = nullptr;
+}
+
+// The graph of the classes in `mask`: captured on the first step that holds
+// exactly these classes (serial until the schedule is decided, then in the
+// decided schedule).  With `decide` the step captures both schedules, times
+// them and keeps the faster for every later graph; graphs captured earlier
+// are serial and are kept if serial won, recaptured on their next step if not.
+static int graph_for(Engine *e, int mask, bool decide, int &index)
+{
+    if (!decide)
+        for (size_t k = 0; k < e->graphs.size(); k++)
+            if (e->graphs[k].mask == mask) {
+                index = (int)k;
+                return QLP_OK;
+            }
     const bool decided = e->schedule_fixed || e->schedule != QLP_SCHEDULE_TUNED;
-    const bool both = !decided && classes > 1;
-    const int only = decided ? e->schedule : 0;
+    const int only = decided ? e->schedule : QLP_SCHEDULE_SERIAL;
     hipGraph_t g[2] = {nullptr, nullptr};
     hipGraphExec_t x[2] = {nullptr, nullptr};
     float t[2] = {0.0f, 0.0f};
     int rc = QLP_OK;
     for (int conc = 0; conc < 2 && !rc; conc++) {
-        if (!both && conc != only) continue;
+        if (!decide && conc != only) continue;
         rc = capture(e, mask, conc == 1, g[conc], x[conc]);
-        if (!rc && both) rc = time_graph(e, x[conc], t[conc]);
+        if (!rc && decide) rc = time_graph(e, x[conc], t[conc]);
     }
-    const int pick = both ? (t[1] < t[0] ? 1 : 0) : only;
+    const int pick = decide ? (t[1] < t[0] ? QLP_SCHEDULE_CONCURRENT : QLP_SCHEDULE_SERIAL) : only;
     for (int k = 0; k < 2; k++) {
         if (!rc && k == pick) continue;
         if (x[k]) (void)hipGraphExecDestroy(x[k]);
         if (g[k]) (void)hipGraphDestroy(g[k]);
     }
     if (rc) return rc;
-    if (both) e->schedule = pick;
+    if (decide) {
+        e->schedule = pick;
+        // this mask's earlier graph, and every serial one if concurrent won
+        for (size_t k = 0; k < e->graphs.size();) {
+            if (e->graphs[k].mask == mask || e->graphs[k].schedule != pick) {
+                drop_graph(e->graphs[k]);
+                e->graphs.erase(e->graphs.begin() + (long)k);
+            } else {
+                k++;
+            }
+        }
+    }
     Engine::Graph G;
     G.mask = mask;
     G.schedule = pick;
-    G.serial_ms = both ? t[0] : 0.0f;
-    G.concurrent_ms = both ? t[1] : 0.0f;
+    G.serial_ms = decide ? t[0] : 0.0f;
+    G.concurrent_ms = decide ? t[1] : 0.0f;
     G.graph = g[pick];
     G.exec = x[pick];
     e->graphs.push_back(G);
@@ -729,13 +755,20 @@ int engine_step(Engine *e, const std::vector<StepSeg> segs[ENG_CLASSES])
     int gi = -1;
     for (size_t g = 0; g < e->graphs.size(); g++)
         if (e->graphs[g].mask == mask) gi = (int)g;
-    if (gi < 0 || fresh) {
+    int classes = 0, total = 0;
+    for (int c = 0; c < ENG_CLASSES; c++) {
+        classes += mask >> c & 1;
+        total += counts[c];
+    }
+    const bool decide = !e->schedule_fixed && e->schedule == QLP_SCHEDULE_TUNED && classes > 1 &&
+                        total >= kScheduleTuneIAs;
+    if (gi < 0 || fresh || decide) {
         // a new set of classes: tables, shapes and the graph, on this step's table
         rc = check_hip(hipStreamSynchronize(ctx->stream), "engine build");
         if (!rc && !e->prepared) rc = prepare_tables(e);
         e->prepared = e->prepared || !rc;
         if (!rc) rc = tune_classes(e, counts);
-        if (!rc) rc = graph_for(e, mask, gi);
+        if (!rc) rc = graph_for(e, mask, decide, gi);
         if (rc) return rc;
         e->built = true;
     }
