@@ -649,8 +649,9 @@ static void drop_graph(Engine::Graph &G)
 // The graph of the classes in `mask`: captured on the first step that holds
 // exactly these classes (serial until the schedule is decided, then in the
 // decided schedule).  With `decide` the step captures both schedules, times
-// them and keeps the faster for every later graph; graphs captured earlier
-// are serial and are kept if serial won, recaptured on their next step if not.
+// them and keeps the faster (concurrent only when 10 % faster) for every later
+// graph; graphs captured earlier are serial and are kept if serial won,
+// recaptured on their next step if not.
 static int graph_for(Engine *e, int mask, bool decide, int &index)
 {
     if (!decide)
@@ -670,7 +671,9 @@ static int graph_for(Engine *e, int mask, bool decide, int &index)
         rc = capture(e, mask, conc == 1, g[conc], x[conc]);
         if (!rc && decide) rc = time_graph(e, x[conc], t[conc]);
     }
-    const int pick = decide ? (t[1] < t[0] ? QLP_SCHEDULE_CONCURRENT : QLP_SCHEDULE_SERIAL) : only;
+    // concurrent only by a clear margin: one step's timing does not cover the
+    // mix of step sizes that follows
+    const int pick = decide ? (t[1] < 0.9f * t[0] ? QLP_SCHEDULE_CONCURRENT : QLP_SCHEDULE_SERIAL) : only;
     for (int k = 0; k < 2; k++) {
         if (!rc && k == pick) continue;
         if (x[k]) (void)hipGraphExecDestroy(x[k]);
