@@ -96,6 +96,11 @@ int ensure_bigwork(Context *ctx, size_t bytes)
     return grow_buffer(ctx, ctx->bigwork, ctx->bigwork_bytes, bytes, "hipMalloc IA-128 workspace");
 }
 
+int ensure_mpwork(Context *ctx, size_t bytes)
+{
+    return grow_buffer(ctx, ctx->mpwork, ctx->mpwork_bytes, bytes, "hipMalloc MultiPeaks workspace");
+}
+
 static int sync_ctx(Context *ctx) { return check_hip(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize"); }
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -182,6 +187,7 @@ try {
     if (ctx->vwork) (void)hipFree(ctx->vwork);
     if (ctx->iwork) (void)hipFree(ctx->iwork);
     if (ctx->bigwork) (void)hipFree(ctx->bigwork);
+    if (ctx->mpwork) (void)hipFree(ctx->mpwork);
     for (int a = 0; a < 3; a++)
         for (int b = 0; b < 2; b++)
             if (ctx->mfma_tab[a][b]) (void)hipFree(ctx->mfma_tab[a][b]);

@@ -60,6 +60,9 @@ struct Context {
     // per-work-group tile slots of the IA-128 correlation (grown on demand)
     void *bigwork = nullptr;
     size_t bigwork_bytes = 0;
+    // MultiPeaks entry workspace (grown on demand; plans size it when created)
+    void *mpwork = nullptr;
+    size_t mpwork_bytes = 0;
     // constant DFT fragments of the MFMA correlation form, [N 16/32/64][fp16/bf16]
     void *mfma_tab[3][2] = {};
     // kernel shapes measured by the batch autotuner, by class key
@@ -118,6 +121,11 @@ struct PassParams {
     int nbr_half = 0;
     int arith = QLP_XCORR_EXACT;  // QLP_XCORR_*: arithmetic form of the correlation
     int shape = 0;                // PassShape of an exact pass
+    // MultiPeaks entries (an exact pass from the LDS plane, no fused peak):
+    // per IA mp_geom(D).stride entries at mp_ent, chunk maxima at mp_cmax, the
+    // plane minimum at mp_min (pass_writes_entries tells which passes do)
+    uint2 *mp_ent = nullptr;
+    float *mp_cmax = nullptr, *mp_min = nullptr;
     // Persistent launch: the IA count lives in device memory (written before the
     // launch, e.g. by a step of a captured graph) and n is only the capacity of
     // the buffers.  The grid is then sized to the work-groups the device holds
@@ -150,6 +158,9 @@ static inline int pass_grid(const PassParams &p, int per_block, int resident)
 int launch_xcorr(Context *ctx, int n, int N, const uint8_t *tA, const uint8_t *tB,
                  const float *maxA, const float *maxB, int mode, float *planes);
 int launch_pass(Context *ctx, const PassParams &p);
+// Whether launch_pass(ctx, p) runs a kernel that writes p.mp_* (the exact pass
+// with its plane in LDS).
+bool pass_writes_entries(const Context *ctx, const PassParams &p);
 // The allocation-free correlation: the IA-128 slots in a static device array
 // whose per-device address init_noalloc_scratch (with the device current)
 // looks up once; launch_xcorr_noalloc allocates nothing.
@@ -166,6 +177,17 @@ size_t big_work_need(int n, int mode, bool from_image, bool peak, int shape);
 int prepare_big(Context *ctx, int n, int mode, bool from_image, bool peak, int shape);
 size_t big_work_estimate(int n);
 int ensure_bigwork(Context *ctx, size_t bytes);
+// MultiPeaks entries form (kernel width 3, at most 4 peaks): each plane's block
+// maxima in visit order in a workspace (entries, then chunk maxima), replayed
+// with one lane per plane.  multipeak_work_bytes: the workspace of n planes of
+// D x D (0 where the form does not apply); multipeak_work_carve: its two parts;
+// launch_multipeak_replay: the replay (peaks, count) from the entries.
+bool multipeak_entries_form(int D, int n_peaks, int kernel_width);
+size_t multipeak_work_bytes(int n, int D, int n_peaks, int kernel_width);
+void multipeak_work_carve(void *work, int n, int D, uint2 *&ent, float *&cmax);
+int launch_multipeak_replay(Context *ctx, int n, int D, int P, const uint2 *ent,
+                            const float *cmax, float *peaks, int *count);
+int ensure_mpwork(Context *ctx, size_t bytes);
 // The throughput form (qlpiv_xcorr_mfma.hip): form is QLP_XCORR_MFMA_*.
 int launch_pass_mfma(Context *ctx, const PassParams &p, int form, bool from_image, bool peak);
 int launch_find_peaks(Context *ctx, int kind, int n, int D, const float *planes, int n_peaks,

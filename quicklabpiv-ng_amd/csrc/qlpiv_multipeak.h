@@ -80,6 +80,178 @@ __device__ __forceinline__ void mp_insert(MpList &L, int P, int kd, float value,
     }
 }
 
+// addInAscendOrder :98-237 on a list in one lane's registers: PM >= P slots,
+// ascending, positions packed as i << 8 | j.  The decisions are those of
+// mp_insert; every update is a select gated by the accept decision, and the
+// deletions and the shift around the new entry move values between
+// neighbouring slots only, so that the slots stay in registers.
+template <int PM>
+__device__ __forceinline__ void mp_insert_reg(float (&lv)[PM], int (&lp)[PM], int &size, int P,
+                                              int kd, float value, int pij)
+{
+    const int pi = pij >> 8, pj = pij & 255;
+    int pos = 0, ndel = 0;
+    bool reject = false, del[PM];
+#pragma unroll
+    for (int e = 0; e < PM; e++) {
+        const bool have = e < size;
+        const int di = pi - (lp[e] >> 8), dj = pj - (lp[e] & 255);
+        const bool near = (float)(di * di + dj * dj) < (float)kd;
+        const bool lt = have && lv[e] < value;  // the entries below the insertion point
+        pos += lt ? 1 : 0;
+        del[e] = lt && near;  // value > lv[e]: deleted if the insertion happens
+        ndel += del[e] ? 1 : 0;
+        reject = reject || (have && !lt && near && value <= lv[e]);
+    }
+    const int size1 = size - ndel, ins = pos - ndel;
+    // a full list rejects a value not above its smallest; ins == 0 with a full
+    // list after the deletions only happens without deletions
+    const bool accept = !reject && !(size == P && pos == 0) && !(ins == 0 && size1 == P);
+    // deletions, highest slot first (the last slot needs no shift)
+#pragma unroll
+    for (int e = PM - 2; e >= 0; e--) {
+        const bool d = accept && del[e];
+#pragma unroll
+        for (int t = e; t < PM - 1; t++) {
+            lv[t] = d ? lv[t + 1] : lv[t];
+            lp[t] = d ? lp[t + 1] : lp[t];
+        }
+    }
+    // insertion at ins; with a full list remove() drops the smallest entry, so
+    // that the entries below the new one move down a slot instead
+    const bool drop = ins > 0 && size1 + 1 > P;
+    const int at = drop ? ins - 1 : ins;
+    float nv[PM];
+    int np[PM];
+#pragma unroll
+    for (int t = 0; t < PM; t++) {
+        const float vdn = t + 1 < PM ? lv[t + 1] : 0.0f, vup = t > 0 ? lv[t - 1] : 0.0f;
+        const int pdn = t + 1 < PM ? lp[t + 1] : 0, pup = t > 0 ? lp[t - 1] : 0;
+        nv[t] = t < at ? (drop ? vdn : lv[t]) : (t == at ? value : (drop ? lv[t] : vup));
+        np[t] = t < at ? (drop ? pdn : lp[t]) : (t == at ? pij : (drop ? lp[t] : pup));
+    }
+#pragma unroll
+    for (int t = 0; t < PM; t++) {
+        lv[t] = accept ? nv[t] : lv[t];
+        lp[t] = accept ? np[t] : lp[t];
+    }
+    size = accept ? size1 + 1 - (drop ? 1 : 0) : size;
+}
+
+// MultiPeaks entries for kernel width 3: the blocks of a plane in the
+// reference's visit order (slices of the top-left quadrant, then the four
+// quadrants), each a block maximum (strict >, starting at 0: 0 for a block
+// without a value above 0) and the position (i << 8 | j) of the first cell
+// holding it; padded with -2 to a multiple of 16 (a chunk), each chunk's
+// maximum beside them.  Geometry for D x D planes:
+struct MpGeom {
+    int iT;      // slices of the top (and left) half; the planes are square
+    int BW;      // blocks per block row (and block rows), both halves
+    int nent;    // entries per plane
+    int stride;  // entries per plane in the workspace, padded to a chunk
+    int nchunk;  // chunks of 16 entries
+};
+
+__host__ __device__ constexpr MpGeom mp_geom(int D)
+{
+    const int iS = (D + 2) / 3, iT = iS - iS / 2, BW = 2 * iT, nent = BW * BW;
+    const int stride = (nent + 15) & ~15;
+    return MpGeom{iT, BW, nent, stride, stride / 16};
+}
+
+// The entry of 3x3 block (br, bc) of a D x D plane from its nine cells c (row-
+// major), read with rows and columns past the edge clamped to the last row or
+// column: those cells repeat cells of the same block that come earlier in
+// row-major order, so they change neither the maximum, nor the first cell
+// holding it, nor the minimum.  *bmin is the block minimum (+inf for a block
+// wholly past the edge, which has no update).
+__device__ __forceinline__ uint2 mp_block_vals(int D, int br, int bc, const float *c, float &bmin)
+{
+    const int top = br * 3, left = bc * 3;
+    const bool out = top >= D || left >= D;
+    const float bmax = fmaxf(fmaxf(fmaxf(c[0], c[1]), fmaxf(c[2], c[3])),
+                             fmaxf(fmaxf(c[4], c[5]), fmaxf(fmaxf(c[6], c[7]), c[8])));
+    const float mn = fminf(fminf(fminf(c[0], c[1]), fminf(c[2], c[3])),
+                           fminf(fminf(c[4], c[5]), fminf(fminf(c[6], c[7]), c[8])));
+    bmin = out ? FLT_MAX : mn;
+    int cell = 8;  // the first cell (row-major) holding the maximum
+#pragma unroll
+    for (int q = 7; q >= 0; q--) cell = c[q] == bmax ? q : cell;
+    const int ci = cell >= 6 ? 2 : (cell >= 3 ? 1 : 0);
+    const bool upd = !out && bmax > 0.0f;
+    return make_uint2(__float_as_uint(upd ? bmax : 0.0f),
+                      (uint32_t)(((top + ci) << 8) | (left + cell - 3 * ci)));
+}
+
+// Offsets (row * rs + column) of the nine clamped cells of block (br, bc).
+__device__ __forceinline__ void mp_block_offsets(int D, int rs, int br, int bc, int (&o)[9])
+{
+    const int top = br * 3, left = bc * 3;
+    const int r0 = min(top, D - 1) * rs, r1 = min(top + 1, D - 1) * rs,
+              r2 = min(top + 2, D - 1) * rs;
+    const int c0 = min(left, D - 1), c1 = min(left + 1, D - 1), c2 = min(left + 2, D - 1);
+    o[0] = r0 + c0, o[1] = r0 + c1, o[2] = r0 + c2;
+    o[3] = r1 + c0, o[4] = r1 + c1, o[5] = r1 + c2;
+    o[6] = r2 + c0, o[7] = r2 + c1, o[8] = r2 + c2;
+}
+
+// Visit-order index of block (br, bc) (block rows and columns of both halves).
+__device__ __forceinline__ int mp_entry_index(const MpGeom &G, int br, int bc)
+{
+    const int qd = (br >= G.iT ? 2 : 0) | (bc >= G.iT ? 1 : 0);
+    const int si = br - ((qd & 2) ? G.iT : 0), sj = bc - ((qd & 1) ? G.iT : 0);
+    return ((si * G.iT + sj) << 2) | qd;
+}
+
+// The entries, chunk maxima and minimum of a D x D plane held in LDS (cell
+// (i, j) at pl[i * S + j]) by a work-group of NT threads (a multiple of 64).
+// Thread t takes entries t, t + NT, ... in visit order, so that the 16 entries
+// of a chunk sit in 16 consecutive lanes: the entries go to ent in coalesced
+// 8-byte stores and each chunk's maximum is folded across its lanes.  red
+// holds NT / 64 floats (the plane minimum across the wavefronts).
+template <int D, int S, int NT>
+__device__ __forceinline__ void mp_entries_lds(const float *pl, float *red, uint2 *__restrict__ ent,
+                                               float *__restrict__ cmax,
+                                               float *__restrict__ min_floor, int tid)
+{
+    constexpr MpGeom G = mp_geom(D);
+    static_assert(NT % 64 == 0, "whole wavefronts");
+    float mn = FLT_MAX;
+    for (int e = tid; e < G.stride; e += NT) {  // whole chunks of 16 lanes take part
+        uint2 en = make_uint2(__float_as_uint(-2.0f), 0u);  // padding
+        if (e < G.nent) {
+            const int qd = e & 3, s = e >> 2, si = s / G.iT, sj = s - si * G.iT;
+            const int br = si + ((qd & 2) ? G.iT : 0), bc = sj + ((qd & 1) ? G.iT : 0);
+            int o[9];
+            mp_block_offsets(D, S, br, bc, o);
+            float c[9];
+#pragma unroll
+            for (int k = 0; k < 9; k++) c[k] = pl[o[k]];
+            float bmin;
+            en = mp_block_vals(D, br, bc, c, bmin);
+            if (bmin < mn) mn = bmin;
+        }
+        ent[e] = en;
+        float cm = __uint_as_float(en.x);
+#pragma unroll
+        for (int o = 8; o >= 1; o >>= 1) cm = fmaxf(cm, __shfl_xor(cm, o, 64));
+        if ((e & 15) == 0) cmax[e >> 4] = cm;
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        const float om = __shfl_xor(mn, o, 64);
+        if (om < mn) mn = om;
+    }
+    if ((tid & 63) == 0) red[tid >> 6] = mn;
+    __syncthreads();
+    if (tid == 0) {
+        float m = red[0];
+        for (int w = 1; w < NT / 64; w++)
+            if (red[w] < m) m = red[w];
+        *min_floor = m;
+    }
+}
+
 // Maximum (strict >, from 0, row-major first) and minimum of one block; rows
 // and columns past the plane edge are not part of the block.  Cell (r, c) is
 // m[r * rs + c * cs].  KW > 0 fixes the kernel width at compile time so that

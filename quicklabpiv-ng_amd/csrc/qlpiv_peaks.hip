@@ -12,7 +12,10 @@
 //
 // SIMPLE (maximum/FindMaximumSimple.java:19-48): see peak_simple_kernel.
 //
-// MULTI (maximum/FindMaximumMultiPeaks.java:42-162): multipeak_segment
+// MULTI (maximum/FindMaximumMultiPeaks.java:42-162): the entries form (kernel
+// width 3, at most 4 peaks) replays each plane's block maxima with one lane per
+// plane, from entries the exact correlation pass writes or
+// peak_multi_blocks_kernel finds; otherwise multipeak_segment
 // (qlpiv_multipeak.h) with 8 to 64 lanes per plane.
 #include <float.h>
 #include <stdlib.h>
@@ -150,6 +153,278 @@ __global__ __launch_bounds__(64) void peak_multi_kernel(int n, int D, int P, int
                                  peaks + at * P * 3, min_floor + at, count ? count + at : nullptr);
 }
 
+// MultiPeaks for kernel width 3 and P <= 4 in two kernels over an entry
+// workspace (multipeak_work_bytes): the entries of a plane are its blocks in
+// the reference's visit order (slices of the top-left quadrant, then the four
+// quadrants), each a block maximum (strict >, starting at 0: 0 for a block
+// without a value above 0) and the position of the first cell holding it.
+// Kernel 1 (planes in memory): one wavefront per plane.  The lanes take the
+// plane's 3x3 blocks in memory order (block rows of both halves, then block
+// columns of both halves), one block per lane and step (mp_block_vals), with the
+// loads of the next batch of steps issued before the current batch is
+// evaluated.  The entries go to LDS at their visit-order index and then to the
+// workspace in order, with each chunk's maximum and the plane minimum
+// (*min_floor).  The exact correlation pass writes the same entries from its
+// LDS plane instead (mp_entries_lds), and then only kernel 2 runs.
+template <int DT>
+__global__ __launch_bounds__(64) void peak_multi_blocks_kernel(int Dr,
+                                                               const float *__restrict__ planes,
+                                                               uint2 *__restrict__ ent,
+                                                               float *__restrict__ cmax,
+                                                               float *__restrict__ min_floor)
+{
+    extern __shared__ uint2 s_ent[];
+    const int D = DT > 0 ? DT : Dr;
+    const MpGeom G = mp_geom(D);
+    const int lane = threadIdx.x, ia = blockIdx.x;
+    const float *__restrict__ m = planes + (size_t)ia * D * D;
+    constexpr int U = DT == 31 ? 3 : (DT > 0 ? 8 : 4);
+    const int spp = (G.nent + 63) >> 6;  // steps per plane
+    const int nb = (spp + U - 1) / U;    // batches
+    auto load_batch = [&](int b, float (&dst)[U * 9]) {
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int task = lane + 64 * (b * U + u);
+            const int br = task / G.BW, bc = task - br * G.BW;
+            int o[9];
+            mp_block_offsets(D, D, br, bc, o);
+#pragma unroll
+            for (int k = 0; k < 9; k++) dst[u * 9 + k] = m[o[k]];
+        }
+    };
+    float mn = FLT_MAX;
+    auto eval_batch = [&](int b, const float (&v)[U * 9]) {
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int st = b * U + u;
+            const int task = lane + 64 * st;
+            const int br = task / G.BW, bc = task - br * G.BW;
+            float bmin;
+            const uint2 en = mp_block_vals(D, br, bc, v + u * 9, bmin);
+            if (st < spp && br < G.BW) {
+                if (bmin < mn) mn = bmin;
+                s_ent[mp_entry_index(G, br, bc)] = en;
+            }
+        }
+    };
+    float bufA[U * 9], bufB[U * 9];
+    load_batch(0, bufA);
+    for (int b = 0; b < nb; b += 2) {
+        load_batch(min(b + 1, nb - 1), bufB);
+        eval_batch(b, bufA);
+        if (b + 1 < nb) {
+            load_batch(min(b + 2, nb - 1), bufA);
+            eval_batch(b + 1, bufB);
+        }
+    }
+    if (G.nent + lane < G.stride) s_ent[G.nent + lane] = make_uint2(__float_as_uint(-2.0f), 0u);
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        const float om = __shfl_xor(mn, o, 64);
+        if (om < mn) mn = om;
+    }
+    if (lane == 0) min_floor[ia] = mn;
+    mp_wave_sync();
+    uint4 *__restrict__ dst = (uint4 *)(ent + (size_t)ia * G.stride);
+    const uint4 *src = (const uint4 *)s_ent;
+    for (int i = lane; i < G.stride / 2; i += 64) dst[i] = src[i];
+    for (int c = lane; c < G.nchunk; c += 64) {
+        float cm = __uint_as_float(s_ent[16 * c].x);
+#pragma unroll
+        for (int k = 1; k < 16; k++) cm = fmaxf(cm, __uint_as_float(s_ent[16 * c + k].x));
+        cmax[(size_t)ia * G.nchunk + c] = cm;
+    }
+}
+
+// Kernel 2: one lane per plane (blockDim.x planes per work-group) replays the
+// plane's entries in visit order with its list in registers (mp_insert_reg).
+// An entry changes the list only if the list has room or the entry is above the
+// list's smallest value (addInAscendOrder :127-129), so a chunk whose maximum
+// is not above that bound is skipped without reading its entries (the chunk
+// maxima are staged in LDS first), and four entries are skipped together the
+// same way.  A block without an update is inserted as 0 at the position its
+// quadrant recorded last (FindMaximumMultiPeaks :62-64 keep maxsI/maxsJ across
+// blocks): that position is looked up only when such an entry is replayed,
+// walking back over the quadrant's entries not yet seen.
+template <int PM>
+__global__ __launch_bounds__(64) void peak_multi_replay_kernel(int n, int D, int P,
+                                                               const uint2 *__restrict__ ent,
+                                                               const float *__restrict__ cmax,
+                                                               float *__restrict__ peaks,
+                                                               int *__restrict__ count)
+{
+    constexpr int KD = 6;  // (kernelWidth - 1) * kernelWidth
+    extern __shared__ float s_cm[];
+    const MpGeom G = mp_geom(D);
+    const int g = threadIdx.x, GW = blockDim.x;
+    const int ia = blockIdx.x * GW + g;
+    const bool act = ia < n;
+    const uint2 *__restrict__ pe = ent + (size_t)(act ? ia : 0) * G.stride;
+    {  // chunk maxima of the work-group's planes (consecutive in memory) into LDS
+        const int ia0 = blockIdx.x * GW;
+        const int cnt = (min(GW, n - ia0)) * G.nchunk;
+        const float *src = cmax + (size_t)ia0 * G.nchunk;
+        for (int i = g; i < cnt; i += GW) s_cm[i] = src[i];
+        __syncthreads();
+    }
+    const float *cm = s_cm + (act ? g : 0) * G.nchunk;
+    float lv[PM];
+    int lp[PM];
+#pragma unroll
+    for (int t = 0; t < PM; t++) {
+        lv[t] = 0.0f;
+        lp[t] = 0;
+    }
+    int size = 0;
+    // an entry (>= 0) can change the list only if it is above lo: -1 while the
+    // list has room, else the list's smallest value
+    float lo = act ? -1.0f : FLT_MAX;
+    int stale[4] = {0, 0, 0, 0};
+    int seen[4] = {-4, -3, -2, -1};  // last entry of each quadrant taken into stale
+    for (int c = 0; c < G.nchunk; c++) {
+        if (!(cm[c] > lo)) continue;
+        uint4 q[8];
+        const uint4 *src = (const uint4 *)(pe + 16 * c);
+#pragma unroll
+        for (int k = 0; k < 8; k++) q[k] = src[k];
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            const float v4[4] = {__uint_as_float(q[2 * b].x), __uint_as_float(q[2 * b].z),
+                                 __uint_as_float(q[2 * b + 1].x), __uint_as_float(q[2 * b + 1].z)};
+            const int p4[4] = {(int)q[2 * b].y, (int)q[2 * b].w, (int)q[2 * b + 1].y,
+                               (int)q[2 * b + 1].w};
+            if (!(fmaxf(fmaxf(v4[0], v4[1]), fmaxf(v4[2], v4[3])) > lo)) continue;
+            const int e = 16 * c + 4 * b;
+#pragma unroll
+            for (int qd = 0; qd < 4; qd++) {
+                const float v = v4[qd];
+                if (v > lo) {
+                    int pij = p4[qd];
+                    if (!(v > 0.0f)) {  // no update: the quadrant's last recorded position
+                        for (int f = e + qd - 4; f > seen[qd]; f -= 4)
+                            if (__uint_as_float(pe[f].x) > 0.0f) {
+                                stale[qd] = (int)pe[f].y;
+                                break;
+                            }
+                        seen[qd] = e + qd;
+                        pij = stale[qd];
+                    }
+                    mp_insert_reg<PM>(lv, lp, size, P, KD, v, pij);
+                    lo = size < P ? -1.0f : lv[0];
+                }
+            }
+        }
+    }
+    if (act) {
+        float *__restrict__ pk = peaks + (size_t)ia * P * 3;
+        // descending: peak 0 is the largest
+#pragma unroll
+        for (int s = 0; s < PM; s++) {
+            if (s < size) {
+                const int idx = size - 1 - s;
+                pk[3 * idx + 0] = (float)(lp[s] >> 8);
+                pk[3 * idx + 1] = (float)(lp[s] & 255);
+                pk[3 * idx + 2] = lv[s];
+            }
+        }
+        for (int s = size; s < P; s++) pk[3 * s] = pk[3 * s + 1] = pk[3 * s + 2] = 0.0f;
+        if (count) count[ia] = size > 1 ? size : 1;
+    }
+}
+
+bool multipeak_entries_form(int D, int n_peaks, int kernel_width)
+{
+    // kernel 1 holds one plane's entries in LDS
+    return kernel_width == 3 && n_peaks >= 1 && n_peaks <= 4 && D >= 3 && D <= 255 &&
+           (size_t)mp_geom(D).stride * sizeof(uint2) <= 65536;
+}
+
+size_t multipeak_work_bytes(int n, int D, int n_peaks, int kernel_width)
+{
+    if (n <= 0 || !multipeak_entries_form(D, n_peaks, kernel_width)) return 0;
+    const MpGeom G = mp_geom(D);
+    return (size_t)n * (G.stride * sizeof(uint2) + G.nchunk * sizeof(float));
+}
+
+void multipeak_work_carve(void *work, int n, int D, uint2 *&ent, float *&cmax)
+{
+    const MpGeom G = mp_geom(D);
+    ent = (uint2 *)work;
+    cmax = (float *)((char *)work + (size_t)n * G.stride * sizeof(uint2));
+}
+
+// Planes per work-group of the replay kernel (one lane each).  Measured on
+// MI355X (C2 planes, P = 3; us for 4 / 8 / 16 / 32 / 64 planes): 65,025 x 31^2:
+// 116 / 82 / 58 / 49 / 47; 16,129 x 63^2: 57 / 52 / 49 / 51 / 52; 3,969 x
+// 127^2: 67 / 61 / 66 / 78 / 82.  QLPIV_MULTIPEAK_PLANES (1..64) overrides it.
+static int replay_planes(int n)
+{
+    int G = n >= 32768 ? 64 : (n >= 8192 ? 16 : 8);
+    if (const char *env = getenv("QLPIV_MULTIPEAK_PLANES")) {
+        const int g = atoi(env);
+        if (g >= 1 && g <= 64) G = g;
+    }
+    return G;
+}
+
+int launch_multipeak_replay(Context *ctx, int n, int D, int P, const uint2 *ent,
+                            const float *cmax, float *peaks, int *count)
+{
+    if (n == 0) return QLP_OK;
+    const MpGeom G = mp_geom(D);
+    const int GW = replay_planes(n);
+    const unsigned grid = (unsigned)((n + GW - 1) / GW);
+    const size_t lds = (size_t)GW * G.nchunk * sizeof(float);
+    switch (P) {
+    case 1:
+        hipLaunchKernelGGL(peak_multi_replay_kernel<1>, dim3(grid), dim3(GW), lds, ctx->stream, n,
+                           D, P, ent, cmax, peaks, count);
+        break;
+    case 2:
+        hipLaunchKernelGGL(peak_multi_replay_kernel<2>, dim3(grid), dim3(GW), lds, ctx->stream, n,
+                           D, P, ent, cmax, peaks, count);
+        break;
+    case 3:
+        hipLaunchKernelGGL(peak_multi_replay_kernel<3>, dim3(grid), dim3(GW), lds, ctx->stream, n,
+                           D, P, ent, cmax, peaks, count);
+        break;
+    default:
+        hipLaunchKernelGGL(peak_multi_replay_kernel<4>, dim3(grid), dim3(GW), lds, ctx->stream, n,
+                           D, P, ent, cmax, peaks, count);
+    }
+    return check_hip(hipGetLastError(), "peak kernel launch");
+}
+
+// Both kernels over the context's workspace (planes in memory).
+static int launch_entries_form(Context *ctx, int n, int D, int P, const float *planes,
+                               float *peaks, float *min_floor, int *count)
+{
+    int rc = ensure_mpwork(ctx, multipeak_work_bytes(n, D, P, 3));
+    if (rc) return rc;
+    uint2 *ent;
+    float *cm;
+    multipeak_work_carve(ctx->mpwork, n, D, ent, cm);
+    const size_t lds = (size_t)mp_geom(D).stride * sizeof(uint2);
+    switch (D) {
+    case 31:
+        hipLaunchKernelGGL(peak_multi_blocks_kernel<31>, dim3(n), dim3(64), lds, ctx->stream, D,
+                           planes, ent, cm, min_floor);
+        break;
+    case 63:
+        hipLaunchKernelGGL(peak_multi_blocks_kernel<63>, dim3(n), dim3(64), lds, ctx->stream, D,
+                           planes, ent, cm, min_floor);
+        break;
+    case 127:
+        hipLaunchKernelGGL(peak_multi_blocks_kernel<127>, dim3(n), dim3(64), lds, ctx->stream, D,
+                           planes, ent, cm, min_floor);
+        break;
+    default:
+        hipLaunchKernelGGL(peak_multi_blocks_kernel<0>, dim3(n), dim3(64), lds, ctx->stream, D,
+                           planes, ent, cm, min_floor);
+    }
+    return launch_multipeak_replay(ctx, n, D, P, ent, cm, peaks, count);
+}
+
 template <int KW>
 static void launch_multi(Context *ctx, int W, int n, int D, int P, int k, const float *planes,
                          float *peaks, float *min_floor, int *count)
@@ -208,6 +483,19 @@ int launch_find_peaks(Context *ctx, int kind, int n, int D, const float *planes,
         // W = 8/16/32/64): 65,025 x 31^2: 168/213/292/407; 16,129 x 63^2:
         // 188/148/156/185; 3,969 x 127^2: 388/235/158/115.
         // QLPIV_MULTIPEAK_LANES (8, 16, 32 or 64) overrides it for tuning.
+        // planes in memory: the entries form (block maxima kernel, then the
+        // replay) up to 63^2 planes, the segment form for larger ones.  Measured
+        // on MI355X (C2 step, MultiPeaks + Guo per level, ms, segment ->
+        // entries): exact 31^2 0.172 -> 0.152, MFMA 31^2 0.149 -> 0.150, MFMA
+        // 63^2 0.150 -> 0.145, MFMA 127^2 0.119 -> 0.163.  QLPIV_MULTIPEAK_FORM
+        // (segment or entries) overrides the choice.
+        bool entries = D <= 63;
+        if (const char *env = getenv("QLPIV_MULTIPEAK_FORM")) {
+            if (strcmp(env, "segment") == 0) entries = false;
+            if (strcmp(env, "entries") == 0) entries = true;
+        }
+        if (entries && multipeak_entries_form(D, n_peaks, kernel_width))
+            return launch_entries_form(ctx, n, D, n_peaks, planes, peaks, min_floor, count);
         int W = 8;
         while (W < 64 && (size_t)n * W < (size_t)2048 * 64) W *= 2;
         if (const char *env = getenv("QLPIV_MULTIPEAK_LANES")) {

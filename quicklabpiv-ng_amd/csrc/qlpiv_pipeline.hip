@@ -170,6 +170,7 @@ struct qlp_piv_plan {
     size_t ftiles_bytes = 0;
     // IAs of the largest IA-128 pass: sizes the context's tile-slot workspace
     int big_ias = 0;
+    size_t mp_work = 0;  // MultiPeaks entry workspace of the largest level (context-owned)
     // optional per-level timing: events before the state kernel, after the
     // correlation pass, after the fit and after the validation of every level
     bool timing = false;
@@ -347,6 +348,11 @@ static int plan_layout(int H, int W, int pairs, const qlp_piv_config *cfg, qlp_p
             return fail(QLP_ERR_UNSUPPORTED,
                         "piv_plan_create: spline warps need 2 tiles in each direction");
         if (tile == 128) p->big_ias = std::max(p->big_ias, (int)pn);
+        // MultiPeaks entries: the exact LDS passes write them, the finder forms
+        // them from planes of 63^2 or less
+        if (multi && (tile <= 32 || (tile == 64 && cfg->xcorr_form == QLP_XCORR_EXACT)))
+            p->mp_work = std::max(p->mp_work, multipeak_work_bytes((int)pn, (int)D, P,
+                                                                   cfg->peak_kernel_width));
         if (l > 0 && tiles_route(*cfg) && !f32)
             p->tiles_bytes = std::max(p->tiles_bytes, pn * (size_t)tile * tile);
         if (f32) p->ftiles_bytes = std::max(p->ftiles_bytes, pn * (size_t)tile * tile * sizeof(float));
@@ -377,6 +383,8 @@ try {
         for (const Level &L : p.levels) need += L.n * 9 * 8 + 64 * 1024;
     // the context's IA-128 workspace (shared by the context's plans)
     if (p.big_ias) need += big_work_estimate(p.big_ias);
+    // the context's MultiPeaks workspace (shared by the context's plans)
+    need += p.mp_work;
     *bytes = need;
     return QLP_OK;
 } catch (...) {
@@ -469,6 +477,8 @@ try {
     }
     // the plan's IA-128 passes read the images and find peaks
     if (!rc && p->big_ias) rc = prepare_big(ctx, p->big_ias, cfg->norm_mode, !f32, true, SHAPE_AUTO);
+    // the MultiPeaks workspace at its largest level's size, so that runs do not grow it
+    if (!rc && p->mp_work) rc = ensure_mpwork(ctx, p->mp_work);
     p->bytes = need;
     ctx->hbm_in_use += need;
     if (rc) {
@@ -778,13 +788,30 @@ static int run_levels(qlp_piv_plan *p, const uint8_t *imgsA, const uint8_t *imgs
             q.nbr = L.nbr;
             q.nbr_half = NBR_HALF;
         }
+        // MultiPeaks: the exact LDS pass writes each plane's block maxima (the
+        // entries) and only their replay runs after it
+        uint2 *ment = nullptr;
+        float *mcm = nullptr;
+        const bool entries = multi && multipeak_entries_form((int)D, p->P, c.peak_kernel_width) &&
+                             pass_writes_entries(ctx, q);
+        if (entries) {
+            rc = ensure_mpwork(ctx, multipeak_work_bytes((int)pn, (int)D, p->P, c.peak_kernel_width));
+            if (rc) return rc;
+            multipeak_work_carve(ctx->mpwork, (int)pn, (int)D, ment, mcm);
+            q.mp_ent = ment;
+            q.mp_cmax = mcm;
+            q.mp_min = L.mf;
+        }
         rc = launch_pass(ctx, q);
         if (rc) return rc;
         if (p->timing) (void)hipEventRecord(p->ev[4 * l + 1], ctx->stream);
         const int kind = ((int)l >= c.interpolator_start_step) ? c.subpix_kind : -1;
         if (separate) {
-            rc = launch_find_peaks(ctx, c.peak_kind, pn, D, L.planes, p->P, c.peak_kernel_width,
-                                   L.peaks, L.mf, multi ? L.count : nullptr);
+            if (entries)
+                rc = launch_multipeak_replay(ctx, (int)pn, (int)D, p->P, ment, mcm, L.peaks, L.count);
+            else
+                rc = launch_find_peaks(ctx, c.peak_kind, pn, D, L.planes, p->P, c.peak_kernel_width,
+                                       L.peaks, L.mf, multi ? L.count : nullptr);
             if (!rc)
                 rc = launch_subpixel(ctx, kind, c.subpix_pixels, c.subpix_iterations, pn, D, p->P,
                                      L.planes, L.mf, L.peaks, L.sub);

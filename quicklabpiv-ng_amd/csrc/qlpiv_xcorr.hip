@@ -30,6 +30,7 @@
 #include "qlpiv_device.h"
 #include "qlpiv_epilogue.h"
 #include "qlpiv_internal.h"
+#include "qlpiv_multipeak.h"
 
 namespace qlpiv {
 
@@ -440,6 +441,11 @@ __global__ __launch_bounds__((XcShape<LOGM, NT_>::THREADS)) void xcorr_pass_kern
                 const int i = idx / D, j = idx - (idx / D) * D;
                 out[idx] = s_re[i * S + j];
             }
+        }
+        if (!PEAK && !PERSIST && p.mp_ent) {  // MultiPeaks entries from the plane in LDS
+            constexpr MpGeom MG = mp_geom(D);
+            mp_entries_lds<D, S, NT>(s_re, s_min, p.mp_ent + (size_t)ia * MG.stride,
+                                     p.mp_cmax + (size_t)ia * MG.nchunk, p.mp_min + ia, tid);
         }
         if (PEAK) {
             // FromCenter in two phases (from_center_tie): the largest value of
@@ -897,6 +903,20 @@ static void launch_pass_m(Context *ctx, const PassParams &p, bool from_image, bo
         launch_pass_mc<LOGM, false>(ctx, p, from_image, peak);
 }
 
+static bool lane_form(const Context *ctx, const PassParams &p)
+{
+    const bool ftiles = p.tAf != nullptr || p.tBf != nullptr;
+    return !ftiles && p.N == 16 &&
+           (p.shape == SHAPE_LANE || (p.shape == SHAPE_AUTO && ctx->xcorr_form == FORM_LANE));
+}
+
+bool pass_writes_entries(const Context *ctx, const PassParams &p)
+{
+    if (p.n_dev || p.peaks || p.min_floor || p.nbr) return false;  // persistent or fused peak
+    if (p.arith != QLP_XCORR_EXACT || lane_form(ctx, p)) return false;
+    return p.N == 16 || p.N == 32 || p.N == 64;
+}
+
 int launch_pass(Context *ctx, const PassParams &p)
 {
     if (p.n == 0) return QLP_OK;
@@ -919,11 +939,11 @@ int launch_pass(Context *ctx, const PassParams &p)
     const bool ftiles = p.tAf != nullptr || p.tBf != nullptr;
     if (ftiles && (from_image || p.arith != QLP_XCORR_EXACT))
         return fail(QLP_ERR_UNSUPPORTED, "xcorr: float tiles take the exact form from tiles");
+    if (p.mp_ent && !pass_writes_entries(ctx, p))
+        return fail(QLP_ERR_ARG, "pass: MultiPeaks entries come from the exact LDS pass only");
     if (p.arith != QLP_XCORR_EXACT && p.N <= 64)
         return launch_pass_mfma(ctx, p, p.arith, from_image, peak);
-    if (!ftiles && p.N == 16 &&
-        (p.shape == SHAPE_LANE || (p.shape == SHAPE_AUTO && ctx->xcorr_form == FORM_LANE)))
-        return launch_pass_lane(ctx, p, from_image, peak);
+    if (lane_form(ctx, p)) return launch_pass_lane(ctx, p, from_image, peak);
     switch (p.N) {
     case 16: launch_pass_m<5>(ctx, p, from_image, peak); break;
     case 32: launch_pass_m<6>(ctx, p, from_image, peak); break;

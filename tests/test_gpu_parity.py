@@ -176,9 +176,11 @@ def test_multipeaks_every_segment_width_and_list_form(ctx, W, lst):
     equal to and above the width (the LDS list then), kernel widths 1, 2, 3."""
     import qlpiv
 
-    saved = {k: os.environ.get(k) for k in ("QLPIV_MULTIPEAK_LANES", "QLPIV_MULTIPEAK_LIST")}
+    saved = {k: os.environ.get(k) for k in ("QLPIV_MULTIPEAK_LANES", "QLPIV_MULTIPEAK_LIST",
+                                            "QLPIV_MULTIPEAK_FORM")}
     os.environ["QLPIV_MULTIPEAK_LANES"] = str(W)
     os.environ["QLPIV_MULTIPEAK_LIST"] = lst
+    os.environ["QLPIV_MULTIPEAK_FORM"] = "segment"
     try:
         for D, P, kw in [(31, 3, 3), (31, 8, 2), (63, 16, 3), (63, 1, 3), (15, 5, 1)]:
             rng = np.random.default_rng(W + D + P)
@@ -196,6 +198,79 @@ def test_multipeaks_every_segment_width_and_list_form(ctx, W, lst):
                 os.environ.pop(k, None)
             else:
                 os.environ[k] = v
+
+
+@pytest.mark.parametrize("G", [1, 4, 16, 64])
+def test_multipeaks_replay_every_plane_count(ctx, G):
+    """The two-kernel form (kernel width 3, P <= 4: block maxima by one
+    wavefront per plane, then one lane per plane replaying the list) at several
+    planes-per-work-group counts of the replay, against the oracle: the C2
+    plane sizes and odd ones, P = 1..4, batches that leave lanes of the last
+    work-group idle, planes with ties, zeros, negatives (blocks without an
+    update) and equal maxima."""
+    import qlpiv
+
+    saved = {k: os.environ.get(k) for k in ("QLPIV_MULTIPEAK_PLANES", "QLPIV_MULTIPEAK_FORM")}
+    os.environ["QLPIV_MULTIPEAK_PLANES"] = str(G)
+    os.environ["QLPIV_MULTIPEAK_FORM"] = "entries"
+    try:
+        for D, P, n in [(31, 3, 48), (63, 3, 37), (127, 3, 21), (63, 4, 40), (31, 1, 33),
+                        (127, 2, 9), (15, 4, 50), (10, 3, 19), (5, 2, 12)]:
+            rng = np.random.default_rng(G * 1000 + D * 10 + P)
+            planes = planes_for_peaks(rng, max(n, 8), D) if D >= 10 else \
+                rng.random((n, D, D)).astype(np.float32) * 10
+            planes = planes[:n]
+            if D >= 10:
+                planes[7] = np.round(planes[7] / 10)  # many equal block maxima
+            say(f"multipeaks replay G={G} D={D} P={P} n={n}")
+            peaks, mf = ctx.find_peaks(planes, qlpiv.QLP_PEAK_MULTI, P, 3)
+            for k in range(n):
+                ref, found, rmf = O.find_max_multipeaks(planes[k], P, 3)
+                assert np.array_equal(peaks[k], ref), (D, P, k, peaks[k], ref)
+                assert mf[k] == rmf, (D, P, k)
+    finally:
+        for k, v in saved.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+
+
+def test_multipeaks_replay_on_correlation_planes(ctx):
+    """The entries form (block maxima kernel, then the replay) against the
+    segment form and the oracle on correlation planes of particle images at the
+    three C2 sizes (the planes the step's search sees: one sharp peak over a
+    particle-pair background)."""
+    import qlpiv
+    from qlpiv.geometry import level_grid
+    from qlpiv.synth import particle_pair
+
+    for N in (16, 32, 64):
+        a, b, _ = particle_pair(384, 384, seed=N, shift=(2.3, -1.6))
+        tops, lefts = level_grid(384, 384, N, 0.5)
+        top, left = tops.ravel().astype(np.int32), lefts.ravel().astype(np.int32)
+        n = top.size
+        u = np.full(n, 2.0, np.float32)
+        v = np.full(n, -2.0, np.float32)
+        _, _, pl = O.level_batch(a, b, top, left, u, v, N, 1, mode=1, peak_kind=1, P=3, k=3,
+                                 planes=True)
+        say(f"multipeaks replay on correlation planes N={N} n={n}")
+        saved = os.environ.get("QLPIV_MULTIPEAK_FORM")
+        try:
+            os.environ["QLPIV_MULTIPEAK_FORM"] = "segment"
+            speaks, smf = ctx.find_peaks(pl, qlpiv.QLP_PEAK_MULTI, 3, 3)
+            os.environ["QLPIV_MULTIPEAK_FORM"] = "entries"
+            peaks, mf = ctx.find_peaks(pl, qlpiv.QLP_PEAK_MULTI, 3, 3)
+        finally:
+            if saved is None:
+                os.environ.pop("QLPIV_MULTIPEAK_FORM", None)
+            else:
+                os.environ["QLPIV_MULTIPEAK_FORM"] = saved
+        assert np.array_equal(peaks, speaks) and np.array_equal(mf, smf)
+        for k in range(0, n, 7):
+            ref, found, rmf = O.find_max_multipeaks(pl[k], 3, 3)
+            assert np.array_equal(peaks[k], ref), (N, k)
+            assert mf[k] == rmf, (N, k)
 
 
 def test_multipeaks_reference_kat(ctx):
