@@ -203,30 +203,29 @@ __device__ __forceinline__ int mp_entry_index(const MpGeom &G, int br, int bc)
     return ((si * G.iT + sj) << 2) | qd;
 }
 
-// The entries, chunk maxima and minimum of a D x D plane held in LDS (cell
-// (i, j) at pl[i * S + j]) by a work-group of NT threads (a multiple of 64).
-// Thread t takes entries t, t + NT, ... in visit order, so that the 16 entries
-// of a chunk sit in 16 consecutive lanes: the entries go to ent in coalesced
-// 8-byte stores and each chunk's maximum is folded across its lanes.  red
-// holds NT / 64 floats (the plane minimum across the wavefronts).
-template <int D, int S, int NT>
-__device__ __forceinline__ void mp_entries_lds(const float *pl, float *red, uint2 *__restrict__ ent,
-                                               float *__restrict__ cmax,
-                                               float *__restrict__ min_floor, int tid)
+// The entries and chunk maxima of a D x D plane held in LDS, cell (i, j) read by
+// rd(i * RS + j), by NTH threads (a multiple of 16; t = 0 .. NTH-1): thread t
+// takes entries t, t + NTH, ... in visit order, so that the 16 entries of a
+// chunk sit in 16 consecutive lanes.  The entries go to ent in coalesced
+// 8-byte stores and each chunk's maximum is folded across its lanes.  Returns
+// the minimum of the thread's blocks.
+template <int D, int RS, int NTH, typename F>
+__device__ __forceinline__ float mp_entries_core(F rd, int t, uint2 *__restrict__ ent,
+                                                 float *__restrict__ cmax)
 {
     constexpr MpGeom G = mp_geom(D);
-    static_assert(NT % 64 == 0, "whole wavefronts");
+    static_assert(NTH % 16 == 0, "whole chunks of 16 lanes");
     float mn = FLT_MAX;
-    for (int e = tid; e < G.stride; e += NT) {  // whole chunks of 16 lanes take part
+    for (int e = t; e < G.stride; e += NTH) {  // whole chunks of 16 lanes take part
         uint2 en = make_uint2(__float_as_uint(-2.0f), 0u);  // padding
         if (e < G.nent) {
             const int qd = e & 3, s = e >> 2, si = s / G.iT, sj = s - si * G.iT;
             const int br = si + ((qd & 2) ? G.iT : 0), bc = sj + ((qd & 1) ? G.iT : 0);
             int o[9];
-            mp_block_offsets(D, S, br, bc, o);
+            mp_block_offsets(D, RS, br, bc, o);
             float c[9];
 #pragma unroll
-            for (int k = 0; k < 9; k++) c[k] = pl[o[k]];
+            for (int k = 0; k < 9; k++) c[k] = rd(o[k]);
             float bmin;
             en = mp_block_vals(D, br, bc, c, bmin);
             if (bmin < mn) mn = bmin;
@@ -237,6 +236,19 @@ __device__ __forceinline__ void mp_entries_lds(const float *pl, float *red, uint
         for (int o = 8; o >= 1; o >>= 1) cm = fmaxf(cm, __shfl_xor(cm, o, 64));
         if ((e & 15) == 0) cmax[e >> 4] = cm;
     }
+    return mn;
+}
+
+// mp_entries_core by a whole work-group of NT threads (a multiple of 64) over
+// a plane at pl[i * S + j]; the minimum is folded across the wavefronts through
+// red (NT / 64 floats) and written to *min_floor.
+template <int D, int S, int NT>
+__device__ __forceinline__ void mp_entries_lds(const float *pl, float *red, uint2 *__restrict__ ent,
+                                               float *__restrict__ cmax,
+                                               float *__restrict__ min_floor, int tid)
+{
+    static_assert(NT % 64 == 0, "whole wavefronts");
+    float mn = mp_entries_core<D, S, NT>([&](int o) { return pl[o]; }, tid, ent, cmax);
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) {
         const float om = __shfl_xor(mn, o, 64);

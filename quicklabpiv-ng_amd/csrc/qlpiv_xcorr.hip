@@ -912,8 +912,9 @@ static bool lane_form(const Context *ctx, const PassParams &p)
 
 bool pass_writes_entries(const Context *ctx, const PassParams &p)
 {
+    (void)ctx;  // both exact forms of 16 px (lane and LDS) write them
     if (p.n_dev || p.peaks || p.min_floor || p.nbr) return false;  // persistent or fused peak
-    if (p.arith != QLP_XCORR_EXACT || lane_form(ctx, p)) return false;
+    if (p.arith != QLP_XCORR_EXACT) return false;
     return p.N == 16 || p.N == 32 || p.N == 64;
 }
 
@@ -940,7 +941,7 @@ int launch_pass(Context *ctx, const PassParams &p)
     if (ftiles && (from_image || p.arith != QLP_XCORR_EXACT))
         return fail(QLP_ERR_UNSUPPORTED, "xcorr: float tiles take the exact form from tiles");
     if (p.mp_ent && !pass_writes_entries(ctx, p))
-        return fail(QLP_ERR_ARG, "pass: MultiPeaks entries come from the exact LDS pass only");
+        return fail(QLP_ERR_ARG, "pass: MultiPeaks entries come from the exact passes only");
     if (p.arith != QLP_XCORR_EXACT && p.N <= 64)
         return launch_pass_mfma(ctx, p, p.arith, from_image, peak);
     if (lane_form(ctx, p)) return launch_pass_lane(ctx, p, from_image, peak);

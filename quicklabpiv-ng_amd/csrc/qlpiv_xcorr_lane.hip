@@ -23,6 +23,7 @@
 #include <type_traits>
 #include "qlpiv_device.h"
 #include "qlpiv_internal.h"
+#include "qlpiv_multipeak.h"
 
 namespace qlpiv {
 
@@ -378,6 +379,31 @@ __global__ __launch_bounds__(Lx<LOGM>::THREADS) void xcorr_lane_kernel(
                 for (int e = 0; e < E; e++) {
                     const int row = half * E + e;
                     if (row < D) out[row * D + c] = re[e];
+                }
+            }
+        }
+        if constexpr (!PEAK && !PERSIST && X::IAS * X::TPI == 64 && X::LPL == 1) {
+            if (p.mp_ent) {
+                // MultiPeaks entries: the IA's columns into its LDS tile (its last
+                // reads are done: one wavefront holds the work-group), then the
+                // blocks from there by the IA's lanes
+                if (c < D) {
+                    #pragma unroll
+                    for (int e = 0; e < E; e++)
+                        if (e < D) sx[e * S + c].x = re[e];
+                }
+                mp_wave_sync();
+                if (valid) {
+                    constexpr MpGeom MG = mp_geom(D);
+                    float mn = mp_entries_core<D, S, TPI>(
+                        [&](int o) { return sx[o].x; }, t, p.mp_ent + (size_t)ia * MG.stride,
+                        p.mp_cmax + (size_t)ia * MG.nchunk);
+                    #pragma unroll
+                    for (int o = TPI / 2; o >= 1; o >>= 1) {
+                        const float om = __shfl_xor(mn, o, 64);
+                        if (om < mn) mn = om;
+                    }
+                    if (t == 0) p.mp_min[ia] = mn;
                 }
             }
         }
