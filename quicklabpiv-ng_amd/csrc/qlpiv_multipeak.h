@@ -342,7 +342,7 @@ __device__ __forceinline__ void mp_wave_sync()
 //  * the insertions are replayed in visit order, but only for entries that can
 //    change the list: once the list holds P entries, a value not above its
 //    smallest one is rejected without any other effect (addInAscendOrder
-//    :127-129), so a ballot finds each segment's next entry that is either
+//    :132-134), so a ballot finds each segment's next entry that is either
 //    above that minimum or meets a list with free room, and the segment's first
 //    lane inserts it.  Entries skipped between two insertions are exactly those
 //    the list would reject.

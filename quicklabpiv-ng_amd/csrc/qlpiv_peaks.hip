@@ -239,7 +239,7 @@ __global__ __launch_bounds__(64) void peak_multi_blocks_kernel(int Dr,
 // Kernel 2: one lane per plane (blockDim.x planes per work-group) replays the
 // plane's entries in visit order with its list in registers (mp_insert_reg).
 // An entry changes the list only if the list has room or the entry is above the
-// list's smallest value (addInAscendOrder :127-129), so a chunk whose maximum
+// list's smallest value (addInAscendOrder :132-134), so a chunk whose maximum
 // is not above that bound is skipped without reading its entries (the chunk
 // maxima are staged in LDS first), and four entries are skipped together the
 // same way.  A block without an update is inserted as 0 at the position its
