@@ -334,6 +334,9 @@ __global__ __launch_bounds__(64) void peak_multi_replay_kernel(int n, int D, int
 
 bool multipeak_entries_form(int D, int n_peaks, int kernel_width)
 {
+    // QLPIV_MULTIPEAK_FORM = segment turns the form off everywhere (comparisons)
+    if (const char *env = getenv("QLPIV_MULTIPEAK_FORM"))
+        if (strcmp(env, "segment") == 0) return false;
     // kernel 1 holds one plane's entries in LDS
     return kernel_width == 3 && n_peaks >= 1 && n_peaks <= 4 && D >= 3 && D <= 255 &&
            (size_t)mp_geom(D).stride * sizeof(uint2) <= 65536;

@@ -229,3 +229,51 @@ def test_plan_finders_fits_and_start_step(ctx, pair256, case):
     plan.run(torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda())
     check_plan_levels(plan, a, b, log=say, **oopt)
     plan.close()
+
+
+@pytest.mark.parametrize("form", ["exact", "mfma_f16", "mfma_bf16"])
+def test_multipeaks_entries_in_plans_equal_segment_form(ctx, form):
+    """MultiPeaks in a multi-pass plan, with the block maxima written by the
+    correlation passes (exact: from the LDS plane; matrix-core forms: from the
+    plane each pass has just stored) and replayed one lane per plane, gives at
+    every level the u, v, peak records and validation of the same plan with the
+    segment form searching the stored planes (QLPIV_MULTIPEAK_FORM=segment):
+    both replay the reference's list on the same planes."""
+    import os
+
+    import torch
+
+    import qlpiv
+    from qlpiv.synth import particle_pair
+
+    arith = {"exact": qlpiv.QLP_XCORR_EXACT, "mfma_f16": qlpiv.QLP_XCORR_MFMA_F16,
+             "mfma_bf16": qlpiv.QLP_XCORR_MFMA_BF16}[form]
+    a, b, _ = particle_pair(512, 384, seed=41, shift=(2.6, -1.2), core_radius=80.0, u_max=5.0)
+    cfg = dict(ia_start=64, ia_end=16, warp_mode=1, peak_kind=qlpiv.QLP_PEAK_MULTI, n_peaks=3,
+               peak_kernel_width=3, xcorr_form=arith,
+               validation=qlpiv._lib.multipeak_median_stages(3, 2.0, 0.1))
+    da, db = torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda()
+    out = {}
+    saved = os.environ.get("QLPIV_MULTIPEAK_FORM")
+    try:
+        for name in ("entries", "segment"):
+            if name == "segment":
+                os.environ["QLPIV_MULTIPEAK_FORM"] = "segment"
+            else:
+                os.environ.pop("QLPIV_MULTIPEAK_FORM", None)
+            say(f"multipeaks plan {form} {name}")
+            plan = qlpiv.PivPlan(ctx, 512, 384, 1, **cfg)
+            plan.run(da, db)
+            out[name] = [plan.result(lv) for lv in range(len(plan.levels))]
+            out[name + "_valid"] = [plan.validation(lv) for lv in range(len(plan.levels))]
+            plan.close()
+    finally:
+        if saved is None:
+            os.environ.pop("QLPIV_MULTIPEAK_FORM", None)
+        else:
+            os.environ["QLPIV_MULTIPEAK_FORM"] = saved
+    for lv, (r1, r2) in enumerate(zip(out["entries"], out["segment"])):
+        for k in range(3):
+            assert np.array_equal(r1[k], r2[k]), (form, lv, k)
+        v1, v2 = out["entries_valid"][lv], out["segment_valid"][lv]
+        assert np.array_equal(v1[0], v2[0]) and np.array_equal(v1[1], v2[1]), (form, lv)
