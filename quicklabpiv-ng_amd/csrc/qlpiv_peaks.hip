@@ -158,14 +158,12 @@ __global__ __launch_bounds__(64) void peak_multi_kernel(int n, int D, int P, int
 // the reference's visit order (slices of the top-left quadrant, then the four
 // quadrants), each a block maximum (strict >, starting at 0: 0 for a block
 // without a value above 0) and the position of the first cell holding it.
-// Kernel 1 (planes in memory): one wavefront per plane.  The lanes take the
-// plane's 3x3 blocks in memory order (block rows of both halves, then block
-// columns of both halves), one block per lane and step (mp_block_vals), with the
-// loads of the next batch of steps issued before the current batch is
-// evaluated.  The entries go to LDS at their visit-order index and then to the
-// workspace in order, with each chunk's maximum and the plane minimum
-// (*min_floor).  The exact correlation pass writes the same entries from its
-// LDS plane instead (mp_entries_lds), and then only kernel 2 runs.
+// Kernel 1 (planes in memory): one wavefront per plane.  The plane is copied
+// into LDS with coalesced loads (consecutive lanes, consecutive cells), then its
+// entries are formed from there as the exact passes form them (mp_entries_core:
+// visit order, coalesced 8-byte entry stores, chunk maxima across 16 lanes),
+// with the plane minimum at *min_floor.  The exact correlation passes write the
+// same entries from the plane they still hold, and then only kernel 2 runs.
 template <int DT>
 __global__ __launch_bounds__(64) void peak_multi_blocks_kernel(int Dr,
                                                                const float *__restrict__ planes,
@@ -173,67 +171,34 @@ __global__ __launch_bounds__(64) void peak_multi_blocks_kernel(int Dr,
                                                                float *__restrict__ cmax,
                                                                float *__restrict__ min_floor)
 {
-    extern __shared__ uint2 s_ent[];
-    const int D = DT > 0 ? DT : Dr;
-    const MpGeom G = mp_geom(D);
+    extern __shared__ float s_pl[];
+    static_assert(DT > 0, "compile-time plane size");
+    constexpr int D = DT;
+    constexpr MpGeom G = mp_geom(D);
+    (void)Dr;
     const int lane = threadIdx.x, ia = blockIdx.x;
     const float *__restrict__ m = planes + (size_t)ia * D * D;
-    constexpr int U = DT == 31 ? 3 : (DT > 0 ? 8 : 4);
-    const int spp = (G.nent + 63) >> 6;  // steps per plane
-    const int nb = (spp + U - 1) / U;    // batches
-    auto load_batch = [&](int b, float (&dst)[U * 9]) {
+    constexpr int PER = (D * D + 63) / 64;
+    float v[PER];
 #pragma unroll
-        for (int u = 0; u < U; u++) {
-            const int task = lane + 64 * (b * U + u);
-            const int br = task / G.BW, bc = task - br * G.BW;
-            int o[9];
-            mp_block_offsets(D, D, br, bc, o);
-#pragma unroll
-            for (int k = 0; k < 9; k++) dst[u * 9 + k] = m[o[k]];
-        }
-    };
-    float mn = FLT_MAX;
-    auto eval_batch = [&](int b, const float (&v)[U * 9]) {
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-            const int st = b * U + u;
-            const int task = lane + 64 * st;
-            const int br = task / G.BW, bc = task - br * G.BW;
-            float bmin;
-            const uint2 en = mp_block_vals(D, br, bc, v + u * 9, bmin);
-            if (st < spp && br < G.BW) {
-                if (bmin < mn) mn = bmin;
-                s_ent[mp_entry_index(G, br, bc)] = en;
-            }
-        }
-    };
-    float bufA[U * 9], bufB[U * 9];
-    load_batch(0, bufA);
-    for (int b = 0; b < nb; b += 2) {
-        load_batch(min(b + 1, nb - 1), bufB);
-        eval_batch(b, bufA);
-        if (b + 1 < nb) {
-            load_batch(min(b + 2, nb - 1), bufA);
-            eval_batch(b + 1, bufB);
-        }
+    for (int k = 0; k < PER; k++) {
+        const int idx = lane + 64 * k;
+        v[k] = m[idx < D * D ? idx : D * D - 1];
     }
-    if (G.nent + lane < G.stride) s_ent[G.nent + lane] = make_uint2(__float_as_uint(-2.0f), 0u);
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        const int idx = lane + 64 * k;
+        if (idx < D * D) s_pl[idx] = v[k];
+    }
+    mp_wave_sync();
+    float mn = mp_entries_core<D, D, 64>([&](int o) { return s_pl[o]; }, lane,
+                                         ent + (size_t)ia * G.stride, cmax + (size_t)ia * G.nchunk);
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) {
         const float om = __shfl_xor(mn, o, 64);
         if (om < mn) mn = om;
     }
     if (lane == 0) min_floor[ia] = mn;
-    mp_wave_sync();
-    uint4 *__restrict__ dst = (uint4 *)(ent + (size_t)ia * G.stride);
-    const uint4 *src = (const uint4 *)s_ent;
-    for (int i = lane; i < G.stride / 2; i += 64) dst[i] = src[i];
-    for (int c = lane; c < G.nchunk; c += 64) {
-        float cm = __uint_as_float(s_ent[16 * c].x);
-#pragma unroll
-        for (int k = 1; k < 16; k++) cm = fmaxf(cm, __uint_as_float(s_ent[16 * c + k].x));
-        cmax[(size_t)ia * G.nchunk + c] = cm;
-    }
 }
 
 // Kernel 2: one lane per plane (blockDim.x planes per work-group) replays the
@@ -407,7 +372,7 @@ static int launch_entries_form(Context *ctx, int n, int D, int P, const float *p
     uint2 *ent;
     float *cm;
     multipeak_work_carve(ctx->mpwork, n, D, ent, cm);
-    const size_t lds = (size_t)mp_geom(D).stride * sizeof(uint2);
+    const size_t lds = (size_t)D * D * sizeof(float);
     switch (D) {
     case 31:
         hipLaunchKernelGGL(peak_multi_blocks_kernel<31>, dim3(n), dim3(64), lds, ctx->stream, D,
@@ -422,8 +387,7 @@ static int launch_entries_form(Context *ctx, int n, int D, int P, const float *p
                            planes, ent, cm, min_floor);
         break;
     default:
-        hipLaunchKernelGGL(peak_multi_blocks_kernel<0>, dim3(n), dim3(64), lds, ctx->stream, D,
-                           planes, ent, cm, min_floor);
+        return fail(QLP_ERR_UNSUPPORTED, "find_peaks: entries form for 31, 63 and 127 only");
     }
     return launch_multipeak_replay(ctx, n, D, P, ent, cm, peaks, count);
 }
@@ -487,17 +451,18 @@ int launch_find_peaks(Context *ctx, int kind, int n, int D, const float *planes,
         // 188/148/156/185; 3,969 x 127^2: 388/235/158/115.
         // QLPIV_MULTIPEAK_LANES (8, 16, 32 or 64) overrides it for tuning.
         // planes in memory: the entries form (block maxima kernel, then the
-        // replay) up to 63^2 planes, the segment form for larger ones.  Measured
-        // on MI355X (C2 step, MultiPeaks + Guo per level, ms, segment ->
-        // entries): exact 31^2 0.172 -> 0.152, MFMA 31^2 0.149 -> 0.150, MFMA
-        // 63^2 0.150 -> 0.145, MFMA 127^2 0.119 -> 0.163.  QLPIV_MULTIPEAK_FORM
-        // (segment or entries) overrides the choice.
+        // replay) up to 63^2 planes, the segment form for 127^2, whose plane
+        // does not leave room in LDS for more than two block wavefronts per
+        // CU.  Measured on MI355X (C2 planes, us, segment -> blocks + replay):
+        // 65,025 x 31^2 130 -> 51 + 46; 16,129 x 63^2 117 -> 62 + 49.
+        // QLPIV_MULTIPEAK_FORM (segment or entries) overrides the choice.
         bool entries = D <= 63;
         if (const char *env = getenv("QLPIV_MULTIPEAK_FORM")) {
             if (strcmp(env, "segment") == 0) entries = false;
             if (strcmp(env, "entries") == 0) entries = true;
         }
-        if (entries && multipeak_entries_form(D, n_peaks, kernel_width))
+        if (entries && (D == 31 || D == 63 || D == 127) &&
+            multipeak_entries_form(D, n_peaks, kernel_width))
             return launch_entries_form(ctx, n, D, n_peaks, planes, peaks, min_floor, count);
         int W = 8;
         while (W < 64 && (size_t)n * W < (size_t)2048 * 64) W *= 2;
