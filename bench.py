@@ -42,12 +42,23 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 H = W = 2048
 
 
+def entry_bytes(N):
+    """MultiPeaks entries an exact pass writes per IA (kernel width 3): 8 bytes
+    per 3x3 block of the (2N-1)^2 plane in the reference's visit order, padded to
+    chunks of 16 entries, 4 bytes per chunk maximum and 4 for the plane minimum."""
+    D = 2 * N - 1
+    slices = (D + 2) // 3
+    top = slices - slices // 2
+    stride = (4 * top * top + 15) // 16 * 16
+    return stride * 8 + stride // 16 * 4 + 4
+
+
 def pass_bytes(n_ia, N):
-    """Algorithmic HBM bytes of one plane-writing correlation pass: the A and B
-    tile pixels (2 N^2), the (2N-1)^2 float32 plane and the 8-byte inherited
-    displacement.  (With MultiPeaks the peak search reads the stored planes in
-    its own kernel.)"""
-    return n_ia * (2 * N * N + 4 * (2 * N - 1) ** 2 + 8)
+    """Algorithmic HBM bytes of one plane-writing correlation pass of the C2
+    step: the A and B tile pixels (2 N^2), the (2N-1)^2 float32 plane, the
+    8-byte inherited displacement and the MultiPeaks entries it writes for the
+    peak search (entry_bytes)."""
+    return n_ia * (2 * N * N + 4 * (2 * N - 1) ** 2 + 8 + entry_bytes(N))
 
 
 def cpu_model():
