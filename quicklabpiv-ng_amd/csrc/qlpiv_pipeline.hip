@@ -348,9 +348,10 @@ static int plan_layout(int H, int W, int pairs, const qlp_piv_config *cfg, qlp_p
             return fail(QLP_ERR_UNSUPPORTED,
                         "piv_plan_create: spline warps need 2 tiles in each direction");
         if (tile == 128) p->big_ias = std::max(p->big_ias, (int)pn);
-        // MultiPeaks entries: the exact 16 / 32 / 64 px passes write them, the
-        // finder forms them from planes of 63^2 or less
-        if (multi && (tile <= 32 || (tile == 64 && cfg->xcorr_form == QLP_XCORR_EXACT)))
+        // MultiPeaks entries: the exact 16 / 32 / 64 px passes and the 64 px
+        // matrix-core pass write them, the finder forms them from planes of 63^2
+        // or less
+        if (multi && tile <= 64)
             p->mp_work = std::max(p->mp_work, multipeak_work_bytes((int)pn, (int)D, P,
                                                                    cfg->peak_kernel_width));
         if (l > 0 && tiles_route(*cfg) && !f32)

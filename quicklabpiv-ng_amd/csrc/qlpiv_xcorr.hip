@@ -914,10 +914,13 @@ bool pass_writes_entries(const Context *ctx, const PassParams &p)
 {
     (void)ctx;  // both exact forms of 16 px (lane and LDS) write them
     if (p.n_dev || p.peaks || p.min_floor || p.nbr) return false;  // persistent or fused peak
-    // the matrix-core forms hold their planes in accumulators spread over the
-    // lanes; reading back the stored plane there measured slower than the
-    // separate block kernel (C2 MFMA passes +78..95 us for -50..80 us of search)
-    return p.arith == QLP_XCORR_EXACT && (p.N == 16 || p.N == 32 || p.N == 64);
+    // the 64 px matrix-core pass puts its plane into its free exchange buffer;
+    // the 16 / 32 px ones hold their planes in accumulators spread over the
+    // lanes with no LDS to spare, and reading back the stored plane there
+    // measured slower than the separate block kernel (C2 MFMA passes +78..95 us
+    // for -50..80 us of search)
+    if (p.arith != QLP_XCORR_EXACT) return p.N == 64;
+    return p.N == 16 || p.N == 32 || p.N == 64;
 }
 
 int launch_pass(Context *ctx, const PassParams &p)

@@ -35,6 +35,7 @@
 #include "qlpiv_device.h"
 #include "qlpiv_epilogue.h"
 #include "qlpiv_internal.h"
+#include "qlpiv_multipeak.h"
 
 #include <math.h>
 #include <string.h>
@@ -1102,6 +1103,27 @@ __global__ __launch_bounds__(m64::NT, 2) void xcorr_mfma64_kernel(PassParams p,
                         const int a = MT * (sc + 2 * pi) + mrow<MT>(v, g), b = MT * (sr + 2 * qi) + j;
                         if (a < D && b < D) out[(unsigned)(a * D + b)] = R[pi][qi][v] * unscale;
                     }
+        }
+        if (!PEAK && p.mp_ent) {
+            // MultiPeaks entries: the plane into the exchange buffer (free after
+            // the barrier above; 127^2 floats fit its 16,384), then the blocks
+            // from there by the work-group
+            float *s_pl = (float *)&s_x[0];
+            static_assert(D * D <= XCH * 4, "the plane fits the exchange buffer");
+#pragma unroll
+            for (int pi = 0; pi < 2; pi++)
+#pragma unroll
+                for (int qi = 0; qi < 2; qi++)
+#pragma unroll
+                    for (int v = 0; v < NS; v++) {
+                        const int a = MT * (sc + 2 * pi) + mrow<MT>(v, g), b = MT * (sr + 2 * qi) + j;
+                        if (a < D && b < D) s_pl[a * D + b] = R[pi][qi][v] * unscale;
+                    }
+            __syncthreads();
+            constexpr MpGeom MG = mp_geom(D);
+            mp_entries_lds<D, D, NT>(s_pl, s_min, p.mp_ent + (size_t)ia * MG.stride,
+                                     p.mp_cmax + (size_t)ia * MG.nchunk, p.mp_min + ia,
+                                     (int)threadIdx.x);
         }
         if (PEAK) {
             // FromCenter in two phases (from_center_tie) on the accumulator
