@@ -101,6 +101,12 @@ int ensure_mpwork(Context *ctx, size_t bytes)
     return grow_buffer(ctx, ctx->mpwork, ctx->mpwork_bytes, bytes, "hipMalloc MultiPeaks workspace");
 }
 
+bool mpwork_fits(const Context *ctx, size_t bytes)
+{
+    if (bytes <= ctx->mpwork_bytes) return true;
+    return !ctx->hbm_budget || ctx->hbm_in_use - ctx->mpwork_bytes + bytes <= ctx->hbm_budget;
+}
+
 static int sync_ctx(Context *ctx) { return check_hip(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize"); }
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }

@@ -188,6 +188,9 @@ void multipeak_work_carve(void *work, int n, int D, uint2 *&ent, float *&cmax);
 int launch_multipeak_replay(Context *ctx, int n, int D, int P, const uint2 *ent,
                             const float *cmax, float *peaks, int *count);
 int ensure_mpwork(Context *ctx, size_t bytes);
+// Whether the workspace holds 'bytes' or can grow to it within the HBM budget
+// (the MultiPeaks callers take the segment form, which needs none, otherwise).
+bool mpwork_fits(const Context *ctx, size_t bytes);
 // The throughput form (qlpiv_xcorr_mfma.hip): form is QLP_XCORR_MFMA_*.
 int launch_pass_mfma(Context *ctx, const PassParams &p, int form, bool from_image, bool peak);
 int launch_find_peaks(Context *ctx, int kind, int n, int D, const float *planes, int n_peaks,

@@ -462,7 +462,8 @@ int launch_find_peaks(Context *ctx, int kind, int n, int D, const float *planes,
             if (strcmp(env, "entries") == 0) entries = true;
         }
         if (entries && (D == 31 || D == 63 || D == 127) &&
-            multipeak_entries_form(D, n_peaks, kernel_width))
+            multipeak_entries_form(D, n_peaks, kernel_width) &&
+            mpwork_fits(ctx, multipeak_work_bytes(n, D, n_peaks, kernel_width)))
             return launch_entries_form(ctx, n, D, n_peaks, planes, peaks, min_floor, count);
         int W = 8;
         while (W < 64 && (size_t)n * W < (size_t)2048 * 64) W *= 2;

@@ -478,10 +478,11 @@ try {
     }
     // the plan's IA-128 passes read the images and find peaks
     if (!rc && p->big_ias) rc = prepare_big(ctx, p->big_ias, cfg->norm_mode, !f32, true, SHAPE_AUTO);
-    // the MultiPeaks workspace at its largest level's size, so that runs do not grow it
-    if (!rc && p->mp_work) rc = ensure_mpwork(ctx, p->mp_work);
     p->bytes = need;
     ctx->hbm_in_use += need;
+    // the MultiPeaks workspace at its largest level's size, so that runs do not
+    // grow it, when it fits the budget beside the plan (else the segment form)
+    if (!rc && p->mp_work && mpwork_fits(ctx, p->mp_work)) rc = ensure_mpwork(ctx, p->mp_work);
     if (rc) {
         plan_free(p);
         delete p;
@@ -793,10 +794,11 @@ static int run_levels(qlp_piv_plan *p, const uint8_t *imgsA, const uint8_t *imgs
         // entries) and only their replay runs after it
         uint2 *ment = nullptr;
         float *mcm = nullptr;
+        const size_t mp_need = multipeak_work_bytes((int)pn, (int)D, p->P, c.peak_kernel_width);
         const bool entries = multi && multipeak_entries_form((int)D, p->P, c.peak_kernel_width) &&
-                             pass_writes_entries(ctx, q);
+                             pass_writes_entries(ctx, q) && mpwork_fits(ctx, mp_need);
         if (entries) {
-            rc = ensure_mpwork(ctx, multipeak_work_bytes((int)pn, (int)D, p->P, c.peak_kernel_width));
+            rc = ensure_mpwork(ctx, mp_need);
             if (rc) return rc;
             multipeak_work_carve(ctx->mpwork, (int)pn, (int)D, ment, mcm);
             q.mp_ent = ment;

@@ -277,3 +277,34 @@ def test_multipeaks_entries_in_plans_equal_segment_form(ctx, form):
             assert np.array_equal(r1[k], r2[k]), (form, lv, k)
         v1, v2 = out["entries_valid"][lv], out["segment_valid"][lv]
         assert np.array_equal(v1[0], v2[0]) and np.array_equal(v1[1], v2[1]), (form, lv)
+
+
+def test_multipeaks_plan_without_room_for_the_entries(ctx):
+    """A context budget that holds a MultiPeaks plan's buffers but not the
+    entries workspace (qlp_piv_plan_estimate counts it): the plan is created and
+    runs on the segment form, with the results of the same plan on a context
+    with room for the workspace."""
+    import torch
+
+    import qlpiv
+    from qlpiv.synth import particle_pair
+
+    a, b, _ = particle_pair(512, 384, seed=43, shift=(1.6, 2.2), core_radius=80.0, u_max=5.0)
+    cfg = dict(ia_start=64, ia_end=16, warp_mode=1, peak_kind=qlpiv.QLP_PEAK_MULTI, n_peaks=3,
+               peak_kernel_width=3, validation=qlpiv._lib.multipeak_median_stages(3, 2.0, 0.1))
+    est = qlpiv.plan_estimate(512, 384, **cfg)
+    da, db = torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda()
+    res = {}
+    small = qlpiv.Context(0, hbm_budget=est - 1)
+    try:
+        for name, c in (("tight", small), ("room", ctx)):
+            say(f"multipeaks plan budget {name} (estimate {est} bytes)")
+            plan = qlpiv.PivPlan(c, 512, 384, 1, **cfg)
+            plan.run(da, db)
+            res[name] = [plan.result(lv) for lv in range(len(plan.levels))]
+            plan.close()
+    finally:
+        small.close()
+    for lv, (r1, r2) in enumerate(zip(res["tight"], res["room"])):
+        for k in range(3):
+            assert np.array_equal(r1[k], r2[k]), (lv, k)
