@@ -50,6 +50,14 @@ struct XcShape {
 
 __device__ __forceinline__ int bitrev_n(int x, int bits) { return (int)(__brev((unsigned)x) >> (32 - bits)); }
 
+// Bit reversal of x over 'bits' bits, for indices known at compile time.
+__host__ __device__ constexpr unsigned brev_const(int x, int bits)
+{
+    unsigned r = 0;
+    for (int b = 0; b < bits; b++) r |= (unsigned)((x >> b) & 1) << (bits - 1 - b);
+    return r;
+}
+
 // The thread index, hidden from the optimiser: address arithmetic built on it
 // is then recomputed in each iteration of a loop over IAs (or line batches)
 // instead of being hoisted out of the loop into registers for every unrolled
@@ -103,12 +111,15 @@ __device__ __forceinline__ void fft_lines(float *__restrict__ re, float *__restr
         const int task = tx + rd * NT;
         if (task < ntasks) {
             const int l = task % nlines, t = task / nlines;
-            const int base = l * LS;
+            // bitrev(t*E + q) over LOGM bits = bitrev(q) over LOGE bits, shifted
+            // up by B, plus bitrev(t) over B bits (t < T = 2^B): one reversal
+            // per line, and a compile-time offset per element
+            const int base = l * LS + bitrev_n(t, B) * PS;
             #pragma unroll
             for (int q = 0; q < E; q += (ZERO_HALF ? 2 : 1)) {
-                const int o = bitrev_n(t * E + q, LOGM);
-                rr[rd][q] = re[base + o * PS];
-                ri[rd][q] = im[base + o * PS];
+                const int oq = (int)(brev_const(q, LOGE) << B);
+                rr[rd][q] = re[base + oq * PS];
+                ri[rd][q] = im[base + oq * PS];
             }
         }
     }
@@ -914,12 +925,13 @@ bool pass_writes_entries(const Context *ctx, const PassParams &p)
 {
     (void)ctx;  // both exact forms of 16 px (lane and LDS) write them
     if (p.n_dev || p.peaks || p.min_floor || p.nbr) return false;  // persistent or fused peak
-    // the 64 px matrix-core pass puts its plane into its free exchange buffer;
-    // the 16 / 32 px ones hold their planes in accumulators spread over the
-    // lanes with no LDS to spare, and reading back the stored plane there
-    // measured slower than the separate block kernel (C2 MFMA passes +78..95 us
-    // for -50..80 us of search)
-    if (p.arith != QLP_XCORR_EXACT) return p.N == 64;
+    // the 64 px matrix-core pass puts its plane into its free exchange buffer,
+    // the 16 px one into an LDS slot per wavefront (3.8 KB); the 32 px one holds
+    // its plane in accumulators spread over the lanes with no LDS to spare for
+    // 16 KB per wavefront, and reading back the stored plane there measured
+    // slower than the separate block kernel (C2 MFMA passes +78..95 us for
+    // -50..80 us of search)
+    if (p.arith != QLP_XCORR_EXACT) return p.N == 64 || p.N == 16;
     return p.N == 16 || p.N == 32 || p.N == 64;
 }
 

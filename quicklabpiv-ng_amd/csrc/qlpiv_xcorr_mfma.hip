@@ -238,6 +238,8 @@ __global__ __launch_bounds__(64 * MF_WAVES, QLPIV_MF_MINW) void xcorr_mfma_kerne
     // (then the Hout^T fragments again, scaled by 2^(E1 + E2): the inverse's
     // second product then returns the plane at its true scale)
     __shared__ u4 s_frag[(X::NF * 2 + 1 + X::NF_HO * 2) * 64];
+    // 16 px: each wavefront's plane, for the MultiPeaks entries (3.8 KB each)
+    __shared__ float s_plane[N == 16 ? MF_WAVES : 1][N == 16 ? D * D : 1];
 
     for (int k = threadIdx.x; k < (X::NF * 2 + 1 + X::NF_HO * 2) * 64; k += 64 * MF_WAVES)
         s_frag[k] = mp.frags[k];
@@ -539,6 +541,35 @@ __global__ __launch_bounds__(64 * MF_WAVES, QLPIV_MF_MINW) void xcorr_mfma_kerne
                             if (rt == 0 || v < NS - 1 || a < D) out[(unsigned)(a * D + b)] = R[rt][ct][v] * unscale;
                         }
                 }
+            }
+        }
+        if constexpr (N == 16 && !PEAK) {
+            if (p.mp_ent && valid) {
+                // MultiPeaks entries: the wavefront's plane into its LDS slot,
+                // then the blocks from there
+                float *s_pl = s_plane[wave];
+#pragma unroll
+                for (int ct = 0; ct < 2; ct++) {
+                    const int b = MT * ct + j;
+#pragma unroll
+                    for (int rt = 0; rt < 2; rt++)
+#pragma unroll
+                        for (int v = 0; v < NS; v++) {
+                            const int a = MT * rt + mrow<MT>(v, g);
+                            if (a < D && b < D) s_pl[a * D + b] = R[rt][ct][v] * unscale;
+                        }
+                }
+                mp_wave_sync();
+                constexpr MpGeom MG = mp_geom(D);
+                float mn = mp_entries_core<D, D, 64>([&](int o) { return s_pl[o]; }, lane,
+                                                     p.mp_ent + (size_t)ia * MG.stride,
+                                                     p.mp_cmax + (size_t)ia * MG.nchunk);
+#pragma unroll
+                for (int o = 32; o >= 1; o >>= 1) {
+                    const float om = __shfl_xor(mn, o, 64);
+                    if (om < mn) mn = om;
+                }
+                if (lane == 0) p.mp_min[ia] = mn;
             }
         }
         if (PEAK) {
