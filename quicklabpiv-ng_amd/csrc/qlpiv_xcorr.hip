@@ -926,12 +926,9 @@ bool pass_writes_entries(const Context *ctx, const PassParams &p)
     (void)ctx;  // both exact forms of 16 px (lane and LDS) write them
     if (p.n_dev || p.peaks || p.min_floor || p.nbr) return false;  // persistent or fused peak
     // the 64 px matrix-core pass puts its plane into its free exchange buffer,
-    // the 16 px one into an LDS slot per wavefront (3.8 KB); the 32 px one holds
-    // its plane in accumulators spread over the lanes with no LDS to spare for
-    // 16 KB per wavefront, and reading back the stored plane there measured
-    // slower than the separate block kernel (C2 MFMA passes +78..95 us for
-    // -50..80 us of search)
-    if (p.arith != QLP_XCORR_EXACT) return p.N == 64 || p.N == 16;
+    // the 16 px one into an LDS slot per wavefront (3.8 KB), the 32 px one six
+    // rows at a time into a band slot per wavefront (1.5 KB)
+    if (p.arith != QLP_XCORR_EXACT) return p.N == 16 || p.N == 32 || p.N == 64;
     return p.N == 16 || p.N == 32 || p.N == 64;
 }
 

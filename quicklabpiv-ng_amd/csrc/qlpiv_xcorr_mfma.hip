@@ -238,8 +238,11 @@ __global__ __launch_bounds__(64 * MF_WAVES, QLPIV_MF_MINW) void xcorr_mfma_kerne
     // (then the Hout^T fragments again, scaled by 2^(E1 + E2): the inverse's
     // second product then returns the plane at its true scale)
     __shared__ u4 s_frag[(X::NF * 2 + 1 + X::NF_HO * 2) * 64];
-    // 16 px: each wavefront's plane, for the MultiPeaks entries (3.8 KB each)
+    // MultiPeaks entries: 16 px, each wavefront's plane (3.8 KB); 32 px, a band
+    // of six plane rows per wavefront (1.5 KB) and its chunk maxima
     __shared__ float s_plane[N == 16 ? MF_WAVES : 1][N == 16 ? D * D : 1];
+    __shared__ float s_band[N == 32 ? MF_WAVES : 1][N == 32 ? 6 * D : 1];
+    __shared__ uint32_t s_cmx[N == 32 ? MF_WAVES : 1][N == 32 ? 32 : 1];
 
     for (int k = threadIdx.x; k < (X::NF * 2 + 1 + X::NF_HO * 2) * 64; k += 64 * MF_WAVES)
         s_frag[k] = mp.frags[k];
@@ -564,6 +567,69 @@ __global__ __launch_bounds__(64 * MF_WAVES, QLPIV_MF_MINW) void xcorr_mfma_kerne
                 float mn = mp_entries_core<D, D, 64>([&](int o) { return s_pl[o]; }, lane,
                                                      p.mp_ent + (size_t)ia * MG.stride,
                                                      p.mp_cmax + (size_t)ia * MG.nchunk);
+#pragma unroll
+                for (int o = 32; o >= 1; o >>= 1) {
+                    const float om = __shfl_xor(mn, o, 64);
+                    if (om < mn) mn = om;
+                }
+                if (lane == 0) p.mp_min[ia] = mn;
+            }
+        }
+        if constexpr (N == 32 && !PEAK) {
+            if (p.mp_ent && valid) {
+                // MultiPeaks entries band by band: plane rows 6k .. 6k+5 (block
+                // rows 2k, 2k+1 of both halves) into the wavefront's slot, then
+                // those 2 x BW blocks by as many lanes; the chunk maxima by LDS
+                // atomics (entries are >= 0, so their bit patterns order as the
+                // values; the padding entries of the last chunk are not counted)
+                constexpr MpGeom MG = mp_geom(D);
+                static_assert(MG.nchunk <= 32 && 2 * MG.BW <= 64 && MG.stride - MG.nent <= 64,
+                              "one lane per block of a band");
+                float *slot = s_band[wave];
+                uint32_t *cmx = s_cmx[wave];
+                uint2 *__restrict__ ent = p.mp_ent + (size_t)ia * MG.stride;
+                if (lane < MG.nchunk) cmx[lane] = 0u;
+                if (MG.nent + lane < MG.stride)
+                    ent[MG.nent + lane] = make_uint2(__float_as_uint(-2.0f), 0u);
+                float mn = FLT_MAX;
+#pragma unroll
+                for (int k = 0; k < (D + 5) / 6; k++) {
+                    // the lane holds rows a0 + 4 g of each slot: only the slots
+                    // reaching this band are stored (a compile-time choice)
+#pragma unroll
+                    for (int rt = 0; rt < 2; rt++)
+#pragma unroll
+                        for (int v = 0; v < NS; v++) {
+                            const int a0 = MT * rt + mrow<MT>(v, 0);
+                            if (a0 + 4 < 6 * k || a0 >= 6 * k + 6) continue;
+                            const int a = a0 + 4 * g;
+#pragma unroll
+                            for (int ct = 0; ct < 2; ct++) {
+                                const int b = MT * ct + j;
+                                if (a >= 6 * k && a < 6 * k + 6 && a < D && b < D)
+                                    slot[(a - 6 * k) * D + b] = R[rt][ct][v] * unscale;
+                            }
+                        }
+                    mp_wave_sync();
+                    const int br = 2 * k + (lane >= MG.BW ? 1 : 0);
+                    const int bc = lane - (lane >= MG.BW ? MG.BW : 0);
+                    if (lane < 2 * MG.BW && br < MG.BW) {
+                        // rows past the plane edge read the last row (in this band)
+                        int o[9];
+                        mp_block_offsets(D, D, br, bc, o);
+                        float c[9];
+#pragma unroll
+                        for (int q = 0; q < 9; q++) c[q] = slot[o[q] - 6 * k * D];
+                        float bmin;
+                        const uint2 en = mp_block_vals(D, br, bc, c, bmin);
+                        if (bmin < mn) mn = bmin;
+                        const int e = mp_entry_index(MG, br, bc);
+                        ent[e] = en;
+                        atomicMax(&cmx[e >> 4], en.x);
+                    }
+                    mp_wave_sync();  // the slot's reads before the next band's stores
+                }
+                if (lane < MG.nchunk) p.mp_cmax[(size_t)ia * MG.nchunk + lane] = __uint_as_float(cmx[lane]);
 #pragma unroll
                 for (int o = 32; o >= 1; o >>= 1) {
                     const float om = __shfl_xor(mn, o, 64);
