@@ -236,9 +236,10 @@ def test_multipeaks_entries_in_plans_equal_segment_form(ctx, form):
     """MultiPeaks in a multi-pass plan, with the block maxima written by the
     correlation passes (exact: from the LDS plane; matrix-core forms: from the
     plane each pass has just stored) and replayed one lane per plane, gives at
-    every level the u, v, peak records and validation of the same plan with the
-    segment form searching the stored planes (QLPIV_MULTIPEAK_FORM=segment):
-    both replay the reference's list on the same planes."""
+    every level the u, v, peak records, validation and stored planes of the same
+    plan with the segment form searching the stored planes
+    (QLPIV_MULTIPEAK_FORM=segment): both replay the reference's list on the same
+    planes."""
     import os
 
     import torch
@@ -266,6 +267,7 @@ def test_multipeaks_entries_in_plans_equal_segment_form(ctx, form):
             plan.run(da, db)
             out[name] = [plan.result(lv) for lv in range(len(plan.levels))]
             out[name + "_valid"] = [plan.validation(lv) for lv in range(len(plan.levels))]
+            out[name + "_planes"] = [plan.planes(lv) for lv in range(len(plan.levels))]
             plan.close()
     finally:
         if saved is None:
@@ -277,6 +279,9 @@ def test_multipeaks_entries_in_plans_equal_segment_form(ctx, form):
             assert np.array_equal(r1[k], r2[k]), (form, lv, k)
         v1, v2 = out["entries_valid"][lv], out["segment_valid"][lv]
         assert np.array_equal(v1[0], v2[0]) and np.array_equal(v1[1], v2[1]), (form, lv)
+        # the stored planes too (the 32 px matrix-core pass stores them band by
+        # band from LDS when it writes the entries)
+        assert np.array_equal(out["entries_planes"][lv], out["segment_planes"][lv]), (form, lv)
 
 
 def test_multipeaks_plan_without_room_for_the_entries(ctx):
