@@ -264,6 +264,104 @@ __device__ __forceinline__ void mp_entries_lds(const float *pl, float *red, uint
     }
 }
 
+// The replay of one plane per lane (blockDim.x planes per work-group, the
+// work-group's planes consecutive from blockIdx.x * blockDim.x): the chunk
+// maxima of the work-group's planes are staged in s_cm (blockDim.x * nchunk
+// floats), then each lane runs addInAscendOrder over its plane's entries in
+// visit order, skipping the chunks and groups of four whose maximum is not above
+// the list's smallest value while the list is full.  A block without an update
+// is inserted as 0 at the position its quadrant recorded last
+// (FindMaximumMultiPeaks :62-64 keep maxsI/maxsJ across blocks), looked up by a
+// walk back over the quadrant's entries not seen yet.  Leaves the ascending list
+// in lv / lp (positions i << 8 | j) with size entries.
+template <int PM>
+__device__ __forceinline__ void mp_replay(int n, int D, int P, const uint2 *__restrict__ ent,
+                                          const float *__restrict__ cmax, float *s_cm,
+                                          float (&lv)[PM], int (&lp)[PM], int &size)
+{
+    constexpr int KD = 6;  // (kernelWidth - 1) * kernelWidth
+    const MpGeom G = mp_geom(D);
+    const int g = threadIdx.x, GW = blockDim.x;
+    const int ia = blockIdx.x * GW + g;
+    const bool act = ia < n;
+    const uint2 *__restrict__ pe = ent + (size_t)(act ? ia : 0) * G.stride;
+    {
+        const int ia0 = blockIdx.x * GW;
+        const int cnt = (min(GW, n - ia0)) * G.nchunk;
+        const float *src = cmax + (size_t)ia0 * G.nchunk;
+        for (int i = g; i < cnt; i += GW) s_cm[i] = src[i];
+        __syncthreads();
+    }
+    const float *cm = s_cm + (act ? g : 0) * G.nchunk;
+#pragma unroll
+    for (int t = 0; t < PM; t++) {
+        lv[t] = 0.0f;
+        lp[t] = 0;
+    }
+    size = 0;
+    // an entry (>= 0) can change the list only if it is above lo: -1 while the
+    // list has room, else the list's smallest value
+    float lo = act ? -1.0f : FLT_MAX;
+    int stale[4] = {0, 0, 0, 0};
+    int seen[4] = {-4, -3, -2, -1};  // last entry of each quadrant taken into stale
+    for (int c = 0; c < G.nchunk; c++) {
+        if (!(cm[c] > lo)) continue;
+        uint4 q[8];
+        const uint4 *src = (const uint4 *)(pe + 16 * c);
+#pragma unroll
+        for (int k = 0; k < 8; k++) q[k] = src[k];
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            const float v4[4] = {__uint_as_float(q[2 * b].x), __uint_as_float(q[2 * b].z),
+                                 __uint_as_float(q[2 * b + 1].x), __uint_as_float(q[2 * b + 1].z)};
+            const int p4[4] = {(int)q[2 * b].y, (int)q[2 * b].w, (int)q[2 * b + 1].y,
+                               (int)q[2 * b + 1].w};
+            if (!(fmaxf(fmaxf(v4[0], v4[1]), fmaxf(v4[2], v4[3])) > lo)) continue;
+            const int e = 16 * c + 4 * b;
+#pragma unroll
+            for (int qd = 0; qd < 4; qd++) {
+                const float v = v4[qd];
+                if (v > lo) {
+                    int pij = p4[qd];
+                    if (!(v > 0.0f)) {  // no update: the quadrant's last recorded position
+                        for (int f = e + qd - 4; f > seen[qd]; f -= 4)
+                            if (__uint_as_float(pe[f].x) > 0.0f) {
+                                stale[qd] = (int)pe[f].y;
+                                break;
+                            }
+                        seen[qd] = e + qd;
+                        pij = stale[qd];
+                    }
+                    mp_insert_reg<PM>(lv, lp, size, P, KD, v, pij);
+                    lo = size < P ? -1.0f : lv[0];
+                }
+            }
+        }
+    }
+}
+
+// The peaks (P x 3: i, j, value, descending; slots past the list zero) and the
+// peak count (MaxCrossResult.getTotalPeaks :109-115: the secondary arrays exist
+// only once a peak index >= 1 was stored) of IA ia from its replayed list.
+template <int PM>
+__device__ __forceinline__ void mp_store_peaks(int ia, int P, const float (&lv)[PM],
+                                               const int (&lp)[PM], int size,
+                                               float *__restrict__ peaks, int *__restrict__ count)
+{
+    float *__restrict__ pk = peaks + (size_t)ia * P * 3;
+#pragma unroll
+    for (int s = 0; s < PM; s++) {
+        if (s < size) {
+            const int idx = size - 1 - s;
+            pk[3 * idx + 0] = (float)(lp[s] >> 8);
+            pk[3 * idx + 1] = (float)(lp[s] & 255);
+            pk[3 * idx + 2] = lv[s];
+        }
+    }
+    for (int s = size; s < P; s++) pk[3 * s] = pk[3 * s + 1] = pk[3 * s + 2] = 0.0f;
+    if (count) count[ia] = size > 1 ? size : 1;
+}
+
 // Maximum (strict >, from 0, row-major first) and minimum of one block; rows
 // and columns past the plane edge are not part of the block.  Cell (r, c) is
 // m[r * rs + c * cs].  KW > 0 fixes the kernel width at compile time so that

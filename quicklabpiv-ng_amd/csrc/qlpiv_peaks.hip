@@ -202,7 +202,8 @@ __global__ __launch_bounds__(64) void peak_multi_blocks_kernel(int Dr,
 }
 
 // Kernel 2: one lane per plane (blockDim.x planes per work-group) replays the
-// plane's entries in visit order with its list in registers (mp_insert_reg).
+// plane's entries in visit order with its list in registers (mp_replay,
+// mp_insert_reg).
 // An entry changes the list only if the list has room or the entry is above the
 // list's smallest value (addInAscendOrder :132-134), so a chunk whose maximum
 // is not above that bound is skipped without reading its entries (the chunk
@@ -218,83 +219,12 @@ __global__ __launch_bounds__(64) void peak_multi_replay_kernel(int n, int D, int
                                                                float *__restrict__ peaks,
                                                                int *__restrict__ count)
 {
-    constexpr int KD = 6;  // (kernelWidth - 1) * kernelWidth
     extern __shared__ float s_cm[];
-    const MpGeom G = mp_geom(D);
-    const int g = threadIdx.x, GW = blockDim.x;
-    const int ia = blockIdx.x * GW + g;
-    const bool act = ia < n;
-    const uint2 *__restrict__ pe = ent + (size_t)(act ? ia : 0) * G.stride;
-    {  // chunk maxima of the work-group's planes (consecutive in memory) into LDS
-        const int ia0 = blockIdx.x * GW;
-        const int cnt = (min(GW, n - ia0)) * G.nchunk;
-        const float *src = cmax + (size_t)ia0 * G.nchunk;
-        for (int i = g; i < cnt; i += GW) s_cm[i] = src[i];
-        __syncthreads();
-    }
-    const float *cm = s_cm + (act ? g : 0) * G.nchunk;
+    const int ia = blockIdx.x * blockDim.x + threadIdx.x;
     float lv[PM];
-    int lp[PM];
-#pragma unroll
-    for (int t = 0; t < PM; t++) {
-        lv[t] = 0.0f;
-        lp[t] = 0;
-    }
-    int size = 0;
-    // an entry (>= 0) can change the list only if it is above lo: -1 while the
-    // list has room, else the list's smallest value
-    float lo = act ? -1.0f : FLT_MAX;
-    int stale[4] = {0, 0, 0, 0};
-    int seen[4] = {-4, -3, -2, -1};  // last entry of each quadrant taken into stale
-    for (int c = 0; c < G.nchunk; c++) {
-        if (!(cm[c] > lo)) continue;
-        uint4 q[8];
-        const uint4 *src = (const uint4 *)(pe + 16 * c);
-#pragma unroll
-        for (int k = 0; k < 8; k++) q[k] = src[k];
-#pragma unroll
-        for (int b = 0; b < 4; b++) {
-            const float v4[4] = {__uint_as_float(q[2 * b].x), __uint_as_float(q[2 * b].z),
-                                 __uint_as_float(q[2 * b + 1].x), __uint_as_float(q[2 * b + 1].z)};
-            const int p4[4] = {(int)q[2 * b].y, (int)q[2 * b].w, (int)q[2 * b + 1].y,
-                               (int)q[2 * b + 1].w};
-            if (!(fmaxf(fmaxf(v4[0], v4[1]), fmaxf(v4[2], v4[3])) > lo)) continue;
-            const int e = 16 * c + 4 * b;
-#pragma unroll
-            for (int qd = 0; qd < 4; qd++) {
-                const float v = v4[qd];
-                if (v > lo) {
-                    int pij = p4[qd];
-                    if (!(v > 0.0f)) {  // no update: the quadrant's last recorded position
-                        for (int f = e + qd - 4; f > seen[qd]; f -= 4)
-                            if (__uint_as_float(pe[f].x) > 0.0f) {
-                                stale[qd] = (int)pe[f].y;
-                                break;
-                            }
-                        seen[qd] = e + qd;
-                        pij = stale[qd];
-                    }
-                    mp_insert_reg<PM>(lv, lp, size, P, KD, v, pij);
-                    lo = size < P ? -1.0f : lv[0];
-                }
-            }
-        }
-    }
-    if (act) {
-        float *__restrict__ pk = peaks + (size_t)ia * P * 3;
-        // descending: peak 0 is the largest
-#pragma unroll
-        for (int s = 0; s < PM; s++) {
-            if (s < size) {
-                const int idx = size - 1 - s;
-                pk[3 * idx + 0] = (float)(lp[s] >> 8);
-                pk[3 * idx + 1] = (float)(lp[s] & 255);
-                pk[3 * idx + 2] = lv[s];
-            }
-        }
-        for (int s = size; s < P; s++) pk[3 * s] = pk[3 * s + 1] = pk[3 * s + 2] = 0.0f;
-        if (count) count[ia] = size > 1 ? size : 1;
-    }
+    int lp[PM], size;
+    mp_replay<PM>(n, D, P, ent, cmax, s_cm, lv, lp, size);
+    if (ia < n) mp_store_peaks<PM>(ia, P, lv, lp, size, peaks, count);
 }
 
 bool multipeak_entries_form(int D, int n_peaks, int kernel_width)

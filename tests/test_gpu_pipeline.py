@@ -231,15 +231,16 @@ def test_plan_finders_fits_and_start_step(ctx, pair256, case):
     plan.close()
 
 
+@pytest.mark.parametrize("pixels", [3, 5, 7])
 @pytest.mark.parametrize("form", ["exact", "mfma_f16", "mfma_bf16"])
-def test_multipeaks_entries_in_plans_equal_segment_form(ctx, form):
+def test_multipeaks_entries_in_plans_equal_segment_form(ctx, form, pixels):
     """MultiPeaks in a multi-pass plan, with the block maxima written by the
     correlation passes (exact: from the LDS plane; matrix-core forms: from the
     plane each pass has just stored) and replayed one lane per plane, gives at
     every level the u, v, peak records, validation and stored planes of the same
     plan with the segment form searching the stored planes
     (QLPIV_MULTIPEAK_FORM=segment): both replay the reference's list on the same
-    planes."""
+    planes; Guo fits of 3, 5 and 7 pixels follow."""
     import os
 
     import torch
@@ -251,8 +252,8 @@ def test_multipeaks_entries_in_plans_equal_segment_form(ctx, form):
              "mfma_bf16": qlpiv.QLP_XCORR_MFMA_BF16}[form]
     a, b, _ = particle_pair(512, 384, seed=41, shift=(2.6, -1.2), core_radius=80.0, u_max=5.0)
     cfg = dict(ia_start=64, ia_end=16, warp_mode=1, peak_kind=qlpiv.QLP_PEAK_MULTI, n_peaks=3,
-               peak_kernel_width=3, xcorr_form=arith,
-               validation=qlpiv._lib.multipeak_median_stages(3, 2.0, 0.1))
+               peak_kernel_width=3, xcorr_form=arith, subpix_kind=qlpiv.QLP_SUBPIX_GUO,
+               subpix_pixels=pixels, validation=qlpiv._lib.multipeak_median_stages(3, 2.0, 0.1))
     da, db = torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda()
     out = {}
     saved = os.environ.get("QLPIV_MULTIPEAK_FORM")
@@ -262,7 +263,7 @@ def test_multipeaks_entries_in_plans_equal_segment_form(ctx, form):
                 os.environ["QLPIV_MULTIPEAK_FORM"] = "segment"
             else:
                 os.environ.pop("QLPIV_MULTIPEAK_FORM", None)
-            say(f"multipeaks plan {form} {name}")
+            say(f"multipeaks plan {form} guo {pixels} px {name}")
             plan = qlpiv.PivPlan(ctx, 512, 384, 1, **cfg)
             plan.run(da, db)
             out[name] = [plan.result(lv) for lv in range(len(plan.levels))]
