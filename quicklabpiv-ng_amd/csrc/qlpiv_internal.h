@@ -195,9 +195,11 @@ bool mpwork_fits(const Context *ctx, size_t bytes);
 int launch_pass_mfma(Context *ctx, const PassParams &p, int form, bool from_image, bool peak);
 int launch_find_peaks(Context *ctx, int kind, int n, int D, const float *planes, int n_peaks,
                       int kernel_width, float *peaks, float *min_floor, int *count = nullptr);
+// u, v (Guo only, or null): the fit kernel also applies the multi-pass loop's
+// displacement update from each IA's main peak (as launch_accumulate).
 int launch_subpixel(Context *ctx, int kind, int pixels, int iterations, int n, int D, int n_peaks,
                     const float *planes, const float *min_floor, const float *peaks_in,
-                    float *peaks_out);
+                    float *peaks_out, float *u = nullptr, float *v = nullptr);
 // n_dev: persistent launch (n is the capacity, the count is in device memory);
 // fit_on: per-IA flag, 0 keeps the integer peak (status SKIPPED).
 struct FitScatter;

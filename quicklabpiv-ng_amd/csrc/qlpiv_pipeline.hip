@@ -815,10 +815,13 @@ static int run_levels(qlp_piv_plan *p, const uint8_t *imgsA, const uint8_t *imgs
             else
                 rc = launch_find_peaks(ctx, c.peak_kind, pn, D, L.planes, p->P, c.peak_kernel_width,
                                        L.peaks, L.mf, multi ? L.count : nullptr);
+            // Guo applies the displacement update itself; the other fits after it
+            const bool guo = kind == QLP_SUBPIX_GUO;
             if (!rc)
                 rc = launch_subpixel(ctx, kind, c.subpix_pixels, c.subpix_iterations, pn, D, p->P,
-                                     L.planes, L.mf, L.peaks, L.sub);
-            if (!rc) rc = launch_accumulate(ctx, pn, p->P, D, L.sub, L.u, L.v);
+                                     L.planes, L.mf, L.peaks, L.sub, guo ? L.u : nullptr,
+                                     guo ? L.v : nullptr);
+            if (!rc && !guo) rc = launch_accumulate(ctx, pn, p->P, D, L.sub, L.u, L.v);
         } else {
             rc = launch_subpixel_nbr(ctx, kind, c.subpix_pixels, c.subpix_iterations, pn, D,
                                      NBR_HALF, L.nbr, L.mf, L.peaks, L.sub, L.u, L.v);

@@ -539,6 +539,13 @@ __device__ __forceinline__ void guo_item(int gid, int pixels, int iterations, in
         res[3] = (float)((yValid ? ST_I : 0) | (xValid ? ST_J : 0));
     }
     fit_result(item, D, res, peaks_out, FROM_NBR ? du : nullptr, dv, sc);
+    // planes in HBM: the displacement update from each IA's main peak (peak 0),
+    // as accumulate_kernel does after the other fits
+    if (!FROM_NBR && du && item - ia * P == 0) {
+        const float c = (float)(D / 2);
+        du[ia] = du[ia] + (res[0] - c);
+        dv[ia] = dv[ia] + (res[1] - c);
+    }
 }
 
 //   n_dev  : persistent launch: the item count in device memory (items is then
@@ -685,8 +692,13 @@ static int check_subpixel_args(int kind, int pixels, int iterations)
 
 int launch_subpixel(Context *ctx, int kind, int pixels, int iterations, int n, int D, int n_peaks,
                     const float *planes, const float *min_floor, const float *peaks_in,
-                    float *peaks_out)
+                    float *peaks_out, float *u, float *v)
 {
+    // u, v (or null): the Guo kernel also applies the displacement update of the
+    // multi-pass loop from each IA's main peak; the other fits leave it to
+    // launch_accumulate
+    if (u && kind != QLP_SUBPIX_GUO)
+        return fail(QLP_ERR_ARG, "subpixel: the fused displacement update is Guo's only");
     if (n == 0) return QLP_OK;
     if (kind >= 0) {  // kind < 0: integer peaks copied (multi-pass levels before the start step)
         int rc = check_subpixel_args(kind, pixels, iterations);
@@ -698,7 +710,7 @@ int launch_subpixel(Context *ctx, int kind, int pixels, int iterations, int n, i
                : pixels == 7 ? guo_kernel<false, 3> : guo_kernel<false, NBR_MAX_HALF>;
         hipLaunchKernelGGL(k, dim3((2 * total + 255) / 256), dim3(256), 0, ctx->stream, pixels,
                            iterations, total, D, n_peaks, planes, 0, min_floor, peaks_in,
-                           peaks_out, nullptr, nullptr, nullptr, nullptr, FitScatter{});
+                           peaks_out, u, v, nullptr, nullptr, FitScatter{});
     } else
         hipLaunchKernelGGL(subpixel_kernel, dim3((total + 255) / 256), dim3(256), 0, ctx->stream,
                            kind, pixels, iterations, n, D, n_peaks, planes, min_floor, peaks_in,
