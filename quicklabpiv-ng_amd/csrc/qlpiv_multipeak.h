@@ -221,11 +221,20 @@ __device__ __forceinline__ float mp_entries_core(F rd, int t, uint2 *__restrict_
         if (e < G.nent) {
             const int qd = e & 3, s = e >> 2, si = s / G.iT, sj = s - si * G.iT;
             const int br = si + ((qd & 2) ? G.iT : 0), bc = sj + ((qd & 1) ? G.iT : 0);
-            int o[9];
-            mp_block_offsets(D, RS, br, bc, o);
             float c[9];
+            if constexpr (D % 3 == 0) {
+                // every block lies wholly inside the plane or wholly past its
+                // edge (no update, no minimum): no cell needs clamping
+                const int base = 3 * br * RS + 3 * bc;
+                const bool inside = 3 * br < D && 3 * bc < D;
 #pragma unroll
-            for (int k = 0; k < 9; k++) c[k] = rd(o[k]);
+                for (int k = 0; k < 9; k++) c[k] = inside ? rd(base + (k / 3) * RS + k % 3) : 0.0f;
+            } else {
+                int o[9];
+                mp_block_offsets(D, RS, br, bc, o);
+#pragma unroll
+                for (int k = 0; k < 9; k++) c[k] = rd(o[k]);
+            }
             float bmin;
             en = mp_block_vals(D, br, bc, c, bmin);
             if (bmin < mn) mn = bmin;
